@@ -1,0 +1,147 @@
+"""Benchmark: acoustic forward shot on one MI355X.
+
+Workload (default): 2001 (z) x 4001 (x) interior + 50-cell C-PML on every
+side (4101 x 2101 cells stored), dx = dz = 10 m, dt = 0.8 ms, nt = 6000,
+velocity = seeded Gaussian-smoothed noise (sigma = 20 cells) mapped to
+1500-4500 m/s, constant density, 20 Hz Ricker at interior (z=2, x=2000), 4001
+receivers at interior z=2, radius-8 staggered stencil (the reference's C1..C8).
+
+One "step" of this benchmark is one whole shot (nt time steps) with the model,
+wavelet and geometry already resident in device memory.  value = Mcell-updates/s
+over all stored cells (boundary layers included).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "Mcell-updates/s & % HBM roofline; batched fwd+adjoint shots/s vs plain fwd"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_step(shot, plan_pml_w=None):
+    """Compulsory HBM bytes of one time step: P, Vx, Vz read and written
+    (24 B), the two per-cell coefficients read (8 B), and for C-PML the two
+    memory variables of each strip cell read and written (16 B)."""
+    nx, nz = shot.vp.shape
+    R = shot.radius
+    b = 32.0 * nx * nz
+    if shot.boundary == "cpml":
+        w = shot.width + 1
+        xrows = min(2 * w, nx)
+        zcols = min(2 * w, nz)
+        b += 16.0 * (xrows * nz + zcols * nx)
+    return b
+
+
+def cpu_baseline(shot, steps):
+    import oracle
+    cores = min(16, len(os.sched_getaffinity(0)))
+    den = np.full_like(shot.vp, 2400.0)
+    extra = {}
+    if shot.boundary == "cpml":
+        extra = dict(boundary="cpml", pml_r=shot.pml_r, pml_f0=shot.pml_f0, pml_vmax=float(shot.vp.max()))
+    t0 = time.perf_counter()
+    oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, dx=shot.dx, dz=shot.dz, dt=shot.dt,
+                    radius=shot.radius, sponge=shot.width, nt=steps, nthreads=cores, **extra)
+    dt = time.perf_counter() - t0
+    nx, nz = shot.vp.shape
+    return {"value": round(nx * nz * steps / dt / 1e6, 2), "unit": "Mcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"first {steps} of {len(shot.wavelet)} time steps of the same shot, FP32 oracle "
+                      f"(reference arithmetic), OpenMP over x rows, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed shots")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots")
+    ap.add_argument("--nt", type=int, default=6000)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: shots are independent and this build measures one "
+              "GPU's throughput (see DESIGN.md for how shots shard across GPUs).", file=sys.stderr)
+        sys.exit(2)
+
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+
+    shot = W.smooth_random_2001x4001(nt=args.nt)
+    nx, nz = shot.vp.shape
+    log(f"workload: {nx}x{nz} cells, nt={shot.nt}, R={shot.radius}, {shot.boundary}")
+    ctx = sfd.Context(0)
+    plan = sfd.AcousticPlan(shot, ctx)
+    if args.waves or args.chunk:
+        plan.set_tiling(args.waves, args.chunk)
+    for i in range(args.warmup):
+        ms = plan.run()
+        log(f"warmup shot {i}: {ms:.1f} ms")
+    # Single process: no barrier needed; plan.run() synchronizes the device.
+    dev_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dev_ms.append(plan.run())
+        log(f"shot {i}: {dev_ms[-1]:.1f} ms (device)")
+    wall = time.perf_counter() - t0
+    rec, _ = plan.fetch()
+    assert np.isfinite(rec).all(), "non-finite gather"
+
+    cells = nx * nz * shot.nt
+    value = cells * args.steps / wall / 1e6
+    avg_launch_s = (sum(dev_ms) / len(dev_ms)) / 1e3 / shot.nt
+    bytes_step = algorithmic_bytes_per_step(shot)
+    achieved = bytes_step / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "acoustic_step_cpml_2101x4101_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded smooth random velocity, Ricker source)",
+        "config": {"workload": f"acoustic 2D velocity-pressure staggered grid, R={shot.radius}, "
+                               f"C-PML {shot.width}, {nz - 2 * shot.width}x{nx - 2 * shot.width} (z x x) interior, "
+                               f"{nx}x{nz} stored, nt={shot.nt}, dt=0.8 ms, one shot per step",
+                   "cells": nx * nz, "nt": shot.nt, "radius": shot.radius, "boundary": shot.boundary,
+                   "receivers": int(len(shot.rec[0]))},
+        "shots_per_s": round(args.steps / wall, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch": bytes_step, "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(shot, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

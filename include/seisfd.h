@@ -1,0 +1,140 @@
+/*
+ * seisfd — MI355X (gfx950) finite-difference wave propagators: C ABI.
+ *
+ * Drop-in for the time loop of stone0312/Seismic_code_github
+ * forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp (the reference program
+ * hard-codes its configuration at :28-45,137-216 and runs the loop at
+ * :237-380).  Every input the reference program fixes in its constant block is
+ * an argument here: grid (Nx, Nz, Nt, dx, dz, dt), stencil coefficients
+ * (C1..C8, :38-45), velocity and density (Vp, DEN, :137-142), sponge width
+ * (PML, :30), source wavelet and position (signal, nx_location/nz_location,
+ * :199-216) and receiver cells (:376-377).
+ *
+ * Conventions
+ *   - Fields are [nx][nz] float32, z fastest, 0-based: the reference's own
+ *     layout (`float*` rows of Nz floats, :51-100).  No transpose is needed.
+ *   - All host pointers are owned by the caller and only used during the call.
+ *     Every call is synchronous with respect to the host buffers it takes.
+ *   - Every entry point returns an int status (SFD_OK = 0, negative on
+ *     failure) and never exits or aborts; sfd_last_error() gives the text.
+ *   - One context per host thread; contexts are independent.
+ */
+#ifndef SEISFD_H
+#define SEISFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEISFD_ABI_VERSION 1
+
+enum sfd_status {
+    SFD_OK = 0,
+    SFD_EINVAL = -1,   /* invalid argument */
+    SFD_ENOMEM = -2,   /* device or host allocation failed */
+    SFD_EDEVICE = -3,  /* HIP runtime error */
+    SFD_ECFL = -4,     /* the time step violates the stencil's stability bound */
+    SFD_ENAN = -5,     /* a non-finite value appeared in the wavefield */
+    SFD_ENOTSUP = -6   /* combination of options not supported */
+};
+
+enum sfd_boundary_kind {
+    /* The reference's quadratic sponge, Acoustic_1order_FD.cpp:164-194,
+     * 1 - 0.38 d^2/L^2 on every field after each step (:329-340). */
+    SFD_BOUNDARY_SPONGE = 0,
+    /* Convolutional PML (no reference counterpart). */
+    SFD_BOUNDARY_CPML = 1
+};
+
+enum sfd_arith {
+    /* FP32 with fused multiply-adds and per-cell coefficients folded once. */
+    SFD_ARITH_FAST = 0,
+    /* The reference's own arithmetic, operation for operation, including its
+     * double-promoted sub-expressions (:259,:276,:293-311): results are
+     * bitwise equal to the reference program's.  Sponge boundary only. */
+    SFD_ARITH_REFERENCE = 1
+};
+
+enum sfd_field { SFD_FIELD_P = 1, SFD_FIELD_VX = 2, SFD_FIELD_VZ = 4 };
+
+typedef struct sfd_context sfd_context;
+typedef struct sfd_acoustic_plan sfd_acoustic_plan;
+
+/* One acoustic shot.  Mirrors the reference's constant block. */
+typedef struct sfd_acoustic_params {
+    int nx, nz;            /* full grid incl. absorbing layers (reference Nx, Nz) */
+    int nt;                /* time steps (reference Nt) */
+    float dx, dz, dt;      /* metres, seconds */
+    int radius;            /* stencil half-width R: 4, 5 or 8 (reference: 8) */
+    const float *coef;     /* C_1..C_R; NULL = the reference's table for R
+                              (C1..C8 at :38-45; a80..a83 / a100..a104 of
+                              yang_version_2016_2_2/Iso_multishot_fd_module_zy_m2.f90:17-26) */
+    int boundary;          /* enum sfd_boundary_kind */
+    int width;             /* absorbing layer width in cells (reference PML) */
+    float pml_r;           /* C-PML: target reflection coefficient (e.g. 1e-3) */
+    float pml_f0;          /* C-PML: frequency of the alpha profile (Hz) */
+    int sx, sz;            /* source cell (reference nx_location, nz_location) */
+    int nrec;
+    const int *rx, *rz;    /* receiver cells; reference: rx = Nx/2, rz = PML..Nz-PML-1 */
+    int nsnap;
+    const int *snap_steps; /* ascending steps k after which fields are copied out */
+    int snap_fields;       /* mask of enum sfd_field */
+    int arith;             /* enum sfd_arith */
+} sfd_acoustic_params;
+
+/* Context: owns the HIP stream and the error text. device = HIP ordinal. */
+int sfd_context_create(int device, sfd_context **ctx);
+int sfd_context_destroy(sfd_context *ctx);
+const char *sfd_last_error(const sfd_context *ctx);
+int sfd_abi_version(void);
+
+/* One shot, end to end: the drop-in for Acoustic_1order_FD.cpp:237-393.
+ *   vp, den  [nx*nz]   model (den NULL = 2400 kg/m^3 everywhere, :141)
+ *   wavelet  [nt]      source samples (reference `signal`, :199-209); the
+ *                      reference injects P[sx][sz] -= dt*den*vp*wavelet[k]
+ *                      before the update of step k (:247-251)
+ *   record   [nrec*nt] trace-major, like record.dat (:385-393); may be NULL
+ *   snaps    [nsnap][popcount(snap_fields)][nx*nz], fields in P, Vx, Vz order
+ *            (the reference writes Vx at k == 500 to Snap.dat, :343-355, and
+ *            P every 10 steps to Pall.dat, :358-373); may be NULL */
+int sfd_acoustic_forward(sfd_context *ctx, const sfd_acoustic_params *p,
+                         const float *vp, const float *den, const float *wavelet,
+                         float *record, float *snaps);
+
+/* The same shot split in phases, so inputs stay resident in device memory:
+ * create uploads the model and wavelet; run zeroes the wavefield and
+ * propagates all nt steps on the device; fetch copies record and snapshots to
+ * the host.  run may be repeated.  elapsed_ms (may be NULL) receives the
+ * device time of the propagation measured with HIP events on the plan's
+ * stream. */
+int sfd_acoustic_plan_create(sfd_context *ctx, const sfd_acoustic_params *p,
+                             const float *vp, const float *den, const float *wavelet,
+                             sfd_acoustic_plan **plan);
+int sfd_acoustic_plan_run(sfd_acoustic_plan *plan, float *elapsed_ms);
+int sfd_acoustic_plan_fetch(sfd_acoustic_plan *plan, float *record, float *snaps);
+int sfd_acoustic_plan_destroy(sfd_acoustic_plan *plan);
+
+/* Step-level access (checkpoint / resume).  state is [3][nx*nz] (P, Vx, Vz);
+ * pml is [4][nx*nz] (psi_px, psi_pz, psi_vx, psi_vz; C-PML only, NULL
+ * otherwise).  step advances nsteps steps starting at step index k0 and
+ * records into the plan's gather at columns k0..k0+nsteps-1. */
+int sfd_acoustic_plan_set_state(sfd_acoustic_plan *plan, const float *state, const float *pml);
+int sfd_acoustic_plan_get_state(sfd_acoustic_plan *plan, float *state, float *pml);
+int sfd_acoustic_plan_step(sfd_acoustic_plan *plan, int k0, int nsteps);
+
+/* Launch shape of the step kernel: waves per workgroup (1, 2 or 4) and rows
+ * marched per workgroup.  0 = choose automatically. */
+int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan *plan, int waves, int chunk_rows);
+
+/* Stability bound of the staggered leapfrog: the largest
+ * vmax * dt * sqrt(1/dx^2 + 1/dz^2) the stencil allows, 1 / sum |C_m|.
+ * Plans whose model exceeds it are rejected with SFD_ECFL. */
+double sfd_acoustic_cfl_limit(int radius, const float *coef);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

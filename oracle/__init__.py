@@ -1,0 +1,179 @@
+"""ctypes binding of the CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, by ``__graft_entry__.smoke()``
+and by ``bench.py``'s CPU-baseline leg, always as the checker.  The product
+(``seismic_code_github_amd``) never imports it.
+
+The oracle restates forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:126-393
+in C (``acoustic.c``); see ``oracle.h`` for the argument meaning.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+# Reference coefficients C1..C8 as written in Acoustic_1order_FD.cpp:38-45 (float).
+REF_C8 = np.array([1.2340911, -1.0664985e-1, 2.3036367e-2, -5.3423856e-3,
+                   1.0772712e-3, -1.6641888e-4, 1.7021711e-5, -8.5234642e-7],
+                  dtype=np.float32)
+# a80..a83 and a100..a104, yang_version_2016_2_2/Iso_multishot_fd_module_zy_m2.f90:17-26.
+REF_A8 = np.array([1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0],
+                  dtype=np.float32)
+REF_A10 = np.array([19845.0 / 16384.0, -735.0 / 8192.0, 567.0 / 40960.0,
+                    -405.0 / 229376.0, 35.0 / 294912.0], dtype=np.float32)
+
+
+def coefs_for_radius(radius: int) -> np.ndarray:
+    """The reference's own table for R = 8, 4 or 5; Taylor weights otherwise."""
+    if radius == 8:
+        return REF_C8.copy()
+    if radius == 4:
+        return REF_A8.copy()
+    if radius == 5:
+        return REF_A10.copy()
+    return staggered_coefs(radius).astype(np.float32)
+
+
+class _Args(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nt", ctypes.c_int), ("k0", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float),
+        ("radius", ctypes.c_int), ("coef", ctypes.POINTER(ctypes.c_float)),
+        ("sponge", ctypes.c_int), ("absorb", ctypes.POINTER(ctypes.c_float)),
+        ("vp", ctypes.POINTER(ctypes.c_float)), ("den", ctypes.POINTER(ctypes.c_float)),
+        ("wavelet", ctypes.POINTER(ctypes.c_float)),
+        ("sx", ctypes.c_int), ("sz", ctypes.c_int),
+        ("nrec", ctypes.c_int), ("rx", ctypes.POINTER(ctypes.c_int)),
+        ("rz", ctypes.POINTER(ctypes.c_int)),
+        ("nsnap", ctypes.c_int), ("snap_steps", ctypes.POINTER(ctypes.c_int)),
+        ("nthreads", ctypes.c_int),
+        ("boundary", ctypes.c_int), ("pml_r", ctypes.c_float),
+        ("pml_vmax", ctypes.c_float), ("pml_f0", ctypes.c_float),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so (and the reference program when its source is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_acoustic_f32.argtypes = [ctypes.POINTER(_Args), fp, fp, fp, fp]
+        L.oracle_acoustic_f64.argtypes = [ctypes.POINTER(_Args), dp, dp, dp, dp]
+        L.oracle_sponge_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
+        L.oracle_ricker.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, fp]
+        L.oracle_staggered_coefs.argtypes = [ctypes.c_int, dp]
+        L.oracle_cpml_coef.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype=ctypes.c_float):
+    if a is None:
+        return ctypes.POINTER(ctype)()
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def sponge_table(nx: int, nz: int, width: int) -> np.ndarray:
+    out = np.empty((nx, nz), dtype=np.float32)
+    lib().oracle_sponge_table(nx, nz, width, _ptr(out))
+    return out
+
+
+def ricker(nt: int, dt: float, freq: float, amp: float = 50.0) -> np.ndarray:
+    out = np.empty(nt, dtype=np.float32)
+    lib().oracle_ricker(nt, dt, freq, amp, _ptr(out))
+    return out
+
+
+def staggered_coefs(radius: int) -> np.ndarray:
+    out = np.empty(radius, dtype=np.float64)
+    lib().oracle_staggered_coefs(radius, _ptr(out, ctypes.c_double))
+    return out
+
+
+def cpml_coef(i, shift, n, w, h, dt, R, vmax, f0):
+    b = ctypes.c_double()
+    a = ctypes.c_double()
+    lib().oracle_cpml_coef(i, shift, n, w, h, dt, R, vmax, f0, ctypes.byref(b), ctypes.byref(a))
+    return b.value, a.value
+
+
+def acoustic(vp, den, wavelet, src, rec, *, dx=10.0, dz=10.0, dt=5e-4, radius=8,
+             sponge=20, nt=None, k0=0, snap_steps=(), state=None, pml_state=None,
+             precision="f32", nthreads=1, boundary="sponge", pml_r=1e-3,
+             pml_vmax=None, pml_f0=None, coef=None, absorb=None):
+    """Run the oracle.  vp, den: [nx, nz] float32 (z fastest).  src = (sx, sz);
+    rec = (rx[], rz[]).  Returns dict(record [nrec, nt], snaps [nsnap, 3, nx, nz],
+    state [3, nx, nz], pml [4, nx, nz] or None)."""
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    den = np.ascontiguousarray(den, dtype=np.float32)
+    nx, nz = vp.shape
+    wavelet = np.ascontiguousarray(wavelet, dtype=np.float32)
+    if nt is None:
+        nt = wavelet.shape[0] - k0
+    assert wavelet.shape[0] >= k0 + nt
+    coef = coefs_for_radius(radius) if coef is None else np.ascontiguousarray(coef, dtype=np.float32)
+    rx = np.ascontiguousarray(rec[0], dtype=np.int32)
+    rz = np.ascontiguousarray(rec[1], dtype=np.int32)
+    snaps_idx = np.ascontiguousarray(sorted(snap_steps), dtype=np.int32)
+    absorb_arr = None if absorb is None else np.ascontiguousarray(absorb, dtype=np.float32)
+    bflag = {"sponge": 0, "cpml": 1}[boundary]
+    args = _Args(nx, nz, nt, k0, dx, dz, dt, radius, _ptr(coef), sponge, _ptr(absorb_arr),
+                 _ptr(vp), _ptr(den), _ptr(wavelet), int(src[0]), int(src[1]),
+                 rx.size, _ptr(rx, ctypes.c_int), _ptr(rz, ctypes.c_int),
+                 snaps_idx.size, _ptr(snaps_idx, ctypes.c_int), nthreads, bflag,
+                 pml_r, float(pml_vmax if pml_vmax is not None else vp.max()),
+                 float(pml_f0 if pml_f0 is not None else 0.0))
+    dtype = np.float32 if precision == "f32" else np.float64
+    ct = ctypes.c_float if precision == "f32" else ctypes.c_double
+    record = np.zeros((rx.size, nt), dtype=dtype)
+    snaps = np.zeros((snaps_idx.size, 3, nx, nz), dtype=dtype)
+    st = np.zeros((3, nx, nz), dtype=dtype) if state is None else np.array(state, dtype=dtype, copy=True)
+    pml = None
+    if bflag == 1:
+        pml = np.zeros((4, nx, nz), dtype=dtype) if pml_state is None else np.array(pml_state, dtype=dtype, copy=True)
+    fn = lib().oracle_acoustic_f32 if precision == "f32" else lib().oracle_acoustic_f64
+    rc = fn(ctypes.byref(args), _ptr(record, ct), _ptr(snaps, ct) if snaps_idx.size else ctypes.POINTER(ct)(),
+            _ptr(st, ct), _ptr(pml, ct))
+    if rc != 0:
+        raise ValueError(f"oracle_acoustic returned {rc}")
+    return {"record": record, "snaps": snaps, "state": st, "pml": pml}
+
+
+def reference_config():
+    """The reference program's hard-coded configuration, Acoustic_1order_FD.cpp:28-45,137-216:
+    NX=NZ=100, PML=20, Nt=2000, dx=dz=10, dt=0.5 ms, Vp=2000, DEN=2400, 28 Hz
+    Ricker (amplitude 50), source at (Nx/2, Nz/2), receivers P1[Nx/2][PML..Nz-PML)."""
+    NX = NZ = 100
+    PML = 20
+    Nx, Nz = NX + 2 * PML, NZ + 2 * PML
+    nt = 2000
+    dt = np.float32(0.0005)
+    vp = np.full((Nx, Nz), 2000.0, dtype=np.float32)
+    den = np.full((Nx, Nz), 2400.0, dtype=np.float32)
+    w = ricker(nt, dt, 28.0, 50.0)
+    sx, sz = Nx // 2, Nz // 2
+    rz = np.arange(PML, Nx - PML, dtype=np.int32)
+    rx = np.full_like(rz, sx)
+    return dict(vp=vp, den=den, wavelet=w, src=(sx, sz), rec=(rx, rz), dx=10.0, dz=10.0,
+                dt=float(dt), radius=8, sponge=PML, nt=nt)

@@ -1,0 +1,88 @@
+/*
+ * CPU oracle for the 2-D finite-difference wave propagators of
+ * stone0312/Seismic_code_github `forward/`.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithms, used by the test-suite, by the smoke check and by the benchmark's
+ * CPU-baseline leg as the checker.  No product code path links or calls it.
+ *
+ * Layout (as in the reference): every field is [nx][nz], z fastest
+ * (forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:51-100 allocates
+ * `float*[Nx]` rows of `Nz` floats).  Indices are 0-based.
+ *
+ * Two arithmetics are provided for every propagator:
+ *   *_f32 : the reference's own arithmetic, operation for operation, including
+ *           the double-promoted sub-expressions of the C++ source (built with
+ *           -ffp-contract=off and no -march, so no FMA is formed, as in the
+ *           reference's `g++ -O3` build, Makefile:14).
+ *   *_f64 : the same algorithm with every operation in double, on the same
+ *           float inputs; used to state the FP32 rounding error of a result.
+ */
+#ifndef SEISFD_ORACLE_H
+#define SEISFD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Acoustic velocity-pressure staggered-grid propagator,
+ * Acoustic_1order_FD.cpp:126-393, generalised to any grid, model, source
+ * position, receiver list, sponge width and stencil radius. */
+typedef struct oracle_acoustic_args {
+    int nx, nz;              /* full grid incl. sponge (reference Nx, Nz) */
+    int nt;                  /* number of time steps to run */
+    int k0;                  /* index of the first step (wavelet sample) */
+    float dx, dz, dt;
+    int radius;              /* R: number of coefficient pairs, 1..8 */
+    const float *coef;       /* C_1..C_R (reference C1..C8, :38-45) */
+    int sponge;              /* quadratic sponge width on every side (reference PML) */
+    const float *absorb;     /* optional [nx*nz] sponge table; NULL = build it */
+    const float *vp;         /* [nx*nz] */
+    const float *den;        /* [nx*nz] */
+    const float *wavelet;    /* [k0+nt] source samples (reference `signal`) */
+    int sx, sz;              /* source cell (reference nx_location, nz_location) */
+    int nrec;
+    const int *rx, *rz;      /* receiver cells */
+    int nsnap;
+    const int *snap_steps;   /* steps k after which P, Vx, Vz are copied out */
+    int nthreads;            /* OpenMP threads for the stencil loops (1 = serial) */
+    int boundary;            /* 0 = reference quadratic sponge, 1 = C-PML */
+    float pml_r;             /* C-PML target reflection coefficient */
+    float pml_vmax;          /* C-PML reference velocity */
+    float pml_f0;            /* C-PML alpha frequency (Hz) */
+} oracle_acoustic_args;
+
+/* Outputs (any may be NULL):
+ *   record [nrec][nt]  trace-major, like record.dat (:385-393)
+ *   snaps  [nsnap][3][nx*nz]  P, Vx, Vz after the absorb of step snap_steps[s]
+ *   state  [3][nx*nz]  initial P, Vx, Vz on entry (NULL = zero), final on exit
+ *   pml    [4][nx*nz]  initial/final C-PML memory fields (psi_px, psi_pz,
+ *                      psi_vx, psi_vz), only used when boundary == 1
+ * Returns 0, or a negative value on invalid arguments. */
+int oracle_acoustic_f32(const oracle_acoustic_args *a, float *record,
+                        float *snaps, float *state, float *pml);
+int oracle_acoustic_f64(const oracle_acoustic_args *a, double *record,
+                        double *snaps, double *state, double *pml);
+
+/* The reference's quadratic sponge table, Acoustic_1order_FD.cpp:126-132,164-194:
+ * 1 - 0.38 d^2 / L^2 (computed in double, stored as float).  The x-edge
+ * strips (rows) overwrite the z-edge strips in the corners. */
+void oracle_sponge_table(int nx, int nz, int width, float *absorb);
+
+/* The reference's Ricker wavelet, Acoustic_1order_FD.cpp:199-209:
+ * amp (1 - 2 pi^2 f^2 (t-1/f)^2) exp(-pi^2 f^2 (t-1/f)^2), t = k dt in float,
+ * pi = 3.1415926, double arithmetic, stored as float. */
+void oracle_ricker(int nt, float dt, float freq, float amp, float *w);
+
+/* Staggered first-derivative Taylor coefficients of half-width R, in double. */
+void oracle_staggered_coefs(int radius, double *c);
+
+/* C-PML profile (b, a) at x = i + shift cells on an axis of n cells whose first
+ * and last w cells absorb. */
+void oracle_cpml_coef(int i, double shift, int n, int w, double h, double dt, double R,
+                      double vmax, double f0, double *b, double *a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

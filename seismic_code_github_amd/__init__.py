@@ -1,0 +1,291 @@
+"""seismic_code_github_amd — MI355X (gfx950) finite-difference wave propagators.
+
+Python mirror of the C ABI in ``include/seisfd.h`` (ctypes over the in-tree
+``libseisfd.so``).  The propagators themselves are HIP kernels; this module only
+marshals arguments.  It raises if the native library is missing: there is no
+CPU fallback.
+
+Reference: stone0312/Seismic_code_github forward/Acoustic_FD_by_ss/src/
+Acoustic_1order_FD.cpp — its constant block (:28-45, 137-216) becomes the
+arguments of :func:`acoustic_forward`, its time loop (:237-393) the library.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libseisfd.so")
+
+SFD_OK = 0
+SFD_EINVAL = -1
+SFD_ENOMEM = -2
+SFD_EDEVICE = -3
+SFD_ECFL = -4
+SFD_ENAN = -5
+SFD_ENOTSUP = -6
+
+BOUNDARY_SPONGE = 0
+BOUNDARY_CPML = 1
+ARITH_FAST = 0
+ARITH_REFERENCE = 1
+FIELD_P, FIELD_VX, FIELD_VZ = 1, 2, 4
+
+_STATUS = {SFD_EINVAL: "EINVAL", SFD_ENOMEM: "ENOMEM", SFD_EDEVICE: "EDEVICE",
+           SFD_ECFL: "ECFL", SFD_ENAN: "ENAN", SFD_ENOTSUP: "ENOTSUP"}
+
+
+class SeisfdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+class AcousticParamsC(ctypes.Structure):
+    """Mirror of ``struct sfd_acoustic_params``."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nt", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float),
+        ("radius", ctypes.c_int), ("coef", _FP),
+        ("boundary", ctypes.c_int), ("width", ctypes.c_int),
+        ("pml_r", ctypes.c_float), ("pml_f0", ctypes.c_float),
+        ("sx", ctypes.c_int), ("sz", ctypes.c_int),
+        ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
+        ("nsnap", ctypes.c_int), ("snap_steps", _IP), ("snap_fields", ctypes.c_int),
+        ("arith", ctypes.c_int),
+    ]
+
+
+# Every symbol declared in include/seisfd.h, with its ctypes signature.
+_SIGNATURES = {
+    "sfd_abi_version": (ctypes.c_int, []),
+    "sfd_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_context_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "sfd_acoustic_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AcousticParamsC),
+                                            _FP, _FP, _FP, _FP, _FP]),
+    "sfd_acoustic_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AcousticParamsC),
+                                                _FP, _FP, _FP, ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_acoustic_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_acoustic_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
+    "sfd_acoustic_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_acoustic_plan_set_state": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
+    "sfd_acoustic_plan_get_state": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
+    "sfd_acoustic_plan_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_cfl_limit": (ctypes.c_double, [ctypes.c_int, _FP]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libseisfd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library {LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, t=ctypes.c_float):
+    if a is None:
+        return ctypes.POINTER(t)()
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class Context:
+    """An ``sfd_context``: one HIP device and stream."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().sfd_context_create(device, ctypes.byref(h))
+        if rc != SFD_OK:
+            raise SeisfdError(rc, f"sfd_context_create(device={device}) failed")
+        self.handle = h
+
+    def check(self, rc: int, what: str):
+        if rc != SFD_OK:
+            msg = lib().sfd_last_error(self.handle) or b""
+            raise SeisfdError(rc, f"{what}: {msg.decode()}")
+
+    def close(self):
+        if self.handle:
+            lib().sfd_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class AcousticShot:
+    """One acoustic shot: the reference program's constant block as data.
+
+    vp, den: [nx, nz] float32, z fastest.  wavelet: [nt].  src = (sx, sz).
+    rec = (rx[], rz[]).  width: sponge/PML width (reference PML)."""
+    vp: np.ndarray
+    wavelet: np.ndarray
+    src: tuple
+    rec: tuple
+    dx: float = 10.0
+    dz: float = 10.0
+    dt: float = 5e-4
+    den: Optional[np.ndarray] = None
+    radius: int = 8
+    coef: Optional[np.ndarray] = None
+    boundary: str = "sponge"
+    width: int = 20
+    pml_r: float = 1e-3
+    pml_f0: float = 0.0
+    snap_steps: Sequence[int] = ()
+    snap_fields: int = FIELD_P | FIELD_VX | FIELD_VZ
+    arith: str = "fast"
+    nt: Optional[int] = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def params(self) -> AcousticParamsC:
+        vp = _f32(self.vp)
+        nx, nz = vp.shape
+        nt = int(self.nt if self.nt is not None else np.asarray(self.wavelet).shape[0])
+        rx = np.ascontiguousarray(self.rec[0], dtype=np.int32)
+        rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
+        steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
+        coef = None if self.coef is None else _f32(self.coef)
+        self._keep = [rx, rz, steps, coef]
+        return AcousticParamsC(
+            nx, nz, nt, self.dx, self.dz, self.dt, self.radius, _ptr(coef),
+            {"sponge": BOUNDARY_SPONGE, "cpml": BOUNDARY_CPML}[self.boundary], self.width,
+            self.pml_r, self.pml_f0, int(self.src[0]), int(self.src[1]),
+            rx.size, _ptr(rx, ctypes.c_int), _ptr(rz, ctypes.c_int),
+            steps.size, _ptr(steps, ctypes.c_int), self.snap_fields,
+            {"fast": ARITH_FAST, "reference": ARITH_REFERENCE}[self.arith])
+
+    def nfields(self) -> int:
+        return bin(self.snap_fields & 7).count("1")
+
+
+class AcousticPlan:
+    """``sfd_acoustic_plan``: model and wavelet resident on the device."""
+
+    def __init__(self, shot: AcousticShot, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.shot = shot
+        self.p = shot.params()
+        self.nx, self.nz = np.asarray(shot.vp).shape
+        self.nt = self.p.nt
+        vp = _f32(shot.vp)
+        den = None if shot.den is None else _f32(shot.den)
+        w = _f32(shot.wavelet)
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_acoustic_plan_create(self.ctx.handle, ctypes.byref(self.p), _ptr(vp),
+                                                      _ptr(den), _ptr(w), ctypes.byref(h)),
+                       "sfd_acoustic_plan_create")
+        self.handle = h
+
+    def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
+        self.ctx.check(lib().sfd_acoustic_plan_set_tiling(self.handle, waves, chunk_rows), "set_tiling")
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_acoustic_plan_run(self.handle, ctypes.byref(ms)), "sfd_acoustic_plan_run")
+        return ms.value
+
+    def step(self, k0: int, nsteps: int):
+        self.ctx.check(lib().sfd_acoustic_plan_step(self.handle, k0, nsteps), "sfd_acoustic_plan_step")
+
+    def fetch(self):
+        nrec = self.p.nrec
+        record = np.zeros((nrec, self.nt), dtype=np.float32)
+        snaps = np.zeros((self.p.nsnap, self.shot.nfields(), self.nx, self.nz), dtype=np.float32)
+        self.ctx.check(lib().sfd_acoustic_plan_fetch(self.handle, _ptr(record) if nrec else None,
+                                                     _ptr(snaps) if self.p.nsnap else None),
+                       "sfd_acoustic_plan_fetch")
+        return record, snaps
+
+    def get_state(self):
+        st = np.zeros((3, self.nx, self.nz), dtype=np.float32)
+        pml = np.zeros((4, self.nx, self.nz), dtype=np.float32) if self.p.boundary == BOUNDARY_CPML else None
+        self.ctx.check(lib().sfd_acoustic_plan_get_state(self.handle, _ptr(st), _ptr(pml)), "get_state")
+        return st, pml
+
+    def set_state(self, state, pml=None):
+        st = _f32(state)
+        pm = None if pml is None else _f32(pml)
+        self.ctx.check(lib().sfd_acoustic_plan_set_state(self.handle, _ptr(st), _ptr(pm)), "set_state")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_acoustic_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def acoustic_forward(shot: AcousticShot, ctx: Optional[Context] = None):
+    """One shot end to end through ``sfd_acoustic_forward``.
+    Returns (record [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
+    ctx = ctx or default_context()
+    p = shot.params()
+    nx, nz = np.asarray(shot.vp).shape
+    vp = _f32(shot.vp)
+    den = None if shot.den is None else _f32(shot.den)
+    w = _f32(shot.wavelet)
+    record = np.zeros((p.nrec, p.nt), dtype=np.float32)
+    snaps = np.zeros((p.nsnap, shot.nfields(), nx, nz), dtype=np.float32)
+    ctx.check(lib().sfd_acoustic_forward(ctx.handle, ctypes.byref(p), _ptr(vp), _ptr(den), _ptr(w),
+                                         _ptr(record) if p.nrec else None, _ptr(snaps) if p.nsnap else None),
+              "sfd_acoustic_forward")
+    return record, snaps
+
+
+def cfl_limit(radius: int, coef=None) -> float:
+    c = None if coef is None else _f32(coef)
+    return lib().sfd_acoustic_cfl_limit(radius, _ptr(c))

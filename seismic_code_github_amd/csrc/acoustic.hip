@@ -1,0 +1,593 @@
+// Acoustic shot plan: device buffers, launch dispatch and the C ABI entry
+// points declared in include/seisfd.h (drop-in for the time loop of
+// forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:237-393).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <tuple>
+
+#include "acoustic_kernel.h"
+#include "common.h"
+
+namespace sfd {
+
+// ---------------------------------------------------------------- helper kernels
+
+// record[r][k] = rec_buf[k][slot[r]] : time-major gather buffer to the
+// reference's trace-major record layout (:385-393).
+__global__ void gather_transpose_kernel(const float* __restrict__ rec_buf, const int* __restrict__ slot,
+                                        float* __restrict__ record, int nrec, int nt, int nslot) {
+    __shared__ float tile[32][33];
+    const int k0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int q = ty; q < 32; q += 8) {
+        const int k = k0 + q, r = r0 + tx;
+        tile[q][tx] = (k < nt && r < nrec) ? rec_buf[(size_t)k * nslot + slot[r]] : 0.f;
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const int r = r0 + q, k = k0 + tx;
+        if (r < nrec && k < nt) record[(size_t)r * nt + k] = tile[tx][q];
+    }
+}
+
+// ---------------------------------------------------------------- launch table
+
+using StepKernel = void (*)(AcousticStepArgs);
+
+template <int R, int NW>
+StepKernel pick_mode(int mode) {
+    switch (mode) {
+    case MODE_FAST_SPONGE: return acoustic_step_kernel<R, NW, MODE_FAST_SPONGE>;
+    case MODE_FAST_CPML: return acoustic_step_kernel<R, NW, MODE_FAST_CPML>;
+    case MODE_REF_SPONGE: return acoustic_step_kernel<R, NW, MODE_REF_SPONGE>;
+    default: return nullptr;
+    }
+}
+
+template <int R>
+StepKernel pick_nw(int nw, int mode) {
+    switch (nw) {
+    case 1: return pick_mode<R, 1>(mode);
+    case 2: return pick_mode<R, 2>(mode);
+    case 4: return pick_mode<R, 4>(mode);
+    default: return nullptr;
+    }
+}
+
+StepKernel pick_step_kernel(int radius, int nw, int mode) {
+    switch (radius) {
+    case 4: return pick_nw<4>(nw, mode);
+    case 5: return pick_nw<5>(nw, mode);
+    case 8: return pick_nw<8>(nw, mode);
+    default: return nullptr;
+    }
+}
+
+}  // namespace sfd
+
+using namespace sfd;
+
+struct sfd_acoustic_plan {
+    sfd_context* ctx = nullptr;
+    int nx = 0, nz = 0, nt = 0, radius = 0, mode = 0;
+    float dx = 0, dz = 0, dt = 0;
+    int width = 0;
+    int nrec = 0, nslot = 0, nsnap = 0, snap_fields = 0, nsnap_fields = 0;
+    std::vector<int> snap_steps;
+    float coef[8] = {};
+    int pml_xw = 0, pml_zw = 0;
+    int nw = 0, chunk = 0;
+    int cur = 0;  // ping-pong index holding the current time level
+
+    DeviceBuffer p[2], vx[2], vz[2];
+    DeviceBuffer kf, bf, ax, az, pml_x, pml_z;
+    DeviceBuffer psi_px[2], psi_vx[2], psi_pz[2], psi_vz[2];
+    DeviceBuffer amp, rec_first, rec_span, rec_buf, rec_slot, record, snaps;
+    StepKernel kernel = nullptr;
+};
+
+namespace {
+
+int validate(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* wavelet) {
+    if (!p || !vp || !wavelet) return fail(ctx, SFD_EINVAL, "null params, vp or wavelet");
+    if (p->radius != 4 && p->radius != 5 && p->radius != 8)
+        return fail(ctx, SFD_EINVAL, "radius must be 4, 5 or 8 (got %d)", p->radius);
+    if (p->nx <= 2 * p->radius || p->nz <= 2 * p->radius)
+        return fail(ctx, SFD_EINVAL, "grid %dx%d too small for radius %d", p->nx, p->nz, p->radius);
+    if (p->nt < 0) return fail(ctx, SFD_EINVAL, "nt < 0");
+    if (!(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0)) return fail(ctx, SFD_EINVAL, "dx, dz, dt must be > 0");
+    if (p->boundary != SFD_BOUNDARY_SPONGE && p->boundary != SFD_BOUNDARY_CPML)
+        return fail(ctx, SFD_EINVAL, "unknown boundary kind %d", p->boundary);
+    if (p->width < 0 || 2 * p->width >= p->nx || 2 * p->width >= p->nz)
+        return fail(ctx, SFD_EINVAL, "boundary width %d does not fit the grid", p->width);
+    if (p->boundary == SFD_BOUNDARY_CPML && !(p->pml_r > 0 && p->pml_r < 1))
+        return fail(ctx, SFD_EINVAL, "C-PML reflection coefficient must be in (0,1)");
+    if (p->arith != SFD_ARITH_FAST && p->arith != SFD_ARITH_REFERENCE)
+        return fail(ctx, SFD_EINVAL, "unknown arithmetic %d", p->arith);
+    if (p->arith == SFD_ARITH_REFERENCE && p->boundary != SFD_BOUNDARY_SPONGE)
+        return fail(ctx, SFD_ENOTSUP, "reference arithmetic exists for the sponge boundary only");
+    if (p->sx < 0 || p->sx >= p->nx || p->sz < 0 || p->sz >= p->nz)
+        return fail(ctx, SFD_EINVAL, "source (%d,%d) outside the grid", p->sx, p->sz);
+    if (p->nrec < 0 || (p->nrec > 0 && (!p->rx || !p->rz))) return fail(ctx, SFD_EINVAL, "bad receiver list");
+    for (int r = 0; r < p->nrec; ++r)
+        if (p->rx[r] < 0 || p->rx[r] >= p->nx || p->rz[r] < 0 || p->rz[r] >= p->nz)
+            return fail(ctx, SFD_EINVAL, "receiver %d (%d,%d) outside the grid", r, p->rx[r], p->rz[r]);
+    if (p->nsnap < 0 || (p->nsnap > 0 && !p->snap_steps)) return fail(ctx, SFD_EINVAL, "bad snapshot list");
+    for (int s = 0; s < p->nsnap; ++s) {
+        if (p->snap_steps[s] < 0 || p->snap_steps[s] >= p->nt || (s > 0 && p->snap_steps[s] <= p->snap_steps[s - 1]))
+            return fail(ctx, SFD_EINVAL, "snapshot steps must be ascending and in [0, nt)");
+    }
+    if (p->nsnap > 0 && (p->snap_fields & 7) == 0) return fail(ctx, SFD_EINVAL, "snap_fields is empty");
+    return SFD_OK;
+}
+
+// Sponge factors of the reference's table (:164-194) as two 1-D profiles.
+// ax[i]: factor of x-edge row i, or 2.0 where row i is interior; az[j]: factor
+// of z-edge column j, 1.0 inside.  The formula is evaluated in double exactly
+// as the reference writes it and rounded to float once.
+float sponge_factor(int d, int L) {
+    return (float)(-0.38 * d * d / (1.0 * L * L) + 1);
+}
+
+void sponge_profiles(int nx, int nz, int L, std::vector<float>& ax, std::vector<float>& az) {
+    ax.assign(nx, 2.0f);
+    az.assign(nz, 1.0f);
+    if (L <= 0) return;
+    for (int i = 0; i < L; ++i) ax[i] = sponge_factor(L - i - 1, L);
+    for (int i = nx - L; i < nx; ++i) ax[i] = sponge_factor(i - nx + L, L);
+    for (int j = 0; j < L; ++j) az[j] = sponge_factor(L - j - 1, L);
+    for (int j = nz - L; j < nz; ++j) az[j] = sponge_factor(j - nz + L, L);
+}
+
+// C-PML (Roden & Gedney 2000; Komatitsch & Martin 2007 with kappa = 1):
+// d(x) = d0 xi^2, d0 = -3 vmax ln(R) / (2 w h), alpha = pi f0 (1 - xi),
+// b = exp(-(d + alpha) dt), a = d (b - 1) / (d + alpha), xi = depth / w at
+// x = i + shift cells; the absorbing cells are [0, w) and [n - w, n).
+void cpml_profile(int n, int w, double h, double dt, double R, double vmax, double f0,
+                  std::vector<float>& prof) {
+    prof.assign(4 * (size_t)n, 0.f);
+    for (int half = 0; half < 2; ++half) {
+        for (int i = 0; i < n; ++i) {
+            const double x = i + 0.5 * half;
+            const double lo = w, hi = n - 1 - w;
+            double depth = 0.0;
+            if (x < lo) depth = lo - x;
+            else if (x > hi) depth = x - hi;
+            double b = 1.0, a = 0.0;
+            if (w > 0 && depth > 0.0) {
+                const double xi = std::min(depth / w, 1.0);
+                const double d = -3.0 * vmax * std::log(R) / (2.0 * w * h) * xi * xi;
+                const double alpha = M_PI * f0 * (1.0 - xi);
+                b = std::exp(-(d + alpha) * dt);
+                a = d * (b - 1.0) / (d + alpha);
+            }
+            prof[(size_t)(2 * half) * n + i] = (float)b;
+            prof[(size_t)(2 * half + 1) * n + i] = (float)a;
+        }
+    }
+}
+
+template <class T>
+hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
+    hipError_t e = b.alloc(n * sizeof(T));
+    if (e != hipSuccess || n == 0) return e;
+    e = hipMemcpyAsync(b.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+void auto_tiling(const sfd_acoustic_plan* pl, int* nw, int* chunk) {
+    // Workgroups of 2 waves; the row chunk is chosen so that the grid holds at
+    // least ~8 waves per CU of the 256 CUs, and never shorter than the 4R-2
+    // rows the queues need to fill.
+    const int upd_z = pl->nz - 2 * pl->radius, upd_x = pl->nx - 2 * pl->radius;
+    int w = 2;
+    const int ntz = (upd_z + 64 * w - 1) / (64 * w);
+    const long target_wg = 256L * 8 / w;
+    int ch = (int)std::max<long>(1, (long)upd_x * ntz / target_wg);
+    ch = std::max(ch, 8 * pl->radius);
+    ch = std::min(ch, upd_x);
+    *nw = w;
+    *chunk = ch;
+}
+
+int zero_state(sfd_acoustic_plan* pl) {
+    hipStream_t s = pl->ctx->stream;
+    const size_t n = (size_t)pl->nx * pl->nz * sizeof(float);
+    for (int b = 0; b < 2; ++b) {
+        SFD_HIP(pl->ctx, hipMemsetAsync(pl->p[b].ptr, 0, n, s));
+        SFD_HIP(pl->ctx, hipMemsetAsync(pl->vx[b].ptr, 0, n, s));
+        SFD_HIP(pl->ctx, hipMemsetAsync(pl->vz[b].ptr, 0, n, s));
+        for (DeviceBuffer* d : {&pl->psi_px[b], &pl->psi_vx[b], &pl->psi_pz[b], &pl->psi_vz[b]})
+            if (d->ptr) SFD_HIP(pl->ctx, hipMemsetAsync(d->ptr, 0, d->bytes, s));
+    }
+    if (pl->rec_buf.ptr) SFD_HIP(pl->ctx, hipMemsetAsync(pl->rec_buf.ptr, 0, pl->rec_buf.bytes, s));
+    pl->cur = 0;
+    return SFD_OK;
+}
+
+// Enqueues steps k0 .. k0+nsteps-1 on the context stream.
+int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
+    sfd_context* ctx = pl->ctx;
+    AcousticStepArgs a{};
+    a.nx = pl->nx;
+    a.nz = pl->nz;
+    a.chunk = pl->chunk;
+    a.kf = pl->kf.as<float>();
+    a.bf = pl->bf.as<float>();
+    a.ax = pl->ax.as<float>();
+    a.az = pl->az.as<float>();
+    a.pml_x = pl->pml_x.as<float>();
+    a.pml_z = pl->pml_z.as<float>();
+    a.pml_xw = pl->pml_xw;
+    a.pml_zw = pl->pml_zw;
+    a.sx = 0;
+    a.sz = 0;
+    a.amp = pl->amp.as<float>();
+    a.rec_first = pl->rec_first.as<int>();
+    a.rec_span = pl->rec_span.as<int4>();
+    a.rec_buf = pl->rec_buf.as<float>();
+    a.nslot = pl->nslot;
+    const bool ref = pl->mode == MODE_REF_SPONGE;
+    for (int m = 0; m < 8; ++m) {
+        a.cx[m] = ref ? pl->coef[m] : pl->coef[m] / pl->dx;
+        a.cz[m] = ref ? pl->coef[m] : pl->coef[m] / pl->dz;
+    }
+    a.dtdx = pl->dt / pl->dx;
+    a.dtdz = pl->dt / pl->dz;
+    a.inv_dx = 1.0 / (double)pl->dx;
+    a.inv_dz = 1.0 / (double)pl->dz;
+
+    const int TZ = 64 * pl->nw;
+    const dim3 grid((pl->nz - 2 * pl->radius + TZ - 1) / TZ,
+                    (pl->nx - 2 * pl->radius + pl->chunk - 1) / pl->chunk);
+    const dim3 block(TZ);
+    const size_t nbytes = (size_t)pl->nx * pl->nz * sizeof(float);
+    size_t snap_next = std::lower_bound(pl->snap_steps.begin(), pl->snap_steps.end(), k0) - pl->snap_steps.begin();
+
+    for (int s = 0; s < nsteps; ++s) {
+        const int k = k0 + s;
+        const int c = pl->cur, o = c ^ 1;
+        a.k = k;
+        a.p_in = pl->p[c].as<float>();
+        a.vx_in = pl->vx[c].as<float>();
+        a.vz_in = pl->vz[c].as<float>();
+        a.p_out = pl->p[o].as<float>();
+        a.vx_out = pl->vx[o].as<float>();
+        a.vz_out = pl->vz[o].as<float>();
+        a.psi_px_in = pl->psi_px[c].as<float>();
+        a.psi_px_out = pl->psi_px[o].as<float>();
+        a.psi_vx_in = pl->psi_vx[c].as<float>();
+        a.psi_vx_out = pl->psi_vx[o].as<float>();
+        a.psi_pz_in = pl->psi_pz[c].as<float>();
+        a.psi_pz_out = pl->psi_pz[o].as<float>();
+        a.psi_vz_in = pl->psi_vz[c].as<float>();
+        a.psi_vz_out = pl->psi_vz[o].as<float>();
+        hipLaunchKernelGGL(pl->kernel, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        pl->cur = o;
+        while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
+            float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
+            const float* src[3] = {pl->p[o].as<float>(), pl->vx[o].as<float>(), pl->vz[o].as<float>()};
+            for (int f = 0; f < 3; ++f) {
+                if (!(pl->snap_fields & (1 << f))) continue;
+                SFD_HIP(ctx, hipMemcpyAsync(dst, src[f], nbytes, hipMemcpyDeviceToDevice, ctx->stream));
+                dst += (size_t)pl->nx * pl->nz;
+            }
+            ++snap_next;
+        }
+    }
+    return SFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp,
+                             const float* den, const float* wavelet, sfd_acoustic_plan** out) {
+    if (!ctx || !out) return SFD_EINVAL;
+    *out = nullptr;
+    int rc = validate(ctx, p, vp, wavelet);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+
+    sfd_acoustic_plan* pl = new (std::nothrow) sfd_acoustic_plan;
+    if (!pl) return fail(ctx, SFD_ENOMEM, "host allocation failed");
+    pl->ctx = ctx;
+    pl->nx = p->nx;
+    pl->nz = p->nz;
+    pl->nt = p->nt;
+    pl->dx = p->dx;
+    pl->dz = p->dz;
+    pl->dt = p->dt;
+    pl->radius = p->radius;
+    pl->width = p->width;
+    pl->mode = (p->boundary == SFD_BOUNDARY_CPML) ? MODE_FAST_CPML
+               : (p->arith == SFD_ARITH_REFERENCE) ? MODE_REF_SPONGE : MODE_FAST_SPONGE;
+    if (p->coef) {
+        for (int m = 0; m < p->radius; ++m) pl->coef[m] = p->coef[m];
+    } else {
+        default_coefs(p->radius, pl->coef);
+    }
+    auto bail = [&](int code) {
+        delete pl;
+        return code;
+    };
+
+    const int nx = p->nx, nz = p->nz;
+    const size_t n = (size_t)nx * nz;
+
+    // Stability: vmax dt sqrt(1/dx^2 + 1/dz^2) sum|C| <= 1.
+    float vmax = 0.f;
+    for (size_t c = 0; c < n; ++c) {
+        if (!(vp[c] > 0.f) || !std::isfinite(vp[c]))
+            return bail(fail(ctx, SFD_EINVAL, "vp must be finite and > 0 (cell %zu)", c));
+        vmax = std::max(vmax, vp[c]);
+        if (den && (!(den[c] > 0.f) || !std::isfinite(den[c])))
+            return bail(fail(ctx, SFD_EINVAL, "den must be finite and > 0 (cell %zu)", c));
+    }
+    const double courant = vmax * (double)p->dt * std::sqrt(1.0 / ((double)p->dx * p->dx) + 1.0 / ((double)p->dz * p->dz));
+    const double limit = sfd_acoustic_cfl_limit(p->radius, pl->coef);
+    if (courant > limit)
+        return bail(fail(ctx, SFD_ECFL, "unstable: vmax*dt*sqrt(1/dx^2+1/dz^2) = %.4f exceeds %.4f", courant, limit));
+
+    // Per-cell coefficients.  FAST: kf = rho vp^2 dt, bf = dt/rho.  REF: kf is
+    // the reference's float product DEN*Vp*Vp*dt (:293) and bf holds DEN.
+    std::vector<float> hk(n), hb(n);
+    const bool ref = pl->mode == MODE_REF_SPONGE;
+    for (size_t c = 0; c < n; ++c) {
+        const float d = den ? den[c] : 2400.0f;
+        hk[c] = d * vp[c] * vp[c] * p->dt;
+        hb[c] = ref ? d : (float)((double)p->dt / (double)d);
+    }
+    // Source amplitude per step, the reference's float product dt*DEN*Vp*signal (:249-250).
+    std::vector<float> hamp(std::max(p->nt, 1), 0.f);
+    {
+        const size_t s = (size_t)p->sx * nz + p->sz;
+        const float d = den ? den[s] : 2400.0f;
+        for (int k = 0; k < p->nt; ++k) hamp[k] = p->dt * d * vp[s] * wavelet[k];
+    }
+    // Receivers: one slot per distinct cell; runs of consecutive z cells in a
+    // row become spans (jlo, jhi, slot).
+    pl->nrec = p->nrec;
+    std::vector<int> hslot(p->nrec);
+    std::vector<int> hfirst(nx + 1, 0);
+    std::vector<int4> hspan;
+    {
+        std::vector<std::tuple<int, int, int>> cells(p->nrec);
+        for (int r = 0; r < p->nrec; ++r) cells[r] = {p->rx[r], p->rz[r], r};
+        std::sort(cells.begin(), cells.end());
+        int slot = -1, prev_x = -1, prev_z = -1;
+        for (auto& [x, z, r] : cells) {
+            if (x != prev_x || z != prev_z) {
+                ++slot;
+                if (!hspan.empty() && x == prev_x && z == prev_z + 1) {
+                    hspan.back().y = z + 1;
+                } else {
+                    hspan.push_back(make_int4(z, z + 1, slot, x));
+                }
+                prev_x = x;
+                prev_z = z;
+            }
+            hslot[r] = slot;
+        }
+        pl->nslot = slot + 1;
+        for (auto& sp : hspan) hfirst[sp.w + 1]++;
+        for (int i = 0; i < nx; ++i) hfirst[i + 1] += hfirst[i];
+    }
+    // Absorbing boundary tables.
+    std::vector<float> hax, haz, hpx, hpz;
+    sponge_profiles(nx, nz, p->boundary == SFD_BOUNDARY_SPONGE ? p->width : 0, hax, haz);
+    if (p->boundary == SFD_BOUNDARY_CPML) {
+        cpml_profile(nx, p->width, p->dx, p->dt, p->pml_r, vmax, p->pml_f0, hpx);
+        cpml_profile(nz, p->width, p->dz, p->dt, p->pml_r, vmax, p->pml_f0, hpz);
+        pl->pml_xw = p->width + 1;
+        pl->pml_zw = p->width + 1;
+    }
+
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        if (e == hipSuccess) e = pl->p[b].alloc(n * 4);
+        if (e == hipSuccess) e = pl->vx[b].alloc(n * 4);
+        if (e == hipSuccess) e = pl->vz[b].alloc(n * 4);
+        if (pl->mode == MODE_FAST_CPML) {
+            if (e == hipSuccess) e = pl->psi_px[b].alloc((size_t)2 * pl->pml_xw * nz * 4);
+            if (e == hipSuccess) e = pl->psi_vx[b].alloc((size_t)2 * pl->pml_xw * nz * 4);
+            if (e == hipSuccess) e = pl->psi_pz[b].alloc((size_t)2 * pl->pml_zw * nx * 4);
+            if (e == hipSuccess) e = pl->psi_vz[b].alloc((size_t)2 * pl->pml_zw * nx * 4);
+        }
+    }
+    if (e == hipSuccess) e = upload(pl->kf, hk.data(), n, s);
+    if (e == hipSuccess) e = upload(pl->bf, hb.data(), n, s);
+    if (e == hipSuccess) e = upload(pl->ax, hax.data(), hax.size(), s);
+    if (e == hipSuccess) e = upload(pl->az, haz.data(), haz.size(), s);
+    if (e == hipSuccess && !hpx.empty()) e = upload(pl->pml_x, hpx.data(), hpx.size(), s);
+    if (e == hipSuccess && !hpz.empty()) e = upload(pl->pml_z, hpz.data(), hpz.size(), s);
+    if (e == hipSuccess) e = upload(pl->amp, hamp.data(), hamp.size(), s);
+    if (e == hipSuccess) e = upload(pl->rec_first, hfirst.data(), hfirst.size(), s);
+    if (e == hipSuccess && !hspan.empty()) e = upload(pl->rec_span, hspan.data(), hspan.size(), s);
+    if (e == hipSuccess && p->nrec > 0) e = upload(pl->rec_slot, hslot.data(), hslot.size(), s);
+    if (e == hipSuccess && p->nrec > 0) e = pl->rec_buf.alloc((size_t)pl->nslot * std::max(p->nt, 1) * 4);
+    if (e == hipSuccess && p->nrec > 0) e = pl->record.alloc((size_t)p->nrec * std::max(p->nt, 1) * 4);
+    pl->nsnap = p->nsnap;
+    pl->snap_fields = p->snap_fields & 7;
+    pl->nsnap_fields = ((pl->snap_fields >> 0) & 1) + ((pl->snap_fields >> 1) & 1) + ((pl->snap_fields >> 2) & 1);
+    pl->snap_steps.assign(p->snap_steps, p->snap_steps + p->nsnap);
+    if (e == hipSuccess && p->nsnap > 0) e = pl->snaps.alloc((size_t)p->nsnap * pl->nsnap_fields * n * 4);
+    if (e != hipSuccess) return bail(hip_check(ctx, e, "plan allocation"));
+
+    auto_tiling(pl, &pl->nw, &pl->chunk);
+    pl->kernel = pick_step_kernel(pl->radius, pl->nw, pl->mode);
+    if (!pl->kernel) return bail(fail(ctx, SFD_ENOTSUP, "no kernel for this configuration"));
+    rc = zero_state(pl);
+    if (rc) return bail(rc);
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return bail(hip_check(ctx, e, "plan setup"));
+    *out = pl;
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan* pl, int waves, int chunk_rows) {
+    if (!pl) return SFD_EINVAL;
+    int nw = waves, ch = chunk_rows;
+    int anw, ach;
+    auto_tiling(pl, &anw, &ach);
+    if (nw == 0) nw = anw;
+    if (ch == 0) ch = ach;
+    if (ch < 1) return fail(pl->ctx, SFD_EINVAL, "chunk_rows must be >= 1");
+    StepKernel k = pick_step_kernel(pl->radius, nw, pl->mode);
+    if (!k) return fail(pl->ctx, SFD_EINVAL, "waves must be 1, 2 or 4 (got %d)", waves);
+    pl->nw = nw;
+    pl->chunk = ch;
+    pl->kernel = k;
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_run(sfd_acoustic_plan* pl, float* elapsed_ms) {
+    if (!pl) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = zero_state(pl);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    rc = enqueue_steps(pl, 0, pl->nt);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_step(sfd_acoustic_plan* pl, int k0, int nsteps) {
+    if (!pl) return SFD_EINVAL;
+    if (k0 < 0 || nsteps < 0 || k0 + nsteps > pl->nt)
+        return fail(pl->ctx, SFD_EINVAL, "steps [%d, %d) outside [0, nt=%d)", k0, k0 + nsteps, pl->nt);
+    SFD_HIP(pl->ctx, hipSetDevice(pl->ctx->device));
+    int rc = enqueue_steps(pl, k0, nsteps);
+    if (rc) return rc;
+    SFD_HIP(pl->ctx, hipStreamSynchronize(pl->ctx->stream));
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_fetch(sfd_acoustic_plan* pl, float* record, float* snaps) {
+    if (!pl) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    if (record && pl->nrec > 0 && pl->nt > 0) {
+        const dim3 grid((pl->nt + 31) / 32, (pl->nrec + 31) / 32), block(32, 8);
+        hipLaunchKernelGGL(gather_transpose_kernel, grid, block, 0, ctx->stream, pl->rec_buf.as<float>(),
+                           pl->rec_slot.as<int>(), pl->record.as<float>(), pl->nrec, pl->nt, pl->nslot);
+        SFD_HIP(ctx, hipGetLastError());
+        SFD_HIP(ctx, hipMemcpyAsync(record, pl->record.ptr, (size_t)pl->nrec * pl->nt * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
+    if (snaps && pl->nsnap > 0)
+        SFD_HIP(ctx, hipMemcpyAsync(snaps, pl->snaps.ptr, pl->snaps.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_set_state(sfd_acoustic_plan* pl, const float* state, const float* pml) {
+    if (!pl || !state) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)pl->nx * pl->nz;
+    hipStream_t s = ctx->stream;
+    SFD_HIP(ctx, hipMemcpyAsync(pl->p[pl->cur].ptr, state, n * 4, hipMemcpyHostToDevice, s));
+    SFD_HIP(ctx, hipMemcpyAsync(pl->vx[pl->cur].ptr, state + n, n * 4, hipMemcpyHostToDevice, s));
+    SFD_HIP(ctx, hipMemcpyAsync(pl->vz[pl->cur].ptr, state + 2 * n, n * 4, hipMemcpyHostToDevice, s));
+    if (pl->mode == MODE_FAST_CPML) {
+        if (!pml) return fail(ctx, SFD_EINVAL, "C-PML plans need the pml state");
+        // Full-grid psi fields -> strips.
+        const int nx = pl->nx, nz = pl->nz, xw = pl->pml_xw, zw = pl->pml_zw;
+        std::vector<float> xs((size_t)2 * xw * nz), zs((size_t)2 * zw * nx);
+        const float* f[4] = {pml, pml + n, pml + 2 * n, pml + 3 * n};  // px, pz, vx, vz
+        DeviceBuffer* xb[2] = {&pl->psi_px[pl->cur], &pl->psi_vx[pl->cur]};
+        const float* xf[2] = {f[0], f[2]};
+        for (int q = 0; q < 2; ++q) {
+            for (int r = 0; r < 2 * xw; ++r) {
+                const int i = r < xw ? r : nx - xw + (r - xw);
+                std::memcpy(&xs[(size_t)r * nz], xf[q] + (size_t)i * nz, nz * 4);
+            }
+            SFD_HIP(ctx, hipMemcpyAsync(xb[q]->ptr, xs.data(), xs.size() * 4, hipMemcpyHostToDevice, s));
+            SFD_HIP(ctx, hipStreamSynchronize(s));
+        }
+        DeviceBuffer* zb[2] = {&pl->psi_pz[pl->cur], &pl->psi_vz[pl->cur]};
+        const float* zf[2] = {f[1], f[3]};
+        for (int q = 0; q < 2; ++q) {
+            for (int i = 0; i < nx; ++i)
+                for (int c = 0; c < 2 * zw; ++c) {
+                    const int jj = c < zw ? c : nz - zw + (c - zw);
+                    zs[(size_t)i * 2 * zw + c] = zf[q][(size_t)i * nz + jj];
+                }
+            SFD_HIP(ctx, hipMemcpyAsync(zb[q]->ptr, zs.data(), zs.size() * 4, hipMemcpyHostToDevice, s));
+            SFD_HIP(ctx, hipStreamSynchronize(s));
+        }
+    }
+    SFD_HIP(ctx, hipStreamSynchronize(s));
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_get_state(sfd_acoustic_plan* pl, float* state, float* pml) {
+    if (!pl || !state) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)pl->nx * pl->nz;
+    hipStream_t s = ctx->stream;
+    SFD_HIP(ctx, hipMemcpyAsync(state, pl->p[pl->cur].ptr, n * 4, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipMemcpyAsync(state + n, pl->vx[pl->cur].ptr, n * 4, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipMemcpyAsync(state + 2 * n, pl->vz[pl->cur].ptr, n * 4, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipStreamSynchronize(s));
+    if (pml && pl->mode == MODE_FAST_CPML) {
+        const int nx = pl->nx, nz = pl->nz, xw = pl->pml_xw, zw = pl->pml_zw;
+        std::fill(pml, pml + 4 * n, 0.f);
+        std::vector<float> xs((size_t)2 * xw * nz), zs((size_t)2 * zw * nx);
+        DeviceBuffer* xb[2] = {&pl->psi_px[pl->cur], &pl->psi_vx[pl->cur]};
+        float* xf[2] = {pml, pml + 2 * n};
+        for (int q = 0; q < 2; ++q) {
+            SFD_HIP(ctx, hipMemcpyAsync(xs.data(), xb[q]->ptr, xs.size() * 4, hipMemcpyDeviceToHost, s));
+            SFD_HIP(ctx, hipStreamSynchronize(s));
+            for (int r = 0; r < 2 * xw; ++r) {
+                const int i = r < xw ? r : nx - xw + (r - xw);
+                std::memcpy(xf[q] + (size_t)i * nz, &xs[(size_t)r * nz], nz * 4);
+            }
+        }
+        DeviceBuffer* zb[2] = {&pl->psi_pz[pl->cur], &pl->psi_vz[pl->cur]};
+        float* zf[2] = {pml + n, pml + 3 * n};
+        for (int q = 0; q < 2; ++q) {
+            SFD_HIP(ctx, hipMemcpyAsync(zs.data(), zb[q]->ptr, zs.size() * 4, hipMemcpyDeviceToHost, s));
+            SFD_HIP(ctx, hipStreamSynchronize(s));
+            for (int i = 0; i < nx; ++i)
+                for (int c = 0; c < 2 * zw; ++c) {
+                    const int jj = c < zw ? c : nz - zw + (c - zw);
+                    zf[q][(size_t)i * nz + jj] = zs[(size_t)i * 2 * zw + c];
+                }
+        }
+    }
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_destroy(sfd_acoustic_plan* pl) {
+    if (!pl) return SFD_EINVAL;
+    (void)hipSetDevice(pl->ctx->device);
+    (void)hipStreamSynchronize(pl->ctx->stream);
+    delete pl;
+    return SFD_OK;
+}
+
+int sfd_acoustic_forward(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
+                         const float* wavelet, float* record, float* snaps) {
+    sfd_acoustic_plan* pl = nullptr;
+    int rc = sfd_acoustic_plan_create(ctx, p, vp, den, wavelet, &pl);
+    if (rc) return rc;
+    rc = sfd_acoustic_plan_run(pl, nullptr);
+    if (!rc) rc = sfd_acoustic_plan_fetch(pl, record, snaps);
+    sfd_acoustic_plan_destroy(pl);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Shared host-side helpers of libseisfd: context, error reporting, device
+// buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "seisfd.h"
+
+struct sfd_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+namespace sfd {
+
+// Records a formatted error on the context and returns `code`.
+int fail(sfd_context* ctx, int code, const char* fmt, ...);
+
+// Returns SFD_EDEVICE with the HIP error text when `e` is an error.
+int hip_check(sfd_context* ctx, hipError_t e, const char* what);
+
+#define SFD_HIP(ctx, call)                                         \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return ::sfd::hip_check(ctx, e_, #call); \
+    } while (0)
+
+// Owning device buffer (hipMalloc / hipFree).
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e == hipSuccess) bytes = n;
+        else ptr = nullptr;
+        return e;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Staggered Taylor coefficients used when the caller passes none: the
+// reference's own tables (Acoustic_1order_FD.cpp:38-45 for R = 8;
+// Iso_multishot_fd_module_zy_m2.f90:17-26 for R = 4 and 5).
+bool default_coefs(int radius, float* c);
+
+}  // namespace sfd

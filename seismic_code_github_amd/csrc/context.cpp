@@ -1,0 +1,106 @@
+// Context and shared helpers of libseisfd.
+#include <cmath>
+#include <new>
+
+#include "common.h"
+
+namespace sfd {
+
+int fail(sfd_context* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int hip_check(sfd_context* ctx, hipError_t e, const char* what) {
+    const int code = (e == hipErrorOutOfMemory) ? SFD_ENOMEM : SFD_EDEVICE;
+    return fail(ctx, code, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool default_coefs(int radius, float* c) {
+    switch (radius) {
+    case 8: {
+        // C1..C8, Acoustic_1order_FD.cpp:38-45.
+        const float t[8] = {1.2340911f, -1.0664985e-1f, 2.3036367e-2f, -5.3423856e-3f,
+                            1.0772712e-3f, -1.6641888e-4f, 1.7021711e-5f, -8.5234642e-7f};
+        for (int m = 0; m < 8; ++m) c[m] = t[m];
+        return true;
+    }
+    case 4: {
+        // a80..a83, Iso_multishot_fd_module_zy_m2.f90:17-20.
+        const float t[4] = {1225.0f / 1024.0f, -245.0f / 3072.0f, 49.0f / 5120.0f,
+                            -5.0f / 7168.0f};
+        for (int m = 0; m < 4; ++m) c[m] = t[m];
+        return true;
+    }
+    case 5: {
+        // a100..a104, Iso_multishot_fd_module_zy_m2.f90:22-26.
+        const float t[5] = {19845.0f / 16384.0f, -735.0f / 8192.0f, 567.0f / 40960.0f,
+                            -405.0f / 229376.0f, 35.0f / 294912.0f};
+        for (int m = 0; m < 5; ++m) c[m] = t[m];
+        return true;
+    }
+    default:
+        return false;
+    }
+}
+
+}  // namespace sfd
+
+extern "C" {
+
+int sfd_abi_version(void) { return SEISFD_ABI_VERSION; }
+
+int sfd_context_create(int device, sfd_context** out) {
+    if (!out) return SFD_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SFD_EDEVICE;
+    if (device < 0 || device >= n) return SFD_EINVAL;
+    sfd_context* ctx = new (std::nothrow) sfd_context;
+    if (!ctx) return SFD_ENOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        sfd_context_destroy(ctx);
+        return SFD_EDEVICE;
+    }
+    *out = ctx;
+    return SFD_OK;
+}
+
+int sfd_context_destroy(sfd_context* ctx) {
+    if (!ctx) return SFD_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SFD_OK;
+}
+
+const char* sfd_last_error(const sfd_context* ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+double sfd_acoustic_cfl_limit(int radius, const float* coef) {
+    float c[8];
+    if (radius < 1 || radius > 8) return 0.0;
+    if (coef) {
+        for (int m = 0; m < radius; ++m) c[m] = coef[m];
+    } else if (!sfd::default_coefs(radius, c)) {
+        return 0.0;
+    }
+    double s = 0.0;
+    for (int m = 0; m < radius; ++m) s += std::fabs((double)c[m]);
+    return s > 0.0 ? 1.0 / s : 0.0;
+}
+
+}  // extern "C"

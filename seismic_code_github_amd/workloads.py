@@ -1,0 +1,113 @@
+"""Deterministic synthetic shots for the acoustic propagator.
+
+The same arrays feed the HIP library and the CPU oracle.  Sizes, models and
+geometries:
+
+* ``reference_program`` — the reference program's own constant block,
+  forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:28-45,137-216: 100x100
+  interior + 20-cell sponge, Nt=2000, dt=0.5 ms, dx=dz=10 m, Vp=2000, DEN=2400,
+  28 Hz Ricker (amplitude 50, delay 1/f), source at the centre, receivers on
+  the vertical line x = Nx/2 through the source (:376-377), R=8 (C1..C8).
+* ``layered_301`` — 301x301 interior + 40-cell sponge (381x381), dx=dz=10 m,
+  dt=1 ms, nt=2000, three layers 1500/2500/3500 m/s with interfaces at interior
+  z=100 and z=200, constant density 2400, 25 Hz Ricker at interior (z=2,
+  x=150), 301 receivers at interior z=2; radius 4 (8th order) by default.
+* ``smooth_random_2001x4001`` — 2001 (z) x 4001 (x) interior + 50-cell C-PML
+  (4101 x 2101 stored), dx=dz=10 m, dt=0.8 ms, nt=6000, velocity = seeded
+  Gaussian-smoothed noise (sigma = 20 cells) mapped to 1500-4500 m/s, 20 Hz
+  Ricker at interior (z=2, x=2000), 4001 receivers at interior z=2, R=8.
+
+Ricker wavelets follow the reference's formula (:199-209): t = k dt in float,
+amp (1 - 2 pi^2 f^2 (t - 1/f)^2) exp(-pi^2 f^2 (t - 1/f)^2) with pi = 3.1415926,
+double arithmetic, stored as float.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import AcousticShot, FIELD_P, FIELD_VX, FIELD_VZ
+
+PI_REF = 3.1415926
+
+
+def ricker(nt: int, dt: float, freq: float, amp: float = 50.0) -> np.ndarray:
+    """Acoustic_1order_FD.cpp:199-209 evaluated the same way (float t, float t1)."""
+    dt32 = np.float32(dt)
+    t = dt32 * np.arange(nt, dtype=np.float32)          # float t = dt*i
+    t1 = np.float32(1.0 / np.float32(freq))              # float t1 = 1.0/freq
+    d = (t - t1).astype(np.float64)                      # float subtraction, then promoted
+    f = np.float64(np.float32(freq))
+    a = 1 - 2 * PI_REF * PI_REF * f * f * d * d
+    e = np.exp(-PI_REF * PI_REF * f * f * d * d)
+    return (np.float64(amp) * a * e).astype(np.float32)
+
+
+def smooth_random_model(nx: int, nz: int, sigma: float, vmin: float, vmax: float, seed: int) -> np.ndarray:
+    """Seeded N(0,1) noise, separable Gaussian filter (sigma cells, truncated at
+    4 sigma, reflecting edges), min-max mapped to [vmin, vmax]."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal((nx, nz), dtype=np.float64)
+    sm = gaussian_filter(noise, sigma=sigma, mode="reflect", truncate=4.0)
+    lo, hi = sm.min(), sm.max()
+    return (vmin + (sm - lo) * (vmax - vmin) / (hi - lo)).astype(np.float32)
+
+
+def reference_program(**overrides) -> AcousticShot:
+    NX = NZ = 100
+    PML = 20
+    Nx, Nz = NX + 2 * PML, NZ + 2 * PML
+    nt = 2000
+    dt = 0.0005
+    vp = np.full((Nx, Nz), 2000.0, dtype=np.float32)
+    den = np.full((Nx, Nz), 2400.0, dtype=np.float32)
+    sx, sz = Nx // 2, Nz // 2
+    rz = np.arange(PML, Nx - PML, dtype=np.int32)
+    rx = np.full_like(rz, sx)
+    kw = dict(vp=vp, den=den, wavelet=ricker(nt, dt, 28.0), src=(sx, sz), rec=(rx, rz),
+              dx=10.0, dz=10.0, dt=dt, radius=8, boundary="sponge", width=PML,
+              snap_steps=(500,), snap_fields=FIELD_VX)
+    kw.update(overrides)
+    return AcousticShot(**kw)
+
+
+def layered_301(radius: int = 4, nt: int = 2000, **overrides) -> AcousticShot:
+    nb = 40
+    n = 301 + 2 * nb
+    z = np.arange(n) - nb
+    col = np.where(z < 100, 1500.0, np.where(z < 200, 2500.0, 3500.0)).astype(np.float32)
+    vp = np.ascontiguousarray(np.broadcast_to(col[None, :], (n, n)), dtype=np.float32)
+    dt = 0.001
+    rx = np.arange(nb, nb + 301, dtype=np.int32)
+    rz = np.full_like(rx, nb + 2)
+    kw = dict(vp=vp, den=None, wavelet=ricker(nt, dt, 25.0), src=(nb + 150, nb + 2), rec=(rx, rz),
+              dx=10.0, dz=10.0, dt=dt, radius=radius, boundary="sponge", width=nb,
+              snap_steps=(), snap_fields=FIELD_P | FIELD_VX | FIELD_VZ)
+    kw.update(overrides)
+    return AcousticShot(**kw)
+
+
+def smooth_random_2001x4001(nt: int = 6000, radius: int = 8, seed: int = 1, scale: int = 1,
+                            **overrides) -> AcousticShot:
+    """``scale`` > 1 shrinks the interior by that factor (same physics per cell)
+    for parity tests at oracle-friendly sizes."""
+    nb = 50
+    nzi, nxi = 2000 // scale + 1, 4000 // scale + 1
+    nx, nz = nxi + 2 * nb, nzi + 2 * nb
+    vp = smooth_random_model(nx, nz, 20.0 / scale, 1500.0, 4500.0, seed)
+    dt = 0.0008
+    rx = np.arange(nb, nb + nxi, dtype=np.int32)
+    rz = np.full_like(rx, nb + 2)
+    kw = dict(vp=vp, den=None, wavelet=ricker(nt, dt, 20.0), src=(nb + nxi // 2, nb + 2), rec=(rx, rz),
+              dx=10.0, dz=10.0, dt=dt, radius=radius, boundary="cpml", width=nb, pml_r=1e-3,
+              pml_f0=20.0, snap_steps=(), snap_fields=FIELD_P)
+    kw.update(overrides)
+    return AcousticShot(**kw)
+
+
+def cell_updates(shot: AcousticShot) -> int:
+    """Cells advanced per shot: every stored cell (boundary layers included)
+    times the number of steps."""
+    nx, nz = np.asarray(shot.vp).shape
+    nt = shot.nt if shot.nt is not None else np.asarray(shot.wavelet).shape[0]
+    return nx * nz * nt

@@ -1,0 +1,178 @@
+"""Pins of the CPU oracle (oracle/acoustic.c) — CPU only.
+
+* Against the reference's own outputs: the full record.dat and Snap.dat the
+  reference ships (forward/Acoustic_FD_by_ss/file/, kept here as sha256 +
+  traces, tests/golden/) and, where it was built, the reference program
+  compiled from its source (oracle/_ref).
+* Against mathematics: Taylor conditions of the staggered coefficients, the
+  Ricker and sponge closed forms, exact energy conservation of the leapfrog
+  without the sponge (which fails for any stencil that is not the negative
+  transpose of its partner), linearity, FP64 agreement.
+"""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import oracle_threads
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+
+
+@pytest.fixture(scope="module")
+def ref_run():
+    cfg = oracle.reference_config()
+    return cfg, oracle.acoustic(**cfg, snap_steps=[500])
+
+
+def _meta():
+    with open(os.path.join(GOLD, "acoustic_reference_program.json")) as f:
+        return json.load(f)
+
+
+def test_record_matches_shipped_reference_output_bitwise(ref_run):
+    cfg, out = ref_run
+    rec = out["record"]
+    meta = _meta()
+    assert rec.shape == (100, 2000)
+    assert hashlib.sha256(rec.astype("<f4").tobytes()).hexdigest() == meta["record.dat"]["sha256"]
+    traces = np.load(os.path.join(GOLD, "acoustic_record_traces.npy"))
+    np.testing.assert_array_equal(rec[meta["record_traces"]], traces)
+
+
+def test_snapshot_matches_shipped_reference_output_bitwise(ref_run):
+    _, out = ref_run
+    vx = out["snaps"][0, 1, 20:120, 20:120]  # Vx interior at k = 500 (:343-355)
+    gold = np.load(os.path.join(GOLD, "acoustic_snap_vx500.npy"))
+    np.testing.assert_array_equal(vx, gold)
+    assert hashlib.sha256(vx.astype("<f4").tobytes()).hexdigest() == _meta()["Snap.dat"]["sha256"]
+
+
+def test_against_reference_program_built_here(ref_run):
+    d = os.path.join(oracle.REF_DIR, "run", "file")
+    if not os.path.exists(os.path.join(d, "record.dat")):
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    _, out = ref_run
+    rec = np.fromfile(os.path.join(d, "record.dat"), dtype="<f4").reshape(100, 2000)
+    np.testing.assert_array_equal(out["record"], rec)
+    # P movie: P1 interior every 10 steps (:358-373).
+    pall = np.fromfile(os.path.join(d, "Pall.dat"), dtype="<f4").reshape(200, 100, 100)
+    cfg = oracle.reference_config()
+    mv = oracle.acoustic(**cfg, snap_steps=list(range(0, 2000, 10)))["snaps"][:, 0, 20:120, 20:120]
+    np.testing.assert_array_equal(mv, pall)
+
+
+def test_fp64_agreement_reference_program(ref_run):
+    cfg, out = ref_run
+    o64 = oracle.acoustic(**cfg, precision="f64")
+    r64 = o64["record"]
+    rel = np.linalg.norm(out["record"] - r64) / np.linalg.norm(r64)
+    assert rel < 2e-6, rel
+
+
+def test_staggered_coefficients_taylor_conditions():
+    for R in range(1, 9):
+        c = oracle.staggered_coefs(R)
+        for k in range(1, R + 1):
+            # sum_m c_m ((m-1/2)^(2k-1) - (-(m-1/2))^(2k-1)) = [k == 1]
+            terms = [c[m - 1] * 2 * (m - 0.5) ** (2 * k - 1) for m in range(1, R + 1)]
+            scale = sum(abs(t) for t in terms)
+            assert abs(sum(terms) - (1.0 if k == 1 else 0.0)) < 1e-13 * scale, (R, k, sum(terms))
+    # The reference's tables.
+    c8 = oracle.staggered_coefs(8)
+    np.testing.assert_allclose(oracle.REF_C8.astype(np.float64), c8, rtol=1e-7)
+    np.testing.assert_array_equal(oracle.REF_A8, oracle.staggered_coefs(4).astype(np.float32))
+    np.testing.assert_array_equal(oracle.REF_A10, oracle.staggered_coefs(5).astype(np.float32))
+
+
+def test_ricker_closed_form():
+    nt, dt, f = 2000, np.float32(0.0005), 28.0
+    w = oracle.ricker(nt, float(dt), f, 50.0)
+    pi = 3.1415926
+    for k in [0, 1, 35, 36, 70, 1999]:
+        t = float(np.float32(dt * np.float32(k)))
+        d = float(np.float32(np.float32(t) - np.float32(1.0 / f)))
+        x = pi * pi * f * f * d * d
+        assert w[k] == np.float32(50 * (1 - 2 * x) * math.exp(-x))
+    # peak of 50 at t = 1/f (between samples here: 1/f = 71.4 dt)
+    assert 49.9 < w.max() <= 50.0 and abs(int(np.argmax(w)) - round((1 / f) / float(dt))) <= 1
+
+
+def test_sponge_closed_form_and_corners():
+    nx, nz, L = 50, 60, 10
+    a = oracle.sponge_table(nx, nz, L)
+    f = lambda d: np.float32(-0.38 * d * d / (1.0 * L * L) + 1)
+    assert a[25, 30] == 1.0
+    assert a[0, 30] == f(L - 1) and a[L - 1, 30] == 1.0 and a[nx - 1, 30] == f(L - 1)
+    assert a[25, 0] == f(L - 1) and a[25, nz - 1] == f(L - 1)
+    # corners take the x-edge (row) factor (the row loops run last, :180-194)
+    assert a[0, 0] == f(L - 1) and a[L - 1, 0] == 1.0 and a[nx - 2, nz - 1] == f(L - 2)
+    assert abs(float(f(L - 1)) - 0.6922) < 1e-4
+
+
+def _bump_state(nx, nz, cx, cz, sig):
+    i = np.arange(nx)[:, None]
+    j = np.arange(nz)[None, :]
+    p = np.exp(-((i - cx) ** 2 + (j - cz) ** 2) / (2.0 * sig * sig))
+    st = np.zeros((3, nx, nz))
+    st[0] = p
+    return st
+
+
+@pytest.mark.parametrize("radius", [4, 8])
+def test_energy_conserved_without_sponge(radius):
+    """The staggered leapfrog conserves E_k = <p,p>/K + rho <v_{k+1/2}, v_{k-1/2}>
+    exactly when each difference operator is the negative transpose of its
+    partner; with no sponge and no source the FP64 oracle must keep it to
+    round-off."""
+    nx, nz, R = 64, 48, radius
+    rng = np.random.default_rng(0)
+    vp = (2000 + 500 * rng.random((nx, nz))).astype(np.float32)
+    den = (2000 + 400 * rng.random((nx, nz))).astype(np.float32)
+    st = _bump_state(nx, nz, 30, 22, 3.0)
+    st[0][:R, :] = 0; st[0][-R:, :] = 0; st[0][:, :R] = 0; st[0][:, -R:] = 0
+    n = 120
+    out = oracle.acoustic(vp, den, np.zeros(n, np.float32), (30, 22), ([], []), dt=1e-3, radius=R,
+                          sponge=0, snap_steps=list(range(n)), state=st, precision="f64")
+    s = out["snaps"]
+    K = den.astype(np.float64) * vp.astype(np.float64) ** 2
+    rho = den.astype(np.float64)
+    E = [np.sum(s[k, 0] ** 2 / K) + np.sum(rho * (s[k + 1, 1] * s[k, 1] + s[k + 1, 2] * s[k, 2]))
+         for k in range(n - 1)]
+    E = np.array(E)
+    assert E[0] > 0
+    assert np.max(np.abs(E - E[0])) / E[0] < 1e-10, np.max(np.abs(E - E[0])) / E[0]
+
+
+def test_linearity(ref_run):
+    """Doubling the wavelet doubles the record.  Not bitwise: values in the
+    subnormal range (the stencil's far tail in the first steps) round
+    differently at 2y than at y, and the last-bit differences then spread."""
+    cfg, out = ref_run
+    cfg2 = dict(cfg)
+    cfg2["wavelet"] = (cfg["wavelet"] * np.float32(2.0)).astype(np.float32)
+    out2 = oracle.acoustic(**cfg2)
+    ref = out["record"].astype(np.float64) * 2
+    rel = np.linalg.norm(out2["record"] - ref) / np.linalg.norm(ref)
+    assert rel < 1e-6, rel
+
+
+def test_layered_301_fp32_vs_fp64():
+    from seismic_code_github_amd import workloads as W
+    shot = W.layered_301(radius=4, nt=600)
+    kw = dict(dt=shot.dt, radius=4, sponge=shot.width, nthreads=oracle_threads())
+    den = np.full_like(shot.vp, 2400.0)
+    o32 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, **kw)
+    o64 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, precision="f64", **kw)
+    rel = np.linalg.norm(o32["record"] - o64["record"]) / np.linalg.norm(o64["record"])
+    assert rel < 1e-5, rel
