@@ -39,9 +39,9 @@ using StepKernel = void (*)(AcousticStepArgs);
 template <int R, int NW>
 StepKernel pick_mode(int mode) {
     switch (mode) {
-    case MODE_FAST_SPONGE: return acoustic_step_kernel<R, NW, MODE_FAST_SPONGE>;
-    case MODE_FAST_CPML: return acoustic_step_kernel<R, NW, MODE_FAST_CPML>;
-    case MODE_REF_SPONGE: return acoustic_step_kernel<R, NW, MODE_REF_SPONGE>;
+    case MODE_FAST_SPONGE: return acoustic_step_kernel<R, NW, MODE_FAST_SPONGE, default_slots<R>()>;
+    case MODE_FAST_CPML: return acoustic_step_kernel<R, NW, MODE_FAST_CPML, default_slots<R>()>;
+    case MODE_REF_SPONGE: return acoustic_step_kernel<R, NW, MODE_REF_SPONGE, default_slots<R>()>;
     default: return nullptr;
     }
 }
@@ -78,6 +78,7 @@ struct sfd_acoustic_plan {
     std::vector<int> snap_steps;
     float coef[8] = {};
     int pml_xw = 0, pml_zw = 0;
+    int sx = 0, sz = 0;
     int nw = 0, chunk = 0;
     int cur = 0;  // ping-pong index holding the current time level
 
@@ -223,8 +224,8 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     a.pml_z = pl->pml_z.as<float>();
     a.pml_xw = pl->pml_xw;
     a.pml_zw = pl->pml_zw;
-    a.sx = 0;
-    a.sz = 0;
+    a.sx = pl->sx;
+    a.sz = pl->sz;
     a.amp = pl->amp.as<float>();
     a.rec_first = pl->rec_first.as<int>();
     a.rec_span = pl->rec_span.as<int4>();
@@ -305,6 +306,8 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
     pl->dt = p->dt;
     pl->radius = p->radius;
     pl->width = p->width;
+    pl->sx = p->sx;
+    pl->sz = p->sz;
     pl->mode = (p->boundary == SFD_BOUNDARY_CPML) ? MODE_FAST_CPML
                : (p->arith == SFD_ARITH_REFERENCE) ? MODE_REF_SPONGE : MODE_FAST_SPONGE;
     if (p->coef) {

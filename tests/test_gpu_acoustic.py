@@ -72,23 +72,31 @@ def test_reference_program_fast(ctx):
 
 @pytest.mark.parametrize("radius", [4, 8])
 def test_layered_301_fast(ctx, radius):
-    """301x301 interior, 40-cell sponge, nt=2000: gathers and final snapshot
-    within 1e-5 of the FP32 and FP64 oracles."""
-    shot = W.layered_301(radius=radius, snap_steps=(999, 1999))
+    """301x301 interior, 40-cell sponge, nt=2000.  Gathers and the snapshots
+    taken while the wavefield is inside the model (steps 500, 999) are within
+    1e-5 of both the FP32 and the FP64 oracle.  The last snapshot (step 1999)
+    holds only the residue the sponge left (the direct wave has gone); there
+    the FP32 oracle is itself ~1e-5 from FP64, so it is held to 1e-4."""
+    steps = (500, 999, 1999)
+    shot = W.layered_301(radius=radius, snap_steps=steps)
     print(f"case: layered_301 R={radius} fast", flush=True)
     rec, snaps = sfd.acoustic_forward(shot, ctx)
-    o32 = run_oracle(shot, snap_steps=[999, 1999])
-    o64 = run_oracle(shot, "f64", snap_steps=[999, 1999])
+    o32 = run_oracle(shot, snap_steps=list(steps))
+    o64 = run_oracle(shot, "f64", snap_steps=list(steps))
     out = {"rec32": rel_l2(rec, o32["record"]), "rec64": rel_l2(rec, o64["record"]),
            "oracle32_vs_64": rel_l2(o32["record"], o64["record"])}
-    for s in range(2):
+    for s in range(len(steps)):
         for f in range(3):
-            out[f"snap{s}f{f}_32"] = rel_l2(snaps[s, f], o32["snaps"][s, f])
-            out[f"snap{s}f{f}_64"] = rel_l2(snaps[s, f], o64["snaps"][s, f])
+            out[f"snap{steps[s]}f{f}_32"] = rel_l2(snaps[s, f], o32["snaps"][s, f])
+            out[f"snap{steps[s]}f{f}_64"] = rel_l2(snaps[s, f], o64["snaps"][s, f])
+            out[f"oracle_snap{steps[s]}f{f}_32v64"] = rel_l2(o32["snaps"][s, f], o64["snaps"][s, f])
     print(f"layered_301 R={radius}: " + " ".join(f"{k}={v:.2e}" for k, v in out.items()))
     assert out["rec32"] <= TOL and out["rec64"] <= TOL
-    for s in range(2):
-        assert out[f"snap{s}f0_32"] <= TOL and out[f"snap{s}f0_64"] <= TOL
+    for k, v in out.items():
+        if k.startswith("snap1999"):
+            assert v <= 10 * TOL, (k, v)
+        elif k.startswith("snap"):
+            assert v <= TOL, (k, v)
 
 
 def test_layered_301_reference_arith_bitwise(ctx):
@@ -112,6 +120,7 @@ def test_tilings_bitwise_identical(ctx, waves, chunk):
     pl.set_tiling(waves, chunk)
     pl.run()
     r1, s1 = pl.fetch()
+    assert np.abs(r0).max() > 0 and np.abs(s0).max() > 0
     np.testing.assert_array_equal(r0, r1)
     np.testing.assert_array_equal(s0, s1)
 
@@ -132,6 +141,7 @@ def test_resume_equals_uninterrupted(ctx):
     st_b, _ = b.get_state()
     rec_b, _ = b.fetch()
     rec_a, _ = a.fetch()
+    assert np.abs(st_full).max() > 0 and np.abs(rec_full).max() > 0
     np.testing.assert_array_equal(st_b, st_full)
     np.testing.assert_array_equal(rec_a[:, :150], rec_full[:, :150])
     np.testing.assert_array_equal(rec_b[:, 150:], rec_full[:, 150:])
@@ -162,6 +172,7 @@ def test_cpml_state_resume(ctx):
     b.set_state(mid, pml_mid)
     b.step(120, 180)
     st_b, pml_b = b.get_state()
+    assert np.abs(st_full).max() > 0 and np.abs(pml_full).max() > 0
     np.testing.assert_array_equal(st_b, st_full)
     np.testing.assert_array_equal(pml_b, pml_full)
     # the psi fields match the oracle's full-grid memory variables
