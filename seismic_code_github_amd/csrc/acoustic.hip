@@ -56,6 +56,15 @@ StepKernel pick_nw(int nw, int mode) {
     }
 }
 
+int pad_rows(int radius) {
+    switch (radius) {
+    case 4: return kPadRows<4>;
+    case 5: return kPadRows<5>;
+    case 8: return kPadRows<8>;
+    default: return 0;
+    }
+}
+
 StepKernel pick_step_kernel(int radius, int nw, int mode) {
     switch (radius) {
     case 4: return pick_nw<4>(nw, mode);
@@ -79,14 +88,22 @@ struct sfd_acoustic_plan {
     float coef[8] = {};
     int pml_xw = 0, pml_zw = 0;
     int sx = 0, sz = 0;
+    int pad = 0;  // zero rows above and below the grid in the state buffers
     int nw = 0, chunk = 0;
     int cur = 0;  // ping-pong index holding the current time level
 
-    DeviceBuffer p[2], vx[2], vz[2];
-    DeviceBuffer kf, bf, ax, az, pml_x, pml_z;
-    DeviceBuffer psi_px[2], psi_vx[2], psi_pz[2], psi_vz[2];
-    DeviceBuffer amp, rec_first, rec_span, rec_buf, rec_slot, record, snaps;
+    // One arena holds every field the step kernel addresses (state, model,
+    // C-PML strips), so the kernel needs a single buffer descriptor.
+    DeviceBuffer arena;
+    Field p[2], vx[2], vz[2];
+    Field kf, bf;
+    Field psi_px[2], psi_vx[2], psi_pz[2], psi_vz[2];
+    DeviceBuffer ax, az, pml_x, pml_z;
+    DeviceBuffer amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
     StepKernel kernel = nullptr;
+
+    // Row 0 of a padded per-row allocation.
+    float* grid(const Field& b) const { return b.as<float>() + (size_t)pad * nz; }
 };
 
 namespace {
@@ -179,29 +196,34 @@ hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
     return hipStreamSynchronize(s);
 }
 
+// Chunks start at rows congruent to R modulo 2R (the kernel's queue slots are
+// compile-time constants), so the row count of a chunk is a multiple of 2R.
+int round_chunk(int ch, int radius) {
+    const int q = 2 * radius;
+    return std::max(q, (ch + q - 1) / q * q);
+}
+
 void auto_tiling(const sfd_acoustic_plan* pl, int* nw, int* chunk) {
-    // Workgroups of 2 waves; the row chunk is chosen so that the grid holds at
-    // least ~8 waves per CU of the 256 CUs, and never shorter than the 4R-2
-    // rows the queues need to fill.
+    // Workgroups of 2 waves; the row chunk is chosen so the grid holds about
+    // 3072 waves (12 per CU), and never shorter than 48 rows: the queue fill
+    // costs about 4R marched rows per chunk.  (Measured on MI355X for the
+    // 2101 x 4101 grid: 2 waves x 48-64 rows is the fastest shape.)
     const int upd_z = pl->nz - 2 * pl->radius, upd_x = pl->nx - 2 * pl->radius;
-    int w = 2;
-    const int ntz = (upd_z + 64 * w - 1) / (64 * w);
-    const long target_wg = 256L * 8 / w;
-    int ch = (int)std::max<long>(1, (long)upd_x * ntz / target_wg);
-    ch = std::max(ch, 8 * pl->radius);
-    ch = std::min(ch, upd_x);
+    const int w = upd_z >= 128 ? 2 : 1;
+    const long waves_z = (upd_z + 63) / 64;
+    int ch = (int)std::max<long>(48, (long)upd_x * waves_z / 3072);
+    ch = std::min(ch, std::max(upd_x, 1));
     *nw = w;
-    *chunk = ch;
+    *chunk = round_chunk(ch, pl->radius);
 }
 
 int zero_state(sfd_acoustic_plan* pl) {
     hipStream_t s = pl->ctx->stream;
-    const size_t n = (size_t)pl->nx * pl->nz * sizeof(float);
     for (int b = 0; b < 2; ++b) {
-        SFD_HIP(pl->ctx, hipMemsetAsync(pl->p[b].ptr, 0, n, s));
-        SFD_HIP(pl->ctx, hipMemsetAsync(pl->vx[b].ptr, 0, n, s));
-        SFD_HIP(pl->ctx, hipMemsetAsync(pl->vz[b].ptr, 0, n, s));
-        for (DeviceBuffer* d : {&pl->psi_px[b], &pl->psi_vx[b], &pl->psi_pz[b], &pl->psi_vz[b]})
+        SFD_HIP(pl->ctx, hipMemsetAsync(pl->p[b].ptr, 0, pl->p[b].bytes, s));
+        SFD_HIP(pl->ctx, hipMemsetAsync(pl->vx[b].ptr, 0, pl->vx[b].bytes, s));
+        SFD_HIP(pl->ctx, hipMemsetAsync(pl->vz[b].ptr, 0, pl->vz[b].bytes, s));
+        for (Field* d : {&pl->psi_px[b], &pl->psi_vx[b], &pl->psi_pz[b], &pl->psi_vz[b]})
             if (d->ptr) SFD_HIP(pl->ctx, hipMemsetAsync(d->ptr, 0, d->bytes, s));
     }
     if (pl->rec_buf.ptr) SFD_HIP(pl->ctx, hipMemsetAsync(pl->rec_buf.ptr, 0, pl->rec_buf.bytes, s));
@@ -213,22 +235,25 @@ int zero_state(sfd_acoustic_plan* pl) {
 int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     sfd_context* ctx = pl->ctx;
     AcousticStepArgs a{};
+    a.arena = pl->arena.as<float>();
+    a.arena_bytes = (uint32_t)pl->arena.bytes;
     a.nx = pl->nx;
     a.nz = pl->nz;
     a.chunk = pl->chunk;
-    a.kf = pl->kf.as<float>();
-    a.bf = pl->bf.as<float>();
-    a.ax = pl->ax.as<float>();
+    a.kf = pl->grid(pl->kf);
+    a.bf = pl->grid(pl->bf);
+    a.ax = pl->ax.as<float>() + pl->pad;
     a.az = pl->az.as<float>();
     a.pml_x = pl->pml_x.as<float>();
     a.pml_z = pl->pml_z.as<float>();
     a.pml_xw = pl->pml_xw;
     a.pml_zw = pl->pml_zw;
+    a.sponge_w = pl->mode == MODE_FAST_CPML ? 0 : pl->width;
     a.sx = pl->sx;
     a.sz = pl->sz;
     a.amp = pl->amp.as<float>();
-    a.rec_first = pl->rec_first.as<int>();
-    a.rec_span = pl->rec_span.as<int4>();
+    a.rec_row = pl->rec_row.as<int4>() + pl->pad;
+    a.rec_more = pl->rec_more.as<int4>();
     a.rec_buf = pl->rec_buf.as<float>();
     a.nslot = pl->nslot;
     const bool ref = pl->mode == MODE_REF_SPONGE;
@@ -252,12 +277,12 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
         const int k = k0 + s;
         const int c = pl->cur, o = c ^ 1;
         a.k = k;
-        a.p_in = pl->p[c].as<float>();
-        a.vx_in = pl->vx[c].as<float>();
-        a.vz_in = pl->vz[c].as<float>();
-        a.p_out = pl->p[o].as<float>();
-        a.vx_out = pl->vx[o].as<float>();
-        a.vz_out = pl->vz[o].as<float>();
+        a.p_in = pl->grid(pl->p[c]);
+        a.vx_in = pl->grid(pl->vx[c]);
+        a.vz_in = pl->grid(pl->vz[c]);
+        a.p_out = pl->grid(pl->p[o]);
+        a.vx_out = pl->grid(pl->vx[o]);
+        a.vz_out = pl->grid(pl->vz[o]);
         a.psi_px_in = pl->psi_px[c].as<float>();
         a.psi_px_out = pl->psi_px[o].as<float>();
         a.psi_vx_in = pl->psi_vx[c].as<float>();
@@ -271,7 +296,7 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
         pl->cur = o;
         while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
             float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
-            const float* src[3] = {pl->p[o].as<float>(), pl->vx[o].as<float>(), pl->vz[o].as<float>()};
+            const float* src[3] = {pl->grid(pl->p[o]), pl->grid(pl->vx[o]), pl->grid(pl->vz[o])};
             for (int f = 0; f < 3; ++f) {
                 if (!(pl->snap_fields & (1 << f))) continue;
                 SFD_HIP(ctx, hipMemcpyAsync(dst, src[f], nbytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -341,10 +366,17 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
     // the reference's float product DEN*Vp*Vp*dt (:293) and bf holds DEN.
     std::vector<float> hk(n), hb(n);
     const bool ref = pl->mode == MODE_REF_SPONGE;
-    for (size_t c = 0; c < n; ++c) {
-        const float d = den ? den[c] : 2400.0f;
-        hk[c] = d * vp[c] * vp[c] * p->dt;
-        hb[c] = ref ? d : (float)((double)p->dt / (double)d);
+    // In fast arithmetic both are zero outside the updated cells [R, N-R)^2,
+    // so the kernel's reads there yield zero velocities without masking.
+    const int R = p->radius;
+    for (int i = 0; i < nx; ++i) {
+        for (int jj = 0; jj < nz; ++jj) {
+            const size_t c = (size_t)i * nz + jj;
+            const float d = den ? den[c] : 2400.0f;
+            const bool upd = i >= R && i < nx - R && jj >= R && jj < nz - R;
+            hk[c] = (ref || upd) ? d * vp[c] * vp[c] * p->dt : 0.f;
+            hb[c] = ref ? d : upd ? (float)((double)p->dt / (double)d) : 0.f;
+        }
     }
     // Source amplitude per step, the reference's float product dt*DEN*Vp*signal (:249-250).
     std::vector<float> hamp(std::max(p->nt, 1), 0.f);
@@ -354,23 +386,26 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
         for (int k = 0; k < p->nt; ++k) hamp[k] = p->dt * d * vp[s] * wavelet[k];
     }
     // Receivers: one slot per distinct cell; runs of consecutive z cells in a
-    // row become spans (jlo, jhi, slot).
+    // row become spans (jlo, jhi, slot).  A row's first span goes to its
+    // entry in rec_row, further spans to rec_more, chained.
     pl->nrec = p->nrec;
+    pl->pad = pad_rows(p->radius);
+    const int pad = pl->pad;
     std::vector<int> hslot(p->nrec);
-    std::vector<int> hfirst(nx + 1, 0);
-    std::vector<int4> hspan;
+    std::vector<int4> hrow(nx + 2 * pad, make_int4(0, 0, 0, -1)), hmore;
     {
         std::vector<std::tuple<int, int, int>> cells(p->nrec);
         for (int r = 0; r < p->nrec; ++r) cells[r] = {p->rx[r], p->rz[r], r};
         std::sort(cells.begin(), cells.end());
+        std::vector<int4> spans;  // (jlo, jhi, slot, row)
         int slot = -1, prev_x = -1, prev_z = -1;
         for (auto& [x, z, r] : cells) {
             if (x != prev_x || z != prev_z) {
                 ++slot;
-                if (!hspan.empty() && x == prev_x && z == prev_z + 1) {
-                    hspan.back().y = z + 1;
+                if (!spans.empty() && x == prev_x && z == prev_z + 1) {
+                    spans.back().y = z + 1;
                 } else {
-                    hspan.push_back(make_int4(z, z + 1, slot, x));
+                    spans.push_back(make_int4(z, z + 1, slot, x));
                 }
                 prev_x = x;
                 prev_z = z;
@@ -378,8 +413,22 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
             hslot[r] = slot;
         }
         pl->nslot = slot + 1;
-        for (auto& sp : hspan) hfirst[sp.w + 1]++;
-        for (int i = 0; i < nx; ++i) hfirst[i + 1] += hfirst[i];
+        // tail[row]: -2 = no span yet, -1 = head only, else last index in hmore.
+        std::vector<int> tail(nx, -2);
+        for (const int4& s4 : spans) {
+            const int row = s4.w;
+            const int4 sp = make_int4(s4.x, s4.y, s4.z, -1);
+            if (tail[row] == -2) {
+                hrow[row + pad] = sp;
+                tail[row] = -1;
+            } else {
+                hmore.push_back(sp);
+                const int idx = (int)hmore.size() - 1;
+                if (tail[row] == -1) hrow[row + pad].w = idx;
+                else hmore[tail[row]].w = idx;
+                tail[row] = idx;
+            }
+        }
     }
     // Absorbing boundary tables.
     std::vector<float> hax, haz, hpx, hpz;
@@ -393,26 +442,49 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
 
     hipStream_t s = ctx->stream;
     hipError_t e = hipSuccess;
-    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
-        if (e == hipSuccess) e = pl->p[b].alloc(n * 4);
-        if (e == hipSuccess) e = pl->vx[b].alloc(n * 4);
-        if (e == hipSuccess) e = pl->vz[b].alloc(n * 4);
-        if (pl->mode == MODE_FAST_CPML) {
-            if (e == hipSuccess) e = pl->psi_px[b].alloc((size_t)2 * pl->pml_xw * nz * 4);
-            if (e == hipSuccess) e = pl->psi_vx[b].alloc((size_t)2 * pl->pml_xw * nz * 4);
-            if (e == hipSuccess) e = pl->psi_pz[b].alloc((size_t)2 * pl->pml_zw * nx * 4);
-            if (e == hipSuccess) e = pl->psi_vz[b].alloc((size_t)2 * pl->pml_zw * nx * 4);
+    const size_t npad = (size_t)(nx + 2 * pad) * nz;
+    const bool cpml = pl->mode == MODE_FAST_CPML;
+    const size_t xs = cpml ? (size_t)2 * pl->pml_xw * nz * 4 : 0, zs = cpml ? (size_t)2 * pl->pml_zw * nx * 4 : 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        Carver cv;
+        if (pass == 1) cv.base = pl->arena.as<char>();
+        for (int b = 0; b < 2; ++b) {
+            pl->p[b] = cv.take(npad * 4);
+            pl->vx[b] = cv.take(npad * 4);
+            pl->vz[b] = cv.take(npad * 4);
+        }
+        pl->kf = cv.take(npad * 4);
+        pl->bf = cv.take(npad * 4);
+        for (int b = 0; b < 2; ++b) {
+            pl->psi_px[b] = cv.take(xs);
+            pl->psi_vx[b] = cv.take(xs);
+            pl->psi_pz[b] = cv.take(zs);
+            pl->psi_vz[b] = cv.take(zs);
+        }
+        if (pass == 0) {
+            if (cv.used >= (size_t)1 << 31)
+                return bail(fail(ctx, SFD_ENOTSUP, "grid needs %zu bytes of fields; the step kernel addresses < 2 GiB",
+                                 cv.used));
+            e = pl->arena.alloc(cv.used);
+            if (e != hipSuccess) return bail(hip_check(ctx, e, "arena allocation"));
         }
     }
-    if (e == hipSuccess) e = upload(pl->kf, hk.data(), n, s);
-    if (e == hipSuccess) e = upload(pl->bf, hb.data(), n, s);
-    if (e == hipSuccess) e = upload(pl->ax, hax.data(), hax.size(), s);
+    // Per-row arrays are padded with kPadRows rows on both sides (zeros; the
+    // sponge row table with 2.0, "not an x-edge row").
+    std::vector<float> hk_pad((size_t)(nx + 2 * pad) * nz, 0.f), hb_pad(hk_pad.size(), 0.f);
+    std::copy(hk.begin(), hk.end(), hk_pad.begin() + (size_t)pad * nz);
+    std::copy(hb.begin(), hb.end(), hb_pad.begin() + (size_t)pad * nz);
+    std::vector<float> hax_pad(nx + 2 * pad, 2.0f);
+    std::copy(hax.begin(), hax.end(), hax_pad.begin() + pad);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->kf.ptr, hk_pad.data(), pl->kf.bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pl->bf.ptr, hb_pad.data(), pl->bf.bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = upload(pl->ax, hax_pad.data(), hax_pad.size(), s);
     if (e == hipSuccess) e = upload(pl->az, haz.data(), haz.size(), s);
     if (e == hipSuccess && !hpx.empty()) e = upload(pl->pml_x, hpx.data(), hpx.size(), s);
     if (e == hipSuccess && !hpz.empty()) e = upload(pl->pml_z, hpz.data(), hpz.size(), s);
     if (e == hipSuccess) e = upload(pl->amp, hamp.data(), hamp.size(), s);
-    if (e == hipSuccess) e = upload(pl->rec_first, hfirst.data(), hfirst.size(), s);
-    if (e == hipSuccess && !hspan.empty()) e = upload(pl->rec_span, hspan.data(), hspan.size(), s);
+    if (e == hipSuccess) e = upload(pl->rec_row, hrow.data(), hrow.size(), s);
+    if (e == hipSuccess && !hmore.empty()) e = upload(pl->rec_more, hmore.data(), hmore.size(), s);
     if (e == hipSuccess && p->nrec > 0) e = upload(pl->rec_slot, hslot.data(), hslot.size(), s);
     if (e == hipSuccess && p->nrec > 0) e = pl->rec_buf.alloc((size_t)pl->nslot * std::max(p->nt, 1) * 4);
     if (e == hipSuccess && p->nrec > 0) e = pl->record.alloc((size_t)p->nrec * std::max(p->nt, 1) * 4);
@@ -442,6 +514,7 @@ int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan* pl, int waves, int chunk_row
     if (nw == 0) nw = anw;
     if (ch == 0) ch = ach;
     if (ch < 1) return fail(pl->ctx, SFD_EINVAL, "chunk_rows must be >= 1");
+    ch = round_chunk(ch, pl->radius);
     StepKernel k = pick_step_kernel(pl->radius, nw, pl->mode);
     if (!k) return fail(pl->ctx, SFD_EINVAL, "waves must be 1, 2 or 4 (got %d)", waves);
     pl->nw = nw;
@@ -500,16 +573,16 @@ int sfd_acoustic_plan_set_state(sfd_acoustic_plan* pl, const float* state, const
     SFD_HIP(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)pl->nx * pl->nz;
     hipStream_t s = ctx->stream;
-    SFD_HIP(ctx, hipMemcpyAsync(pl->p[pl->cur].ptr, state, n * 4, hipMemcpyHostToDevice, s));
-    SFD_HIP(ctx, hipMemcpyAsync(pl->vx[pl->cur].ptr, state + n, n * 4, hipMemcpyHostToDevice, s));
-    SFD_HIP(ctx, hipMemcpyAsync(pl->vz[pl->cur].ptr, state + 2 * n, n * 4, hipMemcpyHostToDevice, s));
+    SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->p[pl->cur]), state, n * 4, hipMemcpyHostToDevice, s));
+    SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vx[pl->cur]), state + n, n * 4, hipMemcpyHostToDevice, s));
+    SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vz[pl->cur]), state + 2 * n, n * 4, hipMemcpyHostToDevice, s));
     if (pl->mode == MODE_FAST_CPML) {
         if (!pml) return fail(ctx, SFD_EINVAL, "C-PML plans need the pml state");
         // Full-grid psi fields -> strips.
         const int nx = pl->nx, nz = pl->nz, xw = pl->pml_xw, zw = pl->pml_zw;
         std::vector<float> xs((size_t)2 * xw * nz), zs((size_t)2 * zw * nx);
         const float* f[4] = {pml, pml + n, pml + 2 * n, pml + 3 * n};  // px, pz, vx, vz
-        DeviceBuffer* xb[2] = {&pl->psi_px[pl->cur], &pl->psi_vx[pl->cur]};
+        Field* xb[2] = {&pl->psi_px[pl->cur], &pl->psi_vx[pl->cur]};
         const float* xf[2] = {f[0], f[2]};
         for (int q = 0; q < 2; ++q) {
             for (int r = 0; r < 2 * xw; ++r) {
@@ -519,7 +592,7 @@ int sfd_acoustic_plan_set_state(sfd_acoustic_plan* pl, const float* state, const
             SFD_HIP(ctx, hipMemcpyAsync(xb[q]->ptr, xs.data(), xs.size() * 4, hipMemcpyHostToDevice, s));
             SFD_HIP(ctx, hipStreamSynchronize(s));
         }
-        DeviceBuffer* zb[2] = {&pl->psi_pz[pl->cur], &pl->psi_vz[pl->cur]};
+        Field* zb[2] = {&pl->psi_pz[pl->cur], &pl->psi_vz[pl->cur]};
         const float* zf[2] = {f[1], f[3]};
         for (int q = 0; q < 2; ++q) {
             for (int i = 0; i < nx; ++i)
@@ -541,15 +614,15 @@ int sfd_acoustic_plan_get_state(sfd_acoustic_plan* pl, float* state, float* pml)
     SFD_HIP(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)pl->nx * pl->nz;
     hipStream_t s = ctx->stream;
-    SFD_HIP(ctx, hipMemcpyAsync(state, pl->p[pl->cur].ptr, n * 4, hipMemcpyDeviceToHost, s));
-    SFD_HIP(ctx, hipMemcpyAsync(state + n, pl->vx[pl->cur].ptr, n * 4, hipMemcpyDeviceToHost, s));
-    SFD_HIP(ctx, hipMemcpyAsync(state + 2 * n, pl->vz[pl->cur].ptr, n * 4, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipMemcpyAsync(state, pl->grid(pl->p[pl->cur]), n * 4, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipMemcpyAsync(state + n, pl->grid(pl->vx[pl->cur]), n * 4, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipMemcpyAsync(state + 2 * n, pl->grid(pl->vz[pl->cur]), n * 4, hipMemcpyDeviceToHost, s));
     SFD_HIP(ctx, hipStreamSynchronize(s));
     if (pml && pl->mode == MODE_FAST_CPML) {
         const int nx = pl->nx, nz = pl->nz, xw = pl->pml_xw, zw = pl->pml_zw;
         std::fill(pml, pml + 4 * n, 0.f);
         std::vector<float> xs((size_t)2 * xw * nz), zs((size_t)2 * zw * nx);
-        DeviceBuffer* xb[2] = {&pl->psi_px[pl->cur], &pl->psi_vx[pl->cur]};
+        Field* xb[2] = {&pl->psi_px[pl->cur], &pl->psi_vx[pl->cur]};
         float* xf[2] = {pml, pml + 2 * n};
         for (int q = 0; q < 2; ++q) {
             SFD_HIP(ctx, hipMemcpyAsync(xs.data(), xb[q]->ptr, xs.size() * 4, hipMemcpyDeviceToHost, s));
@@ -559,7 +632,7 @@ int sfd_acoustic_plan_get_state(sfd_acoustic_plan* pl, float* state, float* pml)
                 std::memcpy(xf[q] + (size_t)i * nz, &xs[(size_t)r * nz], nz * 4);
             }
         }
-        DeviceBuffer* zb[2] = {&pl->psi_pz[pl->cur], &pl->psi_vz[pl->cur]};
+        Field* zb[2] = {&pl->psi_pz[pl->cur], &pl->psi_vz[pl->cur]};
         float* zf[2] = {pml + n, pml + 3 * n};
         for (int q = 0; q < 2; ++q) {
             SFD_HIP(ctx, hipMemcpyAsync(zs.data(), zb[q]->ptr, zs.size() * 4, hipMemcpyDeviceToHost, s));

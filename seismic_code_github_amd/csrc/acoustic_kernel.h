@@ -40,6 +40,11 @@ enum KernelMode : int {
 };
 
 struct AcousticStepArgs {
+    // Every field pointer below (state, model, C-PML strips) lies in one
+    // device allocation starting at `arena` (< 2 GiB), which the kernel
+    // addresses through a single buffer descriptor.
+    const float* arena;
+    uint32_t arena_bytes;
     int nx, nz;
     int chunk;     // output rows per workgroup (x direction)
     int k;         // step index: wavelet sample and gather column
@@ -55,6 +60,8 @@ struct AcousticStepArgs {
     const float* __restrict__ bf;
     // Sponge: ax[i] is the x-edge factor of row i, or 2.0 where row i is not
     // in an x-edge strip; az[j] is the z-edge factor of column j (1.0 inside).
+    // Every per-row array (state, model, ax, rec_row) points at row 0 of an
+    // allocation with kPadRows rows of padding before and after.
     const float* __restrict__ ax;
     const float* __restrict__ az;
     // C-PML: profiles [4][n] = b at nodes, a at nodes, b at half nodes, a at
@@ -63,6 +70,7 @@ struct AcousticStepArgs {
     const float* __restrict__ pml_x;
     const float* __restrict__ pml_z;
     int pml_xw, pml_zw;
+    int sponge_w;  // sponge width (x-edge rows read ax[]); 0 without sponge
     const float* __restrict__ psi_px_in;
     float* __restrict__ psi_px_out;
     const float* __restrict__ psi_vx_in;
@@ -74,11 +82,12 @@ struct AcousticStepArgs {
     // Source: P[sx][sz] -= amp[k] before the update.
     int sx, sz;
     const float* __restrict__ amp;
-    // Receivers: row i owns spans rec_first[i] .. rec_first[i+1]-1 of
-    // rec_span = (jlo, jhi, slot) ; the absorbed new P of cell (i, j),
-    // jlo <= j < jhi, goes to rec_buf[k * nslot + slot + j - jlo].
-    const int* __restrict__ rec_first;
-    const int4* __restrict__ rec_span;
+    // Receivers: rec_row[i] = (jlo, jhi, slot, more) is row i's first span;
+    // the absorbed new P of cell (i, j), jlo <= j < jhi, goes to
+    // rec_buf[k * nslot + slot + j - jlo].  more >= 0 indexes further spans
+    // of the row in rec_more, chained by their 4th field (-1 ends).
+    const int4* __restrict__ rec_row;
+    const int4* __restrict__ rec_more;
     float* __restrict__ rec_buf;
     int nslot;
     // FAST: cx = C/dx, cz = C/dz.  REF: cx = C (unscaled), cz unused.
@@ -159,87 +168,151 @@ __device__ __forceinline__ T ro(const T* p, int i) {
     return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
+// A wave-uniform float moved to a scalar register (bit pattern preserved).
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
 __device__ __forceinline__ int floor_div(int a, int b) {
     return (a >= 0) ? a / b : -((-a + b - 1) / b);
 }
 
-// Values one lane loads from global memory for marched row i.  The loads of
-// row i + NS - 1 are issued while row i is computed (software pipelining), so
-// NS - 1 rows of memory traffic are in flight per lane.
-struct RowLoads {
-    float pf;           // P(f, j), f = i + 2R - 1 (front row)
-    float ph;           // P(f, halo column) on the P-halo lanes
-    float vx, bx;       // Vx(g, j), bf(g, j), g = i + R - 1
-    float vz, bz, kf;   // Vz(i, j), bf(i, j), kf(i, j)
-    float vzh, bzh;     // Vz(i, jh), bf(i, jh) on the Vz-halo lanes
-    float psx;          // C-PML psi_px(g, j)
-    float psvx;         // C-PML psi_vx(i, j)
-    float pspz, psvz;   // C-PML psi_pz(i, j), psi_vz(i, j)
-    float pspzh;        // C-PML psi_pz(i, jh) on the Vz-halo lanes
+// Raw buffer view of a device array: loads and stores take a per-lane byte
+// offset (VGPR) and a per-row byte offset (SGPR), so one marched row costs a
+// scalar add per field instead of 64-bit address arithmetic per lane.  Lane
+// offsets at or beyond the array size (kOOB) read 0 and drop stores, which
+// masks lanes without branches.
+constexpr int kOOB = (int)0x80000000u;
+
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Buf(const void* p, uint32_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000)) {}
+    __device__ __forceinline__ float ld(int voff, uint32_t soff) const {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, (int)soff, 0));
+    }
+    __device__ __forceinline__ void st(float v, int voff, uint32_t soff) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, (int)soff, 0);
+    }
 };
 
+// Values one lane loads for marched iteration i (output row i).  The loads
+// of iteration i + NS - 1 are issued while iteration i computes (software
+// pipelining): every iteration issues the same loads, so NS - 1 rows of
+// memory traffic stay in flight per lane.
+struct RowLoads {
+    float pf;           // P(i + 2R, j): the front row entering the queue
+    float ph;           // P(i + 2R, halo column) on the P-halo lanes
+    float vx, bx;       // Vx(g, j), bf(g, j), g = i + R
+    float vz, bz;       // Vz(i + 1, j), bf(i + 1, j)
+    float vzh, bzh;     // Vz(i + 1, jh), bf(i + 1, jh) on the Vz-halo lanes
+    float kf;           // kf(i, j)
+    int rjlo, rjhi, rslot, rmore;  // receiver span of row i (wave-uniform)
+    float psx;          // C-PML psi_px(g, j)
+    float psvx, psvz;   // C-PML psi_vx(i, j), psi_vz(i, j)
+    float pspz, pspzh;  // C-PML psi_pz(i + 1, j), psi_pz(i + 1, jh)
+};
+
+// Zero rows above and below every per-row allocation: the march's prefetch
+// reads up to 4R + 2NS rows before a chunk and 2R + 2NS rows after it
+// without clamping.
+template <int R>
+constexpr int kPadRows = 6 * R + 16;
+
+// One launch = one time step.  Iteration i of a workgroup's march does three
+// independent pieces of work, so their latencies overlap:
+//   Vx(i + R)   from the P queue (rows i+1 .. i+2R)           -> qvx
+//   Vz(i + 1)   from the LDS ring row i+1                     -> LDS Vz row
+//   P(i)        from qvx rows i-R .. i+R-1 and LDS Vz row i
+// then one barrier.  Vz rows rotate through VS LDS buffers (VS >= 3), so the
+// write of row i+1 never meets a wave still reading row i-1.
 template <int R, int NW, int MODE, int NS>
 __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticStepArgs a) {
     constexpr int TZ = 64 * NW;         // z cells per workgroup
     constexpr int Q = 2 * R;            // register queue length
     constexpr int HR = 2 * R - 1;       // P halo on each side of the tile
-    constexpr int RW = TZ + 2 * HR;     // LDS ring row width
-    constexpr int VW = TZ + 2 * R - 1;  // LDS Vz row width: R left, R-1 right
+    constexpr int RW = TZ + 2 * HR;     // LDS ring row width (+1 scratch slot)
+    constexpr int VW = TZ + 2 * R - 1;  // LDS Vz row width: R left, R-1 right (+1 scratch)
+    constexpr int VS = (Q % 4 == 0) ? 4 : 5;  // Vz row buffers; VS divides Q
     constexpr int PD = NS - 1;          // prefetch distance in rows
+    constexpr int PADR = kPadRows<R>;
     constexpr bool CPML = (MODE == MODE_FAST_CPML);
-    static_assert(Q % NS == 0, "prefetch slots must divide the queue length");
+    constexpr bool REF = (MODE == MODE_REF_SPONGE);
+    static_assert(Q % NS == 0 && Q % VS == 0, "slot counts must divide the queue length");
     static_assert(TZ >= 2 * HR, "tile narrower than its halos");
     using A = ArithFor<R, MODE>;
 
-    __shared__ float s_ring[Q][RW];
-    __shared__ float s_vz[2][VW];
+    __shared__ float s_ring[Q][RW + 1];
+    __shared__ float s_vz[VS][VW + 1];
 
     const int nx = a.nx, nz = a.nz;
     const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int j0 = R + (int)blockIdx.x * TZ;
     const int j = j0 + tid;
-    const bool jload = j < nz;       // lane maps to a stored column
-    const bool jupd = j < nz - R;    // lane's column is updated
+    const bool jupd = j < nz - R;  // lane's column is updated
     const int i0 = R + (int)blockIdx.y * a.chunk;
     const int i1 = min(i0 + a.chunk, nx - R);
     if (i0 >= i1) return;
 
-    // P-halo lanes: the first HR lanes load the columns left of the tile, the
-    // last HR lanes those right of it (into ring positions ph_pos).
+    // Every per-row field starts kPadRows rows before row 0; all fields share
+    // one buffer descriptor and differ by their byte offset in the arena.
+    const uint32_t rowb = (uint32_t)nz * 4u;
+    const Buf AR(a.arena, a.arena_bytes);
+    auto fo = [&](const float* p) -> uint32_t {
+        return (uint32_t)((const char*)p - (const char*)a.arena);
+    };
+    const uint32_t padb = (uint32_t)PADR * rowb;
+    const uint32_t o_pin = fo(a.p_in) - padb, o_vxin = fo(a.vx_in) - padb, o_vzin = fo(a.vz_in) - padb;
+    const uint32_t o_pout = fo(a.p_out) - padb, o_vxout = fo(a.vx_out) - padb, o_vzout = fo(a.vz_out) - padb;
+    const uint32_t o_bf = fo(a.bf) - padb, o_kf = fo(a.kf) - padb;
+    const Buf RR(reinterpret_cast<const int*>(a.rec_row) - 4 * PADR, (uint32_t)(nx + 2 * PADR) * 16u);
+    auto so = [&](int r) -> uint32_t { return (uint32_t)(r + PADR) * rowb; };
+
+    // Lane byte offsets.  P reads every stored column; the other fields only
+    // updated columns (elsewhere they read 0 and stores are dropped).
+    const int vo_p = (j < nz) ? 4 * j : kOOB;
+    const int vo_u = jupd ? 4 * j : kOOB;
+    // P-halo lanes: the first HR lanes read the columns left of the tile, the
+    // last HR lanes those right of it.  Other lanes use the scratch slot.
     const bool has_ph = (tid < HR) || (tid >= TZ - HR);
     const int ph_col = (tid < HR) ? (j0 - HR + tid) : (j + HR);
-    const int ph_pos = (tid < HR) ? tid : (tid + 2 * HR);
-    const bool ph_ok = has_ph && ph_col >= 0 && ph_col < nz;
+    const int ph_pos = !has_ph ? RW : (tid < HR) ? tid : (tid + 2 * HR);
+    const int vo_ph = (has_ph && ph_col >= 0 && ph_col < nz) ? 4 * ph_col : kOOB;
+    const bool wave_halo = (wave == 0) || (wave == NW - 1);
     // Vz-halo lanes: R columns left of the tile, R-1 right of it.
     const bool has_vh = (tid < R) || (tid >= TZ - (R - 1));
     const int jh = (tid < R) ? (j0 - R + tid) : (j + R - 1);
-    const int vh_pos = (tid < R) ? tid : (tid + 2 * R - 1);
-    const int vh_rpos = (tid < R) ? (tid + R - 1) : (tid + 3 * R - 2);
     const bool vh_upd = has_vh && jh >= R && jh < nz - R;
+    const int vh_pos = !has_vh ? VW : (tid < R) ? tid : (tid + 2 * R - 1);
+    const int vh_rpos = !has_vh ? (tid + HR) : (tid < R) ? (tid + R - 1) : (tid + 3 * R - 2);
+    const int vo_vh = vh_upd ? 4 * jh : kOOB;
 
-    const float az_j = jload ? a.az[j] : 1.0f;
+    const float az_j = (j < nz) ? a.az[j] : 1.0f;
     const float amp = ro(a.amp, a.k);
     float cx[R], cz[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
         cx[m] = a.cx[m];
-        cz[m] = (MODE == MODE_REF_SPONGE) ? a.cx[m] : a.cz[m];
+        cz[m] = REF ? a.cx[m] : a.cz[m];
     }
-    // C-PML strip columns (-1 outside the strips) and z profiles.
-    auto zstrip = [&](int c) -> int {
-        if (!CPML) return -1;
-        if (c < a.pml_zw) return c;
-        if (c >= nz - a.pml_zw) return c - (nz - a.pml_zw) + a.pml_zw;
-        return -1;
-    };
+
+    // C-PML strips: psi_x* are [2*xw][nz] (x-strip rows), psi_z* [nx][2*zw].
+    const int xw = a.pml_xw, zw = a.pml_zw, zw2 = 2 * zw;
     auto xstrip = [&](int r) -> int {
-        if (!CPML) return -1;
-        if (r < a.pml_xw) return r;
-        if (r >= nx - a.pml_xw) return r - (nx - a.pml_xw) + a.pml_xw;
+        if (r < xw) return r;
+        if (r >= nx - xw) return r - (nx - xw) + xw;
         return -1;
     };
-    const int zcol = jload ? zstrip(j) : -1;
-    const int zcolh = vh_upd ? zstrip(jh) : -1;
+    auto zstrip = [&](int c) -> int {
+        if (c < zw) return c;
+        if (c >= nz - zw) return c - (nz - zw) + zw;
+        return -1;
+    };
+    const int zcol = (CPML && jupd) ? zstrip(j) : -1;
+    const int zcolh = (CPML && vh_upd) ? zstrip(jh) : -1;
+    const int vo_zc = zcol >= 0 ? 4 * zcol : kOOB;
+    const int vo_zch = zcolh >= 0 ? 4 * zcolh : kOOB;
     float pz_bi = 1.f, pz_ai = 0.f, pz_bh = 1.f, pz_ah = 0.f, ph_bh = 1.f, ph_ah = 0.f;
     if (CPML) {
         if (zcol >= 0) {
@@ -253,53 +326,68 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
             ph_ah = a.pml_z[3 * nz + jh];
         }
     }
-    const int zw2 = 2 * a.pml_zw;
+    uint32_t o_pxin = 0, o_pxout = 0, o_pvxin = 0, o_pvxout = 0, o_pzin = 0, o_pzout = 0, o_pvzin = 0,
+             o_pvzout = 0;
+    if (CPML) {
+        o_pxin = fo(a.psi_px_in);
+        o_pxout = fo(a.psi_px_out);
+        o_pvxin = fo(a.psi_vx_in);
+        o_pvxout = fo(a.psi_vx_out);
+        o_pzin = fo(a.psi_pz_in);
+        o_pzout = fo(a.psi_pz_out);
+        o_pvzin = fo(a.psi_vz_in);
+        o_pvzout = fo(a.psi_vz_out);
+    }
     // Drain the lane-constant loads now: left pending into the march loop,
     // their first use in every row would wait for the whole prefetch queue.
     __builtin_amdgcn_s_waitcnt(0);
 
-    const int i_first = i0 - 4 * R + 2;  // first marched row
-    const int g_lo = max(R, i0 - R), g_hi = min(nx - R, i1 + R - 1);
-
-    // Sponge factor of cell (r, j): the x-edge strips override the z-edge
-    // strips in the corners, as the reference's table does (:164-194).
-    // ax_r: the row's x-edge factor (2.0 when the row is interior).
-    auto absorb_with = [&](float ax_r) -> float {
+    // Sponge factor of row r at this lane: the x-edge strips (scalar table
+    // read, edge rows only) override the z-edge strips in the corners, as the
+    // reference's table does (:164-194).
+    const int sw = a.sponge_w;
+    auto absorb_row = [&](int r) -> float {
         if constexpr (CPML) {
             return 1.0f;
         } else {
-            return (ax_r <= 1.0f) ? ax_r : az_j;
+            if (r < sw || r >= nx - sw) return ro(a.ax, r);
+            return az_j;
         }
     };
 
-    // Every lane issues the same loads for every row (clamped, in-bounds
-    // addresses; values are masked where they are used), so the count of
-    // memory operations in flight is static and the waits stay partial.
-    // Clamped rows and columns fall on lines the workgroup reads anyway.
-    const int jc = min(j, nz - 1);
-    const int phc = ph_ok ? ph_col : jc;
-    const int jhc = vh_upd ? jh : jc;
-    const int zc0 = max(zcol, 0), zch0 = max(zcolh, 0);
+    const int i_first = i0 - 4 * R + 1;  // first marched iteration
+    // Prefetches run past both ends of the chunk into the zero padding.
     auto load_row = [&](int i, RowLoads& L) {
-        const int f = min(max(i + Q - 1, 0), nx - 1);
-        L.pf = a.p_in[(size_t)f * nz + jc];
-        L.ph = a.p_in[(size_t)f * nz + phc];
-        const int g = min(max(i + R - 1, g_lo), g_hi - 1);
-        L.vx = a.vx_in[(size_t)g * nz + jc];
-        L.bx = a.bf[(size_t)g * nz + jc];
-        const int ic = min(max(i, i0), i1 - 1);
-        const size_t row = (size_t)ic * nz;
-        L.vz = a.vz_in[row + jc];
-        L.bz = a.bf[row + jc];
-        L.kf = a.kf[row + jc];
-        L.vzh = a.vz_in[row + jhc];
-        L.bzh = a.bf[row + jhc];
+        const uint32_t oi = so(i);
+        const uint32_t of = oi + (uint32_t)Q * rowb;
+        const uint32_t og = oi + (uint32_t)R * rowb;
+        const uint32_t o1 = oi + rowb;
+        L.pf = AR.ld(vo_p, o_pin + of);
+        L.ph = AR.ld(vo_ph, o_pin + of);
+        L.vx = AR.ld(vo_u, o_vxin + og);
+        L.bx = AR.ld(vo_u, o_bf + og);
+        L.vz = AR.ld(vo_u, o_vzin + o1);
+        L.bz = AR.ld(vo_u, o_bf + o1);
+        L.vzh = AR.ld(vo_vh, o_vzin + o1);
+        L.bzh = AR.ld(vo_vh, o_bf + o1);
+        L.kf = AR.ld(vo_u, o_kf + oi);
+        {
+            const auto rv = __builtin_amdgcn_raw_buffer_load_b128(RR.r, 0, (int)((uint32_t)(i + PADR) * 16u), 0);
+            L.rjlo = (int)rv[0];
+            L.rjhi = (int)rv[1];
+            L.rslot = (int)rv[2];
+            L.rmore = (int)rv[3];
+        }
         if constexpr (CPML) {
-            L.psx = a.psi_px_in[(size_t)max(xstrip(g), 0) * nz + jc];
-            L.psvx = a.psi_vx_in[(size_t)max(xstrip(ic), 0) * nz + jc];
-            L.pspz = a.psi_pz_in[(size_t)ic * zw2 + zc0];
-            L.psvz = a.psi_vz_in[(size_t)ic * zw2 + zc0];
-            L.pspzh = a.psi_pz_in[(size_t)ic * zw2 + zch0];
+            const int g = min(max(i + R, 0), nx - 1);
+            const int ic = min(max(i, 0), nx - 1);
+            const int i1c = min(max(i + 1, 0), nx - 1);
+            const int xg = xstrip(g), xi = xstrip(ic);
+            L.psx = AR.ld(xg >= 0 ? vo_u : kOOB, o_pxin + (uint32_t)max(xg, 0) * rowb);
+            L.psvx = AR.ld(xi >= 0 ? vo_u : kOOB, o_pvxin + (uint32_t)max(xi, 0) * rowb);
+            L.psvz = AR.ld(vo_zc, o_pvzin + (uint32_t)ic * zw2 * 4u);
+            L.pspz = AR.ld(vo_zc, o_pzin + (uint32_t)i1c * zw2 * 4u);
+            L.pspzh = AR.ld(vo_zch, o_pzin + (uint32_t)i1c * zw2 * 4u);
         }
     };
 
@@ -314,132 +402,133 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
         static_for<Q>([&](auto U_) {
             constexpr int U = decltype(U_)::value;
             const int i = base + U;
-            // Unconditional, so every marched row issues the same loads.
+            // Unconditional, so every iteration issues the same loads.
             load_row(i + PD, ld[(U + PD) % NS]);
             if (i < i_first || i >= i1) return;
             const RowLoads& L = ld[U % NS];
 
-            // ---- 1. front row of P: queue (and LDS ring for output rows) ----
-            const int f = i + Q - 1;
-            constexpr int sf = (U + Q - 1) % Q;
-            {
-                float pf = (f >= 0 && f < nx && jload) ? L.pf : 0.f;
-                if (f == a.sx && j == a.sz) pf -= amp;
-                qp[sf] = pf;
-                if (f >= i0 && f < i1) {
-                    s_ring[sf][tid + HR] = pf;
-                    if (has_ph) {
-                        float v = ph_ok ? L.ph : 0.f;
-                        if (f == a.sx && ph_col == a.sz) v -= amp;
-                        s_ring[sf][ph_pos] = v;
-                    }
-                }
+            // ---- P(i): x part from the Vx queue (before Vx(i+R) replaces
+            //      row i-R in it) and the base value P(i) (before the front
+            //      row replaces it in the P queue) ----
+            const bool do_p = i >= i0;
+            const float p_i = qp[U % Q];
+            float sxp = 0.f;
+            if (do_p) {
+                sxp = A::stencil(cx, [&](int m) {
+                    return qvx[(U + m - 1) % Q] - qvx[(U - m + Q) % Q];
+                });
             }
 
-            // ---- 2. new Vx at row g = i + R - 1 from the P queue ----
-            const int g = i + R - 1;
-            constexpr int sg = (U + R - 1) % Q;
-            // Row-uniform tables, read in uniform control flow (scalar loads).
-            const int gt = min(max(g, 0), nx - 1);
-            const float ax_g = CPML ? 1.f : ro(a.ax, gt);
-            const float pbh_g = CPML ? ro(a.pml_x, 2 * nx + gt) : 0.f;
-            const float pah_g = CPML ? ro(a.pml_x, 3 * nx + gt) : 0.f;
-            if (g >= i0 - R && g < i1 + R - 1) {
-                float vxn = 0.f;
-                if (g >= g_lo && g < g_hi && jupd) {
-                    const float s = A::stencil(cx, [&](int m) {
-                        return qp[(U + R - 1 + m) % Q] - qp[(U + R - m) % Q];
+            // ---- front row i + 2R into the P queue and the LDS ring ----
+            const int f = i + Q;
+            {
+                float pf = L.pf, ph = L.ph;
+                if (f == a.sx) {
+                    if (j == a.sz) pf -= amp;
+                    if (ph_col == a.sz && vo_ph != kOOB) ph -= amp;
+                }
+                qp[U % Q] = pf;
+                // Rows before i0 are never read from the ring; writing them
+                // is harmless (their slot is next used by row f + 2R).
+                s_ring[U % Q][tid + HR] = pf;
+                if (wave_halo && has_ph) s_ring[U % Q][ph_pos] = ph;
+            }
+
+            // ---- Vx(g), g = i + R, from P rows i+1 .. i+2R ----
+            const int g = i + R;
+            if (g >= i0 - R) {
+                const float s = A::stencil(cx, [&](int m) {
+                    return qp[(U + R + m) % Q] - qp[(U + R - m + 1) % Q];
+                });
+                float d = s;
+                const bool own = g >= i0 && g < i1;  // rows below i0 belong to the previous chunk
+                if constexpr (CPML) {
+                    const int xr = xstrip(g);
+                    if (xr >= 0) {
+                        const float psi = __builtin_fmaf(ro(a.pml_x, 3 * nx + g), s, ro(a.pml_x, 2 * nx + g) * L.psx);
+                        d = s + psi;
+                        if (own) AR.st(psi, vo_u, o_pxout + (uint32_t)xr * rowb);
+                    }
+                }
+                float vxn = A::vel(L.vx, L.bx, d, a.dtdx);
+                if constexpr (REF) vxn = (g >= R && g < nx - R && jupd) ? vxn : 0.f;
+                qvx[(U + R) % Q] = vxn;
+                if (own) AR.st(vxn * absorb_row(g), vo_u, o_vxout + so(g));
+            }
+
+            if (i < i0 - 1) return;
+            // Ring rows written while the queues were filling become visible.
+            if (i == i0 - 1) __syncthreads();
+
+            // ---- Vz(i+1) (own lane and halo lanes) into LDS buffer (i+1) % VS ----
+            const int i1r = i + 1;
+            if (i1r < i1) {
+                constexpr int sr = (U + 1) % Q;
+                constexpr int vw = (U + 1) % VS;
+                auto vz_new = [&](int rpos, float vold, float b, int zc, int vo, float bh, float ah,
+                                  float psi_old, bool own) -> float {
+                    const float s = A::stencil(cz, [&](int m) {
+                        return s_ring[sr][rpos + m] - s_ring[sr][rpos - m + 1];
                     });
                     float d = s;
                     if constexpr (CPML) {
-                        const int xr = xstrip(g);
-                        if (xr >= 0) {
-                            const float psi = __builtin_fmaf(pah_g, s, pbh_g * L.psx);
+                        if (zc >= 0) {
+                            const float psi = __builtin_fmaf(ah, s, bh * psi_old);
                             d = s + psi;
-                            if (g >= i0 && g < i1) a.psi_px_out[(size_t)xr * nz + j] = psi;
+                            if (own) AR.st(psi, vo, o_pzout + (uint32_t)i1r * zw2 * 4u);
                         }
                     }
-                    vxn = A::vel(L.vx, L.bx, d, a.dtdx);
-                    if (g >= i0 && g < i1) a.vx_out[(size_t)g * nz + j] = vxn * absorb_with(ax_g);
-                }
-                qvx[sg] = vxn;
-            }
-
-            if (i < i0) return;
-            // Ring rows written while the queues were filling become visible.
-            if (i == i0) __syncthreads();
-            const float ax_i = CPML ? 1.f : ro(a.ax, i);
-            const float pbi_i = CPML ? ro(a.pml_x, i) : 0.f;
-            const float pai_i = CPML ? ro(a.pml_x, nx + i) : 0.f;
-            const int rec0 = ro(a.rec_first, i), rec1 = ro(a.rec_first, i + 1);
-
-            // ---- 3. new Vz at row i (own lane and halo lanes) into LDS ----
-            constexpr int si = U % Q;
-            const int vb = i & 1;
-            const size_t rowoff = (size_t)i * nz;
-            auto vz_new = [&](int rpos, float vold, float b, int zc, float bh, float ah, float psi_old,
-                              bool own) -> float {
-                const float s = A::stencil(cz, [&](int m) {
-                    return s_ring[si][rpos + m] - s_ring[si][rpos - m + 1];
-                });
-                float d = s;
-                if constexpr (CPML) {
-                    if (zc >= 0) {
-                        const float psi = __builtin_fmaf(ah, s, bh * psi_old);
-                        d = s + psi;
-                        if (own) a.psi_pz_out[(size_t)i * zw2 + zc] = psi;
-                    }
-                }
-                return A::vel(vold, b, d, a.dtdz);
-            };
-            {
-                float vzn = 0.f;
-                if (jupd) {
-                    vzn = vz_new(tid + HR, L.vz, L.bz, zcol, pz_bh, pz_ah, L.pspz, true);
-                    a.vz_out[rowoff + j] = vzn * absorb_with(ax_i);
-                }
-                s_vz[vb][tid + R] = vzn;
-                if (has_vh) {
-                    float vh = 0.f;
-                    if (vh_upd) vh = vz_new(vh_rpos, L.vzh, L.bzh, zcolh, ph_bh, ph_ah, L.pspzh, false);
-                    s_vz[vb][vh_pos] = vh;
+                    return A::vel(vold, b, d, a.dtdz);
+                };
+                float vzn = vz_new(tid + HR, L.vz, L.bz, zcol, vo_zc, pz_bh, pz_ah, L.pspz, true);
+                if constexpr (REF) vzn = jupd ? vzn : 0.f;
+                s_vz[vw][tid + R] = vzn;
+                AR.st(vzn * absorb_row(i1r), vo_u, o_vzout + so(i1r));
+                if (wave_halo) {
+                    float vh = vz_new(vh_rpos, L.vzh, L.bzh, zcolh, vo_zch, ph_bh, ph_ah, L.pspzh, false);
+                    if constexpr (REF) vh = vh_upd ? vh : 0.f;
+                    if (has_vh) s_vz[vw][vh_pos] = vh;
                 }
             }
-            __syncthreads();
 
-            // ---- 4. new P at row i ----
-            float pn = 0.f;
-            if (jupd) {
-                const size_t c = rowoff + j;
-                float sx = A::stencil(cx, [&](int m) {
-                    return qvx[(U + m - 1) % Q] - qvx[(U - m + Q) % Q];
-                });
+            // ---- P(i) ----
+            if (do_p) {
+                constexpr int vr = U % VS;
+                float sx = sxp;
                 float sz = A::stencil(cz, [&](int m) {
-                    return s_vz[vb][tid + R + m - 1] - s_vz[vb][tid + R - m];
+                    return s_vz[vr][tid + R + m - 1] - s_vz[vr][tid + R - m];
                 });
                 if constexpr (CPML) {
                     const int xr = xstrip(i);
                     if (xr >= 0) {
-                        const float psi = __builtin_fmaf(pai_i, sx, pbi_i * L.psvx);
-                        a.psi_vx_out[(size_t)xr * nz + j] = psi;
+                        const float psi = __builtin_fmaf(ro(a.pml_x, nx + i), sx, ro(a.pml_x, i) * L.psvx);
+                        AR.st(psi, vo_u, o_pvxout + (uint32_t)xr * rowb);
                         sx += psi;
                     }
                     if (zcol >= 0) {
                         const float psi = __builtin_fmaf(pz_ai, sz, pz_bi * L.psvz);
-                        a.psi_vz_out[(size_t)i * zw2 + zcol] = psi;
+                        AR.st(psi, vo_zc, o_pvzout + (uint32_t)i * zw2 * 4u);
                         sz += psi;
                     }
                 }
-                pn = A::pres(qp[si], L.kf, sx, sz, a.inv_dx, a.inv_dz) * absorb_with(ax_i);
-                a.p_out[c] = pn;
+                const float pn = A::pres(p_i, L.kf, sx, sz, a.inv_dx, a.inv_dz) * absorb_row(i);
+                AR.st(pn, vo_u, o_pout + so(i));
+                // Receivers of row i: the first span from the prefetch, further
+                // spans (rare) from the chained list.
+                const int jlo = __builtin_amdgcn_readfirstlane(L.rjlo);
+                const int jhi = __builtin_amdgcn_readfirstlane(L.rjhi);
+                const int slot = __builtin_amdgcn_readfirstlane(L.rslot);
+                int more = __builtin_amdgcn_readfirstlane(L.rmore);
+                float* gk = a.rec_buf + (size_t)a.k * a.nslot;
+                if (jupd && j >= jlo && j < jhi) gk[slot + (j - jlo)] = pn;
+                while (more >= 0) {
+                    const int* sp = reinterpret_cast<const int*>(a.rec_more) + 4 * more;
+                    const int mlo = ro(sp, 0), mhi = ro(sp, 1), mslot = ro(sp, 2);
+                    if (jupd && j >= mlo && j < mhi) gk[mslot + (j - mlo)] = pn;
+                    more = ro(sp, 3);
+                }
             }
-            // Receivers of row i (uniform loop over its spans).
-            for (int r = rec0; r < rec1; ++r) {
-                const int* spn = reinterpret_cast<const int*>(a.rec_span) + 4 * r;
-                const int jlo = ro(spn, 0), jhi = ro(spn, 1), slot = ro(spn, 2);
-                if (jupd && j >= jlo && j < jhi)
-                    a.rec_buf[(size_t)a.k * a.nslot + slot + (j - jlo)] = pn;
-            }
+            __syncthreads();
         });
     }
 }

@@ -56,6 +56,29 @@ struct DeviceBuffer {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Non-owning view of a region inside a larger device allocation.
+struct Field {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Carves consecutive 256-byte aligned fields out of one allocation: first a
+// sizing pass (base == nullptr), then the real one.
+struct Carver {
+    char* base = nullptr;
+    size_t used = 0;
+    Field take(size_t bytes) {
+        used = (used + 255) / 256 * 256;
+        Field f;
+        f.ptr = base ? base + used : nullptr;
+        f.bytes = bytes;
+        used += bytes;
+        return f;
+    }
+};
+
 // Staggered Taylor coefficients used when the caller passes none: the
 // reference's own tables (Acoustic_1order_FD.cpp:38-45 for R = 8;
 // Iso_multishot_fd_module_zy_m2.f90:17-26 for R = 4 and 5).
