@@ -257,18 +257,17 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     a.rec_buf = pl->rec_buf.as<float>();
     a.nslot = pl->nslot;
     const bool ref = pl->mode == MODE_REF_SPONGE;
-    for (int m = 0; m < 8; ++m) {
-        a.cx[m] = ref ? pl->coef[m] : pl->coef[m] / pl->dx;
-        a.cz[m] = ref ? pl->coef[m] : pl->coef[m] / pl->dz;
-    }
+    for (int m = 0; m < 8; ++m) a.cx[m] = ref ? pl->coef[m] : pl->coef[m] / pl->dx;
+    a.zx = pl->dx / pl->dz;
     a.dtdx = pl->dt / pl->dx;
     a.dtdz = pl->dt / pl->dz;
     a.inv_dx = 1.0 / (double)pl->dx;
     a.inv_dz = 1.0 / (double)pl->dz;
 
     const int TZ = 64 * pl->nw;
-    const dim3 grid((pl->nz - 2 * pl->radius + TZ - 1) / TZ,
-                    (pl->nx - 2 * pl->radius + pl->chunk - 1) / pl->chunk);
+    a.ntz = (pl->nz - 2 * pl->radius + TZ - 1) / TZ;
+    a.nchunk = (pl->nx - 2 * pl->radius + pl->chunk - 1) / pl->chunk;
+    const dim3 grid(a.ntz * a.nchunk);
     const dim3 block(TZ);
     const size_t nbytes = (size_t)pl->nx * pl->nz * sizeof(float);
     size_t snap_next = std::lower_bound(pl->snap_steps.begin(), pl->snap_steps.end(), k0) - pl->snap_steps.begin();

@@ -37,6 +37,9 @@ enum KernelMode : int {
     MODE_FAST_SPONGE = 0,
     MODE_FAST_CPML = 1,
     MODE_REF_SPONGE = 2,
+    // Internal: fast arithmetic with no absorbing terms; the C-PML kernel runs
+    // it on tiles that touch no PML strip.
+    MODE_FAST_NONE = 3,
 };
 
 struct AcousticStepArgs {
@@ -47,6 +50,7 @@ struct AcousticStepArgs {
     uint32_t arena_bytes;
     int nx, nz;
     int chunk;     // output rows per workgroup (x direction)
+    int ntz, nchunk;  // z tiles and x chunks; the grid is 1-D with ntz*nchunk workgroups
     int k;         // step index: wavelet sample and gather column
     const float* __restrict__ p_in;
     const float* __restrict__ vx_in;
@@ -90,9 +94,10 @@ struct AcousticStepArgs {
     const int4* __restrict__ rec_more;
     float* __restrict__ rec_buf;
     int nslot;
-    // FAST: cx = C/dx, cz = C/dz.  REF: cx = C (unscaled), cz unused.
+    // FAST: cx = C/dx and zx = dx/dz (z stencils use cx * zx).  REF: cx = C
+    // (unscaled), zx unused.
     float cx[8];
-    float cz[8];
+    float zx;
     // REF only: the float quotients dt/dx, dt/dz and the double 1.0/dx, 1.0/dz.
     float dtdx, dtdz;
     double inv_dx, inv_dz;
@@ -208,8 +213,7 @@ struct RowLoads {
     float vzh, bzh;     // Vz(i + 1, jh), bf(i + 1, jh) on the Vz-halo lanes
     float kf;           // kf(i, j)
     int rjlo, rjhi, rslot, rmore;  // receiver span of row i (wave-uniform)
-    float psx;          // C-PML psi_px(g, j)
-    float psvx, psvz;   // C-PML psi_vx(i, j), psi_vz(i, j)
+    float psvz;         // C-PML psi_vz(i, j)
     float pspz, pspzh;  // C-PML psi_pz(i + 1, j), psi_pz(i + 1, jh)
 };
 
@@ -226,8 +230,17 @@ constexpr int kPadRows = 6 * R + 16;
 //   P(i)        from qvx rows i-R .. i+R-1 and LDS Vz row i
 // then one barrier.  Vz rows rotate through VS LDS buffers (VS >= 3), so the
 // write of row i+1 never meets a wave still reading row i-1.
+template <int R, int NW>
+struct StepLds {
+    static constexpr int TZ = 64 * NW, Q = 2 * R, HR = 2 * R - 1;
+    static constexpr int RW = TZ + 2 * HR, VW = TZ + 2 * R - 1;
+    static constexpr int VS = (Q % 4 == 0) ? 4 : 5;
+    float ring[Q][RW + 1];
+    float vz[VS][VW + 1];
+};
+
 template <int R, int NW, int MODE, int NS>
-__global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticStepArgs a) {
+__device__ __forceinline__ void acoustic_march(const AcousticStepArgs& a, int tile, StepLds<R, NW>& lds) {
     constexpr int TZ = 64 * NW;         // z cells per workgroup
     constexpr int Q = 2 * R;            // register queue length
     constexpr int HR = 2 * R - 1;       // P halo on each side of the tile
@@ -238,20 +251,20 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
     constexpr int PADR = kPadRows<R>;
     constexpr bool CPML = (MODE == MODE_FAST_CPML);
     constexpr bool REF = (MODE == MODE_REF_SPONGE);
+    constexpr bool NOABS = CPML || (MODE == MODE_FAST_NONE);
     static_assert(Q % NS == 0 && Q % VS == 0, "slot counts must divide the queue length");
     static_assert(TZ >= 2 * HR, "tile narrower than its halos");
     using A = ArithFor<R, MODE>;
-
-    __shared__ float s_ring[Q][RW + 1];
-    __shared__ float s_vz[VS][VW + 1];
+    auto& s_ring = lds.ring;
+    auto& s_vz = lds.vz;
 
     const int nx = a.nx, nz = a.nz;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int j0 = R + (int)blockIdx.x * TZ;
+    const int j0 = R + (tile % a.ntz) * TZ;
     const int j = j0 + tid;
     const bool jupd = j < nz - R;  // lane's column is updated
-    const int i0 = R + (int)blockIdx.y * a.chunk;
+    const int i0 = R + (tile / a.ntz) * a.chunk;
     const int i1 = min(i0 + a.chunk, nx - R);
     if (i0 >= i1) return;
 
@@ -290,12 +303,13 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
 
     const float az_j = (j < nz) ? a.az[j] : 1.0f;
     const float amp = ro(a.amp, a.k);
-    float cx[R], cz[R];
+    // One coefficient set in scalar registers: the z stencils use the x
+    // coefficients and scale their sums by zx = dx/dz (fast arithmetic; the
+    // reference arithmetic uses the unscaled C_m for both axes).
+    float cx[R];
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-        cx[m] = a.cx[m];
-        cz[m] = REF ? a.cx[m] : a.cz[m];
-    }
+    for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
+    const float zx = REF ? 1.0f : a.zx;
 
     // C-PML strips: psi_x* are [2*xw][nz] (x-strip rows), psi_z* [nx][2*zw].
     const int xw = a.pml_xw, zw = a.pml_zw, zw2 = 2 * zw;
@@ -347,7 +361,7 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
     // reference's table does (:164-194).
     const int sw = a.sponge_w;
     auto absorb_row = [&](int r) -> float {
-        if constexpr (CPML) {
+        if constexpr (NOABS) {
             return 1.0f;
         } else {
             if (r < sw || r >= nx - sw) return ro(a.ax, r);
@@ -357,11 +371,13 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
 
     const int i_first = i0 - 4 * R + 1;  // first marched iteration
     // Prefetches run past both ends of the chunk into the zero padding.
+    // While the queues fill, fields the iteration does not use yet are read
+    // from the first row the chunk will use (so they cost no extra traffic).
     auto load_row = [&](int i, RowLoads& L) {
-        const uint32_t oi = so(i);
-        const uint32_t of = oi + (uint32_t)Q * rowb;
-        const uint32_t og = oi + (uint32_t)R * rowb;
-        const uint32_t o1 = oi + rowb;
+        const uint32_t oi = so(max(i, i0));
+        const uint32_t of = so(max(i + Q, i_first + Q));
+        const uint32_t og = so(max(i + R, i0 - R));
+        const uint32_t o1 = so(max(i + 1, i0));
         L.pf = AR.ld(vo_p, o_pin + of);
         L.ph = AR.ld(vo_ph, o_pin + of);
         L.vx = AR.ld(vo_u, o_vxin + og);
@@ -372,19 +388,16 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
         L.bzh = AR.ld(vo_vh, o_bf + o1);
         L.kf = AR.ld(vo_u, o_kf + oi);
         {
-            const auto rv = __builtin_amdgcn_raw_buffer_load_b128(RR.r, 0, (int)((uint32_t)(i + PADR) * 16u), 0);
+            const auto rv = __builtin_amdgcn_raw_buffer_load_b128(RR.r, 0, (int)((uint32_t)(max(i, i0) + PADR) * 16u), 0);
             L.rjlo = (int)rv[0];
             L.rjhi = (int)rv[1];
             L.rslot = (int)rv[2];
             L.rmore = (int)rv[3];
         }
         if constexpr (CPML) {
-            const int g = min(max(i + R, 0), nx - 1);
-            const int ic = min(max(i, 0), nx - 1);
-            const int i1c = min(max(i + 1, 0), nx - 1);
-            const int xg = xstrip(g), xi = xstrip(ic);
-            L.psx = AR.ld(xg >= 0 ? vo_u : kOOB, o_pxin + (uint32_t)max(xg, 0) * rowb);
-            L.psvx = AR.ld(xi >= 0 ? vo_u : kOOB, o_pvxin + (uint32_t)max(xi, 0) * rowb);
+            // x-strip memory fields are read where used (strip rows only).
+            const int ic = min(max(i, i0), nx - 1);
+            const int i1c = min(max(i + 1, i0), nx - 1);
             L.psvz = AR.ld(vo_zc, o_pvzin + (uint32_t)ic * zw2 * 4u);
             L.pspz = AR.ld(vo_zc, o_pzin + (uint32_t)i1c * zw2 * 4u);
             L.pspzh = AR.ld(vo_zch, o_pzin + (uint32_t)i1c * zw2 * 4u);
@@ -445,7 +458,8 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
                 if constexpr (CPML) {
                     const int xr = xstrip(g);
                     if (xr >= 0) {
-                        const float psi = __builtin_fmaf(ro(a.pml_x, 3 * nx + g), s, ro(a.pml_x, 2 * nx + g) * L.psx);
+                        const float psx = AR.ld(vo_u, o_pxin + (uint32_t)xr * rowb);
+                        const float psi = __builtin_fmaf(ro(a.pml_x, 3 * nx + g), s, ro(a.pml_x, 2 * nx + g) * psx);
                         d = s + psi;
                         if (own) AR.st(psi, vo_u, o_pxout + (uint32_t)xr * rowb);
                     }
@@ -467,7 +481,7 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
                 constexpr int vw = (U + 1) % VS;
                 auto vz_new = [&](int rpos, float vold, float b, int zc, int vo, float bh, float ah,
                                   float psi_old, bool own) -> float {
-                    const float s = A::stencil(cz, [&](int m) {
+                    const float s = zx * A::stencil(cx, [&](int m) {
                         return s_ring[sr][rpos + m] - s_ring[sr][rpos - m + 1];
                     });
                     float d = s;
@@ -495,13 +509,14 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
             if (do_p) {
                 constexpr int vr = U % VS;
                 float sx = sxp;
-                float sz = A::stencil(cz, [&](int m) {
+                float sz = zx * A::stencil(cx, [&](int m) {
                     return s_vz[vr][tid + R + m - 1] - s_vz[vr][tid + R - m];
                 });
                 if constexpr (CPML) {
                     const int xr = xstrip(i);
                     if (xr >= 0) {
-                        const float psi = __builtin_fmaf(ro(a.pml_x, nx + i), sx, ro(a.pml_x, i) * L.psvx);
+                        const float psvx = AR.ld(vo_u, o_pvxin + (uint32_t)xr * rowb);
+                        const float psi = __builtin_fmaf(ro(a.pml_x, nx + i), sx, ro(a.pml_x, i) * psvx);
                         AR.st(psi, vo_u, o_pvxout + (uint32_t)xr * rowb);
                         sx += psi;
                     }
@@ -530,6 +545,36 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
             }
             __syncthreads();
         });
+    }
+}
+
+template <int R, int NW, int MODE, int NS>
+__global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticStepArgs a) {
+    __shared__ StepLds<R, NW> lds;
+    // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+    // Remap so each XCD gets a contiguous run of tiles in (x chunk, z tile)
+    // order, z fastest: an XCD holds a band of x chunks across all z tiles,
+    // so both halo directions are re-read from its own L2, and the boundary
+    // tiles spread over all XCDs.
+    int tile;
+    {
+        const int n = a.ntz * a.nchunk, L = (int)blockIdx.x;
+        const int xcd = L % 8, idx = L / 8, q = n / 8, rem = n % 8;
+        tile = xcd * q + min(xcd, rem) + idx;
+    }
+    if constexpr (MODE == MODE_FAST_CPML) {
+        // Tiles whose rows and columns (halos included) miss every PML strip
+        // run without the memory-variable terms.
+        constexpr int TZ = 64 * NW;
+        const int j0 = R + (tile % a.ntz) * TZ;
+        const int i0 = R + (tile / a.ntz) * a.chunk;
+        const int i1 = min(i0 + a.chunk, a.nx - R);
+        const bool xs = (i0 - 2 * R < a.pml_xw) || (i1 + 2 * R > a.nx - a.pml_xw);
+        const bool zs = (j0 - 2 * R < a.pml_zw) || (j0 + TZ + 2 * R > a.nz - a.pml_zw);
+        if (xs || zs) acoustic_march<R, NW, MODE_FAST_CPML, NS>(a, tile, lds);
+        else acoustic_march<R, NW, MODE_FAST_NONE, NS>(a, tile, lds);
+    } else {
+        acoustic_march<R, NW, MODE, NS>(a, tile, lds);
     }
 }
 
