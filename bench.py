@@ -64,6 +64,32 @@ def cpu_baseline(shot, steps):
                       f"(reference arithmetic), OpenMP over x rows, {dt:.1f} s"}
 
 
+def batched_leg(nt):
+    """64 shots on uneven aperture windows of a 1001 x 6001 model (C-PML 50,
+    nt=3000 per shot): one launch per step for all shots replayed from a
+    hipGraph, against the plain path (shot after shot, one launch per shot
+    and step).  Timed on the first `nt` steps; shots/s is scaled to the full
+    3000 steps (every step costs the same)."""
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    _, shots = W.windowed_shots(nshot=64, nt=nt)
+    cells = sum(s.vp.size for s in shots)
+    b = sfd.AcousticBatch(shots)
+    b.run(sfd.BATCH_GRAPH)
+    g_ms = min(b.run(sfd.BATCH_GRAPH) for _ in range(2))
+    p_ms = b.run(sfd.BATCH_PER_SHOT)
+    b.close()
+    scale = 3000.0 / nt
+    return {"workload": "64 shots, windows of 1000-2500 columns of a 1001x6001 model, C-PML 50, R=8, nt=3000",
+            "cells": int(cells), "timed_steps": nt,
+            "graph_Mcell_per_s": round(cells * nt / g_ms / 1e3, 1),
+            "plain_Mcell_per_s": round(cells * nt / p_ms / 1e3, 1),
+            "graph_shots_per_s": round(64 / (g_ms * scale / 1e3), 3),
+            "plain_shots_per_s": round(64 / (p_ms * scale / 1e3), 3),
+            "speedup": round(p_ms / g_ms, 2),
+            "graph_frac_of_8TBs": round(cells * nt * 32.0 / (g_ms / 1e3) / 8e12, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,8 +98,10 @@ def main():
     ap.add_argument("--nt", type=int, default=6000)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=600)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: shots are independent and this build measures one "
@@ -138,6 +166,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_step, "avg_launch_us": round(avg_launch_s * 1e6, 2)},
     }
+    if not args.no_batch:
+        log("batched leg ...")
+        out["batched"] = batched_leg(args.batch_steps)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shot, args.cpu_steps)

@@ -129,6 +129,48 @@ int sfd_acoustic_plan_step(sfd_acoustic_plan *plan, int k0, int nsteps);
  * marched per workgroup.  0 = choose automatically. */
 int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan *plan, int waves, int chunk_rows);
 
+/* ---------------------------------------------------------------- batches
+ * Many shots advanced together.  The reference runs its shots one after
+ * another, one per MPI rank (forward/Q_forward/Q_multishot_fd_main_fsbc_zy_rotated.f90:124,
+ * yang_version_2016_2_2/Iso_multishot_fd_main_fsbc_zy_m2.f90:71), each on its
+ * own aperture-windowed grid (get_currshot_parameters, Q sub:288-428); a
+ * batch holds such shots of different sizes and advances all of them with
+ * one kernel launch per time step.  Every shot needs the same nt, radius,
+ * boundary kind and arithmetic; batches record gathers (no snapshots). */
+typedef struct sfd_acoustic_batch sfd_acoustic_batch;
+
+enum sfd_batch_run {
+    SFD_BATCH_GRAPH = 0,     /* one launch per step for all shots, the nt launches
+                                captured once in a hipGraph and replayed */
+    SFD_BATCH_LAUNCHES = 1,  /* one launch per step for all shots, no graph */
+    SFD_BATCH_PER_SHOT = 2   /* the plain path: shot after shot, one launch per
+                                shot and step, no graph */
+};
+
+/* params[s], vp[s], den[s] (den may be NULL, or den[s] NULL), wavelet[s] as
+ * in sfd_acoustic_forward, for shots s = 0 .. nshot-1. */
+int sfd_acoustic_batch_create(sfd_context *ctx, int nshot, const sfd_acoustic_params *params,
+                              const float *const *vp, const float *const *den,
+                              const float *const *wavelet, sfd_acoustic_batch **batch);
+/* Same tiling for every shot (0 = automatic); invalidates a captured graph. */
+int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch *batch, int waves, int chunk_rows);
+/* Zeroes every wavefield and propagates all shots nt steps (how: enum
+ * sfd_batch_run).  elapsed_ms: device time of the propagation. */
+int sfd_acoustic_batch_run(sfd_acoustic_batch *batch, int how, float *elapsed_ms);
+/* Gather of one shot, [nrec][nt] trace-major. */
+int sfd_acoustic_batch_fetch(sfd_acoustic_batch *batch, int shot, float *record);
+int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
+
+/* Aperture window of one shot, the drop-in for get_currshot_parameters
+ * (forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:315-373;
+ * the same routine in yang_version_2016_2_2/Iso_multishot_fd_subroutine_fsbc_zy_m2_pub.f90:216-266).
+ * model is [mnx][mnz] (z fastest).  Window column w (0 <= w < nxe) takes
+ * model column clamp(x0 + w, 0, mnx-1); the window is then padded by edge
+ * replication with pad_xlo / pad_xhi columns and pad_zlo / pad_zhi rows.
+ * out is [nxe+pad_xlo+pad_xhi][mnz+pad_zlo+pad_zhi].  Host only. */
+int sfd_window_model(const float *model, int mnx, int mnz, int x0, int nxe, int pad_xlo,
+                     int pad_xhi, int pad_zlo, int pad_zhi, float *out);
+
 /* Stability bound of the staggered leapfrog: the largest
  * vmax * dt * sqrt(1/dx^2 + 1/dz^2) the stencil allows, 1 / sum |C_m|.
  * Plans whose model exceeds it are rejected with SFD_ECFL. */

@@ -80,6 +80,8 @@ def lib():
         L.oracle_sponge_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
         L.oracle_ricker.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, fp]
         L.oracle_staggered_coefs.argtypes = [ctypes.c_int, dp]
+        L.oracle_window.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
         L.oracle_cpml_coef.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, dp, dp]
@@ -108,6 +110,15 @@ def ricker(nt: int, dt: float, freq: float, amp: float = 50.0) -> np.ndarray:
 def staggered_coefs(radius: int) -> np.ndarray:
     out = np.empty(radius, dtype=np.float64)
     lib().oracle_staggered_coefs(radius, _ptr(out, ctypes.c_double))
+    return out
+
+
+def window(model, ixmine, nxe, npxn, npxp, npzn, npzp):
+    """get_currshot_parameters' window + edge padding (oracle/window.c)."""
+    m = np.ascontiguousarray(model, dtype=np.float32)
+    nx, nz = m.shape
+    out = np.full((nxe + npxn + npxp, nz + npzn + npzp), np.nan, dtype=np.float32)
+    lib().oracle_window(_ptr(m), nx, nz, ixmine, nxe, npxn, npxp, npzn, npzp, _ptr(out))
     return out
 
 

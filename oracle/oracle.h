@@ -82,6 +82,11 @@ void oracle_staggered_coefs(int radius, double *c);
 void oracle_cpml_coef(int i, double shift, int n, int w, double h, double dt, double R,
                       double vmax, double f0, double *b, double *a);
 
+/* Aperture window + edge padding of one shot, get_currshot_parameters
+ * (forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:288-373). */
+void oracle_window(const float *model, int nx, int nz, int ixmine, int nxe, int npxn, int npxp,
+                   int npzn, int npzp, float *out);
+
 #ifdef __cplusplus
 }
 #endif

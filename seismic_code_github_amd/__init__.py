@@ -82,7 +82,20 @@ _SIGNATURES = {
     "sfd_acoustic_plan_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_cfl_limit": (ctypes.c_double, [ctypes.c_int, _FP]),
+    "sfd_acoustic_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(AcousticParamsC),
+                                                 ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
+                                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_acoustic_batch_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_acoustic_batch_fetch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_acoustic_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_window_model": (ctypes.c_int, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP]),
 }
+
+BATCH_GRAPH = 0
+BATCH_LAUNCHES = 1
+BATCH_PER_SHOT = 2
 
 _lib = None
 
@@ -289,3 +302,71 @@ def acoustic_forward(shot: AcousticShot, ctx: Optional[Context] = None):
 def cfl_limit(radius: int, coef=None) -> float:
     c = None if coef is None else _f32(coef)
     return lib().sfd_acoustic_cfl_limit(radius, _ptr(c))
+
+
+def window_model(model, x0: int, nxe: int, pad_x: tuple, pad_z: tuple) -> np.ndarray:
+    """Aperture window of one shot (``sfd_window_model``): columns
+    clamp(x0 + w) of ``model`` [mnx, mnz] for w < nxe, edge-padded by
+    pad_x = (lo, hi) columns and pad_z = (lo, hi) rows."""
+    m = _f32(model)
+    mnx, mnz = m.shape
+    out = np.empty((nxe + pad_x[0] + pad_x[1], mnz + pad_z[0] + pad_z[1]), dtype=np.float32)
+    rc = lib().sfd_window_model(_ptr(m), mnx, mnz, x0, nxe, pad_x[0], pad_x[1], pad_z[0], pad_z[1], _ptr(out))
+    if rc != SFD_OK:
+        raise SeisfdError(rc, "sfd_window_model")
+    return out
+
+
+class AcousticBatch:
+    """``sfd_acoustic_batch``: many shots advanced by one launch per step."""
+
+    def __init__(self, shots, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.shots = list(shots)
+        n = len(self.shots)
+        params = (AcousticParamsC * n)()
+        self._keep = []
+        vps, dens, ws = (_FP * n)(), (_FP * n)(), (_FP * n)()
+        for i, sh in enumerate(self.shots):
+            params[i] = sh.params()
+            vp = _f32(sh.vp)
+            w = _f32(sh.wavelet)
+            den = None if sh.den is None else _f32(sh.den)
+            self._keep += [vp, w, den, sh]
+            vps[i], ws[i], dens[i] = _ptr(vp), _ptr(w), _ptr(den)
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_acoustic_batch_create(self.ctx.handle, n, params, vps, dens, ws, ctypes.byref(h)),
+                       "sfd_acoustic_batch_create")
+        self.handle = h
+        self.nt = params[0].nt
+        self.nrec = [params[i].nrec for i in range(n)]
+
+    def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
+        self.ctx.check(lib().sfd_acoustic_batch_set_tiling(self.handle, waves, chunk_rows), "batch_set_tiling")
+
+    def run(self, how: int = BATCH_GRAPH) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_acoustic_batch_run(self.handle, how, ctypes.byref(ms)), "sfd_acoustic_batch_run")
+        return ms.value
+
+    def fetch(self, shot: int) -> np.ndarray:
+        rec = np.zeros((self.nrec[shot], self.nt), dtype=np.float32)
+        self.ctx.check(lib().sfd_acoustic_batch_fetch(self.handle, shot, _ptr(rec)), "sfd_acoustic_batch_fetch")
+        return rec
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_acoustic_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -74,6 +74,37 @@ StepKernel pick_step_kernel(int radius, int nw, int mode) {
     }
 }
 
+using BatchKernel = void (*)(BatchStepArgs);
+
+template <int R, int NW>
+BatchKernel pick_batch_mode(int mode) {
+    switch (mode) {
+    case MODE_FAST_SPONGE: return acoustic_batch_step_kernel<R, NW, MODE_FAST_SPONGE, default_slots<R>()>;
+    case MODE_FAST_CPML: return acoustic_batch_step_kernel<R, NW, MODE_FAST_CPML, default_slots<R>()>;
+    case MODE_REF_SPONGE: return acoustic_batch_step_kernel<R, NW, MODE_REF_SPONGE, default_slots<R>()>;
+    default: return nullptr;
+    }
+}
+
+template <int R>
+BatchKernel pick_batch_nw(int nw, int mode) {
+    switch (nw) {
+    case 1: return pick_batch_mode<R, 1>(mode);
+    case 2: return pick_batch_mode<R, 2>(mode);
+    case 4: return pick_batch_mode<R, 4>(mode);
+    default: return nullptr;
+    }
+}
+
+BatchKernel pick_batch_kernel(int radius, int nw, int mode) {
+    switch (radius) {
+    case 4: return pick_batch_nw<4>(nw, mode);
+    case 5: return pick_batch_nw<5>(nw, mode);
+    case 8: return pick_batch_nw<8>(nw, mode);
+    default: return nullptr;
+    }
+}
+
 }  // namespace sfd
 
 using namespace sfd;
@@ -231,15 +262,16 @@ int zero_state(sfd_acoustic_plan* pl) {
     return SFD_OK;
 }
 
-// Enqueues steps k0 .. k0+nsteps-1 on the context stream.
-int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
-    sfd_context* ctx = pl->ctx;
+// Step-kernel arguments of a plan whose current time level is in buffers
+// `cur` (inputs) and cur ^ 1 (outputs).
+AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
     AcousticStepArgs a{};
     a.arena = pl->arena.as<float>();
     a.arena_bytes = (uint32_t)pl->arena.bytes;
     a.nx = pl->nx;
     a.nz = pl->nz;
     a.chunk = pl->chunk;
+    a.k = k;
     a.kf = pl->grid(pl->kf);
     a.bf = pl->grid(pl->bf);
     a.ax = pl->ax.as<float>() + pl->pad;
@@ -263,33 +295,40 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     a.dtdz = pl->dt / pl->dz;
     a.inv_dx = 1.0 / (double)pl->dx;
     a.inv_dz = 1.0 / (double)pl->dz;
-
     const int TZ = 64 * pl->nw;
     a.ntz = (pl->nz - 2 * pl->radius + TZ - 1) / TZ;
     a.nchunk = (pl->nx - 2 * pl->radius + pl->chunk - 1) / pl->chunk;
+    const int c = cur, o = cur ^ 1;
+    a.p_in = pl->grid(pl->p[c]);
+    a.vx_in = pl->grid(pl->vx[c]);
+    a.vz_in = pl->grid(pl->vz[c]);
+    a.p_out = pl->grid(pl->p[o]);
+    a.vx_out = pl->grid(pl->vx[o]);
+    a.vz_out = pl->grid(pl->vz[o]);
+    a.psi_px_in = pl->psi_px[c].as<float>();
+    a.psi_px_out = pl->psi_px[o].as<float>();
+    a.psi_vx_in = pl->psi_vx[c].as<float>();
+    a.psi_vx_out = pl->psi_vx[o].as<float>();
+    a.psi_pz_in = pl->psi_pz[c].as<float>();
+    a.psi_pz_out = pl->psi_pz[o].as<float>();
+    a.psi_vz_in = pl->psi_vz[c].as<float>();
+    a.psi_vz_out = pl->psi_vz[o].as<float>();
+    return a;
+}
+
+// Enqueues steps k0 .. k0+nsteps-1 on the context stream.
+int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
+    sfd_context* ctx = pl->ctx;
+    AcousticStepArgs a = make_args(pl, pl->cur, k0);
     const dim3 grid(a.ntz * a.nchunk);
-    const dim3 block(TZ);
+    const dim3 block(64 * pl->nw);
     const size_t nbytes = (size_t)pl->nx * pl->nz * sizeof(float);
     size_t snap_next = std::lower_bound(pl->snap_steps.begin(), pl->snap_steps.end(), k0) - pl->snap_steps.begin();
 
     for (int s = 0; s < nsteps; ++s) {
         const int k = k0 + s;
-        const int c = pl->cur, o = c ^ 1;
-        a.k = k;
-        a.p_in = pl->grid(pl->p[c]);
-        a.vx_in = pl->grid(pl->vx[c]);
-        a.vz_in = pl->grid(pl->vz[c]);
-        a.p_out = pl->grid(pl->p[o]);
-        a.vx_out = pl->grid(pl->vx[o]);
-        a.vz_out = pl->grid(pl->vz[o]);
-        a.psi_px_in = pl->psi_px[c].as<float>();
-        a.psi_px_out = pl->psi_px[o].as<float>();
-        a.psi_vx_in = pl->psi_vx[c].as<float>();
-        a.psi_vx_out = pl->psi_vx[o].as<float>();
-        a.psi_pz_in = pl->psi_pz[c].as<float>();
-        a.psi_pz_out = pl->psi_pz[o].as<float>();
-        a.psi_vz_in = pl->psi_vz[c].as<float>();
-        a.psi_vz_out = pl->psi_vz[o].as<float>();
+        const int o = pl->cur ^ 1;
+        a = make_args(pl, pl->cur, k);
         hipLaunchKernelGGL(pl->kernel, grid, block, 0, ctx->stream, a);
         SFD_HIP(ctx, hipGetLastError());
         pl->cur = o;
@@ -663,6 +702,191 @@ int sfd_acoustic_forward(sfd_context* ctx, const sfd_acoustic_params* p, const f
     if (!rc) rc = sfd_acoustic_plan_fetch(pl, record, snaps);
     sfd_acoustic_plan_destroy(pl);
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+struct sfd_acoustic_batch {
+    sfd_context* ctx = nullptr;
+    std::vector<sfd_acoustic_plan*> plans;
+    int nt = 0, radius = 0, mode = 0, nw = 0;
+    int ntile = 0;
+    DeviceBuffer shot_args, tile_shot, tile_base;
+    BatchKernel kernel = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~sfd_acoustic_batch() {
+        drop_graph();
+        for (auto* p : plans) sfd_acoustic_plan_destroy(p);
+    }
+};
+
+namespace {
+
+// Tile tables and per-shot argument blocks (both ping-pong orders).
+int build_batch_tables(sfd_acoustic_batch* b) {
+    const int n = (int)b->plans.size();
+    std::vector<AcousticStepArgs> args(2 * (size_t)n);
+    std::vector<int> base(n), tshot;
+    int t = 0;
+    for (int s = 0; s < n; ++s) {
+        sfd_acoustic_plan* pl = b->plans[s];
+        for (int par = 0; par < 2; ++par) args[(size_t)par * n + s] = make_args(pl, par, 0);
+        const int nt = args[s].ntz * args[s].nchunk;
+        base[s] = t;
+        tshot.insert(tshot.end(), nt, s);
+        t += nt;
+    }
+    b->ntile = t;
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
+    if (e == hipSuccess) e = upload(b->tile_shot, tshot.data(), tshot.size(), st);
+    if (e == hipSuccess) e = upload(b->tile_base, base.data(), base.size(), st);
+    if (e != hipSuccess) return hip_check(b->ctx, e, "batch tables");
+    b->drop_graph();
+    return SFD_OK;
+}
+
+int enqueue_batch_steps(sfd_acoustic_batch* b) {
+    BatchStepArgs a{};
+    a.shots = b->shot_args.as<AcousticStepArgs>();
+    a.tile_shot = b->tile_shot.as<int>();
+    a.tile_base = b->tile_base.as<int>();
+    a.nshot = (int)b->plans.size();
+    a.ntile = b->ntile;
+    for (int k = 0; k < b->nt; ++k) {
+        a.k = k;
+        a.parity = k & 1;
+        hipLaunchKernelGGL(b->kernel, dim3(b->ntile), dim3(64 * b->nw), 0, b->ctx->stream, a);
+        SFD_HIP(b->ctx, hipGetLastError());
+    }
+    return SFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfd_acoustic_batch_create(sfd_context* ctx, int nshot, const sfd_acoustic_params* params,
+                              const float* const* vp, const float* const* den, const float* const* wavelet,
+                              sfd_acoustic_batch** out) {
+    if (!ctx || !out) return SFD_EINVAL;
+    *out = nullptr;
+    if (nshot <= 0 || !params || !vp || !wavelet) return fail(ctx, SFD_EINVAL, "empty batch");
+    for (int s = 0; s < nshot; ++s) {
+        if (params[s].nt != params[0].nt || params[s].radius != params[0].radius ||
+            params[s].boundary != params[0].boundary || params[s].arith != params[0].arith)
+            return fail(ctx, SFD_EINVAL, "shot %d: every shot of a batch needs the same nt, radius, boundary kind "
+                                         "and arithmetic", s);
+        if (params[s].nsnap != 0) return fail(ctx, SFD_ENOTSUP, "shot %d: batches record gathers only", s);
+    }
+    sfd_acoustic_batch* b = new (std::nothrow) sfd_acoustic_batch;
+    if (!b) return fail(ctx, SFD_ENOMEM, "host allocation failed");
+    b->ctx = ctx;
+    for (int s = 0; s < nshot; ++s) {
+        sfd_acoustic_plan* pl = nullptr;
+        const int rc = sfd_acoustic_plan_create(ctx, &params[s], vp[s], den ? den[s] : nullptr, wavelet[s], &pl);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        b->plans.push_back(pl);
+    }
+    b->nt = params[0].nt;
+    b->radius = params[0].radius;
+    b->mode = b->plans[0]->mode;
+    const int rc = sfd_acoustic_batch_set_tiling(b, 0, 0);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SFD_OK;
+}
+
+int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch* b, int waves, int chunk_rows) {
+    if (!b) return SFD_EINVAL;
+    // Batches hold far more cells than one shot, so the default chunk is
+    // longer (less queue fill per cell) while the grid still fills the GPU.
+    const int nw = waves ? waves : 2;
+    long cells = 0;
+    for (auto* pl : b->plans) cells += (long)pl->nx * pl->nz;
+    const int ch = chunk_rows ? chunk_rows : (cells > 64L * 1000 * 1000 ? 128 : 0);
+    for (auto* pl : b->plans) {
+        const int rc = sfd_acoustic_plan_set_tiling(pl, nw, ch);
+        if (rc) return rc;
+    }
+    b->nw = nw;
+    b->kernel = pick_batch_kernel(b->radius, nw, b->mode);
+    if (!b->kernel) return fail(b->ctx, SFD_EINVAL, "waves must be 1, 2 or 4 (got %d)", waves);
+    return build_batch_tables(b);
+}
+
+int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
+    if (!b) return SFD_EINVAL;
+    sfd_context* ctx = b->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    if (how != SFD_BATCH_GRAPH && how != SFD_BATCH_LAUNCHES && how != SFD_BATCH_PER_SHOT)
+        return fail(ctx, SFD_EINVAL, "unknown batch run mode %d", how);
+    for (auto* pl : b->plans) {
+        const int rc = zero_state(pl);  // outside any graph: no memset nodes
+        if (rc) return rc;
+    }
+    if (how == SFD_BATCH_GRAPH && !b->exec) {
+        SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SFD_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_batch_steps(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        SFD_HIP(ctx, e);
+        b->graph = g;
+        SFD_HIP(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (how == SFD_BATCH_GRAPH) {
+        SFD_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    } else if (how == SFD_BATCH_LAUNCHES) {
+        const int rc = enqueue_batch_steps(b);
+        if (rc) return rc;
+    } else {
+        for (auto* pl : b->plans) {
+            const int rc = enqueue_steps(pl, 0, pl->nt);
+            if (rc) return rc;
+        }
+    }
+    SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
+    if (how != SFD_BATCH_PER_SHOT)
+        for (auto* pl : b->plans) pl->cur = b->nt & 1;
+    return SFD_OK;
+}
+
+int sfd_acoustic_batch_fetch(sfd_acoustic_batch* b, int shot, float* record) {
+    if (!b) return SFD_EINVAL;
+    if (shot < 0 || shot >= (int)b->plans.size()) return fail(b->ctx, SFD_EINVAL, "shot %d out of range", shot);
+    return sfd_acoustic_plan_fetch(b->plans[shot], record, nullptr);
+}
+
+int sfd_acoustic_batch_destroy(sfd_acoustic_batch* b) {
+    if (!b) return SFD_EINVAL;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return SFD_OK;
 }
 
 }  // extern "C"

@@ -239,8 +239,10 @@ struct StepLds {
     float vz[VS][VW + 1];
 };
 
-template <int R, int NW, int MODE, int NS>
-__device__ __forceinline__ void acoustic_march(const AcousticStepArgs& a, int tile, StepLds<R, NW>& lds) {
+// Args is AcousticStepArgs, held in kernel arguments or (batched shots) in
+// device memory read through the constant address space; k is the step.
+template <int R, int NW, int MODE, int NS, class Args>
+__device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, StepLds<R, NW>& lds) {
     constexpr int TZ = 64 * NW;         // z cells per workgroup
     constexpr int Q = 2 * R;            // register queue length
     constexpr int HR = 2 * R - 1;       // P halo on each side of the tile
@@ -302,7 +304,7 @@ __device__ __forceinline__ void acoustic_march(const AcousticStepArgs& a, int ti
     const int vo_vh = vh_upd ? 4 * jh : kOOB;
 
     const float az_j = (j < nz) ? a.az[j] : 1.0f;
-    const float amp = ro(a.amp, a.k);
+    const float amp = ro(a.amp, k);
     // One coefficient set in scalar registers: the z stencils use the x
     // coefficients and scale their sums by zx = dx/dz (fast arithmetic; the
     // reference arithmetic uses the unscaled C_m for both axes).
@@ -534,7 +536,7 @@ __device__ __forceinline__ void acoustic_march(const AcousticStepArgs& a, int ti
                 const int jhi = __builtin_amdgcn_readfirstlane(L.rjhi);
                 const int slot = __builtin_amdgcn_readfirstlane(L.rslot);
                 int more = __builtin_amdgcn_readfirstlane(L.rmore);
-                float* gk = a.rec_buf + (size_t)a.k * a.nslot;
+                float* gk = a.rec_buf + (size_t)k * a.nslot;
                 if (jupd && j >= jlo && j < jhi) gk[slot + (j - jlo)] = pn;
                 while (more >= 0) {
                     const int* sp = reinterpret_cast<const int*>(a.rec_more) + 4 * more;
@@ -548,34 +550,61 @@ __device__ __forceinline__ void acoustic_march(const AcousticStepArgs& a, int ti
     }
 }
 
-template <int R, int NW, int MODE, int NS>
-__global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticStepArgs a) {
-    __shared__ StepLds<R, NW> lds;
-    // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
-    // Remap so each XCD gets a contiguous run of tiles in (x chunk, z tile)
-    // order, z fastest: an XCD holds a band of x chunks across all z tiles,
-    // so both halo directions are re-read from its own L2, and the boundary
-    // tiles spread over all XCDs.
-    int tile;
-    {
-        const int n = a.ntz * a.nchunk, L = (int)blockIdx.x;
-        const int xcd = L % 8, idx = L / 8, q = n / 8, rem = n % 8;
-        tile = xcd * q + min(xcd, rem) + idx;
-    }
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Remap so each XCD gets a contiguous run of tiles in (x chunk, z tile)
+// order, z fastest: an XCD holds a band of x chunks across all z tiles, so
+// both halo directions are re-read from its own L2, and the boundary tiles
+// spread over all XCDs.
+__device__ __forceinline__ int xcd_tile(int L, int n) {
+    const int xcd = L % 8, idx = L / 8, q = n / 8, rem = n % 8;
+    return xcd * q + min(xcd, rem) + idx;
+}
+
+template <int R, int NW, int MODE, int NS, class Args>
+__device__ __forceinline__ void acoustic_tile(const Args& a, int tile, int k, StepLds<R, NW>& lds) {
     if constexpr (MODE == MODE_FAST_CPML) {
         // Tiles whose rows and columns (halos included) miss every PML strip
         // run without the memory-variable terms.
         constexpr int TZ = 64 * NW;
-        const int j0 = R + (tile % a.ntz) * TZ;
-        const int i0 = R + (tile / a.ntz) * a.chunk;
-        const int i1 = min(i0 + a.chunk, a.nx - R);
-        const bool xs = (i0 - 2 * R < a.pml_xw) || (i1 + 2 * R > a.nx - a.pml_xw);
-        const bool zs = (j0 - 2 * R < a.pml_zw) || (j0 + TZ + 2 * R > a.nz - a.pml_zw);
-        if (xs || zs) acoustic_march<R, NW, MODE_FAST_CPML, NS>(a, tile, lds);
-        else acoustic_march<R, NW, MODE_FAST_NONE, NS>(a, tile, lds);
+        const int ntz = a.ntz;
+        const int j0 = R + (tile % ntz) * TZ;
+        const int i0 = R + (tile / ntz) * a.chunk;
+        const int nx = a.nx, nz = a.nz, xw = a.pml_xw, zw = a.pml_zw;
+        const int i1 = min(i0 + a.chunk, nx - R);
+        const bool xs = (i0 - 2 * R < xw) || (i1 + 2 * R > nx - xw);
+        const bool zs = (j0 - 2 * R < zw) || (j0 + TZ + 2 * R > nz - zw);
+        if (xs || zs) acoustic_march<R, NW, MODE_FAST_CPML, NS>(a, tile, k, lds);
+        else acoustic_march<R, NW, MODE_FAST_NONE, NS>(a, tile, k, lds);
     } else {
-        acoustic_march<R, NW, MODE, NS>(a, tile, lds);
+        acoustic_march<R, NW, MODE, NS>(a, tile, k, lds);
     }
+}
+
+template <int R, int NW, int MODE, int NS>
+__global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticStepArgs a) {
+    __shared__ StepLds<R, NW> lds;
+    acoustic_tile<R, NW, MODE, NS>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, lds);
+}
+
+// Many shots in one launch.  shots[parity * nshot + s] is the argument block
+// of shot s with its ping-pong buffers in the given order; the workgroups of
+// one launch cover the tiles of every shot, tile_shot maps a global tile to
+// its shot and tile_base[s] is the shot's first global tile.
+struct BatchStepArgs {
+    const AcousticStepArgs* shots;
+    const int* tile_shot;
+    const int* tile_base;
+    int nshot, ntile, k, parity;
+};
+
+template <int R, int NW, int MODE, int NS>
+__global__ __launch_bounds__(64 * NW) void acoustic_batch_step_kernel(const BatchStepArgs b) {
+    __shared__ StepLds<R, NW> lds;
+    const int tile = xcd_tile((int)blockIdx.x, b.ntile);
+    const int s = ro(b.tile_shot, tile);
+    using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
+    CArgs& a = *(CArgs*)(b.shots + (size_t)b.parity * b.nshot + s);
+    acoustic_tile<R, NW, MODE, NS>(a, tile - ro(b.tile_base, s), b.k, lds);
 }
 
 // Prefetch slots per radius: the slot count must divide 2R.

@@ -1,4 +1,5 @@
 // Context and shared helpers of libseisfd.
+#include <algorithm>
 #include <cmath>
 #include <new>
 
@@ -88,6 +89,26 @@ int sfd_context_destroy(sfd_context* ctx) {
 
 const char* sfd_last_error(const sfd_context* ctx) {
     return ctx ? ctx->err.c_str() : "null context";
+}
+
+int sfd_window_model(const float* model, int mnx, int mnz, int x0, int nxe, int pad_xlo, int pad_xhi, int pad_zlo,
+                     int pad_zhi, float* out) {
+    if (!model || !out || mnx <= 0 || mnz <= 0 || nxe <= 0 || pad_xlo < 0 || pad_xhi < 0 || pad_zlo < 0 ||
+        pad_zhi < 0)
+        return SFD_EINVAL;
+    const int nx = nxe + pad_xlo + pad_xhi, nz = mnz + pad_zlo + pad_zhi;
+    for (int x = 0; x < nx; ++x) {
+        // Window column (edge-replicated into the x padding), then model
+        // column clamped to the model.
+        const int w = std::min(std::max(x - pad_xlo, 0), nxe - 1);
+        const int mx = std::min(std::max(x0 + w, 0), mnx - 1);
+        const float* src = model + (size_t)mx * mnz;
+        float* dst = out + (size_t)x * nz;
+        for (int z = 0; z < pad_zlo; ++z) dst[z] = src[0];
+        for (int z = 0; z < mnz; ++z) dst[pad_zlo + z] = src[z];
+        for (int z = 0; z < pad_zhi; ++z) dst[pad_zlo + mnz + z] = src[mnz - 1];
+    }
+    return SFD_OK;
 }
 
 double sfd_acoustic_cfl_limit(int radius, const float* coef) {

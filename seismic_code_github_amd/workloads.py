@@ -25,7 +25,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import AcousticShot, FIELD_P, FIELD_VX, FIELD_VZ
+from . import AcousticShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model
 
 PI_REF = 3.1415926
 
@@ -103,6 +103,34 @@ def smooth_random_2001x4001(nt: int = 6000, radius: int = 8, seed: int = 1, scal
               pml_f0=20.0, snap_steps=(), snap_fields=FIELD_P)
     kw.update(overrides)
     return AcousticShot(**kw)
+
+
+def windowed_shots(nshot: int = 64, nt: int = 3000, scale: int = 1, radius: int = 8, width: int = 50,
+                   model_seed: int = 4, window_seed: int = 5, boundary: str = "cpml"):
+    """Many shots on aperture windows of one model: a 1001 (z) x 6001 (x)
+    seeded smooth velocity model (sigma = 20 cells, 1500-4500 m/s); shot s
+    uses a window of W_s ~ U{1000..2500} columns at a seeded origin where it
+    fits, padded by `width` absorbing cells on every side (edge replication,
+    as get_currshot_parameters does), source at the window centre 2 cells
+    below the top of the model, receivers in every window column at that
+    depth.  dx = dz = 10 m, dt = 1 ms, 20 Hz Ricker.  scale > 1 shrinks the
+    model and the windows by that factor."""
+    mnz, mnx = 1000 // scale + 1, 6000 // scale + 1
+    model = smooth_random_model(mnx, mnz, 20.0 / scale, 1500.0, 4500.0, model_seed)
+    rng = np.random.default_rng(window_seed)
+    dt = 0.001
+    w = ricker(nt, dt, 20.0)
+    shots = []
+    for s in range(nshot):
+        ws = int(rng.integers(1000 // scale, 2500 // scale + 1))
+        x0 = int(rng.integers(0, mnx - ws + 1))
+        vp = window_model(model, x0, ws, (width, width), (width, width))
+        rx = np.arange(width, width + ws, dtype=np.int32)
+        rz = np.full_like(rx, width + 2)
+        shots.append(AcousticShot(vp=vp, den=None, wavelet=w, src=(width + ws // 2, width + 2), rec=(rx, rz),
+                                  dx=10.0, dz=10.0, dt=dt, radius=radius, boundary=boundary, width=width,
+                                  pml_r=1e-3, pml_f0=20.0))
+    return model, shots
 
 
 def cell_updates(shot: AcousticShot) -> int:
