@@ -161,6 +161,28 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch *batch, int how, float *elapsed_ms
 int sfd_acoustic_batch_fetch(sfd_acoustic_batch *batch, int shot, float *record);
 int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
 
+/* ---------------------------------------------------------------- gradient
+ * Velocity gradient by the discrete adjoint of the fast-arithmetic sponge
+ * step (no reference counterpart: the reference models data only).  The
+ * wavefield is checkpointed every ckpt_every steps and recomputed one
+ * segment at a time during the reverse sweep, so device memory grows with
+ * nt / ckpt_every rather than nt.
+ *
+ * sfd_acoustic_gradient: J = sum_{r,k} seed[r][k] record[r][k] (seed is
+ *   [nrec][nt], the vector of a vector-Jacobian product); grad_vp [nx*nz]
+ *   receives dJ/dvp, record (may be NULL) the forward gather.
+ * sfd_acoustic_misfit_gradient: J = sum_s 1/2 |record_s - d_obs[s]|^2 over
+ *   nshot shots on one model (vp, den of params[0]'s grid; wavelet[s] and
+ *   the geometry of params[s] per shot); grad_vp receives dJ/dvp summed over
+ *   shots, misfit (may be NULL) the value of J. */
+int sfd_acoustic_gradient(sfd_context *ctx, const sfd_acoustic_params *p, const float *vp,
+                          const float *den, const float *wavelet, const float *seed,
+                          int ckpt_every, float *grad_vp, float *record);
+int sfd_acoustic_misfit_gradient(sfd_context *ctx, int nshot, const sfd_acoustic_params *params,
+                                 const float *vp, const float *den, const float *const *wavelet,
+                                 const float *const *d_obs, int ckpt_every, float *grad_vp,
+                                 double *misfit);
+
 /* Aperture window of one shot, the drop-in for get_currshot_parameters
  * (forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:315-373;
  * the same routine in yang_version_2016_2_2/Iso_multishot_fd_subroutine_fsbc_zy_m2_pub.f90:216-266).

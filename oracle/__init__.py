@@ -80,6 +80,7 @@ def lib():
         L.oracle_sponge_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
         L.oracle_ricker.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, fp]
         L.oracle_staggered_coefs.argtypes = [ctypes.c_int, dp]
+        L.oracle_acoustic_gradient_f64.argtypes = [ctypes.POINTER(_Args), dp, dp, dp]
         L.oracle_window.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
         L.oracle_cpml_coef.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -169,6 +170,32 @@ def acoustic(vp, den, wavelet, src, rec, *, dx=10.0, dz=10.0, dt=5e-4, radius=8,
     if rc != 0:
         raise ValueError(f"oracle_acoustic returned {rc}")
     return {"record": record, "snaps": snaps, "state": st, "pml": pml}
+
+
+def gradient(vp, den, wavelet, src, rec, seed, *, dx=10.0, dz=10.0, dt=5e-4, radius=8, sponge=20,
+             coef=None):
+    """FP64 discrete adjoint (oracle/adjoint.c): dJ/dvp for
+    J = sum(seed * record); returns (grad [nx, nz], record [nrec, nt])."""
+    vp = np.ascontiguousarray(vp, dtype=np.float32)
+    den = np.ascontiguousarray(den, dtype=np.float32)
+    nx, nz = vp.shape
+    wavelet = np.ascontiguousarray(wavelet, dtype=np.float32)
+    nt = wavelet.shape[0]
+    coef = coefs_for_radius(radius) if coef is None else np.ascontiguousarray(coef, dtype=np.float32)
+    rx = np.ascontiguousarray(rec[0], dtype=np.int32)
+    rz = np.ascontiguousarray(rec[1], dtype=np.int32)
+    seed = np.ascontiguousarray(seed, dtype=np.float64)
+    assert seed.shape == (rx.size, nt)
+    args = _Args(nx, nz, nt, 0, dx, dz, dt, radius, _ptr(coef), sponge, _ptr(None), _ptr(vp), _ptr(den),
+                 _ptr(wavelet), int(src[0]), int(src[1]), rx.size, _ptr(rx, ctypes.c_int), _ptr(rz, ctypes.c_int),
+                 0, _ptr(None, ctypes.c_int), 1, 0, 0.0, 0.0, 0.0)
+    grad = np.zeros((nx, nz), dtype=np.float64)
+    record = np.zeros((rx.size, nt), dtype=np.float64)
+    rc = lib().oracle_acoustic_gradient_f64(ctypes.byref(args), _ptr(seed, ctypes.c_double),
+                                            _ptr(grad, ctypes.c_double), _ptr(record, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"oracle_acoustic_gradient_f64 returned {rc}")
+    return grad, record
 
 
 def reference_config():

@@ -82,6 +82,12 @@ void oracle_staggered_coefs(int radius, double *c);
 void oracle_cpml_coef(int i, double shift, int n, int w, double h, double dt, double R,
                       double vmax, double f0, double *b, double *a);
 
+/* Discrete adjoint (FP64) of the acoustic step with the sponge boundary:
+ * for J = sum_{r,k} seed[r][k] * record[r][k], returns dJ/dvp [nx*nz] and,
+ * if record != NULL, the forward record [nrec][nt] (k0 must be 0). */
+int oracle_acoustic_gradient_f64(const oracle_acoustic_args *a, const double *seed, double *grad_vp,
+                                 double *record);
+
 /* Aperture window + edge padding of one shot, get_currshot_parameters
  * (forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:288-373). */
 void oracle_window(const float *model, int nx, int nz, int ixmine, int nxe, int npxn, int npxp,

@@ -89,6 +89,11 @@ _SIGNATURES = {
     "sfd_acoustic_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
     "sfd_acoustic_batch_fetch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
     "sfd_acoustic_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_acoustic_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AcousticParamsC), _FP, _FP, _FP, _FP,
+                                             ctypes.c_int, _FP, _FP]),
+    "sfd_acoustic_misfit_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(AcousticParamsC),
+                                                    _FP, _FP, ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.c_int,
+                                                    _FP, ctypes.POINTER(ctypes.c_double)]),
     "sfd_window_model": (ctypes.c_int, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP]),
 }
@@ -297,6 +302,54 @@ def acoustic_forward(shot: AcousticShot, ctx: Optional[Context] = None):
                                          _ptr(record) if p.nrec else None, _ptr(snaps) if p.nsnap else None),
               "sfd_acoustic_forward")
     return record, snaps
+
+
+def acoustic_gradient(shot: AcousticShot, seed, ckpt_every: int = 50, ctx: Optional[Context] = None):
+    """dJ/dvp for J = sum(seed * record) through ``sfd_acoustic_gradient``
+    (discrete adjoint of the fast sponge step, checkpoints every
+    ``ckpt_every`` steps).  Returns (grad [nx, nz], record [nrec, nt])."""
+    ctx = ctx or default_context()
+    p = shot.params()
+    nx, nz = np.asarray(shot.vp).shape
+    vp = _f32(shot.vp)
+    den = None if shot.den is None else _f32(shot.den)
+    w = _f32(shot.wavelet)
+    sd = _f32(seed)
+    if sd.shape != (p.nrec, p.nt):
+        raise ValueError(f"seed must be [nrec, nt] = {(p.nrec, p.nt)}, got {sd.shape}")
+    grad = np.zeros((nx, nz), dtype=np.float32)
+    record = np.zeros((p.nrec, p.nt), dtype=np.float32)
+    ctx.check(lib().sfd_acoustic_gradient(ctx.handle, ctypes.byref(p), _ptr(vp), _ptr(den), _ptr(w), _ptr(sd),
+                                          ckpt_every, _ptr(grad), _ptr(record)), "sfd_acoustic_gradient")
+    return grad, record
+
+
+def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: Optional[Context] = None):
+    """J = sum_s 1/2 |record_s - d_obs[s]|^2 over shots sharing one model and
+    its dJ/dvp, through ``sfd_acoustic_misfit_gradient``.  The model is
+    ``vp``/``den`` (default: the first shot's).  Returns (J, grad [nx, nz])."""
+    ctx = ctx or default_context()
+    shots = list(shots)
+    n = len(shots)
+    vp = _f32(shots[0].vp if vp is None else vp)
+    den = shots[0].den if den is None else den
+    den = None if den is None else _f32(den)
+    params = (AcousticParamsC * n)()
+    ws, ds = (_FP * n)(), (_FP * n)()
+    keep = []
+    for i, sh in enumerate(shots):
+        params[i] = sh.params()
+        w = _f32(sh.wavelet)
+        d = _f32(d_obs[i])
+        if d.shape != (params[i].nrec, params[i].nt):
+            raise ValueError(f"d_obs[{i}] must be [nrec, nt]")
+        keep += [w, d, sh]
+        ws[i], ds[i] = _ptr(w), _ptr(d)
+    grad = np.zeros(vp.shape, dtype=np.float32)
+    J = ctypes.c_double()
+    ctx.check(lib().sfd_acoustic_misfit_gradient(ctx.handle, n, params, _ptr(vp), _ptr(den), ws, ds, ckpt_every,
+                                                 _ptr(grad), ctypes.byref(J)), "sfd_acoustic_misfit_gradient")
+    return J.value, grad
 
 
 def cfl_limit(radius: int, coef=None) -> float:

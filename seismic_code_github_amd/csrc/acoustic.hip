@@ -7,8 +7,7 @@
 #include <new>
 #include <tuple>
 
-#include "acoustic_kernel.h"
-#include "common.h"
+#include "acoustic_plan.h"
 
 namespace sfd {
 
@@ -34,7 +33,6 @@ __global__ void gather_transpose_kernel(const float* __restrict__ rec_buf, const
 
 // ---------------------------------------------------------------- launch table
 
-using StepKernel = void (*)(AcousticStepArgs);
 
 template <int R, int NW>
 StepKernel pick_mode(int mode) {
@@ -42,6 +40,8 @@ StepKernel pick_mode(int mode) {
     case MODE_FAST_SPONGE: return acoustic_step_kernel<R, NW, MODE_FAST_SPONGE, default_slots<R>()>;
     case MODE_FAST_CPML: return acoustic_step_kernel<R, NW, MODE_FAST_CPML, default_slots<R>()>;
     case MODE_REF_SPONGE: return acoustic_step_kernel<R, NW, MODE_REF_SPONGE, default_slots<R>()>;
+    case MODE_FWD_DIV: return acoustic_step_kernel<R, NW, MODE_FWD_DIV, default_slots<R>()>;
+    case MODE_ADJ: return acoustic_step_kernel<R, NW, MODE_ADJ, default_slots<R>()>;
     default: return nullptr;
     }
 }
@@ -74,7 +74,6 @@ StepKernel pick_step_kernel(int radius, int nw, int mode) {
     }
 }
 
-using BatchKernel = void (*)(BatchStepArgs);
 
 template <int R, int NW>
 BatchKernel pick_batch_mode(int mode) {
@@ -109,35 +108,8 @@ BatchKernel pick_batch_kernel(int radius, int nw, int mode) {
 
 using namespace sfd;
 
-struct sfd_acoustic_plan {
-    sfd_context* ctx = nullptr;
-    int nx = 0, nz = 0, nt = 0, radius = 0, mode = 0;
-    float dx = 0, dz = 0, dt = 0;
-    int width = 0;
-    int nrec = 0, nslot = 0, nsnap = 0, snap_fields = 0, nsnap_fields = 0;
-    std::vector<int> snap_steps;
-    float coef[8] = {};
-    int pml_xw = 0, pml_zw = 0;
-    int sx = 0, sz = 0;
-    int pad = 0;  // zero rows above and below the grid in the state buffers
-    int nw = 0, chunk = 0;
-    int cur = 0;  // ping-pong index holding the current time level
 
-    // One arena holds every field the step kernel addresses (state, model,
-    // C-PML strips), so the kernel needs a single buffer descriptor.
-    DeviceBuffer arena;
-    Field p[2], vx[2], vz[2];
-    Field kf, bf;
-    Field psi_px[2], psi_vx[2], psi_pz[2], psi_vz[2];
-    DeviceBuffer ax, az, pml_x, pml_z;
-    DeviceBuffer amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
-    StepKernel kernel = nullptr;
-
-    // Row 0 of a padded per-row allocation.
-    float* grid(const Field& b) const { return b.as<float>() + (size_t)pad * nz; }
-};
-
-namespace {
+namespace sfd {
 
 int validate(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* wavelet) {
     if (!p || !vp || !wavelet) return fail(ctx, SFD_EINVAL, "null params, vp or wavelet");
@@ -218,14 +190,6 @@ void cpml_profile(int n, int w, double h, double dt, double R, double vmax, doub
     }
 }
 
-template <class T>
-hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
-    hipError_t e = b.alloc(n * sizeof(T));
-    if (e != hipSuccess || n == 0) return e;
-    e = hipMemcpyAsync(b.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
-}
 
 // Chunks start at rows congruent to R modulo 2R (the kernel's queue slots are
 // compile-time constants), so the row count of a chunk is a multiple of 2R.
@@ -263,8 +227,9 @@ int zero_state(sfd_acoustic_plan* pl) {
 }
 
 // Step-kernel arguments of a plan whose current time level is in buffers
-// `cur` (inputs) and cur ^ 1 (outputs).
-AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
+// `cur` (inputs) and cur ^ 1 (outputs) of the given state fields.
+AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* P, const Field* VX, const Field* VZ, int cur,
+                           int k) {
     AcousticStepArgs a{};
     a.arena = pl->arena.as<float>();
     a.arena_bytes = (uint32_t)pl->arena.bytes;
@@ -299,12 +264,12 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
     a.ntz = (pl->nz - 2 * pl->radius + TZ - 1) / TZ;
     a.nchunk = (pl->nx - 2 * pl->radius + pl->chunk - 1) / pl->chunk;
     const int c = cur, o = cur ^ 1;
-    a.p_in = pl->grid(pl->p[c]);
-    a.vx_in = pl->grid(pl->vx[c]);
-    a.vz_in = pl->grid(pl->vz[c]);
-    a.p_out = pl->grid(pl->p[o]);
-    a.vx_out = pl->grid(pl->vx[o]);
-    a.vz_out = pl->grid(pl->vz[o]);
+    a.p_in = pl->grid(P[c]);
+    a.vx_in = pl->grid(VX[c]);
+    a.vz_in = pl->grid(VZ[c]);
+    a.p_out = pl->grid(P[o]);
+    a.vx_out = pl->grid(VX[o]);
+    a.vz_out = pl->grid(VZ[o]);
     a.psi_px_in = pl->psi_px[c].as<float>();
     a.psi_px_out = pl->psi_px[o].as<float>();
     a.psi_vx_in = pl->psi_vx[c].as<float>();
@@ -313,7 +278,26 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
     a.psi_pz_out = pl->psi_pz[o].as<float>();
     a.psi_vz_in = pl->psi_vz[c].as<float>();
     a.psi_vz_out = pl->psi_vz[o].as<float>();
+    a.div = pl->grid_period_div();
+    a.div_stride = pl->div_stride();
+    a.div_period = pl->grad_period > 0 ? pl->grad_period : 1;
+    a.grad = pl->grad_period > 0 ? pl->grid(pl->grad) : nullptr;
+    a.src_adj = pl->src_adj.as<float>();
     return a;
+}
+
+AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
+    return make_args(pl, pl->p, pl->vx, pl->vz, cur, k);
+}
+
+float sponge_at(const sfd_acoustic_plan* pl, int i, int j) {
+    if (pl->mode == MODE_FAST_CPML || pl->width <= 0) return 1.0f;
+    const int L = pl->width;
+    if (i < L) return sponge_factor(L - i - 1, L);
+    if (i >= pl->nx - L) return sponge_factor(i - pl->nx + L, L);
+    if (j < L) return sponge_factor(L - j - 1, L);
+    if (j >= pl->nz - L) return sponge_factor(j - pl->nz + L, L);
+    return 1.0f;
 }
 
 // Enqueues steps k0 .. k0+nsteps-1 on the context stream.
@@ -346,12 +330,19 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     return SFD_OK;
 }
 
-}  // namespace
+}  // namespace sfd
 
 extern "C" {
 
 int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp,
                              const float* den, const float* wavelet, sfd_acoustic_plan** out) {
+    return plan_create(ctx, p, vp, den, wavelet, 0, out);
+}
+
+}  // extern "C"
+
+int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
+                     const float* wavelet, int grad_period, sfd_acoustic_plan** out) {
     if (!ctx || !out) return SFD_EINVAL;
     *out = nullptr;
     int rc = validate(ctx, p, vp, wavelet);
@@ -361,6 +352,7 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
     sfd_acoustic_plan* pl = new (std::nothrow) sfd_acoustic_plan;
     if (!pl) return fail(ctx, SFD_ENOMEM, "host allocation failed");
     pl->ctx = ctx;
+    pl->grad_period = grad_period;
     pl->nx = p->nx;
     pl->nz = p->nz;
     pl->nt = p->nt;
@@ -499,6 +491,15 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
             pl->psi_pz[b] = cv.take(zs);
             pl->psi_vz[b] = cv.take(zs);
         }
+        if (grad_period > 0) {
+            for (int b = 0; b < 2; ++b) {
+                pl->ap[b] = cv.take(npad * 4);
+                pl->avx[b] = cv.take(npad * 4);
+                pl->avz[b] = cv.take(npad * 4);
+            }
+            pl->grad = cv.take(npad * 4);
+            pl->div = cv.take(npad * 4 * (size_t)grad_period);
+        }
         if (pass == 0) {
             if (cv.used >= (size_t)1 << 31)
                 return bail(fail(ctx, SFD_ENOTSUP, "grid needs %zu bytes of fields; the step kernel addresses < 2 GiB",
@@ -543,6 +544,8 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
     *out = pl;
     return SFD_OK;
 }
+
+extern "C" {
 
 int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan* pl, int waves, int chunk_rows) {
     if (!pl) return SFD_EINVAL;

@@ -1,0 +1,85 @@
+// Acoustic shot plan shared by the forward, batch and gradient entry points.
+#pragma once
+#include <vector>
+
+#include "acoustic_kernel.h"
+#include "common.h"
+
+namespace sfd {
+
+using StepKernel = void (*)(AcousticStepArgs);
+using BatchKernel = void (*)(BatchStepArgs);
+
+StepKernel pick_step_kernel(int radius, int nw, int mode);
+BatchKernel pick_batch_kernel(int radius, int nw, int mode);
+int pad_rows(int radius);
+int round_chunk(int ch, int radius);
+
+template <class T>
+hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
+    hipError_t e = b.alloc(n * sizeof(T));
+    if (e != hipSuccess || n == 0) return e;
+    e = hipMemcpyAsync(b.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+}  // namespace sfd
+
+struct sfd_acoustic_plan {
+    using StepKernel = sfd::StepKernel;
+    using Field = sfd::Field;
+    using DeviceBuffer = sfd::DeviceBuffer;
+    sfd_context* ctx = nullptr;
+    int nx = 0, nz = 0, nt = 0, radius = 0, mode = 0;
+    float dx = 0, dz = 0, dt = 0;
+    int width = 0;
+    int nrec = 0, nslot = 0, nsnap = 0, snap_fields = 0, nsnap_fields = 0;
+    std::vector<int> snap_steps;
+    float coef[8] = {};
+    int pml_xw = 0, pml_zw = 0;
+    int sx = 0, sz = 0;
+    int pad = 0;  // zero rows above and below the grid in the state buffers
+    int nw = 0, chunk = 0;
+    int cur = 0;  // ping-pong index holding the current time level
+
+    // One arena holds every field the step kernel addresses (state, model,
+    // C-PML strips), so the kernel needs a single buffer descriptor.
+    DeviceBuffer arena;
+    Field p[2], vx[2], vz[2];
+    Field kf, bf;
+    Field psi_px[2], psi_vx[2], psi_pz[2], psi_vz[2];
+    DeviceBuffer ax, az, pml_x, pml_z;
+    DeviceBuffer amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
+    StepKernel kernel = nullptr;
+
+    // Gradient workspace (plans made by the gradient entry points): adjoint
+    // state, gradient accumulator and the divergence of one checkpoint
+    // segment live in the arena; checkpoints, residuals and injection weights
+    // in their own buffers.
+    int grad_period = 0;  // checkpoint interval; 0 = no gradient workspace
+    Field ap[2], avx[2], avz[2], grad, div;
+    DeviceBuffer ckpt, res, inj_w, inj_cell, src_adj;
+
+    // Row 0 of a padded per-row allocation.
+    float* grid(const Field& b) const { return b.as<float>() + (size_t)pad * nz; }
+    // Floats between consecutive divergence fields of the gradient workspace.
+    size_t div_stride() const { return (size_t)(nx + 2 * pad) * nz; }
+    float* grid_period_div() const { return grad_period > 0 ? grid(div) : nullptr; }
+};
+
+namespace sfd {
+
+// Creates a plan; grad_period > 0 also reserves the gradient workspace.
+int plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
+                const float* wavelet, int grad_period, sfd_acoustic_plan** out);
+// Step arguments over the state fields (p, vx, vz)[cur] -> [cur ^ 1].
+AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* p, const Field* vx, const Field* vz, int cur,
+                           int k);
+AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k);
+int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps);
+int zero_state(sfd_acoustic_plan* pl);
+// Sponge factor of cell (i, j) as the step kernel applies it.
+float sponge_at(const sfd_acoustic_plan* pl, int i, int j);
+
+}  // namespace sfd

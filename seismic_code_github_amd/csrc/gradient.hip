@@ -1,0 +1,269 @@
+// Velocity gradient of the acoustic propagator by the discrete adjoint
+// (no counterpart in the reference, which has no adjoint or inversion code).
+//
+// For J = sum_{r,k} seed[r][k] * record[r][k] (the L2 misfit is the case
+// seed = record - observed) the adjoint of the fast-arithmetic sponge step is
+// the same staggered step applied to the scaled adjoint fields
+// (K lambda_P, -b lambda_V), K = rho vp^2 dt, b = dt / rho.  Per shot:
+//   1. forward over all nt steps, saving the wavefield every `period` steps;
+//   2. seed (residual) per receiver cell, time-major, on the device;
+//   3. backward, segment by segment from the last: restore the segment's
+//      checkpoint, recompute its steps storing div_k = D-Vx' + D-Vz', then run
+//      the adjoint steps k = end-1 .. start, each accumulating
+//      g += P_adj_in * div_k and followed by the injection of seed_{k-1};
+//   4. dJ/dvp = -2 g / vp, plus the source term -sum_k adj_src_k w_k / vp_s^2
+//      (the injected amplitude dt rho_s vp_s w_k depends on vp at the source).
+// Memory: one segment of divergences plus nt/period checkpoints per shot.
+#include <algorithm>
+#include <cmath>
+#include <new>
+
+#include "acoustic_plan.h"
+
+using namespace sfd;
+
+namespace sfd {
+
+// P[cell[s]] += w[s] * res[k * nslot + s]: the seed of step k at the
+// receiver cells, absorbed (w = sponge factor * K of the cell).
+__global__ void adjoint_inject_kernel(float* __restrict__ p, const int* __restrict__ cell,
+                                      const float* __restrict__ w, const float* __restrict__ res, int nslot,
+                                      int k) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslot) p[cell[s]] += w[s] * res[(size_t)k * nslot + s];
+}
+
+namespace {
+
+struct GradShot {
+    sfd_acoustic_plan* pl = nullptr;
+    int nseg = 0;
+    std::vector<int> slot_cell;
+};
+
+int check_gradient_params(sfd_context* ctx, const sfd_acoustic_params* p) {
+    if (p->boundary != SFD_BOUNDARY_SPONGE || p->arith != SFD_ARITH_FAST)
+        return fail(ctx, SFD_ENOTSUP, "the gradient exists for the sponge boundary in fast arithmetic");
+    if (p->nt < 1) return fail(ctx, SFD_EINVAL, "nt < 1");
+    const int R = p->radius;
+    if (p->sx < R || p->sx >= p->nx - R || p->sz < R || p->sz >= p->nz - R)
+        return fail(ctx, SFD_EINVAL, "the gradient needs the source inside the updated cells");
+    return SFD_OK;
+}
+
+// Creates the plan with its gradient workspace and the receiver-cell tables.
+int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
+               const float* wavelet, int period, GradShot* g) {
+    sfd_acoustic_params q = *p;
+    q.nsnap = 0;
+    q.snap_steps = nullptr;
+    int rc = plan_create(ctx, &q, vp, den, wavelet, period, &g->pl);
+    if (rc) return rc;
+    sfd_acoustic_plan* pl = g->pl;
+    g->nseg = (pl->nt + period - 1) / period;
+    // slot -> cell (row-major over the padded state field) and weight.
+    std::vector<int> cell(std::max(pl->nslot, 1), 0);
+    std::vector<float> w(std::max(pl->nslot, 1), 0.f);
+    std::vector<int> slot_of(p->nrec);
+    {
+        // Slots were assigned to distinct cells in (x, z) order.
+        std::vector<std::pair<long, int>> cells(p->nrec);
+        for (int r = 0; r < p->nrec; ++r) cells[r] = {(long)p->rx[r] * pl->nz + p->rz[r], r};
+        std::sort(cells.begin(), cells.end());
+        int slot = -1;
+        long prev = -1;
+        for (auto& [c, r] : cells) {
+            if (c != prev) ++slot;
+            prev = c;
+            slot_of[r] = slot;
+            const int i = (int)(c / pl->nz), j = (int)(c % pl->nz);
+            const bool upd = i >= p->radius && i < pl->nx - p->radius && j >= p->radius && j < pl->nz - p->radius;
+            const float d = den ? den[c] : 2400.0f;
+            cell[slot] = (int)((long)(i + pl->pad) * pl->nz + j);
+            w[slot] = upd ? sponge_at(pl, i, j) * (d * vp[c] * vp[c] * p->dt) : 0.f;
+        }
+    }
+    g->slot_cell = slot_of;
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)pl->nx * pl->nz;
+    hipError_t e = upload(pl->inj_cell, cell.data(), cell.size(), s);
+    if (e == hipSuccess) e = upload(pl->inj_w, w.data(), w.size(), s);
+    if (e == hipSuccess) e = pl->res.alloc((size_t)std::max(pl->nslot, 1) * pl->nt * 4);
+    if (e == hipSuccess) e = pl->src_adj.alloc((size_t)pl->nt * 4);
+    if (e == hipSuccess) e = pl->ckpt.alloc((size_t)std::max(g->nseg - 1, 1) * 3 * n * 4);
+    if (e != hipSuccess) return hip_check(ctx, e, "gradient workspace");
+    return SFD_OK;
+}
+
+// Forward over all steps, saving the state before every segment but the first.
+int grad_forward(GradShot* g) {
+    sfd_acoustic_plan* pl = g->pl;
+    sfd_context* ctx = pl->ctx;
+    int rc = zero_state(pl);
+    if (rc) return rc;
+    const int T = pl->grad_period;
+    const size_t n = (size_t)pl->nx * pl->nz;
+    for (int seg = 0; seg < g->nseg; ++seg) {
+        const int k0 = seg * T;
+        if (seg > 0) {
+            float* dst = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
+            const Field* f[3] = {&pl->p[pl->cur], &pl->vx[pl->cur], &pl->vz[pl->cur]};
+            for (int q = 0; q < 3; ++q)
+                SFD_HIP(ctx, hipMemcpyAsync(dst + q * n, pl->grid(*f[q]), n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        rc = enqueue_steps(pl, k0, std::min(T, pl->nt - k0));
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
+// Time-major per-slot seed from the trace-major per-receiver seed.
+int grad_upload_seed(GradShot* g, const float* seed) {
+    sfd_acoustic_plan* pl = g->pl;
+    const int nt = pl->nt, ns = std::max(pl->nslot, 1);
+    std::vector<float> res((size_t)ns * nt, 0.f);
+    for (int r = 0; r < (int)g->slot_cell.size(); ++r)
+        for (int k = 0; k < nt; ++k) res[(size_t)k * ns + g->slot_cell[r]] += seed[(size_t)r * nt + k];
+    SFD_HIP(pl->ctx, hipMemcpyAsync(pl->res.ptr, res.data(), res.size() * 4, hipMemcpyHostToDevice,
+                                    pl->ctx->stream));
+    SFD_HIP(pl->ctx, hipStreamSynchronize(pl->ctx->stream));
+    return SFD_OK;
+}
+
+int launch_inject(sfd_acoustic_plan* pl, float* p_grid, int k) {
+    if (pl->nslot <= 0) return SFD_OK;
+    const int bs = 256;
+    hipLaunchKernelGGL(adjoint_inject_kernel, dim3((pl->nslot + bs - 1) / bs), dim3(bs), 0, pl->ctx->stream,
+                       p_grid - (size_t)pl->pad * pl->nz, pl->inj_cell.as<int>(), pl->inj_w.as<float>(),
+                       pl->res.as<float>(), pl->nslot, k);
+    SFD_HIP(pl->ctx, hipGetLastError());
+    return SFD_OK;
+}
+
+int grad_backward(GradShot* g) {
+    sfd_acoustic_plan* pl = g->pl;
+    sfd_context* ctx = pl->ctx;
+    hipStream_t s = ctx->stream;
+    const int T = pl->grad_period, nt = pl->nt;
+    const size_t n = (size_t)pl->nx * pl->nz;
+    for (int b = 0; b < 2; ++b)
+        for (Field* f : {&pl->ap[b], &pl->avx[b], &pl->avz[b]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+    SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
+    SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
+    StepKernel kdiv = pick_step_kernel(pl->radius, pl->nw, MODE_FWD_DIV);
+    StepKernel kadj = pick_step_kernel(pl->radius, pl->nw, MODE_ADJ);
+    if (!kdiv || !kadj) return fail(ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
+    int acur = 0;
+    int rc = launch_inject(pl, pl->grid(pl->ap[acur]), nt - 1);
+    if (rc) return rc;
+    for (int seg = g->nseg - 1; seg >= 0; --seg) {
+        const int k0 = seg * T, k1 = std::min(k0 + T, nt);
+        // Restore the forward state of step k0 and recompute the divergences.
+        pl->cur = 0;
+        if (seg == 0) {
+            for (Field* f : {&pl->p[0], &pl->vx[0], &pl->vz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+        } else {
+            const float* src = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
+            const Field* f[3] = {&pl->p[0], &pl->vx[0], &pl->vz[0]};
+            for (int q = 0; q < 3; ++q)
+                SFD_HIP(ctx, hipMemcpyAsync(pl->grid(*f[q]), src + q * n, n * 4, hipMemcpyDeviceToDevice, s));
+        }
+        for (int k = k0; k < k1; ++k) {
+            AcousticStepArgs a = make_args(pl, pl->cur, k);
+            hipLaunchKernelGGL(kdiv, dim3(a.ntz * a.nchunk), dim3(64 * pl->nw), 0, s, a);
+            SFD_HIP(ctx, hipGetLastError());
+            pl->cur ^= 1;
+        }
+        // Adjoint steps of the segment, in reverse.
+        for (int k = k1 - 1; k >= k0; --k) {
+            AcousticStepArgs a = make_args(pl, pl->ap, pl->avx, pl->avz, acur, k);
+            hipLaunchKernelGGL(kadj, dim3(a.ntz * a.nchunk), dim3(64 * pl->nw), 0, s, a);
+            SFD_HIP(ctx, hipGetLastError());
+            acur ^= 1;
+            if (k >= 1) {
+                rc = launch_inject(pl, pl->grid(pl->ap[acur]), k - 1);
+                if (rc) return rc;
+            }
+        }
+    }
+    return SFD_OK;
+}
+
+// dJ/dvp from the accumulator and the source adjoint; adds into grad_vp.
+int grad_finish(GradShot* g, const float* vp, const float* den, const float* wavelet, float* grad_vp) {
+    sfd_acoustic_plan* pl = g->pl;
+    sfd_context* ctx = pl->ctx;
+    const size_t n = (size_t)pl->nx * pl->nz;
+    std::vector<float> acc(n), sa(pl->nt);
+    SFD_HIP(ctx, hipMemcpyAsync(acc.data(), pl->grid(pl->grad), n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SFD_HIP(ctx, hipMemcpyAsync(sa.data(), pl->src_adj.ptr, (size_t)pl->nt * 4, hipMemcpyDeviceToHost, ctx->stream));
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t c = 0; c < n; ++c) grad_vp[c] += -2.0f * acc[c] / vp[c];
+    const size_t sc = (size_t)pl->sx * pl->nz + pl->sz;
+    double t = 0.0;
+    for (int k = 0; k < pl->nt; ++k) t += (double)sa[k] * wavelet[k];
+    (void)den;
+    grad_vp[sc] += (float)(-t / ((double)vp[sc] * vp[sc]));
+    return SFD_OK;
+}
+
+int grad_one(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den, const float* wavelet,
+             const float* seed, const float* d_obs, int period, float* grad_vp, float* record, double* misfit) {
+    int rc = check_gradient_params(ctx, p);
+    if (rc) return rc;
+    if (period < 1) return fail(ctx, SFD_EINVAL, "checkpoint interval must be >= 1");
+    GradShot g;
+    rc = grad_setup(ctx, p, vp, den, wavelet, std::min(period, p->nt), &g);
+    if (!rc) rc = grad_forward(&g);
+    std::vector<float> rec((size_t)p->nrec * p->nt), sd;
+    if (!rc) rc = sfd_acoustic_plan_fetch(g.pl, rec.data(), nullptr);
+    if (!rc) {
+        if (d_obs) {
+            sd.resize(rec.size());
+            double m = 0.0;
+            for (size_t q = 0; q < rec.size(); ++q) {
+                sd[q] = rec[q] - d_obs[q];
+                m += 0.5 * (double)sd[q] * sd[q];
+            }
+            if (misfit) *misfit += m;
+            seed = sd.data();
+        }
+        if (record) std::copy(rec.begin(), rec.end(), record);
+        rc = grad_upload_seed(&g, seed);
+    }
+    if (!rc) rc = grad_backward(&g);
+    if (!rc) rc = grad_finish(&g, vp, den, wavelet, grad_vp);
+    if (g.pl) sfd_acoustic_plan_destroy(g.pl);
+    return rc;
+}
+
+}  // namespace
+}  // namespace sfd
+
+extern "C" {
+
+int sfd_acoustic_gradient(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
+                          const float* wavelet, const float* seed, int ckpt_every, float* grad_vp, float* record) {
+    if (!ctx || !p || !vp || !wavelet || !seed || !grad_vp) return SFD_EINVAL;
+    std::fill(grad_vp, grad_vp + (size_t)p->nx * p->nz, 0.f);
+    return grad_one(ctx, p, vp, den, wavelet, seed, nullptr, ckpt_every, grad_vp, record, nullptr);
+}
+
+int sfd_acoustic_misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp,
+                                 const float* den, const float* const* wavelet, const float* const* d_obs,
+                                 int ckpt_every, float* grad_vp, double* misfit) {
+    if (!ctx || nshot < 1 || !params || !vp || !wavelet || !d_obs || !grad_vp) return SFD_EINVAL;
+    for (int s = 1; s < nshot; ++s)
+        if (params[s].nx != params[0].nx || params[s].nz != params[0].nz)
+            return fail(ctx, SFD_EINVAL, "shot %d: every shot of a gradient shares the model grid", s);
+    std::fill(grad_vp, grad_vp + (size_t)params[0].nx * params[0].nz, 0.f);
+    if (misfit) *misfit = 0.0;
+    for (int s = 0; s < nshot; ++s) {
+        const int rc = grad_one(ctx, &params[s], vp, den, wavelet[s], nullptr, d_obs[s], ckpt_every, grad_vp,
+                                nullptr, misfit);
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
+}  // extern "C"

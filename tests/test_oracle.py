@@ -176,3 +176,51 @@ def test_layered_301_fp32_vs_fp64():
     o64 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, precision="f64", **kw)
     rel = np.linalg.norm(o32["record"] - o64["record"]) / np.linalg.norm(o64["record"])
     assert rel < 1e-5, rel
+
+
+def _gradient_case(nx=46, nz=40, nt=140, radius=4, sponge=8, seed=0):
+    """Small heterogeneous shot for the adjoint pins: smooth vp and density,
+    source inside the updated cells, receivers on a row, one duplicated and
+    one in the non-updated outer ring (its seed must not count)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    vp = (2000 + 300 * gaussian_filter(rng.standard_normal((nx, nz)), 3)).astype(np.float32)
+    den = (2200 + 100 * gaussian_filter(rng.standard_normal((nx, nz)), 3)).astype(np.float32)
+    w = oracle.ricker(nt, np.float32(0.001), 30.0, 50.0)
+    rx = np.concatenate([np.arange(radius + 4, nx - radius - 4), [nx // 2, 1]]).astype(np.int32)
+    rz = np.concatenate([np.full(nx - 2 * radius - 8, radius + 6), [radius + 6, 1]]).astype(np.int32)
+    src = (nx // 2 + 1, radius + 5)
+    sd = rng.standard_normal((rx.size, nt))
+    return dict(vp=vp, den=den, wavelet=w, src=src, rec=(rx, rz), dt=0.001, radius=radius, sponge=sponge), sd
+
+
+def test_oracle_gradient_matches_finite_differences():
+    """Central differences of the FP64 forward along smooth directions and
+    along the source cell alone (the source amplitude dt rho vp w depends on
+    vp there); perturbations are the float32-rounded steps actually taken."""
+    from scipy.ndimage import gaussian_filter
+    case, sd = _gradient_case()
+    g, rec = oracle.gradient(**case, seed=sd)
+    fwd = oracle.acoustic(**case, precision="f64", nthreads=oracle_threads())
+    # Same scheme in double; the two codes order a few sums differently.
+    np.testing.assert_allclose(rec, fwd["record"], rtol=1e-9, atol=1e-9 * np.abs(rec).max())
+    vp = case["vp"]
+    rng = np.random.default_rng(7)
+
+    def J(v):
+        return float(np.sum(sd * oracle.acoustic(**{**case, "vp": v}, precision="f64",
+                                                  nthreads=oracle_threads())["record"]))
+
+    dirs = [gaussian_filter(rng.standard_normal(vp.shape), 3) for _ in range(2)]
+    e = np.zeros(vp.shape)
+    e[case["src"]] = 1.0
+    dirs.append(e)
+    for d in dirs:
+        vpp = (vp.astype(np.float64) + 1e-2 * d).astype(np.float32)
+        vpm = (vp.astype(np.float64) - 1e-2 * d).astype(np.float32)
+        step = (vpp.astype(np.float64) - vpm.astype(np.float64)) / 2
+        fd = (J(vpp) - J(vpm)) / 2
+        an = float(np.sum(g * step))
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    # The outer-ring receiver (last) never sees a nonzero pressure.
+    assert np.all(rec[-1] == 0)
