@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=600)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (a known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: shots are independent and this build measures one "
@@ -110,6 +113,16 @@ def main():
 
     import seismic_code_github_amd as sfd
     from seismic_code_github_amd import workloads as W
+
+    if args.copy_mib:
+        import torch
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_mib} MiB device to device")
+        del src, dst
 
     shot = W.smooth_random_2001x4001(nt=args.nt)
     nx, nz = shot.vp.shape
