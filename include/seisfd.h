@@ -198,6 +198,62 @@ int sfd_window_model(const float *model, int mnx, int mnz, int x0, int nxe, int 
  * Plans whose model exceeds it are rejected with SFD_ECFL. */
 double sfd_acoustic_cfl_limit(int radius, const float *coef);
 
+
+/* ---------------------------------------------------------------- visco-acoustic
+ * Rotated-staggered-grid visco-acoustic propagator: the drop-in for the time
+ * loop of forward/Q_forward, single_shot_modelling
+ * (Q_multishot_fd_main_fsbc_zy_rotated.f90:169-484, step
+ * extrapolation_one_step_10nd_absbc, Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:695-839)
+ * with vs = 0, where the reference's strxx and strzz are one field sigma and
+ * strxz stays zero.  L standard linear solids (the reference: L = 1) fitted by
+ * the tau-method of pcalpas (Q sub:473-551).  Grid = the reference's padded
+ * shot grid nxea x nzea (get_currshot_parameters, or sfd_window_model),
+ * [nx][nz] z fastest, 0-based. */
+typedef struct sfd_visco_plan sfd_visco_plan;
+
+typedef struct sfd_visco_params {
+    int nx, nz;            /* padded shot grid (reference nxea, nzea) */
+    int nt;                /* recorded samples (reference nt) */
+    int delay;             /* steps run before the first recorded one (reference
+                              decay, main:279-280); < 0 = nint(1.2 / (f0 dt)) */
+    float dx, dz, dt;
+    float f0;              /* Hz: the built-in wavelet and the tau-method band */
+    int radius;            /* 5 (a100..a104, the reference) or 4 (a80..a83) */
+    const float *coef;     /* a_0..a_{R-1}; NULL = the reference's table */
+    int nmech;             /* L = 1 (reference), 2 or 3 */
+    float q_scale;         /* Q multiplier before the fit; 0 = the reference's 0.9 (Q sub:525-526) */
+    float f_low, f_high;   /* tau-method band in Hz; <= 0 = the reference's 2 Hz and 2.5 f0 */
+    int sponge_xlo, sponge_xhi, sponge_zlo, sponge_zhi;  /* reference npxn, npxp, npzn, npzp:
+                              strips of n+1 cells damped by cos(acos(R) d / n) (csponge, Q sub:555-605) */
+    float sponge_r;        /* reference R (module constant 1e-3); 0 = 1e-3 */
+    int sx, sz;            /* source cell (reference isxe - 1, isze - 1) */
+    float src_sigma;       /* Gaussian source width in cells; 0 = the reference's 1.5 (addsrc) */
+    int nrec;
+    const int *rx, *rz;    /* receiver cells (reference: row itopo + nrz0 of every window column) */
+    int nsnap;
+    const int *snap_steps; /* ascending step indices in [0, delay + nt) after which fields are copied */
+    int snap_fields;       /* mask of enum sfd_field: P = sigma (= strxx = strzz), VX, VZ */
+} sfd_visco_params;
+
+/* vp, qp [nx*nz] required; rho NULL = 2400 kg/m^3 everywhere (averaged over 4
+ * cells as substagger does); vs must be NULL or all zero (visco-elastic is
+ * SFD_ENOTSUP); wavelet [delay + nt] or NULL = addsrc's Ricker
+ * exp(-(pi f0 (it dt - 1.2/f0))^2)(1 - 2 (pi f0 (it dt - 1.2/f0))^2), it = 1..
+ *   record [nrec][nt]: sigma at the receivers after steps delay .. delay+nt-1
+ *          (reference record1 = strzz = record2 = strxx, main:450-459)
+ *   snaps  [nsnap][popcount(snap_fields)][nx*nz], fields in sigma, vx, vz order */
+int sfd_visco_forward(sfd_context *ctx, const sfd_visco_params *p, const float *vp, const float *vs,
+                      const float *rho, const float *qp, const float *wavelet, float *record, float *snaps);
+int sfd_visco_plan_create(sfd_context *ctx, const sfd_visco_params *p, const float *vp, const float *vs,
+                          const float *rho, const float *qp, const float *wavelet, sfd_visco_plan **plan);
+/* Zeroes the fields and runs all delay + nt steps; elapsed_ms (may be NULL)
+ * receives the device time measured with HIP events on the plan's stream. */
+int sfd_visco_plan_run(sfd_visco_plan *plan, float *elapsed_ms);
+int sfd_visco_plan_fetch(sfd_visco_plan *plan, float *record, float *snaps);
+/* Waves per workgroup (2 or 4) and rows marched per workgroup; 0 = automatic. */
+int sfd_visco_plan_set_tiling(sfd_visco_plan *plan, int waves, int chunk_rows);
+int sfd_visco_plan_destroy(sfd_visco_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -215,3 +215,87 @@ def reference_config():
     rx = np.full_like(rz, sx)
     return dict(vp=vp, den=den, wavelet=w, src=(sx, sz), rec=(rx, rz), dx=10.0, dz=10.0,
                 dt=float(dt), radius=8, sponge=PML, nt=nt)
+
+
+class _VArgs(ctypes.Structure):
+    """Mirror of ``oracle_visco_args`` (oracle.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nsteps", ctypes.c_int), ("it0", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float), ("f0", ctypes.c_float),
+        ("radius", ctypes.c_int), ("coef", ctypes.POINTER(ctypes.c_float)),
+        ("nmech", ctypes.c_int), ("q_scale", ctypes.c_float),
+        ("f_low", ctypes.c_float), ("f_high", ctypes.c_float),
+        ("npxn", ctypes.c_int), ("npxp", ctypes.c_int), ("npzn", ctypes.c_int), ("npzp", ctypes.c_int),
+        ("sponge_r", ctypes.c_float),
+        ("vp", ctypes.POINTER(ctypes.c_float)), ("vs", ctypes.POINTER(ctypes.c_float)),
+        ("rho", ctypes.POINTER(ctypes.c_float)), ("qp", ctypes.POINTER(ctypes.c_float)),
+        ("qs", ctypes.POINTER(ctypes.c_float)),
+        ("isx", ctypes.c_int), ("isz", ctypes.c_int), ("src_sigma", ctypes.c_float),
+        ("wavelet", ctypes.POINTER(ctypes.c_float)),
+        ("nrec", ctypes.c_int), ("rx", ctypes.POINTER(ctypes.c_int)), ("rz", ctypes.POINTER(ctypes.c_int)),
+        ("rec_delay", ctypes.c_int),
+        ("nsnap", ctypes.c_int), ("snap_steps", ctypes.POINTER(ctypes.c_int)),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+def _bind_visco(L):
+    fp = ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.oracle_visco_f32.argtypes = [ctypes.POINTER(_VArgs), fp, fp, fp]
+    L.oracle_visco_f64.argtypes = [ctypes.POINTER(_VArgs), dp, dp, dp]
+    d = ctypes.c_double
+    L.oracle_visco_tau_f32.argtypes = [ctypes.c_int, d, d, d, d, d, fp, fp]
+    L.oracle_visco_tau_f64.argtypes = [ctypes.c_int, d, d, d, d, d, dp, dp]
+
+
+def visco_tau(nmech, f0, q, q_scale=0.9, f_low=0.0, f_high=0.0, precision="f32"):
+    """(tau_sigma [L], tau) of the tau-method (pcalpas, Q sub:506-542)."""
+    L = lib()
+    if not hasattr(L, "_visco_bound"):
+        _bind_visco(L)
+        L._visco_bound = True
+    dtype, ct = (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
+    ts = np.zeros(nmech, dtype=dtype)
+    tau = ct()
+    fn = L.oracle_visco_tau_f32 if precision == "f32" else L.oracle_visco_tau_f64
+    fn(nmech, f0, q, q_scale, f_low, f_high, _ptr(ts, ct), ctypes.byref(tau))
+    return ts, tau.value
+
+
+def visco(vp, rho, qp, *, src, rec, nsteps, dx=10.0, dz=10.0, dt=1e-3, f0=20.0, vs=None, qs=None,
+          radius=5, coef=None, nmech=1, q_scale=0.9, f_low=0.0, f_high=0.0, sponge=(40, 40, 40, 40),
+          sponge_r=1e-3, src_sigma=1.5, wavelet=None, rec_delay=0, snap_steps=(), it0=1,
+          precision="f32", nthreads=1):
+    """Run the visco-elastic RSG oracle (oracle/visco_body.h).  Fields
+    [nx, nz] float32 (z fastest); sponge = (npxn, npxp, npzn, npzp).
+    Returns dict(rec_zz, rec_xx [nrec, nsteps - rec_delay],
+    snaps [nsnap, 5, nx, nz] = sxx, szz, sxz, vx, vz)."""
+    L = lib()
+    if not hasattr(L, "_visco_bound"):
+        _bind_visco(L)
+        L._visco_bound = True
+    f32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    vp, rho, qp, vs, qs = f32(vp), f32(rho), f32(qp), f32(vs), f32(qs)
+    nx, nz = vp.shape
+    coef = coefs_for_radius(radius) if coef is None else f32(coef)
+    rx = np.ascontiguousarray(rec[0], dtype=np.int32)
+    rz = np.ascontiguousarray(rec[1], dtype=np.int32)
+    steps = np.ascontiguousarray(sorted(snap_steps), dtype=np.int32)
+    w = f32(wavelet)
+    if w is not None:
+        assert w.shape[0] >= nsteps
+    args = _VArgs(nx, nz, nsteps, it0, dx, dz, dt, f0, radius, _ptr(coef), nmech, q_scale, f_low, f_high,
+                  *sponge, sponge_r, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(qp), _ptr(qs),
+                  int(src[0]), int(src[1]), src_sigma, _ptr(w), rx.size, _ptr(rx, ctypes.c_int),
+                  _ptr(rz, ctypes.c_int), rec_delay, steps.size, _ptr(steps, ctypes.c_int), nthreads)
+    dtype, ct = (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
+    nrt = max(nsteps - rec_delay, 0)
+    rzz = np.zeros((rx.size, nrt), dtype=dtype)
+    rxx = np.zeros((rx.size, nrt), dtype=dtype)
+    snaps = np.zeros((steps.size, 5, nx, nz), dtype=dtype)
+    fn = L.oracle_visco_f32 if precision == "f32" else L.oracle_visco_f64
+    rc = fn(ctypes.byref(args), _ptr(rzz, ct), _ptr(rxx, ct), _ptr(snaps, ct) if steps.size else ctypes.POINTER(ct)())
+    if rc != 0:
+        raise ValueError(f"oracle_visco returned {rc}")
+    return {"rec_zz": rzz, "rec_xx": rxx, "snaps": snaps}

@@ -93,6 +93,50 @@ int oracle_acoustic_gradient_f64(const oracle_acoustic_args *a, const double *se
 void oracle_window(const float *model, int nx, int nz, int ixmine, int nxe, int npxn, int npxp,
                    int npzn, int npzp, float *out);
 
+
+/* Visco-elastic rotated-staggered-grid propagator, forward/Q_forward
+ * (Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:434-1444 and the time loop
+ * of Q_multishot_fd_main_fsbc_zy_rotated.f90:436-459), with L standard
+ * linear solids (the reference: L = 1).  Grid = the reference's padded shot
+ * grid (nxea x nzea); indices 0-based. */
+#define ORACLE_MAX_MECH 8
+typedef struct oracle_visco_args {
+    int nx, nz;              /* padded grid: reference nxea (x), nzea (z) */
+    int nsteps;              /* steps run (reference nt + decay) */
+    int it0;                 /* Fortran index of the first step (1 = fresh run) */
+    float dx, dz, dt, f0;
+    int radius;              /* 5 = a100..a104 (reference), 1..5 */
+    const float *coef;       /* a_0..a_{R-1} */
+    int nmech;               /* L */
+    float q_scale;           /* Q multiplier before the fit (reference 18/20) */
+    float f_low, f_high;     /* fit band in Hz; <= 0 = reference (2 Hz, 2.5 f0) */
+    int npxn, npxp, npzn, npzp;  /* sponge table lengths - 1 (reference np*) */
+    float sponge_r;          /* reference R = 1e-3 */
+    const float *vp, *vs, *rho, *qp, *qs;  /* [nx*nz]; vs, qs may be NULL (0, qp) */
+    int isx, isz;            /* source cell (reference isxe-1, isze-1) */
+    float src_sigma;         /* Gaussian source width in cells (reference 1.5) */
+    const float *wavelet;    /* [nsteps] or NULL = the reference's formula */
+    int nrec;
+    const int *rx, *rz;      /* receiver cells */
+    int rec_delay;           /* steps before recording (reference decay) */
+    int nsnap;
+    const int *snap_steps;   /* step indices s (0-based) after which fields are copied */
+    int nthreads;
+} oracle_visco_args;
+
+/* rec_zz, rec_xx [nrec][nsteps - rec_delay] (reference record1 = strzz,
+ * record2 = strxx); snaps [nsnap][5][nx*nz]: sxx, szz, sxz, vx, vz.  Any
+ * output may be NULL.  Returns 0, or a negative value on invalid arguments. */
+int oracle_visco_f32(const oracle_visco_args *a, float *rec_zz, float *rec_xx, float *snaps);
+int oracle_visco_f64(const oracle_visco_args *a, double *rec_zz, double *rec_xx, double *snaps);
+
+/* Relaxation times tau_sigma[L] and the tau-method's tau for quality factor q
+ * (pcalpas, Q sub:506-542, generalised to L mechanisms). */
+void oracle_visco_tau_f32(int L, double f0, double q, double q_scale, double f_low, double f_high,
+                          float *tau_sigma, float *tau);
+void oracle_visco_tau_f64(int L, double f0, double q, double q_scale, double f_low, double f_high,
+                          double *tau_sigma, double *tau);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,21 @@ class AcousticParamsC(ctypes.Structure):
     ]
 
 
+class ViscoParamsC(ctypes.Structure):
+    """Mirror of ``struct sfd_visco_params``."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nt", ctypes.c_int), ("delay", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float), ("f0", ctypes.c_float),
+        ("radius", ctypes.c_int), ("coef", _FP), ("nmech", ctypes.c_int), ("q_scale", ctypes.c_float),
+        ("f_low", ctypes.c_float), ("f_high", ctypes.c_float),
+        ("sponge_xlo", ctypes.c_int), ("sponge_xhi", ctypes.c_int),
+        ("sponge_zlo", ctypes.c_int), ("sponge_zhi", ctypes.c_int), ("sponge_r", ctypes.c_float),
+        ("sx", ctypes.c_int), ("sz", ctypes.c_int), ("src_sigma", ctypes.c_float),
+        ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
+        ("nsnap", ctypes.c_int), ("snap_steps", _IP), ("snap_fields", ctypes.c_int),
+    ]
+
+
 # Every symbol declared in include/seisfd.h, with its ctypes signature.
 _SIGNATURES = {
     "sfd_abi_version": (ctypes.c_int, []),
@@ -94,6 +109,14 @@ _SIGNATURES = {
     "sfd_acoustic_misfit_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(AcousticParamsC),
                                                     _FP, _FP, ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.c_int,
                                                     _FP, ctypes.POINTER(ctypes.c_double)]),
+    "sfd_visco_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViscoParamsC), _FP, _FP, _FP, _FP, _FP,
+                                         _FP, _FP]),
+    "sfd_visco_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViscoParamsC), _FP, _FP, _FP, _FP, _FP,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_visco_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_visco_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
+    "sfd_visco_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_window_model": (ctypes.c_int, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP]),
 }
@@ -423,3 +446,137 @@ class AcousticBatch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- visco-acoustic
+
+@dataclass
+class ViscoShot:
+    """One visco-acoustic shot on the reference's padded shot grid
+    (forward/Q_forward single_shot_modelling, vs = 0).
+
+    vp, qp, rho: [nx, nz] float32, z fastest (rho None = 2400).  src = (sx, sz);
+    rec = (rx[], rz[]).  sponge = (npxn, npxp, npzn, npzp).  nt = recorded
+    samples; delay = steps before the first (None = the reference's
+    nint(1.2 / (f0 dt))).  wavelet None = the reference's built-in Ricker."""
+    vp: np.ndarray
+    qp: np.ndarray
+    src: tuple
+    rec: tuple
+    nt: int
+    dx: float = 10.0
+    dz: float = 10.0
+    dt: float = 1e-3
+    f0: float = 20.0
+    rho: Optional[np.ndarray] = None
+    vs: Optional[np.ndarray] = None
+    radius: int = 5
+    coef: Optional[np.ndarray] = None
+    nmech: int = 1
+    q_scale: float = 0.9
+    f_low: float = 0.0
+    f_high: float = 0.0
+    sponge: tuple = (40, 40, 40, 40)
+    sponge_r: float = 1e-3
+    src_sigma: float = 1.5
+    delay: Optional[int] = None
+    wavelet: Optional[np.ndarray] = None
+    snap_steps: Sequence[int] = ()
+    snap_fields: int = FIELD_P
+    _keep: list = field(default_factory=list, repr=False)
+
+    def params(self) -> ViscoParamsC:
+        nx, nz = np.asarray(self.vp).shape
+        rx = np.ascontiguousarray(self.rec[0], dtype=np.int32)
+        rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
+        steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
+        coef = None if self.coef is None else _f32(self.coef)
+        self._keep = [rx, rz, steps, coef]
+        return ViscoParamsC(nx, nz, int(self.nt), -1 if self.delay is None else int(self.delay),
+                            self.dx, self.dz, self.dt, self.f0, self.radius, _ptr(coef), self.nmech,
+                            self.q_scale, self.f_low, self.f_high, *[int(v) for v in self.sponge], self.sponge_r,
+                            int(self.src[0]), int(self.src[1]), self.src_sigma, rx.size, _ptr(rx, ctypes.c_int),
+                            _ptr(rz, ctypes.c_int), steps.size, _ptr(steps, ctypes.c_int), self.snap_fields)
+
+    def nfields(self) -> int:
+        return bin(self.snap_fields & 7).count("1")
+
+    def ref_delay(self) -> int:
+        """The reference's decay = nint(1.2 / (f0 dt)) (main:279-280), in float."""
+        if self.delay is not None:
+            return int(self.delay)
+        v = np.float32(1.2) / (np.float32(self.f0) * np.float32(self.dt))
+        return int(np.floor(float(v) + 0.5))
+
+    def _arrays(self):
+        vp = _f32(self.vp)
+        qp = _f32(self.qp)
+        rho = None if self.rho is None else _f32(self.rho)
+        vs = None if self.vs is None else _f32(self.vs)
+        w = None if self.wavelet is None else _f32(self.wavelet)
+        return vp, vs, rho, qp, w
+
+
+class ViscoPlan:
+    """``sfd_visco_plan``: model resident on the device."""
+
+    def __init__(self, shot: ViscoShot, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.shot = shot
+        self.p = shot.params()
+        self.nx, self.nz = np.asarray(shot.vp).shape
+        self._arr = shot._arrays()
+        vp, vs, rho, qp, w = self._arr
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_visco_plan_create(self.ctx.handle, ctypes.byref(self.p), _ptr(vp), _ptr(vs),
+                                                   _ptr(rho), _ptr(qp), _ptr(w), ctypes.byref(h)),
+                       "sfd_visco_plan_create")
+        self.handle = h
+
+    def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
+        self.ctx.check(lib().sfd_visco_plan_set_tiling(self.handle, waves, chunk_rows), "visco set_tiling")
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_visco_plan_run(self.handle, ctypes.byref(ms)), "sfd_visco_plan_run")
+        return ms.value
+
+    def fetch(self):
+        nrec = self.p.nrec
+        record = np.zeros((nrec, self.p.nt), dtype=np.float32)
+        snaps = np.zeros((self.p.nsnap, self.shot.nfields(), self.nx, self.nz), dtype=np.float32)
+        self.ctx.check(lib().sfd_visco_plan_fetch(self.handle, _ptr(record) if nrec and self.p.nt else None,
+                                                  _ptr(snaps) if self.p.nsnap else None), "sfd_visco_plan_fetch")
+        return record, snaps
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_visco_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def visco_forward(shot: ViscoShot, ctx: Optional[Context] = None):
+    """One visco-acoustic shot through ``sfd_visco_forward``.
+    Returns (record [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
+    ctx = ctx or default_context()
+    p = shot.params()
+    nx, nz = np.asarray(shot.vp).shape
+    vp, vs, rho, qp, w = shot._arrays()
+    record = np.zeros((p.nrec, p.nt), dtype=np.float32)
+    snaps = np.zeros((p.nsnap, shot.nfields(), nx, nz), dtype=np.float32)
+    ctx.check(lib().sfd_visco_forward(ctx.handle, ctypes.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(qp), _ptr(w),
+                                      _ptr(record) if p.nrec and p.nt else None, _ptr(snaps) if p.nsnap else None),
+              "sfd_visco_forward")
+    return record, snaps

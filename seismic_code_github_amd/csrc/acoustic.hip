@@ -421,45 +421,9 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
     pl->nrec = p->nrec;
     pl->pad = pad_rows(p->radius);
     const int pad = pl->pad;
-    std::vector<int> hslot(p->nrec);
-    std::vector<int4> hrow(nx + 2 * pad, make_int4(0, 0, 0, -1)), hmore;
-    {
-        std::vector<std::tuple<int, int, int>> cells(p->nrec);
-        for (int r = 0; r < p->nrec; ++r) cells[r] = {p->rx[r], p->rz[r], r};
-        std::sort(cells.begin(), cells.end());
-        std::vector<int4> spans;  // (jlo, jhi, slot, row)
-        int slot = -1, prev_x = -1, prev_z = -1;
-        for (auto& [x, z, r] : cells) {
-            if (x != prev_x || z != prev_z) {
-                ++slot;
-                if (!spans.empty() && x == prev_x && z == prev_z + 1) {
-                    spans.back().y = z + 1;
-                } else {
-                    spans.push_back(make_int4(z, z + 1, slot, x));
-                }
-                prev_x = x;
-                prev_z = z;
-            }
-            hslot[r] = slot;
-        }
-        pl->nslot = slot + 1;
-        // tail[row]: -2 = no span yet, -1 = head only, else last index in hmore.
-        std::vector<int> tail(nx, -2);
-        for (const int4& s4 : spans) {
-            const int row = s4.w;
-            const int4 sp = make_int4(s4.x, s4.y, s4.z, -1);
-            if (tail[row] == -2) {
-                hrow[row + pad] = sp;
-                tail[row] = -1;
-            } else {
-                hmore.push_back(sp);
-                const int idx = (int)hmore.size() - 1;
-                if (tail[row] == -1) hrow[row + pad].w = idx;
-                else hmore[tail[row]].w = idx;
-                tail[row] = idx;
-            }
-        }
-    }
+    std::vector<int> hslot;
+    std::vector<int4> hrow, hmore;
+    receiver_spans(nx, pad, p->nrec, p->rx, p->rz, hslot, hrow, hmore, &pl->nslot);
     // Absorbing boundary tables.
     std::vector<float> hax, haz, hpx, hpz;
     sponge_profiles(nx, nz, p->boundary == SFD_BOUNDARY_SPONGE ? p->width : 0, hax, haz);

@@ -15,15 +15,6 @@ BatchKernel pick_batch_kernel(int radius, int nw, int mode);
 int pad_rows(int radius);
 int round_chunk(int ch, int radius);
 
-template <class T>
-hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
-    hipError_t e = b.alloc(n * sizeof(T));
-    if (e != hipSuccess || n == 0) return e;
-    e = hipMemcpyAsync(b.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
-}
-
 }  // namespace sfd
 
 struct sfd_acoustic_plan {

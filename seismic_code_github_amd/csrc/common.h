@@ -79,9 +79,26 @@ struct Carver {
     }
 };
 
+// Allocates `b` for n elements and copies them from the host (synchronous).
+template <class T>
+hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
+    hipError_t e = b.alloc(n * sizeof(T));
+    if (e != hipSuccess || n == 0) return e;
+    e = hipMemcpyAsync(b.ptr, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
 // Staggered Taylor coefficients used when the caller passes none: the
 // reference's own tables (Acoustic_1order_FD.cpp:38-45 for R = 8;
 // Iso_multishot_fd_module_zy_m2.f90:17-26 for R = 4 and 5).
 bool default_coefs(int radius, float* c);
+
+// Receiver table of the step kernels.  One slot per distinct receiver cell
+// (slot_of[r] for receiver r); runs of consecutive z cells in a row become
+// spans (jlo, jhi, slot).  row[x + pad] holds row x's first span, further
+// spans of the row go to `more`, chained by their 4th field (-1 ends).
+void receiver_spans(int nx, int pad, int nrec, const int* rx, const int* rz, std::vector<int>& slot_of,
+                    std::vector<int4>& row, std::vector<int4>& more, int* nslot);
 
 }  // namespace sfd

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <new>
+#include <tuple>
 
 #include "common.h"
 
@@ -49,6 +50,49 @@ bool default_coefs(int radius, float* c) {
     }
     default:
         return false;
+    }
+}
+
+
+void receiver_spans(int nx, int pad, int nrec, const int* rx, const int* rz, std::vector<int>& slot_of,
+                    std::vector<int4>& row, std::vector<int4>& more, int* nslot) {
+    slot_of.assign(nrec, 0);
+    row.assign((size_t)nx + 2 * pad, make_int4(0, 0, 0, -1));
+    more.clear();
+    std::vector<std::tuple<int, int, int>> cells(nrec);
+    for (int r = 0; r < nrec; ++r) cells[r] = {rx[r], rz[r], r};
+    std::sort(cells.begin(), cells.end());
+    std::vector<int4> spans;  // (jlo, jhi, slot, row)
+    int slot = -1, prev_x = -1, prev_z = -1;
+    for (auto& [x, z, r] : cells) {
+        if (x != prev_x || z != prev_z) {
+            ++slot;
+            if (!spans.empty() && x == prev_x && z == prev_z + 1) {
+                spans.back().y = z + 1;
+            } else {
+                spans.push_back(make_int4(z, z + 1, slot, x));
+            }
+            prev_x = x;
+            prev_z = z;
+        }
+        slot_of[r] = slot;
+    }
+    *nslot = slot + 1;
+    // tail[row]: -2 = no span yet, -1 = head only, else last index in `more`.
+    std::vector<int> tail(nx, -2);
+    for (const int4& s4 : spans) {
+        const int x = s4.w;
+        const int4 sp = make_int4(s4.x, s4.y, s4.z, -1);
+        if (tail[x] == -2) {
+            row[x + pad] = sp;
+            tail[x] = -1;
+        } else {
+            more.push_back(sp);
+            const int idx = (int)more.size() - 1;
+            if (tail[x] == -1) row[x + pad].w = idx;
+            else more[tail[x]].w = idx;
+            tail[x] = idx;
+        }
     }
 }
 

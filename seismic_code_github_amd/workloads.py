@@ -12,6 +12,14 @@ geometries:
   dt=1 ms, nt=2000, three layers 1500/2500/3500 m/s with interfaces at interior
   z=100 and z=200, constant density 2400, 25 Hz Ricker at interior (z=2,
   x=150), 301 receivers at interior z=2; radius 4 (8th order) by default.
+* ``visco_1001x2001`` — visco-acoustic (vs = 0): 1001 (z) x 2001 (x) model
+  + 40-cell cosine sponge (the reference's npxn = npxp = npzn = npzp = 40, so
+  1081 x 2081 stored), dx = dz = 10 m, dt = 1 ms, nt = 4000 recorded samples
+  (+ the reference's 60-step delay), vp = seeded smooth noise (seed 2,
+  sigma = 20 cells) mapped to 1500-4000 m/s, Q = seeded smooth noise (seed 3)
+  mapped to 20-200, density 2400, 3 relaxation mechanisms, 20 Hz source
+  (the reference's built-in wavelet and 1.5-cell Gaussian footprint) at
+  model (z=2, x=1000), 2001 receivers at model z=2, R = 5 (a100..a104).
 * ``smooth_random_2001x4001`` — 2001 (z) x 4001 (x) interior + 50-cell C-PML
   (4101 x 2101 stored), dx=dz=10 m, dt=0.8 ms, nt=6000, velocity = seeded
   Gaussian-smoothed noise (sigma = 20 cells) mapped to 1500-4500 m/s, 20 Hz
@@ -25,7 +33,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import AcousticShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model
+from . import AcousticShot, ViscoShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model
 
 PI_REF = 3.1415926
 
@@ -139,3 +147,21 @@ def cell_updates(shot: AcousticShot) -> int:
     nx, nz = np.asarray(shot.vp).shape
     nt = shot.nt if shot.nt is not None else np.asarray(shot.wavelet).shape[0]
     return nx * nz * nt
+
+
+def visco_1001x2001(nt: int = 4000, nmech: int = 3, scale: int = 1, vp_seed: int = 2, q_seed: int = 3,
+                    **overrides) -> ViscoShot:
+    """``scale`` > 1 shrinks the model by that factor (same physics per cell)
+    for parity tests at oracle-friendly sizes; the sponge keeps 40 cells unless
+    overridden."""
+    nb = overrides.pop("nb", 40)
+    nzi, nxi = 1000 // scale + 1, 2000 // scale + 1
+    nx, nz = nxi + 2 * nb, nzi + 2 * nb
+    vp = smooth_random_model(nx, nz, 20.0 / scale, 1500.0, 4000.0, vp_seed)
+    qp = smooth_random_model(nx, nz, 20.0 / scale, 20.0, 200.0, q_seed)
+    rx = np.arange(nb, nb + nxi, dtype=np.int32)
+    rz = np.full_like(rx, nb + 2)
+    kw = dict(vp=vp, qp=qp, rho=None, src=(nb + nxi // 2, nb + 2), rec=(rx, rz), nt=nt, dx=10.0, dz=10.0,
+              dt=0.001, f0=20.0, radius=5, nmech=nmech, sponge=(nb, nb, nb, nb))
+    kw.update(overrides)
+    return ViscoShot(**kw)
