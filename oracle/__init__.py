@@ -247,6 +247,50 @@ def _bind_visco(L):
     d = ctypes.c_double
     L.oracle_visco_tau_f32.argtypes = [ctypes.c_int, d, d, d, d, d, fp, fp]
     L.oracle_visco_tau_f64.argtypes = [ctypes.c_int, d, d, d, d, d, dp, dp]
+    L.oracle_rsg_deriv_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, d,
+                                       ctypes.c_int, fp]
+    L.oracle_csponge_f32.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, fp, fp]
+    L.oracle_visco_source_weights_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_float, fp]
+
+
+def _vlib():
+    L = lib()
+    if not hasattr(L, "_visco_bound"):
+        _bind_visco(L)
+        L._visco_bound = True
+    return L
+
+
+def rsg_deriv(u, direction, shift, h=10.0, radius=5, coef=None):
+    """One rotated-grid derivative of the visco restatement (FP64):
+    direction "x" (calderxabs) or "z" (calderzabs), xflag/zflag = shift."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nx, nz = u.shape
+    coef = coefs_for_radius(radius) if coef is None else np.ascontiguousarray(coef, dtype=np.float32)
+    out = np.zeros_like(u)
+    rc = _vlib().oracle_rsg_deriv_f64({"x": 0, "z": 1}[direction], shift, nx, nz, _ptr(u, ctypes.c_double),
+                                      _ptr(out, ctypes.c_double), h, radius, _ptr(coef))
+    if rc != 0:
+        raise ValueError(f"oracle_rsg_deriv_f64 returned {rc}")
+    return out
+
+
+def csponge(np_, h, r=1e-3, lo=True):
+    """(node, half) tables of one csponge side, outermost cell first."""
+    node = np.zeros(np_ + 1, dtype=np.float32)
+    half = np.zeros(np_ + 1, dtype=np.float32)
+    _vlib().oracle_csponge_f32(np_, h, r, 1 if lo else 0, _ptr(node), _ptr(half))
+    return node, half
+
+
+def visco_source_weights(nx, nz, isx, isz, sigma=1.5):
+    """Per-cell weight of addsrc's Gaussian footprint [nx, nz]."""
+    w = np.zeros((nx, nz), dtype=np.float32)
+    rc = _vlib().oracle_visco_source_weights_f32(nx, nz, isx, isz, sigma, _ptr(w))
+    if rc != 0:
+        raise ValueError(rc)
+    return w
 
 
 def visco_tau(nmech, f0, q, q_scale=0.9, f_low=0.0, f_high=0.0, precision="f32"):

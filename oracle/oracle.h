@@ -137,6 +137,15 @@ void oracle_visco_tau_f32(int L, double f0, double q, double q_scale, double f_l
 void oracle_visco_tau_f64(int L, double f0, double q, double q_scale, double f_low, double f_high,
                           double *tau_sigma, double *tau);
 
+/* Single routines of the visco restatement, for the oracle's own pins:
+ * one rotated-grid derivative (calderxabs dir 0 / calderzabs dir 1, xflag =
+ * shift) of u [nx][nz] (zero outside), FP64; one side of csponge; and the
+ * per-cell weight of addsrc's Gaussian footprint. */
+int oracle_rsg_deriv_f64(int dir, int shift, int nx, int nz, const double *u, double *der, double h, int radius,
+                         const float *coef);
+void oracle_csponge_f32(int np, float h, float r, int lo, float *node, float *half);
+int oracle_visco_source_weights_f32(int nx, int nz, int isx, int isz, float sigma, float *w);
+
 #ifdef __cplusplus
 }
 #endif

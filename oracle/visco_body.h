@@ -110,6 +110,34 @@ static void FN(csponge_side)(int np, REAL h, REAL Rc, REAL *spg, REAL *spgh, int
 #undef COS_
 }
 
+/* get_gauss_filter_coe (Q sub:1398-1444): g2c(i,j), i = 1..n1, j = 1..n2,
+ * column-major, 1/(2 pi sigma^2) exp(-dis / (2 sigma^2)) with dis the squared
+ * index distance to ((n1+1)/2, (n2+1)/2) (integer division), normalised by
+ * its sum accumulated j-outer, i-inner. */
+static void FN(gauss_filter)(int n1, int n2, REAL sigma, REAL *g2c)
+{
+#if ORACLE_F32
+    const float PI = 3.1415926f;
+#else
+    const double PI = 3.1415926;
+#endif
+    const REAL sigma2 = sigma * sigma;
+    const int ixcen = (n1 + 1) / 2, izcen = (n2 + 1) / 2;
+    REAL sum = 0;
+    for (int j = 1; j <= n2; ++j)
+        for (int i = 1; i <= n1; ++i) {
+            const REAL dis = (REAL)(abs(j - izcen) * abs(j - izcen) + abs(i - ixcen) * abs(i - ixcen));
+#if ORACLE_F32
+            g2c[(size_t)(j - 1) * n1 + (i - 1)] = 1 / (2 * PI * sigma2) * expf(-dis / (2 * sigma2));
+#else
+            g2c[(size_t)(j - 1) * n1 + (i - 1)] = 1 / (2 * PI * sigma2) * exp(-dis / (2 * sigma2));
+#endif
+        }
+    for (int j = 1; j <= n2; ++j)
+        for (int i = 1; i <= n1; ++i) sum = sum + g2c[(size_t)(j - 1) * n1 + (i - 1)];
+    for (size_t c = 0; c < (size_t)n1 * n2; ++c) g2c[c] = g2c[c] / sum;
+}
+
 /* pcalpas' relaxation times (:506-542).  One mechanism (L = 1), as written:
  * tau_sigma = 1 / ((w_up - w_low) / 2) and Blanch's closed form
  *   tau = (1/Q') (ln(1 + w^2 ts^2)|) / ((atan(w ts) - w ts / (1 + w^2 ts^2))|),
@@ -276,23 +304,7 @@ int FN(oracle_visco)(const oracle_visco_args *a, REAL *rec_zz, REAL *rec_xx, REA
     const int n1 = nz, n2 = nx;
     g2c = malloc(n * sizeof(REAL));
     if (!g2c) goto done;
-    {
-        const REAL sigma = (REAL)a->src_sigma, sigma2 = sigma * sigma;
-        const int ixcen = (n1 + 1) / 2, izcen = (n2 + 1) / 2;
-        REAL sum = 0;
-        for (int j = 1; j <= n2; ++j)
-            for (int i = 1; i <= n1; ++i) {
-                const REAL dis = (REAL)(abs(j - izcen) * abs(j - izcen) + abs(i - ixcen) * abs(i - ixcen));
-#if ORACLE_F32
-                g2c[(size_t)(j - 1) * n1 + (i - 1)] = 1 / (2 * PI * sigma2) * expf(-dis / (2 * sigma2));
-#else
-                g2c[(size_t)(j - 1) * n1 + (i - 1)] = 1 / (2 * PI * sigma2) * exp(-dis / (2 * sigma2));
-#endif
-            }
-        for (int j = 1; j <= n2; ++j)
-            for (int i = 1; i <= n1; ++i) sum = sum + g2c[(size_t)(j - 1) * n1 + (i - 1)];
-        for (size_t c = 0; c < n; ++c) g2c[c] = g2c[c] / sum;
-    }
+    FN(gauss_filter)(n1, n2, (REAL)a->src_sigma, g2c);
     /* addsrc's placement (:1360-1389): hw = nint((n+1)/2.0); the grid column
      * n2 index idx2 = n - (isxe - (hw2 - 1)) + 1, idx1 likewise in z. */
     const int hw1 = (int)lround((n1 + 1) / 2.0), hw2 = (int)lround((n2 + 1) / 2.0);
