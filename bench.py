@@ -90,6 +90,88 @@ def batched_leg(nt):
             "graph_frac_of_8TBs": round(cells * nt * 32.0 / (g_ms / 1e3) / 8e12, 4)}
 
 
+def visco_leg(nt_err, nt, no_cpu):
+    """Visco-acoustic 1001 x 2001 + 40-cell sponge, L = 3 (one shot, nt = 4000
+    recorded + the reference's 60-step delay): Mcell-updates/s over all stored
+    cells, the algorithmic bytes of the step kernel, the error against the
+    FP64 oracle on a bounded prefix of the same shot (the first `nt_err`
+    recorded samples) and the FP32 oracle's rate on `cpu_steps` steps."""
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    shot = W.visco_1001x2001(nt=nt)
+    nx, nz = shot.vp.shape
+    L = shot.nmech
+    nsteps = shot.ref_delay() + shot.nt
+    with sfd.ViscoPlan(shot) as pl:
+        pl.run()
+        ms = min(pl.run() for _ in range(2))
+        rec, _ = pl.fetch()
+    assert np.isfinite(rec).all()
+    sp = shot.sponge
+    strip_cells = (sp[0] + sp[1] + 2) * nz + (sp[2] + sp[3] + 2) * nx
+    bytes_step = (36.0 + 8.0 * L) * nx * nz + 16.0 * (1 + L) * strip_cells
+    out = {"workload": f"visco-acoustic RSG, R=5, L={L}, 1001x2001 interior + 40-cell cosine sponge "
+                       f"({nx}x{nz} stored), dt=1 ms, nt=4000 (+{shot.ref_delay()} delay steps), 20 Hz",
+           "cells": nx * nz, "steps": nsteps, "ms_per_shot": round(ms, 2),
+           "Mcell_per_s": round(nx * nz * nsteps / ms / 1e3, 1),
+           "bytes_per_cell_update": 36 + 8 * L,
+           "achieved_GBs": round(bytes_step * nsteps / (ms / 1e3) / 1e9, 1),
+           "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4)}
+    if not no_cpu:
+        import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        short = W.visco_1001x2001(nt=nt_err)
+        rs, _ = sfd.visco_forward(short)
+        args = dict(src=short.src, rec=short.rec, nsteps=short.ref_delay() + nt_err, dt=short.dt, f0=short.f0,
+                    nmech=L, sponge=short.sponge, rec_delay=short.ref_delay(), nthreads=cores)
+        rho = np.full_like(short.vp, 2400.0)
+        o64 = oracle.visco(short.vp, rho, short.qp, precision="f64", **args)["rec_zz"]
+        t0 = time.perf_counter()
+        o32 = oracle.visco(short.vp, rho, short.qp, precision="f32", **args)["rec_zz"]
+        dt32 = time.perf_counter() - t0
+        nrm = np.linalg.norm(o64)
+        out["fp64_rel_l2"] = float(np.linalg.norm(rs - o64) / nrm)
+        out["fp32_oracle_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
+        out["error_sample"] = f"first {nt_err} recorded samples ({args['nsteps']} steps) of the same shot"
+        out["cpu_baseline"] = {"value": round(nx * nz * args["nsteps"] / dt32 / 1e6, 2), "unit": "Mcell-updates/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"{args['nsteps']} steps of the same shot, FP32 oracle (split fields as "
+                                         f"Q_forward stores them), OpenMP over x rows, {dt32:.1f} s"}
+    return out
+
+
+def gradient_leg(nt):
+    """Adjoint gradient: 16 shots on 801 x 801 + 40-cell sponge, 10 Hz,
+    checkpoints every 50 steps: forward + checkpointed recomputation + adjoint
+    per shot (sfd_acoustic_misfit_gradient) against the plain forward of the
+    same 16 shots (one launch per shot and step) and the batched forward.
+    Timed on the first `nt` of 2500 steps; shots/s scaled to 2500."""
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    true, bg, shots = W.fwi_801(nt=nt)
+    obs_shots = [sfd.AcousticShot(**{**s.__dict__, "vp": true, "_keep": []}) for s in shots]
+    with sfd.AcousticBatch(obs_shots) as b:
+        b.run(sfd.BATCH_GRAPH)
+        d_obs = [b.fetch(i) for i in range(len(shots))]
+    with sfd.AcousticBatch(shots) as b:
+        b.run(sfd.BATCH_PER_SHOT)
+        plain_ms = b.run(sfd.BATCH_PER_SHOT)
+        graph_ms = min(b.run(sfd.BATCH_GRAPH) for _ in range(2))
+    sfd.misfit_gradient(shots[:1], d_obs[:1], vp=bg, ckpt_every=50)
+    t0 = time.perf_counter()
+    J, g = sfd.misfit_gradient(shots, d_obs, vp=bg, ckpt_every=50)
+    grad_s = time.perf_counter() - t0
+    assert np.isfinite(g).all() and J > 0
+    scale = 2500.0 / nt
+    n = len(shots)
+    return {"workload": "16 shots, 801x801 + 40-cell sponge (881x881), R=8, 10 Hz, checkpoint every 50 steps",
+            "timed_steps": nt, "misfit": J,
+            "fwd_adj_shots_per_s": round(n / (grad_s * scale), 3),
+            "plain_fwd_shots_per_s": round(n / (plain_ms / 1e3 * scale), 3),
+            "batched_fwd_shots_per_s": round(n / (graph_ms / 1e3 * scale), 3),
+            "fwd_adj_vs_plain_fwd": round((plain_ms / 1e3) / grad_s, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +184,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=600)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-visco", action="store_true")
+    ap.add_argument("--visco-err-nt", type=int, default=40)
+    ap.add_argument("--visco-nt", type=int, default=4000)
+    ap.add_argument("--only-visco", action="store_true",
+                    help="run the visco-acoustic leg alone (profiling) and print its JSON")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--grad-steps", type=int, default=500)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -113,6 +202,10 @@ def main():
 
     import seismic_code_github_amd as sfd
     from seismic_code_github_amd import workloads as W
+
+    if args.only_visco:
+        print(json.dumps(visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)), flush=True)
+        return
 
     if args.copy_mib:
         import torch
@@ -182,6 +275,12 @@ def main():
     if not args.no_batch:
         log("batched leg ...")
         out["batched"] = batched_leg(args.batch_steps)
+    if not args.no_visco:
+        log("visco leg ...")
+        out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)
+    if not args.no_grad:
+        log("gradient leg ...")
+        out["gradient"] = gradient_leg(args.grad_steps)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shot, args.cpu_steps)

@@ -237,6 +237,255 @@ int grad_one(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, co
     return rc;
 }
 
+// ------------------------------------------------------------ shot groups
+// Shots of one gradient that share a model grid advance together: one launch
+// per time step for all of them in the forward, the recomputation and the
+// adjoint sweep, and one launch per step for all their residual injections.
+
+// One receiver slot of one shot: p[shot][cell] += w * res[shot][k * nslot + slot].
+struct InjEntry {
+    int shot, cell;
+    float w;
+    int slot;
+};
+
+__global__ void adjoint_inject_group_kernel(float* const* __restrict__ p, const InjEntry* __restrict__ e, int n,
+                                            const float* const* __restrict__ res, const int* __restrict__ nslot,
+                                            int k) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const InjEntry t = e[q];
+    p[t.shot][t.cell] += t.w * res[t.shot][(size_t)k * nslot[t.shot] + t.slot];
+}
+
+template <int R>
+BatchKernel group_kernel_r(int mode) {
+    switch (mode) {
+    case MODE_FAST_SPONGE: return acoustic_batch_step_kernel<R, 2, MODE_FAST_SPONGE, default_slots<R>()>;
+    case MODE_FWD_DIV: return acoustic_batch_step_kernel<R, 2, MODE_FWD_DIV, default_slots<R>()>;
+    case MODE_ADJ: return acoustic_batch_step_kernel<R, 2, MODE_ADJ, default_slots<R>()>;
+    default: return nullptr;
+    }
+}
+
+BatchKernel group_kernel(int radius, int mode) {
+    switch (radius) {
+    case 4: return group_kernel_r<4>(mode);
+    case 5: return group_kernel_r<5>(mode);
+    case 8: return group_kernel_r<8>(mode);
+    default: return nullptr;
+    }
+}
+
+struct GradGroup {
+    sfd_context* ctx = nullptr;
+    std::vector<GradShot> shots;
+    int nt = 0, period = 0, nseg = 0, ntile = 0, ninj = 0;
+    BatchKernel kfwd = nullptr, kdiv = nullptr, kadj = nullptr;
+    DeviceBuffer fwd_args, adj_args, tile_shot, tile_base, inj, inj_p, inj_res, inj_nslot;
+    ~GradGroup() {
+        for (auto& g : shots)
+            if (g.pl) sfd_acoustic_plan_destroy(g.pl);
+    }
+    int nshot() const { return (int)shots.size(); }
+    BatchStepArgs args(const DeviceBuffer& blocks, int k, int parity) const {
+        BatchStepArgs a{};
+        a.shots = blocks.as<AcousticStepArgs>();
+        a.tile_shot = tile_shot.as<int>();
+        a.tile_base = tile_base.as<int>();
+        a.nshot = nshot();
+        a.ntile = ntile;
+        a.k = k;
+        a.parity = parity;
+        return a;
+    }
+};
+
+int group_tables(GradGroup* G) {
+    sfd_context* ctx = G->ctx;
+    const int n = G->nshot();
+    std::vector<AcousticStepArgs> fa(2 * (size_t)n), aa(2 * (size_t)n);
+    std::vector<int> base(n), tshot;
+    std::vector<float*> pp(2 * (size_t)n);
+    std::vector<const float*> rp(n);
+    std::vector<int> ns(n);
+    std::vector<InjEntry> inj;
+    int t = 0;
+    for (int s = 0; s < n; ++s) {
+        sfd_acoustic_plan* pl = G->shots[s].pl;
+        for (int par = 0; par < 2; ++par) {
+            fa[(size_t)par * n + s] = make_args(pl, par, 0);
+            aa[(size_t)par * n + s] = make_args(pl, pl->ap, pl->avx, pl->avz, par, 0);
+            pp[(size_t)par * n + s] = pl->grid(pl->ap[par]) - (size_t)pl->pad * pl->nz;
+        }
+        const int nt = fa[s].ntz * fa[s].nchunk;
+        base[s] = t;
+        tshot.insert(tshot.end(), nt, s);
+        t += nt;
+        rp[s] = pl->res.as<float>();
+        ns[s] = std::max(pl->nslot, 1);
+        std::vector<int> cell(pl->nslot);
+        std::vector<float> w(pl->nslot);
+        if (pl->nslot > 0) {
+            SFD_HIP(ctx, hipMemcpy(cell.data(), pl->inj_cell.ptr, pl->nslot * sizeof(int), hipMemcpyDeviceToHost));
+            SFD_HIP(ctx, hipMemcpy(w.data(), pl->inj_w.ptr, pl->nslot * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        for (int q = 0; q < pl->nslot; ++q) inj.push_back({s, cell[q], w[q], q});
+    }
+    G->ntile = t;
+    G->ninj = (int)inj.size();
+    hipStream_t st = ctx->stream;
+    hipError_t e = upload(G->fwd_args, fa.data(), fa.size(), st);
+    if (e == hipSuccess) e = upload(G->adj_args, aa.data(), aa.size(), st);
+    if (e == hipSuccess) e = upload(G->tile_shot, tshot.data(), tshot.size(), st);
+    if (e == hipSuccess) e = upload(G->tile_base, base.data(), base.size(), st);
+    if (e == hipSuccess) e = upload(G->inj_p, pp.data(), pp.size(), st);
+    if (e == hipSuccess) e = upload(G->inj_res, rp.data(), rp.size(), st);
+    if (e == hipSuccess) e = upload(G->inj_nslot, ns.data(), ns.size(), st);
+    if (e == hipSuccess && !inj.empty()) e = upload(G->inj, inj.data(), inj.size(), st);
+    if (e != hipSuccess) return hip_check(ctx, e, "gradient group tables");
+    return SFD_OK;
+}
+
+int group_inject(GradGroup* G, int parity, int k) {
+    if (G->ninj == 0) return SFD_OK;
+    const int bs = 256;
+    hipLaunchKernelGGL(adjoint_inject_group_kernel, dim3((G->ninj + bs - 1) / bs), dim3(bs), 0, G->ctx->stream,
+                       G->inj_p.as<float*>() + (size_t)parity * G->nshot(), G->inj.as<InjEntry>(), G->ninj,
+                       G->inj_res.as<const float*>(), G->inj_nslot.as<int>(), k);
+    SFD_HIP(G->ctx, hipGetLastError());
+    return SFD_OK;
+}
+
+int group_launch(GradGroup* G, BatchKernel kern, const DeviceBuffer& blocks, int k, int parity) {
+    const BatchStepArgs a = G->args(blocks, k, parity);
+    hipLaunchKernelGGL(kern, dim3(G->ntile), dim3(128), 0, G->ctx->stream, a);
+    SFD_HIP(G->ctx, hipGetLastError());
+    return SFD_OK;
+}
+
+// Copies the current state of every shot to (or from) its checkpoint `seg`.
+int group_checkpoint(GradGroup* G, int seg, bool save) {
+    sfd_context* ctx = G->ctx;
+    for (auto& g : G->shots) {
+        sfd_acoustic_plan* pl = g.pl;
+        const size_t n = (size_t)pl->nx * pl->nz;
+        float* ck = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
+        const int c = save ? pl->cur : 0;
+        Field* f[3] = {&pl->p[c], &pl->vx[c], &pl->vz[c]};
+        for (int q = 0; q < 3; ++q) {
+            float* dst = save ? ck + q * n : pl->grid(*f[q]);
+            const float* src = save ? pl->grid(*f[q]) : ck + q * n;
+            SFD_HIP(ctx, hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
+    return SFD_OK;
+}
+
+// Misfit gradient of up to G->shots.size() shots on one model; adds into
+// grad_vp and *misfit.
+int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp, const float* den,
+              const float* const* wavelet, const float* const* d_obs, int period, float* grad_vp, double* misfit) {
+    GradGroup G;
+    G.ctx = ctx;
+    G.nt = params[0].nt;
+    G.period = std::min(period, G.nt);
+    G.shots.resize(nshot);
+    for (int s = 0; s < nshot; ++s) {
+        int rc = check_gradient_params(ctx, &params[s]);
+        if (!rc && params[s].nt != G.nt) rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same nt", s);
+        if (!rc && params[s].radius != params[0].radius)
+            rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same radius", s);
+        if (!rc) rc = grad_setup(ctx, &params[s], vp, den, wavelet[s], G.period, &G.shots[s]);
+        if (!rc) rc = sfd_acoustic_plan_set_tiling(G.shots[s].pl, 2, 0);
+        if (rc) return rc;
+    }
+    // One chunk length for the group (the first shot's automatic choice).
+    const int chunk = G.shots[0].pl->chunk;
+    for (auto& g : G.shots) g.pl->chunk = chunk;
+    const int R = params[0].radius;
+    G.kfwd = group_kernel(R, MODE_FAST_SPONGE);
+    G.kdiv = group_kernel(R, MODE_FWD_DIV);
+    G.kadj = group_kernel(R, MODE_ADJ);
+    if (!G.kfwd || !G.kdiv || !G.kadj) return fail(ctx, SFD_ENOTSUP, "no gradient kernel for radius %d", R);
+    G.nseg = (G.nt + G.period - 1) / G.period;
+    int rc = group_tables(&G);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+
+    // 1. forward, checkpointing before every segment but the first.
+    for (auto& g : G.shots) {
+        rc = zero_state(g.pl);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < G.nt; ++k) {
+        if (k > 0 && k % G.period == 0) {
+            for (auto& g : G.shots) g.pl->cur = k & 1;
+            rc = group_checkpoint(&G, k / G.period, true);
+            if (rc) return rc;
+        }
+        rc = group_launch(&G, G.kfwd, G.fwd_args, k, k & 1);
+        if (rc) return rc;
+    }
+    for (auto& g : G.shots) g.pl->cur = G.nt & 1;
+    // 2. residuals, misfit, seeds.
+    for (int q = 0; q < nshot; ++q) {
+        GradShot& g = G.shots[q];
+        std::vector<float> rec((size_t)params[q].nrec * G.nt), sd(rec.size());
+        rc = sfd_acoustic_plan_fetch(g.pl, rec.data(), nullptr);
+        if (rc) return rc;
+        double m = 0.0;
+        for (size_t c = 0; c < rec.size(); ++c) {
+            sd[c] = rec[c] - d_obs[q][c];
+            m += 0.5 * (double)sd[c] * sd[c];
+        }
+        if (misfit) *misfit += m;
+        rc = grad_upload_seed(&g, sd.data());
+        if (rc) return rc;
+    }
+    // 3. backward: per segment, restore, recompute with divergences, adjoint.
+    for (auto& g : G.shots) {
+        sfd_acoustic_plan* pl = g.pl;
+        for (int b = 0; b < 2; ++b)
+            for (Field* f : {&pl->ap[b], &pl->avx[b], &pl->avz[b]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+        SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
+        SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
+    }
+    int acur = 0;
+    rc = group_inject(&G, acur, G.nt - 1);
+    if (rc) return rc;
+    for (int seg = G.nseg - 1; seg >= 0; --seg) {
+        const int k0 = seg * G.period, k1 = std::min(k0 + G.period, G.nt);
+        if (seg == 0) {
+            for (auto& g : G.shots)
+                for (Field* f : {&g.pl->p[0], &g.pl->vx[0], &g.pl->vz[0]})
+                    SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+        } else {
+            rc = group_checkpoint(&G, seg, false);
+            if (rc) return rc;
+        }
+        for (int k = k0; k < k1; ++k) {
+            rc = group_launch(&G, G.kdiv, G.fwd_args, k, (k - k0) & 1);
+            if (rc) return rc;
+        }
+        for (int k = k1 - 1; k >= k0; --k) {
+            rc = group_launch(&G, G.kadj, G.adj_args, k, acur);
+            if (rc) return rc;
+            acur ^= 1;
+            if (k >= 1) {
+                rc = group_inject(&G, acur, k - 1);
+                if (rc) return rc;
+            }
+        }
+    }
+    // 4. dJ/dvp summed over the group's shots.
+    for (int q = 0; q < nshot; ++q) {
+        rc = grad_finish(&G.shots[q], vp, den, wavelet[q], grad_vp);
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
 }  // namespace
 }  // namespace sfd
 
@@ -256,11 +505,19 @@ int sfd_acoustic_misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic
     for (int s = 1; s < nshot; ++s)
         if (params[s].nx != params[0].nx || params[s].nz != params[0].nz)
             return fail(ctx, SFD_EINVAL, "shot %d: every shot of a gradient shares the model grid", s);
+    if (ckpt_every < 1) return fail(ctx, SFD_EINVAL, "checkpoint interval must be >= 1");
     std::fill(grad_vp, grad_vp + (size_t)params[0].nx * params[0].nz, 0.f);
     if (misfit) *misfit = 0.0;
-    for (int s = 0; s < nshot; ++s) {
-        const int rc = grad_one(ctx, &params[s], vp, den, wavelet[s], nullptr, d_obs[s], ckpt_every, grad_vp,
-                                nullptr, misfit);
+    // Groups sized so that their workspaces (state, adjoint state, one
+    // segment of divergences and the checkpoints of every shot) stay below
+    // 64 GiB of device memory.
+    const double field = (double)params[0].nx * params[0].nz * 4.0;
+    const int period = std::min(ckpt_every, std::max(params[0].nt, 1));
+    const double per_shot = field * (16.0 + period) * 1.2 + field * 3.0 * ((params[0].nt + period - 1) / period);
+    const int gmax = (int)std::max(1.0, std::min((double)nshot, 64.0 * (1ull << 30) / per_shot));
+    for (int s0 = 0; s0 < nshot; s0 += gmax) {
+        const int n = std::min(gmax, nshot - s0);
+        const int rc = group_run(ctx, n, params + s0, vp, den, wavelet + s0, d_obs + s0, period, grad_vp, misfit);
         if (rc) return rc;
     }
     return SFD_OK;

@@ -165,3 +165,36 @@ def visco_1001x2001(nt: int = 4000, nmech: int = 3, scale: int = 1, vp_seed: int
               dt=0.001, f0=20.0, radius=5, nmech=nmech, sponge=(nb, nb, nb, nb))
     kw.update(overrides)
     return ViscoShot(**kw)
+
+
+def fwi_801(nshot: int = 16, nt: int = 2500, scale: int = 1, seed: int = 6):
+    """Adjoint-gradient workload: 801 x 801 model + 40-cell sponge (881 x 881
+    stored), dx = dz = 10 m, dt = 1 ms, nt = 2500, 10 Hz Ricker.  Background:
+    linear gradient with depth 2000 -> 3500 m/s; true model: background + 3
+    Gaussian anomalies of +-300 m/s (seeded centres, radii 3-8 % of the model
+    size, signs); initial model: the background.  nshot sources evenly spaced
+    at model z = 2, receivers in every model column at z = 2.  scale > 1
+    shrinks the model by that factor.  Returns (true_vp, initial_vp, shots)
+    with shots on the initial model."""
+    nb = 40
+    ni = 800 // scale + 1
+    n = ni + 2 * nb
+    z = np.clip(np.arange(n) - nb, 0, ni - 1) / max(ni - 1, 1)
+    bg = np.ascontiguousarray(np.broadcast_to((2000.0 + 1500.0 * z)[None, :], (n, n)), dtype=np.float32)
+    rng = np.random.default_rng(seed)
+    xx, zz = np.meshgrid(np.arange(n) - nb, np.arange(n) - nb, indexing="ij")
+    true = bg.astype(np.float64).copy()
+    for _ in range(3):
+        cx, cz = rng.uniform(0.2, 0.8, 2) * (ni - 1)
+        rad = rng.uniform(0.03, 0.08) * ni
+        sign = 1.0 if rng.random() < 0.5 else -1.0
+        true += sign * 300.0 * np.exp(-((xx - cx) ** 2 + (zz - cz) ** 2) / (2 * rad * rad))
+    true = true.astype(np.float32)
+    dt = 0.001
+    w = ricker(nt, dt, 10.0)
+    rx = np.arange(nb, nb + ni, dtype=np.int32)
+    rz = np.full_like(rx, nb + 2)
+    xs = np.linspace(nb, nb + ni - 1, nshot).round().astype(int)
+    shots = [AcousticShot(vp=bg, den=None, wavelet=w, src=(int(x), nb + 2), rec=(rx, rz), dx=10.0, dz=10.0, dt=dt,
+                          radius=8, boundary="sponge", width=nb) for x in xs]
+    return true, bg, shots

@@ -132,3 +132,22 @@ def test_gradient_errors(ctx):
     ref = sfd.AcousticShot(**{**shot.__dict__, "arith": "reference", "_keep": []})
     with pytest.raises(sfd.SeisfdError, match="ENOTSUP"):
         sfd.acoustic_gradient(ref, sd, ctx=ctx)
+
+
+def test_shot_group_matches_single_shots_uneven_segments(ctx):
+    """Five shots advanced together (one launch per step for all of them),
+    nt = 150 with checkpoints every 40 steps (last segment 30 steps): equal to
+    the single-shot gradients of the same residuals."""
+    print("case: shot group, 5 shots, nt=150, ckpt 40", flush=True)
+    shots, obs = [], []
+    for sx in (18, 26, 34, 42, 50):
+        shot, _, _ = case(nt=150, src=(sx, 15))
+        shots.append(shot)
+        true = sfd.AcousticShot(**{**shot.__dict__, "vp": shot.vp * np.float32(0.985), "_keep": []})
+        obs.append(sfd.acoustic_forward(true, ctx)[0])
+    J, g = sfd.misfit_gradient(shots, obs, ckpt_every=40, ctx=ctx)
+    gref = np.zeros_like(g, dtype=np.float64)
+    for shot, d in zip(shots, obs):
+        r = sfd.acoustic_forward(shot, ctx)[0] - d
+        gref += sfd.acoustic_gradient(shot, r, ckpt_every=40, ctx=ctx)[0]
+    assert rel_l2(g, gref) <= 1e-6
