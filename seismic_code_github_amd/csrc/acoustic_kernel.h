@@ -288,8 +288,8 @@ __device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, S
     // reference arithmetic uses the unscaled C_m for both axes).
     float cx[R];
 #pragma unroll
-    for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
-    const float zx = REF ? 1.0f : a.zx;
+    for (int m = 0; m < R; ++m) cx[m] = in_vgpr(a.cx[m]);
+    const float zx = REF ? 1.0f : in_vgpr(a.zx);
 
     // C-PML strips: psi_x* are [2*xw][nz] (x-strip rows), psi_z* [nx][2*zw].
     const int xw = a.pml_xw, zw = a.pml_zw, zw2 = 2 * zw;

@@ -31,6 +31,15 @@ __device__ __forceinline__ float uniform_f(float x) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 
+// A wave-uniform float the compiler is told lives in a vector register: used
+// for stencil coefficients, which would otherwise take scalar registers the
+// march needs for its row offsets (and be reloaded from the kernel arguments
+// with scalar loads whose waits also drain the LDS queue).
+__device__ __forceinline__ float in_vgpr(float x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 __device__ __forceinline__ int floor_div(int a, int b) {
     return (a >= 0) ? a / b : -((-a + b - 1) / b);
 }

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(64 * NW) void visco_step_kernel(const ViscoStepArgs
 
     float co[R];
 #pragma unroll
-    for (int m = 0; m < R; ++m) co[m] = a.a[m];
+    for (int m = 0; m < R; ++m) co[m] = in_vgpr(a.a[m]);
     float cl[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) cl[l] = a.c[l];
