@@ -343,3 +343,100 @@ def visco(vp, rho, qp, *, src, rec, nsteps, dx=10.0, dz=10.0, dt=1e-3, f0=20.0, 
     if rc != 0:
         raise ValueError(f"oracle_visco returned {rc}")
     return {"rec_zz": rzz, "rec_xx": rxx, "snaps": snaps}
+
+
+class _EArgs(ctypes.Structure):
+    """Mirror of ``oracle_elastic_args`` (oracle.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nsteps", ctypes.c_int), ("delay", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float), ("f0", ctypes.c_float),
+        ("vp", ctypes.POINTER(ctypes.c_float)), ("vs", ctypes.POINTER(ctypes.c_float)),
+        ("rho", ctypes.POINTER(ctypes.c_float)), ("itopo", ctypes.POINTER(ctypes.c_int)),
+        ("npxn", ctypes.c_int), ("npxp", ctypes.c_int), ("npzp", ctypes.c_int),
+        ("sponge_r", ctypes.c_float), ("isx", ctypes.c_int), ("isz", ctypes.c_int), ("rgau", ctypes.c_int),
+        ("wavelet", ctypes.POINTER(ctypes.c_float)),
+        ("nrec", ctypes.c_int), ("rx", ctypes.POINTER(ctypes.c_int)), ("rz", ctypes.POINTER(ctypes.c_int)),
+        ("nsnap", ctypes.c_int), ("snap_steps", ctypes.POINTER(ctypes.c_int)),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+def _elib():
+    L = lib()
+    if not hasattr(L, "_elastic_bound"):
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_elastic_f32.argtypes = [ctypes.POINTER(_EArgs), fp, fp, fp]
+        L.oracle_elastic_f64.argtypes = [ctypes.POINTER(_EArgs), dp, dp, dp]
+        L.oracle_ssg_deriv_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ip, dp]
+        L.oracle_elastic_stagger_f32.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, fp, ip, fp]
+        L.oracle_elastic_stagger_f64.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, fp, ip, dp]
+        L._elastic_bound = True
+    return L
+
+
+def ssg_deriv(u, axis, flag, itopo=None):
+    """One calux ("x") / caluz ("z") stencil sum of the elastic restatement
+    (FP64) with flag = xflag / zflag; rows above itopo are left 0."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nx, nz = u.shape
+    it = np.zeros(nx, np.int32) if itopo is None else np.ascontiguousarray(itopo, dtype=np.int32)
+    out = np.zeros_like(u)
+    rc = _elib().oracle_ssg_deriv_f64({"x": 0, "z": 1}[axis], flag, nx, nz, _ptr(u, ctypes.c_double),
+                                      _ptr(it, ctypes.c_int), _ptr(out, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"oracle_ssg_deriv_f64 returned {rc}")
+    return out
+
+
+def elastic_stagger(vp, vs, rho, itopo, precision="f32"):
+    """substagger's staggered fields as a dict of [nx, nz] arrays."""
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    vp, vs, rho = f32(vp), f32(vs), f32(rho)
+    nx, nz = vp.shape
+    it = np.ascontiguousarray(itopo, dtype=np.int32)
+    if precision == "f32":
+        out = np.zeros((5, nx, nz), np.float32)
+        rc = _elib().oracle_elastic_stagger_f32(nx, nz, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(it, ctypes.c_int),
+                                                _ptr(out))
+    else:
+        out = np.zeros((5, nx, nz), np.float64)
+        rc = _elib().oracle_elastic_stagger_f64(nx, nz, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(it, ctypes.c_int),
+                                                _ptr(out, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"oracle_elastic_stagger_f32 returned {rc}")
+    return dict(zip(("busg2e", "busg1e", "lambmusg", "lambsg", "musge"), out))
+
+
+def elastic(vp, vs, rho, itopo, *, src, rec, nsteps, delay=0, dx=5.0, dz=5.0, dt=1e-3, f0=25.0,
+            sponge=(40, 40, 40), sponge_r=1e-3, rgau=50, wavelet=None, snap_steps=(), precision="f32",
+            nthreads=1):
+    """Run the elastic free-surface oracle (oracle/elastic_body.h).  Fields
+    [nx, nz] (z fastest); itopo [nx] 0-based surface rows; sponge = (npxn,
+    npxp, npzp).  Returns dict(rec_vx, rec_vz [nrec, nsteps - delay],
+    snaps [nsnap, 5, nx, nz] = vx, vz, sxx, szz, sxz)."""
+    L = _elib()
+    f32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    vp, vs, rho = f32(vp), f32(vs), f32(rho)
+    nx, nz = vp.shape
+    it = np.ascontiguousarray(itopo, dtype=np.int32)
+    rx = np.ascontiguousarray(rec[0], dtype=np.int32)
+    rz = np.ascontiguousarray(rec[1], dtype=np.int32)
+    steps = np.ascontiguousarray(sorted(snap_steps), dtype=np.int32)
+    w = f32(wavelet)
+    if w is not None:
+        assert w.shape[0] >= nsteps
+    args = _EArgs(nx, nz, nsteps, delay, dx, dz, dt, f0, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(it, ctypes.c_int),
+                  *sponge, sponge_r, int(src[0]), int(src[1]), rgau, _ptr(w), rx.size, _ptr(rx, ctypes.c_int),
+                  _ptr(rz, ctypes.c_int), steps.size, _ptr(steps, ctypes.c_int), nthreads)
+    dtype, ct = (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
+    nrt = max(nsteps - delay, 0)
+    rvx = np.zeros((rx.size, nrt), dtype=dtype)
+    rvz = np.zeros((rx.size, nrt), dtype=dtype)
+    snaps = np.zeros((steps.size, 5, nx, nz), dtype=dtype)
+    fn = L.oracle_elastic_f32 if precision == "f32" else L.oracle_elastic_f64
+    rc = fn(ctypes.byref(args), _ptr(rvx, ct), _ptr(rvz, ct), _ptr(snaps, ct) if steps.size else ctypes.POINTER(ct)())
+    if rc != 0:
+        raise ValueError(f"oracle_elastic returned {rc}")
+    return {"rec_vx": rvx, "rec_vz": rvz, "snaps": snaps}

@@ -146,6 +146,40 @@ int oracle_rsg_deriv_f64(int dir, int shift, int nx, int nz, const double *u, do
 void oracle_csponge_f32(int np, float h, float r, int lo, float *node, float *half);
 int oracle_visco_source_weights_f32(int nx, int nz, int isx, int isz, float sigma, float *w);
 
+/* Isotropic elastic standard-staggered-grid propagator with a free surface
+ * over topography: isoewm2dfsbc (yang_version_2016_2_2, pri:57-230) with its
+ * set-up (substagger, csponge, get_gauss_filter_coe). */
+typedef struct oracle_elastic_args {
+    int nx, nz;              /* padded shot grid: reference nxea (x), nzea (z) */
+    int nsteps;              /* steps run (reference nt + idecay) */
+    int delay;               /* reference idecay: step `delay` is the first recorded */
+    float dx, dz, dt, f0;
+    const float *vp, *vs, *rho;  /* [nx*nz] before substagger */
+    const int *itopo;        /* [nx] 0-based free-surface rows (reference itopoe - 1) */
+    int npxn, npxp, npzp;    /* sponge table lengths - 1 (reference np*) */
+    float sponge_r;          /* reference R = 1e-3 */
+    int isx, isz;            /* source cell (reference isxe - 1, isze - 1) */
+    int rgau;                /* Gaussian window half-width + 1 (window 2 rgau - 1) */
+    const float *wavelet;    /* [nsteps] or NULL = addsrc's Ricker */
+    int nrec;
+    const int *rx, *rz;      /* receiver cells */
+    int nsnap;
+    const int *snap_steps;   /* ascending step indices s after which fields are copied */
+    int nthreads;
+} oracle_elastic_args;
+/* rec_vx, rec_vz [nrec][nsteps - delay] (reference record1 = vx, record2 = vz);
+ * snaps [nsnap][5][nx*nz]: vx, vz, tau_xx, tau_zz, tau_xz. */
+int oracle_elastic_f32(const oracle_elastic_args *a, float *rec_vx, float *rec_vz, float *snaps);
+int oracle_elastic_f64(const oracle_elastic_args *a, double *rec_vx, double *rec_vz, double *snaps);
+/* Single routines for the oracle's own pins: one calux (axis 0) / caluz
+ * (axis 1) stencil sum with flag = xflag / zflag, FP64; substagger's staggered
+ * fields [5][nx*nz] (busg2e, busg1e, lambmusg, lambsg, musge). */
+int oracle_ssg_deriv_f64(int axis, int flag, int nx, int nz, const double *u, const int *itopo, double *out);
+int oracle_elastic_stagger_f32(int nx, int nz, const float *vp, const float *vs, const float *rho, const int *itopo,
+                               float *out);
+int oracle_elastic_stagger_f64(int nx, int nz, const float *vp, const float *vs, const float *rho, const int *itopo,
+                               double *out);
+
 #ifdef __cplusplus
 }
 #endif
