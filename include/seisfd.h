@@ -58,7 +58,11 @@ enum sfd_arith {
     SFD_ARITH_REFERENCE = 1
 };
 
-enum sfd_field { SFD_FIELD_P = 1, SFD_FIELD_VX = 2, SFD_FIELD_VZ = 4 };
+enum sfd_field {
+    SFD_FIELD_P = 1, SFD_FIELD_VX = 2, SFD_FIELD_VZ = 4,
+    /* stresses of the elastic propagator */
+    SFD_FIELD_SXX = 8, SFD_FIELD_SZZ = 16, SFD_FIELD_SXZ = 32
+};
 
 typedef struct sfd_context sfd_context;
 typedef struct sfd_acoustic_plan sfd_acoustic_plan;
@@ -253,6 +257,59 @@ int sfd_visco_plan_fetch(sfd_visco_plan *plan, float *record, float *snaps);
 /* Waves per workgroup (2 or 4) and rows marched per workgroup; 0 = automatic. */
 int sfd_visco_plan_set_tiling(sfd_visco_plan *plan, int waves, int chunk_rows);
 int sfd_visco_plan_destroy(sfd_visco_plan *plan);
+
+/* ---------------------------------------------------------------- elastic
+ * Isotropic elastic standard-staggered-grid propagator with a free surface
+ * over topography: the drop-in for the time loop of
+ * forward/yang_version_2016_2_2, isoewm2dfsbc
+ * (Iso_multishot_fd_subroutine_fsbc_zy_m2_pri.f90:57-230, called from
+ * Iso_multishot_fd_main_fsbc_zy_m2.f90:193-201 in 'fsbc' mode), with its
+ * set-up substagger (pub:323-547), csponge (pub:796-847) and the Gaussian
+ * source window (pub:1470-1501).  10th-order x derivatives; z derivatives of
+ * order 10 reduced to 8, 6, 4, 2 on the rows just below the surface; cosine
+ * sponges on the left, right and bottom strips; tau_zz = 0 on and above the
+ * surface, the tau_xz image and vz above it from tau_zz = 0.  Grid = the
+ * reference's padded shot grid nxea x nzea (get_currshot_parameters, pub:188-313,
+ * or sfd_window_model), [nx][nz] z fastest, 0-based. */
+typedef struct sfd_elastic_plan sfd_elastic_plan;
+
+typedef struct sfd_elastic_params {
+    int nx, nz;            /* padded shot grid (reference nxea, nzea) */
+    int nt;                /* recorded samples (reference nt) */
+    int delay;             /* steps run before the first recorded one (reference
+                              idecay = nint(decay/dt), pri:28-29); < 0 = nint(1.2 / (f0 dt)) */
+    float dx, dz, dt;
+    float f0;              /* Hz: the built-in wavelet */
+    const int *itopo;      /* [nx] free-surface row of every column, 0-based (reference itopoe - 1,
+                              itopoe = nint(topo/dz) + npzn, pub:270-272) */
+    int sponge_xlo, sponge_xhi, sponge_zhi;  /* reference npxn, npxp, npzp: strips of n+1 cells
+                              damped by cos(acos(R) d / n) (csponge); no top strip */
+    float sponge_r;        /* reference R (module constant 1e-3); 0 = 1e-3 */
+    int sx, sz;            /* source cell (reference isxe - 1, isze - 1) */
+    int src_radius;        /* reference rgau: Gaussian window of 2 rgau - 1 cells, sigma 2; 0 = 50 */
+    int nrec;
+    const int *rx, *rz;    /* receiver cells (reference: row itopo + nrz0 of every window column) */
+    int nsnap;
+    const int *snap_steps; /* ascending step indices in [0, delay + nt) after which fields are copied */
+    int snap_fields;       /* mask of enum sfd_field: VX, VZ, SXX, SZZ, SXZ */
+} sfd_elastic_params;
+
+/* vp, vs, rho [nx*nz] required (vs = 0 cells are fluid); wavelet [delay + nt]
+ * or NULL = addsrc's Ricker exp(-(pi f0 (it dt - delay dt))^2)(1 - 2 (...)^2),
+ * it = 1.., injected as wavelet g dt / dz^2 into tau_zz and, when the source
+ * lies below the surface, tau_xx.
+ *   record_vx, record_vz [nrec][nt]: vx, vz at the receivers after steps
+ *          delay .. delay+nt-1 (reference record1, record2, pri:199-210)
+ *   snaps  [nsnap][popcount(snap_fields)][nx*nz], fields in vx, vz, sxx, szz, sxz order */
+int sfd_elastic_forward(sfd_context *ctx, const sfd_elastic_params *p, const float *vp, const float *vs,
+                        const float *rho, const float *wavelet, float *record_vx, float *record_vz, float *snaps);
+int sfd_elastic_plan_create(sfd_context *ctx, const sfd_elastic_params *p, const float *vp, const float *vs,
+                            const float *rho, const float *wavelet, sfd_elastic_plan **plan);
+/* Zeroes the fields and runs all delay + nt steps; elapsed_ms (may be NULL)
+ * receives the device time measured with HIP events on the plan's stream. */
+int sfd_elastic_plan_run(sfd_elastic_plan *plan, float *elapsed_ms);
+int sfd_elastic_plan_fetch(sfd_elastic_plan *plan, float *record_vx, float *record_vz, float *snaps);
+int sfd_elastic_plan_destroy(sfd_elastic_plan *plan);
 
 #ifdef __cplusplus
 }

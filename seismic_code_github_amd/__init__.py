@@ -34,6 +34,7 @@ BOUNDARY_CPML = 1
 ARITH_FAST = 0
 ARITH_REFERENCE = 1
 FIELD_P, FIELD_VX, FIELD_VZ = 1, 2, 4
+FIELD_SXX, FIELD_SZZ, FIELD_SXZ = 8, 16, 32
 
 _STATUS = {SFD_EINVAL: "EINVAL", SFD_ENOMEM: "ENOMEM", SFD_EDEVICE: "EDEVICE",
            SFD_ECFL: "ECFL", SFD_ENAN: "ENAN", SFD_ENOTSUP: "ENOTSUP"}
@@ -79,6 +80,20 @@ class ViscoParamsC(ctypes.Structure):
     ]
 
 
+class ElasticParamsC(ctypes.Structure):
+    """Mirror of ``struct sfd_elastic_params``."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nt", ctypes.c_int), ("delay", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float), ("f0", ctypes.c_float),
+        ("itopo", _IP),
+        ("sponge_xlo", ctypes.c_int), ("sponge_xhi", ctypes.c_int), ("sponge_zhi", ctypes.c_int),
+        ("sponge_r", ctypes.c_float),
+        ("sx", ctypes.c_int), ("sz", ctypes.c_int), ("src_radius", ctypes.c_int),
+        ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
+        ("nsnap", ctypes.c_int), ("snap_steps", _IP), ("snap_fields", ctypes.c_int),
+    ]
+
+
 # Every symbol declared in include/seisfd.h, with its ctypes signature.
 _SIGNATURES = {
     "sfd_abi_version": (ctypes.c_int, []),
@@ -117,6 +132,13 @@ _SIGNATURES = {
     "sfd_visco_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
     "sfd_visco_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_elastic_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ElasticParamsC), _FP, _FP, _FP, _FP,
+                                           _FP, _FP, _FP]),
+    "sfd_elastic_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ElasticParamsC), _FP, _FP, _FP, _FP,
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_elastic_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_elastic_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP, _FP]),
+    "sfd_elastic_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_window_model": (ctypes.c_int, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP]),
 }
@@ -580,3 +602,130 @@ def visco_forward(shot: ViscoShot, ctx: Optional[Context] = None):
                                       _ptr(record) if p.nrec and p.nt else None, _ptr(snaps) if p.nsnap else None),
               "sfd_visco_forward")
     return record, snaps
+
+
+# ---------------------------------------------------------------- elastic
+
+@dataclass
+class ElasticShot:
+    """One elastic free-surface shot on the reference's padded shot grid
+    (forward/yang_version_2016_2_2, isoewm2dfsbc in 'fsbc' mode).
+
+    vp, vs, rho: [nx, nz] float32, z fastest.  itopo [nx]: 0-based surface row
+    of every column.  src = (sx, sz); rec = (rx[], rz[]).  sponge = (npxn,
+    npxp, npzp).  nt = recorded samples; delay = steps before the first (None =
+    nint(1.2 / (f0 dt))).  wavelet None = the reference's built-in Ricker.
+    snap_fields: mask of FIELD_VX, FIELD_VZ, FIELD_SXX, FIELD_SZZ, FIELD_SXZ."""
+    vp: np.ndarray
+    vs: np.ndarray
+    rho: np.ndarray
+    itopo: np.ndarray
+    src: tuple
+    rec: tuple
+    nt: int
+    dx: float = 5.0
+    dz: float = 5.0
+    dt: float = 1e-3
+    f0: float = 25.0
+    sponge: tuple = (40, 40, 40)
+    sponge_r: float = 1e-3
+    src_radius: int = 50
+    delay: Optional[int] = None
+    wavelet: Optional[np.ndarray] = None
+    snap_steps: Sequence[int] = ()
+    snap_fields: int = FIELD_VX | FIELD_VZ
+    _keep: list = field(default_factory=list, repr=False)
+
+    def params(self) -> ElasticParamsC:
+        nx, nz = np.asarray(self.vp).shape
+        it = np.ascontiguousarray(self.itopo, dtype=np.int32)
+        rx = np.ascontiguousarray(self.rec[0], dtype=np.int32)
+        rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
+        steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
+        self._keep = [it, rx, rz, steps]
+        return ElasticParamsC(nx, nz, int(self.nt), -1 if self.delay is None else int(self.delay),
+                              self.dx, self.dz, self.dt, self.f0, _ptr(it, ctypes.c_int),
+                              *[int(v) for v in self.sponge], self.sponge_r, int(self.src[0]), int(self.src[1]),
+                              int(self.src_radius), rx.size, _ptr(rx, ctypes.c_int), _ptr(rz, ctypes.c_int),
+                              steps.size, _ptr(steps, ctypes.c_int), self.snap_fields)
+
+    def nfields(self) -> int:
+        return bin(self.snap_fields & 62).count("1")
+
+    def ref_delay(self) -> int:
+        """nint(1.2 / (f0 dt)) in float, the delay used when none is given."""
+        if self.delay is not None:
+            return int(self.delay)
+        v = np.float32(1.2) / (np.float32(self.f0) * np.float32(self.dt))
+        return int(np.floor(float(v) + 0.5))
+
+    def _arrays(self):
+        w = None if self.wavelet is None else _f32(self.wavelet)
+        return _f32(self.vp), _f32(self.vs), _f32(self.rho), w
+
+
+class ElasticPlan:
+    """``sfd_elastic_plan``: model resident on the device."""
+
+    def __init__(self, shot: ElasticShot, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.shot = shot
+        self.p = shot.params()
+        self.nx, self.nz = np.asarray(shot.vp).shape
+        self._arr = shot._arrays()
+        vp, vs, rho, w = self._arr
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_elastic_plan_create(self.ctx.handle, ctypes.byref(self.p), _ptr(vp), _ptr(vs),
+                                                     _ptr(rho), _ptr(w), ctypes.byref(h)), "sfd_elastic_plan_create")
+        self.handle = h
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_elastic_plan_run(self.handle, ctypes.byref(ms)), "sfd_elastic_plan_run")
+        return ms.value
+
+    def fetch(self):
+        """(record_vx [nrec, nt], record_vz [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
+        nrec, nt = self.p.nrec, self.p.nt
+        rvx = np.zeros((nrec, nt), dtype=np.float32)
+        rvz = np.zeros((nrec, nt), dtype=np.float32)
+        snaps = np.zeros((self.p.nsnap, self.shot.nfields(), self.nx, self.nz), dtype=np.float32)
+        has = nrec and nt
+        self.ctx.check(lib().sfd_elastic_plan_fetch(self.handle, _ptr(rvx) if has else None,
+                                                    _ptr(rvz) if has else None,
+                                                    _ptr(snaps) if self.p.nsnap else None), "sfd_elastic_plan_fetch")
+        return rvx, rvz, snaps
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_elastic_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def elastic_forward(shot: ElasticShot, ctx: Optional[Context] = None):
+    """One elastic shot through ``sfd_elastic_forward``.
+    Returns (record_vx, record_vz [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
+    ctx = ctx or default_context()
+    p = shot.params()
+    nx, nz = np.asarray(shot.vp).shape
+    vp, vs, rho, w = shot._arrays()
+    rvx = np.zeros((p.nrec, p.nt), dtype=np.float32)
+    rvz = np.zeros((p.nrec, p.nt), dtype=np.float32)
+    snaps = np.zeros((p.nsnap, shot.nfields(), nx, nz), dtype=np.float32)
+    has = p.nrec and p.nt
+    ctx.check(lib().sfd_elastic_forward(ctx.handle, ctypes.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(w),
+                                        _ptr(rvx) if has else None, _ptr(rvz) if has else None,
+                                        _ptr(snaps) if p.nsnap else None), "sfd_elastic_forward")
+    return rvx, rvz, snaps

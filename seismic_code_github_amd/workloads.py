@@ -20,6 +20,18 @@ geometries:
   mapped to 20-200, density 2400, 3 relaxation mechanisms, 20 Hz source
   (the reference's built-in wavelet and 1.5-cell Gaussian footprint) at
   model (z=2, x=1000), 2001 receivers at model z=2, R = 5 (a100..a104).
+* ``elastic_par_903x603`` — the elastic free-surface program's own parameter
+  file, forward/yang_version_2016_2_2/iso_2d_modelling_sin_test.par: a
+  501 x 401 model at dx = dz = 5 m, one shot at x = 1250 m on the surface,
+  offsets -1250..1250 m, 1000 m sponges left, right and below (npxn = npxp =
+  npzp = 201) and npzn = 1 above, so the padded shot grid is nxea x nzea =
+  903 x 603; nt = 3001 recorded samples after idecay = 48 steps (decay 0.048 s),
+  dt = 1 ms, f0 = 25 Hz, Gaussian source window rgau = 50, receivers on every
+  window column one row below the surface.  The model is the four-layer model
+  whose harmonic shear modulus the program wrote (musge_test: mu = 5.625e8,
+  8.1e8, 1.1025e9, 1.44e9 Pa below model rows 0, 101, 201, 301), taken as
+  rho = 2250 kg/m^3 and vs = 500, 600, 700, 800 m/s, with vp = 2 vs; flat
+  topography (surface on padded row 0).
 * ``smooth_random_2001x4001`` — 2001 (z) x 4001 (x) interior + 50-cell C-PML
   (4101 x 2101 stored), dx=dz=10 m, dt=0.8 ms, nt=6000, velocity = seeded
   Gaussian-smoothed noise (sigma = 20 cells) mapped to 1500-4500 m/s, 20 Hz
@@ -33,7 +45,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import AcousticShot, ViscoShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model
+from . import AcousticShot, ElasticShot, ViscoShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model
 
 PI_REF = 3.1415926
 
@@ -198,3 +210,45 @@ def fwi_801(nshot: int = 16, nt: int = 2500, scale: int = 1, seed: int = 6):
     shots = [AcousticShot(vp=bg, den=None, wavelet=w, src=(int(x), nb + 2), rec=(rx, rz), dx=10.0, dz=10.0, dt=dt,
                           radius=8, boundary="sponge", width=nb) for x in xs]
     return true, bg, shots
+
+
+def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
+    """``topo`` (optional, [501] metres, >= 0) replaces the flat topography:
+    itopoe = nint(topo / dz) + npzn (get_currshot_parameters, pub:269-272)."""
+    nxv, nzv, dx = 501, 401, 5.0
+    npxn = npxp = npzp = 201
+    npzn = 1
+    nxe = 501
+    nx, nz = nxe + npxn + npxp, nzv + npzn + npzp
+    vs_l = np.empty(nzv, np.float32)
+    vs_l[:101], vs_l[101:201], vs_l[201:301], vs_l[301:] = 500.0, 600.0, 700.0, 800.0
+    vsm = np.repeat(vs_l[None, :], nxv, 0)
+    model = {"vs": vsm, "vp": 2.0 * vsm, "rho": np.full((nxv, nzv), 2250.0, np.float32)}
+    tp = np.zeros(nxv, np.float32) if topo is None else np.asarray(topo, np.float32)
+    # get_currshot_parameters: window columns i = 1..nxe take model column
+    # i + ixmine (clipped), ixmine = 1 for this shot; rows pad by repetition.
+    ixmine = 1
+    cols = np.clip(np.arange(1, nxe + 1) + ixmine, 1, nxv) - 1
+    grids = {}
+    for k, m in model.items():
+        g = np.empty((nx, nz), np.float32)
+        g[npxn:npxn + nxe, npzn:npzn + nzv] = m[cols]
+        g[npxn:npxn + nxe, :npzn] = g[npxn:npxn + nxe, npzn:npzn + 1]
+        g[npxn:npxn + nxe, npzn + nzv:] = g[npxn:npxn + nxe, npzn + nzv - 1:npzn + nzv]
+        g[:npxn] = g[npxn]
+        g[npxn + nxe:] = g[npxn + nxe - 1]
+        grids[k] = g
+    topoe = np.empty(nx, np.float32)
+    topoe[npxn:npxn + nxe] = tp[cols]
+    topoe[:npxn] = topoe[npxn]
+    topoe[npxn + nxe:] = topoe[npxn + nxe - 1]
+    itopo = (np.floor(topoe / dx + 0.5).astype(np.int32) + npzn) - 1  # 0-based
+    isxe = 251 - ixmine + npxn + 1                  # spcalpa + get_currshot_parameters (1-based)
+    sx = isxe - 1
+    sz = 1 + (itopo[sx] + 1) - 1                   # isze = isz + itopoe(isxe), isz = 1
+    rx = np.arange(npxn, npxn + nxe, dtype=np.int32)
+    rz = (itopo[rx] + 1 + 1 - 1).astype(np.int32)  # itopo(i + npxn) + nrz0, nrz0 = 1
+    kw = dict(vp=grids["vp"], vs=grids["vs"], rho=grids["rho"], itopo=itopo, src=(sx, sz), rec=(rx, rz), nt=nt,
+              dx=dx, dz=dx, dt=1e-3, f0=25.0, sponge=(npxn, npxp, npzp), src_radius=50, delay=48)
+    kw.update(overrides)
+    return ElasticShot(**kw)

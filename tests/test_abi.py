@@ -53,3 +53,11 @@ def test_params_struct_layout():
     # float pml_r,pml_f0; int sx,sz,nrec; ptr rx,rz; int nsnap; ptr steps;
     # int snap_fields, arith
     assert ctypes.sizeof(sfd.AcousticParamsC) == 112  # sizeof(sfd_acoustic_params), x86-64
+
+
+def test_elastic_params_struct_layout():
+    import seismic_code_github_amd as sfd
+    C = sfd.ElasticParamsC
+    # sizeof / offsetof of sfd_elastic_params as gcc lays it out on x86-64
+    assert ctypes.sizeof(C) == 112
+    assert (C.itopo.offset, C.src_radius.offset, C.snap_steps.offset, C.snap_fields.offset) == (32, 64, 96, 104)
