@@ -147,6 +147,68 @@ def visco_leg(nt_err, nt, no_cpu):
     return out
 
 
+def elastic_leg(nt_err, nt, no_cpu):
+    """Elastic free surface on the reference program's own shot grid
+    (iso_2d_modelling_sin_test.par: 903 x 603 padded, nt = 3001 recorded + 48
+    delay steps): Mcell-updates/s over all stored cells, the algorithmic bytes
+    of the two step kernels, the error against the FP64 oracle on the first
+    `nt_err` recorded samples, and the FP32 oracle's rate on the same prefix."""
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    shot = W.elastic_par_903x603(nt=nt)
+    nx, nz = shot.vp.shape
+    nsteps = shot.ref_delay() + shot.nt
+    with sfd.ElasticPlan(shot) as pl:
+        pl.run()
+        ms = min(pl.run() for _ in range(3))
+        rvx, rvz, _ = pl.fetch()
+    assert np.isfinite(rvx).all() and np.isfinite(rvz).all()
+    sp = shot.sponge
+    strip_cells = (sp[0] + sp[1] + 2) * nz + (nx - sp[0] - sp[1] - 2) * (sp[2] + 1)
+    # Per cell and step: velocity kernel reads tau_xx, tau_zz, tau_xz, vx, vz,
+    # busg2e, busg1e and writes vx, vz (36 B); stress kernel reads vx, vz,
+    # tau_xx, tau_zz, tau_xz, lambmusg, lambsg, musge and writes the three
+    # stresses (44 B).  Per sponge-strip cell the x and z parts of all five
+    # fields are read and written too (80 B).
+    bytes_step = 80.0 * nx * nz + 80.0 * strip_cells
+    out = {"workload": f"elastic SSG free surface, 10th order, iso_2d_modelling_sin_test.par shot grid "
+                       f"{nx}x{nz} (501x401 model + 201-cell cosine sponges), dt=1 ms, nt={nt} "
+                       f"(+{shot.ref_delay()} delay steps), 25 Hz, rgau=50",
+           "cells": nx * nz, "strip_cells": strip_cells, "steps": nsteps, "ms_per_shot": round(ms, 2),
+           "Mcell_per_s": round(nx * nz * nsteps / ms / 1e3, 1),
+           "us_per_step": round(ms * 1e3 / nsteps, 2),
+           "bytes_per_step": bytes_step,
+           "achieved_GBs": round(bytes_step * nsteps / (ms / 1e3) / 1e9, 1),
+           "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4)}
+    tpath = os.path.join(ROOT, "profiles", "elastic_step_903x603_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            out["traffic"] = json.load(f).get("hbm_bytes_per_step")
+    if not no_cpu:
+        import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        short = W.elastic_par_903x603(nt=nt_err)
+        svx, svz, _ = sfd.elastic_forward(short)
+        args = dict(src=short.src, rec=short.rec, nsteps=short.ref_delay() + nt_err, delay=short.ref_delay(),
+                    dx=short.dx, dz=short.dz, dt=short.dt, f0=short.f0, sponge=short.sponge, rgau=short.src_radius,
+                    nthreads=cores)
+        o64 = oracle.elastic(short.vp, short.vs, short.rho, short.itopo, precision="f64", **args)
+        t0 = time.perf_counter()
+        o32 = oracle.elastic(short.vp, short.vs, short.rho, short.itopo, precision="f32", **args)
+        dt32 = time.perf_counter() - t0
+        for k, got in (("vx", svx), ("vz", svz)):
+            ref = o64["rec_" + k]
+            nrm = np.linalg.norm(ref)
+            out[f"fp64_rel_l2_{k}"] = float(np.linalg.norm(got - ref) / nrm)
+            out[f"fp32_oracle_vs_fp64_rel_l2_{k}"] = float(np.linalg.norm(o32["rec_" + k] - ref) / nrm)
+        out["error_sample"] = f"first {nt_err} recorded samples ({args['nsteps']} steps) of the same shot"
+        out["cpu_baseline"] = {"value": round(nx * nz * args["nsteps"] / dt32 / 1e6, 2), "unit": "Mcell-updates/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"{args['nsteps']} steps of the same shot, FP32 oracle (split fields as "
+                                         f"the reference stores them), OpenMP over x columns, {dt32:.1f} s"}
+    return out
+
+
 def gradient_leg(nt):
     """Adjoint gradient: 16 shots on 801 x 801 + 40-cell sponge, 10 Hz,
     checkpoints every 50 steps: forward + checkpointed recomputation + adjoint
@@ -195,7 +257,10 @@ def main():
     ap.add_argument("--no-visco", action="store_true")
     ap.add_argument("--visco-err-nt", type=int, default=40)
     ap.add_argument("--visco-nt", type=int, default=4000)
-    ap.add_argument("--only", choices=["visco", "grad", "batch"], default=None,
+    ap.add_argument("--no-elastic", action="store_true")
+    ap.add_argument("--elastic-err-nt", type=int, default=100)
+    ap.add_argument("--elastic-nt", type=int, default=3001)
+    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic"], default=None,
                     help="run one secondary leg alone (profiling) and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--grad-steps", type=int, default=2500)
@@ -224,7 +289,8 @@ def main():
     if args.only:
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
                "grad": lambda: gradient_leg(args.grad_steps),
-               "batch": lambda: batched_leg(args.batch_steps)}[args.only]
+               "batch": lambda: batched_leg(args.batch_steps),
+               "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)}[args.only]
         print(json.dumps(leg()), flush=True)
         return
 
@@ -289,6 +355,9 @@ def main():
     if not args.no_visco:
         log("visco leg ...")
         out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)
+    if not args.no_elastic:
+        log("elastic leg ...")
+        out["elastic"] = elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)
     if not args.no_grad:
         log("gradient leg ...")
         out["gradient"] = gradient_leg(args.grad_steps)

@@ -309,6 +309,9 @@ int sfd_elastic_plan_create(sfd_context *ctx, const sfd_elastic_params *p, const
  * receives the device time measured with HIP events on the plan's stream. */
 int sfd_elastic_plan_run(sfd_elastic_plan *plan, float *elapsed_ms);
 int sfd_elastic_plan_fetch(sfd_elastic_plan *plan, float *record_vx, float *record_vz, float *snaps);
+/* Tile width in columns (8, 16 or 32) and waves per workgroup (4, 8 or 16;
+ * not every pair is built); 0 = the default. */
+int sfd_elastic_plan_set_tiling(sfd_elastic_plan *plan, int tile_cols, int waves);
 int sfd_elastic_plan_destroy(sfd_elastic_plan *plan);
 
 #ifdef __cplusplus

@@ -139,6 +139,7 @@ _SIGNATURES = {
     "sfd_elastic_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_elastic_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP, _FP]),
     "sfd_elastic_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_elastic_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_window_model": (ctypes.c_int, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP]),
 }
@@ -678,6 +679,9 @@ class ElasticPlan:
         self.ctx.check(lib().sfd_elastic_plan_create(self.ctx.handle, ctypes.byref(self.p), _ptr(vp), _ptr(vs),
                                                      _ptr(rho), _ptr(w), ctypes.byref(h)), "sfd_elastic_plan_create")
         self.handle = h
+
+    def set_tiling(self, tile_cols: int = 0, waves: int = 0):
+        self.ctx.check(lib().sfd_elastic_plan_set_tiling(self.handle, tile_cols, waves), "elastic set_tiling")
 
     def run(self) -> float:
         ms = ctypes.c_float()

@@ -24,6 +24,9 @@ struct sfd_elastic_plan {
     int nrec = 0, nsnap = 0, snap_fields = 0, nsnap_fields = 0;
     std::vector<int> snap_steps;
     sfd::ElasticStepArgs args{};
+    int tx = 0, nw = 0;
+    void (*vel)(sfd::ElasticStepArgs) = nullptr;
+    void (*str)(sfd::ElasticStepArgs) = nullptr;
     sfd::DeviceBuffer arena;
     sfd::Field vx, vz, sxx, szz, sxz, parts, ghost, b2e, b1e, lm, ls, mu;
     sfd::DeviceBuffer itopo, srow, fxn, fxh, ratio, fzn, fzh, g, amp, rx, rz, rec_vx, rec_vz, snaps;
@@ -146,6 +149,47 @@ int validate(sfd_context* ctx, const sfd_elastic_params* p, const float* vp, con
     return SFD_OK;
 }
 
+using ElasticKernel = void (*)(ElasticStepArgs);
+
+// Default tiling, measured on MI355X for the 903 x 603 shot grid (us per
+// step, both kernels): 8 x 8 24.2, 8 x 4 25.9, 16 x 8 26.0, 16 x 16 29.5,
+// 32 x 16 32.3, 16 x 4 36.8, 32 x 8 38.5.  One column per wave and small
+// tiles win: the step is latency-bound and more workgroups hide it.
+constexpr int kElDefaultTX = 8, kElDefaultNW = 8;
+
+template <int TX, int NW>
+void el_pair(ElasticKernel* v, ElasticKernel* s) {
+    *v = elastic_velocity_kernel<TX, NW>;
+    *s = elastic_stress_kernel<TX, NW>;
+}
+
+// The instantiated tilings: tile columns x waves per workgroup.
+bool el_pick(int tx, int nw, ElasticKernel* v, ElasticKernel* s) {
+    switch (tx * 100 + nw) {
+    case 804: el_pair<8, 4>(v, s); return true;
+    case 808: el_pair<8, 8>(v, s); return true;
+    case 1604: el_pair<16, 4>(v, s); return true;
+    case 1608: el_pair<16, 8>(v, s); return true;
+    case 1616: el_pair<16, 16>(v, s); return true;
+    case 3208: el_pair<32, 8>(v, s); return true;
+    case 3216: el_pair<32, 16>(v, s); return true;
+    default: return false;
+    }
+}
+
+int el_set_tiling(sfd_elastic_plan* pl, int tx, int nw) {
+    ElasticKernel v, s;
+    if (!el_pick(tx, nw, &v, &s))
+        return fail(pl->ctx, SFD_EINVAL, "no elastic kernel for %d tile columns x %d waves", tx, nw);
+    pl->tx = tx;
+    pl->nw = nw;
+    pl->vel = v;
+    pl->str = s;
+    pl->args.ntz = (pl->nz + kElTZ - 1) / kElTZ;
+    pl->args.ntiles = pl->args.ntz * ((pl->nx + tx - 1) / tx);
+    return SFD_OK;
+}
+
 int elastic_zero(sfd_elastic_plan* pl) {
     sfd_context* ctx = pl->ctx;
     // Wavefields, strip parts and the pending surface values lie at the front
@@ -157,16 +201,16 @@ int elastic_zero(sfd_elastic_plan* pl) {
 
 int elastic_enqueue(sfd_elastic_plan* pl) {
     sfd_context* ctx = pl->ctx;
-    const dim3 grid((pl->nz + kElTZ - 1) / kElTZ, (pl->nx + kElTX - 1) / kElTX), block(64 * kElNW);
+    const dim3 grid(pl->args.ntiles), block(64 * pl->nw);
     ElasticStepArgs a = pl->args;
     size_t snap_next = 0;
     const size_t cell_bytes = (size_t)pl->nz * sizeof(float);
     for (int s = 0; s < pl->nsteps; ++s) {
         a.step = s;
         a.rec_col = (s >= pl->delay && pl->nrec > 0) ? s - pl->delay : -1;
-        hipLaunchKernelGGL(elastic_velocity_kernel, grid, block, 0, ctx->stream, a);
+        hipLaunchKernelGGL(pl->vel, grid, block, 0, ctx->stream, a);
         SFD_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(elastic_stress_kernel, grid, block, 0, ctx->stream, a);
+        hipLaunchKernelGGL(pl->str, grid, block, 0, ctx->stream, a);
         SFD_HIP(ctx, hipGetLastError());
         while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == s) {
             float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
@@ -236,7 +280,9 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
         if (p->snap_steps[s] < 0 || p->snap_steps[s] >= pl->nsteps || (s > 0 && p->snap_steps[s] <= p->snap_steps[s - 1]))
             return bail(fail(ctx, SFD_EINVAL, "snapshot steps must be ascending and in [0, delay + nt)"));
     pl->pitch = (nz + kElTZ - 1) / kElTZ * kElTZ + 2 * kElPad;
-    pl->ncols = (nx + kElTX - 1) / kElTX * kElTX + 2 * kElPad;
+    // Columns -kElPad .. nx + kElTXMax + kElPad - 1: the widest tile's last
+    // halo column stays inside.
+    pl->ncols = nx + kElTXMax + 2 * kElPad;
 
     // ---- model ----
     std::vector<float> st[5];
@@ -436,12 +482,19 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
     a.rec_vx = pl->rec_vx.as<float>();
     a.rec_vz = pl->rec_vz.as<float>();
 
+    rc = el_set_tiling(pl, kElDefaultTX, kElDefaultNW);
+    if (rc) return bail(rc);
     rc = elastic_zero(pl);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(sm);
     if (e != hipSuccess) return bail(hip_check(ctx, e, "plan setup"));
     *out = pl;
     return SFD_OK;
+}
+
+int sfd_elastic_plan_set_tiling(sfd_elastic_plan* pl, int tile_cols, int waves) {
+    if (!pl) return SFD_EINVAL;
+    return el_set_tiling(pl, tile_cols ? tile_cols : kElDefaultTX, waves ? waves : kElDefaultNW);
 }
 
 int sfd_elastic_plan_run(sfd_elastic_plan* pl, float* elapsed_ms) {

@@ -7,10 +7,11 @@
 // derivatives are 10th order everywhere (calux); z derivatives drop to orders
 // 8, 6, 4, 2 on the rows next to the column's surface row t (caluz).
 //
-// One workgroup = 4 waves over a tile of 64 rows x 16 columns; each wave owns
-// 4 columns, one row per lane.  The tile and its 5-cell halo of every field a
-// stencil reads is staged in LDS once, so each field is read from HBM about
-// once per launch (halo re-reads hit L2).
+// One workgroup = NW waves over a tile of 64 rows x TX columns; each wave
+// owns TX / NW columns, one row per lane.  The tile and its 5-cell halo of
+// every field a stencil reads is staged in LDS once, so each field is read
+// from HBM about once per launch; tiles are dealt to the XCDs in bands of
+// whole x-columns, so the halo re-reads hit the XCD's own L2.
 //
 // Cells are updated unsplit, T += dt/h pa D, except in the sponge strips
 // (columns of the left / right strips, rows of the bottom strip), where the
@@ -30,11 +31,12 @@
 #include <climits>
 #include <cstdint>
 
+#include "device_util.h"
+
 namespace sfd {
 
 constexpr int kElTZ = 64;   // tile rows (one per lane)
-constexpr int kElTX = 16;   // tile columns
-constexpr int kElNW = 4;    // waves per workgroup
+constexpr int kElTXMax = 32;  // widest tile (columns); the field border covers it
 constexpr int kElH = 5;     // stencil half-width
 constexpr int kElPad = 8;   // zero rows / columns around the grid in every field
 
@@ -50,6 +52,7 @@ struct ElasticStepArgs {
     float* ghost;                           // per column: vz(t - 1) for the next step
     float* parts;                           // [10][nstrip] x / z parts of vx, vz, sxx, szz, sxz
     int nstrip, nl, nr, kb0, nb;
+    int ntz, ntiles;                        // z tiles, all tiles
     float cx, cz;                           // dt/dx, dt/dz
     float a10[5], a8[4], a6[3], a4[2];      // Taylor tables (module:9-26); a20 = 1
     int sx0, sx1, sz0, sz1, src_xx;         // source box, and whether tau_xx takes it
@@ -106,35 +109,54 @@ __device__ __forceinline__ float el_xsum(const ElasticStepArgs& a, const float* 
     return s;
 }
 
-// Stages columns [x0 - hx, x0 + kElTX + hx) x rows [z0 - hz, z0 + kElTZ + hz)
-// of a padded field into lds[c][r].
-template <int HX, int HZ>
+// Stages columns [x0 - HX, x0 + TX + HX) x rows [z0 - HZ, z0 + kElTZ + HZ)
+// of a padded field into lds[c][r] (a fixed, unrolled count per thread, so
+// all loads are in flight before the first LDS store waits).
+template <int TX, int NW, int HX, int HZ>
 __device__ __forceinline__ void el_stage(float* lds, const float* f, int pitch, int x0, int z0) {
-    constexpr int W = kElTX + 2 * HX, Hh = kElTZ + 2 * HZ;
-    for (int e = threadIdx.x; e < W * Hh; e += 64 * kElNW) {
+    constexpr int W = TX + 2 * HX, Hh = kElTZ + 2 * HZ, N = W * Hh, T = 64 * NW;
+    constexpr int ITER = (N + T - 1) / T;
+    float v[ITER];
+#pragma unroll
+    for (int q = 0; q < ITER; ++q) {
+        const int e = threadIdx.x + q * T;
         const int c = e / Hh, r = e - c * Hh;
-        lds[e] = f[(ptrdiff_t)(x0 - HX + c) * pitch + (z0 - HZ + r)];
+        v[q] = (e < N) ? f[(ptrdiff_t)(x0 - HX + c) * pitch + (z0 - HZ + r)] : 0.f;
     }
+#pragma unroll
+    for (int q = 0; q < ITER; ++q) {
+        const int e = threadIdx.x + q * T;
+        if (e < N) lds[e] = v[q];
+    }
+}
+
+// Tile of this workgroup: XCD-aware order over (x band, z tile), z fastest.
+__device__ __forceinline__ void el_tile(const ElasticStepArgs& a, int tx, int* z0, int* x0) {
+    const int t = xcd_tile(blockIdx.x, a.ntiles);
+    *z0 = (t % a.ntz) * kElTZ;
+    *x0 = (t / a.ntz) * tx;
 }
 
 // Velocity half-step (pri:119-151): vx from tau_xx, tau_xz; vz from tau_xz,
 // tau_zz; the pending vz(t - 1) of every column.
-__global__ __launch_bounds__(64 * kElNW) void elastic_velocity_kernel(ElasticStepArgs a) {
+template <int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepArgs a) {
     constexpr int HX = kElH, HZ = kElH;
-    constexpr int WX = kElTX + 2 * HX, HZZ = kElTZ + 2 * HZ;
+    constexpr int WX = TX + 2 * HX, HZZ = kElTZ + 2 * HZ;
     __shared__ float s_xx[WX * kElTZ];
     __shared__ float s_xz[WX * HZZ];
-    __shared__ float s_zz[kElTX * HZZ];
-    const int z0 = blockIdx.x * kElTZ, x0 = blockIdx.y * kElTX;
-    el_stage<HX, 0>(s_xx, a.sxx, a.pitch, x0, z0);
-    el_stage<HX, HZ>(s_xz, a.sxz, a.pitch, x0, z0);
-    el_stage<0, HZ>(s_zz, a.szz, a.pitch, x0, z0);
+    __shared__ float s_zz[TX * HZZ];
+    int z0, x0;
+    el_tile(a, TX, &z0, &x0);
+    el_stage<TX, NW, HX, 0>(s_xx, a.sxx, a.pitch, x0, z0);
+    el_stage<TX, NW, HX, HZ>(s_xz, a.sxz, a.pitch, x0, z0);
+    el_stage<TX, NW, 0, HZ>(s_zz, a.szz, a.pitch, x0, z0);
     __syncthreads();
     const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = z0 + tz;
     if (k >= a.nz) return;
-    for (int j = 0; j < kElTX / kElNW; ++j) {
-        const int cl = wv + j * kElNW, col = x0 + cl;
+    for (int j = 0; j < TX / NW; ++j) {
+        const int cl = wv + j * NW, col = x0 + cl;
         if (col >= a.nx) break;
         const int t = a.itopo[col];
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
@@ -177,29 +199,31 @@ __global__ __launch_bounds__(64 * kElNW) void elastic_velocity_kernel(ElasticSte
 }
 
 // Stress half-step (pri:153-196) and the step's receiver samples.
-__global__ __launch_bounds__(64 * kElNW) void elastic_stress_kernel(ElasticStepArgs a) {
+template <int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs a) {
     constexpr int HX = kElH, HZ = kElH;
-    constexpr int WX = kElTX + 2 * HX, HZZ = kElTZ + 2 * HZ;
+    constexpr int WX = TX + 2 * HX, HZZ = kElTZ + 2 * HZ;
     __shared__ float s_vx[WX * HZZ];
     __shared__ float s_vz[WX * HZZ];
-    const int z0 = blockIdx.x * kElTZ, x0 = blockIdx.y * kElTX;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && a.rec_col >= 0) {
-        for (int r = threadIdx.x; r < a.nrec; r += 64 * kElNW) {
+    int z0, x0;
+    el_tile(a, TX, &z0, &x0);
+    if (blockIdx.x == 0 && a.rec_col >= 0) {
+        for (int r = threadIdx.x; r < a.nrec; r += 64 * NW) {
             const ptrdiff_t o = (ptrdiff_t)a.rx[r] * a.pitch + a.rz[r];
             a.rec_vx[(size_t)r * a.nrt + a.rec_col] = a.vx[o];
             a.rec_vz[(size_t)r * a.nrt + a.rec_col] = a.vz[o];
         }
     }
-    el_stage<HX, HZ>(s_vx, a.vx, a.pitch, x0, z0);
-    el_stage<HX, HZ>(s_vz, a.vz, a.pitch, x0, z0);
+    el_stage<TX, NW, HX, HZ>(s_vx, a.vx, a.pitch, x0, z0);
+    el_stage<TX, NW, HX, HZ>(s_vz, a.vz, a.pitch, x0, z0);
     __syncthreads();
     const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = z0 + tz;
     if (k >= a.nz) return;
     const bool in_box_z = k >= a.sz0 && k < a.sz1;
     const float w = a.amp[a.step];
-    for (int j = 0; j < kElTX / kElNW; ++j) {
-        const int cl = wv + j * kElNW, col = x0 + cl;
+    for (int j = 0; j < TX / NW; ++j) {
+        const int cl = wv + j * NW, col = x0 + cl;
         if (col >= a.nx) break;
         const int t = a.itopo[col];
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
