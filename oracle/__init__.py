@@ -357,7 +357,7 @@ class _EArgs(ctypes.Structure):
         ("wavelet", ctypes.POINTER(ctypes.c_float)),
         ("nrec", ctypes.c_int), ("rx", ctypes.POINTER(ctypes.c_int)), ("rz", ctypes.POINTER(ctypes.c_int)),
         ("nsnap", ctypes.c_int), ("snap_steps", ctypes.POINTER(ctypes.c_int)),
-        ("nthreads", ctypes.c_int),
+        ("nthreads", ctypes.c_int), ("mode", ctypes.c_int), ("npzn", ctypes.c_int),
     ]
 
 
@@ -372,6 +372,7 @@ def _elib():
         L.oracle_ssg_deriv_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, ip, dp]
         L.oracle_elastic_stagger_f32.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, fp, ip, fp]
         L.oracle_elastic_stagger_f64.argtypes = [ctypes.c_int, ctypes.c_int, fp, fp, fp, ip, dp]
+        L.oracle_elastic_smooth_f32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, fp]
         L._elastic_bound = True
     return L
 
@@ -409,12 +410,24 @@ def elastic_stagger(vp, vs, rho, itopo, precision="f32"):
     return dict(zip(("busg2e", "busg1e", "lambmusg", "lambsg", "musge"), out))
 
 
+def elastic_smooth(u, itopo, rgau):
+    """modipapml's in-place Gaussian smoothing of rows 0..itopo (FP32)."""
+    u = np.array(u, dtype=np.float32, order="C")
+    nx, nz = u.shape
+    it = np.ascontiguousarray(itopo, dtype=np.int32)
+    rc = _elib().oracle_elastic_smooth_f32(nx, nz, rgau, _ptr(it, ctypes.c_int), _ptr(u))
+    if rc != 0:
+        raise ValueError(f"oracle_elastic_smooth_f32 returned {rc}")
+    return u
+
+
 def elastic(vp, vs, rho, itopo, *, src, rec, nsteps, delay=0, dx=5.0, dz=5.0, dt=1e-3, f0=25.0,
             sponge=(40, 40, 40), sponge_r=1e-3, rgau=50, wavelet=None, snap_steps=(), precision="f32",
-            nthreads=1):
-    """Run the elastic free-surface oracle (oracle/elastic_body.h).  Fields
-    [nx, nz] (z fastest); itopo [nx] 0-based surface rows; sponge = (npxn,
-    npxp, npzp).  Returns dict(rec_vx, rec_vz [nrec, nsteps - delay],
+            nthreads=1, mode="fsbc", npzn=0):
+    """Run the elastic oracle (oracle/elastic_body.h), mode "fsbc" (free
+    surface) or "abc" (absorbing top of npzn + 1 rows above the surface).
+    Fields [nx, nz] (z fastest); itopo [nx] 0-based surface rows; sponge =
+    (npxn, npxp, npzp).  Returns dict(rec_vx, rec_vz [nrec, nsteps - delay],
     snaps [nsnap, 5, nx, nz] = vx, vz, sxx, szz, sxz)."""
     L = _elib()
     f32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
@@ -429,7 +442,8 @@ def elastic(vp, vs, rho, itopo, *, src, rec, nsteps, delay=0, dx=5.0, dz=5.0, dt
         assert w.shape[0] >= nsteps
     args = _EArgs(nx, nz, nsteps, delay, dx, dz, dt, f0, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(it, ctypes.c_int),
                   *sponge, sponge_r, int(src[0]), int(src[1]), rgau, _ptr(w), rx.size, _ptr(rx, ctypes.c_int),
-                  _ptr(rz, ctypes.c_int), steps.size, _ptr(steps, ctypes.c_int), nthreads)
+                  _ptr(rz, ctypes.c_int), steps.size, _ptr(steps, ctypes.c_int), nthreads,
+                  {"fsbc": 0, "abc": 1}[mode], npzn)
     dtype, ct = (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
     nrt = max(nsteps - delay, 0)
     rvx = np.zeros((rx.size, nrt), dtype=dtype)

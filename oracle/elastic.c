@@ -66,7 +66,7 @@ int oracle_ssg_deriv_f64(int axis, int flag, int nx, int nz, const double *u, co
     double tab[5][5];
     el_tables_f64(tab);
     if (axis == 0) el_calux_f64(flag, nx, nz, 1.0, 1.0, uh, acc, pa, itopo, tab[4], 1);
-    else el_caluz_f64(flag, nx, nz, 1.0, 1.0, uh, acc, pa, itopo, tab, 1);
+    else el_caluz_f64(flag, nx, nz, 1.0, 1.0, uh, acc, pa, itopo, itopo, 0, tab, 1);
     for (int i = 0; i < nx; ++i) memcpy(out + (size_t)i * nz, acc + (size_t)(i + Hh) * nzh + Hh, (size_t)nz * sizeof(double));
     free(uh);
     free(acc);
@@ -92,4 +92,28 @@ int oracle_elastic_stagger_f64(int nx, int nz, const float *vp, const float *vs,
     for (int i = 0; i < nx; ++i)
         if (itopo[i] < 0 || itopo[i] >= nz) return -1;
     return el_stagger_f64(nx, nz, vp, vs, rho, itopo, out);
+}
+
+/* gauss_filter_2d (pub:592-634) alone, in place on u [nx][nz]: rows 0 .. itopo(i)
+ * of every column, serial order, window 2 rgau - 1, sigma 2 (normalised). */
+int oracle_elastic_smooth_f32(int nx, int nz, int rgau, const int *itopo, float *u)
+{
+    if (nx < 1 || nz < 1 || rgau < 1) return -1;
+    for (int i = 0; i < nx; ++i)
+        if (itopo[i] < 0 || itopo[i] >= nz) return -1;
+    const int ng = 2 * rgau - 1, cen = (ng + 1) / 2;
+    float *g = malloc((size_t)ng * ng * sizeof(float));
+    if (!g) return -2;
+    const float PI = 3.1415926f, sigma2 = 4.0f;
+    float sum = 0;
+    for (int j = 1; j <= ng; ++j)
+        for (int i = 1; i <= ng; ++i) {
+            const float dis = (float)(abs(j - cen) * abs(j - cen) + abs(i - cen) * abs(i - cen));
+            g[(size_t)(j - 1) * ng + (i - 1)] = 1 / (2 * PI * sigma2) * expf(-dis / (2 * sigma2));
+        }
+    for (int c = 0; c < ng * ng; ++c) sum = sum + g[c];
+    for (int c = 0; c < ng * ng; ++c) g[c] = g[c] / sum;
+    el_gauss_smooth_f32(nx, nz, rgau, itopo, g, u);
+    free(g);
+    return 0;
 }

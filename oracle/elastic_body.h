@@ -49,14 +49,15 @@ static void FN(el_tables)(REAL t[5][5])
     t[4][4] = (REAL)(35.0f / 294912.0f);
 }
 
-/* calux (pub:1026-1084): ux += (dt/dx) pa sum a10n (u(k,i+n+s) - u(k,i-n-1+s)),
- * s = xflag, rows k >= itopo(i). */
+/* calux (pub:1026-1084) / caluxabs (pub:1088-1157): ux += (dt/dx) pa
+ * sum a10n (u(k,i+n+s) - u(k,i-n-1+s)), s = xflag, rows k >= kstart(i)
+ * (itopo in 'fsbc' mode, itopo - npzn in 'abc' mode). */
 static void FN(el_calux)(int xflag, int nx, int nz, REAL dx, REAL dt, const REAL *u, REAL *ux, const REAL *pa,
-                         const int *itopo, const REAL a10[5], int nth)
+                         const int *kstart, const REAL a10[5], int nth)
 {
 #pragma omp parallel for num_threads(nth) schedule(static) if (nth > 1)
     for (int i = 0; i < nx; ++i)
-        for (int k = itopo[i]; k < nz; ++k) {
+        for (int k = kstart[i]; k < nz; ++k) {
             REAL s = 0;
             for (int n = 0; n < 5; ++n) {
                 const REAL d = UF(u, i + n + xflag, k) - UF(u, i - n - 1 + xflag, k);
@@ -68,16 +69,18 @@ static void FN(el_calux)(int xflag, int nx, int nz, REAL dx, REAL dt, const REAL
 
 /* caluz (pub:1160-1337): zflag 0 uses u(k+n) - u(k-n-1), 10th order from
  * itopo+5, then 8th, 6th, 4th, 2nd, 2nd at itopo+4 .. itopo; zflag 1 uses
- * u(k+n+1) - u(k-n), 10th order from itopo+4, then 8th, 6th, 4th, 2nd. */
+ * u(k+n+1) - u(k-n), 10th order from itopo+4, then 8th, 6th, 4th, 2nd.
+ * caluzabs (pub:1341-1406, full != 0): 10th order on every row >= kstart. */
 static void FN(el_caluz)(int zflag, int nx, int nz, REAL dz, REAL dt, const REAL *u, REAL *uz, const REAL *pa,
-                         const int *itopo, REAL tab[5][5], int nth)
+                         const int *kstart, const int *itopo, int full, REAL tab[5][5], int nth)
 {
 #pragma omp parallel for num_threads(nth) schedule(static) if (nth > 1)
     for (int i = 0; i < nx; ++i)
-        for (int k = itopo[i]; k < nz; ++k) {
+        for (int k = kstart[i]; k < nz; ++k) {
             const int d = k - itopo[i];
             int ord; /* table index: 4 = 10th ... 0 = 2nd */
-            if (zflag == 0) ord = d >= 5 ? 4 : (d == 4 ? 3 : d == 3 ? 2 : d == 2 ? 1 : 0);
+            if (full) ord = 4;
+            else if (zflag == 0) ord = d >= 5 ? 4 : (d == 4 ? 3 : d == 3 ? 2 : d == 2 ? 1 : 0);
             else ord = d >= 4 ? 4 : (d == 3 ? 3 : d == 2 ? 2 : d == 1 ? 1 : 0);
             REAL s = 0;
             for (int n = 0; n <= ord; ++n) {
@@ -88,18 +91,47 @@ static void FN(el_caluz)(int zflag, int nx, int nz, REAL dz, REAL dt, const REAL
         }
 }
 
-/* calpmlbclrb (pub:895-934): x parts damped in the left/right strips for
- * rows k >= itopo (the right strip uses the left column's itopo, as
- * written), z parts in the bottom strip. */
-static void FN(el_sponge)(REAL *ux, REAL *uz, const REAL *txn, const REAL *txp, const REAL *tzp, int npxn, int npxp,
-                          int npzp, int nx, int nz, const int *itopo)
+/* calpmlbclrb (pub:895-934) / calpmlbclrtb (pub:851-892): x parts damped in
+ * the left/right strips for rows k >= kstart (the right strip uses the
+ * kstart of the column counted from the left edge, as written); z parts in
+ * the bottom strip and, in 'abc' mode (tzn != NULL), in the npzn + 1 rows
+ * itopo - npzn .. itopo of every column by tzn(k - (itopo - npzn)). */
+static void FN(el_sponge)(REAL *ux, REAL *uz, const REAL *txn, const REAL *txp, const REAL *tzn, const REAL *tzp,
+                          int npxn, int npxp, int npzn, int npzp, int nx, int nz, const int *kstart, const int *itopo)
 {
     for (int i = 0; i <= npxn; ++i)
-        for (int k = itopo[i]; k < nz; ++k) UF(ux, i, k) = UF(ux, i, k) * txn[i];
+        for (int k = kstart[i]; k < nz; ++k) UF(ux, i, k) = UF(ux, i, k) * txn[i];
     for (int i = 0; i <= npxp; ++i)
-        for (int k = itopo[i]; k < nz; ++k) UF(ux, nx - 1 - i, k) = UF(ux, nx - 1 - i, k) * txp[i];
+        for (int k = kstart[i]; k < nz; ++k) UF(ux, nx - 1 - i, k) = UF(ux, nx - 1 - i, k) * txp[i];
+    if (tzn)
+        for (int i = 0; i < nx; ++i)
+            for (int k = itopo[i] - npzn; k <= itopo[i]; ++k)
+                if (k >= 0) UF(uz, i, k) = UF(uz, i, k) * tzn[k - (itopo[i] - npzn)];
     for (int i = 0; i < nx; ++i)
         for (int k = 0; k <= npzp; ++k) UF(uz, i, nz - 1 - k) = UF(uz, i, nz - 1 - k) * tzp[k];
+}
+
+/* gauss_filter_2d (pub:592-634) as one thread runs it: every cell of rows
+ * 0 .. itopo(i) replaced in place, column by column, by the gc2d-weighted sum
+ * of its (2 rgau - 1)^2 window clipped to the grid (not renormalised), so
+ * cells already smoothed feed the later ones.  (The reference's OpenMP loop
+ * over columns races on the shared neighbours; the serial order is the
+ * deterministic reading.) */
+static void FN(el_gauss_smooth)(int nx, int nz, int rgau, const int *itopo, const REAL *g2c, REAL *u)
+{
+    const int ng = 2 * rgau - 1;
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k <= itopo[i]; ++k) {
+            REAL sum = 0;
+            int idx2 = 1;
+            for (int n = i - (rgau - 1); n <= i + (rgau - 1); ++n, ++idx2) {
+                int idx1 = 1;
+                for (int m = k - (rgau - 1); m <= k + (rgau - 1); ++m, ++idx1)
+                    if (n >= 0 && n < nx && m >= 0 && m < nz && idx2 <= ng && idx1 <= ng)
+                        sum = sum + AT(u, n, m) * g2c[(size_t)(idx2 - 1) * ng + (idx1 - 1)];
+            }
+            AT(u, i, k) = sum;
+        }
 }
 
 /* csponge (pub:796-847) for one side, outermost cell first. */
@@ -213,6 +245,8 @@ int FN(oracle_elastic)(const oracle_elastic_args *a, REAL *rec_vx, REAL *rec_vz,
     if (nx < 12 || nz < 12 || a->nsteps < 0 || a->rgau < 1) return -1;
     if (a->npxn < 0 || a->npxp < 0 || a->npzp < 0 || a->npxn + 1 > nx || a->npxp + 1 > nx || a->npzp + 1 > nz)
         return -1;
+    if (a->mode != 0 && a->mode != 1) return -1;
+    if (a->mode == 1 && (a->npzn < 1 || a->npzn + 1 > nz)) return -1;
     if (a->isx < 0 || a->isx >= nx || a->isz < 0 || a->isz >= nz) return -1;
     for (int i = 0; i < nx; ++i)
         if (a->itopo[i] < 0 || a->itopo[i] > nz - 6) return -1;
@@ -223,6 +257,7 @@ int FN(oracle_elastic)(const oracle_elastic_args *a, REAL *rec_vx, REAL *rec_vz,
     int rc = -2;
     REAL *buf = calloc(19 * nh, sizeof(REAL));
     REAL *g2c = NULL, *spt = NULL;
+    int *kst = NULL;
     if (!buf) return -2;
     REAL *vx = buf, *vz = buf + nh, *sxx = buf + 2 * nh, *szz = buf + 3 * nh, *sxz = buf + 4 * nh;
     REAL *vxx = buf + 5 * nh, *vxz = buf + 6 * nh, *vzx = buf + 7 * nh, *vzz = buf + 8 * nh;
@@ -239,41 +274,14 @@ int FN(oracle_elastic)(const oracle_elastic_args *a, REAL *rec_vx, REAL *rec_vz,
 #endif
     const REAL PI = (REAL)3.1415926f; /* the module constant pi (module:4) */
 
-    /* ---- substagger (pub:323-547) ---- */
-    {
-        REAL *st = malloc(5 * n * sizeof(REAL));
-        musge = calloc(nh, sizeof(REAL));
-        if (!st || !musge) {
-            free(st);
-            goto done;
-        }
-        if (FN(el_stagger)(nx, nz, a->vp, a->vs, a->rho, itopo, st) != 0) {
-            free(st);
-            rc = -1;
-            goto done;
-        }
-        REAL *dst[5] = {busg2e, busg1e, lambmusg, lambsg, musge};
-        for (int q = 0; q < 5; ++q)
-            for (int i = 0; i < nx; ++i)
-                for (int k = 0; k < nz; ++k) UF(dst[q], i, k) = AT(st + (size_t)q * n, i, k);
-        free(st);
-    }
+    const int abc = a->mode == 1, npzn = abc ? a->npzn : 0;
+    kst = malloc((size_t)nx * sizeof(int));
+    if (!kst) goto done;
+    /* First updated row of every column: the surface ('fsbc'), or npzn rows
+     * above it ('abc'; row -1 of the split parts is never summed). */
+    for (int i = 0; i < nx; ++i) kst[i] = abc ? (itopo[i] - npzn > 0 ? itopo[i] - npzn : 0) : itopo[i];
 
-
-    REAL tab[5][5];
-    FN(el_tables)(tab);
-    const REAL *a10 = tab[4];
-
-    /* csponge: tables outermost first (the top ones are not used in this mode). */
-    spt = malloc((size_t)2 * ((a->npxn + 1) + (a->npxp + 1) + (a->npzp + 1)) * sizeof(REAL));
-    if (!spt) goto done;
-    REAL *spgxn = spt, *spg1xn = spgxn + a->npxn + 1, *spgxp = spg1xn + a->npxn + 1, *spg2xp = spgxp + a->npxp + 1;
-    REAL *spgzp = spg2xp + a->npxp + 1, *spg2zp = spgzp + a->npzp + 1;
-    FN(el_csponge)(a->npxn, dx, (REAL)a->sponge_r, spgxn, spg1xn, -1);
-    FN(el_csponge)(a->npxp, dx, (REAL)a->sponge_r, spgxp, spg2xp, +1);
-    FN(el_csponge)(a->npzp, dz, (REAL)a->sponge_r, spgzp, spg2zp, +1);
-
-    /* get_gauss_filter_coe(gc2d, 2.0, 2 rgau - 1, 2 rgau - 1). */
+    /* get_gauss_filter_coe(gc2d, 2.0, 2 rgau - 1, 2 rgau - 1) (pub:1470-1501). */
     const int ng = 2 * a->rgau - 1;
     g2c = malloc((size_t)ng * ng * sizeof(REAL));
     if (!g2c) goto done;
@@ -294,6 +302,45 @@ int FN(oracle_elastic)(const oracle_elastic_args *a, REAL *rec_vx, REAL *rec_vz,
             for (int i = 1; i <= ng; ++i) sum = sum + g2c[(size_t)(j - 1) * ng + (i - 1)];
         for (int c = 0; c < ng * ng; ++c) g2c[c] = g2c[c] / sum;
     }
+
+    /* ---- substagger (pub:323-547), then in 'abc' mode modipapml
+     * (pub:549-590): the parameters on and above the surface smoothed ---- */
+    {
+        REAL *st = malloc(5 * n * sizeof(REAL));
+        musge = calloc(nh, sizeof(REAL));
+        if (!st || !musge) {
+            free(st);
+            goto done;
+        }
+        if (FN(el_stagger)(nx, nz, a->vp, a->vs, a->rho, itopo, st) != 0) {
+            free(st);
+            rc = -1;
+            goto done;
+        }
+        if (abc)
+            for (int q = 0; q < 5; ++q) FN(el_gauss_smooth)(nx, nz, a->rgau, itopo, g2c, st + (size_t)q * n);
+        REAL *dst[5] = {busg2e, busg1e, lambmusg, lambsg, musge};
+        for (int q = 0; q < 5; ++q)
+            for (int i = 0; i < nx; ++i)
+                for (int k = 0; k < nz; ++k) UF(dst[q], i, k) = AT(st + (size_t)q * n, i, k);
+        free(st);
+    }
+
+    REAL tab[5][5];
+    FN(el_tables)(tab);
+    const REAL *a10 = tab[4];
+
+    /* csponge (pub:796-847): tables outermost first. */
+    spt = malloc((size_t)2 * ((a->npxn + 1) + (a->npxp + 1) + (a->npzp + 1) + (npzn + 1)) * sizeof(REAL));
+    if (!spt) goto done;
+    REAL *spgxn = spt, *spg1xn = spgxn + a->npxn + 1, *spgxp = spg1xn + a->npxn + 1, *spg2xp = spgxp + a->npxp + 1;
+    REAL *spgzp = spg2xp + a->npxp + 1, *spg2zp = spgzp + a->npzp + 1;
+    REAL *spgzn = spg2zp + a->npzp + 1, *spg1zn = spgzn + npzn + 1;
+    FN(el_csponge)(a->npxn, dx, (REAL)a->sponge_r, spgxn, spg1xn, -1);
+    FN(el_csponge)(a->npxp, dx, (REAL)a->sponge_r, spgxp, spg2xp, +1);
+    FN(el_csponge)(a->npzp, dz, (REAL)a->sponge_r, spgzp, spg2zp, +1);
+    FN(el_csponge)(npzn, dz, (REAL)a->sponge_r, spgzn, spg1zn, -1);
+    const REAL *tzn = abc ? spgzn : NULL, *tzn1 = abc ? spg1zn : NULL;
     const int isxe = a->isx + 1, isze = a->isz + 1; /* Fortran indices */
 
     const int nrec_t = a->nsteps - a->delay;
@@ -326,58 +373,61 @@ int FN(oracle_elastic)(const oracle_elastic_args *a, REAL *rec_vx, REAL *rec_vz,
         }                                                                                                  \
     } while (0)
 
-        /* ---- velocities (pri:119-144) ---- */
-        FN(el_calux)(0, nx, nz, dx, dt, sxx, vxx, busg2e, itopo, a10, nth);
-        FN(el_caluz)(0, nx, nz, dz, dt, sxz, vxz, busg2e, itopo, tab, nth);
-        FN(el_sponge)(vxx, vxz, spgxn, spgxp, spgzp, a->npxn, a->npxp, a->npzp, nx, nz, itopo);
+        /* ---- velocities (pri:119-144; 'abc': pri:300-318) ---- */
+        FN(el_calux)(0, nx, nz, dx, dt, sxx, vxx, busg2e, kst, a10, nth);
+        FN(el_caluz)(0, nx, nz, dz, dt, sxz, vxz, busg2e, kst, itopo, abc, tab, nth);
+        FN(el_sponge)(vxx, vxz, spgxn, spgxp, tzn, spgzp, a->npxn, a->npxp, npzn, a->npzp, nx, nz, kst, itopo);
         for (int i = 0; i < nx; ++i)
             for (int k = 0; k < nz; ++k) UF(vx, i, k) = UF(vxx, i, k) + UF(vxz, i, k);
-        FN(el_calux)(1, nx, nz, dx, dt, sxz, vzx, busg1e, itopo, a10, nth);
-        FN(el_caluz)(1, nx, nz, dz, dt, szz, vzz, busg1e, itopo, tab, nth);
-        FN(el_sponge)(vzx, vzz, spg1xn, spg2xp, spg2zp, a->npxn, a->npxp, a->npzp, nx, nz, itopo);
+        FN(el_calux)(1, nx, nz, dx, dt, sxz, vzx, busg1e, kst, a10, nth);
+        FN(el_caluz)(1, nx, nz, dz, dt, szz, vzz, busg1e, kst, itopo, abc, tab, nth);
+        FN(el_sponge)(vzx, vzz, spg1xn, spg2xp, tzn1, spg2zp, a->npxn, a->npxp, npzn, a->npzp, nx, nz, kst, itopo);
         for (int i = 0; i < nx; ++i)
             for (int k = 0; k < nz; ++k) UF(vz, i, k) = UF(vzx, i, k) + UF(vzz, i, k);
         /* calfsbcv on both pairs of split parts (pri:147-151): vz above the
          * surface for the next step's sums. */
-        for (int pass = 0; pass < 2; ++pass) {
-            REAL *px = pass ? vxz : vxx, *pz = pass ? vzz : vzx;
-            for (int ix = 0; ix < nx - 1; ++ix) {
-                const int izh = itopo[ix];
-                UF(pz, ix, izh - 1) = UF(pz, ix, izh) + (UF(lambsg, ix, izh) / UF(lambmusg, ix, izh)) *
-                                                          (UF(px, ix + 1, izh) - UF(px, ix, izh));
+        if (!abc)
+            for (int pass = 0; pass < 2; ++pass) {
+                REAL *px = pass ? vxz : vxx, *pz = pass ? vzz : vzx;
+                for (int ix = 0; ix < nx - 1; ++ix) {
+                    const int izh = itopo[ix];
+                    UF(pz, ix, izh - 1) = UF(pz, ix, izh) + (UF(lambsg, ix, izh) / UF(lambmusg, ix, izh)) *
+                                                              (UF(px, ix + 1, izh) - UF(px, ix, izh));
+                }
             }
-        }
 
-        /* ---- tau_xx (pri:156-170) ---- */
-        FN(el_calux)(1, nx, nz, dx, dt, vx, sxxx, lambmusg, itopo, a10, nth);
-        FN(el_caluz)(0, nx, nz, dz, dt, vz, sxxz, lambsg, itopo, tab, nth);
-        FN(el_sponge)(sxxx, sxxz, spg1xn, spg2xp, spgzp, a->npxn, a->npxp, a->npzp, nx, nz, itopo);
-        if (isze - 1 > itopo[isxe - 1]) ADDSRC(sxxx);
+        /* ---- tau_xx (pri:156-170; 'abc': pri:322-331, source always) ---- */
+        FN(el_calux)(1, nx, nz, dx, dt, vx, sxxx, lambmusg, kst, a10, nth);
+        FN(el_caluz)(0, nx, nz, dz, dt, vz, sxxz, lambsg, kst, itopo, abc, tab, nth);
+        FN(el_sponge)(sxxx, sxxz, spg1xn, spg2xp, tzn, spgzp, a->npxn, a->npxp, npzn, a->npzp, nx, nz, kst, itopo);
+        if (abc || isze - 1 > itopo[isxe - 1]) ADDSRC(sxxx);
         for (int i = 0; i < nx; ++i)
             for (int k = 0; k < nz; ++k) UF(sxx, i, k) = UF(sxxx, i, k) + UF(sxxz, i, k);
 
-        /* ---- tau_zz (pri:172-184) ---- */
-        FN(el_calux)(1, nx, nz, dx, dt, vx, szzx, lambsg, itopo, a10, nth);
-        FN(el_caluz)(0, nx, nz, dz, dt, vz, szzz, lambmusg, itopo, tab, nth);
-        FN(el_sponge)(szzx, szzz, spg1xn, spg2xp, spgzp, a->npxn, a->npxp, a->npzp, nx, nz, itopo);
-        for (int ix = 0; ix < nx; ++ix)
-            for (int iz = -1; iz <= itopo[ix]; ++iz) {
-                UF(szzz, ix, iz) = 0;
-                UF(szzx, ix, iz) = 0;
-            }
+        /* ---- tau_zz (pri:172-184; 'abc': pri:333-342) ---- */
+        FN(el_calux)(1, nx, nz, dx, dt, vx, szzx, lambsg, kst, a10, nth);
+        FN(el_caluz)(0, nx, nz, dz, dt, vz, szzz, lambmusg, kst, itopo, abc, tab, nth);
+        FN(el_sponge)(szzx, szzz, spg1xn, spg2xp, tzn, spgzp, a->npxn, a->npxp, npzn, a->npzp, nx, nz, kst, itopo);
+        if (!abc)
+            for (int ix = 0; ix < nx; ++ix)
+                for (int iz = -1; iz <= itopo[ix]; ++iz) {
+                    UF(szzz, ix, iz) = 0;
+                    UF(szzx, ix, iz) = 0;
+                }
         ADDSRC(szzx);
         for (int i = 0; i < nx; ++i)
             for (int k = 0; k < nz; ++k) UF(szz, i, k) = UF(szzx, i, k) + UF(szzz, i, k);
 
-        /* ---- tau_xz (pri:186-196) ---- */
-        FN(el_calux)(0, nx, nz, dx, dt, vz, sxzx, musge, itopo, a10, nth);
-        FN(el_caluz)(1, nx, nz, dz, dt, vx, sxzz, musge, itopo, tab, nth);
-        FN(el_sponge)(sxzx, sxzz, spgxn, spgxp, spg2zp, a->npxn, a->npxp, a->npzp, nx, nz, itopo);
-        for (int ix = 0; ix < nx; ++ix) {
-            const int izh = itopo[ix];
-            UF(sxzx, ix, izh - 1) = -UF(sxzx, ix, izh);
-            UF(sxzz, ix, izh - 1) = -UF(sxzz, ix, izh);
-        }
+        /* ---- tau_xz (pri:186-196; 'abc': pri:344-352) ---- */
+        FN(el_calux)(0, nx, nz, dx, dt, vz, sxzx, musge, kst, a10, nth);
+        FN(el_caluz)(1, nx, nz, dz, dt, vx, sxzz, musge, kst, itopo, abc, tab, nth);
+        FN(el_sponge)(sxzx, sxzz, spgxn, spgxp, tzn1, spg2zp, a->npxn, a->npxp, npzn, a->npzp, nx, nz, kst, itopo);
+        if (!abc)
+            for (int ix = 0; ix < nx; ++ix) {
+                const int izh = itopo[ix];
+                UF(sxzx, ix, izh - 1) = -UF(sxzx, ix, izh);
+                UF(sxzz, ix, izh - 1) = -UF(sxzz, ix, izh);
+            }
         for (int i = 0; i < nx; ++i)
             for (int k = 0; k < nz; ++k) UF(sxz, i, k) = UF(sxzx, i, k) + UF(sxzz, i, k);
 #undef ADDSRC
@@ -408,6 +458,7 @@ done:
     free(musge);
     free(g2c);
     free(spt);
+    free(kst);
     return rc;
 }
 

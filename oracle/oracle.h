@@ -146,9 +146,11 @@ int oracle_rsg_deriv_f64(int dir, int shift, int nx, int nz, const double *u, do
 void oracle_csponge_f32(int np, float h, float r, int lo, float *node, float *half);
 int oracle_visco_source_weights_f32(int nx, int nz, int isx, int isz, float sigma, float *w);
 
-/* Isotropic elastic standard-staggered-grid propagator with a free surface
- * over topography: isoewm2dfsbc (yang_version_2016_2_2, pri:57-230) with its
- * set-up (substagger, csponge, get_gauss_filter_coe). */
+/* Isotropic elastic standard-staggered-grid propagator over topography:
+ * isoewm2dfsbc (free surface, yang_version_2016_2_2 pri:57-230) or
+ * isoewm2dabc (absorbing top following the topography, pri:232-386, with
+ * modipapml's smoothing above the surface) and their set-up (substagger,
+ * csponge, get_gauss_filter_coe). */
 typedef struct oracle_elastic_args {
     int nx, nz;              /* padded shot grid: reference nxea (x), nzea (z) */
     int nsteps;              /* steps run (reference nt + idecay) */
@@ -166,6 +168,8 @@ typedef struct oracle_elastic_args {
     int nsnap;
     const int *snap_steps;   /* ascending step indices s after which fields are copied */
     int nthreads;
+    int mode;                /* 0 = 'fsbc' (isoewm2dfsbc), 1 = 'abc' (isoewm2dabc, pri:232-386) */
+    int npzn;                /* 'abc': top sponge table length - 1 (reference npzn) */
 } oracle_elastic_args;
 /* rec_vx, rec_vz [nrec][nsteps - delay] (reference record1 = vx, record2 = vz);
  * snaps [nsnap][5][nx*nz]: vx, vz, tau_xx, tau_zz, tau_xz. */
@@ -179,6 +183,8 @@ int oracle_elastic_stagger_f32(int nx, int nz, const float *vp, const float *vs,
                                float *out);
 int oracle_elastic_stagger_f64(int nx, int nz, const float *vp, const float *vs, const float *rho, const int *itopo,
                                double *out);
+/* modipapml's gauss_filter_2d (pub:592-634), serial order, in place. */
+int oracle_elastic_smooth_f32(int nx, int nz, int rgau, const int *itopo, float *u);
 
 #ifdef __cplusplus
 }

@@ -408,3 +408,63 @@ def test_parameter_file_workload_matches_golden_musge():
     np.testing.assert_array_equal(g[902], z["col_last"])
     rel = np.abs(g[:902].astype(np.float64) - z["col0"][None, :]) / z["col0"][None, :]
     assert rel.max() < 2.5e-7, rel.max()
+
+
+# ---------------------------------------------------------------- 'abc' mode
+
+def test_modipapml_smoothing_is_serial_in_place():
+    """gauss_filter_2d (pub:592-634) replaces each cell of rows 0..itopo, column
+    by column, by the window sum of the field as it stands, so smoothed cells
+    feed later ones.  Row 0 (itopo = 0, rgau = 2, weights w0, w1, wd =
+    exp(-{0,1,2}/8) normalised over the 3 x 3 window, clipped at the grid
+    edges, not renormalised) holds a unit impulse at column 2 over a row of
+    sevens: by hand, each column's value is w1 times the previous column's
+    smoothed value plus its own window's unsmoothed terms (an out-of-place
+    filter would miss the w1 c(i-1) terms).  Rows below itopo stay untouched."""
+    nx, nz = 6, 5
+    u = np.zeros((nx, nz), np.float32)
+    u[2, 0] = 1.0
+    u[:, 1:] = 7.0
+    out = oracle.elastic_smooth(u, np.zeros(nx, np.int32), 2)
+    e = np.exp(-np.array([0.0, 1.0, 2.0]) / 8.0)
+    Z = e[0] + 4 * e[1] + 4 * e[2]
+    w0, w1 = e[0] / Z, e[1] / Z
+    # row 1 below is 7: it enters every row-0 window through the (d, 1) weights
+    wd = e[2] / Z  # diagonal weight
+    c0 = 7.0 * (w1 + wd)                          # col 0: window cols -1..1 clipped; (1,0) still 0
+    c1 = w1 * c0 + w1 * 1.0 + 7.0 * (w1 + 2 * wd)  # smoothed (0,0), impulse (2,0), row 1
+    c2 = w1 * c1 + w0 * 1.0 + 7.0 * (w1 + 2 * wd)
+    c3 = w1 * c2 + 7.0 * (w1 + 2 * wd)
+    c4 = w1 * c3 + 7.0 * (w1 + 2 * wd)
+    c5 = w1 * c4 + 7.0 * (w1 + wd)                # col 5: right edge clipped
+    np.testing.assert_allclose(out[:, 0], [c0, c1, c2, c3, c4, c5], rtol=2e-6)
+    np.testing.assert_array_equal(out[:, 1:], u[:, 1:])
+
+
+def test_abc_mode_absorbs_at_the_top():
+    """'abc' mode (isoewm2dabc, pri:232-386): no free surface, the npzn + 1 rows
+    above the surface row damp the z parts (calpmlbclrtb, pub:851-892) and the
+    update domain starts npzn rows above the surface.  A P wave sent up to the
+    top comes back far weaker than from the free surface, rows above the
+    domain stay zero, and FP32 agrees with FP64."""
+    nx, nz = 120, 140
+    vp, vs, rho = _homog(nx, nz, 2000.0, 1100.0, 2000.0)
+    npzn = 30
+    itopo = np.full(nx, npzn + 2, np.int32)
+    kw = dict(src=(60, 80), rec=([60], [70]), nsteps=520, delay=0, dx=5.0, dz=5.0, dt=5e-4, f0=30.0,
+              sponge=(25, 25, 25), rgau=5, snap_steps=(150,))
+    free = _run(vp, vs, rho, itopo, **kw)
+    abc = _run(vp, vs, rho, itopo, mode="abc", npzn=npzn, **kw)
+    t = np.arange(520) * 5e-4
+    # direct arrival at 50 m, top reflection at 2*(80-32)*5 - 50 = 430 m -> 0.215 s
+    late = (t > 0.17) & (t < 0.32)
+    r_free = np.abs(free["rec_vz"][0, late]).max()
+    r_abc = np.abs(abc["rec_vz"][0, late]).max()
+    assert r_abc < 0.1 * r_free, (r_abc, r_free)
+    s = abc["snaps"][0]
+    top0 = itopo[0] - npzn
+    assert np.all(s[:, :, :top0] == 0)
+    assert np.abs(s[1, :, top0:]).max() > 0
+    d = _run(vp, vs, rho, itopo, mode="abc", npzn=npzn, precision="f64", **kw)
+    rel = np.linalg.norm(abc["rec_vz"] - d["rec_vz"]) / np.linalg.norm(d["rec_vz"])
+    assert rel < 2e-5, rel
