@@ -273,6 +273,17 @@ int sfd_visco_plan_destroy(sfd_visco_plan *plan);
  * or sfd_window_model), [nx][nz] z fastest, 0-based. */
 typedef struct sfd_elastic_plan sfd_elastic_plan;
 
+enum sfd_elastic_mode {
+    /* 'fsbc': free surface on row itopo of every column (isoewm2dfsbc) */
+    SFD_ELASTIC_FREE_SURFACE = 0,
+    /* 'abc' (isoewm2dabc, pri:232-386): no free surface; 10th-order stencils
+     * from npzn rows above the surface, the z parts of those npzn + 1 rows
+     * damped by the top cosine sponge (calpmlbclrtb, pub:851-892), the model
+     * on and above the surface smoothed by the source's Gaussian window
+     * (modipapml, pub:549-634), the source always into tau_xx */
+    SFD_ELASTIC_ABSORBING_TOP = 1
+};
+
 typedef struct sfd_elastic_params {
     int nx, nz;            /* padded shot grid (reference nxea, nzea) */
     int nt;                /* recorded samples (reference nt) */
@@ -292,6 +303,9 @@ typedef struct sfd_elastic_params {
     int nsnap;
     const int *snap_steps; /* ascending step indices in [0, delay + nt) after which fields are copied */
     int snap_fields;       /* mask of enum sfd_field: VX, VZ, SXX, SZZ, SXZ */
+    int mode;              /* enum sfd_elastic_mode */
+    int sponge_zlo;        /* SFD_ELASTIC_ABSORBING_TOP: reference npzn (>= 1), the top sponge
+                              of npzn + 1 rows ending on the surface row */
 } sfd_elastic_params;
 
 /* vp, vs, rho [nx*nz] required (vs = 0 cells are fluid); wavelet [delay + nt]

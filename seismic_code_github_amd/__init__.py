@@ -35,6 +35,7 @@ ARITH_FAST = 0
 ARITH_REFERENCE = 1
 FIELD_P, FIELD_VX, FIELD_VZ = 1, 2, 4
 FIELD_SXX, FIELD_SZZ, FIELD_SXZ = 8, 16, 32
+ELASTIC_FREE_SURFACE, ELASTIC_ABSORBING_TOP = 0, 1
 
 _STATUS = {SFD_EINVAL: "EINVAL", SFD_ENOMEM: "ENOMEM", SFD_EDEVICE: "EDEVICE",
            SFD_ECFL: "ECFL", SFD_ENAN: "ENAN", SFD_ENOTSUP: "ENOTSUP"}
@@ -91,6 +92,7 @@ class ElasticParamsC(ctypes.Structure):
         ("sx", ctypes.c_int), ("sz", ctypes.c_int), ("src_radius", ctypes.c_int),
         ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
         ("nsnap", ctypes.c_int), ("snap_steps", _IP), ("snap_fields", ctypes.c_int),
+        ("mode", ctypes.c_int), ("sponge_zlo", ctypes.c_int),
     ]
 
 
@@ -616,7 +618,9 @@ class ElasticShot:
     of every column.  src = (sx, sz); rec = (rx[], rz[]).  sponge = (npxn,
     npxp, npzp).  nt = recorded samples; delay = steps before the first (None =
     nint(1.2 / (f0 dt))).  wavelet None = the reference's built-in Ricker.
-    snap_fields: mask of FIELD_VX, FIELD_VZ, FIELD_SXX, FIELD_SZZ, FIELD_SXZ."""
+    snap_fields: mask of FIELD_VX, FIELD_VZ, FIELD_SXX, FIELD_SZZ, FIELD_SXZ.
+    mode: ELASTIC_FREE_SURFACE ('fsbc') or ELASTIC_ABSORBING_TOP ('abc', with
+    sponge_zlo = the reference's npzn)."""
     vp: np.ndarray
     vs: np.ndarray
     rho: np.ndarray
@@ -635,6 +639,8 @@ class ElasticShot:
     wavelet: Optional[np.ndarray] = None
     snap_steps: Sequence[int] = ()
     snap_fields: int = FIELD_VX | FIELD_VZ
+    mode: int = 0
+    sponge_zlo: int = 0
     _keep: list = field(default_factory=list, repr=False)
 
     def params(self) -> ElasticParamsC:
@@ -648,7 +654,8 @@ class ElasticShot:
                               self.dx, self.dz, self.dt, self.f0, _ptr(it, ctypes.c_int),
                               *[int(v) for v in self.sponge], self.sponge_r, int(self.src[0]), int(self.src[1]),
                               int(self.src_radius), rx.size, _ptr(rx, ctypes.c_int), _ptr(rz, ctypes.c_int),
-                              steps.size, _ptr(steps, ctypes.c_int), self.snap_fields)
+                              steps.size, _ptr(steps, ctypes.c_int), self.snap_fields, int(self.mode),
+                              int(self.sponge_zlo))
 
     def nfields(self) -> int:
         return bin(self.snap_fields & 62).count("1")

@@ -29,7 +29,8 @@ struct sfd_elastic_plan {
     void (*str)(sfd::ElasticStepArgs) = nullptr;
     sfd::DeviceBuffer arena;
     sfd::Field vx, vz, sxx, szz, sxz, parts, ghost, b2e, b1e, lm, ls, mu;
-    sfd::DeviceBuffer itopo, srow, fxn, fxh, ratio, fzn, fzh, g, amp, rx, rz, rec_vx, rec_vz, snaps;
+    sfd::DeviceBuffer itopo, kst, srow, fxn, fxh, ratio, fzn, fzh, tzn, tzh, g, amp, rx, rz, rec_vx, rec_vz, snaps;
+    bool abc = false;
 
     size_t field_floats() const { return (size_t)ncols * pitch; }
     float* cell0(const sfd::Field& f) const { return f.as<float>() + (size_t)sfd::kElPad * pitch + sfd::kElPad; }
@@ -119,6 +120,26 @@ void csponge(int np, float h, float Rc, bool lo, std::vector<float>& node, std::
     if (lo) half[np] = 1.0f;
 }
 
+// gauss_filter_2d (pub:592-634), as one thread runs it: rows 0 .. itopo(i)
+// of every column replaced in place, column after column, by the window sum
+// (weights g, window 2 rgau - 1 clipped to the grid, not renormalised) of the
+// field as it stands, so smoothed cells feed later ones.  (The reference runs
+// the columns under OpenMP, which races on shared neighbours; the serial order
+// is the deterministic reading.)
+void gauss_smooth(int nx, int nz, int rg, const int* itopo, const std::vector<float>& g, std::vector<float>& u) {
+    const int ng = 2 * rg - 1;
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k <= itopo[i]; ++k) {
+            float sum = 0.f;
+            for (int n = std::max(0, i - (rg - 1)); n <= std::min(nx - 1, i + (rg - 1)); ++n) {
+                const int idx2 = n - (i - (rg - 1));
+                for (int m = std::max(0, k - (rg - 1)); m <= std::min(nz - 1, k + (rg - 1)); ++m)
+                    sum += u[(size_t)n * nz + m] * g[(size_t)idx2 * ng + (m - (k - (rg - 1)))];
+            }
+            u[(size_t)i * nz + k] = sum;
+        }
+}
+
 int validate(sfd_context* ctx, const sfd_elastic_params* p, const float* vp, const float* vs, const float* rho) {
     if (!p || !vp || !vs || !rho || !p->itopo) return fail(ctx, SFD_EINVAL, "null params, vp, vs, rho or itopo");
     if (p->nx < 12 || p->nz < 12) return fail(ctx, SFD_EINVAL, "grid %dx%d too small (12x12 at least)", p->nx, p->nz);
@@ -127,6 +148,10 @@ int validate(sfd_context* ctx, const sfd_elastic_params* p, const float* vp, con
     if (!(p->f0 > 0)) return fail(ctx, SFD_EINVAL, "f0 must be > 0");
     if (p->sponge_xlo < 0 || p->sponge_xhi < 0 || p->sponge_zhi < 0)
         return fail(ctx, SFD_EINVAL, "sponge widths must be >= 0");
+    if (p->mode != SFD_ELASTIC_FREE_SURFACE && p->mode != SFD_ELASTIC_ABSORBING_TOP)
+        return fail(ctx, SFD_EINVAL, "mode must be SFD_ELASTIC_FREE_SURFACE or SFD_ELASTIC_ABSORBING_TOP");
+    if (p->mode == SFD_ELASTIC_ABSORBING_TOP && p->sponge_zlo < 1)
+        return fail(ctx, SFD_EINVAL, "the absorbing top needs sponge_zlo >= 1 (reference npzn)");
     if (p->sponge_xlo + p->sponge_xhi + 2 > p->nx)
         return fail(ctx, SFD_EINVAL, "left and right sponge strips overlap in %d columns", p->nx);
     const int kb0 = p->nz - 1 - p->sponge_zhi;
@@ -158,28 +183,28 @@ using ElasticKernel = void (*)(ElasticStepArgs);
 constexpr int kElDefaultTX = 8, kElDefaultNW = 8;
 
 template <int TX, int NW>
-void el_pair(ElasticKernel* v, ElasticKernel* s) {
-    *v = elastic_velocity_kernel<TX, NW>;
-    *s = elastic_stress_kernel<TX, NW>;
+void el_pair(bool abc, ElasticKernel* v, ElasticKernel* s) {
+    *v = abc ? elastic_velocity_kernel<TX, NW, true> : elastic_velocity_kernel<TX, NW, false>;
+    *s = abc ? elastic_stress_kernel<TX, NW, true> : elastic_stress_kernel<TX, NW, false>;
 }
 
 // The instantiated tilings: tile columns x waves per workgroup.
-bool el_pick(int tx, int nw, ElasticKernel* v, ElasticKernel* s) {
+bool el_pick(int tx, int nw, bool abc, ElasticKernel* v, ElasticKernel* s) {
     switch (tx * 100 + nw) {
-    case 804: el_pair<8, 4>(v, s); return true;
-    case 808: el_pair<8, 8>(v, s); return true;
-    case 1604: el_pair<16, 4>(v, s); return true;
-    case 1608: el_pair<16, 8>(v, s); return true;
-    case 1616: el_pair<16, 16>(v, s); return true;
-    case 3208: el_pair<32, 8>(v, s); return true;
-    case 3216: el_pair<32, 16>(v, s); return true;
+    case 804: el_pair<8, 4>(abc, v, s); return true;
+    case 808: el_pair<8, 8>(abc, v, s); return true;
+    case 1604: el_pair<16, 4>(abc, v, s); return true;
+    case 1608: el_pair<16, 8>(abc, v, s); return true;
+    case 1616: el_pair<16, 16>(abc, v, s); return true;
+    case 3208: el_pair<32, 8>(abc, v, s); return true;
+    case 3216: el_pair<32, 16>(abc, v, s); return true;
     default: return false;
     }
 }
 
 int el_set_tiling(sfd_elastic_plan* pl, int tx, int nw) {
     ElasticKernel v, s;
-    if (!el_pick(tx, nw, &v, &s))
+    if (!el_pick(tx, nw, pl->abc, &v, &s))
         return fail(pl->ctx, SFD_EINVAL, "no elastic kernel for %d tile columns x %d waves", tx, nw);
     pl->tx = tx;
     pl->nw = nw;
@@ -284,9 +309,34 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
     // halo column stays inside.
     pl->ncols = nx + kElTXMax + 2 * kElPad;
 
-    // ---- model ----
+    pl->abc = p->mode == SFD_ELASTIC_ABSORBING_TOP;
+    const int npzn = pl->abc ? p->sponge_zlo : 0;
+
+    // ---- Gaussian window (get_gauss_filter_coe(gc2d, 2.0, 2 rgau - 1, ...),
+    // normalised by its float sum in the reference's order) ----
+    const int rg = p->src_radius > 0 ? p->src_radius : 50;
+    const int ng = 2 * rg - 1;
+    std::vector<float> g2((size_t)ng * ng);
+    {
+        const float sigma2 = 2.0f * 2.0f;
+        const int cen = (ng + 1) / 2;
+        float sum = 0.f;
+        for (int j = 1; j <= ng; ++j)
+            for (int i = 1; i <= ng; ++i) {
+                const float dis = (float)((j - cen) * (j - cen) + (i - cen) * (i - cen));
+                g2[(size_t)(j - 1) * ng + (i - 1)] = 1.0f / (2.0f * kPi * sigma2) * std::exp(-dis / (2.0f * sigma2));
+            }
+        for (int j = 1; j <= ng; ++j)
+            for (int i = 1; i <= ng; ++i) sum += g2[(size_t)(j - 1) * ng + (i - 1)];
+        for (float& v : g2) v = v / sum;
+    }
+
+    // ---- model: substagger, and in 'abc' mode modipapml (pub:549-590), the
+    // Gaussian smoothing of the parameters on and above the surface ----
     std::vector<float> st[5];
     stagger(nx, nz, vp, vs, rho, p->itopo, st);
+    if (pl->abc)
+        for (int q = 0; q < 5; ++q) gauss_smooth(nx, nz, rg, p->itopo, g2, st[q]);
 
     // ---- sponge tables (csponge) per column and row ----
     const int npxn = p->sponge_xlo, npxp = p->sponge_xhi, npzp = p->sponge_zhi;
@@ -295,17 +345,23 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
     csponge(npxn, p->dx, Rc, true, xn, xn1);
     csponge(npxp, p->dx, Rc, false, xp, xp2);
     csponge(npzp, p->dz, Rc, false, zp, zp2);
+    std::vector<float> tn, tn1;
+    csponge(npzn, p->dz, Rc, true, tn, tn1);  // 'abc' top tables spgzn, spg1zn
+    // First updated row of a column: its surface row ('fsbc'), or npzn rows
+    // above it ('abc'; row -1 of the reference's split parts is never summed).
+    std::vector<int> hkst(nx);
+    for (int i = 0; i < nx; ++i) hkst[i] = std::max(0, p->itopo[i] - npzn);
     std::vector<int> hsrow(nx, INT_MAX);
     std::vector<float> hfxn(nx, 1.f), hfxh(nx, 1.f), hfzn(nz, 1.f), hfzh(nz, 1.f), hratio(nx);
     for (int i = 0; i <= npxn; ++i) {
-        hsrow[i] = p->itopo[i];
+        hsrow[i] = hkst[i];
         hfxn[i] = xn[i];
         hfxh[i] = xn1[i];
     }
-    // The right strip's damped rows start at the surface row of the column
-    // counted the same distance from the LEFT edge (pub:917-922).
+    // The right strip's damped rows start at the first updated row of the
+    // column counted the same distance from the LEFT edge (pub:917-922, 871-876).
     for (int i = 0; i <= npxp; ++i) {
-        hsrow[nx - 1 - i] = p->itopo[i];
+        hsrow[nx - 1 - i] = hkst[i];
         hfxn[nx - 1 - i] = xp[i];
         hfxh[nx - 1 - i] = xp2[i];
     }
@@ -318,24 +374,9 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
         hratio[i] = st[3][c] / st[2][c];  // lambsg / lambmusg at the surface (calfsbcv)
     }
 
-    // ---- Gaussian source window (get_gauss_filter_coe(gc2d, 2.0, 2 rgau - 1, ...),
-    // normalised by its float sum in the reference's order; placed by addsrc
-    // centred on the source cell and clipped by the grid) ----
-    const int rg = p->src_radius > 0 ? p->src_radius : 50;
-    const int ng = 2 * rg - 1;
+    // ---- source window: addsrc places gc2d centred on the source cell,
+    // clipped by the grid ----
     {
-        std::vector<float> g2((size_t)ng * ng);
-        const float sigma2 = 2.0f * 2.0f;
-        const int cen = (ng + 1) / 2;
-        float sum = 0.f;
-        for (int j = 1; j <= ng; ++j)
-            for (int i = 1; i <= ng; ++i) {
-                const float dis = (float)((j - cen) * (j - cen) + (i - cen) * (i - cen));
-                g2[(size_t)(j - 1) * ng + (i - 1)] = 1.0f / (2.0f * kPi * sigma2) * std::exp(-dis / (2.0f * sigma2));
-            }
-        for (int j = 1; j <= ng; ++j)
-            for (int i = 1; i <= ng; ++i) sum += g2[(size_t)(j - 1) * ng + (i - 1)];
-        for (float& v : g2) v = v / sum;
         auto weight = [&](int x, int z) -> float {
             const int idx2 = x - p->sx + rg, idx1 = z - p->sz + rg;
             if (idx1 < 1 || idx1 > ng || idx2 < 1 || idx2 > ng) return 0.f;
@@ -375,7 +416,8 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
 
     // ---- device arena: state (5 fields, strip parts, pending surface values), then model ----
     const int nl = npxn + 1, nr = npxp + 1, nb = npzp + 1, kb0 = nz - nb;
-    const size_t nstrip = (size_t)(nl + nr) * nz + (size_t)(nx - nl - nr) * nb;
+    const size_t topbase = (size_t)(nl + nr) * nz + (size_t)(nx - nl - nr) * nb;
+    const size_t nstrip = topbase + (pl->abc ? (size_t)(nx - nl - nr) * (npzn + 1) : 0);
     if (nstrip >= (size_t)INT_MAX / 10) return bail(fail(ctx, SFD_ENOTSUP, "sponge strips too large"));
     const size_t fb = pl->field_floats() * sizeof(float);
     hipStream_t sm = ctx->stream;
@@ -412,7 +454,10 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
         }
     }
     if (e == hipSuccess) e = upload(pl->itopo, p->itopo, nx, sm);
+    if (e == hipSuccess) e = upload(pl->kst, hkst.data(), nx, sm);
     if (e == hipSuccess) e = upload(pl->srow, hsrow.data(), nx, sm);
+    if (e == hipSuccess) e = upload(pl->tzn, tn.data(), tn.size(), sm);
+    if (e == hipSuccess) e = upload(pl->tzh, tn1.data(), tn1.size(), sm);
     if (e == hipSuccess) e = upload(pl->fxn, hfxn.data(), nx, sm);
     if (e == hipSuccess) e = upload(pl->fxh, hfxh.data(), nx, sm);
     if (e == hipSuccess) e = upload(pl->ratio, hratio.data(), nx, sm);
@@ -446,7 +491,12 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
     a.ls = pl->cell0(pl->ls);
     a.mu = pl->cell0(pl->mu);
     a.itopo = pl->itopo.as<int>();
+    a.kst = pl->kst.as<int>();
     a.srow = pl->srow.as<int>();
+    a.tzn = pl->tzn.as<float>();
+    a.tzh = pl->tzh.as<float>();
+    a.npzn = npzn;
+    a.topbase = (int)topbase;
     a.fxn = pl->fxn.as<float>();
     a.fxh = pl->fxh.as<float>();
     a.ratio = pl->ratio.as<float>();
@@ -470,7 +520,8 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
     a.a4[0] = 9.0f / 8.0f;
     a.a4[1] = -1.0f / 24.0f;
     // "if source at free surface, don't increment strxxx" (pri:163-167)
-    a.src_xx = p->sz > p->itopo[p->sx] ? 1 : 0;
+    // ('abc' mode always increments strxxx, pri:330)
+    a.src_xx = (pl->abc || p->sz > p->itopo[p->sx]) ? 1 : 0;
     a.g = pl->g.as<float>();
     a.amp = pl->amp.as<float>();
     a.dt = p->dt;

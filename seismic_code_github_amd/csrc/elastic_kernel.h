@@ -18,6 +18,11 @@
 // reference damps the x- and z-parts of each field apart (calpmlbclrb,
 // pub:895-934): there both parts are kept in `parts` and T = X + Z.
 //
+// Two modes, the reference's two loops: 'fsbc' (isoewm2dfsbc, ABC = false),
+// a free surface on row t of every column, and 'abc' (isoewm2dabc, ABC =
+// true), no free surface: rows from t - npzn down are updated with full-order
+// stencils and the z parts of rows t - npzn .. t are damped (calpmlbclrtb).
+//
 // Free surface, as the reference's call order makes it:
 //   tau_zz = 0 on and above the surface row t (the source still adds there);
 //   tau_xz(t - 1) = -tau_xz(t) (written with the surface value);
@@ -46,12 +51,14 @@ struct ElasticStepArgs {
     int nx, nz, pitch;
     float *vx, *vz, *sxx, *szz, *sxz;
     const float *b2e, *b1e, *lm, *ls, *mu;  // busg2e, busg1e, lambmusg, lambsg, musge
-    const int *itopo, *srow;                // surface row; first damped row of an x-strip column
+    const int *itopo, *kst, *srow;          // surface row; first updated row; first damped row of an x-strip column
     const float *fxn, *fxh, *ratio;         // per column: node / half x-sponge, lambsg/lambmusg at t
     const float *fzn, *fzh;                 // per row: node / half bottom sponge
+    const float *tzn, *tzh;                 // 'abc' top sponge: node / half, by row - (t - npzn)
+    int npzn;                               // 'abc': top sponge rows - 1
     float* ghost;                           // per column: vz(t - 1) for the next step
     float* parts;                           // [10][nstrip] x / z parts of vx, vz, sxx, szz, sxz
-    int nstrip, nl, nr, kb0, nb;
+    int nstrip, nl, nr, kb0, nb, topbase;
     int ntz, ntiles;                        // z tiles, all tiles
     float cx, cz;                           // dt/dx, dt/dz
     float a10[5], a8[4], a6[3], a4[2];      // Taylor tables (module:9-26); a20 = 1
@@ -65,11 +72,14 @@ struct ElasticStepArgs {
 };
 
 // Strip cell index, or -1: left / right strip columns over all rows, then the
-// bottom-strip rows of the columns between them.
-__device__ __forceinline__ int el_strip(const ElasticStepArgs& a, int col, int k) {
+// bottom-strip rows of the columns between them, then ('abc') the npzn + 1
+// rows t - npzn .. t of those columns.
+template <bool ABC>
+__device__ __forceinline__ int el_strip(const ElasticStepArgs& a, int col, int k, int t) {
     if (col < a.nl) return col * a.nz + k;
     if (col >= a.nx - a.nr) return (a.nl + col - (a.nx - a.nr)) * a.nz + k;
     if (k >= a.kb0) return (a.nl + a.nr) * a.nz + (col - a.nl) * a.nb + (k - a.kb0);
+    if (ABC && k <= t && k >= t - a.npzn) return a.topbase + (col - a.nl) * (a.npzn + 1) + (k - (t - a.npzn));
     return -1;
 }
 
@@ -137,9 +147,9 @@ __device__ __forceinline__ void el_tile(const ElasticStepArgs& a, int tx, int* z
     *x0 = (t / a.ntz) * tx;
 }
 
-// Velocity half-step (pri:119-151): vx from tau_xx, tau_xz; vz from tau_xz,
-// tau_zz; the pending vz(t - 1) of every column.
-template <int TX, int NW>
+// Velocity half-step (pri:119-151; 'abc': pri:300-318): vx from tau_xx,
+// tau_xz; vz from tau_xz, tau_zz; in 'fsbc' mode the pending vz(t - 1).
+template <int TX, int NW, bool ABC>
 __global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepArgs a) {
     constexpr int HX = kElH, HZ = kElH;
     constexpr int WX = TX + 2 * HX, HZZ = kElTZ + 2 * HZ;
@@ -160,11 +170,11 @@ __global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepAr
         if (col >= a.nx) break;
         const int t = a.itopo[col];
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
-        if (k < t) {
-            if (k == t - 1) a.vz[o] = a.ghost[col];
+        if (k < (ABC ? a.kst[col] : t)) {
+            if (!ABC && k == t - 1) a.vz[o] = a.ghost[col];
             continue;
         }
-        const int d = k - t;
+        const int d = ABC ? kElTZ : k - t;  // 'abc': 10th order on every row
         const float* pxx = s_xx + (cl + HX) * kElTZ + tz;
         const float* pxz = s_xz + (cl + HX) * HZZ + tz + HZ;
         const float* pzz = s_zz + cl * HZZ + tz + HZ;
@@ -173,7 +183,7 @@ __global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepAr
         const float uz = a.cz * bx * el_zsum<0>(a, pxz, d);       // caluz zflag 0 on tau_xz
         const float wx = a.cx * bz * el_xsum<1>(a, pxz, HZZ);     // calux xflag 1 on tau_xz
         const float wz = a.cz * bz * el_zsum<1>(a, pzz, d);       // caluz zflag 1 on tau_zz
-        const int s = el_strip(a, col, k);
+        const int s = el_strip<ABC>(a, col, k, t);
         if (s < 0) {
             a.vx[o] = a.vx[o] + ux + uz;
             a.vz[o] = a.vz[o] + wx + wz;
@@ -181,25 +191,37 @@ __global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepAr
         }
         float* P = a.parts;
         const size_t n = (size_t)a.nstrip;
-        const bool dx = k >= a.srow[col], dz = k >= a.kb0;
+        const bool dx = k >= a.srow[col];
+        float fzv = 1.f, fzw = 1.f;  // z factors of vx (node table) and vz (half table)
+        bool dz = false;
+        if (k >= a.kb0) {
+            dz = true;
+            fzv = a.fzn[k];
+            fzw = a.fzh[k];
+        } else if (ABC && k <= t && k >= t - a.npzn) {
+            dz = true;
+            fzv = a.tzn[k - (t - a.npzn)];
+            fzw = a.tzh[k - (t - a.npzn)];
+        }
         float X = P[s] + ux, Z = P[n + s] + uz;
         if (dx) X *= a.fxn[col];
-        if (dz) Z *= a.fzn[k];
+        if (dz) Z *= fzv;
         P[s] = X;
         P[n + s] = Z;
         a.vx[o] = X + Z;
         X = P[2 * n + s] + wx;
         Z = P[3 * n + s] + wz;
         if (dx) X *= a.fxh[col];
-        if (dz) Z *= a.fzh[k];
+        if (dz) Z *= fzw;
         P[2 * n + s] = X;
         P[3 * n + s] = Z;
         a.vz[o] = X + Z;
     }
 }
 
-// Stress half-step (pri:153-196) and the step's receiver samples.
-template <int TX, int NW>
+// Stress half-step (pri:153-196; 'abc': pri:320-352) and the step's receiver
+// samples.
+template <int TX, int NW, bool ABC>
 __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs a) {
     constexpr int HX = kElH, HZ = kElH;
     constexpr int WX = TX + 2 * HX, HZZ = kElTZ + 2 * HZ;
@@ -228,34 +250,43 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
         const int t = a.itopo[col];
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
         // addsrc (pub:1429-1466): w g dt / dz^2, into tau_zz always and into
-        // tau_xx when the source lies below the surface.
+        // tau_xx in 'abc' mode or when the source lies below the surface.
         float src = 0.f;
         const bool in_box = in_box_z && col >= a.sx0 && col < a.sx1;
         if (in_box) src = ((w * a.g[(size_t)(col - a.sx0) * (a.sz1 - a.sz0) + (k - a.sz0)]) * a.dt) / a.dz2;
         const float srx = a.src_xx ? src : 0.f;
-        const int s = el_strip(a, col, k);
+        const int s = el_strip<ABC>(a, col, k, t);
         float* P = a.parts;
         const size_t n = (size_t)a.nstrip;
-        if (k < t) {
-            // Above the surface only the source acts: tau_xx accumulates it
-            // (its x-part damped where the strip's rows reach up here), tau_zz
-            // is reset to it.
+        if (k < (ABC ? a.kst[col] : t)) {
+            // Above the updated rows only the source acts: the x parts take it
+            // (damped where a strip's rows reach up here); 'fsbc' resets tau_zz
+            // on and above the surface to it, 'abc' lets it accumulate.
             if (s >= 0) {
+                const float fx = k >= a.srow[col] ? a.fxh[col] : 1.f;
                 float X = P[4 * n + s];
-                if (k >= a.srow[col]) X *= a.fxh[col];
+                if (k >= a.srow[col]) X *= fx;
                 X += srx;
                 P[4 * n + s] = X;
                 a.sxx[o] = X + P[5 * n + s];
-                P[6 * n + s] = src;
-                P[7 * n + s] = 0.f;
-                a.szz[o] = src;
+                if (ABC) {
+                    X = P[6 * n + s];
+                    if (k >= a.srow[col]) X *= fx;
+                    X += src;
+                    P[6 * n + s] = X;
+                    a.szz[o] = X + P[7 * n + s];
+                } else {
+                    P[6 * n + s] = src;
+                    P[7 * n + s] = 0.f;
+                    a.szz[o] = src;
+                }
             } else if (in_box) {
                 a.sxx[o] = a.sxx[o] + srx;
-                a.szz[o] = src;
+                a.szz[o] = ABC ? a.szz[o] + src : src;
             }
             continue;
         }
-        const int d = k - t;
+        const int d = ABC ? kElTZ : k - t;
         const float* pvx = s_vx + (cl + HX) * HZZ + tz + HZ;
         const float* pvz = s_vz + (cl + HX) * HZZ + tz + HZ;
         const float dvx_x = el_xsum<1>(a, pvx, HZZ);  // calux xflag 1 on vx
@@ -266,17 +297,29 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
         const float ax = a.cx * lm * dvx_x, az = a.cz * ls * dvz_z;
         const float bx = a.cx * ls * dvx_x, bz = a.cz * lm * dvz_z;
         const float ex = a.cx * mu * dvz_x, ez = a.cz * mu * dvx_z;
+        const bool top = !ABC && k == t;  // 'fsbc': tau_zz reset on the surface row
         float xz;
         if (s < 0) {
             a.sxx[o] = a.sxx[o] + ax + az + srx;
-            a.szz[o] = k == t ? src : a.szz[o] + bx + bz + src;
+            a.szz[o] = top ? src : a.szz[o] + bx + bz + src;
             xz = a.sxz[o] + ex + ez;
         } else {
-            const bool dx = k >= a.srow[col], dz = k >= a.kb0;
+            const bool dx = k >= a.srow[col];
+            float fzs = 1.f, fzx = 1.f;  // z factors of the normal stresses (node) and tau_xz (half)
+            bool dz = false;
+            if (k >= a.kb0) {
+                dz = true;
+                fzs = a.fzn[k];
+                fzx = a.fzh[k];
+            } else if (ABC && k <= t && k >= t - a.npzn) {
+                dz = true;
+                fzs = a.tzn[k - (t - a.npzn)];
+                fzx = a.tzh[k - (t - a.npzn)];
+            }
             const float fh = a.fxh[col];
             float X = P[4 * n + s] + ax, Z = P[5 * n + s] + az;
             if (dx) X *= fh;
-            if (dz) Z *= a.fzn[k];
+            if (dz) Z *= fzs;
             X += srx;
             P[4 * n + s] = X;
             P[5 * n + s] = Z;
@@ -284,8 +327,8 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
             X = P[6 * n + s] + bx;
             Z = P[7 * n + s] + bz;
             if (dx) X *= fh;
-            if (dz) Z *= a.fzn[k];
-            if (k == t) X = Z = 0.f;
+            if (dz) Z *= fzs;
+            if (top) X = Z = 0.f;
             X += src;
             P[6 * n + s] = X;
             P[7 * n + s] = Z;
@@ -293,13 +336,13 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
             X = P[8 * n + s] + ex;
             Z = P[9 * n + s] + ez;
             if (dx) X *= a.fxn[col];
-            if (dz) Z *= a.fzh[k];
+            if (dz) Z *= fzx;
             P[8 * n + s] = X;
             P[9 * n + s] = Z;
             xz = X + Z;
         }
         a.sxz[o] = xz;
-        if (k != t) continue;
+        if (ABC || k != t) continue;
         if (t >= 1) a.sxz[o - 1] = -xz;  // calfsbcstrxz (pub:1009-1023)
         // calfsbcv (pub:938-972) for the next step's vz(t - 1).  In a right
         // strip whose damped rows reach row t - 1 (they start at the mirrored
@@ -309,7 +352,7 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
         if (t >= 1 && col < a.nx - 1) {
             const float r = a.ratio[col];
             if (col >= a.nx - a.nr && a.srow[col] <= t - 1) {
-                const int s1 = el_strip(a, col + 1, t);
+                const int s1 = el_strip<ABC>(a, col + 1, t, a.itopo[col + 1]);
                 const float gx = P[2 * n + s] + r * (P[s1] - P[s]);
                 const float gz = P[3 * n + s] + r * (P[n + s1] - P[n + s]);
                 gh = gx * a.fxh[col] + gz;

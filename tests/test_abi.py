@@ -59,5 +59,5 @@ def test_elastic_params_struct_layout():
     import seismic_code_github_amd as sfd
     C = sfd.ElasticParamsC
     # sizeof / offsetof of sfd_elastic_params as gcc lays it out on x86-64
-    assert ctypes.sizeof(C) == 112
+    assert ctypes.sizeof(C) == 120
     assert (C.itopo.offset, C.src_radius.offset, C.snap_steps.offset, C.snap_fields.offset) == (32, 64, 96, 104)

@@ -12,8 +12,9 @@ linearly in time), both bounds become 3x that drift.
 Cases: topography whose left and right edges differ both ways (the right
 strip's damped rows follow the left edge's surface), a source on the surface,
 a surface on the top row, a fluid layer (vs = 0), receivers on and above the
-surface, grids that are no multiple of the tiles, and the reference
-program's own 903 x 603 shot grid.
+surface, grids that are no multiple of the tiles, the reference program's
+own 903 x 603 shot grid, and the 'abc' mode (absorbing top following the
+topography, smoothed model above it) on hills and on that grid.
 """
 import numpy as np
 import pytest
@@ -39,7 +40,8 @@ def run_oracle(shot: sfd.ElasticShot, precision="f32"):
     return oracle.elastic(shot.vp, shot.vs, shot.rho, shot.itopo, src=shot.src, rec=shot.rec,
                           nsteps=delay + shot.nt, delay=delay, dx=shot.dx, dz=shot.dz, dt=shot.dt, f0=shot.f0,
                           sponge=shot.sponge, sponge_r=shot.sponge_r, rgau=shot.src_radius, wavelet=shot.wavelet,
-                          snap_steps=shot.snap_steps, precision=precision, nthreads=oracle_threads())
+                          snap_steps=shot.snap_steps, precision=precision, nthreads=oracle_threads(),
+                          mode="abc" if shot.mode == sfd.ELASTIC_ABSORBING_TOP else "fsbc", npzn=shot.sponge_zlo)
 
 
 def small_shot(nx=133, nz=97, nt=420, sponge=(18, 21, 16), topo="hills", vs_scale=0.55, fluid_rows=0, **kw):
@@ -153,6 +155,32 @@ def test_reference_program_grid(ctx):
     assert np.allclose(rvx[c + m], -rvx[c + 1 - m], rtol=0, atol=1e-5 * np.abs(rvx).max())
 
 
+def test_absorbing_top_mode(ctx):
+    """'abc' mode (isoewm2dabc): hills, a top sponge of npzn + 1 rows following
+    them, the model above the surface smoothed by modipapml, the source into
+    tau_xx always; both edge orders of the right strip's mirrored rows."""
+    for label, left, right, npzn in (("abc-left-lower", 14, 19, 12), ("abc-left-higher", 20, 15, 9)):
+        shot = small_shot(nx=121, nz=110, nt=380, sponge=(16, 19, 14), snap_fields=ALL, src_radius=6)
+        t = (shot.itopo + 12).astype(np.int32)
+        t[:22] = left
+        t[-22:] = right
+        shot.itopo = t
+        shot.mode = sfd.ELASTIC_ABSORBING_TOP
+        shot.sponge_zlo = npzn
+        shot.src = (shot.src[0], int(t[shot.src[0]]) + 3)
+        shot.rec = (shot.rec[0], np.minimum(t[shot.rec[0]] + 1, 100).astype(np.int32))
+        shot.snap_steps = (100, 250, shot.ref_delay() + shot.nt - 1)
+        check(shot, ctx, what=label)
+
+
+def test_parameter_file_grid_absorbing_top(ctx):
+    """The parameter file's grid in 'abc' mode: npzn = 1 (ppzn = 0), so the top
+    sponge covers row -1 (never summed) and the surface row 0, whose model
+    values modipapml smooths."""
+    shot = W.elastic_par_903x603(nt=200, mode=sfd.ELASTIC_ABSORBING_TOP, sponge_zlo=1)
+    check(shot, ctx, snaps=False, what="par-abc")
+
+
 def test_invalid_arguments(ctx):
     shot = small_shot(nt=10)
     bad = small_shot(nt=10)
@@ -168,4 +196,8 @@ def test_invalid_arguments(ctx):
     ov = small_shot(nt=10, sponge=(70, 70, 10))
     with pytest.raises(sfd.SeisfdError):
         sfd.elastic_forward(ov, ctx)
+    nz0 = small_shot(nt=10, mode=sfd.ELASTIC_ABSORBING_TOP, sponge_zlo=0)
+    with pytest.raises(sfd.SeisfdError) as e:
+        sfd.elastic_forward(nz0, ctx)
+    assert e.value.code == -1
     del shot
