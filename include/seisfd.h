@@ -203,16 +203,17 @@ int sfd_window_model(const float *model, int mnx, int mnz, int x0, int nxe, int 
 double sfd_acoustic_cfl_limit(int radius, const float *coef);
 
 
-/* ---------------------------------------------------------------- visco-acoustic
- * Rotated-staggered-grid visco-acoustic propagator: the drop-in for the time
+/* ---------------------------------------------------------------- visco-elastic
+ * Rotated-staggered-grid visco-elastic propagator: the drop-in for the time
  * loop of forward/Q_forward, single_shot_modelling
  * (Q_multishot_fd_main_fsbc_zy_rotated.f90:169-484, step
- * extrapolation_one_step_10nd_absbc, Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:695-839)
- * with vs = 0, where the reference's strxx and strzz are one field sigma and
- * strxz stays zero.  L standard linear solids (the reference: L = 1) fitted by
- * the tau-method of pcalpas (Q sub:473-551).  Grid = the reference's padded
- * shot grid nxea x nzea (get_currshot_parameters, or sfd_window_model),
- * [nx][nz] z fastest, 0-based. */
+ * extrapolation_one_step_10nd_absbc, Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:695-839),
+ * with L standard linear solids (the reference: L = 1) fitted by the
+ * tau-method of pcalpas (Q sub:473-551).  vs = 0 everywhere is the
+ * visco-acoustic limit (strxx = strzz = sigma, strxz = 0), run by one fused
+ * kernel per step; any vs > 0 runs the visco-elastic kernels (two launches per
+ * step).  Grid = the reference's padded shot grid nxea x nzea
+ * (get_currshot_parameters, or sfd_window_model), [nx][nz] z fastest, 0-based. */
 typedef struct sfd_visco_plan sfd_visco_plan;
 
 typedef struct sfd_visco_params {
@@ -236,16 +237,18 @@ typedef struct sfd_visco_params {
     const int *rx, *rz;    /* receiver cells (reference: row itopo + nrz0 of every window column) */
     int nsnap;
     const int *snap_steps; /* ascending step indices in [0, delay + nt) after which fields are copied */
-    int snap_fields;       /* mask of enum sfd_field: P = sigma (= strxx = strzz), VX, VZ */
+    int snap_fields;       /* mask of enum sfd_field: P = strzz (= sigma when vs = 0), VX, VZ, and
+                              with vs > 0 also SXX, SZZ, SXZ */
+    const float *qs;       /* [nx*nz] S-wave quality factor (reference qfs); NULL = qp */
 } sfd_visco_params;
 
 /* vp, qp [nx*nz] required; rho NULL = 2400 kg/m^3 everywhere (averaged over 4
- * cells as substagger does); vs must be NULL or all zero (visco-elastic is
- * SFD_ENOTSUP); wavelet [delay + nt] or NULL = addsrc's Ricker
+ * cells as substagger does); vs NULL or all zero = visco-acoustic, otherwise
+ * visco-elastic with mu = rho_avg vs^2; wavelet [delay + nt] or NULL = addsrc's Ricker
  * exp(-(pi f0 (it dt - 1.2/f0))^2)(1 - 2 (pi f0 (it dt - 1.2/f0))^2), it = 1..
- *   record [nrec][nt]: sigma at the receivers after steps delay .. delay+nt-1
- *          (reference record1 = strzz = record2 = strxx, main:450-459)
- *   snaps  [nsnap][popcount(snap_fields)][nx*nz], fields in sigma, vx, vz order */
+ *   record [nrec][nt]: strzz at the receivers after steps delay .. delay+nt-1
+ *          (reference record1, main:450-459; = record2 = strxx when vs = 0)
+ *   snaps  [nsnap][popcount(snap_fields)][nx*nz], fields in P, vx, vz, sxx, szz, sxz order */
 int sfd_visco_forward(sfd_context *ctx, const sfd_visco_params *p, const float *vp, const float *vs,
                       const float *rho, const float *qp, const float *wavelet, float *record, float *snaps);
 int sfd_visco_plan_create(sfd_context *ctx, const sfd_visco_params *p, const float *vp, const float *vs,
@@ -254,7 +257,10 @@ int sfd_visco_plan_create(sfd_context *ctx, const sfd_visco_params *p, const flo
  * receives the device time measured with HIP events on the plan's stream. */
 int sfd_visco_plan_run(sfd_visco_plan *plan, float *elapsed_ms);
 int sfd_visco_plan_fetch(sfd_visco_plan *plan, float *record, float *snaps);
-/* Waves per workgroup (2 or 4) and rows marched per workgroup; 0 = automatic. */
+/* strxx at the receivers [nrec][nt] (reference record2). */
+int sfd_visco_plan_fetch_xx(sfd_visco_plan *plan, float *record_xx);
+/* Waves per workgroup (2 or 4) and rows marched per workgroup; 0 = automatic
+ * (visco-acoustic kernel; the visco-elastic kernels have one tiling). */
 int sfd_visco_plan_set_tiling(sfd_visco_plan *plan, int waves, int chunk_rows);
 int sfd_visco_plan_destroy(sfd_visco_plan *plan);
 

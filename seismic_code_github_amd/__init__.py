@@ -78,6 +78,7 @@ class ViscoParamsC(ctypes.Structure):
         ("sx", ctypes.c_int), ("sz", ctypes.c_int), ("src_sigma", ctypes.c_float),
         ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
         ("nsnap", ctypes.c_int), ("snap_steps", _IP), ("snap_fields", ctypes.c_int),
+        ("qs", _FP),
     ]
 
 
@@ -132,6 +133,7 @@ _SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_void_p)]),
     "sfd_visco_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_visco_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
+    "sfd_visco_plan_fetch_xx": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_visco_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_elastic_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ElasticParamsC), _FP, _FP, _FP, _FP,
@@ -508,6 +510,7 @@ class ViscoShot:
     wavelet: Optional[np.ndarray] = None
     snap_steps: Sequence[int] = ()
     snap_fields: int = FIELD_P
+    qs: Optional[np.ndarray] = None
     _keep: list = field(default_factory=list, repr=False)
 
     def params(self) -> ViscoParamsC:
@@ -516,15 +519,19 @@ class ViscoShot:
         rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
         steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
         coef = None if self.coef is None else _f32(self.coef)
-        self._keep = [rx, rz, steps, coef]
+        qs = None if self.qs is None else _f32(self.qs)
+        self._keep = [rx, rz, steps, coef, qs]
         return ViscoParamsC(nx, nz, int(self.nt), -1 if self.delay is None else int(self.delay),
                             self.dx, self.dz, self.dt, self.f0, self.radius, _ptr(coef), self.nmech,
                             self.q_scale, self.f_low, self.f_high, *[int(v) for v in self.sponge], self.sponge_r,
                             int(self.src[0]), int(self.src[1]), self.src_sigma, rx.size, _ptr(rx, ctypes.c_int),
-                            _ptr(rz, ctypes.c_int), steps.size, _ptr(steps, ctypes.c_int), self.snap_fields)
+                            _ptr(rz, ctypes.c_int), steps.size, _ptr(steps, ctypes.c_int), self.snap_fields, _ptr(qs))
+
+    def elastic(self) -> bool:
+        return self.vs is not None and bool(np.any(np.asarray(self.vs) != 0))
 
     def nfields(self) -> int:
-        return bin(self.snap_fields & 7).count("1")
+        return bin(self.snap_fields & (63 if self.elastic() else 7)).count("1")
 
     def ref_delay(self) -> int:
         """The reference's decay = nint(1.2 / (f0 dt)) (main:279-280), in float."""
@@ -573,6 +580,13 @@ class ViscoPlan:
         self.ctx.check(lib().sfd_visco_plan_fetch(self.handle, _ptr(record) if nrec and self.p.nt else None,
                                                   _ptr(snaps) if self.p.nsnap else None), "sfd_visco_plan_fetch")
         return record, snaps
+
+    def fetch_xx(self) -> np.ndarray:
+        """strxx at the receivers [nrec, nt] (the reference's record2)."""
+        record = np.zeros((self.p.nrec, self.p.nt), dtype=np.float32)
+        self.ctx.check(lib().sfd_visco_plan_fetch_xx(self.handle, _ptr(record) if self.p.nrec and self.p.nt else None),
+                       "sfd_visco_plan_fetch_xx")
+        return record
 
     def close(self):
         if getattr(self, "handle", None):

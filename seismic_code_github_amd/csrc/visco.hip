@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.h"
+#include "visco_el_kernel.h"
 #include "visco_kernel.h"
 
 struct sfd_visco_plan {
@@ -32,6 +33,13 @@ struct sfd_visco_plan {
     sfd::Field s[2], vx[2], vz[2], r, b, ku, kr, xs, zs;
     sfd::DeviceBuffer dvx, dsx, dvz, dsz, g, amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
     void (*kernel)(sfd::ViscoStepArgs) = nullptr;
+    // vs > 0 (visco-elastic): two launches per step on padded fields.
+    bool elastic = false;
+    sfd::ViscoElArgs ve{};
+    sfd::DeviceBuffer ve_arena, ve_xs, ve_zs, rec_zz, rec_xx;
+    int ve_pitch = 0, ve_rows = 0;
+    void (*ve_vel)(sfd::ViscoElArgs) = nullptr;
+    void (*ve_str)(sfd::ViscoElArgs) = nullptr;
 
     float* grid(const sfd::Field& f) const { return f.as<float>() + (size_t)pad * nz; }
     size_t field_floats() const { return (size_t)(nx + 2 * pad) * nz; }
@@ -73,6 +81,25 @@ ViscoKernel visco_pick(int R, int L, int nw) {
     case 5: return visco_pick_l<5>(L, nw);
     default: return nullptr;
     }
+}
+
+constexpr int kVeTX = 8, kVeNW = 8;  // 8 rows x 64 columns, one row per wave
+
+template <int R>
+bool ve_pick_r(int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    *v = visco_el_velocity_kernel<R, kVeTX, kVeNW>;
+    switch (L) {
+    case 1: *st = visco_el_stress_kernel<R, 1, kVeTX, kVeNW>; return true;
+    case 2: *st = visco_el_stress_kernel<R, 2, kVeTX, kVeNW>; return true;
+    case 3: *st = visco_el_stress_kernel<R, 3, kVeTX, kVeNW>; return true;
+    default: return false;
+    }
+}
+
+bool ve_pick(int R, int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    if (R == 4) return ve_pick_r<4>(L, v, st);
+    if (R == 5) return ve_pick_r<5>(L, v, st);
+    return false;
 }
 
 int visco_pad_rows(int R) { return R == 4 ? kViscoPadRows<4> : kViscoPadRows<5>; }
@@ -232,6 +259,52 @@ int visco_zero(sfd_visco_plan* pl) {
     return SFD_OK;
 }
 
+int ve_zero(sfd_visco_plan* pl) {
+    sfd_context* ctx = pl->ctx;
+    // State (vx, vz, three stresses, 3L memory fields) lies at the front of
+    // the arena, before the model fields.
+    const size_t state_bytes = (size_t)((const char*)pl->ve.b - pl->ve_arena.as<char>()) -
+                               ((size_t)kVePad * pl->ve_pitch + kVePad) * sizeof(float);
+    SFD_HIP(ctx, hipMemsetAsync(pl->ve_arena.ptr, 0, state_bytes, ctx->stream));
+    if (pl->ve_xs.bytes) SFD_HIP(ctx, hipMemsetAsync(pl->ve_xs.ptr, 0, pl->ve_xs.bytes, ctx->stream));
+    if (pl->ve_zs.bytes) SFD_HIP(ctx, hipMemsetAsync(pl->ve_zs.ptr, 0, pl->ve_zs.bytes, ctx->stream));
+    return SFD_OK;
+}
+
+int ve_enqueue(sfd_visco_plan* pl) {
+    sfd_context* ctx = pl->ctx;
+    ViscoElArgs a = pl->ve;
+    const dim3 grid(a.ntiles), block(64 * kVeNW);
+    const size_t nb = (size_t)pl->nz * sizeof(float), pb = (size_t)pl->ve_pitch * sizeof(float);
+    const bool rec = pl->nrec > 0 && pl->nt > 0;
+    size_t snap_next = 0;
+    for (int k = 0; k < pl->nsteps; ++k) {
+        a.step = k;
+        // the velocity launch of step k gathers the stresses of step k - 1
+        a.rec_col = (rec && k - 1 >= pl->delay) ? k - 1 - pl->delay : -1;
+        hipLaunchKernelGGL(pl->ve_vel, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(pl->ve_str, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
+            float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
+            const float* src[6] = {a.szz, a.vx, a.vz, a.sxx, a.szz, a.sxz};
+            for (int f = 0; f < 6; ++f) {
+                if (!(pl->snap_fields & (1 << f))) continue;
+                SFD_HIP(ctx, hipMemcpy2DAsync(dst, nb, src[f], pb, nb, pl->nx, hipMemcpyDeviceToDevice, ctx->stream));
+                dst += (size_t)pl->nx * pl->nz;
+            }
+            ++snap_next;
+        }
+    }
+    if (rec && pl->nsteps - 1 >= pl->delay) {
+        a.rec_col = pl->nsteps - 1 - pl->delay;
+        hipLaunchKernelGGL(visco_el_gather_kernel, dim3((pl->nrec + 255) / 256), dim3(256), 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+    }
+    return SFD_OK;
+}
+
 int visco_enqueue(sfd_visco_plan* pl, int k0, int nsteps) {
     sfd_context* ctx = pl->ctx;
     const int ow = 64 * pl->nw - (4 * pl->radius - 2);
@@ -275,11 +348,18 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     SFD_HIP(ctx, hipSetDevice(ctx->device));
     const int nx = p->nx, nz = p->nz, R = p->radius, L = p->nmech;
     const size_t n = (size_t)nx * nz;
+    // Any vs > 0 selects the visco-elastic kernels; vs = 0 everywhere is the
+    // visco-acoustic limit (sxx = szz, sxz = 0), one fused kernel per step.
+    bool elastic = false;
     if (vs) {
-        for (size_t c = 0; c < n; ++c)
-            if (vs[c] != 0.0f)
-                return fail(ctx, SFD_ENOTSUP, "vs > 0 (visco-elastic) is not supported; pass vs = NULL or zeros");
+        for (size_t c = 0; c < n; ++c) {
+            if (!(vs[c] >= 0.f) || !std::isfinite(vs[c])) return fail(ctx, SFD_EINVAL, "vs must be finite and >= 0 (cell %zu)", c);
+            if (vs[c] != 0.0f) elastic = true;
+        }
     }
+    if (p->qs)
+        for (size_t c = 0; c < n; ++c)
+            if (!(p->qs[c] > 0.f) || !std::isfinite(p->qs[c])) return fail(ctx, SFD_EINVAL, "qs must be finite and > 0 (cell %zu)", c);
     float vmax = 0.f;
     for (size_t c = 0; c < n; ++c) {
         if (!(vp[c] > 0.f) || !std::isfinite(vp[c])) return fail(ctx, SFD_EINVAL, "vp must be finite and > 0 (cell %zu)", c);
@@ -339,7 +419,7 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     const double wlo = 2.0 * kPi * (p->f_low > 0 ? p->f_low : 2.0f);
     const double whi = 2.0 * kPi * (p->f_high > 0 ? p->f_high : 2.5f * p->f0);
     if (!(whi > wlo)) return bail(fail(ctx, SFD_EINVAL, "tau-method band: f_high must exceed f_low"));
-    const double qs = p->q_scale > 0 ? p->q_scale : 0.9;
+    const double qscale = p->q_scale > 0 ? p->q_scale : 0.9;
     double ts[kViscoMaxMech];
     relaxation_times(L, wlo, whi, ts);
     for (int l = 0; l < L; ++l) {
@@ -349,13 +429,24 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     }
     // tau is proportional to 1/q: evaluate the band integrals once.
     const double tau1 = tau_of_q(L, wlo, whi, ts, 1.0);
-    std::vector<float> hb(n), hku(n), hkr(n);
+    std::vector<float> hb(n), hku(n), hkr(n), hkus, hkrs;
+    if (elastic) {
+        hkus.resize(n);
+        hkrs.resize(n);
+    }
     for (size_t c = 0; c < n; ++c) {
         const double m = (double)rb[c] * (double)vp[c] * (double)vp[c];
-        const double tau = tau1 / ((double)qp[c] * qs);
+        const double tau = tau1 / ((double)qp[c] * qscale);
         hb[c] = (float)((double)p->dt / rb[c]);
         hku[c] = (float)((double)p->dt * m * (1.0 + L * tau));
         hkr[c] = (float)(m * tau);
+        if (elastic) {
+            // mu = rho_avg vs^2 with the S-wave tau from qs (qp where qs is not given)
+            const double mu = (double)rb[c] * (double)vs[c] * (double)vs[c];
+            const double taus = tau1 / ((double)(p->qs ? p->qs[c] : qp[c]) * qscale);
+            hkus[c] = (float)((double)p->dt * mu * (1.0 + L * taus));
+            hkrs[c] = (float)(mu * taus);
+        }
     }
 
     // ---- sponge profiles (csponge) ----
@@ -451,6 +542,100 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
         hamp[k] = w * p->dt / (p->dz * p->dz);
     }
 
+    if (elastic) {
+        pl->elastic = true;
+        pl->nrec = p->nrec;
+        pl->nsnap = p->nsnap;
+        pl->snap_fields = p->snap_fields & 63;
+        pl->nsnap_fields = __builtin_popcount(pl->snap_fields);
+        pl->snap_steps.assign(p->snap_steps, p->snap_steps + p->nsnap);
+        if (!ve_pick(R, L, &pl->ve_vel, &pl->ve_str)) return bail(fail(ctx, SFD_ENOTSUP, "no visco-elastic kernel"));
+        pl->ve_pitch = (nz + kVeTZ - 1) / kVeTZ * kVeTZ + 2 * kVePad;
+        pl->ve_rows = nx + kVeTX + 2 * kVePad;
+        const size_t fe = (size_t)pl->ve_pitch * pl->ve_rows;  // floats per padded field
+        hipError_t e2 = pl->ve_arena.alloc((5 + 3 * (size_t)L + 5) * fe * sizeof(float));
+        const int xw = pl->xlo + pl->xhi, zw = pl->zlo + pl->zhi;
+        if (e2 == hipSuccess) e2 = pl->ve_xs.alloc((size_t)(5 + 3 * L) * xw * nz * sizeof(float));
+        if (e2 == hipSuccess) e2 = pl->ve_zs.alloc((size_t)(5 + 3 * L) * nx * zw * sizeof(float));
+        if (e2 != hipSuccess) return bail(hip_check(ctx, e2, "visco-elastic allocation"));
+        float* base = pl->ve_arena.as<float>();
+        auto cell0 = [&](int q) { return base + (size_t)q * fe + (size_t)kVePad * pl->ve_pitch + kVePad; };
+        ViscoElArgs& a = pl->ve;
+        a.nx = nx;
+        a.nz = nz;
+        a.pitch = pl->ve_pitch;
+        a.ntz = (nz + kVeTZ - 1) / kVeTZ;
+        a.ntiles = a.ntz * ((nx + kVeTX - 1) / kVeTX);
+        a.vx = cell0(0);
+        a.vz = cell0(1);
+        a.sxx = cell0(2);
+        a.szz = cell0(3);
+        a.sxz = cell0(4);
+        a.r = cell0(5);
+        a.rstride = fe;
+        a.b = cell0(5 + 3 * L);
+        a.kup = cell0(6 + 3 * L);
+        a.kus = cell0(7 + 3 * L);
+        a.krp = cell0(8 + 3 * L);
+        a.krs = cell0(9 + 3 * L);
+        a.hx = pl->hx;
+        a.hz = pl->hz;
+        a.dt = p->dt;
+        for (int m = 0; m < 5; ++m) a.a[m] = m < R ? pl->a[m] : 0.f;
+        for (int l = 0; l < kVeMaxMech; ++l) a.c[l] = l < L ? pl->c[l] : 0.f;
+        a.xlo = pl->xlo;
+        a.xhi = pl->xhi;
+        a.zlo = pl->zlo;
+        a.zhi = pl->zhi;
+        a.xs = pl->ve_xs.as<float>();
+        a.zs = pl->ve_zs.as<float>();
+        a.sx0 = pl->sx0;
+        a.sx1 = pl->sx1;
+        a.sz0 = pl->sz0;
+        a.sz1 = pl->sz1;
+        a.g = pl->g.as<float>();
+        a.nrec = p->nrec;
+        a.nrt = std::max(p->nt, 1);
+        hipStream_t sm = ctx->stream;
+        // Model fields into their padded layout.
+        std::vector<float> padded(fe, 0.f);
+        const std::vector<float>* model[5] = {&hb, &hku, &hkus, &hkr, &hkrs};
+        for (int q = 0; q < 5 && e2 == hipSuccess; ++q) {
+            for (int i = 0; i < nx; ++i)
+                std::copy(model[q]->begin() + (size_t)i * nz, model[q]->begin() + (size_t)(i + 1) * nz,
+                          padded.begin() + (size_t)(i + kVePad) * pl->ve_pitch + kVePad);
+            e2 = hipMemcpyAsync(base + (size_t)(5 + 3 * L + q) * fe, padded.data(), fe * sizeof(float),
+                                hipMemcpyHostToDevice, sm);
+            if (e2 == hipSuccess) e2 = hipStreamSynchronize(sm);
+        }
+        if (e2 == hipSuccess) e2 = upload(pl->dvx, hdvx.data() + pad, nx, sm);
+        if (e2 == hipSuccess) e2 = upload(pl->dsx, hdsx.data() + pad, nx, sm);
+        if (e2 == hipSuccess) e2 = upload(pl->dvz, hdvz.data(), hdvz.size(), sm);
+        if (e2 == hipSuccess) e2 = upload(pl->dsz, hdsz.data(), hdsz.size(), sm);
+        if (e2 == hipSuccess) e2 = upload(pl->amp, hamp.data(), hamp.size(), sm);
+        if (e2 == hipSuccess && p->nrec > 0) e2 = upload(pl->rec_row, p->rx, p->nrec, sm);   // rx
+        if (e2 == hipSuccess && p->nrec > 0) e2 = upload(pl->rec_slot, p->rz, p->nrec, sm);  // rz
+        if (e2 == hipSuccess && p->nrec > 0) e2 = pl->rec_zz.alloc((size_t)p->nrec * a.nrt * 4);
+        if (e2 == hipSuccess && p->nrec > 0) e2 = pl->rec_xx.alloc((size_t)p->nrec * a.nrt * 4);
+        if (e2 == hipSuccess && p->nsnap > 0) e2 = pl->snaps.alloc((size_t)p->nsnap * pl->nsnap_fields * n * 4);
+        if (e2 != hipSuccess) return bail(hip_check(ctx, e2, "visco-elastic set-up"));
+        a.dvx = pl->dvx.as<float>();
+        a.dsx = pl->dsx.as<float>();
+        a.dvz = pl->dvz.as<float>();
+        a.dsz = pl->dsz.as<float>();
+        a.amp = pl->amp.as<float>();
+        a.rx = pl->rec_row.as<int>();
+        a.rz = pl->rec_slot.as<int>();
+        a.rec_zz = pl->rec_zz.as<float>();
+        a.rec_xx = pl->rec_xx.as<float>();
+        rc = ve_zero(pl);
+        if (rc) return bail(rc);
+        e2 = hipStreamSynchronize(sm);
+        if (e2 != hipSuccess) return bail(hip_check(ctx, e2, "plan setup"));
+        *out = pl;
+        return SFD_OK;
+    }
+
     // ---- receivers ----
     pl->nrec = p->nrec;
     std::vector<int> hslot;
@@ -525,6 +710,7 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
 
 int sfd_visco_plan_set_tiling(sfd_visco_plan* pl, int waves, int chunk_rows) {
     if (!pl) return SFD_EINVAL;
+    if (pl->elastic) return SFD_OK;  // the visco-elastic kernels have one tiling
     int nw = pl->nw, ch = pl->chunk;
     if (waves != 0 || chunk_rows != 0) {
         visco_auto_tiling(pl, &nw, &ch);
@@ -547,10 +733,10 @@ int sfd_visco_plan_run(sfd_visco_plan* pl, float* elapsed_ms) {
     if (!pl) return SFD_EINVAL;
     sfd_context* ctx = pl->ctx;
     SFD_HIP(ctx, hipSetDevice(ctx->device));
-    int rc = visco_zero(pl);
+    int rc = pl->elastic ? ve_zero(pl) : visco_zero(pl);
     if (rc) return rc;
     SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    rc = visco_enqueue(pl, 0, pl->nsteps);
+    rc = pl->elastic ? ve_enqueue(pl) : visco_enqueue(pl, 0, pl->nsteps);
     if (rc) return rc;
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
@@ -562,6 +748,15 @@ int sfd_visco_plan_fetch(sfd_visco_plan* pl, float* record, float* snaps) {
     if (!pl) return SFD_EINVAL;
     sfd_context* ctx = pl->ctx;
     SFD_HIP(ctx, hipSetDevice(ctx->device));
+    if (pl->elastic) {
+        if (record && pl->nrec > 0 && pl->nt > 0)
+            SFD_HIP(ctx, hipMemcpyAsync(record, pl->rec_zz.ptr, (size_t)pl->nrec * pl->nt * 4, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        if (snaps && pl->nsnap > 0)
+            SFD_HIP(ctx, hipMemcpyAsync(snaps, pl->snaps.ptr, pl->snaps.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return SFD_OK;
+    }
     if (record && pl->nrec > 0 && pl->nt > 0) {
         const dim3 block(256), grid((pl->nt + 255) / 256, pl->nrec);
         hipLaunchKernelGGL(visco_gather_kernel, grid, block, 0, ctx->stream, pl->rec_buf.as<float>(),
@@ -572,6 +767,18 @@ int sfd_visco_plan_fetch(sfd_visco_plan* pl, float* record, float* snaps) {
     }
     if (snaps && pl->nsnap > 0)
         SFD_HIP(ctx, hipMemcpyAsync(snaps, pl->snaps.ptr, pl->snaps.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SFD_OK;
+}
+
+int sfd_visco_plan_fetch_xx(sfd_visco_plan* pl, float* record_xx) {
+    if (!pl) return SFD_EINVAL;
+    if (!pl->elastic) return sfd_visco_plan_fetch(pl, record_xx, nullptr);  // sxx = szz when vs = 0
+    sfd_context* ctx = pl->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    if (record_xx && pl->nrec > 0 && pl->nt > 0)
+        SFD_HIP(ctx, hipMemcpyAsync(record_xx, pl->rec_xx.ptr, (size_t)pl->nrec * pl->nt * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
     SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return SFD_OK;
 }

@@ -26,10 +26,11 @@ def rel(a, b):
 
 
 def run_oracle(shot: sfd.ViscoShot, precision="f32", snap_steps=()):
+    print(f"oracle {precision}: {shot.vp.shape}", flush=True)
     delay = shot.ref_delay()
     return oracle.visco(shot.vp, shot.rho if shot.rho is not None else np.full_like(shot.vp, 2400.0), shot.qp,
                         src=shot.src, rec=shot.rec, nsteps=delay + shot.nt, dx=shot.dx, dz=shot.dz, dt=shot.dt,
-                        f0=shot.f0, vs=shot.vs, radius=shot.radius, coef=shot.coef, nmech=shot.nmech,
+                        f0=shot.f0, vs=shot.vs, qs=shot.qs, radius=shot.radius, coef=shot.coef, nmech=shot.nmech,
                         q_scale=shot.q_scale, f_low=shot.f_low, f_high=shot.f_high, sponge=shot.sponge,
                         sponge_r=shot.sponge_r, src_sigma=shot.src_sigma, wavelet=shot.wavelet,
                         rec_delay=delay, snap_steps=snap_steps, precision=precision,
@@ -130,8 +131,8 @@ def test_visco_plan_rerun_identical(ctx):
 def test_visco_errors(ctx):
     shot = small_shot(nt=10)
     with pytest.raises(sfd.SeisfdError) as e:
-        sfd.visco_forward(small_shot(nt=10, vs=np.full_like(shot.vp, 1000.0)), ctx)
-    assert e.value.code == sfd.SFD_ENOTSUP
+        sfd.visco_forward(small_shot(nt=10, vs=np.full_like(shot.vp, -5.0)), ctx)
+    assert e.value.code == sfd.SFD_EINVAL
     with pytest.raises(sfd.SeisfdError) as e:
         sfd.visco_forward(small_shot(nt=10, dt=5e-3), ctx)
     assert e.value.code == sfd.SFD_ECFL
@@ -142,3 +143,51 @@ def test_visco_errors(ctx):
         sfd.visco_forward(small_shot(nt=10, src=(-1, 3)), ctx)
     with pytest.raises(sfd.SeisfdError):
         sfd.visco_forward(small_shot(nt=10, sponge=(80, 80, 10, 10)), ctx)
+
+
+# ---------------------------------------------------------------- vs > 0
+
+def elastic_shot(nmech=1, nx=140, nz=120, nt=450, sponge=(18, 22, 16, 20), **kw):
+    shot = small_shot(nmech=nmech, nx=nx, nz=nz, nt=nt, sponge=sponge, **kw)
+    shot.vs = (shot.vp / W.smooth_random_model(nx, nz, 6.0, 1.6, 2.2, 24)).astype(np.float32)
+    shot.qs = W.smooth_random_model(nx, nz, 6.0, 15.0, 80.0, 25)
+    return shot
+
+
+@pytest.mark.parametrize("nmech", [1, 3])
+def test_visco_elastic_matches_oracle(ctx, nmech):
+    """vs > 0: the full visco-elastic RSG step (three stresses, 3L memory
+    variables, split parts damped in all four strips) against the oracle's
+    restatement of extrapolation_one_step_10nd_absbc: both gathers (record1 =
+    strzz, record2 = strxx) and every field of two snapshots."""
+    all_f = sfd.FIELD_P | sfd.FIELD_VX | sfd.FIELD_VZ | sfd.FIELD_SXX | sfd.FIELD_SZZ | sfd.FIELD_SXZ
+    shot = elastic_shot(nmech=nmech, snap_steps=(150, 400), snap_fields=all_f)
+    with sfd.ViscoPlan(shot, ctx) as pl:
+        pl.run()
+        rzz, snaps = pl.fetch()
+        rxx = pl.fetch_xx()
+    o32 = run_oracle(shot, "f32", snap_steps=(150, 400))
+    o64 = run_oracle(shot, "f64", snap_steps=(150, 400))
+    for name, got in (("rec_zz", rzz), ("rec_xx", rxx)):
+        e32, e64, eo = rel(got, o32[name]), rel(got, o64[name]), rel(o32[name], o64[name])
+        print(f"L={nmech} {name}: rel-L2 vs FP32 oracle {e32:.2e}, vs FP64 {e64:.2e} (oracle FP32 vs FP64 {eo:.2e})")
+        assert np.abs(o32[name]).max() > 0
+        assert e32 <= TOL_F32
+        assert e64 <= 10 * max(eo, TOL_F32)
+    assert rel(rzz, rxx) > 1e-3  # vs > 0: the two gathers differ
+    # snaps order here: P (= szz), vx, vz, sxx, szz, sxz; the oracle's sxx, szz, sxz, vx, vz
+    for s_ in range(2):
+        for f, of in ((0, 1), (1, 3), (2, 4), (3, 0), (4, 1), (5, 2)):
+            e = rel(snaps[s_, f], o32["snaps"][s_, of])
+            assert e <= TOL_F32, (s_, f, e)
+
+
+def test_visco_elastic_fluid_cells_and_default_qs(ctx):
+    """A fluid layer (vs = 0 in the top rows) inside a visco-elastic model,
+    qs = None (qp used for both waves)."""
+    shot = elastic_shot(nmech=2, nt=300)
+    shot.vs[:, :30] = 0.0
+    shot.qs = None
+    rzz, _ = sfd.visco_forward(shot, ctx)
+    o32 = run_oracle(shot)
+    assert rel(rzz, o32["rec_zz"]) <= TOL_F32
