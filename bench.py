@@ -120,6 +120,29 @@ def visco_leg(nt_err, nt, no_cpu):
            "bytes_per_cell_update": 36 + 8 * L,
            "achieved_GBs": round(bytes_step * nsteps / (ms / 1e3) / 1e9, 1),
            "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4)}
+    # The same shot with vs > 0 (vs = vp / a seeded smooth ratio in 1.6..2.2,
+    # qs = 0.8 qp): the visco-elastic kernels, two launches per step.
+    vs = (shot.vp / W.smooth_random_model(nx, nz, 20.0, 1.6, 2.2, 7)).astype(np.float32)
+    eshot = W.visco_1001x2001(nt=nt, vs=vs, qs=(0.8 * shot.qp).astype(np.float32))
+    with sfd.ViscoPlan(eshot) as pl:
+        pl.run()
+        ems = min(pl.run() for _ in range(2))
+        erec, _ = pl.fetch()
+    assert np.isfinite(erec).all()
+    xcells = (sp[0] + sp[1] + 2) * nz
+    zcells = (sp[2] + sp[3] + 2) * nx
+    # Per cell: velocity launch reads 3 stresses, vx, vz, b and writes vx, vz
+    # (32 B); stress launch reads vx, vz, 3 stresses, 3L memory variables and
+    # 4 moduli, writes 3 stresses and 3L memory variables (48 + 24 L B).  Per
+    # strip cell and direction: 2 velocity parts and 3 + 3L stress / memory
+    # parts read and written.
+    ebytes = (80.0 + 24.0 * L) * nx * nz + 8.0 * (5 + 3 * L) * (xcells + zcells)
+    out["visco_elastic"] = {"workload": f"same shot, vs = vp / (1.6..2.2), qs = 0.8 qp, L={L}",
+                            "ms_per_shot": round(ems, 2), "Mcell_per_s": round(nx * nz * nsteps / ems / 1e3, 1),
+                            "us_per_step": round(ems * 1e3 / nsteps, 2),
+                            "bytes_per_cell_update": 80 + 24 * L,
+                            "achieved_GBs": round(ebytes * nsteps / (ems / 1e3) / 1e9, 1),
+                            "frac_of_8TBs": round(ebytes * nsteps / (ems / 1e3) / 8e12, 4)}
     tpath = os.path.join(ROOT, "profiles", "visco_step_L3_2081x1081_traffic.json")
     if os.path.exists(tpath) and nt == 4000:
         with open(tpath) as f:
