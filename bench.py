@@ -10,6 +10,14 @@ One "step" of this benchmark is one whole shot (nt time steps) with the model,
 wavelet and geometry already resident in device memory.  value = Mcell-updates/s
 over all stored cells (boundary layers included).
 
+Roofline: the library runs each time step as two launches (velocities, then
+pressure; acoustic_tiled.h), so the roofline's unit is one time step:
+achieved = the fused step's algorithmic bytes (32 B per cell plus the C-PML
+strips' memory variables) / the device time of one step (both launches),
+measured with HIP events on the library's stream over the timed shots;
+traffic = both launches' FETCH_SIZE + WRITE_SIZE per step, scaled by a
+256 MiB copy (profiles/acoustic_step_cpml_2101x4101.md).
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -370,7 +378,8 @@ def main():
         "shots_per_s": round(args.steps / wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": bytes_step, "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+                     "unit_of_measure": "one time step = velocity launch + pressure launch",
+                     "bytes_per_step": bytes_step, "avg_step_us": round(avg_launch_s * 1e6, 2)},
     }
     if not args.no_batch:
         log("batched leg ...")

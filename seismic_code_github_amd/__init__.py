@@ -114,6 +114,7 @@ _SIGNATURES = {
     "sfd_acoustic_plan_get_state": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
     "sfd_acoustic_plan_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_plan_set_tiled": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_cfl_limit": (ctypes.c_double, [ctypes.c_int, _FP]),
     "sfd_acoustic_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(AcousticParamsC),
                                                  ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
@@ -290,6 +291,10 @@ class AcousticPlan:
 
     def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
         self.ctx.check(lib().sfd_acoustic_plan_set_tiling(self.handle, waves, chunk_rows), "set_tiling")
+
+    def set_tiled(self, tile_rows: int = 0, waves: int = 0):
+        """C-PML: the two-launch tiled step (tile_rows < 0 = the march kernel)."""
+        self.ctx.check(lib().sfd_acoustic_plan_set_tiled(self.handle, tile_rows, waves), "set_tiled")
 
     def run(self) -> float:
         ms = ctypes.c_float()

@@ -8,6 +8,7 @@
 #include <tuple>
 
 #include "acoustic_plan.h"
+#include "acoustic_tiled.h"
 
 namespace sfd {
 
@@ -193,6 +194,42 @@ void cpml_profile(int n, int w, double h, double dt, double R, double vmax, doub
 
 // Chunks start at rows congruent to R modulo 2R (the kernel's queue slots are
 // compile-time constants), so the row count of a chunk is a multiple of 2R.
+template <int R>
+bool pick_tiled_r(int tx, int nw, StepKernel* v, StepKernel* p) {
+    switch (tx * 100 + nw) {
+#define SFD_TILED(TX, NW)                                  \
+    case TX * 100 + NW:                                    \
+        *v = acoustic_tiled_velocity<R, TX, NW>;           \
+        *p = acoustic_tiled_pressure<R, TX, NW>;           \
+        return true;
+        SFD_TILED(8, 4)
+        SFD_TILED(8, 8)
+        SFD_TILED(16, 4)
+        SFD_TILED(16, 8)
+        SFD_TILED(16, 16)
+        SFD_TILED(32, 8)
+        SFD_TILED(32, 16)
+#undef SFD_TILED
+    default:
+        return false;
+    }
+}
+
+bool pick_tiled(int radius, int tx, int nw, StepKernel* v, StepKernel* p) {
+    switch (radius) {
+    case 4: return pick_tiled_r<4>(tx, nw, v, p);
+    case 5: return pick_tiled_r<5>(tx, nw, v, p);
+    case 8: return pick_tiled_r<8>(tx, nw, v, p);
+    default: return false;
+    }
+}
+
+// Default tiled shape, measured on MI355X for the 2101 x 4101 C-PML grid (R =
+// 8, us per step, both launches): 16 rows x 4 waves 58.9, 32 x 8 60.4, 16 x 8
+// 60.6, 8 x 4 62.9, 32 x 16 66.7, 8 x 8 78.3, 16 x 16 86.5 (the fused march
+// kernel: 125.1).
+constexpr int kTiledTX = 16, kTiledNW = 4;
+
 int round_chunk(int ch, int radius) {
     const int q = 2 * radius;
     return std::max(q, (ch + q - 1) / q * q);
@@ -309,13 +346,37 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     const size_t nbytes = (size_t)pl->nx * pl->nz * sizeof(float);
     size_t snap_next = std::lower_bound(pl->snap_steps.begin(), pl->snap_steps.end(), k0) - pl->snap_steps.begin();
 
+    const int R = pl->radius;
+    const int tntz = (pl->nz - 2 * R + kTiledTZ - 1) / kTiledTZ;
+    const int tnch = pl->ttx > 0 ? (pl->nx - 2 * R + pl->ttx - 1) / pl->ttx : 0;
     for (int s = 0; s < nsteps; ++s) {
         const int k = k0 + s;
-        const int o = pl->cur ^ 1;
         a = make_args(pl, pl->cur, k);
-        hipLaunchKernelGGL(pl->kernel, grid, block, 0, ctx->stream, a);
+        if (pl->ttx > 0) {
+            // In place: within each launch a cell's outputs are read by no
+            // other workgroup (velocities read P's halo, pressure reads the
+            // velocities' halo), so one set of buffers suffices and the
+            // step's working set is half the ping-pong one (it then fits the
+            // 256 MB MALL at 2101 x 4101).
+            a.p_out = const_cast<float*>(a.p_in);
+            a.vx_out = const_cast<float*>(a.vx_in);
+            a.vz_out = const_cast<float*>(a.vz_in);
+            a.psi_px_out = const_cast<float*>(a.psi_px_in);
+            a.psi_vx_out = const_cast<float*>(a.psi_vx_in);
+            a.psi_pz_out = const_cast<float*>(a.psi_pz_in);
+            a.psi_vz_out = const_cast<float*>(a.psi_vz_in);
+            a.ntz = tntz;
+            a.nchunk = tnch;
+            const dim3 tg(tntz * tnch), tb(64 * pl->tnw);
+            hipLaunchKernelGGL(pl->tvel, tg, tb, 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(pl->tpres, tg, tb, 0, ctx->stream, a);
+        } else {
+            hipLaunchKernelGGL(pl->kernel, grid, block, 0, ctx->stream, a);
+            pl->cur ^= 1;
+        }
         SFD_HIP(ctx, hipGetLastError());
-        pl->cur = o;
+        const int o = pl->cur;
         while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
             float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
             const float* src[3] = {pl->grid(pl->p[o]), pl->grid(pl->vx[o]), pl->grid(pl->vz[o])};
@@ -501,6 +562,10 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
     auto_tiling(pl, &pl->nw, &pl->chunk);
     pl->kernel = pick_step_kernel(pl->radius, pl->nw, pl->mode);
     if (!pl->kernel) return bail(fail(ctx, SFD_ENOTSUP, "no kernel for this configuration"));
+    if (pl->mode == MODE_FAST_CPML && pick_tiled(pl->radius, kTiledTX, kTiledNW, &pl->tvel, &pl->tpres)) {
+        pl->ttx = kTiledTX;
+        pl->tnw = kTiledNW;
+    }
     rc = zero_state(pl);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(s);
@@ -525,6 +590,32 @@ int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan* pl, int waves, int chunk_row
     pl->nw = nw;
     pl->chunk = ch;
     pl->kernel = k;
+    // An explicit march tiling selects the march kernel; automatic restores
+    // the tiled step where it applies.
+    pl->ttx = pl->tnw = 0;
+    if (waves == 0 && chunk_rows == 0 && pl->mode == MODE_FAST_CPML &&
+        pick_tiled(pl->radius, kTiledTX, kTiledNW, &pl->tvel, &pl->tpres)) {
+        pl->ttx = kTiledTX;
+        pl->tnw = kTiledNW;
+    }
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan* pl, int tile_rows, int waves) {
+    if (!pl) return SFD_EINVAL;
+    if (tile_rows < 0) {
+        pl->ttx = pl->tnw = 0;
+        return SFD_OK;
+    }
+    if (pl->mode != MODE_FAST_CPML) return fail(pl->ctx, SFD_ENOTSUP, "the tiled step exists for C-PML in fast arithmetic");
+    const int tx = tile_rows ? tile_rows : kTiledTX, nw = waves ? waves : kTiledNW;
+    StepKernel v, p;
+    if (!pick_tiled(pl->radius, tx, nw, &v, &p))
+        return fail(pl->ctx, SFD_EINVAL, "no tiled step for %d rows x %d waves", tx, nw);
+    pl->ttx = tx;
+    pl->tnw = nw;
+    pl->tvel = v;
+    pl->tpres = p;
     return SFD_OK;
 }
 

@@ -43,6 +43,10 @@ struct sfd_acoustic_plan {
     DeviceBuffer ax, az, pml_x, pml_z;
     DeviceBuffer amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
     StepKernel kernel = nullptr;
+    // C-PML in fast arithmetic runs the two-launch tiled step instead
+    // (acoustic_tiled.h): tile rows and waves, 0 = the march kernel.
+    int ttx = 0, tnw = 0;
+    StepKernel tvel = nullptr, tpres = nullptr;
 
     // Gradient workspace (plans made by the gradient entry points): adjoint
     // state, gradient accumulator and the divergence of one checkpoint

@@ -1,0 +1,210 @@
+// Two-launch tiled time step of the acoustic velocity-pressure staggered-grid
+// propagator with C-PML boundaries (fast arithmetic): the same step as
+// acoustic_step_kernel in MODE_FAST_CPML (reference loop
+// forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:247-377, absorbing
+// strips replaced by the C-PML), split at the reference's own seam:
+//   acoustic_tiled_velocity  Vx, Vz from P (source injected)      (:247-286)
+//   acoustic_tiled_pressure  P from the new Vx, Vz; receivers     (:289-314, :376-377)
+// A workgroup of NW waves owns a tile of TX rows x 64 columns of the updated
+// region [R, nx - R) x [R, nz - R), one column per lane, TX / NW rows per
+// wave.  The tile and the R-cell halo its stencils read are staged in LDS
+// once (loads unrolled so all are in flight before the first store), tiles go
+// to the XCDs in bands of whole rows.  Every field is streamed once per
+// launch: 24 B per cell for the velocity launch (P, Vx, Vz, bf read; Vx, Vz
+// written), 20 B for the pressure launch (Vx, Vz, P, kf read; P written);
+// the fused march kernel needs 32 B per step but cannot keep as many bytes
+// in flight.  Fields keep the plan's layout: rows of nz floats with zero rows
+// above and below (columns past either edge wrap into the neighbouring row;
+// no updated cell's stencil reaches them).  The plan runs these kernels in
+// place (out == in): within a launch no workgroup reads a cell that another
+// writes, and every output depends on its own cell's input.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "acoustic_kernel.h"
+#include "device_util.h"
+
+namespace sfd {
+
+constexpr int kTiledTZ = 64;
+
+__device__ __forceinline__ void tiled_tile(const AcousticStepArgs& a, int tx, int R, int* i0, int* j0) {
+    const int t = xcd_tile((int)blockIdx.x, a.ntz * a.nchunk);
+    *j0 = R + (t % a.ntz) * kTiledTZ;
+    *i0 = R + (t / a.ntz) * tx;
+}
+
+// Rows [i0 - HR, i0 + TX + HR) x columns [j0 - HC, j0 + 64 + HC) of a field
+// (row 0 at f, rows nz floats apart) into lds[row][col].  With src, the
+// source amplitude is taken off cell (sx, sz) on the way.
+template <int TX, int NW, int HR, int HC, bool SRC>
+__device__ __forceinline__ void tiled_stage(float* lds, const float* f, int nz, int i0, int j0, int sx, int sz,
+                                            float amp) {
+    constexpr int W = kTiledTZ + 2 * HC, Hh = TX + 2 * HR, N = W * Hh, T = 64 * NW;
+    constexpr int ITER = (N + T - 1) / T;
+    float v[ITER];
+#pragma unroll
+    for (int q = 0; q < ITER; ++q) {
+        const int e = threadIdx.x + q * T;
+        const int r = e / W, c = e - r * W;
+        const int gi = i0 - HR + r, gj = j0 - HC + c;
+        float x = (e < N) ? f[(ptrdiff_t)gi * nz + gj] : 0.f;
+        if (SRC && gi == sx && gj == sz) x -= amp;
+        v[q] = x;
+    }
+#pragma unroll
+    for (int q = 0; q < ITER; ++q) {
+        const int e = threadIdx.x + q * T;
+        if (e < N) lds[e] = v[q];
+    }
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity(const AcousticStepArgs a) {
+    constexpr int W = kTiledTZ + 2 * R, RPW = TX / NW;
+    __shared__ float s_p[(TX + 2 * R) * W];
+    int i0, j0;
+    tiled_tile(a, TX, R, &i0, &j0);
+    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = j0 + lane;
+    const bool jok = j < nz - R;
+    // This lane's own-cell inputs, loaded before the tile so that their
+    // latency overlaps the staging.
+    float vx0[RPW], vz0[RPW], bf[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int i = i0 + wv + q * NW;
+        const bool ok = jok && i < nx - R;
+        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
+        vx0[q] = ok ? a.vx_in[o] : 0.f;
+        vz0[q] = ok ? a.vz_in[o] : 0.f;
+        bf[q] = ok ? a.bf[o] : 0.f;
+    }
+    const float amp = a.amp[a.k];
+    tiled_stage<TX, NW, R, R, true>(s_p, a.p_in, nz, i0, j0, a.sx, a.sz, amp);
+    __syncthreads();
+    if (!jok) return;
+    float cx[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
+    const float zx = a.zx;
+    const int xw = a.pml_xw, zw = a.pml_zw;
+    int zc = -1;
+    if (j < zw) zc = j;
+    else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int il = wv + q * NW, i = i0 + il;
+        if (i >= nx - R) break;
+        const float* p = s_p + (il + R) * W + lane + R;
+        // Vx(i) sits between P(i) and P(i + 1); Vz(i, j) between P(i, j), P(i, j + 1).
+        float sx = cx[0] * (p[W] - p[0]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) sx = __builtin_fmaf(cx[m - 1], p[m * W] - p[(1 - m) * W], sx);
+        float sz = cx[0] * (p[1] - p[0]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) sz = __builtin_fmaf(cx[m - 1], p[m] - p[1 - m], sz);
+        sz = zx * sz;
+        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
+        float dx = sx, dz = sz;
+        int xr = -1;
+        if (i < xw) xr = i;
+        else if (i >= nx - xw) xr = i - (nx - xw) + xw;
+        if (xr >= 0) {
+            const size_t e = (size_t)xr * nz + j;
+            const float psi = __builtin_fmaf(a.pml_x[3 * nx + i], sx, a.pml_x[2 * nx + i] * a.psi_px_in[e]);
+            a.psi_px_out[e] = psi;
+            dx = sx + psi;
+        }
+        if (zc >= 0) {
+            const size_t e = (size_t)i * (2 * zw) + zc;
+            const float psi = __builtin_fmaf(a.pml_z[3 * nz + j], sz, a.pml_z[2 * nz + j] * a.psi_pz_in[e]);
+            a.psi_pz_out[e] = psi;
+            dz = sz + psi;
+        }
+        a.vx_out[o] = __builtin_fmaf(-bf[q], dx, vx0[q]);
+        a.vz_out[o] = __builtin_fmaf(-bf[q], dz, vz0[q]);
+    }
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const AcousticStepArgs a) {
+    constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
+    __shared__ float s_vx[(TX + 2 * R) * WX];
+    __shared__ float s_vz[TX * WZ];
+    int i0, j0;
+    tiled_tile(a, TX, R, &i0, &j0);
+    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = j0 + lane;
+    const bool jok = j < nz - R;
+    const float amp = a.amp[a.k];
+    // Own-cell inputs and the rows' receiver spans, before the tile.
+    float p0[RPW], kf[RPW];
+    int4 span[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int i = i0 + wv + q * NW;
+        const bool ok = jok && i < nx - R;
+        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
+        p0[q] = ok ? a.p_in[o] : 0.f;
+        kf[q] = ok ? a.kf[o] : 0.f;
+        span[q] = a.rec_row[min(i, nx - 1)];
+    }
+    tiled_stage<TX, NW, R, 0, false>(s_vx, a.vx_out, nz, i0, j0, 0, 0, 0.f);
+    tiled_stage<TX, NW, 0, R, false>(s_vz, a.vz_out, nz, i0, j0, 0, 0, 0.f);
+    __syncthreads();
+    if (!jok) return;
+    float cx[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
+    const float zx = a.zx;
+    const int xw = a.pml_xw, zw = a.pml_zw;
+    int zc = -1;
+    if (j < zw) zc = j;
+    else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+    float* gk = a.rec_buf + (size_t)a.k * a.nslot;
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int il = wv + q * NW, i = i0 + il;
+        if (i >= nx - R) break;
+        const float* px = s_vx + (il + R) * WX + lane;
+        const float* pz = s_vz + il * WZ + lane + R;
+        float sx = cx[0] * (px[0] - px[-WX]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) sx = __builtin_fmaf(cx[m - 1], px[(m - 1) * WX] - px[-m * WX], sx);
+        float sz = cx[0] * (pz[0] - pz[-1]);
+#pragma unroll
+        for (int m = 2; m <= R; ++m) sz = __builtin_fmaf(cx[m - 1], pz[m - 1] - pz[-m], sz);
+        sz = zx * sz;
+        int xr = -1;
+        if (i < xw) xr = i;
+        else if (i >= nx - xw) xr = i - (nx - xw) + xw;
+        if (xr >= 0) {
+            const size_t e = (size_t)xr * nz + j;
+            const float psi = __builtin_fmaf(a.pml_x[nx + i], sx, a.pml_x[i] * a.psi_vx_in[e]);
+            a.psi_vx_out[e] = psi;
+            sx += psi;
+        }
+        if (zc >= 0) {
+            const size_t e = (size_t)i * (2 * zw) + zc;
+            const float psi = __builtin_fmaf(a.pml_z[nz + j], sz, a.pml_z[j] * a.psi_vz_in[e]);
+            a.psi_vz_out[e] = psi;
+            sz += psi;
+        }
+        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
+        float p = p0[q];
+        if (i == a.sx && j == a.sz) p -= amp;
+        const float pn = __builtin_fmaf(-kf[q], sx + sz, p);
+        a.p_out[o] = pn;
+        // Receivers of row i: the first span, then further spans (rare).
+        const int4 sp = span[q];
+        if (j >= sp.x && j < sp.y) gk[sp.z + (j - sp.x)] = pn;
+        for (int more = sp.w; more >= 0;) {
+            const int4 mp = a.rec_more[more];
+            if (j >= mp.x && j < mp.y) gk[mp.z + (j - mp.x)] = pn;
+            more = mp.w;
+        }
+    }
+}
+
+}  // namespace sfd
