@@ -224,6 +224,25 @@ bool pick_tiled(int radius, int tx, int nw, StepKernel* v, StepKernel* p) {
     }
 }
 
+template <int R>
+bool pick_tiled_batch_r(int tx, int nw, BatchKernel* v, BatchKernel* p) {
+    if (tx == 16 && nw == 4) {
+        *v = acoustic_tiled_velocity_batch<R, 16, 4>;
+        *p = acoustic_tiled_pressure_batch<R, 16, 4>;
+        return true;
+    }
+    return false;
+}
+
+bool pick_tiled_batch(int radius, int tx, int nw, BatchKernel* v, BatchKernel* p) {
+    switch (radius) {
+    case 4: return pick_tiled_batch_r<4>(tx, nw, v, p);
+    case 5: return pick_tiled_batch_r<5>(tx, nw, v, p);
+    case 8: return pick_tiled_batch_r<8>(tx, nw, v, p);
+    default: return false;
+    }
+}
+
 // Default tiled shape, measured on MI355X for the 2101 x 4101 C-PML grid (R =
 // 8, us per step, both launches): 16 rows x 4 waves 58.9, 32 x 8 60.4, 16 x 8
 // 60.6, 8 x 4 62.9, 32 x 16 66.7, 8 x 8 78.3, 16 x 16 86.5 (the fused march
@@ -773,6 +792,9 @@ struct sfd_acoustic_batch {
     int ntile = 0;
     DeviceBuffer shot_args, tile_shot, tile_base;
     BatchKernel kernel = nullptr;
+    // C-PML in fast arithmetic: two tiled launches per step, in place.
+    bool tiled = false;
+    BatchKernel tvel = nullptr, tpres = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -790,7 +812,8 @@ struct sfd_acoustic_batch {
 
 namespace {
 
-// Tile tables and per-shot argument blocks (both ping-pong orders).
+// Tile tables and per-shot argument blocks (both ping-pong orders; the
+// tiled step runs in place and needs one).
 int build_batch_tables(sfd_acoustic_batch* b) {
     const int n = (int)b->plans.size();
     std::vector<AcousticStepArgs> args(2 * (size_t)n);
@@ -799,6 +822,18 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = b->plans[s];
         for (int par = 0; par < 2; ++par) args[(size_t)par * n + s] = make_args(pl, par, 0);
+        if (b->tiled) {
+            AcousticStepArgs& a = args[s];
+            a.p_out = const_cast<float*>(a.p_in);
+            a.vx_out = const_cast<float*>(a.vx_in);
+            a.vz_out = const_cast<float*>(a.vz_in);
+            a.psi_px_out = const_cast<float*>(a.psi_px_in);
+            a.psi_vx_out = const_cast<float*>(a.psi_vx_in);
+            a.psi_pz_out = const_cast<float*>(a.psi_pz_in);
+            a.psi_vz_out = const_cast<float*>(a.psi_vz_in);
+            a.ntz = (pl->nz - 2 * pl->radius + kTiledTZ - 1) / kTiledTZ;
+            a.nchunk = (pl->nx - 2 * pl->radius + pl->ttx - 1) / pl->ttx;
+        }
         const int nt = args[s].ntz * args[s].nchunk;
         base[s] = t;
         tshot.insert(tshot.end(), nt, s);
@@ -824,7 +859,14 @@ int enqueue_batch_steps(sfd_acoustic_batch* b) {
     for (int k = 0; k < b->nt; ++k) {
         a.k = k;
         a.parity = k & 1;
-        hipLaunchKernelGGL(b->kernel, dim3(b->ntile), dim3(64 * b->nw), 0, b->ctx->stream, a);
+        if (b->tiled) {
+            const dim3 block(64 * b->plans[0]->tnw);
+            hipLaunchKernelGGL(b->tvel, dim3(b->ntile), block, 0, b->ctx->stream, a);
+            SFD_HIP(b->ctx, hipGetLastError());
+            hipLaunchKernelGGL(b->tpres, dim3(b->ntile), block, 0, b->ctx->stream, a);
+        } else {
+            hipLaunchKernelGGL(b->kernel, dim3(b->ntile), dim3(64 * b->nw), 0, b->ctx->stream, a);
+        }
         SFD_HIP(b->ctx, hipGetLastError());
     }
     return SFD_OK;
@@ -875,6 +917,17 @@ int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch* b, int waves, int chunk_ro
     if (!b) return SFD_EINVAL;
     // Batches hold far more cells than one shot, so the default chunk is
     // longer (less queue fill per cell) while the grid still fills the GPU.
+    // C-PML in fast arithmetic: the tiled two-launch step unless a march
+    // tiling is asked for.
+    b->tiled = waves == 0 && chunk_rows == 0 && b->mode == MODE_FAST_CPML &&
+               pick_tiled_batch(b->radius, kTiledTX, kTiledNW, &b->tvel, &b->tpres);
+    if (b->tiled) {
+        for (auto* pl : b->plans) {
+            const int rc = sfd_acoustic_plan_set_tiled(pl, kTiledTX, kTiledNW);
+            if (rc) return rc;
+        }
+        return build_batch_tables(b);
+    }
     const int nw = waves ? waves : 2;
     long cells = 0;
     for (auto* pl : b->plans) cells += (long)pl->nx * pl->nz;
@@ -928,7 +981,7 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    if (how != SFD_BATCH_PER_SHOT)
+    if (how != SFD_BATCH_PER_SHOT && !b->tiled)
         for (auto* pl : b->plans) pl->cur = b->nt & 1;
     return SFD_OK;
 }

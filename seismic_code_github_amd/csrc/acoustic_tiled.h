@@ -28,10 +28,10 @@ namespace sfd {
 
 constexpr int kTiledTZ = 64;
 
-__device__ __forceinline__ void tiled_tile(const AcousticStepArgs& a, int tx, int R, int* i0, int* j0) {
-    const int t = xcd_tile((int)blockIdx.x, a.ntz * a.nchunk);
-    *j0 = R + (t % a.ntz) * kTiledTZ;
-    *i0 = R + (t / a.ntz) * tx;
+// Tile t of a shot: z tiles fastest (ntz of them), then row bands.
+__device__ __forceinline__ void tiled_origin(int t, int ntz, int tx, int R, int* i0, int* j0) {
+    *j0 = R + (t % ntz) * kTiledTZ;
+    *i0 = R + (t / ntz) * tx;
 }
 
 // Rows [i0 - HR, i0 + TX + HR) x columns [j0 - HC, j0 + 64 + HC) of a field
@@ -59,12 +59,14 @@ __device__ __forceinline__ void tiled_stage(float* lds, const float* f, int nz, 
     }
 }
 
-template <int R, int TX, int NW>
-__global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity(const AcousticStepArgs a) {
+// Args is AcousticStepArgs, in kernel arguments or (batched shots) in device
+// memory read through the constant address space; t is the shot's tile and
+// k the step.
+template <int R, int TX, int NW, class Args>
+__device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, float* s_p) {
     constexpr int W = kTiledTZ + 2 * R, RPW = TX / NW;
-    __shared__ float s_p[(TX + 2 * R) * W];
     int i0, j0;
-    tiled_tile(a, TX, R, &i0, &j0);
+    tiled_origin(t, a.ntz, TX, R, &i0, &j0);
     const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = j0 + lane;
     const bool jok = j < nz - R;
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity(const Acousti
         vz0[q] = ok ? a.vz_in[o] : 0.f;
         bf[q] = ok ? a.bf[o] : 0.f;
     }
-    const float amp = a.amp[a.k];
+    const float amp = a.amp[k];
     tiled_stage<TX, NW, R, R, true>(s_p, a.p_in, nz, i0, j0, a.sx, a.sz, amp);
     __syncthreads();
     if (!jok) return;
@@ -127,17 +129,15 @@ __global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity(const Acousti
     }
 }
 
-template <int R, int TX, int NW>
-__global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const AcousticStepArgs a) {
+template <int R, int TX, int NW, class Args>
+__device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, float* s_vx, float* s_vz) {
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
-    __shared__ float s_vx[(TX + 2 * R) * WX];
-    __shared__ float s_vz[TX * WZ];
     int i0, j0;
-    tiled_tile(a, TX, R, &i0, &j0);
+    tiled_origin(t, a.ntz, TX, R, &i0, &j0);
     const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = j0 + lane;
     const bool jok = j < nz - R;
-    const float amp = a.amp[a.k];
+    const float amp = a.amp[k];
     // Own-cell inputs and the rows' receiver spans, before the tile.
     float p0[RPW], kf[RPW];
     int4 span[RPW];
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const Acousti
     int zc = -1;
     if (j < zw) zc = j;
     else if (j >= nz - zw) zc = j - (nz - zw) + zw;
-    float* gk = a.rec_buf + (size_t)a.k * a.nslot;
+    float* gk = a.rec_buf + (size_t)k * a.nslot;
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int il = wv + q * NW, i = i0 + il;
@@ -205,6 +205,43 @@ __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const Acousti
             more = mp.w;
         }
     }
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity(const AcousticStepArgs a) {
+    __shared__ float s_p[(TX + 2 * R) * (kTiledTZ + 2 * R)];
+    tiled_velocity<R, TX, NW>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, s_p);
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const AcousticStepArgs a) {
+    __shared__ float s_vx[(TX + 2 * R) * kTiledTZ];
+    __shared__ float s_vz[TX * (kTiledTZ + 2 * R)];
+    tiled_pressure<R, TX, NW>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, s_vx, s_vz);
+}
+
+// Many shots per launch: b.shots[s] is shot s's argument block (in place, so
+// one block per shot), tile_shot maps a global tile to its shot and
+// tile_base[s] is the shot's first global tile.
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity_batch(const BatchStepArgs b) {
+    __shared__ float s_p[(TX + 2 * R) * (kTiledTZ + 2 * R)];
+    const int tile = xcd_tile((int)blockIdx.x, b.ntile);
+    const int s = ro(b.tile_shot, tile);
+    using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
+    CArgs& a = *(CArgs*)(b.shots + s);
+    tiled_velocity<R, TX, NW>(a, tile - ro(b.tile_base, s), b.k, s_p);
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure_batch(const BatchStepArgs b) {
+    __shared__ float s_vx[(TX + 2 * R) * kTiledTZ];
+    __shared__ float s_vz[TX * (kTiledTZ + 2 * R)];
+    const int tile = xcd_tile((int)blockIdx.x, b.ntile);
+    const int s = ro(b.tile_shot, tile);
+    using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
+    CArgs& a = *(CArgs*)(b.shots + s);
+    tiled_pressure<R, TX, NW>(a, tile - ro(b.tile_base, s), b.k, s_vx, s_vz);
 }
 
 }  // namespace sfd

@@ -26,10 +26,12 @@ def small_batch(boundary="cpml", nshot=5, nt=300, arith="fast"):
     return shots
 
 
-@pytest.mark.parametrize("boundary", ["cpml", "sponge"])
-def test_batch_modes_equal_single_plans(ctx, boundary):
+@pytest.mark.parametrize("boundary,tiling", [("cpml", "march"), ("cpml", "default"), ("sponge", "march")])
+def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
+    """tiling "default": C-PML batches run the two-launch tiled step (in
+    place), the single plans the fused march kernel; still bit for bit."""
     shots = small_batch(boundary)
-    print(f"case: batch modes {boundary}, shots {[s.vp.shape for s in shots]}", flush=True)
+    print(f"case: batch modes {boundary} {tiling}, shots {[s.vp.shape for s in shots]}", flush=True)
     singles = []
     for sh in shots:
         pl = sfd.AcousticPlan(sh, ctx)
@@ -38,7 +40,8 @@ def test_batch_modes_equal_single_plans(ctx, boundary):
         singles.append(pl.fetch()[0])
         pl.close()
     b = sfd.AcousticBatch(shots, ctx)
-    b.set_tiling(2, 48)
+    if tiling == "march":
+        b.set_tiling(2, 48)
     for how in (sfd.BATCH_GRAPH, sfd.BATCH_LAUNCHES, sfd.BATCH_PER_SHOT):
         b.run(how)
         for s in range(len(shots)):
