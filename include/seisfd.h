@@ -132,11 +132,12 @@ int sfd_acoustic_plan_step(sfd_acoustic_plan *plan, int k0, int nsteps);
 /* Launch shape of the step kernel: waves per workgroup (1, 2 or 4) and rows
  * marched per workgroup.  0 = choose automatically. */
 int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan *plan, int waves, int chunk_rows);
-/* C-PML in fast arithmetic runs by default as two tiled launches per step
- * (velocity, then pressure) instead of the fused march kernel: tile_rows (8,
- * 16 or 32) x 64 columns per workgroup of `waves` (4, 8 or 16; not every pair
- * is built); 0 = the default shape, tile_rows < 0 = the march kernel.  An
- * explicit sfd_acoustic_plan_set_tiling also selects the march kernel. */
+/* Fast arithmetic (C-PML or sponge) runs by default as two tiled launches per
+ * step (velocity, then pressure) instead of the fused march kernel: tile_rows
+ * (8, 16 or 32) x 64 columns per workgroup of `waves` (4, 8 or 16; not every
+ * pair is built); 0 = the default shape, tile_rows < 0 = the march kernel.
+ * An explicit sfd_acoustic_plan_set_tiling also selects the march kernel.
+ * Results are bitwise equal either way; the kernel may change between steps. */
 int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan *plan, int tile_rows, int waves);
 
 /* ---------------------------------------------------------------- batches

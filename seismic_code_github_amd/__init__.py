@@ -293,7 +293,8 @@ class AcousticPlan:
         self.ctx.check(lib().sfd_acoustic_plan_set_tiling(self.handle, waves, chunk_rows), "set_tiling")
 
     def set_tiled(self, tile_rows: int = 0, waves: int = 0):
-        """C-PML: the two-launch tiled step (tile_rows < 0 = the march kernel)."""
+        """Fast arithmetic: the two-launch tiled step at tile_rows x waves
+        (0 = the default shape; tile_rows < 0 = the fused march kernel)."""
         self.ctx.check(lib().sfd_acoustic_plan_set_tiled(self.handle, tile_rows, waves), "set_tiled")
 
     def run(self) -> float:

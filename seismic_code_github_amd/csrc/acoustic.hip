@@ -41,8 +41,6 @@ StepKernel pick_mode(int mode) {
     case MODE_FAST_SPONGE: return acoustic_step_kernel<R, NW, MODE_FAST_SPONGE, default_slots<R>()>;
     case MODE_FAST_CPML: return acoustic_step_kernel<R, NW, MODE_FAST_CPML, default_slots<R>()>;
     case MODE_REF_SPONGE: return acoustic_step_kernel<R, NW, MODE_REF_SPONGE, default_slots<R>()>;
-    case MODE_FWD_DIV: return acoustic_step_kernel<R, NW, MODE_FWD_DIV, default_slots<R>()>;
-    case MODE_ADJ: return acoustic_step_kernel<R, NW, MODE_ADJ, default_slots<R>()>;
     default: return nullptr;
     }
 }
@@ -192,16 +190,42 @@ void cpml_profile(int n, int w, double h, double dt, double R, double vmax, doub
 }
 
 
-// Chunks start at rows congruent to R modulo 2R (the kernel's queue slots are
-// compile-time constants), so the row count of a chunk is a multiple of 2R.
-template <int R>
-bool pick_tiled_r(int tx, int nw, StepKernel* v, StepKernel* p) {
-    switch (tx * 100 + nw) {
-#define SFD_TILED(TX, NW)                                  \
-    case TX * 100 + NW:                                    \
-        *v = acoustic_tiled_velocity<R, TX, NW>;           \
-        *p = acoustic_tiled_pressure<R, TX, NW>;           \
+template <int R, int TX, int NW>
+bool tiled_mode(int mode, StepKernel* v, StepKernel* p) {
+    switch (mode) {
+#define SFD_TILED_MODE(M)                                \
+    case M:                                              \
+        *v = acoustic_tiled_velocity<R, TX, NW, M>;      \
+        *p = acoustic_tiled_pressure<R, TX, NW, M>;      \
         return true;
+        SFD_TILED_MODE(MODE_FAST_CPML)
+        SFD_TILED_MODE(MODE_FAST_SPONGE)
+#undef SFD_TILED_MODE
+    default:
+        return false;
+    }
+}
+
+// Tiled shapes of single plans.  The default shape, measured on MI355X for
+// the 2101 x 4101 C-PML grid (R = 8, us per step, both launches): 16 rows x 4
+// waves 58.9, 32 x 8 60.4, 16 x 8 60.6, 8 x 4 62.9, 32 x 16 66.7, 8 x 8 78.3,
+// 16 x 16 86.5 (the fused march kernel: 125.1).  The gradient's modes exist
+// at the default shape.
+template <int R>
+bool pick_tiled_r(int tx, int nw, int mode, StepKernel* v, StepKernel* p) {
+    if (tx == kTiledTX && nw == kTiledNW && (mode == MODE_FWD_DIV || mode == MODE_ADJ)) {
+        if (mode == MODE_FWD_DIV) {
+            *v = acoustic_tiled_velocity<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
+            *p = acoustic_tiled_pressure<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
+        } else {
+            *v = acoustic_tiled_velocity<R, kTiledTX, kTiledNW, MODE_ADJ>;
+            *p = acoustic_tiled_pressure<R, kTiledTX, kTiledNW, MODE_ADJ>;
+        }
+        return true;
+    }
+    switch (tx * 100 + nw) {
+#define SFD_TILED(TX, NW) \
+    case TX * 100 + NW: return tiled_mode<R, TX, NW>(mode, v, p);
         SFD_TILED(8, 4)
         SFD_TILED(8, 8)
         SFD_TILED(16, 4)
@@ -215,40 +239,48 @@ bool pick_tiled_r(int tx, int nw, StepKernel* v, StepKernel* p) {
     }
 }
 
-bool pick_tiled(int radius, int tx, int nw, StepKernel* v, StepKernel* p) {
+bool pick_tiled(int radius, int tx, int nw, int mode, StepKernel* v, StepKernel* p) {
     switch (radius) {
-    case 4: return pick_tiled_r<4>(tx, nw, v, p);
-    case 5: return pick_tiled_r<5>(tx, nw, v, p);
-    case 8: return pick_tiled_r<8>(tx, nw, v, p);
+    case 4: return pick_tiled_r<4>(tx, nw, mode, v, p);
+    case 5: return pick_tiled_r<5>(tx, nw, mode, v, p);
+    case 8: return pick_tiled_r<8>(tx, nw, mode, v, p);
     default: return false;
     }
 }
 
 template <int R>
-bool pick_tiled_batch_r(int tx, int nw, BatchKernel* v, BatchKernel* p) {
-    if (tx == 16 && nw == 4) {
-        *v = acoustic_tiled_velocity_batch<R, 16, 4>;
-        *p = acoustic_tiled_pressure_batch<R, 16, 4>;
+bool pick_tiled_batch_r(int tx, int nw, int mode, BatchKernel* v, BatchKernel* p) {
+    if (tx != kTiledTX || nw != kTiledNW) return false;
+    switch (mode) {
+#define SFD_TILED_BATCH(M)                                              \
+    case M:                                                             \
+        *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, M>;   \
+        *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, M>;   \
         return true;
+        SFD_TILED_BATCH(MODE_FAST_CPML)
+        SFD_TILED_BATCH(MODE_FAST_SPONGE)
+        SFD_TILED_BATCH(MODE_FWD_DIV)
+        SFD_TILED_BATCH(MODE_ADJ)
+#undef SFD_TILED_BATCH
+    default:
+        return false;
     }
-    return false;
 }
 
-bool pick_tiled_batch(int radius, int tx, int nw, BatchKernel* v, BatchKernel* p) {
+bool pick_tiled_batch(int radius, int tx, int nw, int mode, BatchKernel* v, BatchKernel* p) {
     switch (radius) {
-    case 4: return pick_tiled_batch_r<4>(tx, nw, v, p);
-    case 5: return pick_tiled_batch_r<5>(tx, nw, v, p);
-    case 8: return pick_tiled_batch_r<8>(tx, nw, v, p);
+    case 4: return pick_tiled_batch_r<4>(tx, nw, mode, v, p);
+    case 5: return pick_tiled_batch_r<5>(tx, nw, mode, v, p);
+    case 8: return pick_tiled_batch_r<8>(tx, nw, mode, v, p);
     default: return false;
     }
 }
 
-// Default tiled shape, measured on MI355X for the 2101 x 4101 C-PML grid (R =
-// 8, us per step, both launches): 16 rows x 4 waves 58.9, 32 x 8 60.4, 16 x 8
-// 60.6, 8 x 4 62.9, 32 x 16 66.7, 8 x 8 78.3, 16 x 16 86.5 (the fused march
-// kernel: 125.1).
-constexpr int kTiledTX = 16, kTiledNW = 4;
+// Modes with a tiled step (the reference arithmetic stays on the march kernel).
+bool tiled_mode_ok(int mode) { return mode == MODE_FAST_CPML || mode == MODE_FAST_SPONGE; }
 
+// Chunks start at rows congruent to R modulo 2R (the kernel's queue slots are
+// compile-time constants), so the row count of a chunk is a multiple of 2R.
 int round_chunk(int ch, int radius) {
     const int q = 2 * radius;
     return std::max(q, (ch + q - 1) / q * q);
@@ -279,6 +311,7 @@ int zero_state(sfd_acoustic_plan* pl) {
     }
     if (pl->rec_buf.ptr) SFD_HIP(pl->ctx, hipMemsetAsync(pl->rec_buf.ptr, 0, pl->rec_buf.bytes, s));
     pl->cur = 0;
+    pl->v_raw = false;
     return SFD_OK;
 }
 
@@ -356,52 +389,93 @@ float sponge_at(const sfd_acoustic_plan* pl, int i, int j) {
     return 1.0f;
 }
 
+// Velocities of a tiled sponge run are stored unabsorbed; this writes them
+// absorbed (dst = src * the cell's sponge factor, dst == src allowed), as the
+// fused kernel stores them.  Fields are nx x nz from row 0; cells outside
+// the updated region hold zero either way.
+__global__ void absorb_velocity_kernel(float* dst, const float* src, const float* __restrict__ ax,
+                                       const float* __restrict__ az, int sw, int nx, int nz) {
+    const size_t n = (size_t)nx * nz;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c / nz), j = (int)(c - (size_t)i * nz);
+        dst[c] = src[c] * tiled_sponge(i, nx, sw, ax[i], az[j]);
+    }
+}
+
+int launch_absorb(const sfd_acoustic_plan* pl, float* dst, const float* src) {
+    const size_t n = (size_t)pl->nx * pl->nz;
+    const int bs = 256, nb = (int)std::min<size_t>((n + bs - 1) / bs, 4096);
+    hipLaunchKernelGGL(absorb_velocity_kernel, dim3(nb), dim3(bs), 0, pl->ctx->stream, dst, src,
+                       pl->ax.as<float>() + pl->pad, pl->az.as<float>(), pl->width, pl->nx, pl->nz);
+    SFD_HIP(pl->ctx, hipGetLastError());
+    return SFD_OK;
+}
+
+void tiled_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw) {
+    // In place: within each launch a cell's outputs are read by no other
+    // workgroup (velocities read P's halo, pressure reads the velocities'
+    // halo), so one set of buffers suffices and the step's working set is
+    // half the ping-pong one (it then fits the 256 MB MALL at 2101 x 4101).
+    a.p_out = const_cast<float*>(a.p_in);
+    a.vx_out = const_cast<float*>(a.vx_in);
+    a.vz_out = const_cast<float*>(a.vz_in);
+    a.psi_px_out = const_cast<float*>(a.psi_px_in);
+    a.psi_vx_out = const_cast<float*>(a.psi_vx_in);
+    a.psi_pz_out = const_cast<float*>(a.psi_pz_in);
+    a.psi_vz_out = const_cast<float*>(a.psi_vz_in);
+    a.ntz = (pl->nz - 2 * pl->radius + kTiledTZ - 1) / kTiledTZ;
+    a.nchunk = (pl->nx - 2 * pl->radius + pl->ttx - 1) / pl->ttx;
+    a.v_raw = v_raw ? 1 : 0;
+}
+
+int launch_tiled(const sfd_acoustic_plan* pl, StepKernel v, StepKernel p, const AcousticStepArgs& a) {
+    const dim3 grid(a.ntz * a.nchunk), block(64 * pl->tnw);
+    hipLaunchKernelGGL(v, grid, block, 0, pl->ctx->stream, a);
+    SFD_HIP(pl->ctx, hipGetLastError());
+    hipLaunchKernelGGL(p, grid, block, 0, pl->ctx->stream, a);
+    SFD_HIP(pl->ctx, hipGetLastError());
+    return SFD_OK;
+}
+
 // Enqueues steps k0 .. k0+nsteps-1 on the context stream.
 int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     sfd_context* ctx = pl->ctx;
-    AcousticStepArgs a = make_args(pl, pl->cur, k0);
-    const dim3 grid(a.ntz * a.nchunk);
-    const dim3 block(64 * pl->nw);
     const size_t nbytes = (size_t)pl->nx * pl->nz * sizeof(float);
     size_t snap_next = std::lower_bound(pl->snap_steps.begin(), pl->snap_steps.end(), k0) - pl->snap_steps.begin();
-
-    const int R = pl->radius;
-    const int tntz = (pl->nz - 2 * R + kTiledTZ - 1) / kTiledTZ;
-    const int tnch = pl->ttx > 0 ? (pl->nx - 2 * R + pl->ttx - 1) / pl->ttx : 0;
+    const bool tiled = pl->ttx > 0;
+    if (!tiled && pl->v_raw && nsteps > 0) {
+        // The march kernel reads absorbed velocities.
+        for (Field* f : {&pl->vx[pl->cur], &pl->vz[pl->cur]}) {
+            const int rc = launch_absorb(pl, pl->grid(*f), pl->grid(*f));
+            if (rc) return rc;
+        }
+        pl->v_raw = false;
+    }
     for (int s = 0; s < nsteps; ++s) {
         const int k = k0 + s;
-        a = make_args(pl, pl->cur, k);
-        if (pl->ttx > 0) {
-            // In place: within each launch a cell's outputs are read by no
-            // other workgroup (velocities read P's halo, pressure reads the
-            // velocities' halo), so one set of buffers suffices and the
-            // step's working set is half the ping-pong one (it then fits the
-            // 256 MB MALL at 2101 x 4101).
-            a.p_out = const_cast<float*>(a.p_in);
-            a.vx_out = const_cast<float*>(a.vx_in);
-            a.vz_out = const_cast<float*>(a.vz_in);
-            a.psi_px_out = const_cast<float*>(a.psi_px_in);
-            a.psi_vx_out = const_cast<float*>(a.psi_vx_in);
-            a.psi_pz_out = const_cast<float*>(a.psi_pz_in);
-            a.psi_vz_out = const_cast<float*>(a.psi_vz_in);
-            a.ntz = tntz;
-            a.nchunk = tnch;
-            const dim3 tg(tntz * tnch), tb(64 * pl->tnw);
-            hipLaunchKernelGGL(pl->tvel, tg, tb, 0, ctx->stream, a);
-            SFD_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(pl->tpres, tg, tb, 0, ctx->stream, a);
+        AcousticStepArgs a = make_args(pl, pl->cur, k);
+        if (tiled) {
+            tiled_args(pl, a, pl->v_raw);
+            const int rc = launch_tiled(pl, pl->tvel, pl->tpres, a);
+            if (rc) return rc;
+            pl->v_raw = pl->mode == MODE_FAST_SPONGE;
         } else {
-            hipLaunchKernelGGL(pl->kernel, grid, block, 0, ctx->stream, a);
+            hipLaunchKernelGGL(pl->kernel, dim3(a.ntz * a.nchunk), dim3(64 * pl->nw), 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
             pl->cur ^= 1;
         }
-        SFD_HIP(ctx, hipGetLastError());
         const int o = pl->cur;
         while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
             float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
             const float* src[3] = {pl->grid(pl->p[o]), pl->grid(pl->vx[o]), pl->grid(pl->vz[o])};
             for (int f = 0; f < 3; ++f) {
                 if (!(pl->snap_fields & (1 << f))) continue;
-                SFD_HIP(ctx, hipMemcpyAsync(dst, src[f], nbytes, hipMemcpyDeviceToDevice, ctx->stream));
+                if (f > 0 && pl->v_raw) {
+                    const int rc = launch_absorb(pl, dst, src[f]);
+                    if (rc) return rc;
+                } else {
+                    SFD_HIP(ctx, hipMemcpyAsync(dst, src[f], nbytes, hipMemcpyDeviceToDevice, ctx->stream));
+                }
                 dst += (size_t)pl->nx * pl->nz;
             }
             ++snap_next;
@@ -581,7 +655,7 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
     auto_tiling(pl, &pl->nw, &pl->chunk);
     pl->kernel = pick_step_kernel(pl->radius, pl->nw, pl->mode);
     if (!pl->kernel) return bail(fail(ctx, SFD_ENOTSUP, "no kernel for this configuration"));
-    if (pl->mode == MODE_FAST_CPML && pick_tiled(pl->radius, kTiledTX, kTiledNW, &pl->tvel, &pl->tpres)) {
+    if (tiled_mode_ok(pl->mode) && pick_tiled(pl->radius, kTiledTX, kTiledNW, pl->mode, &pl->tvel, &pl->tpres)) {
         pl->ttx = kTiledTX;
         pl->tnw = kTiledNW;
     }
@@ -612,8 +686,8 @@ int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan* pl, int waves, int chunk_row
     // An explicit march tiling selects the march kernel; automatic restores
     // the tiled step where it applies.
     pl->ttx = pl->tnw = 0;
-    if (waves == 0 && chunk_rows == 0 && pl->mode == MODE_FAST_CPML &&
-        pick_tiled(pl->radius, kTiledTX, kTiledNW, &pl->tvel, &pl->tpres)) {
+    if (waves == 0 && chunk_rows == 0 && tiled_mode_ok(pl->mode) &&
+        pick_tiled(pl->radius, kTiledTX, kTiledNW, pl->mode, &pl->tvel, &pl->tpres)) {
         pl->ttx = kTiledTX;
         pl->tnw = kTiledNW;
     }
@@ -626,10 +700,10 @@ int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan* pl, int tile_rows, int waves)
         pl->ttx = pl->tnw = 0;
         return SFD_OK;
     }
-    if (pl->mode != MODE_FAST_CPML) return fail(pl->ctx, SFD_ENOTSUP, "the tiled step exists for C-PML in fast arithmetic");
+    if (!tiled_mode_ok(pl->mode)) return fail(pl->ctx, SFD_ENOTSUP, "the tiled step exists in fast arithmetic only");
     const int tx = tile_rows ? tile_rows : kTiledTX, nw = waves ? waves : kTiledNW;
     StepKernel v, p;
-    if (!pick_tiled(pl->radius, tx, nw, &v, &p))
+    if (!pick_tiled(pl->radius, tx, nw, pl->mode, &v, &p))
         return fail(pl->ctx, SFD_EINVAL, "no tiled step for %d rows x %d waves", tx, nw);
     pl->ttx = tx;
     pl->tnw = nw;
@@ -691,6 +765,7 @@ int sfd_acoustic_plan_set_state(sfd_acoustic_plan* pl, const float* state, const
     SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->p[pl->cur]), state, n * 4, hipMemcpyHostToDevice, s));
     SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vx[pl->cur]), state + n, n * 4, hipMemcpyHostToDevice, s));
     SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vz[pl->cur]), state + 2 * n, n * 4, hipMemcpyHostToDevice, s));
+    pl->v_raw = false;  // the given velocities are absorbed ones
     if (pl->mode == MODE_FAST_CPML) {
         if (!pml) return fail(ctx, SFD_EINVAL, "C-PML plans need the pml state");
         // Full-grid psi fields -> strips.
@@ -733,6 +808,16 @@ int sfd_acoustic_plan_get_state(sfd_acoustic_plan* pl, float* state, float* pml)
     SFD_HIP(ctx, hipMemcpyAsync(state + n, pl->grid(pl->vx[pl->cur]), n * 4, hipMemcpyDeviceToHost, s));
     SFD_HIP(ctx, hipMemcpyAsync(state + 2 * n, pl->grid(pl->vz[pl->cur]), n * 4, hipMemcpyDeviceToHost, s));
     SFD_HIP(ctx, hipStreamSynchronize(s));
+    if (pl->v_raw) {
+        // Stored unabsorbed by the tiled sponge step: apply the factor as the
+        // fused kernel does when it stores them.
+        for (int i = 0; i < pl->nx; ++i)
+            for (int jj = 0; jj < pl->nz; ++jj) {
+                const float f = sponge_at(pl, i, jj);
+                state[n + (size_t)i * pl->nz + jj] *= f;
+                state[2 * n + (size_t)i * pl->nz + jj] *= f;
+            }
+    }
     if (pml && pl->mode == MODE_FAST_CPML) {
         const int nx = pl->nx, nz = pl->nz, xw = pl->pml_xw, zw = pl->pml_zw;
         std::fill(pml, pml + 4 * n, 0.f);
@@ -792,7 +877,7 @@ struct sfd_acoustic_batch {
     int ntile = 0;
     DeviceBuffer shot_args, tile_shot, tile_base;
     BatchKernel kernel = nullptr;
-    // C-PML in fast arithmetic: two tiled launches per step, in place.
+    // Fast arithmetic: two tiled launches per step, in place.
     bool tiled = false;
     BatchKernel tvel = nullptr, tpres = nullptr;
     hipGraph_t graph = nullptr;
@@ -822,18 +907,9 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = b->plans[s];
         for (int par = 0; par < 2; ++par) args[(size_t)par * n + s] = make_args(pl, par, 0);
-        if (b->tiled) {
-            AcousticStepArgs& a = args[s];
-            a.p_out = const_cast<float*>(a.p_in);
-            a.vx_out = const_cast<float*>(a.vx_in);
-            a.vz_out = const_cast<float*>(a.vz_in);
-            a.psi_px_out = const_cast<float*>(a.psi_px_in);
-            a.psi_vx_out = const_cast<float*>(a.psi_vx_in);
-            a.psi_pz_out = const_cast<float*>(a.psi_pz_in);
-            a.psi_vz_out = const_cast<float*>(a.psi_vz_in);
-            a.ntz = (pl->nz - 2 * pl->radius + kTiledTZ - 1) / kTiledTZ;
-            a.nchunk = (pl->nx - 2 * pl->radius + pl->ttx - 1) / pl->ttx;
-        }
+        // Batches start from zero fields, so the tiled sponge step may read
+        // its velocities as unabsorbed from the first step on.
+        if (b->tiled) tiled_args(pl, args[s], b->mode == MODE_FAST_SPONGE);
         const int nt = args[s].ntz * args[s].nchunk;
         base[s] = t;
         tshot.insert(tshot.end(), nt, s);
@@ -917,10 +993,10 @@ int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch* b, int waves, int chunk_ro
     if (!b) return SFD_EINVAL;
     // Batches hold far more cells than one shot, so the default chunk is
     // longer (less queue fill per cell) while the grid still fills the GPU.
-    // C-PML in fast arithmetic: the tiled two-launch step unless a march
-    // tiling is asked for.
-    b->tiled = waves == 0 && chunk_rows == 0 && b->mode == MODE_FAST_CPML &&
-               pick_tiled_batch(b->radius, kTiledTX, kTiledNW, &b->tvel, &b->tpres);
+    // Fast arithmetic: the tiled two-launch step unless a march tiling is
+    // asked for.
+    b->tiled = waves == 0 && chunk_rows == 0 && tiled_mode_ok(b->mode) &&
+               pick_tiled_batch(b->radius, kTiledTX, kTiledNW, b->mode, &b->tvel, &b->tpres);
     if (b->tiled) {
         for (auto* pl : b->plans) {
             const int rc = sfd_acoustic_plan_set_tiled(pl, kTiledTX, kTiledNW);
@@ -981,8 +1057,12 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    if (how != SFD_BATCH_PER_SHOT && !b->tiled)
-        for (auto* pl : b->plans) pl->cur = b->nt & 1;
+    if (how != SFD_BATCH_PER_SHOT) {
+        for (auto* pl : b->plans) {
+            if (b->tiled) pl->v_raw = b->nt > 0 && b->mode == MODE_FAST_SPONGE;
+            else pl->cur = b->nt & 1;
+        }
+    }
     return SFD_OK;
 }
 

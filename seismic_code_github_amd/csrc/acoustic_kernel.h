@@ -42,12 +42,14 @@ enum KernelMode : int {
     // Internal: fast arithmetic with no absorbing terms; the C-PML kernel runs
     // it on tiles that touch no PML strip.
     MODE_FAST_NONE = 3,
-    // Fast sponge step that also stores div = D-Vx' + D-Vz' of every cell
-    // (the forward recomputation of the gradient).
+    // Tiled step only (acoustic_tiled.h).  Fast sponge step that also stores
+    // div = D-Vx' + D-Vz' of every cell (the forward recomputation of the
+    // gradient).
     MODE_FWD_DIV = 4,
-    // Adjoint step (sponge): the same stencils on the scaled adjoint fields
-    // (K lambda_P, -b lambda_V); no source injection; accumulates
-    // grad += P_in * div_k and keeps the adjoint at the source cell.
+    // Tiled step only.  Adjoint step (sponge): the same stencils on the
+    // scaled adjoint fields (K lambda_P, -b lambda_V); no source injection;
+    // accumulates grad += P_in * div_k and keeps the adjoint at the source
+    // cell.
     MODE_ADJ = 5,
 };
 
@@ -84,6 +86,10 @@ struct AcousticStepArgs {
     const float* __restrict__ pml_z;
     int pml_xw, pml_zw;
     int sponge_w;  // sponge width (x-edge rows read ax[]); 0 without sponge
+    // Tiled sponge step only: 1 when vx_in / vz_in hold velocities not yet
+    // multiplied by their cell's sponge factor (the tiled step keeps them so
+    // between steps and applies the factor when it reads them).
+    int v_raw;
     const float* __restrict__ psi_px_in;
     float* __restrict__ psi_px_out;
     const float* __restrict__ psi_vx_in;
@@ -186,7 +192,6 @@ struct RowLoads {
     float kf;           // kf(i, j)
     int rjlo, rjhi, rslot, rmore;  // receiver span of row i (wave-uniform)
     float psvz;         // C-PML psi_vz(i, j)
-    float dv, gr;       // adjoint: div_k(i, j), grad(i, j)
     float pspz, pspzh;  // C-PML psi_pz(i + 1, j), psi_pz(i + 1, jh)
 };
 
@@ -227,8 +232,6 @@ __device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, S
     constexpr bool CPML = (MODE == MODE_FAST_CPML);
     constexpr bool REF = (MODE == MODE_REF_SPONGE);
     constexpr bool NOABS = CPML || (MODE == MODE_FAST_NONE);
-    constexpr bool ADJ = (MODE == MODE_ADJ);
-    constexpr bool DIVOUT = (MODE == MODE_FWD_DIV);
     static_assert(Q % NS == 0 && Q % VS == 0, "slot counts must divide the queue length");
     static_assert(TZ >= 2 * HR, "tile narrower than its halos");
     using A = ArithFor<R, MODE>;
@@ -258,9 +261,6 @@ __device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, S
     const uint32_t o_bf = fo(a.bf) - padb, o_kf = fo(a.kf) - padb;
     const Buf RR(reinterpret_cast<const int*>(a.rec_row) - 4 * PADR, (uint32_t)(nx + 2 * PADR) * 16u);
     auto so = [&](int r) -> uint32_t { return (uint32_t)(r + PADR) * rowb; };
-    uint32_t o_div = 0, o_grad = 0;
-    if constexpr (ADJ || DIVOUT) o_div = fo(a.div + (size_t)(k % a.div_period) * a.div_stride) - padb;
-    if constexpr (ADJ) o_grad = fo(a.grad) - padb;
 
     // Lane byte offsets.  P reads every stored column; the other fields only
     // updated columns (elsewhere they read 0 and stores are dropped).
@@ -367,10 +367,6 @@ __device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, S
         L.vzh = AR.ld(vo_vh, o_vzin + o1);
         L.bzh = AR.ld(vo_vh, o_bf + o1);
         L.kf = AR.ld(vo_u, o_kf + oi);
-        if constexpr (ADJ) {
-            L.dv = AR.ld(vo_u, o_div + oi);
-            L.gr = AR.ld(vo_u, o_grad + oi);
-        }
         {
             const auto rv = __builtin_amdgcn_raw_buffer_load_b128(RR.r, 0, (int)((uint32_t)(max(i, i0) + PADR) * 16u), 0);
             L.rjlo = (int)rv[0];
@@ -420,7 +416,7 @@ __device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, S
             const int f = i + Q;
             {
                 float pf = L.pf, ph = L.ph;
-                if (!ADJ && f == a.sx) {
+                if (f == a.sx) {
                     if (j == a.sz) pf -= amp;
                     if (ph_col == a.sz && vo_ph != kOOB) ph -= amp;
                 }
@@ -511,17 +507,11 @@ __device__ __forceinline__ void acoustic_march(const Args& a, int tile, int k, S
                     }
                 }
                 const float pc = A::pres(p_i, L.kf, sx, sz, a.inv_dx, a.inv_dz);
-                if constexpr (DIVOUT) AR.st(sx + sz, vo_u, o_div + so(i));
-                if constexpr (ADJ) {
-                    AR.st(__builtin_fmaf(p_i, L.dv, L.gr), vo_u, o_grad + so(i));
-                    if (i == a.sx && j == a.sz) a.src_adj[k] = pc;
-                }
                 const float pn = pc * absorb_row(i);
                 AR.st(pn, vo_u, o_pout + so(i));
                 // Receivers of row i: the first span from the prefetch, further
-                // spans (rare) from the chained list.  (The adjoint injects
-                // its residuals in a separate pass.)
-                if constexpr (!ADJ) {
+                // spans (rare) from the chained list.
+                {
                     const int jlo = __builtin_amdgcn_readfirstlane(L.rjlo);
                     const int jhi = __builtin_amdgcn_readfirstlane(L.rjhi);
                     const int slot = __builtin_amdgcn_readfirstlane(L.rslot);

@@ -12,6 +12,12 @@ using BatchKernel = void (*)(BatchStepArgs);
 
 StepKernel pick_step_kernel(int radius, int nw, int mode);
 BatchKernel pick_batch_kernel(int radius, int nw, int mode);
+// Two-launch tiled step (velocity, pressure) of `mode` at tx rows x nw waves
+// per tile; false where no such instance exists.  Single plans have every
+// shape listed in acoustic.hip for C-PML and sponge, the gradient modes and
+// batches the default shape only.
+bool pick_tiled(int radius, int tx, int nw, int mode, StepKernel* v, StepKernel* p);
+bool pick_tiled_batch(int radius, int tx, int nw, int mode, BatchKernel* v, BatchKernel* p);
 int pad_rows(int radius);
 int round_chunk(int ch, int radius);
 
@@ -43,10 +49,12 @@ struct sfd_acoustic_plan {
     DeviceBuffer ax, az, pml_x, pml_z;
     DeviceBuffer amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
     StepKernel kernel = nullptr;
-    // C-PML in fast arithmetic runs the two-launch tiled step instead
-    // (acoustic_tiled.h): tile rows and waves, 0 = the march kernel.
+    // Fast arithmetic (C-PML or sponge) runs the two-launch tiled step
+    // instead (acoustic_tiled.h): tile rows and waves, 0 = the march kernel.
     int ttx = 0, tnw = 0;
     StepKernel tvel = nullptr, tpres = nullptr;
+    // The velocity fields hold unabsorbed values (after a tiled sponge step).
+    bool v_raw = false;
 
     // Gradient workspace (plans made by the gradient entry points): adjoint
     // state, gradient accumulator and the divergence of one checkpoint
@@ -73,6 +81,13 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* p, const Fi
                            int k);
 AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k);
 int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps);
+// Turns step arguments into those of the tiled step: in place on the input
+// buffers, the tiled tile counts, velocities read as stored (v_raw).
+void tiled_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw);
+// Enqueues the two launches of one tiled step of a plan.
+int launch_tiled(const sfd_acoustic_plan* pl, StepKernel v, StepKernel p, const AcousticStepArgs& a);
+// Tiled step shape used unless one is asked for.
+constexpr int kTiledTX = 16, kTiledNW = 4;
 int zero_state(sfd_acoustic_plan* pl);
 // Sponge factor of cell (i, j) as the step kernel applies it.
 float sponge_at(const sfd_acoustic_plan* pl, int i, int j);

@@ -1,10 +1,11 @@
 // Two-launch tiled time step of the acoustic velocity-pressure staggered-grid
-// propagator with C-PML boundaries (fast arithmetic): the same step as
-// acoustic_step_kernel in MODE_FAST_CPML (reference loop
-// forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:247-377, absorbing
-// strips replaced by the C-PML), split at the reference's own seam:
+// propagator in fast arithmetic: the same step as acoustic_step_kernel
+// (reference loop forward/Acoustic_FD_by_ss/src/Acoustic_1order_FD.cpp:247-377)
+// split at the reference's own seam:
 //   acoustic_tiled_velocity  Vx, Vz from P (source injected)      (:247-286)
 //   acoustic_tiled_pressure  P from the new Vx, Vz; receivers     (:289-314, :376-377)
+// MODE is one of MODE_FAST_CPML, MODE_FAST_SPONGE (sponge of :329-340),
+// MODE_FWD_DIV and MODE_ADJ (the gradient's recomputation and adjoint steps).
 // A workgroup of NW waves owns a tile of TX rows x 64 columns of the updated
 // region [R, nx - R) x [R, nz - R), one column per lane, TX / NW rows per
 // wave.  The tile and the R-cell halo its stencils read are staged in LDS
@@ -18,6 +19,15 @@
 // no updated cell's stencil reaches them).  The plan runs these kernels in
 // place (out == in): within a launch no workgroup reads a cell that another
 // writes, and every output depends on its own cell's input.
+//
+// Sponge: the pressure launch's stencils need the new velocities before the
+// sponge factor is applied, and in place they cannot be overwritten by their
+// absorbed values while other workgroups still read them.  So the velocity
+// fields keep the unabsorbed values between steps and the velocity launch
+// multiplies its own cell's old velocity by the factor when it reads it (the
+// same float product the fused kernel stores), while P is absorbed in place
+// by the pressure launch.  Results are bitwise equal to the fused kernel's;
+// state reads and snapshots apply the factor to the stored velocities.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -59,11 +69,18 @@ __device__ __forceinline__ void tiled_stage(float* lds, const float* f, int nz, 
     }
 }
 
+// Sponge factor of cell (i, j): the x-edge rows' factor ax_i overrides the
+// z-edge columns' az_j in the corners (the fused kernel's absorb_row).
+__device__ __forceinline__ float tiled_sponge(int i, int nx, int sw, float ax_i, float az_j) {
+    return (i < sw || i >= nx - sw) ? ax_i : az_j;
+}
+
 // Args is AcousticStepArgs, in kernel arguments or (batched shots) in device
 // memory read through the constant address space; t is the shot's tile and
 // k the step.
-template <int R, int TX, int NW, class Args>
+template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, float* s_p) {
+    constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ;
     constexpr int W = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
@@ -72,7 +89,7 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
     const bool jok = j < nz - R;
     // This lane's own-cell inputs, loaded before the tile so that their
     // latency overlaps the staging.
-    float vx0[RPW], vz0[RPW], bf[RPW];
+    float vx0[RPW], vz0[RPW], bf[RPW], axr[RPW];
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int i = i0 + wv + q * NW;
@@ -81,19 +98,25 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
         vx0[q] = ok ? a.vx_in[o] : 0.f;
         vz0[q] = ok ? a.vz_in[o] : 0.f;
         bf[q] = ok ? a.bf[o] : 0.f;
+        if constexpr (!CPML) axr[q] = ok ? a.ax[i] : 1.f;
     }
-    const float amp = a.amp[k];
-    tiled_stage<TX, NW, R, R, true>(s_p, a.p_in, nz, i0, j0, a.sx, a.sz, amp);
+    float az_j = 1.f;
+    if constexpr (!CPML) az_j = jok ? a.az[j] : 1.f;
+    const float amp = ADJ ? 0.f : a.amp[k];
+    tiled_stage<TX, NW, R, R, !ADJ>(s_p, a.p_in, nz, i0, j0, a.sx, a.sz, amp);
     __syncthreads();
     if (!jok) return;
     float cx[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
     const float zx = a.zx;
-    const int xw = a.pml_xw, zw = a.pml_zw;
+    const int xw = a.pml_xw, zw = a.pml_zw, sw = a.sponge_w;
+    const bool raw = a.v_raw != 0;
     int zc = -1;
-    if (j < zw) zc = j;
-    else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+    if (CPML) {
+        if (j < zw) zc = j;
+        else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+    }
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int il = wv + q * NW, i = i0 + il;
@@ -109,37 +132,48 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
         sz = zx * sz;
         const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         float dx = sx, dz = sz;
-        int xr = -1;
-        if (i < xw) xr = i;
-        else if (i >= nx - xw) xr = i - (nx - xw) + xw;
-        if (xr >= 0) {
-            const size_t e = (size_t)xr * nz + j;
-            const float psi = __builtin_fmaf(a.pml_x[3 * nx + i], sx, a.pml_x[2 * nx + i] * a.psi_px_in[e]);
-            a.psi_px_out[e] = psi;
-            dx = sx + psi;
+        float v0x = vx0[q], v0z = vz0[q];
+        if constexpr (CPML) {
+            int xr = -1;
+            if (i < xw) xr = i;
+            else if (i >= nx - xw) xr = i - (nx - xw) + xw;
+            if (xr >= 0) {
+                const size_t e = (size_t)xr * nz + j;
+                const float psi = __builtin_fmaf(a.pml_x[3 * nx + i], sx, a.pml_x[2 * nx + i] * a.psi_px_in[e]);
+                a.psi_px_out[e] = psi;
+                dx = sx + psi;
+            }
+            if (zc >= 0) {
+                const size_t e = (size_t)i * (2 * zw) + zc;
+                const float psi = __builtin_fmaf(a.pml_z[3 * nz + j], sz, a.pml_z[2 * nz + j] * a.psi_pz_in[e]);
+                a.psi_pz_out[e] = psi;
+                dz = sz + psi;
+            }
+        } else if (raw) {
+            const float f = tiled_sponge(i, nx, sw, axr[q], az_j);
+            v0x *= f;
+            v0z *= f;
         }
-        if (zc >= 0) {
-            const size_t e = (size_t)i * (2 * zw) + zc;
-            const float psi = __builtin_fmaf(a.pml_z[3 * nz + j], sz, a.pml_z[2 * nz + j] * a.psi_pz_in[e]);
-            a.psi_pz_out[e] = psi;
-            dz = sz + psi;
-        }
-        a.vx_out[o] = __builtin_fmaf(-bf[q], dx, vx0[q]);
-        a.vz_out[o] = __builtin_fmaf(-bf[q], dz, vz0[q]);
+        a.vx_out[o] = __builtin_fmaf(-bf[q], dx, v0x);
+        a.vz_out[o] = __builtin_fmaf(-bf[q], dz, v0z);
     }
 }
 
-template <int R, int TX, int NW, class Args>
+template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, float* s_vx, float* s_vz) {
+    constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ, DIVOUT = MODE == MODE_FWD_DIV;
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
     const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = j0 + lane;
     const bool jok = j < nz - R;
-    const float amp = a.amp[k];
+    const float amp = ADJ ? 0.f : a.amp[k];
+    // Divergences of step k (gradient modes).
+    float* dv = nullptr;
+    if constexpr (ADJ || DIVOUT) dv = a.div + (size_t)(k % a.div_period) * a.div_stride;
     // Own-cell inputs and the rows' receiver spans, before the tile.
-    float p0[RPW], kf[RPW];
+    float p0[RPW], kf[RPW], axr[RPW], dk[RPW], gr[RPW];
     int4 span[RPW];
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
@@ -148,8 +182,16 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
         const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         p0[q] = ok ? a.p_in[o] : 0.f;
         kf[q] = ok ? a.kf[o] : 0.f;
-        span[q] = a.rec_row[min(i, nx - 1)];
+        if constexpr (!CPML) axr[q] = ok ? a.ax[i] : 1.f;
+        if constexpr (ADJ) {
+            dk[q] = ok ? dv[o] : 0.f;
+            gr[q] = ok ? a.grad[o] : 0.f;
+        } else {
+            span[q] = a.rec_row[min(i, nx - 1)];
+        }
     }
+    float az_j = 1.f;
+    if constexpr (!CPML) az_j = jok ? a.az[j] : 1.f;
     tiled_stage<TX, NW, R, 0, false>(s_vx, a.vx_out, nz, i0, j0, 0, 0, 0.f);
     tiled_stage<TX, NW, 0, R, false>(s_vz, a.vz_out, nz, i0, j0, 0, 0, 0.f);
     __syncthreads();
@@ -158,10 +200,12 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
 #pragma unroll
     for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
     const float zx = a.zx;
-    const int xw = a.pml_xw, zw = a.pml_zw;
+    const int xw = a.pml_xw, zw = a.pml_zw, sw = a.sponge_w;
     int zc = -1;
-    if (j < zw) zc = j;
-    else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+    if (CPML) {
+        if (j < zw) zc = j;
+        else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+    }
     float* gk = a.rec_buf + (size_t)k * a.nslot;
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
@@ -176,64 +220,74 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
 #pragma unroll
         for (int m = 2; m <= R; ++m) sz = __builtin_fmaf(cx[m - 1], pz[m - 1] - pz[-m], sz);
         sz = zx * sz;
-        int xr = -1;
-        if (i < xw) xr = i;
-        else if (i >= nx - xw) xr = i - (nx - xw) + xw;
-        if (xr >= 0) {
-            const size_t e = (size_t)xr * nz + j;
-            const float psi = __builtin_fmaf(a.pml_x[nx + i], sx, a.pml_x[i] * a.psi_vx_in[e]);
-            a.psi_vx_out[e] = psi;
-            sx += psi;
-        }
-        if (zc >= 0) {
-            const size_t e = (size_t)i * (2 * zw) + zc;
-            const float psi = __builtin_fmaf(a.pml_z[nz + j], sz, a.pml_z[j] * a.psi_vz_in[e]);
-            a.psi_vz_out[e] = psi;
-            sz += psi;
+        if constexpr (CPML) {
+            int xr = -1;
+            if (i < xw) xr = i;
+            else if (i >= nx - xw) xr = i - (nx - xw) + xw;
+            if (xr >= 0) {
+                const size_t e = (size_t)xr * nz + j;
+                const float psi = __builtin_fmaf(a.pml_x[nx + i], sx, a.pml_x[i] * a.psi_vx_in[e]);
+                a.psi_vx_out[e] = psi;
+                sx += psi;
+            }
+            if (zc >= 0) {
+                const size_t e = (size_t)i * (2 * zw) + zc;
+                const float psi = __builtin_fmaf(a.pml_z[nz + j], sz, a.pml_z[j] * a.psi_vz_in[e]);
+                a.psi_vz_out[e] = psi;
+                sz += psi;
+            }
         }
         const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         float p = p0[q];
-        if (i == a.sx && j == a.sz) p -= amp;
-        const float pn = __builtin_fmaf(-kf[q], sx + sz, p);
+        if (!ADJ && i == a.sx && j == a.sz) p -= amp;
+        const float pc = __builtin_fmaf(-kf[q], sx + sz, p);
+        if constexpr (DIVOUT) dv[o] = sx + sz;
+        if constexpr (ADJ) {
+            a.grad[o] = __builtin_fmaf(p0[q], dk[q], gr[q]);
+            if (i == a.sx && j == a.sz) a.src_adj[k] = pc;
+        }
+        const float pn = CPML ? pc : pc * tiled_sponge(i, nx, sw, axr[q], az_j);
         a.p_out[o] = pn;
-        // Receivers of row i: the first span, then further spans (rare).
-        const int4 sp = span[q];
-        if (j >= sp.x && j < sp.y) gk[sp.z + (j - sp.x)] = pn;
-        for (int more = sp.w; more >= 0;) {
-            const int4 mp = a.rec_more[more];
-            if (j >= mp.x && j < mp.y) gk[mp.z + (j - mp.x)] = pn;
-            more = mp.w;
+        if constexpr (!ADJ) {
+            // Receivers of row i: the first span, then further spans (rare).
+            const int4 sp = span[q];
+            if (j >= sp.x && j < sp.y) gk[sp.z + (j - sp.x)] = pn;
+            for (int more = sp.w; more >= 0;) {
+                const int4 mp = a.rec_more[more];
+                if (j >= mp.x && j < mp.y) gk[mp.z + (j - mp.x)] = pn;
+                more = mp.w;
+            }
         }
     }
 }
 
-template <int R, int TX, int NW>
+template <int R, int TX, int NW, int MODE>
 __global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity(const AcousticStepArgs a) {
     __shared__ float s_p[(TX + 2 * R) * (kTiledTZ + 2 * R)];
-    tiled_velocity<R, TX, NW>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, s_p);
+    tiled_velocity<R, TX, NW, MODE>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, s_p);
 }
 
-template <int R, int TX, int NW>
+template <int R, int TX, int NW, int MODE>
 __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const AcousticStepArgs a) {
     __shared__ float s_vx[(TX + 2 * R) * kTiledTZ];
     __shared__ float s_vz[TX * (kTiledTZ + 2 * R)];
-    tiled_pressure<R, TX, NW>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, s_vx, s_vz);
+    tiled_pressure<R, TX, NW, MODE>(a, xcd_tile((int)blockIdx.x, a.ntz * a.nchunk), a.k, s_vx, s_vz);
 }
 
 // Many shots per launch: b.shots[s] is shot s's argument block (in place, so
 // one block per shot), tile_shot maps a global tile to its shot and
 // tile_base[s] is the shot's first global tile.
-template <int R, int TX, int NW>
+template <int R, int TX, int NW, int MODE>
 __global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity_batch(const BatchStepArgs b) {
     __shared__ float s_p[(TX + 2 * R) * (kTiledTZ + 2 * R)];
     const int tile = xcd_tile((int)blockIdx.x, b.ntile);
     const int s = ro(b.tile_shot, tile);
     using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
     CArgs& a = *(CArgs*)(b.shots + s);
-    tiled_velocity<R, TX, NW>(a, tile - ro(b.tile_base, s), b.k, s_p);
+    tiled_velocity<R, TX, NW, MODE>(a, tile - ro(b.tile_base, s), b.k, s_p);
 }
 
-template <int R, int TX, int NW>
+template <int R, int TX, int NW, int MODE>
 __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure_batch(const BatchStepArgs b) {
     __shared__ float s_vx[(TX + 2 * R) * kTiledTZ];
     __shared__ float s_vz[TX * (kTiledTZ + 2 * R)];
@@ -241,7 +295,7 @@ __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure_batch(const B
     const int s = ro(b.tile_shot, tile);
     using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
     CArgs& a = *(CArgs*)(b.shots + s);
-    tiled_pressure<R, TX, NW>(a, tile - ro(b.tile_base, s), b.k, s_vx, s_vz);
+    tiled_pressure<R, TX, NW, MODE>(a, tile - ro(b.tile_base, s), b.k, s_vx, s_vz);
 }
 
 }  // namespace sfd

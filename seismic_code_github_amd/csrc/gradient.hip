@@ -14,6 +14,9 @@
 //   4. dJ/dvp = -2 g / vp, plus the source term -sum_k adj_src_k w_k / vp_s^2
 //      (the injected amplitude dt rho_s vp_s w_k depends on vp at the source).
 // Memory: one segment of divergences plus nt/period checkpoints per shot.
+// Every step is the two-launch tiled step (acoustic_tiled.h) in place on the
+// first buffer of each field; the velocities, checkpoints included, keep the
+// tiled sponge step's unabsorbed values.
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -60,6 +63,11 @@ int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, 
     int rc = plan_create(ctx, &q, vp, den, wavelet, period, &g->pl);
     if (rc) return rc;
     sfd_acoustic_plan* pl = g->pl;
+    // Forward, recomputation and adjoint all run the tiled step at the
+    // default shape (the velocity fields keep the tiled step's unabsorbed
+    // values, checkpoints included).
+    rc = sfd_acoustic_plan_set_tiled(pl, kTiledTX, kTiledNW);
+    if (rc) return rc;
     g->nseg = (pl->nt + period - 1) / period;
     // slot -> cell (row-major over the padded state field) and weight.
     std::vector<int> cell(std::max(pl->nslot, 1), 0);
@@ -146,15 +154,15 @@ int grad_backward(GradShot* g) {
     hipStream_t s = ctx->stream;
     const int T = pl->grad_period, nt = pl->nt;
     const size_t n = (size_t)pl->nx * pl->nz;
-    for (int b = 0; b < 2; ++b)
-        for (Field* f : {&pl->ap[b], &pl->avx[b], &pl->avz[b]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+    for (Field* f : {&pl->ap[0], &pl->avx[0], &pl->avz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
     SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
     SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
-    StepKernel kdiv = pick_step_kernel(pl->radius, pl->nw, MODE_FWD_DIV);
-    StepKernel kadj = pick_step_kernel(pl->radius, pl->nw, MODE_ADJ);
-    if (!kdiv || !kadj) return fail(ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
-    int acur = 0;
-    int rc = launch_inject(pl, pl->grid(pl->ap[acur]), nt - 1);
+    StepKernel dv, dp, av, ap;
+    if (!pick_tiled(pl->radius, pl->ttx, pl->tnw, MODE_FWD_DIV, &dv, &dp) ||
+        !pick_tiled(pl->radius, pl->ttx, pl->tnw, MODE_ADJ, &av, &ap))
+        return fail(ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
+    // Every step runs in place on buffer 0 of the state and adjoint fields.
+    int rc = launch_inject(pl, pl->grid(pl->ap[0]), nt - 1);
     if (rc) return rc;
     for (int seg = g->nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * T, k1 = std::min(k0 + T, nt);
@@ -169,23 +177,24 @@ int grad_backward(GradShot* g) {
                 SFD_HIP(ctx, hipMemcpyAsync(pl->grid(*f[q]), src + q * n, n * 4, hipMemcpyDeviceToDevice, s));
         }
         for (int k = k0; k < k1; ++k) {
-            AcousticStepArgs a = make_args(pl, pl->cur, k);
-            hipLaunchKernelGGL(kdiv, dim3(a.ntz * a.nchunk), dim3(64 * pl->nw), 0, s, a);
-            SFD_HIP(ctx, hipGetLastError());
-            pl->cur ^= 1;
+            AcousticStepArgs a = make_args(pl, 0, k);
+            tiled_args(pl, a, true);
+            rc = launch_tiled(pl, dv, dp, a);
+            if (rc) return rc;
         }
         // Adjoint steps of the segment, in reverse.
         for (int k = k1 - 1; k >= k0; --k) {
-            AcousticStepArgs a = make_args(pl, pl->ap, pl->avx, pl->avz, acur, k);
-            hipLaunchKernelGGL(kadj, dim3(a.ntz * a.nchunk), dim3(64 * pl->nw), 0, s, a);
-            SFD_HIP(ctx, hipGetLastError());
-            acur ^= 1;
+            AcousticStepArgs a = make_args(pl, pl->ap, pl->avx, pl->avz, 0, k);
+            tiled_args(pl, a, true);
+            rc = launch_tiled(pl, av, ap, a);
+            if (rc) return rc;
             if (k >= 1) {
-                rc = launch_inject(pl, pl->grid(pl->ap[acur]), k - 1);
+                rc = launch_inject(pl, pl->grid(pl->ap[0]), k - 1);
                 if (rc) return rc;
             }
         }
     }
+    pl->v_raw = true;
     return SFD_OK;
 }
 
@@ -238,9 +247,10 @@ int grad_one(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, co
 }
 
 // ------------------------------------------------------------ shot groups
-// Shots of one gradient that share a model grid advance together: one launch
-// per time step for all of them in the forward, the recomputation and the
-// adjoint sweep, and one launch per step for all their residual injections.
+// Shots of one gradient that share a model grid advance together: one tiled
+// step (a velocity and a pressure launch) per time step for all of them in
+// the forward, the recomputation and the adjoint sweep, and one launch per
+// step for all their residual injections.
 
 // One receiver slot of one shot: p[shot][cell] += w * res[shot][k * nslot + slot].
 struct InjEntry {
@@ -258,37 +268,20 @@ __global__ void adjoint_inject_group_kernel(float* const* __restrict__ p, const 
     p[t.shot][t.cell] += t.w * res[t.shot][(size_t)k * nslot[t.shot] + t.slot];
 }
 
-template <int R>
-BatchKernel group_kernel_r(int mode) {
-    switch (mode) {
-    case MODE_FAST_SPONGE: return acoustic_batch_step_kernel<R, 2, MODE_FAST_SPONGE, default_slots<R>()>;
-    case MODE_FWD_DIV: return acoustic_batch_step_kernel<R, 2, MODE_FWD_DIV, default_slots<R>()>;
-    case MODE_ADJ: return acoustic_batch_step_kernel<R, 2, MODE_ADJ, default_slots<R>()>;
-    default: return nullptr;
-    }
-}
-
-BatchKernel group_kernel(int radius, int mode) {
-    switch (radius) {
-    case 4: return group_kernel_r<4>(mode);
-    case 5: return group_kernel_r<5>(mode);
-    case 8: return group_kernel_r<8>(mode);
-    default: return nullptr;
-    }
-}
-
 struct GradGroup {
     sfd_context* ctx = nullptr;
     std::vector<GradShot> shots;
     int nt = 0, period = 0, nseg = 0, ntile = 0, ninj = 0;
-    BatchKernel kfwd = nullptr, kdiv = nullptr, kadj = nullptr;
+    // Tiled step kernels (velocity, pressure) of the forward, the
+    // recomputation and the adjoint; all steps run in place on buffer 0.
+    BatchKernel kfwd[2] = {}, kdiv[2] = {}, kadj[2] = {};
     DeviceBuffer fwd_args, adj_args, tile_shot, tile_base, inj, inj_p, inj_res, inj_nslot;
     ~GradGroup() {
         for (auto& g : shots)
             if (g.pl) sfd_acoustic_plan_destroy(g.pl);
     }
     int nshot() const { return (int)shots.size(); }
-    BatchStepArgs args(const DeviceBuffer& blocks, int k, int parity) const {
+    BatchStepArgs args(const DeviceBuffer& blocks, int k) const {
         BatchStepArgs a{};
         a.shots = blocks.as<AcousticStepArgs>();
         a.tile_shot = tile_shot.as<int>();
@@ -296,7 +289,6 @@ struct GradGroup {
         a.nshot = nshot();
         a.ntile = ntile;
         a.k = k;
-        a.parity = parity;
         return a;
     }
 };
@@ -304,20 +296,20 @@ struct GradGroup {
 int group_tables(GradGroup* G) {
     sfd_context* ctx = G->ctx;
     const int n = G->nshot();
-    std::vector<AcousticStepArgs> fa(2 * (size_t)n), aa(2 * (size_t)n);
+    std::vector<AcousticStepArgs> fa(n), aa(n);
     std::vector<int> base(n), tshot;
-    std::vector<float*> pp(2 * (size_t)n);
+    std::vector<float*> pp(n);
     std::vector<const float*> rp(n);
     std::vector<int> ns(n);
     std::vector<InjEntry> inj;
     int t = 0;
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = G->shots[s].pl;
-        for (int par = 0; par < 2; ++par) {
-            fa[(size_t)par * n + s] = make_args(pl, par, 0);
-            aa[(size_t)par * n + s] = make_args(pl, pl->ap, pl->avx, pl->avz, par, 0);
-            pp[(size_t)par * n + s] = pl->grid(pl->ap[par]) - (size_t)pl->pad * pl->nz;
-        }
+        fa[s] = make_args(pl, 0, 0);
+        aa[s] = make_args(pl, pl->ap, pl->avx, pl->avz, 0, 0);
+        tiled_args(pl, fa[s], true);
+        tiled_args(pl, aa[s], true);
+        pp[s] = pl->grid(pl->ap[0]) - (size_t)pl->pad * pl->nz;
         const int nt = fa[s].ntz * fa[s].nchunk;
         base[s] = t;
         tshot.insert(tshot.end(), nt, s);
@@ -347,20 +339,22 @@ int group_tables(GradGroup* G) {
     return SFD_OK;
 }
 
-int group_inject(GradGroup* G, int parity, int k) {
+int group_inject(GradGroup* G, int k) {
     if (G->ninj == 0) return SFD_OK;
     const int bs = 256;
     hipLaunchKernelGGL(adjoint_inject_group_kernel, dim3((G->ninj + bs - 1) / bs), dim3(bs), 0, G->ctx->stream,
-                       G->inj_p.as<float*>() + (size_t)parity * G->nshot(), G->inj.as<InjEntry>(), G->ninj,
+                       G->inj_p.as<float*>(), G->inj.as<InjEntry>(), G->ninj,
                        G->inj_res.as<const float*>(), G->inj_nslot.as<int>(), k);
     SFD_HIP(G->ctx, hipGetLastError());
     return SFD_OK;
 }
 
-int group_launch(GradGroup* G, BatchKernel kern, const DeviceBuffer& blocks, int k, int parity) {
-    const BatchStepArgs a = G->args(blocks, k, parity);
-    hipLaunchKernelGGL(kern, dim3(G->ntile), dim3(128), 0, G->ctx->stream, a);
-    SFD_HIP(G->ctx, hipGetLastError());
+int group_launch(GradGroup* G, const BatchKernel kern[2], const DeviceBuffer& blocks, int k) {
+    const BatchStepArgs a = G->args(blocks, k);
+    for (int q = 0; q < 2; ++q) {
+        hipLaunchKernelGGL(kern[q], dim3(G->ntile), dim3(64 * kTiledNW), 0, G->ctx->stream, a);
+        SFD_HIP(G->ctx, hipGetLastError());
+    }
     return SFD_OK;
 }
 
@@ -371,8 +365,7 @@ int group_checkpoint(GradGroup* G, int seg, bool save) {
         sfd_acoustic_plan* pl = g.pl;
         const size_t n = (size_t)pl->nx * pl->nz;
         float* ck = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
-        const int c = save ? pl->cur : 0;
-        Field* f[3] = {&pl->p[c], &pl->vx[c], &pl->vz[c]};
+        Field* f[3] = {&pl->p[0], &pl->vx[0], &pl->vz[0]};
         for (int q = 0; q < 3; ++q) {
             float* dst = save ? ck + q * n : pl->grid(*f[q]);
             const float* src = save ? pl->grid(*f[q]) : ck + q * n;
@@ -397,17 +390,13 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (!rc && params[s].radius != params[0].radius)
             rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same radius", s);
         if (!rc) rc = grad_setup(ctx, &params[s], vp, den, wavelet[s], G.period, &G.shots[s]);
-        if (!rc) rc = sfd_acoustic_plan_set_tiling(G.shots[s].pl, 2, 0);
         if (rc) return rc;
     }
-    // One chunk length for the group (the first shot's automatic choice).
-    const int chunk = G.shots[0].pl->chunk;
-    for (auto& g : G.shots) g.pl->chunk = chunk;
     const int R = params[0].radius;
-    G.kfwd = group_kernel(R, MODE_FAST_SPONGE);
-    G.kdiv = group_kernel(R, MODE_FWD_DIV);
-    G.kadj = group_kernel(R, MODE_ADJ);
-    if (!G.kfwd || !G.kdiv || !G.kadj) return fail(ctx, SFD_ENOTSUP, "no gradient kernel for radius %d", R);
+    if (!pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_FAST_SPONGE, &G.kfwd[0], &G.kfwd[1]) ||
+        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_FWD_DIV, &G.kdiv[0], &G.kdiv[1]) ||
+        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_ADJ, &G.kadj[0], &G.kadj[1]))
+        return fail(ctx, SFD_ENOTSUP, "no gradient kernel for radius %d", R);
     G.nseg = (G.nt + G.period - 1) / G.period;
     int rc = group_tables(&G);
     if (rc) return rc;
@@ -420,14 +409,13 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     }
     for (int k = 0; k < G.nt; ++k) {
         if (k > 0 && k % G.period == 0) {
-            for (auto& g : G.shots) g.pl->cur = k & 1;
             rc = group_checkpoint(&G, k / G.period, true);
             if (rc) return rc;
         }
-        rc = group_launch(&G, G.kfwd, G.fwd_args, k, k & 1);
+        rc = group_launch(&G, G.kfwd, G.fwd_args, k);
         if (rc) return rc;
     }
-    for (auto& g : G.shots) g.pl->cur = G.nt & 1;
+    for (auto& g : G.shots) g.pl->v_raw = true;
     // 2. residuals, misfit, seeds.
     for (int q = 0; q < nshot; ++q) {
         GradShot& g = G.shots[q];
@@ -446,13 +434,11 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     // 3. backward: per segment, restore, recompute with divergences, adjoint.
     for (auto& g : G.shots) {
         sfd_acoustic_plan* pl = g.pl;
-        for (int b = 0; b < 2; ++b)
-            for (Field* f : {&pl->ap[b], &pl->avx[b], &pl->avz[b]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+        for (Field* f : {&pl->ap[0], &pl->avx[0], &pl->avz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
         SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
         SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
     }
-    int acur = 0;
-    rc = group_inject(&G, acur, G.nt - 1);
+    rc = group_inject(&G, G.nt - 1);
     if (rc) return rc;
     for (int seg = G.nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * G.period, k1 = std::min(k0 + G.period, G.nt);
@@ -465,15 +451,14 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
             if (rc) return rc;
         }
         for (int k = k0; k < k1; ++k) {
-            rc = group_launch(&G, G.kdiv, G.fwd_args, k, (k - k0) & 1);
+            rc = group_launch(&G, G.kdiv, G.fwd_args, k);
             if (rc) return rc;
         }
         for (int k = k1 - 1; k >= k0; --k) {
-            rc = group_launch(&G, G.kadj, G.adj_args, k, acur);
+            rc = group_launch(&G, G.kadj, G.adj_args, k);
             if (rc) return rc;
-            acur ^= 1;
             if (k >= 1) {
-                rc = group_inject(&G, acur, k - 1);
+                rc = group_inject(&G, k - 1);
                 if (rc) return rc;
             }
         }

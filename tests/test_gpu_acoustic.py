@@ -184,6 +184,70 @@ def test_cpml_tiled_step_bitwise_equals_march(ctx, radius):
         np.testing.assert_array_equal(pm1, pm0)
 
 
+@pytest.mark.parametrize("radius", [4, 5, 8])
+def test_sponge_tiled_step_bitwise_equals_march(ctx, radius):
+    """The tiled sponge step keeps the velocities unabsorbed between steps and
+    applies the sponge factor when it reads them: records, snapshots of all
+    three fields and the state (velocities absorbed on the way out) are
+    bitwise equal to the march kernel's, for every built tile shape."""
+    shot = W.smooth_random_2001x4001(nt=300, scale=10, radius=radius, boundary="sponge", snap_steps=(120, 299),
+                                     snap_fields=7)
+    print(f"case: sponge tiled vs march R={radius}", flush=True)
+    ref = sfd.AcousticPlan(shot, ctx)
+    ref.set_tiled(-1)
+    ref.run()
+    r0, s0 = ref.fetch()
+    st0, _ = ref.get_state()
+    assert np.abs(r0).max() > 0 and np.abs(s0[:, 1:]).max() > 0
+    for tx, nw in ((16, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 8), (32, 16)):
+        pl = sfd.AcousticPlan(shot, ctx)
+        pl.set_tiled(tx, nw)
+        pl.run()
+        r1, s1 = pl.fetch()
+        st1, _ = pl.get_state()
+        np.testing.assert_array_equal(r1, r0, err_msg=f"{tx}x{nw}")
+        np.testing.assert_array_equal(s1, s0, err_msg=f"{tx}x{nw}")
+        np.testing.assert_array_equal(st1, st0, err_msg=f"{tx}x{nw}")
+
+
+def test_sponge_tiled_switches_and_resume(ctx):
+    """Switching between the tiled and the march step mid-run, and resuming a
+    tiled plan through get_state / set_state, reproduce an uninterrupted
+    march run bit for bit (source two cells from the top sponge)."""
+    shot = W.layered_301(radius=8, nt=400, snap_steps=(99, 250, 399))
+    print("case: sponge tiled/march switches", flush=True)
+    ref = sfd.AcousticPlan(shot, ctx)
+    ref.set_tiled(-1)
+    ref.run()
+    r0, s0 = ref.fetch()
+    st0, _ = ref.get_state()
+    a = sfd.AcousticPlan(shot, ctx)  # tiled, then march
+    a.step(0, 150)
+    a.set_tiled(-1)
+    a.step(150, 250)
+    b = sfd.AcousticPlan(shot, ctx)  # march, then tiled
+    b.set_tiled(-1)
+    b.step(0, 150)
+    b.set_tiled()
+    b.step(150, 250)
+    for pl, name in ((a, "tiled->march"), (b, "march->tiled")):
+        r1, s1 = pl.fetch()
+        st1, _ = pl.get_state()
+        np.testing.assert_array_equal(r1, r0, err_msg=name)
+        np.testing.assert_array_equal(s1, s0, err_msg=name)
+        np.testing.assert_array_equal(st1, st0, err_msg=name)
+    c = sfd.AcousticPlan(shot, ctx)
+    c.step(0, 200)
+    mid, _ = c.get_state()
+    d = sfd.AcousticPlan(shot, ctx)
+    d.set_state(mid)
+    d.step(200, 200)
+    st_d, _ = d.get_state()
+    r_d, _ = d.fetch()
+    np.testing.assert_array_equal(st_d, st0)
+    np.testing.assert_array_equal(r_d[:, 200:], r0[:, 200:])
+
+
 def test_cpml_state_resume(ctx):
     shot = W.smooth_random_2001x4001(nt=300, scale=16)
     print("case: cpml resume", flush=True)

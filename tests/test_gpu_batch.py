@@ -26,10 +26,12 @@ def small_batch(boundary="cpml", nshot=5, nt=300, arith="fast"):
     return shots
 
 
-@pytest.mark.parametrize("boundary,tiling", [("cpml", "march"), ("cpml", "default"), ("sponge", "march")])
+@pytest.mark.parametrize("boundary,tiling", [("cpml", "march"), ("cpml", "default"), ("sponge", "march"),
+                                             ("sponge", "default")])
 def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
-    """tiling "default": C-PML batches run the two-launch tiled step (in
-    place), the single plans the fused march kernel; still bit for bit."""
+    """tiling "default": batches run the two-launch tiled step (in place;
+    sponge velocities kept unabsorbed between steps), the single plans the
+    fused march kernel; still bit for bit."""
     shots = small_batch(boundary)
     print(f"case: batch modes {boundary} {tiling}, shots {[s.vp.shape for s in shots]}", flush=True)
     singles = []
