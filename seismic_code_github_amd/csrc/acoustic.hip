@@ -875,7 +875,7 @@ struct sfd_acoustic_batch {
     std::vector<sfd_acoustic_plan*> plans;
     int nt = 0, radius = 0, mode = 0, nw = 0;
     int ntile = 0;
-    DeviceBuffer shot_args, tile_shot, tile_base;
+    DeviceBuffer shot_args, tile_shot, tile_base, tile_map;
     BatchKernel kernel = nullptr;
     // Fast arithmetic: two tiled launches per step, in place.
     bool tiled = false;
@@ -903,6 +903,7 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     const int n = (int)b->plans.size();
     std::vector<AcousticStepArgs> args(2 * (size_t)n);
     std::vector<int> base(n), tshot;
+    std::vector<int2> tmap;
     int t = 0;
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = b->plans[s];
@@ -913,6 +914,7 @@ int build_batch_tables(sfd_acoustic_batch* b) {
         const int nt = args[s].ntz * args[s].nchunk;
         base[s] = t;
         tshot.insert(tshot.end(), nt, s);
+        for (int q = 0; q < nt; ++q) tmap.push_back(make_int2(s, q));
         t += nt;
     }
     b->ntile = t;
@@ -920,6 +922,7 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
     if (e == hipSuccess) e = upload(b->tile_shot, tshot.data(), tshot.size(), st);
     if (e == hipSuccess) e = upload(b->tile_base, base.data(), base.size(), st);
+    if (e == hipSuccess) e = upload(b->tile_map, tmap.data(), tmap.size(), st);
     if (e != hipSuccess) return hip_check(b->ctx, e, "batch tables");
     b->drop_graph();
     return SFD_OK;
@@ -930,6 +933,7 @@ int enqueue_batch_steps(sfd_acoustic_batch* b) {
     a.shots = b->shot_args.as<AcousticStepArgs>();
     a.tile_shot = b->tile_shot.as<int>();
     a.tile_base = b->tile_base.as<int>();
+    a.tile_map = b->tile_map.as<int2>();
     a.nshot = (int)b->plans.size();
     a.ntile = b->ntile;
     for (int k = 0; k < b->nt; ++k) {

@@ -560,11 +560,14 @@ __global__ __launch_bounds__(64 * NW) void acoustic_step_kernel(const AcousticSt
 // Many shots in one launch.  shots[parity * nshot + s] is the argument block
 // of shot s with its ping-pong buffers in the given order; the workgroups of
 // one launch cover the tiles of every shot, tile_shot maps a global tile to
-// its shot and tile_base[s] is the shot's first global tile.
+// its shot and tile_base[s] is the shot's first global tile.  The tiled
+// kernels (in place, one block per shot) read tile_map[g] = (shot, the
+// shot's tile) instead, so tiles of different shots may interleave.
 struct BatchStepArgs {
     const AcousticStepArgs* shots;
     const int* tile_shot;
     const int* tile_base;
+    const int2* tile_map;
     int nshot, ntile, k, parity;
 };
 

@@ -275,27 +275,25 @@ __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure(const Acousti
 }
 
 // Many shots per launch: b.shots[s] is shot s's argument block (in place, so
-// one block per shot), tile_shot maps a global tile to its shot and
-// tile_base[s] is the shot's first global tile.
+// one block per shot) and b.tile_map[g] = (shot, the shot's tile) of global
+// tile g.
 template <int R, int TX, int NW, int MODE>
 __global__ __launch_bounds__(64 * NW) void acoustic_tiled_velocity_batch(const BatchStepArgs b) {
     __shared__ float s_p[(TX + 2 * R) * (kTiledTZ + 2 * R)];
-    const int tile = xcd_tile((int)blockIdx.x, b.ntile);
-    const int s = ro(b.tile_shot, tile);
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile((int)blockIdx.x, b.ntile);
     using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
-    CArgs& a = *(CArgs*)(b.shots + s);
-    tiled_velocity<R, TX, NW, MODE>(a, tile - ro(b.tile_base, s), b.k, s_p);
+    CArgs& a = *(CArgs*)(b.shots + ro(m, 0));
+    tiled_velocity<R, TX, NW, MODE>(a, ro(m, 1), b.k, s_p);
 }
 
 template <int R, int TX, int NW, int MODE>
 __global__ __launch_bounds__(64 * NW) void acoustic_tiled_pressure_batch(const BatchStepArgs b) {
     __shared__ float s_vx[(TX + 2 * R) * kTiledTZ];
     __shared__ float s_vz[TX * (kTiledTZ + 2 * R)];
-    const int tile = xcd_tile((int)blockIdx.x, b.ntile);
-    const int s = ro(b.tile_shot, tile);
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile((int)blockIdx.x, b.ntile);
     using CArgs = const __attribute__((address_space(4))) AcousticStepArgs;
-    CArgs& a = *(CArgs*)(b.shots + s);
-    tiled_pressure<R, TX, NW, MODE>(a, tile - ro(b.tile_base, s), b.k, s_vx, s_vz);
+    CArgs& a = *(CArgs*)(b.shots + ro(m, 0));
+    tiled_pressure<R, TX, NW, MODE>(a, ro(m, 1), b.k, s_vx, s_vz);
 }
 
 }  // namespace sfd

@@ -275,7 +275,7 @@ struct GradGroup {
     // Tiled step kernels (velocity, pressure) of the forward, the
     // recomputation and the adjoint; all steps run in place on buffer 0.
     BatchKernel kfwd[2] = {}, kdiv[2] = {}, kadj[2] = {};
-    DeviceBuffer fwd_args, adj_args, tile_shot, tile_base, inj, inj_p, inj_res, inj_nslot;
+    DeviceBuffer fwd_args, adj_args, tile_map, inj, inj_p, inj_res, inj_nslot;
     ~GradGroup() {
         for (auto& g : shots)
             if (g.pl) sfd_acoustic_plan_destroy(g.pl);
@@ -284,8 +284,7 @@ struct GradGroup {
     BatchStepArgs args(const DeviceBuffer& blocks, int k) const {
         BatchStepArgs a{};
         a.shots = blocks.as<AcousticStepArgs>();
-        a.tile_shot = tile_shot.as<int>();
-        a.tile_base = tile_base.as<int>();
+        a.tile_map = tile_map.as<int2>();
         a.nshot = nshot();
         a.ntile = ntile;
         a.k = k;
@@ -297,23 +296,26 @@ int group_tables(GradGroup* G) {
     sfd_context* ctx = G->ctx;
     const int n = G->nshot();
     std::vector<AcousticStepArgs> fa(n), aa(n);
-    std::vector<int> base(n), tshot;
     std::vector<float*> pp(n);
     std::vector<const float*> rp(n);
     std::vector<int> ns(n);
     std::vector<InjEntry> inj;
-    int t = 0;
+    // The shots share the model grid, vp and den; with equal dt their
+    // per-cell coefficients kf and bf are equal too, and every step reads
+    // the first shot's copy (one copy in the caches for the whole group).
+    bool share = true;
+    for (int s = 1; s < n; ++s) share = share && G->shots[s].pl->dt == G->shots[0].pl->dt;
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = G->shots[s].pl;
         fa[s] = make_args(pl, 0, 0);
         aa[s] = make_args(pl, pl->ap, pl->avx, pl->avz, 0, 0);
         tiled_args(pl, fa[s], true);
         tiled_args(pl, aa[s], true);
+        if (share && s > 0) {
+            fa[s].kf = aa[s].kf = fa[0].kf;
+            fa[s].bf = aa[s].bf = fa[0].bf;
+        }
         pp[s] = pl->grid(pl->ap[0]) - (size_t)pl->pad * pl->nz;
-        const int nt = fa[s].ntz * fa[s].nchunk;
-        base[s] = t;
-        tshot.insert(tshot.end(), nt, s);
-        t += nt;
         rp[s] = pl->res.as<float>();
         ns[s] = std::max(pl->nslot, 1);
         std::vector<int> cell(pl->nslot);
@@ -324,13 +326,20 @@ int group_tables(GradGroup* G) {
         }
         for (int q = 0; q < pl->nslot; ++q) inj.push_back({s, cell[q], w[q], q});
     }
-    G->ntile = t;
+    // Every shot has the same tiles (same grid); tiles go shot after shot.
+    // (Interleaving the shots' tiles, so that each kf / bf tile is read
+    // for all shots in turn, measured 2 % slower on MI355X for 16 shots of
+    // 881 x 881: it loses more halo reuse between a shot's neighbouring
+    // tiles than it gains.)
+    const int per = fa[0].ntz * fa[0].nchunk;
+    std::vector<int2> tmap((size_t)n * per);
+    for (int g = 0; g < n * per; ++g) tmap[g] = make_int2(g / per, g % per);
+    G->ntile = n * per;
     G->ninj = (int)inj.size();
     hipStream_t st = ctx->stream;
     hipError_t e = upload(G->fwd_args, fa.data(), fa.size(), st);
     if (e == hipSuccess) e = upload(G->adj_args, aa.data(), aa.size(), st);
-    if (e == hipSuccess) e = upload(G->tile_shot, tshot.data(), tshot.size(), st);
-    if (e == hipSuccess) e = upload(G->tile_base, base.data(), base.size(), st);
+    if (e == hipSuccess) e = upload(G->tile_map, tmap.data(), tmap.size(), st);
     if (e == hipSuccess) e = upload(G->inj_p, pp.data(), pp.size(), st);
     if (e == hipSuccess) e = upload(G->inj_res, rp.data(), rp.size(), st);
     if (e == hipSuccess) e = upload(G->inj_nslot, ns.data(), ns.size(), st);
