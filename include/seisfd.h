@@ -163,14 +163,48 @@ enum sfd_batch_run {
 int sfd_acoustic_batch_create(sfd_context *ctx, int nshot, const sfd_acoustic_params *params,
                               const float *const *vp, const float *const *den,
                               const float *const *wavelet, sfd_acoustic_batch **batch);
-/* Same tiling for every shot (0 = automatic); invalidates a captured graph. */
+/* Same tiling for every shot; invalidates a captured graph.  Explicit values
+ * select the fused march kernel; (0, 0) = automatic: the shape the context's
+ * autotuner cache holds for the batch's class, else the tiled step's default
+ * (fast arithmetic) or the march kernel's. */
 int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch *batch, int waves, int chunk_rows);
+/* Tiled step for every shot at tile_rows x 64 columns per workgroup of
+ * `waves` (8x4, 16x4, 16x8, 32x8 built); 0 = the default shape, tile_rows < 0
+ * = the march kernel.  Invalidates a captured graph. */
+int sfd_acoustic_batch_set_tiled(sfd_acoustic_batch *batch, int tile_rows, int waves);
 /* Zeroes every wavefield and propagates all shots nt steps (how: enum
  * sfd_batch_run).  elapsed_ms: device time of the propagation. */
 int sfd_acoustic_batch_run(sfd_acoustic_batch *batch, int how, float *elapsed_ms);
 /* Gather of one shot, [nrec][nt] trace-major. */
 int sfd_acoustic_batch_fetch(sfd_acoustic_batch *batch, int shot, float *record);
 int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
+
+/* ---------------------------------------------------------------- autotuner
+ * (no counterpart in the reference).  Times every built launch shape of the
+ * step (the tiled step's shapes in fast arithmetic, the fused march kernel at
+ * 1, 2 and 4 waves) over min(trial_steps, nt) steps from a zero state, keeps
+ * the fastest and records it in the context's cache under the grid class:
+ *   plan:  radius, boundary + arithmetic, nx and nz to the nearest power of 2
+ *   batch: radius, boundary + arithmetic, shot count and total cells likewise.
+ * Plans and batches created later in the context with the same class start
+ * with the cached shape.  The state is reset (as by the run entry points);
+ * results are bitwise the same for every shape.  best_us (may be NULL): device
+ * time per step of the chosen shape. */
+int sfd_acoustic_plan_autotune(sfd_acoustic_plan *plan, int trial_steps, float *best_us);
+int sfd_acoustic_batch_autotune(sfd_acoustic_batch *batch, int trial_steps, float *best_us);
+/* Current launch shape: kind 0 = tiled (a = tile rows, b = waves), kind 1 =
+ * fused march kernel (a = waves, b = rows per chunk). */
+int sfd_acoustic_plan_get_shape(const sfd_acoustic_plan *plan, int *kind, int *a, int *b);
+int sfd_acoustic_batch_get_shape(const sfd_acoustic_batch *batch, int *kind, int *a, int *b);
+/* The context's autotuner cache as a JSON file
+ *   {"seisfd_tuning": 1, "entries": [{"arch", "class", "kind", "a", "b", "us_per_step"}, ...]}.
+ * Saving merges into an existing file (entries of other GPUs are kept);
+ * loading adds the entries measured on this GPU's architecture and rejects a
+ * malformed file as a whole (SFD_EINVAL, nothing loaded). */
+int sfd_context_save_tuning(sfd_context *ctx, const char *path);
+int sfd_context_load_tuning(sfd_context *ctx, const char *path);
+/* Number of grid classes in the context's cache. */
+int sfd_context_tuning_size(const sfd_context *ctx);
 
 /* ---------------------------------------------------------------- gradient
  * Velocity gradient by the discrete adjoint of the fast-arithmetic sponge

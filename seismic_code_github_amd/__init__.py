@@ -120,6 +120,16 @@ _SIGNATURES = {
                                                  ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
                                                  ctypes.POINTER(ctypes.c_void_p)]),
     "sfd_acoustic_batch_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_batch_set_tiled": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_acoustic_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_acoustic_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "sfd_acoustic_batch_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "sfd_context_save_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "sfd_context_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "sfd_context_tuning_size": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_acoustic_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
     "sfd_acoustic_batch_fetch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
     "sfd_acoustic_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -195,6 +205,17 @@ class Context:
         if rc != SFD_OK:
             msg = lib().sfd_last_error(self.handle) or b""
             raise SeisfdError(rc, f"{what}: {msg.decode()}")
+
+    def save_tuning(self, path: str):
+        """Writes (merges) the autotuner cache into a JSON file."""
+        self.check(lib().sfd_context_save_tuning(self.handle, os.fsencode(path)), "save_tuning")
+
+    def load_tuning(self, path: str):
+        """Adds the entries of a JSON tuning file measured on this GPU's architecture."""
+        self.check(lib().sfd_context_load_tuning(self.handle, os.fsencode(path)), "load_tuning")
+
+    def tuning_size(self) -> int:
+        return lib().sfd_context_tuning_size(self.handle)
 
     def close(self):
         if self.handle:
@@ -296,6 +317,20 @@ class AcousticPlan:
         """Fast arithmetic: the two-launch tiled step at tile_rows x waves
         (0 = the default shape; tile_rows < 0 = the fused march kernel)."""
         self.ctx.check(lib().sfd_acoustic_plan_set_tiled(self.handle, tile_rows, waves), "set_tiled")
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        """Times every launch shape, keeps the fastest (cached in the context);
+        returns its device time per step in us.  Resets the state."""
+        us = ctypes.c_float()
+        self.ctx.check(lib().sfd_acoustic_plan_autotune(self.handle, trial_steps, ctypes.byref(us)), "autotune")
+        return us.value
+
+    def shape(self):
+        """(kind, a, b): kind 0 = tiled (rows, waves), 1 = march (waves, chunk rows)."""
+        k, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().sfd_acoustic_plan_get_shape(self.handle, ctypes.byref(k), ctypes.byref(a),
+                                                         ctypes.byref(b)), "get_shape")
+        return k.value, a.value, b.value
 
     def run(self) -> float:
         ms = ctypes.c_float()
@@ -452,6 +487,24 @@ class AcousticBatch:
 
     def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
         self.ctx.check(lib().sfd_acoustic_batch_set_tiling(self.handle, waves, chunk_rows), "batch_set_tiling")
+
+    def set_tiled(self, tile_rows: int = 0, waves: int = 0):
+        self.ctx.check(lib().sfd_acoustic_batch_set_tiled(self.handle, tile_rows, waves), "batch_set_tiled")
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        """Times every launch shape, keeps the fastest (cached in the context);
+        returns its device time per step in us."""
+        us = ctypes.c_float()
+        self.ctx.check(lib().sfd_acoustic_batch_autotune(self.handle, trial_steps, ctypes.byref(us)),
+                       "batch_autotune")
+        return us.value
+
+    def shape(self):
+        """(kind, a, b): kind 0 = tiled (rows, waves), 1 = march (waves, chunk rows)."""
+        k, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().sfd_acoustic_batch_get_shape(self.handle, ctypes.byref(k), ctypes.byref(a),
+                                                          ctypes.byref(b)), "batch_get_shape")
+        return k.value, a.value, b.value
 
     def run(self, how: int = BATCH_GRAPH) -> float:
         ms = ctypes.c_float()

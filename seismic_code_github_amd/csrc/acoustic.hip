@@ -248,22 +248,42 @@ bool pick_tiled(int radius, int tx, int nw, int mode, StepKernel* v, StepKernel*
     }
 }
 
-template <int R>
-bool pick_tiled_batch_r(int tx, int nw, int mode, BatchKernel* v, BatchKernel* p) {
-    if (tx != kTiledTX || nw != kTiledNW) return false;
+template <int R, int TX, int NW>
+bool tiled_batch_mode(int mode, BatchKernel* v, BatchKernel* p) {
     switch (mode) {
-#define SFD_TILED_BATCH(M)                                              \
-    case M:                                                             \
-        *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, M>;   \
-        *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, M>;   \
+#define SFD_TILED_BATCH(M)                                    \
+    case M:                                                   \
+        *v = acoustic_tiled_velocity_batch<R, TX, NW, M>;     \
+        *p = acoustic_tiled_pressure_batch<R, TX, NW, M>;     \
         return true;
         SFD_TILED_BATCH(MODE_FAST_CPML)
         SFD_TILED_BATCH(MODE_FAST_SPONGE)
-        SFD_TILED_BATCH(MODE_FWD_DIV)
-        SFD_TILED_BATCH(MODE_ADJ)
 #undef SFD_TILED_BATCH
     default:
         return false;
+    }
+}
+
+// Tiled shapes of batches (the autotuner's candidates); the gradient's modes
+// at the default shape.
+template <int R>
+bool pick_tiled_batch_r(int tx, int nw, int mode, BatchKernel* v, BatchKernel* p) {
+    if (tx == kTiledTX && nw == kTiledNW && (mode == MODE_FWD_DIV || mode == MODE_ADJ)) {
+        if (mode == MODE_FWD_DIV) {
+            *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
+            *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
+        } else {
+            *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_ADJ>;
+            *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_ADJ>;
+        }
+        return true;
+    }
+    switch (tx * 100 + nw) {
+    case 804: return tiled_batch_mode<R, 8, 4>(mode, v, p);
+    case 1604: return tiled_batch_mode<R, 16, 4>(mode, v, p);
+    case 1608: return tiled_batch_mode<R, 16, 8>(mode, v, p);
+    case 3208: return tiled_batch_mode<R, 32, 8>(mode, v, p);
+    default: return false;
     }
 }
 
@@ -377,6 +397,32 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* P, const Fi
 
 AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
     return make_args(pl, pl->p, pl->vx, pl->vz, cur, k);
+}
+
+// Autotuner grid class of a plan: radius, boundary and arithmetic, nx and nz
+// to the nearest power of 2.
+const char* mode_name(int mode) {
+    switch (mode) {
+    case MODE_FAST_CPML: return "cpml";
+    case MODE_FAST_SPONGE: return "sponge";
+    case MODE_REF_SPONGE: return "sponge-ref";
+    default: return "other";
+    }
+}
+
+int log2_round(double n) { return (int)std::lround(std::log2(std::max(n, 1.0))); }
+
+std::string plan_class(const sfd_acoustic_plan* pl) {
+    char buf[96];
+    snprintf(buf, sizeof buf, "acoustic:R%d:%s:x%d:z%d", pl->radius, mode_name(pl->mode), log2_round(pl->nx),
+             log2_round(pl->nz));
+    return buf;
+}
+
+// Sets a plan's launch shape: kind 0 = tiled (a rows, b waves), 1 = march (a
+// waves, b chunk rows).
+int apply_shape(sfd_acoustic_plan* pl, const TunedShape& t) {
+    return t.kind == 0 ? sfd_acoustic_plan_set_tiled(pl, t.a, t.b) : sfd_acoustic_plan_set_tiling(pl, t.a, t.b);
 }
 
 float sponge_at(const sfd_acoustic_plan* pl, int i, int j) {
@@ -659,6 +705,9 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
         pl->ttx = kTiledTX;
         pl->tnw = kTiledNW;
     }
+    // A shape the context's autotuner measured for this grid class wins.
+    const auto tuned = ctx->tuning.find(plan_class(pl));
+    if (tuned != ctx->tuning.end()) (void)apply_shape(pl, tuned->second);
     rc = zero_state(pl);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(s);
@@ -877,8 +926,10 @@ struct sfd_acoustic_batch {
     int ntile = 0;
     DeviceBuffer shot_args, tile_shot, tile_base, tile_map;
     BatchKernel kernel = nullptr;
-    // Fast arithmetic: two tiled launches per step, in place.
+    // Fast arithmetic: two tiled launches per step, in place, at ttx rows x
+    // tnw waves per tile.
     bool tiled = false;
+    int ttx = 0, tnw = 0;
     BatchKernel tvel = nullptr, tpres = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -896,6 +947,17 @@ struct sfd_acoustic_batch {
 };
 
 namespace {
+
+// Autotuner grid class of a batch: radius, boundary and arithmetic, the shot
+// count and the total cells to the nearest power of 2.
+std::string batch_class(const sfd_acoustic_batch* b) {
+    double cells = 0;
+    for (auto* pl : b->plans) cells += (double)pl->nx * pl->nz;
+    char buf[96];
+    snprintf(buf, sizeof buf, "batch:R%d:%s:n%d:c%d", b->radius, mode_name(b->mode),
+             log2_round((double)b->plans.size()), log2_round(cells));
+    return buf;
+}
 
 // Tile tables and per-shot argument blocks (both ping-pong orders; the
 // tiled step runs in place and needs one).
@@ -928,7 +990,7 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     return SFD_OK;
 }
 
-int enqueue_batch_steps(sfd_acoustic_batch* b) {
+int enqueue_batch_steps(sfd_acoustic_batch* b, int nsteps) {
     BatchStepArgs a{};
     a.shots = b->shot_args.as<AcousticStepArgs>();
     a.tile_shot = b->tile_shot.as<int>();
@@ -936,11 +998,11 @@ int enqueue_batch_steps(sfd_acoustic_batch* b) {
     a.tile_map = b->tile_map.as<int2>();
     a.nshot = (int)b->plans.size();
     a.ntile = b->ntile;
-    for (int k = 0; k < b->nt; ++k) {
+    for (int k = 0; k < nsteps; ++k) {
         a.k = k;
         a.parity = k & 1;
         if (b->tiled) {
-            const dim3 block(64 * b->plans[0]->tnw);
+            const dim3 block(64 * b->tnw);
             hipLaunchKernelGGL(b->tvel, dim3(b->ntile), block, 0, b->ctx->stream, a);
             SFD_HIP(b->ctx, hipGetLastError());
             hipLaunchKernelGGL(b->tpres, dim3(b->ntile), block, 0, b->ctx->stream, a);
@@ -993,21 +1055,44 @@ int sfd_acoustic_batch_create(sfd_context* ctx, int nshot, const sfd_acoustic_pa
     return SFD_OK;
 }
 
+int sfd_acoustic_batch_set_tiled(sfd_acoustic_batch* b, int tile_rows, int waves) {
+    if (!b) return SFD_EINVAL;
+    if (tile_rows < 0) return sfd_acoustic_batch_set_tiling(b, 2, 0);
+    if (!tiled_mode_ok(b->mode)) return fail(b->ctx, SFD_ENOTSUP, "the tiled step exists in fast arithmetic only");
+    const int tx = tile_rows ? tile_rows : kTiledTX, nw = waves ? waves : kTiledNW;
+    BatchKernel v, p;
+    if (!pick_tiled_batch(b->radius, tx, nw, b->mode, &v, &p))
+        return fail(b->ctx, SFD_EINVAL, "no batched tiled step for %d rows x %d waves", tx, nw);
+    for (auto* pl : b->plans) {
+        const int rc = sfd_acoustic_plan_set_tiled(pl, tx, nw);
+        if (rc) return rc;
+    }
+    b->tiled = true;
+    b->ttx = tx;
+    b->tnw = nw;
+    b->tvel = v;
+    b->tpres = p;
+    return build_batch_tables(b);
+}
+
 int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch* b, int waves, int chunk_rows) {
     if (!b) return SFD_EINVAL;
+    // Automatic: the shape the context's autotuner cache holds for this
+    // batch's class, else the tiled step's default shape (fast arithmetic)
+    // or the march kernel's default tiling.
+    if (waves == 0 && chunk_rows == 0) {
+        const auto it = b->ctx->tuning.find(batch_class(b));
+        if (it != b->ctx->tuning.end()) {
+            const TunedShape t = it->second;
+            const int rc = t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b)
+                                       : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
+            if (rc == SFD_OK) return rc;
+        }
+        if (tiled_mode_ok(b->mode)) return sfd_acoustic_batch_set_tiled(b, kTiledTX, kTiledNW);
+    }
+    b->tiled = false;
     // Batches hold far more cells than one shot, so the default chunk is
     // longer (less queue fill per cell) while the grid still fills the GPU.
-    // Fast arithmetic: the tiled two-launch step unless a march tiling is
-    // asked for.
-    b->tiled = waves == 0 && chunk_rows == 0 && tiled_mode_ok(b->mode) &&
-               pick_tiled_batch(b->radius, kTiledTX, kTiledNW, b->mode, &b->tvel, &b->tpres);
-    if (b->tiled) {
-        for (auto* pl : b->plans) {
-            const int rc = sfd_acoustic_plan_set_tiled(pl, kTiledTX, kTiledNW);
-            if (rc) return rc;
-        }
-        return build_batch_tables(b);
-    }
     const int nw = waves ? waves : 2;
     long cells = 0;
     for (auto* pl : b->plans) cells += (long)pl->nx * pl->nz;
@@ -1035,7 +1120,7 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
     if (how == SFD_BATCH_GRAPH && !b->exec) {
         SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
         SFD_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_batch_steps(b);
+        const int rc = enqueue_batch_steps(b, b->nt);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc) {
@@ -1050,7 +1135,7 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
     if (how == SFD_BATCH_GRAPH) {
         SFD_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
     } else if (how == SFD_BATCH_LAUNCHES) {
-        const int rc = enqueue_batch_steps(b);
+        const int rc = enqueue_batch_steps(b, b->nt);
         if (rc) return rc;
     } else {
         for (auto* pl : b->plans) {
@@ -1081,6 +1166,131 @@ int sfd_acoustic_batch_destroy(sfd_acoustic_batch* b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     delete b;
+    return SFD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuner
+
+namespace {
+
+// Device time per step of n steps from a zero state: the better of two timed
+// runs after a two-step warm-up.
+template <class Run>
+int time_steps(sfd_context* ctx, int n, Run run, float* us) {
+    float best = 3.4e38f;
+    for (int rep = 0; rep < 3; ++rep) {
+        const int m = rep == 0 ? std::min(n, 2) : n;
+        int rc = run(-1);  // reset the state (outside the timed region)
+        if (rc) return rc;
+        SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        rc = run(m);
+        if (rc) return rc;
+        SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        SFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (rep > 0) best = std::min(best, ms * 1000.f / m);
+    }
+    *us = best;
+    return SFD_OK;
+}
+
+// Candidates: every built tiled shape (fast arithmetic) and the march kernel
+// at 1, 2 and 4 waves with its automatic chunk.
+std::vector<TunedShape> shape_candidates(int mode, bool batch) {
+    std::vector<TunedShape> c;
+    if (tiled_mode_ok(mode)) {
+        static const int single[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 8}, {32, 16}};
+        static const int batched[][2] = {{8, 4}, {16, 4}, {16, 8}, {32, 8}};
+        if (batch)
+            for (auto& t : batched) c.push_back({0, t[0], t[1], 0.f});
+        else
+            for (auto& t : single) c.push_back({0, t[0], t[1], 0.f});
+    }
+    for (int w : {1, 2, 4}) c.push_back({1, w, 0, 0.f});
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfd_acoustic_plan_autotune(sfd_acoustic_plan* pl, int trial_steps, float* best_us) {
+    if (!pl) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    if (trial_steps < 1) return fail(ctx, SFD_EINVAL, "trial_steps must be >= 1");
+    const int n = std::min(trial_steps, pl->nt);
+    if (n < 1) return fail(ctx, SFD_EINVAL, "the plan has no time steps to time");
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    TunedShape best;
+    best.us = 3.4e38f;
+    for (TunedShape t : shape_candidates(pl->mode, false)) {
+        int rc = apply_shape(pl, t);
+        if (rc) return rc;
+        if (t.kind == 1) t.b = pl->chunk;
+        rc = time_steps(ctx, n, [&](int m) { return m < 0 ? zero_state(pl) : enqueue_steps(pl, 0, m); }, &t.us);
+        if (rc) return rc;
+        if (t.us < best.us) best = t;
+    }
+    int rc = apply_shape(pl, best);
+    if (!rc) rc = zero_state(pl);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->tuning[plan_class(pl)] = best;
+    if (best_us) *best_us = best.us;
+    return SFD_OK;
+}
+
+int sfd_acoustic_batch_autotune(sfd_acoustic_batch* b, int trial_steps, float* best_us) {
+    if (!b) return SFD_EINVAL;
+    sfd_context* ctx = b->ctx;
+    if (trial_steps < 1) return fail(ctx, SFD_EINVAL, "trial_steps must be >= 1");
+    const int n = std::min(trial_steps, b->nt);
+    if (n < 1) return fail(ctx, SFD_EINVAL, "the batch has no time steps to time");
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    auto run = [&](int m) -> int {
+        if (m >= 0) return enqueue_batch_steps(b, m);
+        for (auto* pl : b->plans) {
+            const int rc = zero_state(pl);
+            if (rc) return rc;
+        }
+        return SFD_OK;
+    };
+    TunedShape best;
+    best.us = 3.4e38f;
+    for (TunedShape t : shape_candidates(b->mode, true)) {
+        int rc = t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b) : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
+        if (rc) return rc;
+        if (t.kind == 1) t.b = b->plans[0]->chunk;
+        rc = time_steps(ctx, n, run, &t.us);
+        if (rc) return rc;
+        if (t.us < best.us) best = t;
+    }
+    int rc = best.kind == 0 ? sfd_acoustic_batch_set_tiled(b, best.a, best.b)
+                            : sfd_acoustic_batch_set_tiling(b, best.a, best.b);
+    if (!rc) rc = run(-1);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->tuning[batch_class(b)] = best;
+    if (best_us) *best_us = best.us;
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_get_shape(const sfd_acoustic_plan* pl, int* kind, int* a, int* b) {
+    if (!pl || !kind || !a || !b) return SFD_EINVAL;
+    *kind = pl->ttx > 0 ? 0 : 1;
+    *a = pl->ttx > 0 ? pl->ttx : pl->nw;
+    *b = pl->ttx > 0 ? pl->tnw : pl->chunk;
+    return SFD_OK;
+}
+
+int sfd_acoustic_batch_get_shape(const sfd_acoustic_batch* bt, int* kind, int* a, int* b) {
+    if (!bt || !kind || !a || !b) return SFD_EINVAL;
+    *kind = bt->tiled ? 0 : 1;
+    *a = bt->tiled ? bt->ttx : bt->nw;
+    *b = bt->tiled ? bt->tnw : bt->plans[0]->chunk;
     return SFD_OK;
 }
 

@@ -5,16 +5,31 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
 #include "seisfd.h"
+
+namespace sfd {
+// A launch shape chosen by the autotuner for one grid class: kind 0 = the
+// tiled step (a = tile rows, b = waves), kind 1 = the fused march kernel (a =
+// waves, b = rows per chunk); us = device time per step when it was measured.
+struct TunedShape {
+    int kind = 0, a = 0, b = 0;
+    float us = 0.f;
+};
+}  // namespace sfd
 
 struct sfd_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    // Autotuner cache: grid class -> fastest measured shape on this device
+    // (saved to and loaded from JSON by sfd_context_save/load_tuning).
+    std::string arch;
+    std::map<std::string, sfd::TunedShape> tuning;
 };
 
 namespace sfd {

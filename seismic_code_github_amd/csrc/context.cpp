@@ -111,11 +111,14 @@ int sfd_context_create(int device, sfd_context** out) {
     sfd_context* ctx = new (std::nothrow) sfd_context;
     if (!ctx) return SFD_ENOMEM;
     ctx->device = device;
+    hipDeviceProp_t prop;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipGetDeviceProperties(&prop, device) != hipSuccess) {
         sfd_context_destroy(ctx);
         return SFD_EDEVICE;
     }
+    ctx->arch = prop.gcnArchName;
     *out = ctx;
     return SFD_OK;
 }
