@@ -10,6 +10,11 @@ One "step" of this benchmark is one whole shot (nt time steps) with the model,
 wavelet and geometry already resident in device memory.  value = Mcell-updates/s
 over all stored cells (boundary layers included).
 
+Launch shapes: the library's autotuner (sfd_acoustic_plan_autotune /
+sfd_acoustic_batch_autotune) times every built shape for 20 steps before the
+timed shots and keeps the fastest (--no-tune: the library's defaults); the
+chosen shape is reported as launch_shape.
+
 Roofline: the library runs each time step as two launches (velocities, then
 pressure; acoustic_tiled.h), so the roofline's unit is one time step:
 achieved = the fused step's algorithmic bytes (32 B per cell plus the C-PML
@@ -72,7 +77,14 @@ def cpu_baseline(shot, steps):
                       f"(reference arithmetic), OpenMP over x rows, {dt:.1f} s"}
 
 
-def batched_leg(nt):
+def shape_dict(kind, a, b, us):
+    d = {"kernel": "tiled two-launch step", "tile_rows": a, "waves": b} if kind == 0 else \
+        {"kernel": "fused march kernel", "waves": a, "chunk_rows": b}
+    d["autotuned_us_per_step"] = None if us is None else round(us, 2)
+    return d
+
+
+def batched_leg(nt, no_tune=False):
     """64 shots on uneven aperture windows of a 1001 x 6001 model (C-PML 50,
     nt=3000 per shot): one launch per step for all shots replayed from a
     hipGraph, against the plain path (shot after shot, one launch per shot
@@ -83,6 +95,8 @@ def batched_leg(nt):
     _, shots = W.windowed_shots(nshot=64, nt=nt)
     cells = sum(s.vp.size for s in shots)
     b = sfd.AcousticBatch(shots)
+    tuned_us = None if no_tune else b.autotune(20)
+    kind, ta, tb = b.shape()
     b.run(sfd.BATCH_GRAPH)
     g_ms = min(b.run(sfd.BATCH_GRAPH) for _ in range(2))
     p_ms = b.run(sfd.BATCH_PER_SHOT)
@@ -95,7 +109,8 @@ def batched_leg(nt):
             "graph_shots_per_s": round(64 / (g_ms * scale / 1e3), 3),
             "plain_shots_per_s": round(64 / (p_ms * scale / 1e3), 3),
             "speedup": round(p_ms / g_ms, 2),
-            "graph_frac_of_8TBs": round(cells * nt * 32.0 / (g_ms / 1e3) / 8e12, 4)}
+            "graph_frac_of_8TBs": round(cells * nt * 32.0 / (g_ms / 1e3) / 8e12, 4),
+            "launch_shape": shape_dict(kind, ta, tb, tuned_us)}
 
 
 def visco_leg(nt_err, nt, no_cpu):
@@ -302,6 +317,8 @@ def main():
     ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic"], default=None,
                     help="run one secondary leg alone (profiling) and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (the library's default launch shapes)")
     ap.add_argument("--grad-steps", type=int, default=2500)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a known-size kernel that scales "
@@ -328,7 +345,7 @@ def main():
     if args.only:
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
                "grad": lambda: gradient_leg(args.grad_steps),
-               "batch": lambda: batched_leg(args.batch_steps),
+               "batch": lambda: batched_leg(args.batch_steps, args.no_tune),
                "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)}[args.only]
         print(json.dumps(leg()), flush=True)
         return
@@ -339,8 +356,13 @@ def main():
     log(f"workload: {nx}x{nz} cells, nt={nt}, R={shot.radius}, {shot.boundary}")
     ctx = sfd.Context(0)
     plan = sfd.AcousticPlan(shot, ctx)
+    tuned_us = None
     if args.waves or args.chunk:
         plan.set_tiling(args.waves, args.chunk)
+    elif not args.no_tune:
+        tuned_us = plan.autotune(20)
+    kind, ta, tb = plan.shape()
+    log(f"launch shape: {shape_dict(kind, ta, tb, tuned_us)}")
     for i in range(args.warmup):
         ms = plan.run()
         log(f"warmup shot {i}: {ms:.1f} ms")
@@ -386,12 +408,14 @@ def main():
         "shots_per_s": round(args.steps / wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "unit_of_measure": "one time step = velocity launch + pressure launch",
+                     "unit_of_measure": ("one time step = velocity launch + pressure launch" if kind == 0 else
+                                         "one time step = one launch of the fused march kernel"),
                      "bytes_per_step": bytes_step, "avg_step_us": round(avg_launch_s * 1e6, 2)},
+        "launch_shape": shape_dict(kind, ta, tb, tuned_us),
     }
     if not args.no_batch:
         log("batched leg ...")
-        out["batched"] = batched_leg(args.batch_steps)
+        out["batched"] = batched_leg(args.batch_steps, args.no_tune)
     if not args.no_visco:
         log("visco leg ...")
         out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)

@@ -228,6 +228,25 @@ int sfd_acoustic_misfit_gradient(sfd_context *ctx, int nshot, const sfd_acoustic
                                  const float *const *d_obs, int ckpt_every, float *grad_vp,
                                  double *misfit);
 
+/* How the reverse sweep of sfd_acoustic_misfit_gradient_ex obtains the
+ * forward wavefield. */
+enum sfd_gradient_method {
+    /* recompute each checkpoint segment forward, keeping its ckpt_every
+       divergence fields (what sfd_acoustic_misfit_gradient does) */
+    SFD_GRADIENT_RECOMPUTE = 0,
+    /* boundary saving: the forward keeps the R-deep band just outside the box
+       [max(w, R), n - max(w, R)) of both axes (w = sponge width) after every
+       step; the reverse sweep runs the step backwards inside the box from the
+       band and the checkpoints (reset at every segment end), with one
+       divergence field per shot.  The sponge cannot be run backwards, so
+       grad_vp is zero outside the box (the source cell's term aside). */
+    SFD_GRADIENT_BOUNDARY_SAVING = 1,
+};
+int sfd_acoustic_misfit_gradient_ex(sfd_context *ctx, int nshot, const sfd_acoustic_params *params,
+                                    const float *vp, const float *den, const float *const *wavelet,
+                                    const float *const *d_obs, int ckpt_every, int method,
+                                    float *grad_vp, double *misfit);
+
 /* Aperture window of one shot, the drop-in for get_currshot_parameters
  * (forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:315-373;
  * the same routine in yang_version_2016_2_2/Iso_multishot_fd_subroutine_fsbc_zy_m2_pub.f90:216-266).

@@ -138,6 +138,10 @@ _SIGNATURES = {
     "sfd_acoustic_misfit_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(AcousticParamsC),
                                                     _FP, _FP, ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.c_int,
                                                     _FP, ctypes.POINTER(ctypes.c_double)]),
+    "sfd_acoustic_misfit_gradient_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                                       ctypes.POINTER(AcousticParamsC), _FP, _FP,
+                                                       ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.c_int,
+                                                       ctypes.c_int, _FP, ctypes.POINTER(ctypes.c_double)]),
     "sfd_visco_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViscoParamsC), _FP, _FP, _FP, _FP, _FP,
                                          _FP, _FP]),
     "sfd_visco_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViscoParamsC), _FP, _FP, _FP, _FP, _FP,
@@ -415,10 +419,17 @@ def acoustic_gradient(shot: AcousticShot, seed, ckpt_every: int = 50, ctx: Optio
     return grad, record
 
 
-def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: Optional[Context] = None):
+GRADIENT_METHODS = {"recompute": 0, "boundary": 1}
+
+
+def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: Optional[Context] = None,
+                    method: str = "recompute"):
     """J = sum_s 1/2 |record_s - d_obs[s]|^2 over shots sharing one model and
-    its dJ/dvp, through ``sfd_acoustic_misfit_gradient``.  The model is
-    ``vp``/``den`` (default: the first shot's).  Returns (J, grad [nx, nz])."""
+    its dJ/dvp, through ``sfd_acoustic_misfit_gradient_ex``.  The model is
+    ``vp``/``den`` (default: the first shot's).  method "recompute"
+    (checkpoints + segment recomputation) or "boundary" (checkpoints +
+    boundary saving; the gradient inside the sponge-free box only).
+    Returns (J, grad [nx, nz])."""
     ctx = ctx or default_context()
     shots = list(shots)
     n = len(shots)
@@ -438,8 +449,9 @@ def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: 
         ws[i], ds[i] = _ptr(w), _ptr(d)
     grad = np.zeros(vp.shape, dtype=np.float32)
     J = ctypes.c_double()
-    ctx.check(lib().sfd_acoustic_misfit_gradient(ctx.handle, n, params, _ptr(vp), _ptr(den), ws, ds, ckpt_every,
-                                                 _ptr(grad), ctypes.byref(J)), "sfd_acoustic_misfit_gradient")
+    ctx.check(lib().sfd_acoustic_misfit_gradient_ex(ctx.handle, n, params, _ptr(vp), _ptr(den), ws, ds, ckpt_every,
+                                                    GRADIENT_METHODS[method], _ptr(grad), ctypes.byref(J)),
+              "sfd_acoustic_misfit_gradient_ex")
     return J.value, grad
 
 

@@ -268,13 +268,16 @@ bool tiled_batch_mode(int mode, BatchKernel* v, BatchKernel* p) {
 // at the default shape.
 template <int R>
 bool pick_tiled_batch_r(int tx, int nw, int mode, BatchKernel* v, BatchKernel* p) {
-    if (tx == kTiledTX && nw == kTiledNW && (mode == MODE_FWD_DIV || mode == MODE_ADJ)) {
+    if (tx == kTiledTX && nw == kTiledNW && (mode == MODE_FWD_DIV || mode == MODE_ADJ || mode == MODE_REV)) {
         if (mode == MODE_FWD_DIV) {
             *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
             *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
-        } else {
+        } else if (mode == MODE_ADJ) {
             *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_ADJ>;
             *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_ADJ>;
+        } else {
+            *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_REV>;
+            *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_REV>;
         }
         return true;
     }
@@ -389,9 +392,16 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* P, const Fi
     a.psi_vz_out = pl->psi_vz[o].as<float>();
     a.div = pl->grid_period_div();
     a.div_stride = pl->div_stride();
-    a.div_period = pl->grad_period > 0 ? pl->grad_period : 1;
+    a.div_period = pl->div_fields > 0 ? pl->div_fields : 1;
     a.grad = pl->grad_period > 0 ? pl->grid(pl->grad) : nullptr;
     a.src_adj = pl->src_adj.as<float>();
+    // Box of the reverse reconstruction: the cells the sponge leaves alone
+    // and the updated region contains.
+    const int bx = std::max(pl->width, pl->radius);
+    a.rev_x0 = bx;
+    a.rev_x1 = pl->nx - bx;
+    a.rev_z0 = bx;
+    a.rev_z1 = pl->nz - bx;
     return a;
 }
 
@@ -542,7 +552,7 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
 }  // extern "C"
 
 int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
-                     const float* wavelet, int grad_period, sfd_acoustic_plan** out) {
+                     const float* wavelet, int grad_period, sfd_acoustic_plan** out, int div_fields) {
     if (!ctx || !out) return SFD_EINVAL;
     *out = nullptr;
     int rc = validate(ctx, p, vp, wavelet);
@@ -553,6 +563,7 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
     if (!pl) return fail(ctx, SFD_ENOMEM, "host allocation failed");
     pl->ctx = ctx;
     pl->grad_period = grad_period;
+    pl->div_fields = grad_period > 0 ? (div_fields > 0 ? div_fields : grad_period) : 0;
     pl->nx = p->nx;
     pl->nz = p->nz;
     pl->nt = p->nt;
@@ -662,7 +673,7 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
                 pl->avz[b] = cv.take(npad * 4);
             }
             pl->grad = cv.take(npad * 4);
-            pl->div = cv.take(npad * 4 * (size_t)grad_period);
+            pl->div = cv.take(npad * 4 * (size_t)pl->div_fields);
         }
         if (pass == 0) {
             if (cv.used >= (size_t)1 << 31)

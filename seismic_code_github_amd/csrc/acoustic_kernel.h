@@ -51,6 +51,12 @@ enum KernelMode : int {
     // accumulates grad += P_in * div_k and keeps the adjoint at the source
     // cell.
     MODE_ADJ = 5,
+    // Tiled step only.  The fast sponge step run backwards on the cells of
+    // the box [rev_x0, rev_x1) x [rev_z0, rev_z1), where the sponge factor is
+    // 1: the pressure launch first (P_k = P_k+1 + kf div_k, storing div_k),
+    // then the velocity launch (V_k = V_k+1 + bf D(P_k - source)).  Cells
+    // outside the box are neither read as outputs nor written.
+    MODE_REV = 6,
 };
 
 struct AcousticStepArgs {
@@ -124,6 +130,8 @@ struct AcousticStepArgs {
     int div_period;
     float* grad;
     float* src_adj;
+    // MODE_REV: the box it reconstructs.
+    int rev_x0, rev_x1, rev_z0, rev_z1;
 };
 
 template <int R, int MODE>

@@ -61,8 +61,14 @@ struct sfd_acoustic_plan {
     // segment live in the arena; checkpoints, residuals and injection weights
     // in their own buffers.
     int grad_period = 0;  // checkpoint interval; 0 = no gradient workspace
+    int div_fields = 0;   // divergence fields kept (grad_period, or 1 with boundary saving)
     Field ap[2], avx[2], avz[2], grad, div;
     DeviceBuffer ckpt, res, inj_w, inj_cell, src_adj;
+    // Boundary saving: offsets (from row 0) of the band cells just outside
+    // the reconstructed box and their P, Vx, Vz after every forward step,
+    // [nt][3][nband].
+    int nband = 0;
+    DeviceBuffer band_cell, bands;
 
     // Row 0 of a padded per-row allocation.
     float* grid(const Field& b) const { return b.as<float>() + (size_t)pad * nz; }
@@ -73,9 +79,10 @@ struct sfd_acoustic_plan {
 
 namespace sfd {
 
-// Creates a plan; grad_period > 0 also reserves the gradient workspace.
+// Creates a plan; grad_period > 0 also reserves the gradient workspace with
+// div_fields divergence fields (0 = grad_period of them).
 int plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
-                const float* wavelet, int grad_period, sfd_acoustic_plan** out);
+                const float* wavelet, int grad_period, sfd_acoustic_plan** out, int div_fields = 0);
 // Step arguments over the state fields (p, vx, vz)[cur] -> [cur ^ 1].
 AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* p, const Field* vx, const Field* vz, int cur,
                            int k);

@@ -80,7 +80,7 @@ __device__ __forceinline__ float tiled_sponge(int i, int nx, int sw, float ax_i,
 // k the step.
 template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, float* s_p) {
-    constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ;
+    constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ, REV = MODE == MODE_REV;
     constexpr int W = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
@@ -98,10 +98,10 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
         vx0[q] = ok ? a.vx_in[o] : 0.f;
         vz0[q] = ok ? a.vz_in[o] : 0.f;
         bf[q] = ok ? a.bf[o] : 0.f;
-        if constexpr (!CPML) axr[q] = ok ? a.ax[i] : 1.f;
+        if constexpr (!CPML && !REV) axr[q] = ok ? a.ax[i] : 1.f;
     }
     float az_j = 1.f;
-    if constexpr (!CPML) az_j = jok ? a.az[j] : 1.f;
+    if constexpr (!CPML && !REV) az_j = jok ? a.az[j] : 1.f;
     const float amp = ADJ ? 0.f : a.amp[k];
     tiled_stage<TX, NW, R, R, !ADJ>(s_p, a.p_in, nz, i0, j0, a.sx, a.sz, amp);
     __syncthreads();
@@ -149,19 +149,28 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
                 a.psi_pz_out[e] = psi;
                 dz = sz + psi;
             }
-        } else if (raw) {
-            const float f = tiled_sponge(i, nx, sw, axr[q], az_j);
-            v0x *= f;
-            v0z *= f;
         }
-        a.vx_out[o] = __builtin_fmaf(-bf[q], dx, v0x);
-        a.vz_out[o] = __builtin_fmaf(-bf[q], dz, v0z);
+        if constexpr (REV) {
+            // Inverse of the forward update (sponge factor 1 in the box).
+            if (i < a.rev_x0 || i >= a.rev_x1 || j < a.rev_z0 || j >= a.rev_z1) continue;
+            a.vx_out[o] = __builtin_fmaf(bf[q], dx, v0x);
+            a.vz_out[o] = __builtin_fmaf(bf[q], dz, v0z);
+        } else {
+            if (!CPML && raw) {
+                const float f = tiled_sponge(i, nx, sw, axr[q], az_j);
+                v0x *= f;
+                v0z *= f;
+            }
+            a.vx_out[o] = __builtin_fmaf(-bf[q], dx, v0x);
+            a.vz_out[o] = __builtin_fmaf(-bf[q], dz, v0z);
+        }
     }
 }
 
 template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, float* s_vx, float* s_vz) {
     constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ, DIVOUT = MODE == MODE_FWD_DIV;
+    constexpr bool REV = MODE == MODE_REV;
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
@@ -171,7 +180,7 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
     const float amp = ADJ ? 0.f : a.amp[k];
     // Divergences of step k (gradient modes).
     float* dv = nullptr;
-    if constexpr (ADJ || DIVOUT) dv = a.div + (size_t)(k % a.div_period) * a.div_stride;
+    if constexpr (ADJ || DIVOUT || REV) dv = a.div + (size_t)(k % a.div_period) * a.div_stride;
     // Own-cell inputs and the rows' receiver spans, before the tile.
     float p0[RPW], kf[RPW], axr[RPW], dk[RPW], gr[RPW];
     int4 span[RPW];
@@ -182,16 +191,16 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
         const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         p0[q] = ok ? a.p_in[o] : 0.f;
         kf[q] = ok ? a.kf[o] : 0.f;
-        if constexpr (!CPML) axr[q] = ok ? a.ax[i] : 1.f;
+        if constexpr (!CPML && !REV) axr[q] = ok ? a.ax[i] : 1.f;
         if constexpr (ADJ) {
             dk[q] = ok ? dv[o] : 0.f;
             gr[q] = ok ? a.grad[o] : 0.f;
-        } else {
+        } else if constexpr (!REV) {
             span[q] = a.rec_row[min(i, nx - 1)];
         }
     }
     float az_j = 1.f;
-    if constexpr (!CPML) az_j = jok ? a.az[j] : 1.f;
+    if constexpr (!CPML && !REV) az_j = jok ? a.az[j] : 1.f;
     tiled_stage<TX, NW, R, 0, false>(s_vx, a.vx_out, nz, i0, j0, 0, 0, 0.f);
     tiled_stage<TX, NW, 0, R, false>(s_vz, a.vz_out, nz, i0, j0, 0, 0, 0.f);
     __syncthreads();
@@ -238,24 +247,34 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
             }
         }
         const ptrdiff_t o = (ptrdiff_t)i * nz + j;
-        float p = p0[q];
-        if (!ADJ && i == a.sx && j == a.sz) p -= amp;
-        const float pc = __builtin_fmaf(-kf[q], sx + sz, p);
-        if constexpr (DIVOUT) dv[o] = sx + sz;
-        if constexpr (ADJ) {
-            a.grad[o] = __builtin_fmaf(p0[q], dk[q], gr[q]);
-            if (i == a.sx && j == a.sz) a.src_adj[k] = pc;
-        }
-        const float pn = CPML ? pc : pc * tiled_sponge(i, nx, sw, axr[q], az_j);
-        a.p_out[o] = pn;
-        if constexpr (!ADJ) {
-            // Receivers of row i: the first span, then further spans (rare).
-            const int4 sp = span[q];
-            if (j >= sp.x && j < sp.y) gk[sp.z + (j - sp.x)] = pn;
-            for (int more = sp.w; more >= 0;) {
-                const int4 mp = a.rec_more[more];
-                if (j >= mp.x && j < mp.y) gk[mp.z + (j - mp.x)] = pn;
-                more = mp.w;
+        if constexpr (REV) {
+            // Inverse of the forward update (sponge factor 1 in the box).
+            if (i < a.rev_x0 || i >= a.rev_x1 || j < a.rev_z0 || j >= a.rev_z1) continue;
+            const float s = sx + sz;
+            float p = __builtin_fmaf(kf[q], s, p0[q]);
+            if (i == a.sx && j == a.sz) p += amp;
+            dv[o] = s;
+            a.p_out[o] = p;
+        } else {
+            float p = p0[q];
+            if (!ADJ && i == a.sx && j == a.sz) p -= amp;
+            const float pc = __builtin_fmaf(-kf[q], sx + sz, p);
+            if constexpr (DIVOUT) dv[o] = sx + sz;
+            if constexpr (ADJ) {
+                a.grad[o] = __builtin_fmaf(p0[q], dk[q], gr[q]);
+                if (i == a.sx && j == a.sz) a.src_adj[k] = pc;
+            }
+            const float pn = CPML ? pc : pc * tiled_sponge(i, nx, sw, axr[q], az_j);
+            a.p_out[o] = pn;
+            if constexpr (!ADJ) {
+                // Receivers of row i: the first span, then further spans (rare).
+                const int4 sp = span[q];
+                if (j >= sp.x && j < sp.y) gk[sp.z + (j - sp.x)] = pn;
+                for (int more = sp.w; more >= 0;) {
+                    const int4 mp = a.rec_more[more];
+                    if (j >= mp.x && j < mp.y) gk[mp.z + (j - mp.x)] = pn;
+                    more = mp.w;
+                }
             }
         }
     }

@@ -17,6 +17,17 @@
 // Every step is the two-launch tiled step (acoustic_tiled.h) in place on the
 // first buffer of each field; the velocities, checkpoints included, keep the
 // tiled sponge step's unabsorbed values.
+//
+// Boundary saving (SFD_GRADIENT_BOUNDARY_SAVING, shot groups): instead of
+// recomputing each segment forward and keeping its divergences, the forward
+// saves after every step the P, Vx, Vz values of the R-deep band just outside
+// the box [max(w, R), n - max(w, R)) on both axes (w = sponge width), where
+// the sponge factor is 1.  The backward sweep then runs the forward step in
+// reverse inside the box (MODE_REV: P_k = P_k+1 + kf div_k, V_k = V_k+1 +
+// bf D(P_k - source), the band restored from the saves before each step, the
+// state reset from the checkpoint at every segment end), so one divergence
+// field per shot suffices.  The sponge is dissipative and cannot be run
+// backwards, so the gradient exists inside the box only (zero in the sponge).
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -56,11 +67,12 @@ int check_gradient_params(sfd_context* ctx, const sfd_acoustic_params* p) {
 
 // Creates the plan with its gradient workspace and the receiver-cell tables.
 int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
-               const float* wavelet, int period, GradShot* g) {
+               const float* wavelet, int period, GradShot* g, int method = SFD_GRADIENT_RECOMPUTE) {
     sfd_acoustic_params q = *p;
     q.nsnap = 0;
     q.snap_steps = nullptr;
-    int rc = plan_create(ctx, &q, vp, den, wavelet, period, &g->pl);
+    const bool bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
+    int rc = plan_create(ctx, &q, vp, den, wavelet, period, &g->pl, bsave ? 1 : 0);
     if (rc) return rc;
     sfd_acoustic_plan* pl = g->pl;
     // Forward, recomputation and adjoint all run the tiled step at the
@@ -99,6 +111,22 @@ int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, 
     if (e == hipSuccess) e = pl->res.alloc((size_t)std::max(pl->nslot, 1) * pl->nt * 4);
     if (e == hipSuccess) e = pl->src_adj.alloc((size_t)pl->nt * 4);
     if (e == hipSuccess) e = pl->ckpt.alloc((size_t)std::max(g->nseg - 1, 1) * 3 * n * 4);
+    if (e == hipSuccess && bsave) {
+        // Band cells: within R of the box along x (same columns) or along z
+        // (same rows), the reach of the stencils of the box's cells.
+        const int b = std::max(pl->width, pl->radius), R = pl->radius;
+        const int x0 = b, x1 = pl->nx - b, z0 = b, z1 = pl->nz - b;
+        if (x0 >= x1 || z0 >= z1) return fail(ctx, SFD_EINVAL, "grid too small for boundary saving");
+        std::vector<int> band;
+        for (int i = x0 - R; i < x1 + R; ++i)
+            for (int j = z0 - R; j < z1 + R; ++j) {
+                const bool xin = i >= x0 && i < x1, zin = j >= z0 && j < z1;
+                if (xin != zin) band.push_back(i * pl->nz + j);
+            }
+        pl->nband = (int)band.size();
+        e = upload(pl->band_cell, band.data(), band.size(), s);
+        if (e == hipSuccess) e = pl->bands.alloc((size_t)pl->nt * 3 * band.size() * 4);
+    }
     if (e != hipSuccess) return hip_check(ctx, e, "gradient workspace");
     return SFD_OK;
 }
@@ -268,14 +296,49 @@ __global__ void adjoint_inject_group_kernel(float* const* __restrict__ p, const 
     p[t.shot][t.cell] += t.w * res[t.shot][(size_t)k * nslot[t.shot] + t.slot];
 }
 
+// Boundary saving: one band cell of one shot.
+struct BandEntry {
+    int shot, cell, idx;  // cell: offset from row 0 of the shot's fields; idx: its band slot
+};
+
+// The shot's band of step k: band[k][field][idx] = field[cell], fields P, Vx, Vz.
+__global__ void band_save_kernel(const BandEntry* __restrict__ e, int n, float* const* __restrict__ fields,
+                                 float* const* __restrict__ bands, const int* __restrict__ nband, int k) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const BandEntry t = e[q];
+    const size_t nb = (size_t)nband[t.shot];
+    float* b = bands[t.shot] + (size_t)k * 3 * nb + t.idx;
+    for (int f = 0; f < 3; ++f) b[f * nb] = fields[3 * t.shot + f][t.cell];
+}
+
+// Before the reverse step k: the velocities of the band from save kv (the
+// state after step k) and P from save kp (the state before step k; kp < 0 =
+// the zero initial state).
+__global__ void band_restore_kernel(const BandEntry* __restrict__ e, int n, float* const* __restrict__ fields,
+                                    float* const* __restrict__ bands, const int* __restrict__ nband, int kv,
+                                    int kp) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const BandEntry t = e[q];
+    const size_t nb = (size_t)nband[t.shot];
+    const float* bv = bands[t.shot] + (size_t)kv * 3 * nb + t.idx;
+    fields[3 * t.shot][t.cell] = kp < 0 ? 0.f : bands[t.shot][(size_t)kp * 3 * nb + t.idx];
+    fields[3 * t.shot + 1][t.cell] = bv[nb];
+    fields[3 * t.shot + 2][t.cell] = bv[2 * nb];
+}
+
 struct GradGroup {
     sfd_context* ctx = nullptr;
     std::vector<GradShot> shots;
-    int nt = 0, period = 0, nseg = 0, ntile = 0, ninj = 0;
-    // Tiled step kernels (velocity, pressure) of the forward, the
-    // recomputation and the adjoint; all steps run in place on buffer 0.
-    BatchKernel kfwd[2] = {}, kdiv[2] = {}, kadj[2] = {};
+    int nt = 0, period = 0, nseg = 0, ntile = 0, ninj = 0, nbandall = 0;
+    bool bsave = false;
+    // Tiled step kernels in launch order: (velocity, pressure) of the
+    // forward, the recomputation and the adjoint, (pressure, velocity) of
+    // the reverse step; all steps run in place on buffer 0.
+    BatchKernel kfwd[2] = {}, kdiv[2] = {}, kadj[2] = {}, krev[2] = {};
     DeviceBuffer fwd_args, adj_args, tile_map, inj, inj_p, inj_res, inj_nslot;
+    DeviceBuffer band_e, band_fields, band_ptr, band_n;
     ~GradGroup() {
         for (auto& g : shots)
             if (g.pl) sfd_acoustic_plan_destroy(g.pl);
@@ -344,7 +407,42 @@ int group_tables(GradGroup* G) {
     if (e == hipSuccess) e = upload(G->inj_res, rp.data(), rp.size(), st);
     if (e == hipSuccess) e = upload(G->inj_nslot, ns.data(), ns.size(), st);
     if (e == hipSuccess && !inj.empty()) e = upload(G->inj, inj.data(), inj.size(), st);
+    if (e == hipSuccess && G->bsave) {
+        std::vector<BandEntry> be;
+        std::vector<float*> fp(3 * (size_t)n), bp(n);
+        std::vector<int> nb(n);
+        for (int s = 0; s < n; ++s) {
+            sfd_acoustic_plan* pl = G->shots[s].pl;
+            std::vector<int> cell(pl->nband);
+            e = hipMemcpy(cell.data(), pl->band_cell.ptr, cell.size() * sizeof(int), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) break;
+            for (int q = 0; q < pl->nband; ++q) be.push_back({s, cell[q], q});
+            fp[3 * s] = pl->grid(pl->p[0]);
+            fp[3 * s + 1] = pl->grid(pl->vx[0]);
+            fp[3 * s + 2] = pl->grid(pl->vz[0]);
+            bp[s] = pl->bands.as<float>();
+            nb[s] = pl->nband;
+        }
+        G->nbandall = (int)be.size();
+        if (e == hipSuccess) e = upload(G->band_e, be.data(), be.size(), st);
+        if (e == hipSuccess) e = upload(G->band_fields, fp.data(), fp.size(), st);
+        if (e == hipSuccess) e = upload(G->band_ptr, bp.data(), bp.size(), st);
+        if (e == hipSuccess) e = upload(G->band_n, nb.data(), nb.size(), st);
+    }
     if (e != hipSuccess) return hip_check(ctx, e, "gradient group tables");
+    return SFD_OK;
+}
+
+int group_band(GradGroup* G, bool save, int k) {
+    const int bs = 256, nb = (G->nbandall + bs - 1) / bs;
+    if (save)
+        hipLaunchKernelGGL(band_save_kernel, dim3(nb), dim3(bs), 0, G->ctx->stream, G->band_e.as<BandEntry>(),
+                           G->nbandall, G->band_fields.as<float*>(), G->band_ptr.as<float*>(), G->band_n.as<int>(), k);
+    else
+        hipLaunchKernelGGL(band_restore_kernel, dim3(nb), dim3(bs), 0, G->ctx->stream, G->band_e.as<BandEntry>(),
+                           G->nbandall, G->band_fields.as<float*>(), G->band_ptr.as<float*>(), G->band_n.as<int>(), k,
+                           k - 1);
+    SFD_HIP(G->ctx, hipGetLastError());
     return SFD_OK;
 }
 
@@ -387,24 +485,27 @@ int group_checkpoint(GradGroup* G, int seg, bool save) {
 // Misfit gradient of up to G->shots.size() shots on one model; adds into
 // grad_vp and *misfit.
 int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp, const float* den,
-              const float* const* wavelet, const float* const* d_obs, int period, float* grad_vp, double* misfit) {
+              const float* const* wavelet, const float* const* d_obs, int period, int method, float* grad_vp,
+              double* misfit) {
     GradGroup G;
     G.ctx = ctx;
     G.nt = params[0].nt;
     G.period = std::min(period, G.nt);
+    G.bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
     G.shots.resize(nshot);
     for (int s = 0; s < nshot; ++s) {
         int rc = check_gradient_params(ctx, &params[s]);
         if (!rc && params[s].nt != G.nt) rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same nt", s);
         if (!rc && params[s].radius != params[0].radius)
             rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same radius", s);
-        if (!rc) rc = grad_setup(ctx, &params[s], vp, den, wavelet[s], G.period, &G.shots[s]);
+        if (!rc) rc = grad_setup(ctx, &params[s], vp, den, wavelet[s], G.period, &G.shots[s], method);
         if (rc) return rc;
     }
     const int R = params[0].radius;
     if (!pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_FAST_SPONGE, &G.kfwd[0], &G.kfwd[1]) ||
         !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_FWD_DIV, &G.kdiv[0], &G.kdiv[1]) ||
-        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_ADJ, &G.kadj[0], &G.kadj[1]))
+        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_ADJ, &G.kadj[0], &G.kadj[1]) ||
+        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_REV, &G.krev[1], &G.krev[0]))
         return fail(ctx, SFD_ENOTSUP, "no gradient kernel for radius %d", R);
     G.nseg = (G.nt + G.period - 1) / G.period;
     int rc = group_tables(&G);
@@ -422,6 +523,7 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
             if (rc) return rc;
         }
         rc = group_launch(&G, G.kfwd, G.fwd_args, k);
+        if (!rc && G.bsave) rc = group_band(&G, true, k);
         if (rc) return rc;
     }
     for (auto& g : G.shots) g.pl->v_raw = true;
@@ -446,10 +548,27 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         for (Field* f : {&pl->ap[0], &pl->avx[0], &pl->avz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
         SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
         SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
+        if (G.bsave) SFD_HIP(ctx, hipMemsetAsync(pl->div.ptr, 0, pl->div.bytes, s));
     }
     rc = group_inject(&G, G.nt - 1);
     if (rc) return rc;
-    for (int seg = G.nseg - 1; seg >= 0; --seg) {
+    if (G.bsave) {
+        // One sweep from the final state: each step reconstructs the forward
+        // state before it inside the box (resetting to the checkpoint at a
+        // segment end), then runs the adjoint step with its divergence.
+        for (int k = G.nt - 1; k >= 0; --k) {
+            if ((k + 1) % G.period == 0 && k + 1 < G.nt) {
+                rc = group_checkpoint(&G, (k + 1) / G.period, false);
+                if (rc) return rc;
+            }
+            rc = group_band(&G, false, k);
+            if (!rc) rc = group_launch(&G, G.krev, G.fwd_args, k);
+            if (!rc) rc = group_launch(&G, G.kadj, G.adj_args, k);
+            if (!rc && k >= 1) rc = group_inject(&G, k - 1);
+            if (rc) return rc;
+        }
+    }
+    for (int seg = G.bsave ? -1 : G.nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * G.period, k1 = std::min(k0 + G.period, G.nt);
         if (seg == 0) {
             for (auto& g : G.shots)
@@ -495,23 +614,37 @@ int sfd_acoustic_gradient(sfd_context* ctx, const sfd_acoustic_params* p, const 
 int sfd_acoustic_misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp,
                                  const float* den, const float* const* wavelet, const float* const* d_obs,
                                  int ckpt_every, float* grad_vp, double* misfit) {
+    return sfd_acoustic_misfit_gradient_ex(ctx, nshot, params, vp, den, wavelet, d_obs, ckpt_every,
+                                           SFD_GRADIENT_RECOMPUTE, grad_vp, misfit);
+}
+
+int sfd_acoustic_misfit_gradient_ex(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp,
+                                    const float* den, const float* const* wavelet, const float* const* d_obs,
+                                    int ckpt_every, int method, float* grad_vp, double* misfit) {
     if (!ctx || nshot < 1 || !params || !vp || !wavelet || !d_obs || !grad_vp) return SFD_EINVAL;
+    if (method != SFD_GRADIENT_RECOMPUTE && method != SFD_GRADIENT_BOUNDARY_SAVING)
+        return fail(ctx, SFD_EINVAL, "unknown gradient method %d", method);
     for (int s = 1; s < nshot; ++s)
         if (params[s].nx != params[0].nx || params[s].nz != params[0].nz)
             return fail(ctx, SFD_EINVAL, "shot %d: every shot of a gradient shares the model grid", s);
     if (ckpt_every < 1) return fail(ctx, SFD_EINVAL, "checkpoint interval must be >= 1");
     std::fill(grad_vp, grad_vp + (size_t)params[0].nx * params[0].nz, 0.f);
     if (misfit) *misfit = 0.0;
-    // Groups sized so that their workspaces (state, adjoint state, one
-    // segment of divergences and the checkpoints of every shot) stay below
-    // 64 GiB of device memory.
+    // Groups sized so that their workspaces (state, adjoint state, the
+    // divergences of one segment or, with boundary saving, one divergence
+    // field and the band of every step, and the checkpoints of every shot)
+    // stay below 64 GiB of device memory.
     const double field = (double)params[0].nx * params[0].nz * 4.0;
     const int period = std::min(ckpt_every, std::max(params[0].nt, 1));
-    const double per_shot = field * (16.0 + period) * 1.2 + field * 3.0 * ((params[0].nt + period - 1) / period);
+    const bool bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
+    const double band = 4.0 * params[0].radius * (params[0].nx + params[0].nz) * 4.0 * 3.0 * params[0].nt;
+    const double per_shot = field * (16.0 + (bsave ? 1 : period)) * 1.2 +
+                            field * 3.0 * ((params[0].nt + period - 1) / period) + (bsave ? band : 0.0);
     const int gmax = (int)std::max(1.0, std::min((double)nshot, 64.0 * (1ull << 30) / per_shot));
     for (int s0 = 0; s0 < nshot; s0 += gmax) {
         const int n = std::min(gmax, nshot - s0);
-        const int rc = group_run(ctx, n, params + s0, vp, den, wavelet + s0, d_obs + s0, period, grad_vp, misfit);
+        const int rc = group_run(ctx, n, params + s0, vp, den, wavelet + s0, d_obs + s0, period, method, grad_vp,
+                                 misfit);
         if (rc) return rc;
     }
     return SFD_OK;
