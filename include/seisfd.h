@@ -394,6 +394,56 @@ int sfd_elastic_plan_fetch(sfd_elastic_plan *plan, float *record_vx, float *reco
 int sfd_elastic_plan_set_tiling(sfd_elastic_plan *plan, int tile_cols, int waves);
 int sfd_elastic_plan_destroy(sfd_elastic_plan *plan);
 
+/* ---------------------------------------------------------------- files
+ * The Fortran drivers' files on either side of the propagators (host code;
+ * no device or context needed).  Parameter cards: a comment line before
+ * every value line, strings as whole lines (trimmed), numbers read
+ * list-directed or, where the reference writes '(F)' / '(I)', as Intel
+ * Fortran's F15.7 / I12 fields (first 15 / 12 characters; an F field
+ * without a decimal point is scaled by 1e-7).  err (may be NULL, 256 bytes)
+ * receives the reason on SFD_EINVAL. */
+typedef struct sfd_q_par {  /* Q_forward read_par, Q sub:121-234 */
+    char vp_fn[256], vs_fn[256], rho_fn[256], qfp_fn[256], qfs_fn[256];
+    char shot_fn1[256], shot_fn2[256], shot_fn3[256], shot_fn4[256];
+    int nshot;
+    float offset_min, offset_max, sx0, sz0, rz0, dsx;
+    int nx, nz;
+    float dx, dz;
+    int nt;
+    float dt, f0, ppxn, ppxp, ppzn, ppzp;
+    int fdo2, fdo1;
+} sfd_q_par;
+int sfd_read_q_par(const char *path, sfd_q_par *par, char *err);
+
+typedef struct sfd_iso_par {  /* yang_version read_par, pub:48-145 */
+    char mode[256];           /* "fsbc" or "abc" */
+    char vp_fn[256], vs_fn[256], rho_fn[256], topo_fn[256], snapx_fn[256], snapz_fn[256];
+    char shot_fn1[256], shot_fn2[256], shot_fn3[256], shot_fn4[256];
+    int nshot;
+    float ofsmin, ofsmax, sx0, sz0, rz0, dsx;
+    int nx, nz;
+    float dx, dz;
+    int nt;
+    float dt, dtsnap, f0, decay, ppxn, ppxp, ppzn, ppzp;
+    int rgau;
+} sfd_iso_par;
+int sfd_read_iso_par(const char *path, sfd_iso_par *par, char *err);
+
+/* One shot's gather as an SU file (write_currshot_disk, Q sub:1476-1546 and
+ * pub:1533-1590): per trace a 240-byte SU header (native byte order) then nt
+ * floats; record [ntrace][nt].  Header: fldr = shot_no, trid = 1, sx =
+ * int((src_col - 1) dx), ns = nt, dt = int(dt 1e6) us, per trace gx =
+ * int((first_col - 1 + i - 1) dx) (i = 1..ntrace), offset = gx - sx and,
+ * with with_cdp (Q_forward), cdp = nint((gx + sx) / dx) + 1; columns are
+ * 1-based as in the Fortran. */
+int sfd_write_su_shot(const char *path, const float *record, int ntrace, int nt, int shot_no, int src_col,
+                      int first_col, float dx, float dt, int with_cdp);
+/* merge_shot_files (Q sub:1550-1624, pub:1594-1684): the shot files, each
+ * exactly ntrace traces of nt samples, concatenated in order into out_path;
+ * remove_inputs deletes them afterwards (as the reference does). */
+int sfd_merge_su(const char *const *shot_files, int nshot, int ntrace, int nt, const char *out_path,
+                 int remove_inputs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,31 @@ class ElasticParamsC(ctypes.Structure):
     ]
 
 
+class QParC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_char * 256) for n in ("vp_fn", "vs_fn", "rho_fn", "qfp_fn", "qfs_fn",
+                                                   "shot_fn1", "shot_fn2", "shot_fn3", "shot_fn4")] + [
+        ("nshot", ctypes.c_int),
+        *[(n, ctypes.c_float) for n in ("offset_min", "offset_max", "sx0", "sz0", "rz0", "dsx")],
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float),
+        ("nt", ctypes.c_int),
+        *[(n, ctypes.c_float) for n in ("dt", "f0", "ppxn", "ppxp", "ppzn", "ppzp")],
+        ("fdo2", ctypes.c_int), ("fdo1", ctypes.c_int)]
+
+
+class IsoParC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_char * 256) for n in ("mode", "vp_fn", "vs_fn", "rho_fn", "topo_fn", "snapx_fn",
+                                                   "snapz_fn", "shot_fn1", "shot_fn2", "shot_fn3",
+                                                   "shot_fn4")] + [
+        ("nshot", ctypes.c_int),
+        *[(n, ctypes.c_float) for n in ("ofsmin", "ofsmax", "sx0", "sz0", "rz0", "dsx")],
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float),
+        ("nt", ctypes.c_int),
+        *[(n, ctypes.c_float) for n in ("dt", "dtsnap", "f0", "decay", "ppxn", "ppxp", "ppzn", "ppzp")],
+        ("rgau", ctypes.c_int)]
+
+
 # Every symbol declared in include/seisfd.h, with its ctypes signature.
 _SIGNATURES = {
     "sfd_abi_version": (ctypes.c_int, []),
@@ -127,6 +152,13 @@ _SIGNATURES = {
                                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "sfd_acoustic_batch_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "sfd_read_q_par": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(QParC), ctypes.c_char_p]),
+    "sfd_read_iso_par": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(IsoParC), ctypes.c_char_p]),
+    "sfd_write_su_shot": (ctypes.c_int, [ctypes.c_char_p, _FP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_int]),
+    "sfd_merge_su": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_char_p, ctypes.c_int]),
     "sfd_context_save_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "sfd_context_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "sfd_context_tuning_size": (ctypes.c_int, [ctypes.c_void_p]),
@@ -825,3 +857,50 @@ def elastic_forward(shot: ElasticShot, ctx: Optional[Context] = None):
                                         _ptr(rvx) if has else None, _ptr(rvz) if has else None,
                                         _ptr(snaps) if p.nsnap else None), "sfd_elastic_forward")
     return rvx, rvz, snaps
+
+
+# ---------------------------------------------------------------- files
+
+def _par_dict(st) -> dict:
+    out = {}
+    for name, _ in st._fields_:
+        v = getattr(st, name)
+        out[name] = v.decode() if isinstance(v, bytes) else v
+    return out
+
+
+def _read_par(fn: str, cls, path: str) -> dict:
+    par = cls()
+    err = ctypes.create_string_buffer(256)
+    rc = getattr(lib(), fn)(os.fsencode(path), ctypes.byref(par), err)
+    if rc != SFD_OK:
+        raise SeisfdError(rc, f"{fn}({path}): {err.value.decode()}")
+    return _par_dict(par)
+
+
+def read_q_par(path: str) -> dict:
+    """Q_forward's parameter card (read_par, Q sub:121-234) as a dict."""
+    return _read_par("sfd_read_q_par", QParC, path)
+
+
+def read_iso_par(path: str) -> dict:
+    """yang_version's parameter card (read_par, pub:48-145) as a dict."""
+    return _read_par("sfd_read_iso_par", IsoParC, path)
+
+
+def write_su_shot(path: str, record, shot_no: int, src_col: int, first_col: int, dx: float, dt: float,
+                  with_cdp: bool = False):
+    """One shot's gather [ntrace, nt] as an SU file (write_currshot_disk)."""
+    rec = _f32(record)
+    rc = lib().sfd_write_su_shot(os.fsencode(path), _ptr(rec), rec.shape[0], rec.shape[1], shot_no, src_col,
+                                 first_col, dx, dt, int(with_cdp))
+    if rc != SFD_OK:
+        raise SeisfdError(rc, f"sfd_write_su_shot({path})")
+
+
+def merge_su(shot_files, ntrace: int, nt: int, out_path: str, remove_inputs: bool = False):
+    """Concatenates per-shot SU files of ntrace traces each (merge_shot_files)."""
+    arr = (ctypes.c_char_p * len(shot_files))(*[os.fsencode(f) for f in shot_files])
+    rc = lib().sfd_merge_su(arr, len(shot_files), ntrace, nt, os.fsencode(out_path), int(remove_inputs))
+    if rc != SFD_OK:
+        raise SeisfdError(rc, f"sfd_merge_su({out_path})")
