@@ -33,10 +33,12 @@ struct sfd_visco_plan {
     sfd::Field s[2], vx[2], vz[2], r, b, ku, kr, xs, zs;
     sfd::DeviceBuffer dvx, dsx, dvz, dsz, g, amp, rec_row, rec_more, rec_buf, rec_slot, record, snaps;
     void (*kernel)(sfd::ViscoStepArgs) = nullptr;
-    // vs > 0 (visco-elastic): two launches per step on padded fields.
-    bool elastic = false;
+    // Two launches per step on padded fields (visco_el_kernel.h): always for
+    // vs > 0 (elastic), by default for vs = 0 (ve_on; an explicit march
+    // tiling switches the vs = 0 plan to the fused march kernel).
+    bool elastic = false, ve_on = false;
     sfd::ViscoElArgs ve{};
-    sfd::DeviceBuffer ve_arena, ve_xs, ve_zs, rec_zz, rec_xx;
+    sfd::DeviceBuffer ve_arena, ve_xs, ve_zs, ve_rx, ve_rz, rec_zz, rec_xx;
     int ve_pitch = 0, ve_rows = 0;
     void (*ve_vel)(sfd::ViscoElArgs) = nullptr;
     void (*ve_str)(sfd::ViscoElArgs) = nullptr;
@@ -96,9 +98,21 @@ bool ve_pick_r(int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
     }
 }
 
-bool ve_pick(int R, int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
-    if (R == 4) return ve_pick_r<4>(L, v, st);
-    if (R == 5) return ve_pick_r<5>(L, v, st);
+template <int R>
+bool va_pick_r(int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    *v = visco_ac_velocity_kernel<R, kVeTX, kVeNW>;
+    switch (L) {
+    case 1: *st = visco_ac_stress_kernel<R, 1, kVeTX, kVeNW>; return true;
+    case 2: *st = visco_ac_stress_kernel<R, 2, kVeTX, kVeNW>; return true;
+    case 3: *st = visco_ac_stress_kernel<R, 3, kVeTX, kVeNW>; return true;
+    default: return false;
+    }
+}
+
+// The two tiled launches of the visco-elastic (elastic) or visco-acoustic step.
+bool ve_pick(int R, int L, bool elastic, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    if (R == 4) return elastic ? ve_pick_r<4>(L, v, st) : va_pick_r<4>(L, v, st);
+    if (R == 5) return elastic ? ve_pick_r<5>(L, v, st) : va_pick_r<5>(L, v, st);
     return false;
 }
 
@@ -259,9 +273,107 @@ int visco_zero(sfd_visco_plan* pl) {
     return SFD_OK;
 }
 
+// Padded fields, strips, receiver tables and kernel arguments of the tiled
+// two-launch step: vs > 0 (elastic: vx, vz, sxx, szz, sxz, 3L memory fields;
+// model b, kup, kus, krp, krs) or vs = 0 (vx, vz, sigma, L memory fields;
+// model b, ku, kr).  The sponge tables (x tables with pl->pad rows of
+// padding), amp and the source footprint are on the device already.
+int ve_setup(sfd_visco_plan* pl, const sfd_visco_params* p, bool elastic, const std::vector<float>& hb,
+             const std::vector<float>& hku, const std::vector<float>& hkr, const std::vector<float>* hkus,
+             const std::vector<float>* hkrs) {
+    sfd_context* ctx = pl->ctx;
+    const int nx = pl->nx, nz = pl->nz, L = pl->nmech, R = pl->radius;
+    if (!ve_pick(R, L, elastic, &pl->ve_vel, &pl->ve_str)) return fail(ctx, SFD_ENOTSUP, "no tiled visco kernel");
+    pl->ve_pitch = (nz + kVeTZ - 1) / kVeTZ * kVeTZ + 2 * kVePad;
+    pl->ve_rows = nx + kVeTX + 2 * kVePad;
+    const size_t fe = (size_t)pl->ve_pitch * pl->ve_rows;  // floats per padded field
+    const int nstate = elastic ? 5 + 3 * L : 3 + L, nmodel = elastic ? 5 : 3, nparts = elastic ? 5 + 3 * L : 2 + L;
+    const int xw = pl->xlo + pl->xhi, zw = pl->zlo + pl->zhi;
+    hipError_t e = pl->ve_arena.alloc((size_t)(nstate + nmodel) * fe * sizeof(float));
+    if (e == hipSuccess) e = pl->ve_xs.alloc((size_t)nparts * xw * nz * sizeof(float));
+    if (e == hipSuccess) e = pl->ve_zs.alloc((size_t)nparts * nx * zw * sizeof(float));
+    if (e != hipSuccess) return hip_check(ctx, e, "tiled visco allocation");
+    float* base = pl->ve_arena.as<float>();
+    auto cell0 = [&](int q) { return base + (size_t)q * fe + (size_t)kVePad * pl->ve_pitch + kVePad; };
+    ViscoElArgs& a = pl->ve;
+    a = ViscoElArgs{};
+    a.nx = nx;
+    a.nz = nz;
+    a.pitch = pl->ve_pitch;
+    a.ntz = (nz + kVeTZ - 1) / kVeTZ;
+    a.ntiles = a.ntz * ((nx + kVeTX - 1) / kVeTX);
+    a.vx = cell0(0);
+    a.vz = cell0(1);
+    std::vector<const std::vector<float>*> model;
+    if (elastic) {
+        a.sxx = cell0(2);
+        a.szz = cell0(3);
+        a.sxz = cell0(4);
+        a.r = cell0(5);
+        a.kup = cell0(nstate + 1);
+        a.kus = cell0(nstate + 2);
+        a.krp = cell0(nstate + 3);
+        a.krs = cell0(nstate + 4);
+        model = {&hb, &hku, hkus, &hkr, hkrs};
+    } else {
+        a.sxx = a.szz = cell0(2);
+        a.r = cell0(3);
+        a.kup = cell0(nstate + 1);
+        a.krp = cell0(nstate + 2);
+        model = {&hb, &hku, &hkr};
+    }
+    a.b = cell0(nstate);
+    a.rstride = fe;
+    a.hx = pl->hx;
+    a.hz = pl->hz;
+    a.dt = pl->dt;
+    for (int m = 0; m < 5; ++m) a.a[m] = m < R ? pl->a[m] : 0.f;
+    for (int l = 0; l < kVeMaxMech; ++l) a.c[l] = l < L ? pl->c[l] : 0.f;
+    a.xlo = pl->xlo;
+    a.xhi = pl->xhi;
+    a.zlo = pl->zlo;
+    a.zhi = pl->zhi;
+    a.xs = pl->ve_xs.as<float>();
+    a.zs = pl->ve_zs.as<float>();
+    a.sx0 = pl->sx0;
+    a.sx1 = pl->sx1;
+    a.sz0 = pl->sz0;
+    a.sz1 = pl->sz1;
+    a.g = pl->g.as<float>();
+    a.nrec = p->nrec;
+    a.nrt = std::max(p->nt, 1);
+    hipStream_t sm = ctx->stream;
+    // Model fields into their padded layout.
+    std::vector<float> padded(fe, 0.f);
+    for (int q = 0; q < nmodel && e == hipSuccess; ++q) {
+        for (int i = 0; i < nx; ++i)
+            std::copy(model[q]->begin() + (size_t)i * nz, model[q]->begin() + (size_t)(i + 1) * nz,
+                      padded.begin() + (size_t)(i + kVePad) * pl->ve_pitch + kVePad);
+        e = hipMemcpyAsync(base + (size_t)(nstate + q) * fe, padded.data(), fe * sizeof(float),
+                           hipMemcpyHostToDevice, sm);
+        if (e == hipSuccess) e = hipStreamSynchronize(sm);
+    }
+    if (e == hipSuccess && p->nrec > 0) e = upload(pl->ve_rx, p->rx, p->nrec, sm);
+    if (e == hipSuccess && p->nrec > 0) e = upload(pl->ve_rz, p->rz, p->nrec, sm);
+    if (e == hipSuccess && p->nrec > 0) e = pl->rec_zz.alloc((size_t)p->nrec * a.nrt * 4);
+    if (e == hipSuccess && p->nrec > 0 && elastic) e = pl->rec_xx.alloc((size_t)p->nrec * a.nrt * 4);
+    if (e != hipSuccess) return hip_check(ctx, e, "tiled visco set-up");
+    a.dvx = pl->dvx.as<float>() + pl->pad;
+    a.dsx = pl->dsx.as<float>() + pl->pad;
+    a.dvz = pl->dvz.as<float>();
+    a.dsz = pl->dsz.as<float>();
+    a.amp = pl->amp.as<float>();
+    a.rx = pl->ve_rx.as<int>();
+    a.rz = pl->ve_rz.as<int>();
+    a.rec_zz = pl->rec_zz.as<float>();
+    a.rec_xx = elastic ? pl->rec_xx.as<float>() : nullptr;
+    pl->ve_on = true;
+    return SFD_OK;
+}
+
 int ve_zero(sfd_visco_plan* pl) {
     sfd_context* ctx = pl->ctx;
-    // State (vx, vz, three stresses, 3L memory fields) lies at the front of
+    // State (vx, vz, the stresses, the memory fields) lies at the front of
     // the arena, before the model fields.
     const size_t state_bytes = (size_t)((const char*)pl->ve.b - pl->ve_arena.as<char>()) -
                                ((size_t)kVePad * pl->ve_pitch + kVePad) * sizeof(float);
@@ -288,7 +400,7 @@ int ve_enqueue(sfd_visco_plan* pl) {
         SFD_HIP(ctx, hipGetLastError());
         while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
             float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
-            const float* src[6] = {a.szz, a.vx, a.vz, a.sxx, a.szz, a.sxz};
+            const float* src[6] = {a.szz, a.vx, a.vz, a.sxx, a.szz, a.sxz};  // sxz only with vs > 0
             for (int f = 0; f < 6; ++f) {
                 if (!(pl->snap_fields & (1 << f))) continue;
                 SFD_HIP(ctx, hipMemcpy2DAsync(dst, nb, src[f], pb, nb, pl->nx, hipMemcpyDeviceToDevice, ctx->stream));
@@ -349,7 +461,8 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     const int nx = p->nx, nz = p->nz, R = p->radius, L = p->nmech;
     const size_t n = (size_t)nx * nz;
     // Any vs > 0 selects the visco-elastic kernels; vs = 0 everywhere is the
-    // visco-acoustic limit (sxx = szz, sxz = 0), one fused kernel per step.
+    // visco-acoustic limit (sxx = szz, sxz = 0): two tiled launches per step,
+    // or the fused march kernel on request (sfd_visco_plan_set_tiling).
     bool elastic = false;
     if (vs) {
         for (size_t c = 0; c < n; ++c) {
@@ -549,85 +662,16 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
         pl->snap_fields = p->snap_fields & 63;
         pl->nsnap_fields = __builtin_popcount(pl->snap_fields);
         pl->snap_steps.assign(p->snap_steps, p->snap_steps + p->nsnap);
-        if (!ve_pick(R, L, &pl->ve_vel, &pl->ve_str)) return bail(fail(ctx, SFD_ENOTSUP, "no visco-elastic kernel"));
-        pl->ve_pitch = (nz + kVeTZ - 1) / kVeTZ * kVeTZ + 2 * kVePad;
-        pl->ve_rows = nx + kVeTX + 2 * kVePad;
-        const size_t fe = (size_t)pl->ve_pitch * pl->ve_rows;  // floats per padded field
-        hipError_t e2 = pl->ve_arena.alloc((5 + 3 * (size_t)L + 5) * fe * sizeof(float));
-        const int xw = pl->xlo + pl->xhi, zw = pl->zlo + pl->zhi;
-        if (e2 == hipSuccess) e2 = pl->ve_xs.alloc((size_t)(5 + 3 * L) * xw * nz * sizeof(float));
-        if (e2 == hipSuccess) e2 = pl->ve_zs.alloc((size_t)(5 + 3 * L) * nx * zw * sizeof(float));
-        if (e2 != hipSuccess) return bail(hip_check(ctx, e2, "visco-elastic allocation"));
-        float* base = pl->ve_arena.as<float>();
-        auto cell0 = [&](int q) { return base + (size_t)q * fe + (size_t)kVePad * pl->ve_pitch + kVePad; };
-        ViscoElArgs& a = pl->ve;
-        a.nx = nx;
-        a.nz = nz;
-        a.pitch = pl->ve_pitch;
-        a.ntz = (nz + kVeTZ - 1) / kVeTZ;
-        a.ntiles = a.ntz * ((nx + kVeTX - 1) / kVeTX);
-        a.vx = cell0(0);
-        a.vz = cell0(1);
-        a.sxx = cell0(2);
-        a.szz = cell0(3);
-        a.sxz = cell0(4);
-        a.r = cell0(5);
-        a.rstride = fe;
-        a.b = cell0(5 + 3 * L);
-        a.kup = cell0(6 + 3 * L);
-        a.kus = cell0(7 + 3 * L);
-        a.krp = cell0(8 + 3 * L);
-        a.krs = cell0(9 + 3 * L);
-        a.hx = pl->hx;
-        a.hz = pl->hz;
-        a.dt = p->dt;
-        for (int m = 0; m < 5; ++m) a.a[m] = m < R ? pl->a[m] : 0.f;
-        for (int l = 0; l < kVeMaxMech; ++l) a.c[l] = l < L ? pl->c[l] : 0.f;
-        a.xlo = pl->xlo;
-        a.xhi = pl->xhi;
-        a.zlo = pl->zlo;
-        a.zhi = pl->zhi;
-        a.xs = pl->ve_xs.as<float>();
-        a.zs = pl->ve_zs.as<float>();
-        a.sx0 = pl->sx0;
-        a.sx1 = pl->sx1;
-        a.sz0 = pl->sz0;
-        a.sz1 = pl->sz1;
-        a.g = pl->g.as<float>();
-        a.nrec = p->nrec;
-        a.nrt = std::max(p->nt, 1);
         hipStream_t sm = ctx->stream;
-        // Model fields into their padded layout.
-        std::vector<float> padded(fe, 0.f);
-        const std::vector<float>* model[5] = {&hb, &hku, &hkus, &hkr, &hkrs};
-        for (int q = 0; q < 5 && e2 == hipSuccess; ++q) {
-            for (int i = 0; i < nx; ++i)
-                std::copy(model[q]->begin() + (size_t)i * nz, model[q]->begin() + (size_t)(i + 1) * nz,
-                          padded.begin() + (size_t)(i + kVePad) * pl->ve_pitch + kVePad);
-            e2 = hipMemcpyAsync(base + (size_t)(5 + 3 * L + q) * fe, padded.data(), fe * sizeof(float),
-                                hipMemcpyHostToDevice, sm);
-            if (e2 == hipSuccess) e2 = hipStreamSynchronize(sm);
-        }
-        if (e2 == hipSuccess) e2 = upload(pl->dvx, hdvx.data() + pad, nx, sm);
-        if (e2 == hipSuccess) e2 = upload(pl->dsx, hdsx.data() + pad, nx, sm);
+        hipError_t e2 = upload(pl->dvx, hdvx.data(), hdvx.size(), sm);
+        if (e2 == hipSuccess) e2 = upload(pl->dsx, hdsx.data(), hdsx.size(), sm);
         if (e2 == hipSuccess) e2 = upload(pl->dvz, hdvz.data(), hdvz.size(), sm);
         if (e2 == hipSuccess) e2 = upload(pl->dsz, hdsz.data(), hdsz.size(), sm);
         if (e2 == hipSuccess) e2 = upload(pl->amp, hamp.data(), hamp.size(), sm);
-        if (e2 == hipSuccess && p->nrec > 0) e2 = upload(pl->rec_row, p->rx, p->nrec, sm);   // rx
-        if (e2 == hipSuccess && p->nrec > 0) e2 = upload(pl->rec_slot, p->rz, p->nrec, sm);  // rz
-        if (e2 == hipSuccess && p->nrec > 0) e2 = pl->rec_zz.alloc((size_t)p->nrec * a.nrt * 4);
-        if (e2 == hipSuccess && p->nrec > 0) e2 = pl->rec_xx.alloc((size_t)p->nrec * a.nrt * 4);
         if (e2 == hipSuccess && p->nsnap > 0) e2 = pl->snaps.alloc((size_t)p->nsnap * pl->nsnap_fields * n * 4);
         if (e2 != hipSuccess) return bail(hip_check(ctx, e2, "visco-elastic set-up"));
-        a.dvx = pl->dvx.as<float>();
-        a.dsx = pl->dsx.as<float>();
-        a.dvz = pl->dvz.as<float>();
-        a.dsz = pl->dsz.as<float>();
-        a.amp = pl->amp.as<float>();
-        a.rx = pl->rec_row.as<int>();
-        a.rz = pl->rec_slot.as<int>();
-        a.rec_zz = pl->rec_zz.as<float>();
-        a.rec_xx = pl->rec_xx.as<float>();
+        rc = ve_setup(pl, p, true, hb, hku, hkr, &hkus, &hkrs);
+        if (rc) return bail(rc);
         rc = ve_zero(pl);
         if (rc) return bail(rc);
         e2 = hipStreamSynchronize(sm);
@@ -700,7 +744,11 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     visco_auto_tiling(pl, &pl->nw, &pl->chunk);
     pl->kernel = visco_pick(R, L, pl->nw);
     if (!pl->kernel) return bail(fail(ctx, SFD_ENOTSUP, "no kernel for this configuration"));
-    rc = visco_zero(pl);
+    // The tiled two-launch step runs by default.
+    rc = ve_setup(pl, p, false, hb, hku, hkr, nullptr, nullptr);
+    if (rc) return bail(rc);
+    rc = ve_zero(pl);
+    if (!rc) rc = visco_zero(pl);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bail(hip_check(ctx, e, "plan setup"));
@@ -711,6 +759,9 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
 int sfd_visco_plan_set_tiling(sfd_visco_plan* pl, int waves, int chunk_rows) {
     if (!pl) return SFD_EINVAL;
     if (pl->elastic) return SFD_OK;  // the visco-elastic kernels have one tiling
+    // vs = 0: automatic = the tiled step; an explicit tiling selects the
+    // fused march kernel.
+    pl->ve_on = waves == 0 && chunk_rows == 0;
     int nw = pl->nw, ch = pl->chunk;
     if (waves != 0 || chunk_rows != 0) {
         visco_auto_tiling(pl, &nw, &ch);
@@ -733,10 +784,10 @@ int sfd_visco_plan_run(sfd_visco_plan* pl, float* elapsed_ms) {
     if (!pl) return SFD_EINVAL;
     sfd_context* ctx = pl->ctx;
     SFD_HIP(ctx, hipSetDevice(ctx->device));
-    int rc = pl->elastic ? ve_zero(pl) : visco_zero(pl);
+    int rc = pl->ve_on ? ve_zero(pl) : visco_zero(pl);
     if (rc) return rc;
     SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    rc = pl->elastic ? ve_enqueue(pl) : visco_enqueue(pl, 0, pl->nsteps);
+    rc = pl->ve_on ? ve_enqueue(pl) : visco_enqueue(pl, 0, pl->nsteps);
     if (rc) return rc;
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
@@ -748,7 +799,7 @@ int sfd_visco_plan_fetch(sfd_visco_plan* pl, float* record, float* snaps) {
     if (!pl) return SFD_EINVAL;
     sfd_context* ctx = pl->ctx;
     SFD_HIP(ctx, hipSetDevice(ctx->device));
-    if (pl->elastic) {
+    if (pl->ve_on) {
         if (record && pl->nrec > 0 && pl->nt > 0)
             SFD_HIP(ctx, hipMemcpyAsync(record, pl->rec_zz.ptr, (size_t)pl->nrec * pl->nt * 4, hipMemcpyDeviceToHost,
                                         ctx->stream));

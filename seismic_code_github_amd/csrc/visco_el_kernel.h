@@ -1,4 +1,5 @@
-// Visco-elastic (vs > 0) step of the rotated-staggered-grid propagator:
+// Visco-elastic (vs > 0) and visco-acoustic (vs = 0) steps of the
+// rotated-staggered-grid propagator:
 // forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90,
 // extrapolation_one_step_10nd_absbc (:695-839) with L standard linear solids,
 // as two launches per step (the reference's two half-steps):
@@ -276,6 +277,147 @@ __global__ __launch_bounds__(64 * NW) void visco_el_stress_kernel(ViscoElArgs a)
     }
 }
 
+// Visco-acoustic (vs = 0) step on the same padded layout: one stress sigma
+// (= sxx = szz; a.sxx and a.szz both point at it), L memory variables, the
+// model kup = ku, krp = kr; kus, krs and sxz are not used.  These are the
+// elastic kernels' operations with mu = 0 (kup - 2 kus = kup, krp - 2 krs =
+// krp) minus the terms that vanish: sigma_xz and its memory variables stay
+// zero, so vx has an x part only and vz a z part only.  Strips: xs [2 + L]
+// [xlo + xhi][nz] = vx, sigma, r_l; zs [2 + L][nx][zlo + zhi] = vz, sigma, r_l.
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_ac_velocity_kernel(ViscoElArgs a) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_s[HH * W];
+    int i0, k0;
+    ve_tile(a, TX, &i0, &k0);
+    if (blockIdx.x == 0 && a.rec_col >= 0) {
+        for (int r = threadIdx.x; r < a.nrec; r += 64 * NW)
+            a.rec_zz[(size_t)r * a.nrt + a.rec_col] = a.szz[(ptrdiff_t)a.rx[r] * a.pitch + a.rz[r]];
+    }
+    ve_stage<R, TX, NW>(s_s, a.szz, a.pitch, i0, k0);
+    __syncthreads();
+    float co[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) co[m] = a.a[m];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = k0 + lane;
+    if (k >= a.nz) return;
+    const int zw = a.zlo + a.zhi;
+    int zc = -1;
+    if (k < a.zlo) zc = k;
+    else if (k >= a.nz - a.zhi) zc = k - (a.nz - a.zhi) + a.zlo;
+    for (int j = 0; j < TX / NW; ++j) {
+        const int il = wv + j * NW, i = i0 + il;
+        if (i >= a.nx) break;
+        float dd, da;
+        ve_diag<R, 0, W>(s_s + (il + R) * W + lane + R, co, dd, da);
+        const ptrdiff_t o = (ptrdiff_t)i * a.pitch + k;
+        const float b = a.b[o];
+        const float vxx = b * (a.hx * (dd + da)), vzz = b * (a.hz * (dd - da));
+        float vx = a.vx[o] + vxx, vz = a.vz[o] + vzz;
+        int xr = -1;
+        if (i < a.xlo) xr = i;
+        else if (i >= a.nx - a.xhi) xr = i - (a.nx - a.xhi) + a.xlo;
+        if (xr >= 0) {
+            const float f = a.dvx[i];
+            const size_t e = (size_t)xr * a.nz + k;
+            const float t = a.xs[e] + vxx;
+            vx -= (1.f - f) * t;
+            a.xs[e] = f * t;
+        }
+        if (zc >= 0) {
+            const float f = a.dvz[k];
+            const size_t e = (size_t)i * zw + zc;
+            const float t = a.zs[e] + vzz;
+            vz -= (1.f - f) * t;
+            a.zs[e] = f * t;
+        }
+        a.vx[o] = vx;
+        a.vz[o] = vz;
+    }
+}
+
+template <int R, int L, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_ac_stress_kernel(ViscoElArgs a) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_vx[HH * W], s_vz[HH * W];
+    int i0, k0;
+    ve_tile(a, TX, &i0, &k0);
+    ve_stage<R, TX, NW>(s_vx, a.vx, a.pitch, i0, k0);
+    ve_stage<R, TX, NW>(s_vz, a.vz, a.pitch, i0, k0);
+    __syncthreads();
+    float co[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) co[m] = a.a[m];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = k0 + lane;
+    if (k >= a.nz) return;
+    const int xw = a.xlo + a.xhi, zw = a.zlo + a.zhi;
+    int zc = -1;
+    if (k < a.zlo) zc = k;
+    else if (k >= a.nz - a.zhi) zc = k - (a.nz - a.zhi) + a.zlo;
+    const float amp = a.amp[a.step];
+    const bool src_col = k >= a.sz0 && k < a.sz1;
+    for (int j = 0; j < TX / NW; ++j) {
+        const int il = wv + j * NW, i = i0 + il;
+        if (i >= a.nx) break;
+        const int off = (il + R) * W + lane + R;
+        float dd1, da1, dd4, da4;
+        ve_diag<R, 1, W>(s_vx + off, co, dd1, da1);
+        ve_diag<R, 1, W>(s_vz + off, co, dd4, da4);
+        const float d1 = a.hx * (dd1 + da1), d4 = a.hz * (dd4 - da4);  // Dx vx, Dz vz
+        const ptrdiff_t o = (ptrdiff_t)i * a.pitch + k;
+        const float ku = a.kup[o], kr = a.krp[o];
+        float rold[L], rsum = 0.f;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            rold[l] = a.r[(size_t)l * a.rstride + o];
+            rsum += rold[l];
+        }
+        float ix[1 + L], iz[1 + L], tot[1 + L];
+        ix[0] = ku * d1;
+        iz[0] = ku * d4;
+        tot[0] = a.szz[o] + ix[0] + iz[0] + a.dt * rsum;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const float cl = a.c[l];
+            ix[1 + l] = -cl * (kr * d1);
+            iz[1 + l] = -cl * (kr * d4);
+            tot[1 + l] = rold[l] + ix[1 + l] + iz[1 + l] - cl * rold[l];
+        }
+        int xr = -1;
+        if (i < a.xlo) xr = i;
+        else if (i >= a.nx - a.xhi) xr = i - (a.nx - a.xhi) + a.xlo;
+        // Source (after the x parts' sponge, as addsrc follows calpmlbclrtb):
+        // into sigma's x part and total.
+        float add = 0.f;
+        if (src_col && i >= a.sx0 && i < a.sx1) add = amp * a.g[(size_t)(i - a.sx0) * (a.sz1 - a.sz0) + (k - a.sz0)];
+        if (xr >= 0) {
+            const float f = a.dsx[i];
+            const size_t q = (size_t)xw * a.nz, e = (size_t)xr * a.nz + k;
+#pragma unroll
+            for (int p = 0; p < 1 + L; ++p) {
+                const float t = a.xs[(1 + p) * q + e] + ix[p];
+                tot[p] -= (1.f - f) * t;
+                a.xs[(1 + p) * q + e] = f * t + (p == 0 ? add : 0.f);
+            }
+        }
+        if (zc >= 0) {
+            const float f = a.dsz[k];
+            const size_t q = (size_t)a.nx * zw, e = (size_t)i * zw + zc;
+#pragma unroll
+            for (int p = 0; p < 1 + L; ++p) {
+                const float t = a.zs[(1 + p) * q + e] + iz[p];
+                tot[p] -= (1.f - f) * t;
+                a.zs[(1 + p) * q + e] = f * t;
+            }
+        }
+        a.szz[o] = tot[0] + add;
+#pragma unroll
+        for (int l = 0; l < L; ++l) a.r[(size_t)l * a.rstride + o] = tot[1 + l];
+    }
+}
+
 // Receivers of the last step (the velocity launch of the next step gathers
 // all others).
 __global__ void visco_el_gather_kernel(ViscoElArgs a) {
@@ -283,7 +425,7 @@ __global__ void visco_el_gather_kernel(ViscoElArgs a) {
     if (r < a.nrec && a.rec_col >= 0) {
         const ptrdiff_t o = (ptrdiff_t)a.rx[r] * a.pitch + a.rz[r];
         a.rec_zz[(size_t)r * a.nrt + a.rec_col] = a.szz[o];
-        a.rec_xx[(size_t)r * a.nrt + a.rec_col] = a.sxx[o];
+        if (a.rec_xx) a.rec_xx[(size_t)r * a.nrt + a.rec_col] = a.sxx[o];
     }
 }
 

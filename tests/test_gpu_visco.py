@@ -99,6 +99,32 @@ def test_visco_tilings_bitwise_identical(ctx):
         assert np.array_equal(r, recs[0])
 
 
+@pytest.mark.parametrize("radius,nmech", [(5, 3), (4, 1)])
+def test_visco_tiled_step_and_march_kernel(ctx, radius, nmech):
+    """vs = 0 runs two tiled launches per step by default; an explicit tiling
+    selects the fused march kernel.  Both match the FP32 oracle (gathers and
+    snapshots) and each other."""
+    shot = small_shot(nmech=nmech, radius=radius, nx=170, nz=190, nt=450, snap_steps=(300,),
+                      snap_fields=sfd.FIELD_P | sfd.FIELD_VX | sfd.FIELD_VZ)
+    out = {}
+    for name, tiling in (("tiled", (0, 0)), ("march", (2, 0))):
+        with sfd.ViscoPlan(shot, ctx) as pl:
+            pl.set_tiling(*tiling)
+            pl.run()
+            out[name] = pl.fetch()
+            xx = pl.fetch_xx()
+            assert np.array_equal(xx, out[name][0])  # strxx = strzz when vs = 0
+    o32 = run_oracle(shot, "f32", snap_steps=(300,))
+    for name, (rec, snaps) in out.items():
+        e = rel(rec, o32["rec_zz"])
+        es = [rel(snaps[0, f], o32["snaps"][0, of]) for f, of in ((0, 1), (1, 3), (2, 4))]
+        print(f"R={radius} L={nmech} {name}: gather vs FP32 oracle {e:.2e}, snapshots {max(es):.2e}")
+        assert e <= TOL_F32 and max(es) <= TOL_F32
+    d = rel(out["tiled"][0], out["march"][0].astype(np.float64))
+    print(f"tiled vs march gather {d:.2e}")
+    assert d <= TOL_F32
+
+
 def test_visco_scaled_c2_against_oracle(ctx):
     """The 1001 x 2001 workload shrunk 8x per axis (same 40-cell sponge, same
     seeds' construction), 3 mechanisms."""
