@@ -227,7 +227,17 @@ int grad_backward(GradShot* g) {
 }
 
 // dJ/dvp from the accumulator and the source adjoint; adds into grad_vp.
-int grad_finish(GradShot* g, const float* vp, const float* den, const float* wavelet, float* grad_vp) {
+// The source's share of dJ/dvp: the injected amplitude dt rho_s vp_s w_k
+// depends on vp at the source cell.
+void grad_source_term(const sfd_acoustic_plan* pl, const std::vector<float>& sa, const float* vp,
+                      const float* wavelet, float* grad_vp) {
+    const size_t sc = (size_t)pl->sx * pl->nz + pl->sz;
+    double t = 0.0;
+    for (int k = 0; k < pl->nt; ++k) t += (double)sa[k] * wavelet[k];
+    grad_vp[sc] += (float)(-t / ((double)vp[sc] * vp[sc]));
+}
+
+int grad_finish(GradShot* g, const float* vp, const float* wavelet, float* grad_vp) {
     sfd_acoustic_plan* pl = g->pl;
     sfd_context* ctx = pl->ctx;
     const size_t n = (size_t)pl->nx * pl->nz;
@@ -236,11 +246,7 @@ int grad_finish(GradShot* g, const float* vp, const float* den, const float* wav
     SFD_HIP(ctx, hipMemcpyAsync(sa.data(), pl->src_adj.ptr, (size_t)pl->nt * 4, hipMemcpyDeviceToHost, ctx->stream));
     SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t c = 0; c < n; ++c) grad_vp[c] += -2.0f * acc[c] / vp[c];
-    const size_t sc = (size_t)pl->sx * pl->nz + pl->sz;
-    double t = 0.0;
-    for (int k = 0; k < pl->nt; ++k) t += (double)sa[k] * wavelet[k];
-    (void)den;
-    grad_vp[sc] += (float)(-t / ((double)vp[sc] * vp[sc]));
+    grad_source_term(pl, sa, vp, wavelet, grad_vp);
     return SFD_OK;
 }
 
@@ -269,7 +275,7 @@ int grad_one(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, co
         rc = grad_upload_seed(&g, seed);
     }
     if (!rc) rc = grad_backward(&g);
-    if (!rc) rc = grad_finish(&g, vp, den, wavelet, grad_vp);
+    if (!rc) rc = grad_finish(&g, vp, wavelet, grad_vp);
     if (g.pl) sfd_acoustic_plan_destroy(g.pl);
     return rc;
 }
@@ -294,6 +300,49 @@ __global__ void adjoint_inject_group_kernel(float* const* __restrict__ p, const 
     if (q >= n) return;
     const InjEntry t = e[q];
     p[t.shot][t.cell] += t.w * res[t.shot][(size_t)k * nslot[t.shot] + t.slot];
+}
+
+// Residual of one shot on the device: res[k][slot] = sum over the slot's
+// receivers r (ascending) of record(slot, k) - obs[r][k] (obs trace-major,
+// the gathers' layout), the seed of the adjoint; partial[block] = that
+// block's share of sum 0.5 (record - obs)^2 (a fixed tree, so J does not
+// depend on scheduling).  rec_buf is the time-major gather buffer.
+__global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ rec_buf, int nslot, int nt,
+                                                       const float* __restrict__ obs,
+                                                       const int* __restrict__ slot_start,
+                                                       const int* __restrict__ slot_recv, float* __restrict__ res,
+                                                       double* __restrict__ partial) {
+    __shared__ double red[256];
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (q < (size_t)nslot * nt) {
+        const int slot = (int)(q / nt), k = (int)(q - (size_t)slot * nt);
+        const float v = rec_buf[(size_t)k * nslot + slot];
+        float acc = 0.f;
+        for (int p = slot_start[slot]; p < slot_start[slot + 1]; ++p) {
+            const float sd = v - obs[(size_t)slot_recv[p] * nt + k];
+            acc += sd;
+            m += 0.5 * (double)sd * sd;
+        }
+        res[(size_t)k * nslot + slot] = acc;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// out[c] = sum over the group's shots (in order) of -2 acc_s[c] / vp[c].
+__global__ void grad_sum_kernel(float* __restrict__ out, const float* const* __restrict__ acc, int nshot,
+                                const float* __restrict__ vp, size_t n) {
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+        float g = 0.f;
+        for (int s = 0; s < nshot; ++s) g += -2.0f * acc[s][c] / vp[c];
+        out[c] = g;
+    }
 }
 
 // Boundary saving: one band cell of one shot.
@@ -527,20 +576,43 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (rc) return rc;
     }
     for (auto& g : G.shots) g.pl->v_raw = true;
-    // 2. residuals, misfit, seeds.
-    for (int q = 0; q < nshot; ++q) {
-        GradShot& g = G.shots[q];
-        std::vector<float> rec((size_t)params[q].nrec * G.nt), sd(rec.size());
-        rc = sfd_acoustic_plan_fetch(g.pl, rec.data(), nullptr);
-        if (rc) return rc;
-        double m = 0.0;
-        for (size_t c = 0; c < rec.size(); ++c) {
-            sd[c] = rec[c] - d_obs[q][c];
-            m += 0.5 * (double)sd[c] * sd[c];
+    // 2. residuals, misfit, seeds, on the device from the gather buffers.
+    {
+        DeviceBuffer obs, starts, recvs, partial;
+        size_t obs_max = 1;
+        for (int q = 0; q < nshot; ++q) obs_max = std::max(obs_max, (size_t)params[q].nrec * G.nt);
+        const hipError_t e0 = obs.alloc(obs_max * 4);
+        if (e0 != hipSuccess) return hip_check(ctx, e0, "residual buffers");
+        std::vector<double> parts;
+        for (int q = 0; q < nshot; ++q) {
+            GradShot& g = G.shots[q];
+            sfd_acoustic_plan* pl = g.pl;
+            const int nrec = params[q].nrec, nslot = pl->nslot;
+            if (nrec == 0 || nslot == 0) continue;
+            // Receivers of each slot, in receiver order.
+            std::vector<int> st(nslot + 1, 0), rv(nrec);
+            for (int r = 0; r < nrec; ++r) ++st[g.slot_cell[r] + 1];
+            for (int k = 0; k < nslot; ++k) st[k + 1] += st[k];
+            std::vector<int> fill(st.begin(), st.end() - 1);
+            for (int r = 0; r < nrec; ++r) rv[fill[g.slot_cell[r]]++] = r;
+            const size_t cells = (size_t)nslot * G.nt;
+            const int nb = (int)((cells + 255) / 256);
+            hipError_t e = upload(starts, st.data(), st.size(), s);
+            if (e == hipSuccess) e = upload(recvs, rv.data(), rv.size(), s);
+            if (e == hipSuccess) e = hipMemcpyAsync(obs.ptr, d_obs[q], (size_t)nrec * G.nt * 4, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess && partial.bytes < (size_t)nb * 8) e = partial.alloc((size_t)nb * 8);
+            if (e != hipSuccess) return hip_check(ctx, e, "residual tables");
+            hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, s, pl->rec_buf.as<float>(), nslot, G.nt,
+                               obs.as<float>(), starts.as<int>(), recvs.as<int>(), pl->res.as<float>(),
+                               partial.as<double>());
+            SFD_HIP(ctx, hipGetLastError());
+            parts.resize(nb);
+            SFD_HIP(ctx, hipMemcpyAsync(parts.data(), partial.ptr, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
+            SFD_HIP(ctx, hipStreamSynchronize(s));
+            double m = 0.0;
+            for (double v : parts) m += v;
+            if (misfit) *misfit += m;
         }
-        if (misfit) *misfit += m;
-        rc = grad_upload_seed(&g, sd.data());
-        if (rc) return rc;
     }
     // 3. backward: per segment, restore, recompute with divergences, adjoint.
     for (auto& g : G.shots) {
@@ -591,10 +663,32 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
             }
         }
     }
-    // 4. dJ/dvp summed over the group's shots.
-    for (int q = 0; q < nshot; ++q) {
-        rc = grad_finish(&G.shots[q], vp, den, wavelet[q], grad_vp);
-        if (rc) return rc;
+    // 4. dJ/dvp summed over the group's shots on the device, then the source
+    // terms of each shot.
+    {
+        sfd_acoustic_plan* p0 = G.shots[0].pl;
+        const size_t n = (size_t)p0->nx * p0->nz;
+        DeviceBuffer dvp, sum, accp;
+        std::vector<const float*> acc(nshot);
+        for (int q = 0; q < nshot; ++q) acc[q] = G.shots[q].pl->grid(G.shots[q].pl->grad);
+        hipError_t e = upload(dvp, vp, n, s);
+        if (e == hipSuccess) e = upload(accp, acc.data(), acc.size(), s);
+        if (e == hipSuccess) e = sum.alloc(n * 4);
+        if (e != hipSuccess) return hip_check(ctx, e, "gradient sum");
+        hipLaunchKernelGGL(grad_sum_kernel, dim3(2048), dim3(256), 0, s, sum.as<float>(), accp.as<const float*>(), nshot,
+                           dvp.as<float>(), n);
+        SFD_HIP(ctx, hipGetLastError());
+        std::vector<float> g(n);
+        SFD_HIP(ctx, hipMemcpyAsync(g.data(), sum.ptr, n * 4, hipMemcpyDeviceToHost, s));
+        SFD_HIP(ctx, hipStreamSynchronize(s));
+        for (size_t c = 0; c < n; ++c) grad_vp[c] += g[c];
+        for (int q = 0; q < nshot; ++q) {
+            sfd_acoustic_plan* pl = G.shots[q].pl;
+            std::vector<float> sa(pl->nt);
+            SFD_HIP(ctx, hipMemcpyAsync(sa.data(), pl->src_adj.ptr, (size_t)pl->nt * 4, hipMemcpyDeviceToHost, s));
+            SFD_HIP(ctx, hipStreamSynchronize(s));
+            grad_source_term(pl, sa, vp, wavelet[q], grad_vp);
+        }
     }
     return SFD_OK;
 }
