@@ -11,14 +11,15 @@ wavelet and geometry already resident in device memory.  value = Mcell-updates/s
 over all stored cells (boundary layers included).
 
 Launch shapes: the library's autotuner (sfd_acoustic_plan_autotune /
-sfd_acoustic_batch_autotune) times every built shape for 20 steps before the
-timed shots and keeps the fastest (--no-tune: the library's defaults); the
+sfd_acoustic_batch_autotune) times every built shape (60 steps for the
+single shot, 20 for the batch) before the timed shots and keeps the fastest (--no-tune: the library's defaults); the
 chosen shape is reported as launch_shape.
 
 Roofline: the library runs each time step as two launches (velocities, then
 pressure; acoustic_tiled.h), so the roofline's unit is one time step:
-achieved = the fused step's algorithmic bytes (32 B per cell plus the C-PML
-strips' memory variables) / the device time of one step (both launches),
+achieved = the fused step's algorithmic bytes (28 B per cell with the
+workload's uniform density, plus the C-PML strips' memory variables) / the
+device time of one step (both launches),
 measured with HIP events on the library's stream over the timed shots;
 traffic = both launches' FETCH_SIZE + WRITE_SIZE per step, scaled by a
 256 MiB copy (profiles/acoustic_step_cpml_2101x4101.md).
@@ -46,11 +47,13 @@ def log(*a):
 
 def algorithmic_bytes_per_step(shot, plan_pml_w=None):
     """Compulsory HBM bytes of one time step: P, Vx, Vz read and written
-    (24 B), the two per-cell coefficients read (8 B), and for C-PML the two
-    memory variables of each strip cell read and written (16 B)."""
+    (24 B), the per-cell coefficients read (kf and bf, 8 B; kf only, 4 B,
+    when the density is uniform: the library then passes bf as one number),
+    and for C-PML the two memory variables of each strip cell read and
+    written (16 B)."""
     nx, nz = shot.vp.shape
-    R = shot.radius
-    b = 32.0 * nx * nz
+    uniform = shot.den is None or bool(np.all(np.asarray(shot.den) == np.asarray(shot.den).flat[0]))
+    b = (28.0 if uniform else 32.0) * nx * nz
     if shot.boundary == "cpml":
         w = shot.width + 1
         xrows = min(2 * w, nx)
@@ -109,7 +112,8 @@ def batched_leg(nt, no_tune=False):
             "graph_shots_per_s": round(64 / (g_ms * scale / 1e3), 3),
             "plain_shots_per_s": round(64 / (p_ms * scale / 1e3), 3),
             "speedup": round(p_ms / g_ms, 2),
-            "graph_frac_of_8TBs": round(cells * nt * 32.0 / (g_ms / 1e3) / 8e12, 4),
+            # uniform density: 28 algorithmic bytes per cell-update
+            "graph_frac_of_8TBs": round(cells * nt * 28.0 / (g_ms / 1e3) / 8e12, 4),
             "launch_shape": shape_dict(kind, ta, tb, tuned_us)}
 
 
@@ -289,18 +293,18 @@ def gradient_leg(nt):
     bs_err = float(np.linalg.norm(gbi - gi) / max(np.linalg.norm(gi), 1e-300))
     scale = 2500.0 / nt
     n = len(shots)
-    # Algorithmic bytes per shot-cell and step of the fused step: forward 32,
-    # recomputation 36 (+ the divergence), adjoint 44 (+ divergence read,
-    # gradient read and write).
+    # Algorithmic bytes per shot-cell and step of the fused step with uniform
+    # density (bf passed as one number): forward 28, recomputation 32 (+ the
+    # divergence), adjoint 40 (+ divergence read, gradient read and write).
     cells = int(np.prod(shots[0].vp.shape))
-    gbs = n * cells * 112.0 * nt / grad_s / 1e9
+    gbs = n * cells * 100.0 * nt / grad_s / 1e9
     return {"workload": "16 shots, 801x801 + 40-cell sponge (881x881), R=8, 10 Hz, checkpoint every 50 steps",
             "timed_steps": nt, "misfit": J,
             "fwd_adj_shots_per_s": round(n / (grad_s * scale), 3),
             "plain_fwd_shots_per_s": round(n / (plain_ms / 1e3 * scale), 3),
             "batched_fwd_shots_per_s": round(n / (graph_ms / 1e3 * scale), 3),
             "fwd_adj_vs_plain_fwd": round((plain_ms / 1e3) / grad_s, 3),
-            "bytes_per_shot_cell_step": 112, "achieved_GBs": round(gbs, 1),
+            "bytes_per_shot_cell_step": 100, "achieved_GBs": round(gbs, 1),
             "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4),
             "timing": "whole sfd_acoustic_misfit_gradient call: set-up, residuals and transfers included",
             "boundary_saving": {"fwd_adj_shots_per_s": round(n / (bs_s * scale), 3),
@@ -372,7 +376,7 @@ def main():
     if args.waves or args.chunk:
         plan.set_tiling(args.waves, args.chunk)
     elif not args.no_tune:
-        tuned_us = plan.autotune(20)
+        tuned_us = plan.autotune(60)
     kind, ta, tb = plan.shape()
     log(f"launch shape: {shape_dict(kind, ta, tb, tuned_us)}")
     for i in range(args.warmup):

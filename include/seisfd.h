@@ -180,10 +180,11 @@ int sfd_acoustic_batch_fetch(sfd_acoustic_batch *batch, int shot, float *record)
 int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
 
 /* ---------------------------------------------------------------- autotuner
- * (no counterpart in the reference).  Times every built launch shape of the
- * step (the tiled step's shapes in fast arithmetic, the fused march kernel at
- * 1, 2 and 4 waves) over min(trial_steps, nt) steps from a zero state, keeps
- * the fastest and records it in the context's cache under the grid class:
+ * (no counterpart in the reference).  After a warm-up of up to 4 trials'
+ * steps, times every built launch shape of the step (the tiled step's shapes
+ * in fast arithmetic, the fused march kernel at 1, 2 and 4 waves) twice over
+ * min(trial_steps, nt) steps from a zero state, keeps the fastest and records
+ * it in the context's cache under the grid class:
  *   plan:  radius, boundary + arithmetic, nx and nz to the nearest power of 2
  *   batch: radius, boundary + arithmetic, shot count and total cells likewise.
  * Plans and batches created later in the context with the same class start

@@ -351,6 +351,7 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* P, const Fi
     a.k = k;
     a.kf = pl->grid(pl->kf);
     a.bf = pl->grid(pl->bf);
+    a.bf_const = pl->bf_const;
     a.ax = pl->ax.as<float>() + pl->pad;
     a.az = pl->az.as<float>();
     a.pml_x = pl->pml_x.as<float>();
@@ -618,6 +619,13 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
             hk[c] = (ref || upd) ? d * vp[c] * vp[c] * p->dt : 0.f;
             hb[c] = ref ? d : upd ? (float)((double)p->dt / (double)d) : 0.f;
         }
+    }
+    // Uniform density: every updated cell's bf is the same number.
+    if (!ref) {
+        bool uniform = true;
+        const float d0 = den ? den[0] : 2400.0f;
+        for (size_t c = 1; den && uniform && c < n; ++c) uniform = den[c] == d0;
+        if (uniform) pl->bf_const = (float)((double)p->dt / (double)d0);
     }
     // Source amplitude per step, the reference's float product dt*DEN*Vp*signal (:249-250).
     std::vector<float> hamp(std::max(p->nt, 1), 0.f);
@@ -1235,17 +1243,28 @@ int sfd_acoustic_plan_autotune(sfd_acoustic_plan* pl, int trial_steps, float* be
     const int n = std::min(trial_steps, pl->nt);
     if (n < 1) return fail(ctx, SFD_EINVAL, "the plan has no time steps to time");
     SFD_HIP(ctx, hipSetDevice(ctx->device));
-    TunedShape best;
-    best.us = 3.4e38f;
-    for (TunedShape t : shape_candidates(pl->mode, false)) {
-        int rc = apply_shape(pl, t);
-        if (rc) return rc;
-        if (t.kind == 1) t.b = pl->chunk;
-        rc = time_steps(ctx, n, [&](int m) { return m < 0 ? zero_state(pl) : enqueue_steps(pl, 0, m); }, &t.us);
-        if (rc) return rc;
+    auto run = [&](int m) { return m < 0 ? zero_state(pl) : enqueue_steps(pl, 0, m); };
+    // The GPU first runs a few trials' worth of steps (clocks settle), then
+    // every candidate is timed twice, in two passes, keeping its better time.
+    float warm = 0.f;
+    int rc = time_steps(ctx, std::min(pl->nt, 4 * n), run, &warm);
+    if (rc) return rc;
+    std::vector<TunedShape> cand = shape_candidates(pl->mode, false);
+    for (auto& t : cand) t.us = 3.4e38f;
+    for (int pass = 0; pass < 2; ++pass)
+        for (TunedShape& t : cand) {
+            rc = apply_shape(pl, t);
+            if (rc) return rc;
+            if (t.kind == 1) t.b = pl->chunk;
+            float us = 0.f;
+            rc = time_steps(ctx, n, run, &us);
+            if (rc) return rc;
+            t.us = std::min(t.us, us);
+        }
+    TunedShape best = cand[0];
+    for (const TunedShape& t : cand)
         if (t.us < best.us) best = t;
-    }
-    int rc = apply_shape(pl, best);
+    rc = apply_shape(pl, best);
     if (!rc) rc = zero_state(pl);
     if (rc) return rc;
     SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1269,18 +1288,26 @@ int sfd_acoustic_batch_autotune(sfd_acoustic_batch* b, int trial_steps, float* b
         }
         return SFD_OK;
     };
-    TunedShape best;
-    best.us = 3.4e38f;
-    for (TunedShape t : shape_candidates(b->mode, true)) {
-        int rc = t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b) : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
-        if (rc) return rc;
-        if (t.kind == 1) t.b = b->plans[0]->chunk;
-        rc = time_steps(ctx, n, run, &t.us);
-        if (rc) return rc;
+    float warm = 0.f;
+    int rc = time_steps(ctx, std::min(b->nt, 4 * n), run, &warm);
+    if (rc) return rc;
+    std::vector<TunedShape> cand = shape_candidates(b->mode, true);
+    for (auto& t : cand) t.us = 3.4e38f;
+    for (int pass = 0; pass < 2; ++pass)
+        for (TunedShape& t : cand) {
+            rc = t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b) : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
+            if (rc) return rc;
+            if (t.kind == 1) t.b = b->plans[0]->chunk;
+            float us = 0.f;
+            rc = time_steps(ctx, n, run, &us);
+            if (rc) return rc;
+            t.us = std::min(t.us, us);
+        }
+    TunedShape best = cand[0];
+    for (const TunedShape& t : cand)
         if (t.us < best.us) best = t;
-    }
-    int rc = best.kind == 0 ? sfd_acoustic_batch_set_tiled(b, best.a, best.b)
-                            : sfd_acoustic_batch_set_tiling(b, best.a, best.b);
+    rc = best.kind == 0 ? sfd_acoustic_batch_set_tiled(b, best.a, best.b)
+                        : sfd_acoustic_batch_set_tiling(b, best.a, best.b);
     if (!rc) rc = run(-1);
     if (rc) return rc;
     SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));

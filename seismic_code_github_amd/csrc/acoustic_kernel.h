@@ -79,6 +79,9 @@ struct AcousticStepArgs {
     // REF:  kf = float(DEN*Vp*Vp*dt) evaluated as the reference does, bf = DEN.
     const float* __restrict__ kf;
     const float* __restrict__ bf;
+    // Tiled step: nonzero = the bf of every updated cell (uniform density),
+    // which the velocity launch then does not read from bf.
+    float bf_const;
     // Sponge: ax[i] is the x-edge factor of row i, or 2.0 where row i is not
     // in an x-edge strip; az[j] is the z-edge factor of column j (1.0 inside).
     // Every per-row array (state, model, ax, rec_row) points at row 0 of an

@@ -55,6 +55,9 @@ struct sfd_acoustic_plan {
     StepKernel tvel = nullptr, tpres = nullptr;
     // The velocity fields hold unabsorbed values (after a tiled sponge step).
     bool v_raw = false;
+    // Fast arithmetic with uniform density: bf of every updated cell (the
+    // tiled step takes it as an argument); 0 otherwise.
+    float bf_const = 0.f;
 
     // Gradient workspace (plans made by the gradient entry points): adjoint
     // state, gradient accumulator and the divergence of one checkpoint

@@ -12,7 +12,8 @@
 // once (loads unrolled so all are in flight before the first store), tiles go
 // to the XCDs in bands of whole rows.  Every field is streamed once per
 // launch: 24 B per cell for the velocity launch (P, Vx, Vz, bf read; Vx, Vz
-// written), 20 B for the pressure launch (Vx, Vz, P, kf read; P written);
+// written; 20 B with uniform density, whose bf is one kernel argument), 20 B
+// for the pressure launch (Vx, Vz, P, kf read; P written);
 // the fused march kernel needs 32 B per step but cannot keep as many bytes
 // in flight.  Fields keep the plan's layout: rows of nz floats with zero rows
 // above and below (columns past either edge wrap into the neighbouring row;
@@ -90,6 +91,7 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
     // This lane's own-cell inputs, loaded before the tile so that their
     // latency overlaps the staging.
     float vx0[RPW], vz0[RPW], bf[RPW], axr[RPW];
+    const float bfc = a.bf_const;
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int i = i0 + wv + q * NW;
@@ -97,8 +99,18 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
         const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         vx0[q] = ok ? a.vx_in[o] : 0.f;
         vz0[q] = ok ? a.vz_in[o] : 0.f;
-        bf[q] = ok ? a.bf[o] : 0.f;
         if constexpr (!CPML && !REV) axr[q] = ok ? a.ax[i] : 1.f;
+    }
+    // Uniform density: one bf for every cell, 4 B per cell less to read.
+    if (bfc != 0.f) {
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) bf[q] = bfc;
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int i = i0 + wv + q * NW;
+            bf[q] = (jok && i < nx - R) ? a.bf[(ptrdiff_t)i * nz + j] : 0.f;
+        }
     }
     float az_j = 1.f;
     if constexpr (!CPML && !REV) az_j = jok ? a.az[j] : 1.f;
