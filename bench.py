@@ -89,10 +89,11 @@ def shape_dict(kind, a, b, us):
 
 def batched_leg(nt, no_tune=False):
     """64 shots on uneven aperture windows of a 1001 x 6001 model (C-PML 50,
-    nt=3000 per shot): one launch per step for all shots replayed from a
-    hipGraph, against the plain path (shot after shot, one launch per shot
-    and step).  Timed on the first `nt` steps; shots/s is scaled to the full
-    3000 steps (every step costs the same)."""
+    nt=3000 per shot): one tiled step (two launches) per time step for all
+    shots, replayed from a hipGraph, against the plain path (shot after shot,
+    one tiled step per shot and time step).  Launch shape from the autotuner.
+    Timed on the first `nt` steps; shots/s is scaled to the full 3000 steps
+    (every step costs the same)."""
     import seismic_code_github_amd as sfd
     from seismic_code_github_amd import workloads as W
     _, shots = W.windowed_shots(nshot=64, nt=nt)
