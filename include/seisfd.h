@@ -89,7 +89,8 @@ typedef struct sfd_acoustic_params {
     int arith;             /* enum sfd_arith */
 } sfd_acoustic_params;
 
-/* Context: owns the HIP stream and the error text. device = HIP ordinal. */
+/* Context: owns the HIP stream and the error text. device = HIP ordinal.
+ * Destroy every plan and batch made in a context before the context. */
 int sfd_context_create(int device, sfd_context **ctx);
 int sfd_context_destroy(sfd_context *ctx);
 const char *sfd_last_error(const sfd_context *ctx);
@@ -320,9 +321,17 @@ int sfd_visco_plan_run(sfd_visco_plan *plan, float *elapsed_ms);
 int sfd_visco_plan_fetch(sfd_visco_plan *plan, float *record, float *snaps);
 /* strxx at the receivers [nrec][nt] (reference record2). */
 int sfd_visco_plan_fetch_xx(sfd_visco_plan *plan, float *record_xx);
-/* Waves per workgroup (2 or 4) and rows marched per workgroup; 0 = automatic
- * (visco-acoustic kernel; the visco-elastic kernels have one tiling). */
+/* vs = 0: explicit waves per workgroup (2 or 4) and rows marched per
+ * workgroup select the fused march kernel; (0, 0) = the tiled step.  The
+ * visco-elastic kernels have one tiling (no effect). */
 int sfd_visco_plan_set_tiling(sfd_visco_plan *plan, int waves, int chunk_rows);
+/* Autotuner (no counterpart in the reference), as sfd_acoustic_plan_autotune:
+ * the tiled step against (vs = 0) the fused march kernel at 2 and 4 waves,
+ * cached for the grid class (radius, L, vs = 0 or not, nx and nz to the
+ * nearest power of 2); resets the state.  get_shape: kind 0 = tiled (a = 8
+ * rows, b = 8 waves), 1 = march (a = waves, b = rows per chunk). */
+int sfd_visco_plan_autotune(sfd_visco_plan *plan, int trial_steps, float *best_us);
+int sfd_visco_plan_get_shape(const sfd_visco_plan *plan, int *kind, int *a, int *b);
 int sfd_visco_plan_destroy(sfd_visco_plan *plan);
 
 /* ---------------------------------------------------------------- elastic
@@ -393,6 +402,12 @@ int sfd_elastic_plan_fetch(sfd_elastic_plan *plan, float *record_vx, float *reco
 /* Tile width in columns (8, 16 or 32) and waves per workgroup (4, 8 or 16;
  * not every pair is built); 0 = the default. */
 int sfd_elastic_plan_set_tiling(sfd_elastic_plan *plan, int tile_cols, int waves);
+/* Autotuner (no counterpart in the reference), as sfd_acoustic_plan_autotune:
+ * times the seven built tilings over min(trial_steps, delay + nt) steps,
+ * keeps the fastest and caches it for the grid class (mode, nx and nz to the
+ * nearest power of 2); resets the state. */
+int sfd_elastic_plan_autotune(sfd_elastic_plan *plan, int trial_steps, float *best_us);
+int sfd_elastic_plan_get_shape(const sfd_elastic_plan *plan, int *tile_cols, int *waves);
 int sfd_elastic_plan_destroy(sfd_elastic_plan *plan);
 
 /* ---------------------------------------------------------------- files

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -159,6 +160,12 @@ _SIGNATURES = {
                                          ctypes.c_int]),
     "sfd_merge_su": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_char_p, ctypes.c_int]),
+    "sfd_elastic_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_elastic_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.POINTER(ctypes.c_int)]),
+    "sfd_visco_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_visco_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "sfd_context_save_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "sfd_context_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "sfd_context_tuning_size": (ctypes.c_int, [ctypes.c_void_p]),
@@ -236,6 +243,12 @@ class Context:
         if rc != SFD_OK:
             raise SeisfdError(rc, f"sfd_context_create(device={device}) failed")
         self.handle = h
+        # Plans and batches made in this context: closed before the context
+        # (the library requires it).
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        self._children.add(obj)
 
     def check(self, rc: int, what: str):
         if rc != SFD_OK:
@@ -255,6 +268,8 @@ class Context:
 
     def close(self):
         if self.handle:
+            for c in list(self._children):
+                c.close()
             lib().sfd_context_destroy(self.handle)
             self.handle = None
 
@@ -333,6 +348,7 @@ class AcousticPlan:
 
     def __init__(self, shot: AcousticShot, ctx: Optional[Context] = None):
         self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
         self.shot = shot
         self.p = shot.params()
         self.nx, self.nz = np.asarray(shot.vp).shape
@@ -510,6 +526,7 @@ class AcousticBatch:
 
     def __init__(self, shots, ctx: Optional[Context] = None):
         self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
         self.shots = list(shots)
         n = len(self.shots)
         params = (AcousticParamsC * n)()
@@ -657,6 +674,7 @@ class ViscoPlan:
 
     def __init__(self, shot: ViscoShot, ctx: Optional[Context] = None):
         self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
         self.shot = shot
         self.p = shot.params()
         self.nx, self.nz = np.asarray(shot.vp).shape
@@ -670,6 +688,18 @@ class ViscoPlan:
 
     def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
         self.ctx.check(lib().sfd_visco_plan_set_tiling(self.handle, waves, chunk_rows), "visco set_tiling")
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        us = ctypes.c_float()
+        self.ctx.check(lib().sfd_visco_plan_autotune(self.handle, trial_steps, ctypes.byref(us)), "visco autotune")
+        return us.value
+
+    def shape(self):
+        """(kind, a, b): kind 0 = tiled (rows, waves), 1 = march (waves, chunk rows)."""
+        k, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().sfd_visco_plan_get_shape(self.handle, ctypes.byref(k), ctypes.byref(a),
+                                                      ctypes.byref(b)), "visco get_shape")
+        return k.value, a.value, b.value
 
     def run(self) -> float:
         ms = ctypes.c_float()
@@ -794,6 +824,7 @@ class ElasticPlan:
 
     def __init__(self, shot: ElasticShot, ctx: Optional[Context] = None):
         self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
         self.shot = shot
         self.p = shot.params()
         self.nx, self.nz = np.asarray(shot.vp).shape
@@ -806,6 +837,19 @@ class ElasticPlan:
 
     def set_tiling(self, tile_cols: int = 0, waves: int = 0):
         self.ctx.check(lib().sfd_elastic_plan_set_tiling(self.handle, tile_cols, waves), "elastic set_tiling")
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        us = ctypes.c_float()
+        self.ctx.check(lib().sfd_elastic_plan_autotune(self.handle, trial_steps, ctypes.byref(us)),
+                       "elastic autotune")
+        return us.value
+
+    def shape(self):
+        """(tile columns, waves)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().sfd_elastic_plan_get_shape(self.handle, ctypes.byref(a), ctypes.byref(b)),
+                       "elastic get_shape")
+        return a.value, b.value
 
     def run(self) -> float:
         ms = ctypes.c_float()

@@ -421,7 +421,6 @@ const char* mode_name(int mode) {
     }
 }
 
-int log2_round(double n) { return (int)std::lround(std::log2(std::max(n, 1.0))); }
 
 std::string plan_class(const sfd_acoustic_plan* pl) {
     char buf[96];
@@ -1193,28 +1192,6 @@ int sfd_acoustic_batch_destroy(sfd_acoustic_batch* b) {
 // ------------------------------------------------------------------ autotuner
 
 namespace {
-
-// Device time per step of n steps from a zero state: the better of two timed
-// runs after a two-step warm-up.
-template <class Run>
-int time_steps(sfd_context* ctx, int n, Run run, float* us) {
-    float best = 3.4e38f;
-    for (int rep = 0; rep < 3; ++rep) {
-        const int m = rep == 0 ? std::min(n, 2) : n;
-        int rc = run(-1);  // reset the state (outside the timed region)
-        if (rc) return rc;
-        SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        rc = run(m);
-        if (rc) return rc;
-        SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
-        float ms = 0.f;
-        SFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        if (rep > 0) best = std::min(best, ms * 1000.f / m);
-    }
-    *us = best;
-    return SFD_OK;
-}
 
 // Candidates: every built tiled shape (fast arithmetic) and the march kernel
 // at 1, 2 and 4 waves with its automatic chunk.

@@ -104,6 +104,32 @@ hipError_t upload(DeviceBuffer& b, const T* h, size_t n, hipStream_t s) {
     return hipStreamSynchronize(s);
 }
 
+// Autotuner timing: device time per step of n steps from a reset state, the
+// better of two timed runs after a two-step warm-up.  run(-1) resets the
+// state (outside the timed region), run(m) enqueues m steps.
+template <class Run>
+int time_steps(sfd_context* ctx, int n, Run run, float* us) {
+    float best = 3.4e38f;
+    for (int rep = 0; rep < 3; ++rep) {
+        const int m = rep == 0 ? (n < 2 ? n : 2) : n;
+        int rc = run(-1);
+        if (rc) return rc;
+        SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        rc = run(m);
+        if (rc) return rc;
+        SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        SFD_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (rep > 0 && ms * 1000.f / m < best) best = ms * 1000.f / m;
+    }
+    *us = best;
+    return SFD_OK;
+}
+
+// Nearest power of 2 of n, as an exponent (autotuner grid classes).
+int log2_round(double n);
+
 // Staggered Taylor coefficients used when the caller passes none: the
 // reference's own tables (Acoustic_1order_FD.cpp:38-45 for R = 8;
 // Iso_multishot_fd_module_zy_m2.f90:17-26 for R = 4 and 5).

@@ -25,6 +25,8 @@ int hip_check(sfd_context* ctx, hipError_t e, const char* what) {
     return fail(ctx, code, "%s: %s", what, hipGetErrorString(e));
 }
 
+int log2_round(double n) { return (int)std::lround(std::log2(std::max(n, 1.0))); }
+
 bool default_coefs(int radius, float* c) {
     switch (radius) {
     case 8: {

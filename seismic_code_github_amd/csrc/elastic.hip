@@ -215,6 +215,13 @@ int el_set_tiling(sfd_elastic_plan* pl, int tx, int nw) {
     return SFD_OK;
 }
 
+// Autotuner grid class: mode, nx and nz to the nearest power of 2.
+std::string elastic_class(const sfd_elastic_plan* pl) {
+    char buf[96];
+    snprintf(buf, sizeof buf, "elastic:%s:x%d:z%d", pl->abc ? "abc" : "fsbc", log2_round(pl->nx), log2_round(pl->nz));
+    return buf;
+}
+
 int elastic_zero(sfd_elastic_plan* pl) {
     sfd_context* ctx = pl->ctx;
     // Wavefields, strip parts and the pending surface values lie at the front
@@ -224,13 +231,15 @@ int elastic_zero(sfd_elastic_plan* pl) {
     return SFD_OK;
 }
 
-int elastic_enqueue(sfd_elastic_plan* pl) {
+// Enqueues the first nsteps steps (all of them by default).
+int elastic_enqueue(sfd_elastic_plan* pl, int nsteps = -1) {
     sfd_context* ctx = pl->ctx;
     const dim3 grid(pl->args.ntiles), block(64 * pl->nw);
     ElasticStepArgs a = pl->args;
     size_t snap_next = 0;
     const size_t cell_bytes = (size_t)pl->nz * sizeof(float);
-    for (int s = 0; s < pl->nsteps; ++s) {
+    const int ns = nsteps < 0 ? pl->nsteps : std::min(nsteps, pl->nsteps);
+    for (int s = 0; s < ns; ++s) {
         a.step = s;
         a.rec_col = (s >= pl->delay && pl->nrec > 0) ? s - pl->delay : -1;
         hipLaunchKernelGGL(pl->vel, grid, block, 0, ctx->stream, a);
@@ -535,6 +544,9 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
 
     rc = el_set_tiling(pl, kElDefaultTX, kElDefaultNW);
     if (rc) return bail(rc);
+    // A shape the context's autotuner measured for this grid class wins.
+    const auto tuned = ctx->tuning.find(elastic_class(pl));
+    if (tuned != ctx->tuning.end()) (void)el_set_tiling(pl, tuned->second.a, tuned->second.b);
     rc = elastic_zero(pl);
     if (rc) return bail(rc);
     e = hipStreamSynchronize(sm);
@@ -546,6 +558,52 @@ int sfd_elastic_plan_create(sfd_context* ctx, const sfd_elastic_params* p, const
 int sfd_elastic_plan_set_tiling(sfd_elastic_plan* pl, int tile_cols, int waves) {
     if (!pl) return SFD_EINVAL;
     return el_set_tiling(pl, tile_cols ? tile_cols : kElDefaultTX, waves ? waves : kElDefaultNW);
+}
+
+int sfd_elastic_plan_autotune(sfd_elastic_plan* pl, int trial_steps, float* best_us) {
+    if (!pl) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    if (trial_steps < 1) return fail(ctx, SFD_EINVAL, "trial_steps must be >= 1");
+    const int n = std::min(trial_steps, pl->nsteps);
+    if (n < 1) return fail(ctx, SFD_EINVAL, "the plan has no time steps to time");
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    auto run = [&](int m) { return m < 0 ? elastic_zero(pl) : elastic_enqueue(pl, m); };
+    float warm = 0.f;
+    int rc = time_steps(ctx, std::min(pl->nsteps, 4 * n), run, &warm);
+    if (rc) return rc;
+    static const int shapes[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 8}, {32, 16}};
+    float us[7];
+    for (float& u : us) u = 3.4e38f;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int q = 0; q < 7; ++q) {
+            rc = el_set_tiling(pl, shapes[q][0], shapes[q][1]);
+            float t = 0.f;
+            if (!rc) rc = time_steps(ctx, n, run, &t);
+            if (rc) return rc;
+            us[q] = std::min(us[q], t);
+        }
+    int best = 0;
+    for (int q = 1; q < 7; ++q)
+        if (us[q] < us[best]) best = q;
+    rc = el_set_tiling(pl, shapes[best][0], shapes[best][1]);
+    if (!rc) rc = elastic_zero(pl);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    TunedShape t;
+    t.kind = 0;
+    t.a = shapes[best][0];
+    t.b = shapes[best][1];
+    t.us = us[best];
+    ctx->tuning[elastic_class(pl)] = t;
+    if (best_us) *best_us = us[best];
+    return SFD_OK;
+}
+
+int sfd_elastic_plan_get_shape(const sfd_elastic_plan* pl, int* tile_cols, int* waves) {
+    if (!pl || !tile_cols || !waves) return SFD_EINVAL;
+    *tile_cols = pl->tx;
+    *waves = pl->nw;
+    return SFD_OK;
 }
 
 int sfd_elastic_plan_run(sfd_elastic_plan* pl, float* elapsed_ms) {

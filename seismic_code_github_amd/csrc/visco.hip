@@ -371,6 +371,15 @@ int ve_setup(sfd_visco_plan* pl, const sfd_visco_params* p, bool elastic, const 
     return SFD_OK;
 }
 
+// Autotuner grid class: radius, mechanisms, vs = 0 or not, nx and nz to the
+// nearest power of 2.
+std::string visco_class(const sfd_visco_plan* pl) {
+    char buf[96];
+    snprintf(buf, sizeof buf, "visco:R%d:L%d:%s:x%d:z%d", pl->radius, pl->nmech, pl->elastic ? "elastic" : "acoustic",
+             log2_round(pl->nx), log2_round(pl->nz));
+    return buf;
+}
+
 int ve_zero(sfd_visco_plan* pl) {
     sfd_context* ctx = pl->ctx;
     // State (vx, vz, the stresses, the memory fields) lies at the front of
@@ -383,14 +392,16 @@ int ve_zero(sfd_visco_plan* pl) {
     return SFD_OK;
 }
 
-int ve_enqueue(sfd_visco_plan* pl) {
+// Enqueues the first nsteps steps (all of them by default).
+int ve_enqueue(sfd_visco_plan* pl, int nsteps = -1) {
     sfd_context* ctx = pl->ctx;
     ViscoElArgs a = pl->ve;
     const dim3 grid(a.ntiles), block(64 * kVeNW);
     const size_t nb = (size_t)pl->nz * sizeof(float), pb = (size_t)pl->ve_pitch * sizeof(float);
     const bool rec = pl->nrec > 0 && pl->nt > 0;
     size_t snap_next = 0;
-    for (int k = 0; k < pl->nsteps; ++k) {
+    const int ns = nsteps < 0 ? pl->nsteps : std::min(nsteps, pl->nsteps);
+    for (int k = 0; k < ns; ++k) {
         a.step = k;
         // the velocity launch of step k gathers the stresses of step k - 1
         a.rec_col = (rec && k - 1 >= pl->delay) ? k - 1 - pl->delay : -1;
@@ -409,8 +420,8 @@ int ve_enqueue(sfd_visco_plan* pl) {
             ++snap_next;
         }
     }
-    if (rec && pl->nsteps - 1 >= pl->delay) {
-        a.rec_col = pl->nsteps - 1 - pl->delay;
+    if (rec && ns - 1 >= pl->delay) {
+        a.rec_col = ns - 1 - pl->delay;
         hipLaunchKernelGGL(visco_el_gather_kernel, dim3((pl->nrec + 255) / 256), dim3(256), 0, ctx->stream, a);
         SFD_HIP(ctx, hipGetLastError());
     }
@@ -747,6 +758,11 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     // The tiled two-launch step runs by default.
     rc = ve_setup(pl, p, false, hb, hku, hkr, nullptr, nullptr);
     if (rc) return bail(rc);
+    // A march tiling the context's autotuner measured faster for this grid
+    // class wins over the tiled step.
+    const auto tuned = ctx->tuning.find(visco_class(pl));
+    if (tuned != ctx->tuning.end() && tuned->second.kind == 1)
+        (void)sfd_visco_plan_set_tiling(pl, tuned->second.a, tuned->second.b);
     rc = ve_zero(pl);
     if (!rc) rc = visco_zero(pl);
     if (rc) return bail(rc);
@@ -777,6 +793,67 @@ int sfd_visco_plan_set_tiling(sfd_visco_plan* pl, int waves, int chunk_rows) {
     pl->nw = nw;
     pl->chunk = std::min(ch, pl->nx);
     pl->kernel = k;
+    return SFD_OK;
+}
+
+int sfd_visco_plan_autotune(sfd_visco_plan* pl, int trial_steps, float* best_us) {
+    if (!pl) return SFD_EINVAL;
+    sfd_context* ctx = pl->ctx;
+    if (trial_steps < 1) return fail(ctx, SFD_EINVAL, "trial_steps must be >= 1");
+    const int n = std::min(trial_steps, pl->nsteps);
+    if (n < 1) return fail(ctx, SFD_EINVAL, "the plan has no time steps to time");
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    auto run = [&](int m) {
+        if (m < 0) return pl->ve_on ? ve_zero(pl) : visco_zero(pl);
+        return pl->ve_on ? ve_enqueue(pl, m) : visco_enqueue(pl, 0, m);
+    };
+    // Candidates: the tiled step, and with vs = 0 the march kernel at 2 and
+    // 4 waves with its automatic chunk.
+    std::vector<TunedShape> cand(1);
+    cand[0].kind = 0;
+    cand[0].a = kVeTX;
+    cand[0].b = kVeNW;
+    if (!pl->elastic)
+        for (int w : {2, 4}) {
+            TunedShape t;
+            t.kind = 1;
+            t.a = w;
+            cand.push_back(t);
+        }
+    auto apply = [&](const TunedShape& t) {
+        return t.kind == 0 ? sfd_visco_plan_set_tiling(pl, 0, 0) : sfd_visco_plan_set_tiling(pl, t.a, t.b);
+    };
+    float warm = 0.f;
+    int rc = time_steps(ctx, std::min(pl->nsteps, 4 * n), run, &warm);
+    if (rc) return rc;
+    for (auto& t : cand) t.us = 3.4e38f;
+    for (int pass = 0; pass < 2; ++pass)
+        for (TunedShape& t : cand) {
+            rc = apply(t);
+            if (rc) return rc;
+            if (t.kind == 1) t.b = pl->chunk;
+            float us = 0.f;
+            rc = time_steps(ctx, n, run, &us);
+            if (rc) return rc;
+            t.us = std::min(t.us, us);
+        }
+    TunedShape best = cand[0];
+    for (const TunedShape& t : cand)
+        if (t.us < best.us) best = t;
+    rc = apply(best);
+    if (!rc) rc = pl->ve_on ? ve_zero(pl) : visco_zero(pl);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->tuning[visco_class(pl)] = best;
+    if (best_us) *best_us = best.us;
+    return SFD_OK;
+}
+
+int sfd_visco_plan_get_shape(const sfd_visco_plan* pl, int* kind, int* a, int* b) {
+    if (!pl || !kind || !a || !b) return SFD_EINVAL;
+    *kind = pl->ve_on ? 0 : 1;
+    *a = pl->ve_on ? kVeTX : pl->nw;
+    *b = pl->ve_on ? kVeNW : pl->chunk;
     return SFD_OK;
 }
 

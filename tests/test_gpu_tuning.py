@@ -4,7 +4,8 @@ and its JSON cache (sfd_context_save_tuning / sfd_context_load_tuning).
 * The autotuner picks one of the built shapes, records it under the grid
   class, and later plans / batches of that class in the context start with it.
 * Every shape gives bitwise the same result, so a tuned run equals the
-  default run.
+  default run (acoustic, elastic); the visco-acoustic candidates (tiled step,
+  march kernel) agree to rounding.
 * The cache survives a save / load round trip into a fresh context; saving
   merges into an existing file; a malformed file is rejected as a whole and
   entries of another GPU architecture are ignored.
@@ -135,3 +136,42 @@ def test_batch_autotune_keeps_results(boundary):
         assert b2.shape() == (kind, x, y)
         b.close()
         b2.close()
+
+
+def test_elastic_autotune_keeps_results():
+    shot = W.elastic_par_903x603(nt=120)
+    print("case: elastic autotune", flush=True)
+    with sfd.Context(0) as ctx:
+        base = sfd.ElasticPlan(shot, ctx)
+        base.run()
+        r0 = base.fetch()
+        pl = sfd.ElasticPlan(shot, ctx)
+        us = pl.autotune(10)
+        tx, nw = pl.shape()
+        print(f"tuned: {tx} x {nw} {us:.2f} us/step", flush=True)
+        assert (tx, nw) in TILED | {(8, 4), (8, 8)} and ctx.tuning_size() == 1
+        pl.run()
+        r1 = pl.fetch()
+        for a, b in zip(r0, r1):
+            if a is not None:
+                np.testing.assert_array_equal(a, b)
+        assert sfd.ElasticPlan(shot, ctx).shape() == (tx, nw)
+
+
+def test_visco_autotune():
+    shot = W.visco_1001x2001(nt=200, scale=8)
+    print("case: visco autotune", flush=True)
+    with sfd.Context(0) as ctx:
+        base = sfd.ViscoPlan(shot, ctx)
+        base.run()
+        r0 = base.fetch()[0]
+        pl = sfd.ViscoPlan(shot, ctx)
+        us = pl.autotune(10)
+        kind, a, b = pl.shape()
+        print(f"tuned: kind {kind} ({a}, {b}) {us:.2f} us/step", flush=True)
+        assert (kind, a, b) == (0, 8, 8) or (kind == 1 and a in (2, 4) and b > 0)
+        pl.run()
+        r1 = pl.fetch()[0]
+        e = float(np.linalg.norm(r1.astype(np.float64) - r0) / np.linalg.norm(r0))
+        assert e <= 1e-5
+        assert sfd.ViscoPlan(shot, ctx).shape() == (kind, a, b)
