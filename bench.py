@@ -131,6 +131,8 @@ def visco_leg(nt_err, nt, no_cpu):
     L = shot.nmech
     nsteps = shot.ref_delay() + shot.nt
     with sfd.ViscoPlan(shot) as pl:
+        tuned_us = pl.autotune(20)
+        vshape = pl.shape()
         pl.run()
         ms = min(pl.run() for _ in range(2))
         rec, _ = pl.fetch()
@@ -147,7 +149,8 @@ def visco_leg(nt_err, nt, no_cpu):
            "Mcell_per_s": round(nx * nz * nsteps / ms / 1e3, 1),
            "bytes_per_cell_update": 36 + 8 * L,
            "achieved_GBs": round(bytes_step * nsteps / (ms / 1e3) / 1e9, 1),
-           "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4)}
+           "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4),
+           "launch_shape": shape_dict(*vshape, tuned_us)}
     # The same shot with vs > 0 (vs = vp / a seeded smooth ratio in 1.6..2.2,
     # qs = 0.8 qp): the visco-elastic kernels, two launches per step.
     vs = (shot.vp / W.smooth_random_model(nx, nz, 20.0, 1.6, 2.2, 7)).astype(np.float32)
@@ -210,6 +213,8 @@ def elastic_leg(nt_err, nt, no_cpu):
     nx, nz = shot.vp.shape
     nsteps = shot.ref_delay() + shot.nt
     with sfd.ElasticPlan(shot) as pl:
+        tuned_us = pl.autotune(20)
+        eshape = pl.shape()
         pl.run()
         ms = min(pl.run() for _ in range(3))
         rvx, rvz, _ = pl.fetch()
@@ -230,7 +235,9 @@ def elastic_leg(nt_err, nt, no_cpu):
            "us_per_step": round(ms * 1e3 / nsteps, 2),
            "bytes_per_step": bytes_step,
            "achieved_GBs": round(bytes_step * nsteps / (ms / 1e3) / 1e9, 1),
-           "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4)}
+           "frac_of_8TBs": round(bytes_step * nsteps / (ms / 1e3) / 8e12, 4),
+           "launch_shape": {"kernel": "tiled two-launch step", "tile_cols": eshape[0], "waves": eshape[1],
+                            "autotuned_us_per_step": round(tuned_us, 2)}}
     tpath = os.path.join(ROOT, "profiles", "elastic_step_903x603_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
