@@ -213,7 +213,7 @@ def elastic_leg(nt_err, nt, no_cpu):
     nx, nz = shot.vp.shape
     nsteps = shot.ref_delay() + shot.nt
     with sfd.ElasticPlan(shot) as pl:
-        tuned_us = pl.autotune(20)
+        tuned_us = pl.autotune(60)
         eshape = pl.shape()
         pl.run()
         ms = min(pl.run() for _ in range(3))
