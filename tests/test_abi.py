@@ -61,3 +61,28 @@ def test_elastic_params_struct_layout():
     # sizeof / offsetof of sfd_elastic_params as gcc lays it out on x86-64
     assert ctypes.sizeof(C) == 120
     assert (C.itopo.offset, C.src_radius.offset, C.snap_steps.offset, C.snap_fields.offset) == (32, 64, 96, 104)
+
+
+def test_every_struct_mirror_matches_the_header(tmp_path):
+    """gcc compiles a probe against include/seisfd.h printing sizeof and the
+    offset of every field the ctypes mirrors declare."""
+    import subprocess
+    import seismic_code_github_amd as sfd
+    mirrors = {"sfd_acoustic_params": sfd.AcousticParamsC, "sfd_visco_params": sfd.ViscoParamsC,
+               "sfd_elastic_params": sfd.ElasticParamsC, "sfd_q_par": sfd.QParC, "sfd_iso_par": sfd.IsoParC}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "seisfd.h"', 'int main(void) {']
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    for cname, cls in mirrors.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
