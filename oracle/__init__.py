@@ -454,3 +454,96 @@ def elastic(vp, vs, rho, itopo, *, src, rec, nsteps, delay=0, dx=5.0, dz=5.0, dt
     if rc != 0:
         raise ValueError(f"oracle_elastic returned {rc}")
     return {"rec_vx": rvx, "rec_vz": rvz, "snaps": snaps}
+
+
+# ---------------------------------------------------------------------- VTI
+class _VTArgs(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nsteps", ctypes.c_int), ("delay", ctypes.c_int),
+        ("nt", ctypes.c_int), ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float),
+        ("c2", ctypes.POINTER(ctypes.c_float)), ("c1", ctypes.POINTER(ctypes.c_float)),
+        ("nl", ctypes.c_int), ("nr", ctypes.c_int), ("nu", ctypes.c_int), ("nd", ctypes.c_int),
+        ("r", ctypes.c_float),
+        ("vz", ctypes.POINTER(ctypes.c_float)), ("eps", ctypes.POINTER(ctypes.c_float)),
+        ("dl", ctypes.POINTER(ctypes.c_float)),
+        ("wavelet", ctypes.POINTER(ctypes.c_float)), ("nwav", ctypes.c_int),
+        ("sx", ctypes.c_int), ("sz", ctypes.c_int),
+        ("nrec", ctypes.c_int), ("rx", ctypes.POINTER(ctypes.c_int)), ("rz", ctypes.POINTER(ctypes.c_int)),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+_vt = None
+
+
+def _vtlib():
+    global _vt
+    if _vt is None:
+        L = lib()
+        fp, dp, ip = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        L.oracle_vti_f32.argtypes = [ctypes.POINTER(_VTArgs), fp, fp]
+        L.oracle_vti_f64.argtypes = [ctypes.POINTER(_VTArgs), dp, dp]
+        L.oracle_vti_coefficients_f32.argtypes = [ctypes.c_int, ctypes.c_int, fp]
+        L.oracle_vti_wavelet_f32.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int, fp]
+        L.oracle_vti_window.argtypes = [fp, fp, fp] + [ctypes.c_int] * 9 + [fp, fp, fp]
+        _vt = L
+    return _vt
+
+
+def vti_coefficients(kind: int, k: int = 5) -> np.ndarray:
+    """coefficient_2nd (kind 2) / coefficient_1st (kind 1), FP32 as the reference."""
+    out = np.zeros(k, dtype=np.float32)
+    if _vtlib().oracle_vti_coefficients_f32(kind, k, _ptr(out)) != 0:
+        raise ValueError("bad coefficient request")
+    return out
+
+
+def vti_wavelet(dt: float, f0: float) -> np.ndarray:
+    """The VTI driver's wavelet (nwt samples, it = 1..nwt)."""
+    L = _vtlib()
+    n = L.oracle_vti_wavelet_f32(dt, f0, 0, ctypes.POINTER(ctypes.c_float)())
+    w = np.zeros(n, dtype=np.float32)
+    L.oracle_vti_wavelet_f32(dt, f0, n, _ptr(w))
+    return w
+
+
+def vti_window(vz, eps, delta, xmin, rng, pml, nsz0):
+    """get_currshot_parameters + the isotropic top rows; model [nxv, nzv]."""
+    vz, eps, delta = (np.ascontiguousarray(a, dtype=np.float32) for a in (vz, eps, delta))
+    nxv, nzv = vz.shape
+    nl, nr, nu, nd = pml
+    shape = (rng + nl + nr - 2, nzv + nu + nd - 2)
+    out = [np.zeros(shape, dtype=np.float32) for _ in range(3)]
+    rc = _vtlib().oracle_vti_window(_ptr(vz), _ptr(eps), _ptr(delta), nxv, nzv, xmin, rng, nl, nr, nu, nd, nsz0,
+                                    *(_ptr(o) for o in out))
+    if rc != 0:
+        raise ValueError("bad window")
+    return tuple(out)
+
+
+def vti(vz, eps, delta, *, src, rec, nsteps, delay=0, dx=5.0, dz=5.0, dt=5e-4, pml=(0, 0, 0, 0), r=1e-3,
+        wavelet=None, c2=None, c1=None, precision="f32", nthreads=1, state=False):
+    """Run the VTI oracle (oracle/vti_body.h).  Fields [nx, nz] (z fastest);
+    pml = (left, right, top, bottom) widths; wavelet samples are added to p
+    at src before steps 0, 1, ...  Returns the record [nrec, nsteps - delay]
+    (and with state=True also the final p, q)."""
+    L = _vtlib()
+    vz, eps, delta = (np.ascontiguousarray(a, dtype=np.float32) for a in (vz, eps, delta))
+    nx, nz = vz.shape
+    c2 = vti_coefficients(2) if c2 is None else np.ascontiguousarray(c2, dtype=np.float32)
+    c1 = vti_coefficients(1) if c1 is None else np.ascontiguousarray(c1, dtype=np.float32)
+    w = np.zeros(0, dtype=np.float32) if wavelet is None else np.ascontiguousarray(wavelet, dtype=np.float32)
+    rx = np.ascontiguousarray(rec[0], dtype=np.int32)
+    rz = np.ascontiguousarray(rec[1], dtype=np.int32)
+    nt = max(nsteps - delay, 0)
+    args = _VTArgs(nx, nz, nsteps, delay, nt, dx, dz, dt, _ptr(c2), _ptr(c1), *pml, r, _ptr(vz), _ptr(eps),
+                   _ptr(delta), _ptr(w), w.size, int(src[0]), int(src[1]), rx.size, _ptr(rx, ctypes.c_int),
+                   _ptr(rz, ctypes.c_int), nthreads)
+    dtype, ct = (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
+    record = np.zeros((rx.size, nt), dtype=dtype)
+    st = np.zeros((2, nx, nz), dtype=dtype)
+    fn = L.oracle_vti_f32 if precision == "f32" else L.oracle_vti_f64
+    rc = fn(ctypes.byref(args), _ptr(record, ct), _ptr(st, ct))
+    if rc != 0:
+        raise ValueError(f"oracle_vti returned {rc}")
+    return (record, st[0], st[1]) if state else record

@@ -186,6 +186,41 @@ int oracle_elastic_stagger_f64(int nx, int nz, const float *vp, const float *vs,
 /* modipapml's gauss_filter_2d (pub:592-634), serial order, in place. */
 int oracle_elastic_smooth_f32(int nx, int nz, int rgau, const int *itopo, float *u);
 
+/* VTI qP-acoustic propagator of de-multiple/convolution_srm/vti_qa_2d_c3_demo1:
+ * p (reference u) and q fields, 2nd order in time, 10th-order second
+ * derivatives, split PML on the four sides (extrapolation_one_step_1 / _2,
+ * VTI_multishot_qacoustic_fd_subroutine_c3.f90:432-1258). */
+typedef struct oracle_vti_args {
+    int nx, nz;              /* padded shot grid (reference currshot_range_all, nz) */
+    int nsteps;              /* steps run (reference nt + nint((nwt - 1) / 2)) */
+    int delay;               /* record sample = step - delay (reference iit = it - nint((nwt-1)/2)) */
+    int nt;                  /* recorded samples */
+    float dx, dz, dt;
+    const float *c2;         /* [5] second-derivative weights (fd_coe = fd_coe_pml_2nd) */
+    const float *c1;         /* [5] first-derivative weights (fd_coe_pml_1st) */
+    int nl, nr, nu, nd;      /* PML widths: left, right, top, bottom (nx_bound_l, _r, nz_bound_u, _d) */
+    float r;                 /* target reflection (module constant R); 0 = 1e-3 */
+    const float *vz, *eps, *del;  /* [nx*nz] */
+    const float *wavelet;    /* [nwav]: added to p at the source before steps 0 .. nwav-1 */
+    int nwav;
+    int sx, sz;              /* source cell */
+    int nrec;
+    const int *rx, *rz;      /* receiver cells: p after each step */
+    int nthreads;            /* OpenMP threads over x columns (1 = serial) */
+} oracle_vti_args;
+
+/* record [nrec][nt]; state (may be NULL) [2][nx*nz]: final p and q. */
+int oracle_vti_f32(const oracle_vti_args *a, float *record, float *state);
+int oracle_vti_f64(const oracle_vti_args *a, double *record, double *state);
+/* coefficient_2nd (kind 2) / coefficient_1st (kind 1) with k weights. */
+int oracle_vti_coefficients_f32(int kind, int k, float *out);
+/* The driver's wavelet: writes min(nwt, nmax) samples, returns nwt. */
+int oracle_vti_wavelet_f32(float dt, float f0, int nmax, float *w);
+/* get_currshot_parameters plus the isotropic top rows; outputs
+ * [range + nl + nr - 2][nzv + nu + nd - 2]. */
+int oracle_vti_window(const float *vz, const float *eps, const float *del, int nxv, int nzv, int xmin, int range,
+                      int nl, int nr, int nu, int nd, int nsz0, float *ovz, float *oeps, float *odel);
+
 #ifdef __cplusplus
 }
 #endif
