@@ -267,6 +267,83 @@ def elastic_leg(nt_err, nt, no_cpu):
     return out
 
 
+def vti_bytes_per_step(nx, nz, pml):
+    """Algorithmic HBM bytes of one step of one VTI shot.  Interior cell:
+    read p, q (current), p, q (old), vz, eps, delta, write p, q = 36 B.  Left /
+    right strip cell: no old p, q read (28 B) plus the split parts: U1, Ut,
+    U3, Q1, Qt two levels and U2, Q2 one level read, seven written (76 B).
+    Top / bottom strip cell: no old p read (32 B) plus U1, Ut, U3 two levels,
+    U2 one level read, four written (44 B)."""
+    nl, nr, nu, nd = pml
+    xs = (nl + nr) * nz
+    zs = (nx - nl - nr) * (nu + nd)
+    return 36 * (nx * nz - xs - zs) + (28 + 76) * xs + (32 + 44) * zs
+
+
+def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False):
+    """VTI qP-acoustic survey of the reference card
+    (vti_qa_2d_modelling_new.par): nshot shots of the 698 x 348 padded grid,
+    nt recorded samples + 140 delay steps, all shots advanced together by one
+    launch per step.  Parity: the first nt_err recorded samples of one shot
+    against the FP64 oracle; CPU baseline: the FP32 oracle on that prefix."""
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    V, E, D, kw = W.vti_card_survey(nshot=nshot, nt=nt)
+    _, nx, nz = V.shape
+    with sfd.VtiPlan(V, E, D, **kw) as pl:
+        tuned_us = None if no_tune else pl.autotune(20)
+        shape = pl.shape()
+        pl.run()
+        ms = pl.run()
+        rec = pl.fetch()
+    assert np.isfinite(rec).all()
+    nsteps = 140 + nt
+    bstep = vti_bytes_per_step(nx, nz, kw["pml"])
+    s = ms / 1e3
+    out = {"workload": f"VTI qP-acoustic (p, q; 2nd order in time, 10th order in space, split PML), "
+                       f"vti_qa_2d_modelling_new.par survey: {nshot} shots of {nx}x{nz} (600 window columns "
+                       f"+ 50-cell PML left, right, below), dt=0.5 ms, nt={nt} (+140 delay steps), 30 Hz",
+           "shots": nshot, "cells_per_shot": nx * nz, "steps": nsteps, "ms_per_survey": round(ms, 1),
+           "shots_per_s": round(nshot / s, 2),
+           "Mcell_per_s": round(nx * nz * nshot * nsteps / s / 1e6, 1),
+           "us_per_step": round(ms * 1e3 / nsteps, 2),
+           "bytes_per_shot_step": bstep,
+           "achieved_GBs": round(bstep * nshot * nsteps / s / 1e9, 1),
+           "frac_of_8TBs": round(bstep * nshot * nsteps / s / 8e12, 4),
+           "launch_shape": {"kernel": "one fused launch per step, all shots", "tile_cols": shape[0],
+                            "waves": shape[1], "autotuned_us_per_step": None if tuned_us is None else round(tuned_us, 2)}}
+    tpath = os.path.join(ROOT, "profiles", "vti_step_card_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            out["traffic"] = json.load(f).get("hbm_bytes_per_shot_step")
+    if not no_cpu:
+        import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        V1, E1, D1, kw1 = W.vti_card_survey(nshot=1, nt=nt_err, first=150)
+        with sfd.VtiPlan(V1, E1, D1, **kw1) as pl:
+            pl.run()
+            got = pl.fetch()[0]
+        w = oracle.vti_wavelet(kw1["dt"], kw1["f0"])
+        args = dict(src=kw1["src"], rec=kw1["rec"], nsteps=140 + nt_err, delay=140, dx=kw1["dx"], dz=kw1["dz"],
+                    dt=kw1["dt"], pml=kw1["pml"], wavelet=w, nthreads=cores)
+        o64 = oracle.vti(V1[0], E1[0], D1[0], precision="f64", **args)
+        o32 = oracle.vti(V1[0], E1[0], D1[0], precision="f32", **args)
+        nrm = np.linalg.norm(o64)
+        out["fp64_rel_l2"] = float(np.linalg.norm(got - o64) / nrm)
+        out["fp32_oracle_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
+        out["error_sample"] = f"first {nt_err} recorded samples ({args['nsteps']} steps) of shot 150"
+        # CPU baseline: one whole shot of the survey through the FP32 oracle
+        full = dict(args, nsteps=nsteps, delay=140)
+        t0 = time.perf_counter()
+        oracle.vti(V1[0], E1[0], D1[0], precision="f32", **full)
+        dt32 = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(nx * nz * nsteps / dt32 / 1e6, 2), "unit": "Mcell-updates/s",
+                               "cores": cores, "kind": "port", "shots_per_s": round(1 / dt32, 4),
+                               "sample": f"one whole shot ({nsteps} steps) of the survey, FP32 oracle (the "
+                                         f"reference's split PML parts), OpenMP over x columns, {dt32:.1f} s"}
+    return out
+
+
 def gradient_leg(nt):
     """Adjoint gradient: 16 shots on 801 x 801 + 40-cell sponge, 10 Hz,
     checkpoints every 50 steps: forward + checkpointed recomputation + adjoint
@@ -338,7 +415,11 @@ def main():
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--elastic-err-nt", type=int, default=100)
     ap.add_argument("--elastic-nt", type=int, default=3001)
-    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic"], default=None,
+    ap.add_argument("--no-vti", action="store_true")
+    ap.add_argument("--vti-shots", type=int, default=300)
+    ap.add_argument("--vti-nt", type=int, default=10000)
+    ap.add_argument("--vti-err-nt", type=int, default=600)
+    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic", "vti"], default=None,
                     help="run one secondary leg alone (profiling) and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
@@ -370,7 +451,9 @@ def main():
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
                "grad": lambda: gradient_leg(args.grad_steps),
                "batch": lambda: batched_leg(args.batch_steps, args.no_tune),
-               "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)}[args.only]
+               "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu),
+               "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,
+                                      args.no_tune)}[args.only]
         print(json.dumps(leg()), flush=True)
         return
 
@@ -446,6 +529,9 @@ def main():
     if not args.no_elastic:
         log("elastic leg ...")
         out["elastic"] = elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)
+    if not args.no_vti:
+        log("vti leg ...")
+        out["vti"] = vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu, args.no_tune)
     if not args.no_grad:
         log("gradient leg ...")
         out["gradient"] = gradient_leg(args.grad_steps)

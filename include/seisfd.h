@@ -410,6 +410,72 @@ int sfd_elastic_plan_autotune(sfd_elastic_plan *plan, int trial_steps, float *be
 int sfd_elastic_plan_get_shape(const sfd_elastic_plan *plan, int *tile_cols, int *waves);
 int sfd_elastic_plan_destroy(sfd_elastic_plan *plan);
 
+/* ---------------------------------------------------------------- VTI
+ * VTI qP-acoustic propagator (p and q fields, 2nd order in time, 10th-order
+ * second derivatives, split PML on the four sides): the drop-in for the time
+ * loop of de-multiple/convolution_srm/vti_qa_2d_c3_demo1, single_shot_modelling
+ * (VTI_multishot_qacoustic_fd_main_c3.f90:429-520) with its per-step routines
+ * extrapolation_one_step_1 / _2 (VTI_multishot_qacoustic_fd_subroutine_c3.f90:432-1258).
+ * Grid = the reference's padded shot grid currshot_range_all x nz
+ * (get_currshot_parameters, or sfd_vti_window), [nx][nz] z fastest, 0-based:
+ * a Fortran caller passes its (nx, nz) arrays transposed.  A plan holds
+ * nshot shots on grids of this one shape (the reference's shots all have
+ * it), each with its own model, advanced together. */
+typedef struct sfd_vti_plan sfd_vti_plan;
+
+typedef struct sfd_vti_params {
+    int nx, nz;            /* padded shot grid (reference currshot_range_all, nz) */
+    int nt;                /* recorded samples (reference nt) */
+    int delay;             /* steps run before the first recorded one; < 0 = the reference's
+                              nint((nwt - 1) / 2), nwt = nint(2 * 0.07 / dt) + 1 (main:277, 469) */
+    float dx, dz, dt;
+    float f0;              /* Hz: the built-in wavelet */
+    int pml_xlo, pml_xhi, pml_zlo, pml_zhi;  /* reference nx_bound_l, nx_bound_r, nz_bound_u, nz_bound_d */
+    float pml_r;           /* reference R (module constant 1e-3); 0 = 1e-3 */
+    const float *c2;       /* [5] second-derivative weights; NULL = coefficient_2nd(10) as the
+                              reference solves it (fd_coe = fd_coe_pml_2nd) */
+    const float *c1;       /* [5] first-derivative weights; NULL = coefficient_1st(9) (fd_coe_pml_1st) */
+    int sx, sz;            /* source cell (reference currshot_nsx - 1, currshot_nsz - 1) */
+    int nrec;
+    const int *rx, *rz;    /* receiver cells, the same in every shot (reference: row
+                              nrz0 + nz_bound_u, or row 6 when that is above it, of every window column) */
+} sfd_vti_params;
+
+/* vz, eps, delta: [nshot][nx*nz].  wavelet [delay + nt] or NULL = the
+ * reference's exp(-x^2)(1 - 2 x^2), x = pi f0 (it - 0.07/dt) dt for it = 1..nwt
+ * (main:433-437), added to p at the source cell before each step.
+ *   record [nshot][nrec][nt]: p at the receivers after steps delay .. delay+nt-1
+ *   (reference record(iit, :), main:469-476). */
+int sfd_vti_forward(sfd_context *ctx, const sfd_vti_params *p, int nshot, const float *vz, const float *eps,
+                    const float *delta, const float *wavelet, float *record);
+int sfd_vti_plan_create(sfd_context *ctx, const sfd_vti_params *p, int nshot, const float *vz, const float *eps,
+                        const float *delta, const float *wavelet, sfd_vti_plan **plan);
+/* Zeroes the fields and runs all delay + nt steps; elapsed_ms (may be NULL)
+ * receives the device time measured with HIP events on the plan's stream. */
+int sfd_vti_plan_run(sfd_vti_plan *plan, float *elapsed_ms);
+/* record as above; state (may be NULL) [nshot][2][nx*nz]: p and q after the run. */
+int sfd_vti_plan_fetch(sfd_vti_plan *plan, float *record, float *state);
+/* Tile width in columns (8, 16 or 32) and waves per workgroup (4, 8 or 16;
+ * not every pair is built); 0 = the default. */
+int sfd_vti_plan_set_tiling(sfd_vti_plan *plan, int tile_cols, int waves);
+/* Autotuner (no counterpart in the reference), as sfd_elastic_plan_autotune:
+ * cached for the grid class (nx, nz and nshot to the nearest power of 2);
+ * resets the state. */
+int sfd_vti_plan_autotune(sfd_vti_plan *plan, int trial_steps, float *best_us);
+int sfd_vti_plan_get_shape(const sfd_vti_plan *plan, int *tile_cols, int *waves);
+int sfd_vti_plan_destroy(sfd_vti_plan *plan);
+/* Host helpers (no device or context needed):
+ * coefficient_2nd (kind 2) / coefficient_1st (kind 1) with k <= 8 weights, solved in
+ * single precision as sub:1264-1472 does; the driver's wavelet (returns nwt, writes
+ * min(nwt, nmax) samples); get_currshot_parameters (sub:268-338) plus the
+ * isotropic rows 1..nsz0+nz_bound_u+6 (main:422-425): window column i (1-based)
+ * takes model column clamp(i + xmin, 1, nxv), outputs
+ * [range + nl + nr - 2][nzv + nu + nd - 2] (nr, nd >= 2). */
+int sfd_vti_coefficients(int kind, int k, float *out);
+int sfd_vti_wavelet(float dt, float f0, int nmax, float *w);
+int sfd_vti_window(const float *vz, const float *eps, const float *delta, int nxv, int nzv, int xmin, int range,
+                   int nl, int nr, int nu, int nd, int nsz0, float *ovz, float *oeps, float *odelta);
+
 /* ---------------------------------------------------------------- files
  * The Fortran drivers' files on either side of the propagators (host code;
  * no device or context needed).  Parameter cards: a comment line before

@@ -98,6 +98,18 @@ class ElasticParamsC(ctypes.Structure):
     ]
 
 
+class VtiParamsC(ctypes.Structure):
+    """Mirror of ``struct sfd_vti_params``."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("nt", ctypes.c_int), ("delay", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float), ("f0", ctypes.c_float),
+        ("pml_xlo", ctypes.c_int), ("pml_xhi", ctypes.c_int), ("pml_zlo", ctypes.c_int), ("pml_zhi", ctypes.c_int),
+        ("pml_r", ctypes.c_float), ("c2", _FP), ("c1", _FP),
+        ("sx", ctypes.c_int), ("sz", ctypes.c_int),
+        ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
+    ]
+
+
 class QParC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_char * 256) for n in ("vp_fn", "vs_fn", "rho_fn", "qfp_fn", "qfs_fn",
                                                    "shot_fn1", "shot_fn2", "shot_fn3", "shot_fn4")] + [
@@ -161,6 +173,19 @@ _SIGNATURES = {
     "sfd_merge_su": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_char_p, ctypes.c_int]),
     "sfd_elastic_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_vti_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(VtiParamsC), ctypes.c_int, _FP, _FP, _FP, _FP,
+                                       _FP]),
+    "sfd_vti_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(VtiParamsC), ctypes.c_int, _FP, _FP, _FP,
+                                           _FP, ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_vti_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_vti_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
+    "sfd_vti_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_vti_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_vti_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, _IP, _IP]),
+    "sfd_vti_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_vti_coefficients": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _FP]),
+    "sfd_vti_wavelet": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP]),
+    "sfd_vti_window": (ctypes.c_int, [_FP, _FP, _FP] + [ctypes.c_int] * 9 + [_FP, _FP, _FP]),
     "sfd_elastic_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                                   ctypes.POINTER(ctypes.c_int)]),
     "sfd_visco_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
@@ -901,6 +926,119 @@ def elastic_forward(shot: ElasticShot, ctx: Optional[Context] = None):
                                         _ptr(rvx) if has else None, _ptr(rvz) if has else None,
                                         _ptr(snaps) if p.nsnap else None), "sfd_elastic_forward")
     return rvx, rvz, snaps
+
+
+# ---------------------------------------------------------------- VTI
+
+def vti_coefficients(kind: int, k: int = 5) -> np.ndarray:
+    """coefficient_2nd (kind 2) / coefficient_1st (kind 1), solved as the reference does."""
+    out = np.zeros(k, dtype=np.float32)
+    rc = lib().sfd_vti_coefficients(kind, k, _ptr(out))
+    if rc != 0:
+        raise SeisfdError(rc, "sfd_vti_coefficients: bad request")
+    return out
+
+
+def vti_wavelet(dt: float, f0: float) -> np.ndarray:
+    """The VTI driver's wavelet, nwt = nint(0.14 / dt) + 1 samples."""
+    n = lib().sfd_vti_wavelet(dt, f0, 0, None)
+    if n < 0:
+        raise SeisfdError(n, "sfd_vti_wavelet: bad dt")
+    w = np.zeros(n, dtype=np.float32)
+    lib().sfd_vti_wavelet(dt, f0, n, _ptr(w))
+    return w
+
+
+def vti_window(vz, eps, delta, xmin: int, rng: int, pml, nsz0: int):
+    """get_currshot_parameters plus the isotropic top rows (host).  Model
+    [nxv, nzv] z fastest; pml = (left, right, top, bottom) cells.  Returns the
+    padded shot grid's (vz, eps, delta), each [rng + nl + nr - 2, nzv + nu + nd - 2]."""
+    vz, eps, delta = _f32(vz), _f32(eps), _f32(delta)
+    nxv, nzv = vz.shape
+    nl, nr, nu, nd = (int(v) for v in pml)
+    shape = (rng + nl + nr - 2, nzv + nu + nd - 2)
+    out = [np.zeros(shape, dtype=np.float32) for _ in range(3)]
+    rc = lib().sfd_vti_window(_ptr(vz), _ptr(eps), _ptr(delta), nxv, nzv, int(xmin), int(rng), nl, nr, nu, nd,
+                              int(nsz0), *(_ptr(o) for o in out))
+    if rc != 0:
+        raise SeisfdError(rc, "sfd_vti_window: bad window")
+    return tuple(out)
+
+
+class VtiPlan:
+    """``sfd_vti_plan``: nshot shots on grids of one shape, models resident on
+    the device.  vz, eps, delta [nshot, nx, nz] (or [nx, nz] for one shot);
+    pml = (left, right, top, bottom) cells; rec = (rx, rz) cell lists."""
+
+    def __init__(self, vz, eps, delta, *, dx, dz, dt, nt, src, rec, pml=(0, 0, 0, 0), f0=30.0, delay=None,
+                 pml_r=0.0, wavelet=None, c2=None, c1=None, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
+        vz, eps, delta = (_f32(a) for a in (vz, eps, delta))
+        if vz.ndim == 2:
+            vz, eps, delta = vz[None], eps[None], delta[None]
+        if not (vz.shape == eps.shape == delta.shape):
+            raise ValueError("vz, eps and delta must have one shape")
+        self.nshot, self.nx, self.nz = vz.shape
+        self._rx = np.ascontiguousarray(rec[0], dtype=np.int32)
+        self._rz = np.ascontiguousarray(rec[1], dtype=np.int32)
+        self._c2 = None if c2 is None else _f32(c2)
+        self._c1 = None if c1 is None else _f32(c1)
+        self._w = None if wavelet is None else _f32(wavelet)
+        self._arr = (vz, eps, delta)
+        self.p = VtiParamsC(self.nx, self.nz, int(nt), -1 if delay is None else int(delay), dx, dz, dt, f0,
+                            *(int(v) for v in pml), pml_r, _ptr(self._c2), _ptr(self._c1), int(src[0]), int(src[1]),
+                            self._rx.size, _ptr(self._rx, ctypes.c_int), _ptr(self._rz, ctypes.c_int))
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_vti_plan_create(self.ctx.handle, ctypes.byref(self.p), self.nshot, _ptr(vz),
+                                                 _ptr(eps), _ptr(delta), _ptr(self._w), ctypes.byref(h)),
+                       "sfd_vti_plan_create")
+        self.handle = h
+
+    def set_tiling(self, tile_cols: int = 0, waves: int = 0):
+        self.ctx.check(lib().sfd_vti_plan_set_tiling(self.handle, tile_cols, waves), "vti set_tiling")
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        us = ctypes.c_float()
+        self.ctx.check(lib().sfd_vti_plan_autotune(self.handle, trial_steps, ctypes.byref(us)), "vti autotune")
+        return us.value
+
+    def shape(self):
+        """(tile columns, waves)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().sfd_vti_plan_get_shape(self.handle, ctypes.byref(a), ctypes.byref(b)), "vti get_shape")
+        return a.value, b.value
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_vti_plan_run(self.handle, ctypes.byref(ms)), "sfd_vti_plan_run")
+        return ms.value
+
+    def fetch(self, state: bool = False):
+        """record [nshot, nrec, nt] (and with state=True, p and q [nshot, 2, nx, nz])."""
+        nrec, nt = self.p.nrec, self.p.nt
+        record = np.zeros((self.nshot, nrec, nt), dtype=np.float32)
+        st = np.zeros((self.nshot, 2, self.nx, self.nz), dtype=np.float32) if state else None
+        self.ctx.check(lib().sfd_vti_plan_fetch(self.handle, _ptr(record) if nrec and nt else None,
+                                                _ptr(st) if state else None), "sfd_vti_plan_fetch")
+        return (record, st) if state else record
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_vti_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- files

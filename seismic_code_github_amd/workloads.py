@@ -32,6 +32,19 @@ geometries:
   8.1e8, 1.1025e9, 1.44e9 Pa below model rows 0, 101, 201, 301), taken as
   rho = 2250 kg/m^3 and vs = 500, 600, 700, 800 m/s, with vp = 2 vs; flat
   topography (surface on padded row 0).
+* ``vti_card_survey`` — the VTI qP-acoustic program's own parameter card,
+  de-multiple/convolution_srm/vti_qa_2d_c3_demo1/vti_qa_2d_modelling_new.par:
+  a 300 x 300 model at dx = dz = 5 m, 300 shots from x = 0 every 5 m, offsets
+  -1500 .. 1495 m (600 window columns), PML 250 m (50 cells) left, right and
+  below and none above, so the padded shot grid is 698 x 348; nt = 10000
+  recorded samples after the 140 delay steps of the 281-sample wavelet
+  (dt = 0.5 ms, f0 = 30 Hz); source on the top row at the window centre
+  (column 349), receivers on row 6 of every window column (the card's rz0 = 0
+  is above row 6; the reference then leaves the first sample 0).  The card's
+  model files are absent: vz, eps and delta / eps are seeded smooth noise
+  (seeds 12, 13, 14, sigma = 15 cells) mapped to 1800-3500 m/s, 0-0.25 and
+  0.2-0.6 (delta <= eps: the pseudo-acoustic system is unstable where
+  eps < delta), each shot windowed by get_currshot_parameters (sfd.vti_window).
 * ``smooth_random_2001x4001`` — 2001 (z) x 4001 (x) interior + 50-cell C-PML
   (4101 x 2101 stored), dx=dz=10 m, dt=0.8 ms, nt=6000, velocity = seeded
   Gaussian-smoothed noise (sigma = 20 cells) mapped to 1500-4500 m/s, 20 Hz
@@ -45,7 +58,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import AcousticShot, ElasticShot, ViscoShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model
+from . import AcousticShot, ElasticShot, ViscoShot, FIELD_P, FIELD_VX, FIELD_VZ, window_model, vti_window
 
 PI_REF = 3.1415926
 
@@ -252,3 +265,32 @@ def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
               dx=dx, dz=dx, dt=1e-3, f0=25.0, sponge=(npxn, npxp, npzp), src_radius=50, delay=48)
     kw.update(overrides)
     return ElasticShot(**kw)
+
+
+def vti_card_survey(nshot: int = 300, nt: int = 10000, first: int = 1, model_n: int = 300):
+    """Shots first .. first + nshot - 1 of the VTI card (1-based, as the
+    reference numbers them).  Returns (vz, eps, delta) [nshot, 698, 348] and
+    the keyword arguments of VtiPlan."""
+    nxv = nzv = model_n
+    dx = 5.0
+    vz = smooth_random_model(nxv, nzv, 15, 1800.0, 3500.0, 12)
+    ep = smooth_random_model(nxv, nzv, 15, 0.0, 0.25, 13)
+    de = ep * smooth_random_model(nxv, nzv, 15, 0.2, 0.6, 14)  # delta <= eps: the system is unstable otherwise
+    nl = nr = nd = 50                        # nint(250 / 5)
+    nu = 0
+    nsx0, nsz0, nrz0 = 1, 1, 1               # nint(0 / 5) + 1
+    off_min, off_max, dsx = -1500.0, 1495.0, 5.0
+    rng = int(round(off_max / dx)) - int(round(off_min / dx)) + 1
+    shots = []
+    for i in range(first, first + nshot):
+        nsx = nsx0 + int(round((i - 1) * dsx / dx))
+        xmin = nsx + int(round(off_min / dx))
+        shots.append(vti_window(vz, ep, de, xmin, rng, (nl, nr, nu, nd), nsz0))
+    V, E, D = (np.stack([s[k] for s in shots]) for k in range(3))
+    sx = (nsx - xmin + nl) - 1               # currshot_nsx, the same for every shot
+    sz = (nsz0 + nu) - 1
+    row = (nrz0 + nu if nrz0 + nu >= 6 else 6) - 1
+    rx = np.arange(nl, nl + rng, dtype=np.int32)
+    kw = dict(dx=dx, dz=dx, dt=5e-4, nt=nt, f0=30.0, src=(sx, sz), rec=(rx, np.full(rng, row, np.int32)),
+              pml=(nl, nr, nu, nd))
+    return V, E, D, kw

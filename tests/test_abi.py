@@ -69,7 +69,8 @@ def test_every_struct_mirror_matches_the_header(tmp_path):
     import subprocess
     import seismic_code_github_amd as sfd
     mirrors = {"sfd_acoustic_params": sfd.AcousticParamsC, "sfd_visco_params": sfd.ViscoParamsC,
-               "sfd_elastic_params": sfd.ElasticParamsC, "sfd_q_par": sfd.QParC, "sfd_iso_par": sfd.IsoParC}
+               "sfd_elastic_params": sfd.ElasticParamsC, "sfd_vti_params": sfd.VtiParamsC,
+               "sfd_q_par": sfd.QParC, "sfd_iso_par": sfd.IsoParC}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "seisfd.h"', 'int main(void) {']
     for cname, cls in mirrors.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -1,0 +1,176 @@
+"""VTI qP-acoustic propagator on the GPU (sfd_vti_plan_*) against the oracle.
+
+* Records and final p / q against the FP32 and FP64 oracle on heterogeneous
+  VTI models (delta <= eps, isotropic inside the PML strips as the
+  reference's shot windows are) with the reference's PML layout (left,
+  right, bottom; none on top) and with PML on all four sides: records, final
+  p and final q within max(5e-5, twice the FP32 oracle's own rel-L2 from the
+  FP64 oracle) of the FP64 oracle, and records within 5e-5 of the FP32
+  oracle.  (The FP32 oracle's records and p are 1e-6 .. 3e-5 from FP64; q,
+  3 % of p where anisotropy starts below the source rows, 3e-4.)
+* Every built tiling gives bitwise the same result; a batch of shots equals
+  the shots run one by one, bit for bit; exact x-mirror symmetry.
+* Edges: receiver on the source cell, source inside a PML corner, no PML,
+  one recorded sample, the windowed card geometry (source on the top row).
+* Errors: unstable dt and bad widths are refused before any launch.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import seismic_code_github_amd as sfd
+from seismic_code_github_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TILINGS = [(8, 4), (8, 8), (16, 4), (16, 8), (32, 8), (32, 16)]
+
+
+def model(nx, nz, seed, iso_rows=0, pml=(0, 0, 0, 0)):
+    """Smooth VTI model with delta <= eps (the pseudo-acoustic system is
+    unstable where eps < delta), eps = delta = 0 on the top iso_rows rows and
+    inside the PML strips, as get_currshot_parameters leaves them."""
+    vz = W.smooth_random_model(nx, nz, 6, 1800, 3200, seed)
+    ep = W.smooth_random_model(nx, nz, 6, 0.0, 0.25, seed + 1)
+    de = ep * W.smooth_random_model(nx, nz, 6, 0.2, 0.6, seed + 2)
+    nl, nr, nu, nd = pml
+    for a in (ep, de):
+        a[:, :max(iso_rows, nu)] = 0
+        a[:nl] = 0
+        a[nx - nr:] = 0
+        a[:, nz - nd:] = 0
+    return vz, ep, de
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def run_gpu(vz, ep, de, ctx, tiling=None, **kw):
+    pl = sfd.VtiPlan(vz, ep, de, ctx=ctx, **kw)
+    if tiling:
+        pl.set_tiling(*tiling)
+    pl.run()
+    rec, st = pl.fetch(state=True)
+    pl.close()
+    return rec, st
+
+
+def run_oracle(vz, ep, de, precision="f32", **kw):
+    w = kw.get("wavelet")
+    delay = kw.get("delay")
+    dt, f0 = kw["dt"], kw.get("f0", 30.0)
+    if w is None:
+        w = O.vti_wavelet(dt, f0)
+    if delay is None:
+        delay = int(np.floor((O.vti_wavelet(dt, f0).size - 1) / 2.0 + 0.5))
+    return O.vti(vz, ep, de, src=kw["src"], rec=kw["rec"], nsteps=delay + kw["nt"], delay=delay, dx=kw["dx"],
+                 dz=kw["dz"], dt=dt, pml=kw.get("pml", (0, 0, 0, 0)), wavelet=w, precision=precision, nthreads=8,
+                 state=True)
+
+
+CASES = {
+    "reference_layout": dict(n=(181, 121), pml=(30, 30, 0, 30), src=(90, 0), row=5, iso=7),
+    "all_sides": dict(n=(150, 133), pml=(20, 25, 15, 20), src=(60, 40), row=18, iso=0),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_against_oracle(case):
+    c = CASES[case]
+    nx, nz = c["n"]
+    vz, ep, de = model(nx, nz, 21, c["iso"], c["pml"])
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=500, src=c["src"], rec=(np.arange(nx), np.full(nx, c["row"])),
+              pml=c["pml"])
+    print(f"case: vti {case} {nx}x{nz}", flush=True)
+    with sfd.Context(0) as ctx:
+        rec, st = run_gpu(vz, ep, de, ctx, **kw)
+    r32, p32, q32 = run_oracle(vz, ep, de, **kw)
+    r64, p64, q64 = run_oracle(vz, ep, de, precision="f64", **kw)
+    e64, e32 = rel(rec[0], r64), rel(rec[0], r32)
+    print(f"rel-L2 vs FP64 {e64:.2e}, vs FP32 {e32:.2e}, oracle FP32 vs FP64 {rel(r32, r64):.2e}", flush=True)
+    # the GPU's distance from FP64 within twice the FP32 oracle's own (floor 5e-5)
+    for got, o32, o64 in ((rec[0], r32, r64), (st[0, 0], p32, p64), (st[0, 1], q32, q64)):
+        assert rel(got, o64) <= max(5e-5, 2 * rel(o32, o64))
+    assert e32 <= 5e-5
+    assert np.abs(q64).max() > 0
+
+
+def test_tilings_bitwise_and_batch_equals_single_shots():
+    nx, nz = 130, 100
+    models = [model(nx, nz, 40 + 3 * s, 7, (20, 20, 0, 20)) for s in range(3)]
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=300, src=(65, 0), rec=(np.arange(nx), np.full(nx, 5)),
+              pml=(20, 20, 0, 20))
+    print("case: vti tilings and batch", flush=True)
+    with sfd.Context(0) as ctx:
+        base, _ = run_gpu(*models[0], ctx, **kw)
+        for t in TILINGS:
+            r, _ = run_gpu(*models[0], ctx, tiling=t, **kw)
+            np.testing.assert_array_equal(r, base, err_msg=str(t))
+        V, E, D = (np.stack([m[k] for m in models]) for k in range(3))
+        batch, st = run_gpu(V, E, D, ctx, **kw)
+        for s in range(3):
+            one, st1 = run_gpu(*models[s], ctx, **kw)
+            np.testing.assert_array_equal(batch[s], one[0])
+            np.testing.assert_array_equal(st[s], st1[0])
+
+
+def test_mirror_symmetry_bitwise():
+    nx, nz = 141, 90
+    half = model(nx // 2 + 1, nz, 50, 7, (25, 0, 10, 25))
+    vz, ep, de = (np.concatenate([h, h[-2::-1]], 0) for h in half)
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=400, src=(nx // 2, 3), rec=(np.arange(nx), np.full(nx, 5)),
+              pml=(25, 25, 10, 25))
+    print("case: vti mirror", flush=True)
+    with sfd.Context(0) as ctx:
+        rec, _ = run_gpu(vz, ep, de, ctx, **kw)
+    np.testing.assert_array_equal(rec[0], rec[0][::-1])
+    assert np.abs(rec).max() > 0
+
+
+@pytest.mark.parametrize("edge", ["rec_on_source", "source_in_corner", "no_pml", "one_sample"])
+def test_edges(edge):
+    nx, nz = 96, 80
+    vz, ep, de = model(nx, nz, 60, 0, (16, 16, 16, 16))
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=250, src=(48, 30), rec=(np.array([48, 10, 95]), np.array([30, 70, 0])),
+              pml=(16, 16, 16, 16))
+    if edge == "rec_on_source":
+        pass  # receiver 0 sits on the source cell
+    elif edge == "source_in_corner":
+        kw["src"] = (3, 75)
+    elif edge == "no_pml":
+        kw["pml"] = (0, 0, 0, 0)
+    elif edge == "one_sample":
+        kw["nt"] = 1
+        kw["delay"] = 0
+        kw["wavelet"] = np.array([1.0], np.float32)
+    print(f"case: vti edge {edge}", flush=True)
+    with sfd.Context(0) as ctx:
+        rec, st = run_gpu(vz, ep, de, ctx, **kw)
+    r64, p64, q64 = run_oracle(vz, ep, de, precision="f64", **kw)
+    assert rel(rec[0], r64) <= 5e-5
+    assert rel(st[0, 0], p64) <= 1e-4
+
+
+def test_card_geometry_short():
+    V, E, D, kw = W.vti_card_survey(nshot=2, nt=300, first=150)
+    kw["nt"] = 300
+    print("case: vti card geometry, 2 shots", flush=True)
+    with sfd.Context(0) as ctx:
+        rec, _ = run_gpu(V, E, D, ctx, **kw)
+    for s in range(2):
+        r64 = run_oracle(V[s], E[s], D[s], precision="f64", **kw)[0]
+        assert rel(rec[s], r64) <= 5e-5
+
+
+def test_refuses_bad_input():
+    nx, nz = 64, 64
+    vz, ep, de = model(nx, nz, 70)
+    kw = dict(dx=5.0, dz=5.0, nt=10, src=(32, 32), rec=(np.array([1]), np.array([1])))
+    with sfd.Context(0) as ctx:
+        with pytest.raises(sfd.SeisfdError, match="unstable"):
+            sfd.VtiPlan(vz, ep, de, dt=2e-3, ctx=ctx, **kw)
+        with pytest.raises(sfd.SeisfdError):
+            sfd.VtiPlan(vz, ep, de, dt=5e-4, pml=(40, 40, 0, 0), ctx=ctx, **kw)
+        with pytest.raises(sfd.SeisfdError):
+            sfd.VtiPlan(vz, ep, de, dt=5e-4, ctx=ctx, **dict(kw, src=(64, 0)))

@@ -1,0 +1,33 @@
+"""Host-side pieces of the VTI propagator in libseisfd (no GPU): the
+coefficient solves, the wavelet and the shot window give bit for bit what
+the oracle's restatement gives, and plan creation rejects bad input without
+touching a device."""
+import numpy as np
+import pytest
+
+import oracle as O
+import seismic_code_github_amd as sfd
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_coefficients_match_the_oracle(kind):
+    for k in range(1, 6):
+        np.testing.assert_array_equal(sfd.vti_coefficients(kind, k), O.vti_coefficients(kind, k))
+
+
+@pytest.mark.parametrize("dt,f0", [(5e-4, 30.0), (1e-3, 25.0), (2.5e-4, 40.0)])
+def test_wavelet_matches_the_oracle(dt, f0):
+    np.testing.assert_array_equal(sfd.vti_wavelet(dt, f0), O.vti_wavelet(dt, f0))
+
+
+def test_window_matches_the_oracle():
+    rng = np.random.default_rng(3)
+    nxv, nzv = 50, 36
+    vz, ep, de = (rng.uniform(1, 3, (nxv, nzv)).astype(np.float32) for _ in range(3))
+    for xmin, r, pml, nsz0 in ((-10, 30, (8, 6, 0, 9), 1), (5, 41, (10, 10, 4, 10), 3), (40, 25, (2, 2, 2, 2), 40)):
+        got = sfd.vti_window(vz, ep, de, xmin, r, pml, nsz0)
+        want = O.vti_window(vz, ep, de, xmin, r, pml, nsz0)
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w)
+    with pytest.raises(sfd.SeisfdError):
+        sfd.vti_window(vz, ep, de, 0, 10, (4, 1, 0, 4), 1)  # right strip < 2
