@@ -280,7 +280,7 @@ def vti_bytes_per_step(nx, nz, pml):
     return 36 * (nx * nz - xs - zs) + (28 + 76) * xs + (32 + 44) * zs
 
 
-def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False):
+def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
     """VTI qP-acoustic survey of the reference card
     (vti_qa_2d_modelling_new.par): nshot shots of the 698 x 348 padded grid,
     nt recorded samples + 140 delay steps, all shots advanced together by one
@@ -291,7 +291,11 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False):
     V, E, D, kw = W.vti_card_survey(nshot=nshot, nt=nt)
     _, nx, nz = V.shape
     with sfd.VtiPlan(V, E, D, **kw) as pl:
-        tuned_us = None if no_tune else pl.autotune(20)
+        tuned_us = None
+        if tiling:
+            pl.set_tiling(*tiling)
+        elif not no_tune:
+            tuned_us = pl.autotune(20)
         shape = pl.shape()
         pl.run()
         ms = pl.run()
@@ -419,6 +423,7 @@ def main():
     ap.add_argument("--vti-shots", type=int, default=300)
     ap.add_argument("--vti-nt", type=int, default=10000)
     ap.add_argument("--vti-err-nt", type=int, default=600)
+    ap.add_argument("--vti-tiling", default=None, help="tile columns,waves (skips the autotuner)")
     ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic", "vti"], default=None,
                     help="run one secondary leg alone (profiling) and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
@@ -447,13 +452,14 @@ def main():
         log(f"copied {args.copy_mib} MiB device to device")
         del src, dst
 
+    vti_tiling = tuple(int(v) for v in args.vti_tiling.split(",")) if args.vti_tiling else None
     if args.only:
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
                "grad": lambda: gradient_leg(args.grad_steps),
                "batch": lambda: batched_leg(args.batch_steps, args.no_tune),
                "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu),
                "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,
-                                      args.no_tune)}[args.only]
+                                      args.no_tune, vti_tiling)}[args.only]
         print(json.dumps(leg()), flush=True)
         return
 
@@ -531,7 +537,7 @@ def main():
         out["elastic"] = elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)
     if not args.no_vti:
         log("vti leg ...")
-        out["vti"] = vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu, args.no_tune)
+        out["vti"] = vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu, args.no_tune, vti_tiling)
     if not args.no_grad:
         log("gradient leg ...")
         out["gradient"] = gradient_leg(args.grad_steps)

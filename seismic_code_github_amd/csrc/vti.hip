@@ -115,6 +115,29 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
     const long long so = shot * a.fs;
     const float* uc = a.uc + so;
     const float* qc = a.qc + so;
+    float* un = a.un + so;
+    float* qn = a.qn + so;
+    const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = z0 + tz;
+    const int n_x = a.nx;
+    const bool inz = k >= a.nu && k < a.nz - a.nd;
+    // The cells' own old level and model are loaded first, so they are in
+    // flight together with the staging loads (the old p only where the
+    // interior formula reads it, the old q only outside the x strips).
+    constexpr int J = TX / NW;
+    float r_uo[J], r_qo[J], r_vz[J], r_ep[J], r_de[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int col = x0 + wv + j * NW;
+        const bool ok = k < a.nz && col < n_x;
+        const bool inx = col >= a.nl && col < n_x - a.nr;
+        const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
+        r_vz[j] = ok ? a.vz[so + o] : 0.f;
+        r_ep[j] = ok ? a.ep[so + o] : 0.f;
+        r_de[j] = ok ? a.de[so + o] : 0.f;
+        r_qo[j] = ok && inx ? qn[o] : 0.f;
+        r_uo[j] = ok && inx && inz ? un[o] : 0.f;
+    }
     {
         // s region: WX columns x 64 rows; p region: TX columns x HZ rows.
         constexpr int NS = WX * kVTZ, NU = TX * HZ;
@@ -157,12 +180,9 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
         }
     }
     __syncthreads();
-    const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int k = z0 + tz;
     if (k >= a.nz) return;
-    const bool inz = k >= a.nu && k < a.nz - a.nd;
-    const int n_x = a.nx;
-    for (int j = 0; j < TX / NW; ++j) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
         const int cl = wv + j * NW, col = x0 + cl;
         if (col >= n_x) break;
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
@@ -177,16 +197,14 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
         }
         dxx *= a.idx2;
         dzz *= a.idz2;
-        const float vz = a.vz[so + o], ep = a.ep[so + o], de = a.de[so + o];
+        const float vz = r_vz[j], ep = r_ep[j], de = r_de[j];
         const float t = a.dt2 * (vz * vz);
-        float* un = a.un + so;
-        float* qn = a.qn + so;
         const bool inx = col >= a.nl && col < n_x - a.nr;
         const int po = a.par, pc = 1 - a.par;
         if (inx) {
-            qn[o] = 2.f * q0 - qn[o] + t * (2.f * (ep - de) * dxx);
+            qn[o] = 2.f * q0 - r_qo[j] + t * (2.f * (ep - de) * dxx);
             if (inz) {
-                float uo = un[o];
+                float uo = r_uo[j];
                 if (col == a.sx && k == a.sz) uo += a.w_prev;
                 un[o] = 2.f * u0 - uo + t * ((1.f + 2.f * de) * dxx + dzz);
             } else {
