@@ -348,6 +348,45 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
     return out
 
 
+def srm_leg(nshot, ntrace, lt, no_cpu):
+    """Convolution SRM (de-multiple/convolution_srm/convolution_srm.par:
+    401 shots x 401 traces x 4000 samples): the whole prediction with the
+    gathers resident in HBM, output traces per second; the FP64 oracle (the
+    reference's own algorithm: nshot receiver-trace FFTs per output trace) on
+    sampled output traces as the CPU baseline and the error check."""
+    import seismic_code_github_amd as sfd
+    rng = np.random.default_rng(31)
+    csg = rng.standard_normal((nshot, ntrace, lt), dtype=np.float32)
+    crg = rng.standard_normal((sfd.srm_crg_traces(nshot, ntrace), lt), dtype=np.float32)
+    with sfd.SrmPlan(csg, crg) as pl:
+        pl.run()
+        ms = min(pl.run() for _ in range(2))
+        out = pl.fetch()
+    assert np.isfinite(out).all()
+    ntr = nshot * ntrace
+    res = {"workload": f"convolution SRM, convolution_srm.par: {nshot} shots x {ntrace} traces x {lt} samples "
+                       f"(FFT length {2 * lt}), synthetic N(0,1) gathers resident in HBM",
+           "ms_per_prediction": round(ms, 2), "output_traces_per_s": round(ntr / (ms / 1e3), 1)}
+    if not no_cpu:
+        from oracle import srm
+        check = [(0, 0), (nshot // 2, ntrace // 3), (nshot - 1, ntrace - 1)]
+        want = srm.predict(csg, crg, pairs=check)
+        errs = [float(np.linalg.norm(out[s, t] - want[s, t]) / np.linalg.norm(want[s, t])) for s, t in check]
+        res["fp64_rel_l2_max"] = max(errs)
+        res["error_sample"] = f"output traces {check}"
+        # CPU baseline: the first 200 output traces of shot 1 (the reference's order)
+        pairs = [(1, t) for t in range(min(200, ntrace))]
+        t0 = time.perf_counter()
+        srm.predict(csg, crg, pairs=pairs)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(len(pairs) / dt, 3), "unit": "output traces/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{len(pairs)} output traces of one shot through the FP64 oracle (numpy "
+                                         f"FFTs of the {nshot} receiver traces per output trace, as the reference), "
+                                         f"{dt:.1f} s"}
+    return res
+
+
 def gradient_leg(nt):
     """Adjoint gradient: 16 shots on 801 x 801 + 40-cell sponge, 10 Hz,
     checkpoints every 50 steps: forward + checkpointed recomputation + adjoint
@@ -424,7 +463,9 @@ def main():
     ap.add_argument("--vti-nt", type=int, default=10000)
     ap.add_argument("--vti-err-nt", type=int, default=600)
     ap.add_argument("--vti-tiling", default=None, help="tile columns,waves (skips the autotuner)")
-    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic", "vti"], default=None,
+    ap.add_argument("--no-srm", action="store_true")
+    ap.add_argument("--srm-size", default="401,401,4000", help="nshot,ntrace,lt")
+    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic", "vti", "srm"], default=None,
                     help="run one secondary leg alone (profiling) and print its JSON")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
@@ -453,13 +494,15 @@ def main():
         del src, dst
 
     vti_tiling = tuple(int(v) for v in args.vti_tiling.split(",")) if args.vti_tiling else None
+    srm_size = tuple(int(v) for v in args.srm_size.split(","))
     if args.only:
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
                "grad": lambda: gradient_leg(args.grad_steps),
                "batch": lambda: batched_leg(args.batch_steps, args.no_tune),
                "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu),
                "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,
-                                      args.no_tune, vti_tiling)}[args.only]
+                                      args.no_tune, vti_tiling),
+               "srm": lambda: srm_leg(*srm_size, args.no_cpu)}[args.only]
         print(json.dumps(leg()), flush=True)
         return
 
@@ -538,6 +581,9 @@ def main():
     if not args.no_vti:
         log("vti leg ...")
         out["vti"] = vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu, args.no_tune, vti_tiling)
+    if not args.no_srm:
+        log("srm leg ...")
+        out["srm"] = srm_leg(*srm_size, args.no_cpu)
     if not args.no_grad:
         log("gradient leg ...")
         out["gradient"] = gradient_leg(args.grad_steps)

@@ -476,6 +476,35 @@ int sfd_vti_wavelet(float dt, float f0, int nmax, float *w);
 int sfd_vti_window(const float *vz, const float *eps, const float *delta, int nxv, int nzv, int xmin, int range,
                    int nl, int nr, int nu, int nd, int nsz0, float *ovz, float *oeps, float *odelta);
 
+/* ---------------------------------------------------------------- SRM
+ * Convolution surface-related multiple prediction, the consumer of the
+ * modelled gathers: the drop-in for de-multiple/convolution_srm/convolution_srm.cpp:7-276
+ * (its files become arrays).  Output trace (is, it) = the real part / ltt of
+ * the inverse FFT of sum_k FFT(csg[is][k]) FFT(crg[is*ntrace + it*nshot + is + k])
+ * / (ntrace + nshot) over k < ntrace with is + k < nshot, all traces
+ * zero-padded to ltt = 2 lt, first lt samples kept (:107-108, 141-176, 201-252). */
+typedef struct sfd_srm_plan sfd_srm_plan;
+
+typedef struct sfd_srm_params {
+    int nshot;             /* reference nshot */
+    int ntrace;            /* traces per shot (reference nt) */
+    int lt;                /* samples per trace */
+    int ncrg;              /* traces in crg: at least ntrace (2 nshot - 1), what the reference reads */
+} sfd_srm_params;
+
+/* csg [nshot][ntrace][lt] (the reference's input file), crg [ncrg][lt]
+ * (its inputcp file); out [nshot][ntrace][lt] (its output file). */
+int sfd_srm_predict(sfd_context *ctx, const sfd_srm_params *p, const float *csg, const float *crg, float *out);
+/* Traces the common-receiver array needs: ntrace (2 nshot - 1). */
+int sfd_srm_crg_traces(int nshot, int ntrace);
+/* Plan: uploads csg and crg once; run computes (device time in elapsed_ms,
+ * may be NULL); fetch copies out [nshot][ntrace][lt]. */
+int sfd_srm_plan_create(sfd_context *ctx, const sfd_srm_params *p, const float *csg, const float *crg,
+                        sfd_srm_plan **plan);
+int sfd_srm_plan_run(sfd_srm_plan *plan, float *elapsed_ms);
+int sfd_srm_plan_fetch(sfd_srm_plan *plan, float *out);
+int sfd_srm_plan_destroy(sfd_srm_plan *plan);
+
 /* ---------------------------------------------------------------- files
  * The Fortran drivers' files on either side of the propagators (host code;
  * no device or context needed).  Parameter cards: a comment line before

@@ -110,6 +110,11 @@ class VtiParamsC(ctypes.Structure):
     ]
 
 
+class SrmParamsC(ctypes.Structure):
+    """Mirror of ``struct sfd_srm_params``."""
+    _fields_ = [("nshot", ctypes.c_int), ("ntrace", ctypes.c_int), ("lt", ctypes.c_int), ("ncrg", ctypes.c_int)]
+
+
 class QParC(ctypes.Structure):
     _fields_ = [(n, ctypes.c_char * 256) for n in ("vp_fn", "vs_fn", "rho_fn", "qfp_fn", "qfs_fn",
                                                    "shot_fn1", "shot_fn2", "shot_fn3", "shot_fn4")] + [
@@ -184,6 +189,13 @@ _SIGNATURES = {
     "sfd_vti_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, _IP, _IP]),
     "sfd_vti_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_vti_coefficients": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _FP]),
+    "sfd_srm_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SrmParamsC), _FP, _FP, _FP]),
+    "sfd_srm_crg_traces": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "sfd_srm_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SrmParamsC), _FP, _FP,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_srm_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_srm_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_srm_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_vti_wavelet": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP]),
     "sfd_vti_window": (ctypes.c_int, [_FP, _FP, _FP] + [ctypes.c_int] * 9 + [_FP, _FP, _FP]),
     "sfd_elastic_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
@@ -1039,6 +1051,68 @@ class VtiPlan:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- SRM
+
+def srm_crg_traces(nshot: int, ntrace: int) -> int:
+    """Traces of the common-receiver array the prediction reads: ntrace (2 nshot - 1)."""
+    return lib().sfd_srm_crg_traces(nshot, ntrace)
+
+
+class SrmPlan:
+    """``sfd_srm_plan``: convolution multiple prediction with csg [nshot,
+    ntrace, lt] and crg [ncrg, lt] resident on the device."""
+
+    def __init__(self, csg, crg, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
+        csg, crg = _f32(csg), _f32(crg)
+        if csg.ndim != 3 or crg.ndim != 2 or crg.shape[1] != csg.shape[2]:
+            raise ValueError("csg must be [nshot, ntrace, lt] and crg [ncrg, lt]")
+        self.shape = csg.shape
+        self.p = SrmParamsC(csg.shape[0], csg.shape[1], csg.shape[2], crg.shape[0])
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_srm_plan_create(self.ctx.handle, ctypes.byref(self.p), _ptr(csg), _ptr(crg),
+                                                 ctypes.byref(h)), "sfd_srm_plan_create")
+        self.handle = h
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_srm_plan_run(self.handle, ctypes.byref(ms)), "sfd_srm_plan_run")
+        return ms.value
+
+    def fetch(self) -> np.ndarray:
+        out = np.zeros(self.shape, dtype=np.float32)
+        self.ctx.check(lib().sfd_srm_plan_fetch(self.handle, _ptr(out)), "sfd_srm_plan_fetch")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_srm_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def srm_predict(csg, crg, ctx: Optional[Context] = None) -> np.ndarray:
+    """Predicted multiples [nshot, ntrace, lt] through ``sfd_srm_predict``."""
+    ctx = ctx or default_context()
+    csg, crg = _f32(csg), _f32(crg)
+    p = SrmParamsC(csg.shape[0], csg.shape[1], csg.shape[2], crg.shape[0])
+    out = np.zeros(csg.shape, dtype=np.float32)
+    ctx.check(lib().sfd_srm_predict(ctx.handle, ctypes.byref(p), _ptr(csg), _ptr(crg), _ptr(out)), "sfd_srm_predict")
+    return out
 
 
 # ---------------------------------------------------------------- files

@@ -70,6 +70,7 @@ def test_every_struct_mirror_matches_the_header(tmp_path):
     import seismic_code_github_amd as sfd
     mirrors = {"sfd_acoustic_params": sfd.AcousticParamsC, "sfd_visco_params": sfd.ViscoParamsC,
                "sfd_elastic_params": sfd.ElasticParamsC, "sfd_vti_params": sfd.VtiParamsC,
+               "sfd_srm_params": sfd.SrmParamsC,
                "sfd_q_par": sfd.QParC, "sfd_iso_par": sfd.IsoParC}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "seisfd.h"', 'int main(void) {']
     for cname, cls in mirrors.items():
