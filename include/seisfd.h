@@ -502,6 +502,14 @@ int sfd_srm_crg_traces(int nshot, int ntrace);
 int sfd_srm_plan_create(sfd_context *ctx, const sfd_srm_params *p, const float *csg, const float *crg,
                         sfd_srm_plan **plan);
 int sfd_srm_plan_run(sfd_srm_plan *plan, float *elapsed_ms);
+/* How the spectral sums are formed (no counterpart in the reference, which
+ * loops): DIRECT = one workgroup per (frequency, shot) summing the trace
+ * pairs; GEMM (ntrace = nshot only) = one batched complex matrix product per
+ * frequency over the banded shot spectra (rocBLAS); AUTO = GEMM when
+ * ntrace = nshot.  get_method returns the one a run uses. */
+enum sfd_srm_method { SFD_SRM_AUTO = 0, SFD_SRM_DIRECT = 1, SFD_SRM_GEMM = 2 };
+int sfd_srm_plan_set_method(sfd_srm_plan *plan, int method);
+int sfd_srm_plan_get_method(const sfd_srm_plan *plan);
 int sfd_srm_plan_fetch(sfd_srm_plan *plan, float *out);
 int sfd_srm_plan_destroy(sfd_srm_plan *plan);
 

@@ -196,6 +196,8 @@ _SIGNATURES = {
     "sfd_srm_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_srm_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_srm_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_srm_plan_set_method": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "sfd_srm_plan_get_method": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_vti_wavelet": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP]),
     "sfd_vti_window": (ctypes.c_int, [_FP, _FP, _FP] + [ctypes.c_int] * 9 + [_FP, _FP, _FP]),
     "sfd_elastic_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
@@ -1055,6 +1057,9 @@ class VtiPlan:
 
 # ---------------------------------------------------------------- SRM
 
+SRM_AUTO, SRM_DIRECT, SRM_GEMM = 0, 1, 2
+
+
 def srm_crg_traces(nshot: int, ntrace: int) -> int:
     """Traces of the common-receiver array the prediction reads: ntrace (2 nshot - 1)."""
     return lib().sfd_srm_crg_traces(nshot, ntrace)
@@ -1076,6 +1081,13 @@ class SrmPlan:
         self.ctx.check(lib().sfd_srm_plan_create(self.ctx.handle, ctypes.byref(self.p), _ptr(csg), _ptr(crg),
                                                  ctypes.byref(h)), "sfd_srm_plan_create")
         self.handle = h
+
+    def set_method(self, method: int):
+        """SRM_AUTO, SRM_DIRECT or SRM_GEMM (ntrace = nshot only)."""
+        self.ctx.check(lib().sfd_srm_plan_set_method(self.handle, method), "sfd_srm_plan_set_method")
+
+    def method(self) -> int:
+        return lib().sfd_srm_plan_get_method(self.handle)
 
     def run(self) -> float:
         ms = ctypes.c_float()

@@ -18,8 +18,18 @@
 // reference's 401 x 401 x 4000) stay in one XCD's L2 because the workgroups
 // of a frequency are dealt to one XCD.  The outputs are transposed back and
 // inverse-transformed by a batched C2R.
+//
+// When ntrace = nshot = N (the reference's card) the window of output
+// (is, it) starts at (is + it) N + is, so per frequency the sum is a matrix
+// product: M'[is][r] = sum_j A'[is][j] C[r][j] with A'[is][j] = A[is][j - is]
+// (j >= is, else 0) and C[r][j] the receiver spectra, and output (is, it) =
+// M'[is][is + it].  That path builds A' in its transpose, runs one rocBLAS
+// strided-batched CGEMM over all frequencies (MFMA) and picks the outputs
+// out of M'; it does about 4x the multiply-adds of the direct sum (A' is
+// half zeros, half of M' is unused) at matrix-core rate.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstring>
@@ -33,15 +43,24 @@ namespace {
 
 constexpr int kSrmMaxTrace = 8192;  // shot traces held in LDS (8 B each)
 
-// out[c][r] = in[r][c] for complex [rows][cols]; 32 x 32 tiles through LDS.
+// out[c][r] = in[src(r)][c] for complex [rows][cols]; 32 x 32 tiles through
+// LDS.  src(r) = r, or with shift_n = N > 0 the banded shift of the GEMM
+// path: row r = is N + j reads row is N + j - is when j >= is, else 0.
 __global__ __launch_bounds__(256) void srm_transpose_kernel(const float2* __restrict__ in, float2* __restrict__ out,
-                                                            int rows, int cols) {
+                                                            int rows, int cols, int shift_n) {
     __shared__ float2 tile[32][33];
     const int c0 = blockIdx.x * 32;
     for (int r0 = blockIdx.y * 32; r0 < rows; r0 += gridDim.y * 32) {
         for (int j = threadIdx.y; j < 32; j += 8) {
             const int r = r0 + j, c = c0 + threadIdx.x;
-            if (r < rows && c < cols) tile[j][threadIdx.x] = in[(size_t)r * cols + c];
+            if (r < rows && c < cols) {
+                int src = r;
+                if (shift_n) {
+                    const int is = r / shift_n, jj = r - is * shift_n;
+                    src = jj >= is ? r - is : -1;
+                }
+                tile[j][threadIdx.x] = src >= 0 ? in[(size_t)src * cols + c] : make_float2(0.f, 0.f);
+            }
         }
         __syncthreads();
         for (int j = threadIdx.y; j < 32; j += 8) {
@@ -87,6 +106,20 @@ __global__ __launch_bounds__(256) void srm_contract_kernel(const float2* __restr
     }
 }
 
+// GEMM path: out[f][is N + it] = mp[f][(is + it) N + is] / (2 N), mp the
+// column-major N x (2N - 1) product of each frequency.
+__global__ void srm_extract_kernel(const float2* __restrict__ mp, float2* __restrict__ out, int n, int nf) {
+    const long long nn = (long long)n * n, total = nn * nf, ld = (long long)n * (2 * n - 1);
+    const float div = (float)(2 * n);
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long f = e / nn, rem = e - f * nn;
+        const int is = (int)(rem / n), it = (int)(rem - (long long)is * n);
+        const float2 v = mp[f * ld + (long long)(is + it) * n + is];
+        out[e] = make_float2(v.x / div, v.y / div);
+    }
+}
+
 // out[tr][t] = full[tr][t] / ltt, t < lt.
 __global__ void srm_crop_kernel(const float* __restrict__ full, float* __restrict__ out, long long ntr, int lt,
                                 int ltt) {
@@ -118,15 +151,19 @@ using namespace sfd;
 struct sfd_srm_plan {
     sfd_context* ctx = nullptr;
     int nshot = 0, nt = 0, lt = 0, ltt = 0, nf = 0, ncrg = 0;
+    int method = SFD_SRM_AUTO;
     size_t ntr = 0;
     DeviceBuffer csg, crg, spec_s, spec_c, tr_s, tr_c, out;
     hipfftHandle fwd_s = 0, fwd_c = 0, inv = 0;
     bool has_fwd_s = false, has_fwd_c = false, has_inv = false;
+    rocblas_handle blas = nullptr;
     ~sfd_srm_plan() {
         if (has_fwd_s) hipfftDestroy(fwd_s);
         if (has_fwd_c) hipfftDestroy(fwd_c);
         if (has_inv) hipfftDestroy(inv);
+        if (blas) rocblas_destroy_handle(blas);
     }
+    bool gemm() const { return nt == nshot && method != SFD_SRM_DIRECT; }
 };
 
 namespace sfd {
@@ -138,12 +175,37 @@ namespace {
         if (r_ != HIPFFT_SUCCESS) return fail(ctx, SFD_EDEVICE, "%s: hipFFT %s", #call, fft_err(r_)); \
     } while (0)
 
-int srm_transpose(sfd_context* ctx, const void* in, void* out, size_t rows, int cols) {
+int srm_transpose(sfd_context* ctx, const void* in, void* out, size_t rows, int cols, int shift_n = 0) {
     const dim3 block(32, 8);
     const unsigned gy = (unsigned)std::min<size_t>((rows + 31) / 32, 65535);
     const dim3 grid((cols + 31) / 32, gy);
     hipLaunchKernelGGL(srm_transpose_kernel, grid, block, 0, ctx->stream, (const float2*)in, (float2*)out, (int)rows,
-                       cols);
+                       cols, shift_n);
+    SFD_HIP(ctx, hipGetLastError());
+    return SFD_OK;
+}
+
+// ntrace = nshot = N: spectra into A' and C (frequency-major), the product
+// M' of every frequency into spec_c, outputs picked into tr_s.
+int srm_contract_gemm(sfd_srm_plan* pl) {
+    sfd_context* ctx = pl->ctx;
+    const int n = pl->nshot;
+    int rc = srm_transpose(ctx, pl->spec_s.ptr, pl->tr_s.ptr, pl->ntr, pl->nf, n);
+    if (!rc) rc = srm_transpose(ctx, pl->spec_c.ptr, pl->tr_c.ptr, (size_t)pl->ncrg, pl->nf);
+    if (rc) return rc;
+    const rocblas_float_complex one(1.f, 0.f), zero(0.f, 0.f);
+    const rocblas_stride sa = (rocblas_stride)n * n, sb = (rocblas_stride)pl->ncrg,
+                         sc = (rocblas_stride)n * (2 * n - 1);
+    const rocblas_status st = rocblas_cgemm_strided_batched(
+        pl->blas, rocblas_operation_transpose, rocblas_operation_none, n, 2 * n - 1, n, &one,
+        pl->tr_s.as<rocblas_float_complex>(), n, sa, pl->tr_c.as<rocblas_float_complex>(), n, sb, &zero,
+        pl->spec_c.as<rocblas_float_complex>(), n, sc, pl->nf);
+    if (st != rocblas_status_success)
+        return fail(ctx, SFD_EDEVICE, "rocblas_cgemm_strided_batched: %s", rocblas_status_to_string(st));
+    const long long total = (long long)n * n * pl->nf;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(srm_extract_kernel, dim3(blocks), dim3(256), 0, ctx->stream, pl->spec_c.as<float2>(),
+                       pl->tr_s.as<float2>(), n, pl->nf);
     SFD_HIP(ctx, hipGetLastError());
     return SFD_OK;
 }
@@ -152,16 +214,24 @@ int srm_enqueue(sfd_srm_plan* pl) {
     sfd_context* ctx = pl->ctx;
     SFD_FFT(ctx, hipfftExecR2C(pl->fwd_s, pl->csg.as<hipfftReal>(), pl->spec_s.as<hipfftComplex>()));
     SFD_FFT(ctx, hipfftExecR2C(pl->fwd_c, pl->crg.as<hipfftReal>(), pl->spec_c.as<hipfftComplex>()));
-    int rc = srm_transpose(ctx, pl->spec_s.ptr, pl->tr_s.ptr, pl->ntr, pl->nf);
-    if (!rc) rc = srm_transpose(ctx, pl->spec_c.ptr, pl->tr_c.ptr, (size_t)pl->ncrg, pl->nf);
+    int rc;
+    const void* spec_out;  // output spectra, frequency-major
+    if (pl->gemm()) {
+        rc = srm_contract_gemm(pl);
+        spec_out = pl->tr_s.ptr;
+    } else {
+        rc = srm_transpose(ctx, pl->spec_s.ptr, pl->tr_s.ptr, pl->ntr, pl->nf);
+        if (!rc) rc = srm_transpose(ctx, pl->spec_c.ptr, pl->tr_c.ptr, (size_t)pl->ncrg, pl->nf);
+        if (rc) return rc;
+        hipLaunchKernelGGL(srm_contract_kernel, dim3(pl->nshot * pl->nf), dim3(256),
+                           (size_t)std::min(pl->nt, pl->nshot) * sizeof(float2), ctx->stream, pl->tr_s.as<float2>(),
+                           pl->tr_c.as<float2>(), pl->spec_c.as<float2>(), pl->nshot, pl->nt, pl->ncrg, pl->nf);
+        SFD_HIP(ctx, hipGetLastError());
+        spec_out = pl->spec_c.ptr;
+    }
     if (rc) return rc;
-    // output spectra, frequency-major, into spec_c (free again), then back to
-    // trace-major in spec_s for the inverse transform
-    hipLaunchKernelGGL(srm_contract_kernel, dim3(pl->nshot * pl->nf), dim3(256),
-                       (size_t)std::min(pl->nt, pl->nshot) * sizeof(float2), ctx->stream, pl->tr_s.as<float2>(),
-                       pl->tr_c.as<float2>(), pl->spec_c.as<float2>(), pl->nshot, pl->nt, pl->ncrg, pl->nf);
-    SFD_HIP(ctx, hipGetLastError());
-    rc = srm_transpose(ctx, pl->spec_c.ptr, pl->spec_s.ptr, (size_t)pl->nf, (int)pl->ntr);
+    // back to trace-major in spec_s for the inverse transform
+    rc = srm_transpose(ctx, spec_out, pl->spec_s.ptr, (size_t)pl->nf, (int)pl->ntr);
     if (rc) return rc;
     SFD_FFT(ctx, hipfftExecC2R(pl->inv, pl->spec_s.as<hipfftComplex>(), pl->tr_c.as<hipfftReal>()));
     const long long n = (long long)pl->ntr * pl->lt;
@@ -247,8 +317,29 @@ int sfd_srm_plan_create(sfd_context* ctx, const sfd_srm_params* p, const float* 
     if (r == HIPFFT_SUCCESS) r = hipfftSetStream(pl->fwd_c, sm);
     if (r == HIPFFT_SUCCESS) r = hipfftSetStream(pl->inv, sm);
     if (r != HIPFFT_SUCCESS) return bail(fail(ctx, SFD_EDEVICE, "hipFFT plans: %s", fft_err(r)));
+    if (pl->nt == pl->nshot) {
+        rocblas_status st = rocblas_create_handle(&pl->blas);
+        if (st == rocblas_status_success) st = rocblas_set_stream(pl->blas, sm);
+        if (st != rocblas_status_success)
+            return bail(fail(ctx, SFD_EDEVICE, "rocBLAS handle: %s", rocblas_status_to_string(st)));
+    }
     *out = pl;
     return SFD_OK;
+}
+
+int sfd_srm_plan_set_method(sfd_srm_plan* pl, int method) {
+    if (!pl) return SFD_EINVAL;
+    if (method != SFD_SRM_AUTO && method != SFD_SRM_DIRECT && method != SFD_SRM_GEMM)
+        return fail(pl->ctx, SFD_EINVAL, "unknown SRM method %d", method);
+    if (method == SFD_SRM_GEMM && pl->nt != pl->nshot)
+        return fail(pl->ctx, SFD_EINVAL, "the GEMM path needs ntrace = nshot");
+    pl->method = method;
+    return SFD_OK;
+}
+
+int sfd_srm_plan_get_method(const sfd_srm_plan* pl) {
+    if (!pl) return SFD_EINVAL;
+    return pl->gemm() ? SFD_SRM_GEMM : SFD_SRM_DIRECT;
 }
 
 int sfd_srm_plan_run(sfd_srm_plan* pl, float* elapsed_ms) {

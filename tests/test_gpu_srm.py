@@ -8,6 +8,8 @@
 * A larger survey (48 x 48 x 1000) against the oracle on sampled traces; a
   plan run twice gives bitwise the same output.
 * Too short a common-receiver array is refused.
+* ntrace = nshot: the batched-GEMM path and the direct sum agree with the
+  oracle; ntrace != nshot refuses the GEMM path.
 """
 import numpy as np
 import pytest
@@ -77,3 +79,27 @@ def test_refuses_short_crg():
     csg, crg = data(4, 4, 16, 1)
     with pytest.raises(sfd.SeisfdError, match="common-receiver"):
         sfd.srm_predict(csg, crg[:-1])
+
+
+@pytest.mark.parametrize("n,lt", [(7, 50), (33, 120), (1, 8)])
+def test_gemm_and_direct_paths(n, lt):
+    """ntrace = nshot: the banded batched-GEMM path and the direct sum both
+    agree with the oracle; ntrace != nshot refuses the GEMM path."""
+    csg, crg = data(n, n, lt, 70 + n)
+    want = srm.predict(csg, crg)
+    print(f"case: srm methods {n}x{n}x{lt}", flush=True)
+    outs = {}
+    with sfd.SrmPlan(csg, crg) as pl:
+        assert pl.method() == sfd.SRM_GEMM
+        for m in (sfd.SRM_GEMM, sfd.SRM_DIRECT):
+            pl.set_method(m)
+            assert pl.method() == m
+            pl.run()
+            outs[m] = pl.fetch()
+            print(f"method {m}: rel-L2 vs FP64 {rel(outs[m], want):.2e}", flush=True)
+            assert rel(outs[m], want) <= 1e-5
+    csg2, crg2 = data(5, 3, 16, 2)
+    with sfd.SrmPlan(csg2, crg2) as pl:
+        assert pl.method() == sfd.SRM_DIRECT
+        with pytest.raises(sfd.SeisfdError, match="ntrace = nshot"):
+            pl.set_method(sfd.SRM_GEMM)
