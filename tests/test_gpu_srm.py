@@ -103,3 +103,26 @@ def test_gemm_and_direct_paths(n, lt):
         assert pl.method() == sfd.SRM_DIRECT
         with pytest.raises(sfd.SeisfdError, match="ntrace = nshot"):
             pl.set_method(sfd.SRM_GEMM)
+
+
+@pytest.mark.parametrize("n,lt", [(70, 24), (260, 16)])
+def test_blocked_gemm(n, lt):
+    """Sizes where the GEMM path splits the shot rows into 2 and 4 blocks:
+    sampled output traces (block edges, first and last shot and trace)
+    against the oracle, and the whole output against the direct sum."""
+    csg, crg = data(n, n, lt, n)
+    q = n // 4
+    pairs = [(0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1), (q - 1, n - 1), (q, 0), (n // 2, n // 2),
+             (n // 2 - 1, 1), (3 * q, n - 2), (3 * q - 1, 5)]
+    print(f"case: srm blocked gemm {n}x{n}x{lt}", flush=True)
+    with sfd.SrmPlan(csg, crg) as pl:
+        pl.set_method(sfd.SRM_GEMM)
+        pl.run()
+        g = pl.fetch()
+        pl.set_method(sfd.SRM_DIRECT)
+        pl.run()
+        d = pl.fetch()
+    want = srm.predict(csg, crg, pairs=pairs)
+    for s, t in pairs:
+        assert rel(g[s, t], want[s, t]) <= 1e-5, (s, t)
+    assert rel(g, d) <= 1e-5
