@@ -44,16 +44,17 @@ namespace sfd {
 namespace {
 
 constexpr int kSrmMaxTrace = 8192;  // shot traces held in LDS (8 B each)
+constexpr int kTr = 64;             // transpose tile edge
 
-// out[c][r] = in[src(r)][c] for complex [rows][cols]; 32 x 32 tiles through
-// LDS.  src(r) = r, or with shift_n = N > 0 the banded shift of the GEMM
+// out[c][r] = in[src(r)][c] for complex [rows][cols]; kTr x kTr tiles through
+// LDS (one wave reads or writes a whole 512-byte tile row).  src(r) = r, or with shift_n = N > 0 the banded shift of the GEMM
 // path: row r = is N + j reads row is N + j - is when j >= is, else 0.
 __global__ __launch_bounds__(256) void srm_transpose_kernel(const float2* __restrict__ in, float2* __restrict__ out,
                                                             int rows, int cols, int shift_n) {
-    __shared__ float2 tile[32][33];
-    const int c0 = blockIdx.x * 32;
-    for (int r0 = blockIdx.y * 32; r0 < rows; r0 += gridDim.y * 32) {
-        for (int j = threadIdx.y; j < 32; j += 8) {
+    __shared__ float2 tile[kTr][kTr + 1];
+    const int c0 = blockIdx.x * kTr;
+    for (int r0 = blockIdx.y * kTr; r0 < rows; r0 += gridDim.y * kTr) {
+        for (int j = threadIdx.y; j < kTr; j += 256 / kTr) {
             const int r = r0 + j, c = c0 + threadIdx.x;
             if (r < rows && c < cols) {
                 int src = r;
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(256) void srm_transpose_kernel(const float2* __rest
             }
         }
         __syncthreads();
-        for (int j = threadIdx.y; j < 32; j += 8) {
+        for (int j = threadIdx.y; j < kTr; j += 256 / kTr) {
             const int c = c0 + j, r = r0 + threadIdx.x;
             if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[threadIdx.x][j];
         }
@@ -178,9 +179,9 @@ namespace {
     } while (0)
 
 int srm_transpose(sfd_context* ctx, const void* in, void* out, size_t rows, int cols, int shift_n = 0) {
-    const dim3 block(32, 8);
-    const unsigned gy = (unsigned)std::min<size_t>((rows + 31) / 32, 65535);
-    const dim3 grid((cols + 31) / 32, gy);
+    const dim3 block(kTr, 256 / kTr);
+    const unsigned gy = (unsigned)std::min<size_t>((rows + kTr - 1) / kTr, 65535);
+    const dim3 grid((cols + kTr - 1) / kTr, gy);
     hipLaunchKernelGGL(srm_transpose_kernel, grid, block, 0, ctx->stream, (const float2*)in, (float2*)out, (int)rows,
                        cols, shift_n);
     SFD_HIP(ctx, hipGetLastError());
