@@ -455,8 +455,8 @@ int sfd_vti_plan_create(sfd_context *ctx, const sfd_vti_params *p, int nshot, co
 int sfd_vti_plan_run(sfd_vti_plan *plan, float *elapsed_ms);
 /* record as above; state (may be NULL) [nshot][2][nx*nz]: p and q after the run. */
 int sfd_vti_plan_fetch(sfd_vti_plan *plan, float *record, float *state);
-/* Tile width in columns (8, 16 or 32) and waves per workgroup (4, 8 or 16;
- * not every pair is built); 0 = the default. */
+/* Tile width in columns (8, 16, 32 or 64) and waves per workgroup (4, 8 or
+ * 16; not every pair is built); 0 = the default. */
 int sfd_vti_plan_set_tiling(sfd_vti_plan *plan, int tile_cols, int waves);
 /* Autotuner (no counterpart in the reference), as sfd_elastic_plan_autotune:
  * cached for the grid class (nx, nz and nshot to the nearest power of 2);

@@ -45,7 +45,7 @@ namespace {
 constexpr int kVTZ = 64;     // tile rows (one per lane)
 constexpr int kVH = 5;       // stencil half-width
 constexpr int kVPad = 8;     // zero cells around the grid in every field
-constexpr int kVTXMax = 32;  // widest tile; the field's extra columns cover its staging
+constexpr int kVTXMax = 64;  // widest tile; the field's extra columns cover its staging
 
 struct VtiArgs {
     int nx, nz, pitch, ntz, ntiles;
@@ -262,10 +262,12 @@ bool vti_pick(int tx, int nw, VtiKernel* k) {
     case 1608: *k = vti_step_kernel<16, 8>; return true;
     case 3208: *k = vti_step_kernel<32, 8>; return true;
     case 3216: *k = vti_step_kernel<32, 16>; return true;
+    case 6408: *k = vti_step_kernel<64, 8>; return true;
+    case 6416: *k = vti_step_kernel<64, 16>; return true;
     default: return false;
     }
 }
-constexpr int kVtiShapes[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {32, 8}, {32, 16}};
+constexpr int kVtiShapes[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {32, 8}, {32, 16}, {64, 8}, {64, 16}};
 constexpr int kVtiDefaultTX = 16, kVtiDefaultNW = 4;
 
 // coefficient_2nd / coefficient_1st (sub:1264-1472): the k x k system

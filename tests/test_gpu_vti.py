@@ -23,7 +23,7 @@ from seismic_code_github_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-TILINGS = [(8, 4), (8, 8), (16, 4), (16, 8), (32, 8), (32, 16)]
+TILINGS = [(8, 4), (8, 8), (16, 4), (16, 8), (32, 8), (32, 16), (64, 8), (64, 16)]
 
 
 def model(nx, nz, seed, iso_rows=0, pml=(0, 0, 0, 0)):
