@@ -986,8 +986,6 @@ class VtiPlan:
 
     def __init__(self, vz, eps, delta, *, dx, dz, dt, nt, src, rec, pml=(0, 0, 0, 0), f0=30.0, delay=None,
                  pml_r=0.0, wavelet=None, c2=None, c1=None, ctx: Optional[Context] = None):
-        self.ctx = ctx or default_context()
-        self.ctx.adopt(self)
         vz, eps, delta = (_f32(a) for a in (vz, eps, delta))
         if vz.ndim == 2:
             vz, eps, delta = vz[None], eps[None], delta[None]
@@ -998,11 +996,22 @@ class VtiPlan:
         self._rz = np.ascontiguousarray(rec[1], dtype=np.int32)
         self._c2 = None if c2 is None else _f32(c2)
         self._c1 = None if c1 is None else _f32(c1)
+        for name, c in (("c2", self._c2), ("c1", self._c1)):
+            if c is not None and c.size != 5:
+                raise ValueError(f"{name} must hold 5 weights")
         self._w = None if wavelet is None else _f32(wavelet)
+        if self._w is not None:
+            # the library reads delay + nt samples; the reference's delay is
+            # nint((nwt - 1) / 2) with nwt the built-in wavelet's length
+            d = int(np.floor((vti_wavelet(dt, f0).size - 1) / 2.0 + 0.5)) if delay is None or delay < 0 else delay
+            if self._w.size < d + int(nt):
+                raise ValueError(f"wavelet needs delay + nt = {d + int(nt)} samples, got {self._w.size}")
         self._arr = (vz, eps, delta)
         self.p = VtiParamsC(self.nx, self.nz, int(nt), -1 if delay is None else int(delay), dx, dz, dt, f0,
                             *(int(v) for v in pml), pml_r, _ptr(self._c2), _ptr(self._c1), int(src[0]), int(src[1]),
                             self._rx.size, _ptr(self._rx, ctypes.c_int), _ptr(self._rz, ctypes.c_int))
+        self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
         h = ctypes.c_void_p()
         self.ctx.check(lib().sfd_vti_plan_create(self.ctx.handle, ctypes.byref(self.p), self.nshot, _ptr(vz),
                                                  _ptr(eps), _ptr(delta), _ptr(self._w), ctypes.byref(h)),
@@ -1070,11 +1079,11 @@ class SrmPlan:
     ntrace, lt] and crg [ncrg, lt] resident on the device."""
 
     def __init__(self, csg, crg, ctx: Optional[Context] = None):
-        self.ctx = ctx or default_context()
-        self.ctx.adopt(self)
         csg, crg = _f32(csg), _f32(crg)
         if csg.ndim != 3 or crg.ndim != 2 or crg.shape[1] != csg.shape[2]:
             raise ValueError("csg must be [nshot, ntrace, lt] and crg [ncrg, lt]")
+        self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
         self.shape = csg.shape
         self.p = SrmParamsC(csg.shape[0], csg.shape[1], csg.shape[2], crg.shape[0])
         h = ctypes.c_void_p()
@@ -1119,8 +1128,10 @@ class SrmPlan:
 
 def srm_predict(csg, crg, ctx: Optional[Context] = None) -> np.ndarray:
     """Predicted multiples [nshot, ntrace, lt] through ``sfd_srm_predict``."""
-    ctx = ctx or default_context()
     csg, crg = _f32(csg), _f32(crg)
+    if csg.ndim != 3 or crg.ndim != 2 or crg.shape[1] != csg.shape[2]:
+        raise ValueError("csg must be [nshot, ntrace, lt] and crg [ncrg, lt]")
+    ctx = ctx or default_context()
     p = SrmParamsC(csg.shape[0], csg.shape[1], csg.shape[2], crg.shape[0])
     out = np.zeros(csg.shape, dtype=np.float32)
     ctx.check(lib().sfd_srm_predict(ctx.handle, ctypes.byref(p), _ptr(csg), _ptr(crg), _ptr(out)), "sfd_srm_predict")

@@ -31,3 +31,20 @@ def test_window_matches_the_oracle():
             np.testing.assert_array_equal(g, w)
     with pytest.raises(sfd.SeisfdError):
         sfd.vti_window(vz, ep, de, 0, 10, (4, 1, 0, 4), 1)  # right strip < 2
+
+
+def test_python_mirrors_refuse_short_inputs():
+    """Checks made before any library call (no device needed): a wavelet
+    shorter than delay + nt, coefficient tables of the wrong size, and an SRM
+    receiver array whose traces are not lt long."""
+    vz = np.full((20, 20), 2000, np.float32)
+    z = np.zeros_like(vz)
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=10, src=(10, 10), rec=([1], [1]))
+    with pytest.raises(ValueError, match="delay \\+ nt"):
+        sfd.VtiPlan(vz, z, z, wavelet=np.ones(149, np.float32), **kw)  # reference delay 140
+    with pytest.raises(ValueError, match="delay \\+ nt"):
+        sfd.VtiPlan(vz, z, z, wavelet=np.ones(10, np.float32), delay=3, **kw)
+    with pytest.raises(ValueError, match="5 weights"):
+        sfd.VtiPlan(vz, z, z, c2=np.ones(4), **kw)
+    with pytest.raises(ValueError, match="crg"):
+        sfd.srm_predict(np.zeros((2, 2, 8), np.float32), np.zeros((6, 7), np.float32))
