@@ -123,6 +123,34 @@ __global__ void srm_extract_kernel(const float2* __restrict__ mp, float2* __rest
     }
 }
 
+// The same through LDS, one workgroup per (64 r, 64 is) block of one
+// frequency (blockIdx.z): the block of mp is read along is (contiguous), and
+// each is writes its outputs it = r - is as one contiguous run.
+__global__ __launch_bounds__(256) void srm_extract_tiled_kernel(const float2* __restrict__ mp,
+                                                                float2* __restrict__ out, int n) {
+    __shared__ float2 tile[64][65];
+    const int r0 = blockIdx.x * 64, i0 = blockIdx.y * 64, f = blockIdx.z;
+    // outputs need 0 <= r - is < n: skip blocks with none
+    if (r0 + 63 < i0 || r0 > i0 + 63 + n - 1) return;
+    const long long ld = (long long)n * (2 * n - 1);
+    const float2* m = mp + (long long)f * ld;
+    const int x = threadIdx.x & 63, y0 = threadIdx.x >> 6;
+    for (int y = y0; y < 64; y += 4) {
+        const int r = r0 + y, is = i0 + x;
+        if (r < 2 * n - 1 && is < n) tile[y][x] = m[(long long)r * n + is];
+    }
+    __syncthreads();
+    const float div = (float)(2 * n);
+    float2* o = out + (long long)f * n * n;
+    for (int y = y0; y < 64; y += 4) {
+        const int is = i0 + y, r = r0 + x, it = r - is;
+        if (is < n && it >= 0 && it < n) {
+            const float2 v = tile[x][y];
+            o[(long long)is * n + it] = make_float2(v.x / div, v.y / div);
+        }
+    }
+}
+
 // out[tr][t] = full[tr][t] / ltt, t < lt.
 __global__ void srm_crop_kernel(const float* __restrict__ full, float* __restrict__ out, long long ntr, int lt,
                                 int ltt) {
@@ -218,10 +246,16 @@ int srm_contract_gemm(sfd_srm_plan* pl) {
         if (st != rocblas_status_success)
             return fail(ctx, SFD_EDEVICE, "rocblas_cgemm_strided_batched: %s", rocblas_status_to_string(st));
     }
-    const long long total = (long long)n * n * pl->nf;
-    const int blocks = (int)std::min<long long>((total + 255) / 256, 1 << 20);
-    hipLaunchKernelGGL(srm_extract_kernel, dim3(blocks), dim3(256), 0, ctx->stream, pl->spec_c.as<float2>(),
-                       pl->tr_s.as<float2>(), n, pl->nf);
+    if (pl->nf <= 65535) {
+        const dim3 grid((2 * n - 1 + 63) / 64, (n + 63) / 64, pl->nf);
+        hipLaunchKernelGGL(srm_extract_tiled_kernel, grid, dim3(256), 0, ctx->stream, pl->spec_c.as<float2>(),
+                           pl->tr_s.as<float2>(), n);
+    } else {
+        const long long total = (long long)n * n * pl->nf;
+        const int blocks = (int)std::min<long long>((total + 255) / 256, 1 << 20);
+        hipLaunchKernelGGL(srm_extract_kernel, dim3(blocks), dim3(256), 0, ctx->stream, pl->spec_c.as<float2>(),
+                           pl->tr_s.as<float2>(), n, pl->nf);
+    }
     SFD_HIP(ctx, hipGetLastError());
     return SFD_OK;
 }
