@@ -4,8 +4,8 @@ and its JSON cache (sfd_context_save_tuning / sfd_context_load_tuning).
 * The autotuner picks one of the built shapes, records it under the grid
   class, and later plans / batches of that class in the context start with it.
 * Every shape gives bitwise the same result, so a tuned run equals the
-  default run (acoustic, elastic); the visco-acoustic candidates (tiled step,
-  march kernel) agree to rounding.
+  default run (acoustic, elastic, VTI); the visco-acoustic candidates (tiled
+  step, march kernel) agree to rounding.
 * The cache survives a save / load round trip into a fresh context; saving
   merges into an existing file; a malformed file is rejected as a whole and
   entries of another GPU architecture are ignored.
@@ -175,3 +175,21 @@ def test_visco_autotune():
         e = float(np.linalg.norm(r1.astype(np.float64) - r0) / np.linalg.norm(r0))
         assert e <= 1e-5
         assert sfd.ViscoPlan(shot, ctx).shape() == (kind, a, b)
+
+
+def test_vti_autotune_keeps_results_and_caches():
+    V, E, D, kw = W.vti_card_survey(nshot=3, nt=200, first=100)
+    print("case: vti autotune", flush=True)
+    with sfd.Context(0) as ctx:
+        base = sfd.VtiPlan(V, E, D, ctx=ctx, **kw)
+        base.run()
+        r0 = base.fetch()
+        pl = sfd.VtiPlan(V, E, D, ctx=ctx, **kw)
+        us = pl.autotune(10)
+        shape = pl.shape()
+        print(f"tuned: {shape} {us:.2f} us/step", flush=True)
+        assert shape in {(8, 4), (8, 8), (16, 4), (16, 8), (32, 8), (32, 16), (64, 8), (64, 16)}
+        assert us > 0 and ctx.tuning_size() == 1
+        pl.run()
+        np.testing.assert_array_equal(pl.fetch(), r0)
+        assert sfd.VtiPlan(V, E, D, ctx=ctx, **kw).shape() == shape
