@@ -140,6 +140,14 @@ int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan *plan, int waves, int chunk_r
  * An explicit sfd_acoustic_plan_set_tiling also selects the march kernel.
  * Results are bitwise equal either way; the kernel may change between steps. */
 int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan *plan, int tile_rows, int waves);
+/* Fast arithmetic as ONE launch per step (no counterpart in the reference):
+ * each workgroup of `waves` (4 or 8) updates tile_rows (16 or 32) x 64
+ * cells, recomputing the new velocities its pressure stencils need around
+ * them, on ping-pong buffers, so every field crosses memory once per step
+ * (28 B per cell with uniform density, against the two-launch step's 40 B);
+ * built shapes 16x4, 16x8, 32x4, 32x8; 0 = 32 x 8.  Results are
+ * bitwise equal to the other kernels. */
+int sfd_acoustic_plan_set_fused(sfd_acoustic_plan *plan, int tile_rows, int waves);
 
 /* ---------------------------------------------------------------- batches
  * Many shots advanced together.  The reference runs its shots one after
@@ -173,6 +181,9 @@ int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch *batch, int waves, int chun
  * `waves` (8x4, 16x4, 16x8, 32x8 built); 0 = the default shape, tile_rows < 0
  * = the march kernel.  Invalidates a captured graph. */
 int sfd_acoustic_batch_set_tiled(sfd_acoustic_batch *batch, int tile_rows, int waves);
+/* One launch per step for every shot (as sfd_acoustic_plan_set_fused, the
+ * same shapes).  Invalidates a captured graph. */
+int sfd_acoustic_batch_set_fused(sfd_acoustic_batch *batch, int tile_rows, int waves);
 /* Zeroes every wavefield and propagates all shots nt steps (how: enum
  * sfd_batch_run).  elapsed_ms: device time of the propagation. */
 int sfd_acoustic_batch_run(sfd_acoustic_batch *batch, int how, float *elapsed_ms);
@@ -182,8 +193,9 @@ int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
 
 /* ---------------------------------------------------------------- autotuner
  * (no counterpart in the reference).  After a warm-up of up to 4 trials'
- * steps, times every built launch shape of the step (the tiled step's shapes
- * in fast arithmetic, the fused march kernel at 1, 2 and 4 waves) twice over
+ * steps, times every built launch shape of the step (the two-launch and the
+ * one-launch tiled steps' shapes in fast arithmetic, the fused march kernel
+ * at 1, 2 and 4 waves) twice over
  * min(trial_steps, nt) steps from a zero state, keeps the fastest and records
  * it in the context's cache under the grid class:
  *   plan:  radius, boundary + arithmetic, nx and nz to the nearest power of 2
@@ -195,7 +207,8 @@ int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
 int sfd_acoustic_plan_autotune(sfd_acoustic_plan *plan, int trial_steps, float *best_us);
 int sfd_acoustic_batch_autotune(sfd_acoustic_batch *batch, int trial_steps, float *best_us);
 /* Current launch shape: kind 0 = tiled (a = tile rows, b = waves), kind 1 =
- * fused march kernel (a = waves, b = rows per chunk). */
+ * fused march kernel (a = waves, b = rows per chunk), kind 2 = one-launch
+ * tiled step (a = tile rows, b = waves). */
 int sfd_acoustic_plan_get_shape(const sfd_acoustic_plan *plan, int *kind, int *a, int *b);
 int sfd_acoustic_batch_get_shape(const sfd_acoustic_batch *batch, int *kind, int *a, int *b);
 /* The context's autotuner cache as a JSON file
@@ -562,6 +575,16 @@ int sfd_write_su_shot(const char *path, const float *record, int ntrace, int nt,
  * remove_inputs deletes them afterwards (as the reference does). */
 int sfd_merge_su(const char *const *shot_files, int nshot, int ntrace, int nt, const char *out_path,
                  int remove_inputs);
+
+/* ---------------------------------------------------------------- measurement
+ * Streaming-bandwidth probe (no counterpart in the reference): the rate a
+ * hand-written kernel streams a device buffer of `bytes` bytes at (read: sum
+ * every 16-byte vector; write: store a constant; copy: a second buffer of the
+ * same size, bytes counted twice), the best of `reps` timed launches after one
+ * warm-up over plain and nontemporal loads / stores, in GB/s (1e9 B/s).
+ * Allocates and frees its buffers. */
+enum sfd_bandwidth_kind { SFD_BW_READ = 0, SFD_BW_WRITE = 1, SFD_BW_COPY = 2 };
+int sfd_bandwidth_probe(sfd_context *ctx, size_t bytes, int kind, int reps, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -158,12 +158,14 @@ _SIGNATURES = {
     "sfd_acoustic_plan_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_plan_set_tiled": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_plan_set_fused": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_cfl_limit": (ctypes.c_double, [ctypes.c_int, _FP]),
     "sfd_acoustic_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(AcousticParamsC),
                                                  ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
                                                  ctypes.POINTER(ctypes.c_void_p)]),
     "sfd_acoustic_batch_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_batch_set_tiled": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_acoustic_batch_set_fused": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_acoustic_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
     "sfd_acoustic_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
     "sfd_acoustic_plan_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
@@ -196,6 +198,8 @@ _SIGNATURES = {
     "sfd_srm_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_srm_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_srm_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_bandwidth_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double)]),
     "sfd_srm_plan_set_method": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "sfd_srm_plan_get_method": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_vti_wavelet": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_int, _FP]),
@@ -409,6 +413,10 @@ class AcousticPlan:
         (0 = the default shape; tile_rows < 0 = the fused march kernel)."""
         self.ctx.check(lib().sfd_acoustic_plan_set_tiled(self.handle, tile_rows, waves), "set_tiled")
 
+    def set_fused(self, rows: int = 0, waves: int = 0):
+        """One launch per step: tiles of `rows` x 64 cells per workgroup of `waves` (0 = 32 x 8)."""
+        self.ctx.check(lib().sfd_acoustic_plan_set_fused(self.handle, rows, waves), "set_fused")
+
     def autotune(self, trial_steps: int = 20) -> float:
         """Times every launch shape, keeps the fastest (cached in the context);
         returns its device time per step in us.  Resets the state."""
@@ -590,6 +598,10 @@ class AcousticBatch:
 
     def set_tiled(self, tile_rows: int = 0, waves: int = 0):
         self.ctx.check(lib().sfd_acoustic_batch_set_tiled(self.handle, tile_rows, waves), "batch_set_tiled")
+
+    def set_fused(self, rows: int = 0, waves: int = 0):
+        """One launch per step for every shot (0 = 32-row tiles, 8 waves)."""
+        self.ctx.check(lib().sfd_acoustic_batch_set_fused(self.handle, rows, waves), "batch_set_fused")
 
     def autotune(self, trial_steps: int = 20) -> float:
         """Times every launch shape, keeps the fastest (cached in the context);
@@ -1183,3 +1195,19 @@ def merge_su(shot_files, ntrace: int, nt: int, out_path: str, remove_inputs: boo
     rc = lib().sfd_merge_su(arr, len(shot_files), ntrace, nt, os.fsencode(out_path), int(remove_inputs))
     if rc != SFD_OK:
         raise SeisfdError(rc, f"sfd_merge_su({out_path})")
+
+
+BW_READ = 0
+BW_WRITE = 1
+BW_COPY = 2
+
+
+def bandwidth_probe(nbytes: int, kind: int = BW_COPY, reps: int = 5, ctx: Optional[Context] = None) -> float:
+    """Streaming rate of a hand-written kernel over an nbytes device buffer
+    (BW_READ, BW_WRITE or BW_COPY, the copy counting both buffers), in GB/s:
+    the achievable HBM rate the stencil steps are read against."""
+    ctx = ctx or default_context()
+    g = ctypes.c_double()
+    ctx.check(lib().sfd_bandwidth_probe(ctx.handle, int(nbytes), int(kind), int(reps), ctypes.byref(g)),
+              "sfd_bandwidth_probe")
+    return g.value

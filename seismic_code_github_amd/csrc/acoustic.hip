@@ -299,6 +299,19 @@ bool pick_tiled_batch(int radius, int tx, int nw, int mode, BatchKernel* v, Batc
     }
 }
 
+bool pick_fused_r4(int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
+bool pick_fused_r5(int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
+bool pick_fused_r8(int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
+
+bool pick_fused(int radius, int rows, int nw, int mode, StepKernel* k, BatchKernel* b) {
+    switch (radius) {
+    case 4: return pick_fused_r4(rows, nw, mode, k, b);
+    case 5: return pick_fused_r5(rows, nw, mode, k, b);
+    case 8: return pick_fused_r8(rows, nw, mode, k, b);
+    default: return false;
+    }
+}
+
 // Modes with a tiled step (the reference arithmetic stays on the march kernel).
 bool tiled_mode_ok(int mode) { return mode == MODE_FAST_CPML || mode == MODE_FAST_SPONGE; }
 
@@ -430,8 +443,9 @@ std::string plan_class(const sfd_acoustic_plan* pl) {
 }
 
 // Sets a plan's launch shape: kind 0 = tiled (a rows, b waves), 1 = march (a
-// waves, b chunk rows).
+// waves, b chunk rows), 2 = one-launch tiled (a rows per wave, b waves).
 int apply_shape(sfd_acoustic_plan* pl, const TunedShape& t) {
+    if (t.kind == 2) return sfd_acoustic_plan_set_fused(pl, t.a, t.b);
     return t.kind == 0 ? sfd_acoustic_plan_set_tiled(pl, t.a, t.b) : sfd_acoustic_plan_set_tiling(pl, t.a, t.b);
 }
 
@@ -484,6 +498,12 @@ void tiled_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw) {
     a.v_raw = v_raw ? 1 : 0;
 }
 
+void fused_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw) {
+    a.ntz = (pl->nz - 2 * pl->radius + kTiledTZ - 1) / kTiledTZ;
+    a.nchunk = (pl->nx - 2 * pl->radius + pl->ftx - 1) / pl->ftx;
+    a.v_raw = v_raw ? 1 : 0;
+}
+
 int launch_tiled(const sfd_acoustic_plan* pl, StepKernel v, StepKernel p, const AcousticStepArgs& a) {
     const dim3 grid(a.ntz * a.nchunk), block(64 * pl->tnw);
     hipLaunchKernelGGL(v, grid, block, 0, pl->ctx->stream, a);
@@ -498,8 +518,8 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
     sfd_context* ctx = pl->ctx;
     const size_t nbytes = (size_t)pl->nx * pl->nz * sizeof(float);
     size_t snap_next = std::lower_bound(pl->snap_steps.begin(), pl->snap_steps.end(), k0) - pl->snap_steps.begin();
-    const bool tiled = pl->ttx > 0;
-    if (!tiled && pl->v_raw && nsteps > 0) {
+    const bool tiled = pl->ttx > 0, fused = !tiled && pl->ftx > 0;
+    if (!tiled && !fused && pl->v_raw && nsteps > 0) {
         // The march kernel reads absorbed velocities.
         for (Field* f : {&pl->vx[pl->cur], &pl->vz[pl->cur]}) {
             const int rc = launch_absorb(pl, pl->grid(*f), pl->grid(*f));
@@ -514,6 +534,12 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps) {
             tiled_args(pl, a, pl->v_raw);
             const int rc = launch_tiled(pl, pl->tvel, pl->tpres, a);
             if (rc) return rc;
+            pl->v_raw = pl->mode == MODE_FAST_SPONGE;
+        } else if (fused) {
+            fused_args(pl, a, pl->v_raw);
+            hipLaunchKernelGGL(pl->fstep, dim3(a.ntz * a.nchunk), dim3(64 * pl->fnw), 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            pl->cur ^= 1;
             pl->v_raw = pl->mode == MODE_FAST_SPONGE;
         } else {
             hipLaunchKernelGGL(pl->kernel, dim3(a.ntz * a.nchunk), dim3(64 * pl->nw), 0, ctx->stream, a);
@@ -753,6 +779,7 @@ int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan* pl, int waves, int chunk_row
     // An explicit march tiling selects the march kernel; automatic restores
     // the tiled step where it applies.
     pl->ttx = pl->tnw = 0;
+    pl->ftx = pl->fnw = 0;
     if (waves == 0 && chunk_rows == 0 && tiled_mode_ok(pl->mode) &&
         pick_tiled(pl->radius, kTiledTX, kTiledNW, pl->mode, &pl->tvel, &pl->tpres)) {
         pl->ttx = kTiledTX;
@@ -765,6 +792,7 @@ int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan* pl, int tile_rows, int waves)
     if (!pl) return SFD_EINVAL;
     if (tile_rows < 0) {
         pl->ttx = pl->tnw = 0;
+        pl->ftx = pl->fnw = 0;
         return SFD_OK;
     }
     if (!tiled_mode_ok(pl->mode)) return fail(pl->ctx, SFD_ENOTSUP, "the tiled step exists in fast arithmetic only");
@@ -776,6 +804,22 @@ int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan* pl, int tile_rows, int waves)
     pl->tnw = nw;
     pl->tvel = v;
     pl->tpres = p;
+    pl->ftx = pl->fnw = 0;
+    return SFD_OK;
+}
+
+int sfd_acoustic_plan_set_fused(sfd_acoustic_plan* pl, int rows, int waves) {
+    if (!pl) return SFD_EINVAL;
+    if (!tiled_mode_ok(pl->mode))
+        return fail(pl->ctx, SFD_ENOTSUP, "the one-launch step exists in fast arithmetic only");
+    const int tr = rows ? rows : kFusedShapes[0][0], nw = waves ? waves : kFusedShapes[0][1];
+    StepKernel k;
+    if (!pick_fused(pl->radius, tr, nw, pl->mode, &k, nullptr))
+        return fail(pl->ctx, SFD_EINVAL, "no one-launch step for %d rows x %d waves", tr, nw);
+    pl->ftx = tr;
+    pl->fnw = nw;
+    pl->fstep = k;
+    pl->ttx = pl->tnw = 0;
     return SFD_OK;
 }
 
@@ -829,9 +873,13 @@ int sfd_acoustic_plan_set_state(sfd_acoustic_plan* pl, const float* state, const
     SFD_HIP(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)pl->nx * pl->nz;
     hipStream_t s = ctx->stream;
-    SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->p[pl->cur]), state, n * 4, hipMemcpyHostToDevice, s));
-    SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vx[pl->cur]), state + n, n * 4, hipMemcpyHostToDevice, s));
-    SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vz[pl->cur]), state + 2 * n, n * 4, hipMemcpyHostToDevice, s));
+    // Both ping-pong buffers: cells outside the updated region are never
+    // written by a step, and the one-launch step reads them from either.
+    for (int b = 0; b < 2; ++b) {
+        SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->p[b]), state, n * 4, hipMemcpyHostToDevice, s));
+        SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vx[b]), state + n, n * 4, hipMemcpyHostToDevice, s));
+        SFD_HIP(ctx, hipMemcpyAsync(pl->grid(pl->vz[b]), state + 2 * n, n * 4, hipMemcpyHostToDevice, s));
+    }
     pl->v_raw = false;  // the given velocities are absorbed ones
     if (pl->mode == MODE_FAST_CPML) {
         if (!pml) return fail(ctx, SFD_EINVAL, "C-PML plans need the pml state");
@@ -949,6 +997,10 @@ struct sfd_acoustic_batch {
     bool tiled = false;
     int ttx = 0, tnw = 0;
     BatchKernel tvel = nullptr, tpres = nullptr;
+    // Or the one-launch tiled step (ping-pong): tile rows, waves.
+    bool fused = false;
+    int ftx = 0, fnw = 0;
+    BatchKernel fstep = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -966,6 +1018,12 @@ struct sfd_acoustic_batch {
 
 namespace {
 
+// Sets a batch's launch shape (kinds as apply_shape).
+int apply_batch_shape(sfd_acoustic_batch* b, const TunedShape& t) {
+    if (t.kind == 2) return sfd_acoustic_batch_set_fused(b, t.a, t.b);
+    return t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b) : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
+}
+
 // Autotuner grid class of a batch: radius, boundary and arithmetic, the shot
 // count and the total cells to the nearest power of 2.
 std::string batch_class(const sfd_acoustic_batch* b) {
@@ -978,7 +1036,7 @@ std::string batch_class(const sfd_acoustic_batch* b) {
 }
 
 // Tile tables and per-shot argument blocks (both ping-pong orders; the
-// tiled step runs in place and needs one).
+// two-launch tiled step runs in place and needs one).
 int build_batch_tables(sfd_acoustic_batch* b) {
     const int n = (int)b->plans.size();
     std::vector<AcousticStepArgs> args(2 * (size_t)n);
@@ -988,9 +1046,11 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = b->plans[s];
         for (int par = 0; par < 2; ++par) args[(size_t)par * n + s] = make_args(pl, par, 0);
-        // Batches start from zero fields, so the tiled sponge step may read
-        // its velocities as unabsorbed from the first step on.
+        // Batches start from zero fields, so the tiled sponge steps may read
+        // their velocities as unabsorbed from the first step on.
         if (b->tiled) tiled_args(pl, args[s], b->mode == MODE_FAST_SPONGE);
+        if (b->fused)
+            for (int par = 0; par < 2; ++par) fused_args(pl, args[(size_t)par * n + s], b->mode == MODE_FAST_SPONGE);
         const int nt = args[s].ntz * args[s].nchunk;
         base[s] = t;
         tshot.insert(tshot.end(), nt, s);
@@ -1024,6 +1084,8 @@ int enqueue_batch_steps(sfd_acoustic_batch* b, int nsteps) {
             hipLaunchKernelGGL(b->tvel, dim3(b->ntile), block, 0, b->ctx->stream, a);
             SFD_HIP(b->ctx, hipGetLastError());
             hipLaunchKernelGGL(b->tpres, dim3(b->ntile), block, 0, b->ctx->stream, a);
+        } else if (b->fused) {
+            hipLaunchKernelGGL(b->fstep, dim3(b->ntile), dim3(64 * b->fnw), 0, b->ctx->stream, a);
         } else {
             hipLaunchKernelGGL(b->kernel, dim3(b->ntile), dim3(64 * b->nw), 0, b->ctx->stream, a);
         }
@@ -1086,10 +1148,31 @@ int sfd_acoustic_batch_set_tiled(sfd_acoustic_batch* b, int tile_rows, int waves
         if (rc) return rc;
     }
     b->tiled = true;
+    b->fused = false;
     b->ttx = tx;
     b->tnw = nw;
     b->tvel = v;
     b->tpres = p;
+    return build_batch_tables(b);
+}
+
+int sfd_acoustic_batch_set_fused(sfd_acoustic_batch* b, int rows, int waves) {
+    if (!b) return SFD_EINVAL;
+    if (!tiled_mode_ok(b->mode))
+        return fail(b->ctx, SFD_ENOTSUP, "the one-launch step exists in fast arithmetic only");
+    const int tr = rows ? rows : kFusedShapes[0][0], nw = waves ? waves : kFusedShapes[0][1];
+    BatchKernel k;
+    if (!pick_fused(b->radius, tr, nw, b->mode, nullptr, &k))
+        return fail(b->ctx, SFD_EINVAL, "no batched one-launch step for %d rows x %d waves", tr, nw);
+    for (auto* pl : b->plans) {
+        const int rc = sfd_acoustic_plan_set_fused(pl, tr, nw);
+        if (rc) return rc;
+    }
+    b->tiled = false;
+    b->fused = true;
+    b->ftx = tr;
+    b->fnw = nw;
+    b->fstep = k;
     return build_batch_tables(b);
 }
 
@@ -1101,14 +1184,13 @@ int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch* b, int waves, int chunk_ro
     if (waves == 0 && chunk_rows == 0) {
         const auto it = b->ctx->tuning.find(batch_class(b));
         if (it != b->ctx->tuning.end()) {
-            const TunedShape t = it->second;
-            const int rc = t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b)
-                                       : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
+            const int rc = apply_batch_shape(b, it->second);
             if (rc == SFD_OK) return rc;
         }
         if (tiled_mode_ok(b->mode)) return sfd_acoustic_batch_set_tiled(b, kTiledTX, kTiledNW);
     }
     b->tiled = false;
+    b->fused = false;
     // Batches hold far more cells than one shot, so the default chunk is
     // longer (less queue fill per cell) while the grid still fills the GPU.
     const int nw = waves ? waves : 2;
@@ -1166,8 +1248,8 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
     if (how != SFD_BATCH_PER_SHOT) {
         for (auto* pl : b->plans) {
-            if (b->tiled) pl->v_raw = b->nt > 0 && b->mode == MODE_FAST_SPONGE;
-            else pl->cur = b->nt & 1;
+            if (b->tiled || b->fused) pl->v_raw = b->nt > 0 && b->mode == MODE_FAST_SPONGE;
+            if (!b->tiled) pl->cur = b->nt & 1;
         }
     }
     return SFD_OK;
@@ -1193,8 +1275,9 @@ int sfd_acoustic_batch_destroy(sfd_acoustic_batch* b) {
 
 namespace {
 
-// Candidates: every built tiled shape (fast arithmetic) and the march kernel
-// at 1, 2 and 4 waves with its automatic chunk.
+// Candidates: every built tiled shape and one-launch tiled shape (fast
+// arithmetic) and the march kernel at 1, 2 and 4 waves with its automatic
+// chunk.
 std::vector<TunedShape> shape_candidates(int mode, bool batch) {
     std::vector<TunedShape> c;
     if (tiled_mode_ok(mode)) {
@@ -1204,6 +1287,7 @@ std::vector<TunedShape> shape_candidates(int mode, bool batch) {
             for (auto& t : batched) c.push_back({0, t[0], t[1], 0.f});
         else
             for (auto& t : single) c.push_back({0, t[0], t[1], 0.f});
+        for (auto& t : kFusedShapes) c.push_back({2, t[0], t[1], 0.f});
     }
     for (int w : {1, 2, 4}) c.push_back({1, w, 0, 0.f});
     return c;
@@ -1272,7 +1356,7 @@ int sfd_acoustic_batch_autotune(sfd_acoustic_batch* b, int trial_steps, float* b
     for (auto& t : cand) t.us = 3.4e38f;
     for (int pass = 0; pass < 2; ++pass)
         for (TunedShape& t : cand) {
-            rc = t.kind == 0 ? sfd_acoustic_batch_set_tiled(b, t.a, t.b) : sfd_acoustic_batch_set_tiling(b, t.a, t.b);
+            rc = apply_batch_shape(b, t);
             if (rc) return rc;
             if (t.kind == 1) t.b = b->plans[0]->chunk;
             float us = 0.f;
@@ -1283,8 +1367,7 @@ int sfd_acoustic_batch_autotune(sfd_acoustic_batch* b, int trial_steps, float* b
     TunedShape best = cand[0];
     for (const TunedShape& t : cand)
         if (t.us < best.us) best = t;
-    rc = best.kind == 0 ? sfd_acoustic_batch_set_tiled(b, best.a, best.b)
-                        : sfd_acoustic_batch_set_tiling(b, best.a, best.b);
+    rc = apply_batch_shape(b, best);
     if (!rc) rc = run(-1);
     if (rc) return rc;
     SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1295,17 +1378,17 @@ int sfd_acoustic_batch_autotune(sfd_acoustic_batch* b, int trial_steps, float* b
 
 int sfd_acoustic_plan_get_shape(const sfd_acoustic_plan* pl, int* kind, int* a, int* b) {
     if (!pl || !kind || !a || !b) return SFD_EINVAL;
-    *kind = pl->ttx > 0 ? 0 : 1;
-    *a = pl->ttx > 0 ? pl->ttx : pl->nw;
-    *b = pl->ttx > 0 ? pl->tnw : pl->chunk;
+    *kind = pl->ttx > 0 ? 0 : pl->ftx > 0 ? 2 : 1;
+    *a = pl->ttx > 0 ? pl->ttx : pl->ftx > 0 ? pl->ftx : pl->nw;
+    *b = pl->ttx > 0 ? pl->tnw : pl->ftx > 0 ? pl->fnw : pl->chunk;
     return SFD_OK;
 }
 
 int sfd_acoustic_batch_get_shape(const sfd_acoustic_batch* bt, int* kind, int* a, int* b) {
     if (!bt || !kind || !a || !b) return SFD_EINVAL;
-    *kind = bt->tiled ? 0 : 1;
-    *a = bt->tiled ? bt->ttx : bt->nw;
-    *b = bt->tiled ? bt->tnw : bt->plans[0]->chunk;
+    *kind = bt->tiled ? 0 : bt->fused ? 2 : 1;
+    *a = bt->tiled ? bt->ttx : bt->fused ? bt->ftx : bt->nw;
+    *b = bt->tiled ? bt->tnw : bt->fused ? bt->fnw : bt->plans[0]->chunk;
     return SFD_OK;
 }
 

@@ -1,0 +1,8 @@
+// One-launch tiled acoustic step, radius 4 instances.
+#include "acoustic_fused_inst.h"
+
+namespace sfd {
+bool pick_fused_r4(int rows, int nw, int mode, StepKernel* k, BatchKernel* b) {
+    return fused_pick_r<4>(rows, nw, mode, k, b);
+}
+}  // namespace sfd

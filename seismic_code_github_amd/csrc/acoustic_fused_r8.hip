@@ -1,0 +1,8 @@
+// One-launch tiled acoustic step, radius 8 instances.
+#include "acoustic_fused_inst.h"
+
+namespace sfd {
+bool pick_fused_r8(int rows, int nw, int mode, StepKernel* k, BatchKernel* b) {
+    return fused_pick_r<8>(rows, nw, mode, k, b);
+}
+}  // namespace sfd

@@ -18,6 +18,14 @@ BatchKernel pick_batch_kernel(int radius, int nw, int mode);
 // batches the default shape only.
 bool pick_tiled(int radius, int tx, int nw, int mode, StepKernel* v, StepKernel* p);
 bool pick_tiled_batch(int radius, int tx, int nw, int mode, BatchKernel* v, BatchKernel* p);
+// One-launch tiled step (acoustic_fused.h) of `mode` (C-PML or sponge) at
+// `rows` rows x 64 columns per workgroup of nw waves; k and b (either may be
+// null) receive the single-plan and the batched kernel.  False where no such
+// instance exists.
+bool pick_fused(int radius, int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
+// Launch shapes (tile rows, waves) the one-launch step is built for; the
+// first is the default.
+constexpr int kFusedShapes[][2] = {{32, 8}, {16, 4}, {16, 8}, {32, 4}};
 int pad_rows(int radius);
 int round_chunk(int ch, int radius);
 
@@ -53,6 +61,10 @@ struct sfd_acoustic_plan {
     // instead (acoustic_tiled.h): tile rows and waves, 0 = the march kernel.
     int ttx = 0, tnw = 0;
     StepKernel tvel = nullptr, tpres = nullptr;
+    // Or the one-launch tiled step (acoustic_fused.h, ping-pong buffers):
+    // tile rows and waves per workgroup, 0 = not selected.
+    int ftx = 0, fnw = 0;
+    StepKernel fstep = nullptr;
     // The velocity fields hold unabsorbed values (after a tiled sponge step).
     bool v_raw = false;
     // Fast arithmetic with uniform density: bf of every updated cell (the
@@ -94,6 +106,9 @@ int enqueue_steps(sfd_acoustic_plan* pl, int k0, int nsteps);
 // Turns step arguments into those of the tiled step: in place on the input
 // buffers, the tiled tile counts, velocities read as stored (v_raw).
 void tiled_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw);
+// Turns step arguments into those of the one-launch tiled step (ping-pong,
+// its tile counts, velocities read as stored).
+void fused_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw);
 // Enqueues the two launches of one tiled step of a plan.
 int launch_tiled(const sfd_acoustic_plan* pl, StepKernel v, StepKernel p, const AcousticStepArgs& a);
 // Tiled step shape used unless one is asked for.

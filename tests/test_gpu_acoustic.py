@@ -26,6 +26,8 @@ from conftest import oracle_threads
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 TOL = 1e-5  # relative L2 bound of FP32 results, gathers and snapshots
+# one-launch tiled step shapes (tile rows, waves per workgroup)
+FUSED_SHAPES = ((32, 8), (16, 4), (16, 8), (32, 4))
 
 
 def rel_l2(a, b):
@@ -172,16 +174,20 @@ def test_cpml_tiled_step_bitwise_equals_march(ctx, radius):
     r0, s0 = ref.fetch()
     st0, pm0 = ref.get_state()
     assert np.abs(r0).max() > 0
-    for tx, nw in ((16, 8), (8, 4), (8, 8), (16, 4), (16, 16), (32, 8), (32, 16)):
+    shapes = [("tiled", tx, nw) for tx, nw in ((16, 8), (8, 4), (8, 8), (16, 4), (16, 16), (32, 8), (32, 16))]
+    shapes += [("fused", r, nw) for r, nw in FUSED_SHAPES]
+    for kind, a, b in shapes:
         pl = sfd.AcousticPlan(shot, ctx)
-        pl.set_tiled(tx, nw)
+        pl.set_tiled(a, b) if kind == "tiled" else pl.set_fused(a, b)
         pl.run()
         r1, s1 = pl.fetch()
         st1, pm1 = pl.get_state()
-        np.testing.assert_array_equal(r1, r0)
-        np.testing.assert_array_equal(s1, s0)
-        np.testing.assert_array_equal(st1, st0)
-        np.testing.assert_array_equal(pm1, pm0)
+        msg = f"{kind} {a}x{b}"
+        np.testing.assert_array_equal(r1, r0, err_msg=msg)
+        np.testing.assert_array_equal(s1, s0, err_msg=msg)
+        np.testing.assert_array_equal(st1, st0, err_msg=msg)
+        np.testing.assert_array_equal(pm1, pm0, err_msg=msg)
+        pl.close()
 
 
 @pytest.mark.parametrize("radius", [4, 5, 8])
@@ -199,15 +205,19 @@ def test_sponge_tiled_step_bitwise_equals_march(ctx, radius):
     r0, s0 = ref.fetch()
     st0, _ = ref.get_state()
     assert np.abs(r0).max() > 0 and np.abs(s0[:, 1:]).max() > 0
-    for tx, nw in ((16, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 8), (32, 16)):
+    shapes = [("tiled", tx, nw) for tx, nw in ((16, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 8), (32, 16))]
+    shapes += [("fused", r, nw) for r, nw in FUSED_SHAPES]
+    for kind, a, b in shapes:
         pl = sfd.AcousticPlan(shot, ctx)
-        pl.set_tiled(tx, nw)
+        pl.set_tiled(a, b) if kind == "tiled" else pl.set_fused(a, b)
         pl.run()
         r1, s1 = pl.fetch()
         st1, _ = pl.get_state()
-        np.testing.assert_array_equal(r1, r0, err_msg=f"{tx}x{nw}")
-        np.testing.assert_array_equal(s1, s0, err_msg=f"{tx}x{nw}")
-        np.testing.assert_array_equal(st1, st0, err_msg=f"{tx}x{nw}")
+        msg = f"{kind} {a}x{b}"
+        np.testing.assert_array_equal(r1, r0, err_msg=msg)
+        np.testing.assert_array_equal(s1, s0, err_msg=msg)
+        np.testing.assert_array_equal(st1, st0, err_msg=msg)
+        pl.close()
 
 
 def test_sponge_tiled_switches_and_resume(ctx):
@@ -230,7 +240,15 @@ def test_sponge_tiled_switches_and_resume(ctx):
     b.step(0, 150)
     b.set_tiled()
     b.step(150, 250)
-    for pl, name in ((a, "tiled->march"), (b, "march->tiled")):
+    e = sfd.AcousticPlan(shot, ctx)  # tiled, then one-launch, then march, then one-launch
+    e.step(0, 101)
+    e.set_fused(16, 8)
+    e.step(101, 100)
+    e.set_tiled(-1)
+    e.step(201, 51)
+    e.set_fused(16, 4)
+    e.step(252, 148)
+    for pl, name in ((a, "tiled->march"), (b, "march->tiled"), (e, "tiled->fused->march->fused")):
         r1, s1 = pl.fetch()
         st1, _ = pl.get_state()
         np.testing.assert_array_equal(r1, r0, err_msg=name)
@@ -239,13 +257,16 @@ def test_sponge_tiled_switches_and_resume(ctx):
     c = sfd.AcousticPlan(shot, ctx)
     c.step(0, 200)
     mid, _ = c.get_state()
-    d = sfd.AcousticPlan(shot, ctx)
-    d.set_state(mid)
-    d.step(200, 200)
-    st_d, _ = d.get_state()
-    r_d, _ = d.fetch()
-    np.testing.assert_array_equal(st_d, st0)
-    np.testing.assert_array_equal(r_d[:, 200:], r0[:, 200:])
+    for kind in ("tiled", "fused"):
+        d = sfd.AcousticPlan(shot, ctx)
+        if kind == "fused":
+            d.set_fused()
+        d.set_state(mid)
+        d.step(200, 200)
+        st_d, _ = d.get_state()
+        r_d, _ = d.fetch()
+        np.testing.assert_array_equal(st_d, st0, err_msg=kind)
+        np.testing.assert_array_equal(r_d[:, 200:], r0[:, 200:], err_msg=kind)
 
 
 def test_cpml_state_resume(ctx):
@@ -257,13 +278,16 @@ def test_cpml_state_resume(ctx):
     a = sfd.AcousticPlan(shot, ctx)
     a.step(0, 120)
     mid, pml_mid = a.get_state()
-    b = sfd.AcousticPlan(shot, ctx)
-    b.set_state(mid, pml_mid)
-    b.step(120, 180)
-    st_b, pml_b = b.get_state()
     assert np.abs(st_full).max() > 0 and np.abs(pml_full).max() > 0
-    np.testing.assert_array_equal(st_b, st_full)
-    np.testing.assert_array_equal(pml_b, pml_full)
+    for kind in ("tiled", "fused"):
+        b = sfd.AcousticPlan(shot, ctx)
+        if kind == "fused":
+            b.set_fused(32, 8)
+        b.set_state(mid, pml_mid)
+        b.step(120, 180)
+        st_b, pml_b = b.get_state()
+        np.testing.assert_array_equal(st_b, st_full, err_msg=kind)
+        np.testing.assert_array_equal(pml_b, pml_full, err_msg=kind)
     # the psi fields match the oracle's full-grid memory variables
     o = run_oracle(shot)
     assert rel_l2(pml_full, o["pml"]) < 1e-4

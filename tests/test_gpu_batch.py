@@ -27,11 +27,12 @@ def small_batch(boundary="cpml", nshot=5, nt=300, arith="fast"):
 
 
 @pytest.mark.parametrize("boundary,tiling", [("cpml", "march"), ("cpml", "default"), ("sponge", "march"),
-                                             ("sponge", "default")])
+                                             ("sponge", "default"), ("cpml", "fused"), ("sponge", "fused")])
 def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
     """tiling "default": batches run the two-launch tiled step (in place;
-    sponge velocities kept unabsorbed between steps), the single plans the
-    fused march kernel; still bit for bit."""
+    sponge velocities kept unabsorbed between steps), "fused" the one-launch
+    tiled step (ping-pong), the single plans the fused march kernel; still
+    bit for bit."""
     shots = small_batch(boundary)
     print(f"case: batch modes {boundary} {tiling}, shots {[s.vp.shape for s in shots]}", flush=True)
     singles = []
@@ -44,6 +45,8 @@ def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
     b = sfd.AcousticBatch(shots, ctx)
     if tiling == "march":
         b.set_tiling(2, 48)
+    elif tiling == "fused":
+        b.set_fused(16, 4)
     for how in (sfd.BATCH_GRAPH, sfd.BATCH_LAUNCHES, sfd.BATCH_PER_SHOT):
         b.run(how)
         for s in range(len(shots)):
