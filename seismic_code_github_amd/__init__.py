@@ -365,7 +365,12 @@ class AcousticShot:
     nt: Optional[int] = None
     _keep: list = field(default_factory=list, repr=False)
 
-    def params(self) -> AcousticParamsC:
+    def params(self, keep: Optional[list] = None) -> AcousticParamsC:
+        """The C struct.  Its receiver, snapshot and coefficient arrays live as
+        long as the list they are appended to: ``keep`` (owned by the caller,
+        which must hold it until the library call returns; pass one list for
+        every struct made for the same call) or, without one, the shot's own
+        list, replaced by the next call."""
         vp = _f32(self.vp)
         nx, nz = vp.shape
         nt = int(self.nt if self.nt is not None else np.asarray(self.wavelet).shape[0])
@@ -373,7 +378,9 @@ class AcousticShot:
         rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
         steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
         coef = None if self.coef is None else _f32(self.coef)
-        self._keep = [rx, rz, steps, coef]
+        if keep is None:
+            self._keep = keep = []
+        keep += [rx, rz, steps, coef]
         return AcousticParamsC(
             nx, nz, nt, self.dx, self.dz, self.dt, self.radius, _ptr(coef),
             {"sponge": BOUNDARY_SPONGE, "cpml": BOUNDARY_CPML}[self.boundary], self.width,
@@ -393,7 +400,8 @@ class AcousticPlan:
         self.ctx = ctx or default_context()
         self.ctx.adopt(self)
         self.shot = shot
-        self.p = shot.params()
+        self._pkeep = []
+        self.p = shot.params(self._pkeep)
         self.nx, self.nz = np.asarray(shot.vp).shape
         self.nt = self.p.nt
         vp = _f32(shot.vp)
@@ -481,7 +489,8 @@ def acoustic_forward(shot: AcousticShot, ctx: Optional[Context] = None):
     """One shot end to end through ``sfd_acoustic_forward``.
     Returns (record [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
     ctx = ctx or default_context()
-    p = shot.params()
+    pkeep = []
+    p = shot.params(pkeep)
     nx, nz = np.asarray(shot.vp).shape
     vp = _f32(shot.vp)
     den = None if shot.den is None else _f32(shot.den)
@@ -499,7 +508,8 @@ def acoustic_gradient(shot: AcousticShot, seed, ckpt_every: int = 50, ctx: Optio
     (discrete adjoint of the fast sponge step, checkpoints every
     ``ckpt_every`` steps).  Returns (grad [nx, nz], record [nrec, nt])."""
     ctx = ctx or default_context()
-    p = shot.params()
+    pkeep = []
+    p = shot.params(pkeep)
     nx, nz = np.asarray(shot.vp).shape
     vp = _f32(shot.vp)
     den = None if shot.den is None else _f32(shot.den)
@@ -535,7 +545,7 @@ def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: 
     ws, ds = (_FP * n)(), (_FP * n)()
     keep = []
     for i, sh in enumerate(shots):
-        params[i] = sh.params()
+        params[i] = sh.params(keep)
         w = _f32(sh.wavelet)
         d = _f32(d_obs[i])
         if d.shape != (params[i].nrec, params[i].nt):
@@ -580,7 +590,7 @@ class AcousticBatch:
         self._keep = []
         vps, dens, ws = (_FP * n)(), (_FP * n)(), (_FP * n)()
         for i, sh in enumerate(self.shots):
-            params[i] = sh.params()
+            params[i] = sh.params(self._keep)
             vp = _f32(sh.vp)
             w = _f32(sh.wavelet)
             den = None if sh.den is None else _f32(sh.den)
@@ -684,14 +694,17 @@ class ViscoShot:
     qs: Optional[np.ndarray] = None
     _keep: list = field(default_factory=list, repr=False)
 
-    def params(self) -> ViscoParamsC:
+    def params(self, keep: Optional[list] = None) -> ViscoParamsC:
+        """The C struct; its arrays live in ``keep`` (as AcousticShot.params)."""
         nx, nz = np.asarray(self.vp).shape
         rx = np.ascontiguousarray(self.rec[0], dtype=np.int32)
         rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
         steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
         coef = None if self.coef is None else _f32(self.coef)
         qs = None if self.qs is None else _f32(self.qs)
-        self._keep = [rx, rz, steps, coef, qs]
+        if keep is None:
+            self._keep = keep = []
+        keep += [rx, rz, steps, coef, qs]
         return ViscoParamsC(nx, nz, int(self.nt), -1 if self.delay is None else int(self.delay),
                             self.dx, self.dz, self.dt, self.f0, self.radius, _ptr(coef), self.nmech,
                             self.q_scale, self.f_low, self.f_high, *[int(v) for v in self.sponge], self.sponge_r,
@@ -727,7 +740,8 @@ class ViscoPlan:
         self.ctx = ctx or default_context()
         self.ctx.adopt(self)
         self.shot = shot
-        self.p = shot.params()
+        self._pkeep = []
+        self.p = shot.params(self._pkeep)
         self.nx, self.nz = np.asarray(shot.vp).shape
         self._arr = shot._arrays()
         vp, vs, rho, qp, w = self._arr
@@ -794,7 +808,8 @@ def visco_forward(shot: ViscoShot, ctx: Optional[Context] = None):
     """One visco-acoustic shot through ``sfd_visco_forward``.
     Returns (record [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
     ctx = ctx or default_context()
-    p = shot.params()
+    pkeep = []
+    p = shot.params(pkeep)
     nx, nz = np.asarray(shot.vp).shape
     vp, vs, rho, qp, w = shot._arrays()
     record = np.zeros((p.nrec, p.nt), dtype=np.float32)
@@ -841,13 +856,16 @@ class ElasticShot:
     sponge_zlo: int = 0
     _keep: list = field(default_factory=list, repr=False)
 
-    def params(self) -> ElasticParamsC:
+    def params(self, keep: Optional[list] = None) -> ElasticParamsC:
+        """The C struct; its arrays live in ``keep`` (as AcousticShot.params)."""
         nx, nz = np.asarray(self.vp).shape
         it = np.ascontiguousarray(self.itopo, dtype=np.int32)
         rx = np.ascontiguousarray(self.rec[0], dtype=np.int32)
         rz = np.ascontiguousarray(self.rec[1], dtype=np.int32)
         steps = np.ascontiguousarray(list(self.snap_steps), dtype=np.int32)
-        self._keep = [it, rx, rz, steps]
+        if keep is None:
+            self._keep = keep = []
+        keep += [it, rx, rz, steps]
         return ElasticParamsC(nx, nz, int(self.nt), -1 if self.delay is None else int(self.delay),
                               self.dx, self.dz, self.dt, self.f0, _ptr(it, ctypes.c_int),
                               *[int(v) for v in self.sponge], self.sponge_r, int(self.src[0]), int(self.src[1]),
@@ -877,7 +895,8 @@ class ElasticPlan:
         self.ctx = ctx or default_context()
         self.ctx.adopt(self)
         self.shot = shot
-        self.p = shot.params()
+        self._pkeep = []
+        self.p = shot.params(self._pkeep)
         self.nx, self.nz = np.asarray(shot.vp).shape
         self._arr = shot._arrays()
         vp, vs, rho, w = self._arr
@@ -941,7 +960,8 @@ def elastic_forward(shot: ElasticShot, ctx: Optional[Context] = None):
     """One elastic shot through ``sfd_elastic_forward``.
     Returns (record_vx, record_vz [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
     ctx = ctx or default_context()
-    p = shot.params()
+    pkeep = []
+    p = shot.params(pkeep)
     nx, nz = np.asarray(shot.vp).shape
     vp, vs, rho, w = shot._arrays()
     rvx = np.zeros((p.nrec, p.nt), dtype=np.float32)

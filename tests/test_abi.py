@@ -88,3 +88,27 @@ def test_every_struct_mirror_matches_the_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(cls), cname
         for f in cls._fields_:
             assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
+
+
+def test_params_arrays_stay_alive_for_repeated_shots():
+    """The same shot object twice in one batch or gradient call: the struct of
+    the first entry must still point at live receiver arrays after the second
+    struct is made (each struct's arrays go to the caller's keep list)."""
+    import gc
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    shot = W.layered_301(nt=10)
+    keep = []
+    params = (sfd.AcousticParamsC * 2)()
+    for i in range(2):
+        params[i] = shot.params(keep)
+    gc.collect()
+    junk = [np.full(301, -7, dtype=np.int32) for _ in range(64)]  # reuse freed memory, if any
+    rx = np.asarray(shot.rec[0], dtype=np.int32)
+    rz = np.asarray(shot.rec[1], dtype=np.int32)
+    for i in range(2):
+        got_x = np.ctypeslib.as_array(params[i].rx, shape=(params[i].nrec,))
+        got_z = np.ctypeslib.as_array(params[i].rz, shape=(params[i].nrec,))
+        np.testing.assert_array_equal(got_x, rx)
+        np.testing.assert_array_equal(got_z, rz)
+    assert len(keep) == 8 and len(junk) == 64
