@@ -223,7 +223,7 @@ int sfd_context_load_tuning(sfd_context* ctx, const char* path) {
         TunedShape t;
         if (e.kind != JValue::OBJ || !a || a->kind != JValue::STR || !c || c->kind != JValue::STR ||
             !as_int(e.get("kind"), &t.kind) || !as_int(e.get("a"), &t.a) || !as_int(e.get("b"), &t.b) ||
-            (t.kind != 0 && t.kind != 1) || t.a <= 0 || t.b <= 0)
+            t.kind < 0 || t.kind > 2 || t.a <= 0 || t.b <= 0)
             return fail(ctx, SFD_EINVAL, "%s: entry %zu is malformed", path, i);
         const JValue* us = e.get("us_per_step");
         t.us = (us && us->kind == JValue::NUM) ? (float)us->num : 0.f;

@@ -1,7 +1,8 @@
 """Tile-shape autotuner (sfd_acoustic_plan_autotune / sfd_acoustic_batch_autotune)
 and its JSON cache (sfd_context_save_tuning / sfd_context_load_tuning).
 
-* The autotuner picks one of the built shapes, records it under the grid
+* The autotuner picks one of the built shapes (two-launch tiled, one-launch
+  tiled, march), records it under the grid
   class, and later plans / batches of that class in the context start with it.
 * Every shape gives bitwise the same result, so a tuned run equals the
   default run (acoustic, elastic, VTI); the visco-acoustic candidates (tiled
@@ -21,6 +22,7 @@ from seismic_code_github_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 TILED = {(8, 4), (8, 8), (16, 4), (16, 8), (16, 16), (32, 8), (32, 16)}
+FUSED = {(32, 8), (16, 4), (16, 8), (32, 4)}  # one-launch tiled step (tile rows, waves)
 
 
 def small(boundary="cpml", **kw):
@@ -40,7 +42,8 @@ def test_plan_autotune_keeps_results_and_caches(boundary):
         kind, a, b = pl.shape()
         print(f"tuned: kind {kind} ({a}, {b}) {us:.2f} us/step", flush=True)
         assert us > 0
-        assert (kind == 0 and (a, b) in TILED) or (kind == 1 and a in (1, 2, 4) and b > 0)
+        assert ((kind == 0 and (a, b) in TILED) or (kind == 1 and a in (1, 2, 4) and b > 0)
+                or (kind == 2 and (a, b) in FUSED))
         assert ctx.tuning_size() == 1
         pl.run()
         r1, _ = pl.fetch()
@@ -127,7 +130,8 @@ def test_batch_autotune_keeps_results(boundary):
         us = b.autotune(10)
         kind, x, y = b.shape()
         print(f"tuned: kind {kind} ({x}, {y}) {us:.2f} us/step", flush=True)
-        assert (kind == 0 and (x, y) in {(8, 4), (16, 4), (16, 8), (32, 8)}) or kind == 1
+        assert ((kind == 0 and (x, y) in {(8, 4), (16, 4), (16, 8), (32, 8)}) or kind == 1
+                or (kind == 2 and (x, y) in FUSED))
         for how in (sfd.BATCH_GRAPH, sfd.BATCH_LAUNCHES):
             b.run(how)
             for s in range(len(shots)):
