@@ -576,6 +576,24 @@ int sfd_write_su_shot(const char *path, const float *record, int ntrace, int nt,
 int sfd_merge_su(const char *const *shot_files, int nshot, int ntrace, int nt, const char *out_path,
                  int remove_inputs);
 
+/* ---------------------------------------------------------------- allocation-free shot
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sfd_acoustic_forward in fast arithmetic, allocating
+ * nothing.  Inputs and outputs are DEVICE arrays: d_vp [nx*nz], d_den
+ * [nx*nz] or NULL (2400 kg/m^3), d_wavelet [nt], d_rx / d_rz [nrec]
+ * receiver cells, d_record [nrec][nt]; p->rx / p->rz are not read and
+ * p->nsnap must be 0.  The wavefield, folded model and tables live in one
+ * static device array per GPU of sfd_noalloc_scratch_bytes() bytes (32 MiB),
+ * held until the process ends: grids that need more are rejected with
+ * SFD_EINVAL (about 1.7 M cells at radius 8).  Every call on a GPU shares
+ * that array, so calls that overlap on one GPU race.  The gather is bitwise
+ * the one sfd_acoustic_forward records (same step kernels).  Synchronous. */
+int sfd_acoustic_forward_noalloc(sfd_context *ctx, const sfd_acoustic_params *p, const float *d_vp,
+                                 const float *d_den, const float *d_wavelet, const int *d_rx, const int *d_rz,
+                                 float *d_record);
+size_t sfd_noalloc_scratch_bytes(void);
+
 /* ---------------------------------------------------------------- measurement
  * Streaming-bandwidth probe (no counterpart in the reference): the rate a
  * hand-written kernel streams a device buffer of `bytes` bytes at (read: sum

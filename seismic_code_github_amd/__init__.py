@@ -198,6 +198,10 @@ _SIGNATURES = {
     "sfd_srm_plan_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_srm_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_srm_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_acoustic_forward_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AcousticParamsC),
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "sfd_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "sfd_bandwidth_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_double)]),
     "sfd_srm_plan_set_method": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

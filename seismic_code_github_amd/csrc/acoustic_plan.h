@@ -28,6 +28,8 @@ bool pick_fused(int radius, int rows, int nw, int mode, StepKernel* k, BatchKern
 constexpr int kFusedShapes[][2] = {{32, 8}, {16, 4}, {16, 8}, {32, 4}};
 int pad_rows(int radius);
 int round_chunk(int ch, int radius);
+// Host-side argument checks of an acoustic shot (host receiver arrays).
+int validate(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* wavelet);
 
 }  // namespace sfd
 

@@ -127,6 +127,10 @@ int time_steps(sfd_context* ctx, int n, Run run, float* us) {
     return SFD_OK;
 }
 
+// Looks up the device address of the allocation-free entry point's scratch
+// array on ctx's device (once per device; the device must be current).
+int noalloc_init(sfd_context* ctx);
+
 // Nearest power of 2 of n, as an exponent (autotuner grid classes).
 int log2_round(double n);
 

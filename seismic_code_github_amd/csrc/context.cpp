@@ -121,6 +121,10 @@ int sfd_context_create(int device, sfd_context** out) {
         return SFD_EDEVICE;
     }
     ctx->arch = prop.gcnArchName;
+    if (sfd::noalloc_init(ctx) != SFD_OK) {
+        sfd_context_destroy(ctx);
+        return SFD_EDEVICE;
+    }
     *out = ctx;
     return SFD_OK;
 }
