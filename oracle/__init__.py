@@ -235,7 +235,7 @@ class _VArgs(ctypes.Structure):
         ("nrec", ctypes.c_int), ("rx", ctypes.POINTER(ctypes.c_int)), ("rz", ctypes.POINTER(ctypes.c_int)),
         ("rec_delay", ctypes.c_int),
         ("nsnap", ctypes.c_int), ("snap_steps", ctypes.POINTER(ctypes.c_int)),
-        ("nthreads", ctypes.c_int),
+        ("nthreads", ctypes.c_int), ("init", ctypes.POINTER(ctypes.c_float)),
     ]
 
 
@@ -310,9 +310,11 @@ def visco_tau(nmech, f0, q, q_scale=0.9, f_low=0.0, f_high=0.0, precision="f32")
 def visco(vp, rho, qp, *, src, rec, nsteps, dx=10.0, dz=10.0, dt=1e-3, f0=20.0, vs=None, qs=None,
           radius=5, coef=None, nmech=1, q_scale=0.9, f_low=0.0, f_high=0.0, sponge=(40, 40, 40, 40),
           sponge_r=1e-3, src_sigma=1.5, wavelet=None, rec_delay=0, snap_steps=(), it0=1,
-          precision="f32", nthreads=1):
+          precision="f32", nthreads=1, init=None):
     """Run the visco-elastic RSG oracle (oracle/visco_body.h).  Fields
     [nx, nz] float32 (z fastest); sponge = (npxn, npxp, npzn, npzp).
+    init: None (start from rest, as the reference) or [5, nx, nz] initial
+    sxx, szz, sxz, vx, vz (the oracle's own plane-wave pins).
     Returns dict(rec_zz, rec_xx [nrec, nsteps - rec_delay],
     snaps [nsnap, 5, nx, nz] = sxx, szz, sxz, vx, vz)."""
     L = lib()
@@ -329,10 +331,13 @@ def visco(vp, rho, qp, *, src, rec, nsteps, dx=10.0, dz=10.0, dt=1e-3, f0=20.0, 
     w = f32(wavelet)
     if w is not None:
         assert w.shape[0] >= nsteps
+    ini = f32(init)
+    if ini is not None:
+        assert ini.shape == (5, nx, nz)
     args = _VArgs(nx, nz, nsteps, it0, dx, dz, dt, f0, radius, _ptr(coef), nmech, q_scale, f_low, f_high,
                   *sponge, sponge_r, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(qp), _ptr(qs),
                   int(src[0]), int(src[1]), src_sigma, _ptr(w), rx.size, _ptr(rx, ctypes.c_int),
-                  _ptr(rz, ctypes.c_int), rec_delay, steps.size, _ptr(steps, ctypes.c_int), nthreads)
+                  _ptr(rz, ctypes.c_int), rec_delay, steps.size, _ptr(steps, ctypes.c_int), nthreads, _ptr(ini))
     dtype, ct = (np.float32, ctypes.c_float) if precision == "f32" else (np.float64, ctypes.c_double)
     nrt = max(nsteps - rec_delay, 0)
     rzz = np.zeros((rx.size, nrt), dtype=dtype)

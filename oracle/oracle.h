@@ -122,6 +122,9 @@ typedef struct oracle_visco_args {
     int nsnap;
     const int *snap_steps;   /* step indices s (0-based) after which fields are copied */
     int nthreads;
+    const float *init;       /* NULL, or [5][nx*nz] initial sxx, szz, sxz (time 0) and vx, vz
+                                (time -1/2), memory variables zero: for the oracle's own pins
+                                (plane waves); the reference always starts from rest */
 } oracle_visco_args;
 
 /* rec_zz, rec_xx [nrec][nsteps - rec_delay] (reference record1 = strzz,

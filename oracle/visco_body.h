@@ -317,6 +317,20 @@ int FN(oracle_visco)(const oracle_visco_args *a, REAL *rec_zz, REAL *rec_xx, REA
     REAL *szzx = sp + 3 * n, *szzz = sp + 4 * n, *szzr = sp + 5 * n;
     REAL *sxzx = sp + 6 * n, *sxzz = sp + 7 * n, *sxzr = sp + 8 * n;
     REAL *d1 = der, *d2 = der + n, *d3 = der + 2 * n, *d4 = der + 3 * n;
+    if (a->init) {
+        /* Initial state in the x parts of the split fields (the z and r parts
+         * zero), mirrored into the full fields. */
+        const float *I = a->init;
+        for (int i = 0; i < nx; ++i)
+            for (int k = 0; k < nz; ++k) {
+                const size_t c = (size_t)i * nz + k;
+                sxxx[c] = UF(sxx, i, k) = (REAL)I[c];
+                szzx[c] = UF(szz, i, k) = (REAL)I[n + c];
+                sxzx[c] = UF(sxz, i, k) = (REAL)I[2 * n + c];
+                vxx[c] = UF(vx, i, k) = (REAL)I[3 * n + c];
+                vzx[c] = UF(vz, i, k) = (REAL)I[4 * n + c];
+            }
+    }
     for (int s = 0; s < a->nsteps; ++s) {
         const int it = a->it0 + s;
         /* ---- velocities from the old stresses (:747-773) ---- */

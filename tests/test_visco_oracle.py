@@ -11,7 +11,13 @@ restatement is pinned by what the mathematics fixes:
 * the tau-method against an independent numerical least-squares fit, and the
   Q(w) of the resulting standard-linear-solid model;
 * attenuation measured on propagated waves (spectral ratio) against Q;
-* the Gaussian source footprint and the cosine sponge tables' properties.
+* the Gaussian source footprint and the cosine sponge tables' properties;
+* vs > 0 (the visco-elastic terms): the lossless elastic energy with
+  heterogeneous lambda, mu and density; plane P and S pulses moving at vp and
+  vs with szz = (lam / M) sxx on the P wave; the shear pulse attenuated by
+  Qs, not Qp.  A swapped modulus in sxz, lambda = M - mu, a sign error in
+  sxz or tau_p in the shear memory variable each fails one of them (checked
+  by planting each mistake in the oracle).
 """
 import math
 
@@ -257,3 +263,170 @@ def test_fp32_matches_fp64():
     a = oracle.visco(vp, rho, q, precision="f32", **kw)["rec_zz"]
     b = oracle.visco(vp, rho, q, precision="f64", **kw)["rec_zz"]
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-5
+
+
+# ---------------------------------------------------------------- vs > 0 pins
+# The visco-elastic terms of calpstrxx/zz/xz (Q sub:993-1132), calprxx/zz/xz
+# (:1135-1269) and the shear tau of pcalpas (:473-551).
+
+def _rhoe(rho):
+    """substagger's 4-point density average (last row / column replicated),
+    FP64, independently of the oracle."""
+    r = rho.astype(np.float64)
+    a = r.copy()
+    a[:-1, :-1] = 0.25 * (((r[:-1, :-1] + r[:-1, 1:]) + r[1:, 1:]) + r[1:, :-1])
+    a[:, -1] = a[:, -2]
+    a[-1, :] = a[-2, :]
+    return a
+
+
+@pytest.mark.parametrize("density", ["uniform", "layered"])
+def test_elastic_energy_conserved_lossless_without_sponge(density):
+    """Q = infinity, vs > 0 and heterogeneous lambda, mu, no sponge: after the
+    source stops the discrete energy
+      E = 1/2 sum [(M sxx^2 - 2 lam sxx szz + M szz^2) / (M^2 - lam^2) + sxz^2 / mu]
+        + 1/2 sum rho (v^(n-1/2) . v^(n+1/2)),   M = rho vp^2, mu = rho vs^2, lam = M - 2 mu
+    (rho the 4-point average) stays constant to rounding.  A swapped modulus,
+    a sign error in sxz or a wrong lambda = M - 2 mu in either normal stress
+    breaks the symmetry of the stress-strain map and the identity with it."""
+    nx, nz = 72, 64
+    rng = np.random.default_rng(11)
+    from scipy.ndimage import gaussian_filter
+    vp = gaussian_filter(2200 + 900 * rng.random((nx, nz)), 3).astype(np.float32)
+    vs = (vp / (1.6 + 0.6 * gaussian_filter(rng.random((nx, nz)), 3))).astype(np.float32)
+    rho = np.full((nx, nz), 2100.0, np.float32)
+    if density == "layered":
+        rho[:, 30:] = 2500.0
+        rho[:, 45:] = 1900.0
+    q = np.full((nx, nz), np.inf, np.float32)
+    w = np.zeros(420, np.float32)
+    t = np.arange(60) * 1e-3 - 0.03
+    w[:60] = (1 - 2 * (np.pi * 40 * t) ** 2) * np.exp(-(np.pi * 40 * t) ** 2)
+    steps = list(range(120, 420, 50))
+    snaps_at = sorted(set(steps + [s + 1 for s in steps[:-1]]))
+    o = oracle.visco(vp, rho, q, vs=vs, src=(36, 30), rec=(np.array([5], np.int32), np.array([5], np.int32)),
+                     nsteps=420, dt=1e-3, f0=20.0, sponge=(0, 0, 0, 0), wavelet=w, snap_steps=snaps_at,
+                     precision="f64", nthreads=oracle_threads())
+    sn = {s: o["snaps"][i] for i, s in enumerate(snaps_at)}
+    re = _rhoe(rho)
+    M = re * vp.astype(np.float64) ** 2
+    mu = re * vs.astype(np.float64) ** 2
+    lam = M - 2 * mu
+    energies, shear = [], []
+    for s in steps[:-1]:
+        a, b = sn[s], sn[s + 1]
+        sxx, szz, sxz = a[0], a[1], a[2]
+        w_el = (M * sxx ** 2 - 2 * lam * sxx * szz + M * szz ** 2) / (M ** 2 - lam ** 2) + sxz ** 2 / mu
+        e = 0.5 * np.sum(w_el) + 0.5 * np.sum(re * (a[3] * b[3] + a[4] * b[4]))
+        energies.append(e)
+        shear.append(0.5 * np.sum(sxz ** 2 / mu))
+    energies = np.array(energies)
+    assert energies[0] > 0
+    assert min(shear) > 1e-3 * energies[0]  # the shear terms carry energy
+    assert np.abs(energies / energies[0] - 1).max() < 1e-10, energies
+
+
+def _plane_wave(nx, nz, x0, sigma, wave, rho, vp, vs):
+    """Initial state [5, nx, nz] (sxx, szz, sxz, vx, vz) of a right-going
+    plane pulse, a Gaussian in x uniform in z: P: vx = g, sxx = -rho vp g,
+    szz = (lam / M) sxx; S: vz = g, sxz = -rho vs g (the one-way relations of
+    the continuous system; the half-step stagger leaves a faint left-going
+    remainder)."""
+    init = np.zeros((5, nx, nz), np.float32)
+    x = np.arange(nx, dtype=np.float64)
+    g = np.exp(-0.5 * ((x - x0) / sigma) ** 2)[:, None] * np.ones((1, nz))
+    if wave == "P":
+        M, lam = rho * vp ** 2, rho * vp ** 2 - 2 * rho * vs ** 2
+        init[3] = g
+        init[0] = -rho * vp * g
+        init[1] = -rho * vp * g * lam / M
+    else:
+        init[4] = g
+        init[2] = -rho * vs * g
+    return init
+
+
+def _centroid(u, lo):
+    e = u[lo:].astype(np.float64) ** 2
+    return lo + np.sum(np.arange(e.size) * e) / np.sum(e)
+
+
+@pytest.mark.parametrize("wave", ["P", "S"])
+def test_plane_wave_speed_and_stress_ratio(wave):
+    """A right-going plane P (vx) or S (vz) pulse, uniform in z, in a
+    homogeneous elastic medium (vp 3000, vs 1700) moves at vp or vs within
+    0.5 %; the P wave keeps szz = (lam / M) sxx at every cell (calpstrxx /
+    calpstrzz integrate d vx/dx with M and M - 2 mu) and no shear stress,
+    the S wave sxz only."""
+    nx, nz, x0 = 420, 240, 150
+    vp0, vs0, rho0 = 3000.0, 1700.0, 2000.0
+    vp = np.full((nx, nz), vp0, np.float32)
+    vs = np.full((nx, nz), vs0, np.float32)
+    rho = np.full((nx, nz), rho0, np.float32)
+    q = np.full((nx, nz), np.inf, np.float32)
+    init = _plane_wave(nx, nz, x0, 5.0, wave, rho0, vp0, vs0)
+    c = vp0 if wave == "P" else vs0
+    s1, s2 = 50, 150  # after the faint left-going remainder has separated
+    o = oracle.visco(vp, rho, q, vs=vs, src=(0, 0), rec=(np.array([1], np.int32), np.array([1], np.int32)),
+                     nsteps=s2 + 1, dt=1e-3, f0=20.0, sponge=(0, 0, 0, 0), wavelet=np.zeros(s2 + 1, np.float32),
+                     snap_steps=[s1, s2], precision="f64", nthreads=oracle_threads(), init=init)
+    row = nz // 2
+    fi = 3 if wave == "P" else 4
+    u1, u2 = o["snaps"][0][fi][:, row], o["snaps"][1][fi][:, row]
+    lo = x0 - 30
+    dist = (_centroid(u2, lo) - _centroid(u1, lo)) * 10.0
+    speed = dist / ((s2 - s1) * 1e-3)
+    assert abs(speed / c - 1) < 5e-3, (wave, speed)
+    sxx, szz, sxz = (o["snaps"][1][f][:, row - 5:row + 5] for f in range(3))
+    if wave == "P":
+        M = rho0 * vp0 ** 2
+        lam = M - 2 * rho0 * vs0 ** 2
+        scale = np.abs(sxx).max()
+        assert scale > 0
+        # (the float32 initial state carries 1e-8 relative rounding)
+        assert np.abs(szz - (lam / M) * sxx).max() < 1e-6 * scale
+        assert np.abs(sxz).max() < 1e-9 * scale
+    else:
+        scale = np.abs(sxz).max()
+        assert scale > 0
+        assert np.abs(sxx).max() < 1e-9 * scale and np.abs(szz).max() < 1e-9 * scale
+
+
+def test_shear_attenuation_follows_qs_not_qp():
+    """Right-going plane S pulse in a homogeneous visco-elastic medium (3
+    mechanisms, Qp = 200, Qs = 30): the spatial spectrum of the pulse, between two
+    times and divided by the same ratio of a lossless run, falls as
+    exp(-pi f dt / Q) with f = vs k / 2 pi.  The measured Q is Qs' = 0.9 Qs
+    within 15 % over 4-16 Hz (Qp' would be 6x larger: tau_p used for the
+    shear terms fails), and the P pulse in the same medium measures Qp'."""
+    nx, nz, x0 = 520, 240, 130
+    vp0, vs0, rho0 = 3000.0, 1700.0, 2000.0
+    vp = np.full((nx, nz), vp0, np.float32)
+    vs = np.full((nx, nz), vs0, np.float32)
+    rho = np.full((nx, nz), rho0, np.float32)
+    qp = np.full((nx, nz), 200.0, np.float32)
+    qs = np.full((nx, nz), 30.0, np.float32)
+    row, lo = nz // 2, x0 - 30
+    res = {}
+    for wave, fi, c, qv in (("S", 4, vs0, 30.0), ("P", 3, vp0, 200.0)):
+        init = _plane_wave(nx, nz, x0, 4.0, wave, rho0, vp0, vs0)
+        s1, s2 = 50, 330  # before the z edges' disturbance reaches the middle row
+        kw = dict(vs=vs, src=(0, 0), rec=(np.array([1], np.int32), np.array([1], np.int32)), nsteps=s2 + 1,
+                  dt=1e-3, f0=20.0, sponge=(0, 0, 0, 0), wavelet=np.zeros(s2 + 1, np.float32), nmech=3,
+                  snap_steps=[s1, s2], precision="f64", nthreads=oracle_threads(), init=init)
+        lossy = oracle.visco(vp, rho, qp, qs=qs, **kw)["snaps"]
+        clean = oracle.visco(vp, rho, np.full_like(qp, np.inf), **kw)["snaps"]
+
+        def spec(sn):
+            u = sn[fi][lo:, row]
+            return np.abs(np.fft.rfft(u, 4096))
+
+        ratio = (spec(lossy[1]) / spec(lossy[0])) / (spec(clean[1]) / spec(clean[0]))
+        k = 2 * np.pi * np.fft.rfftfreq(4096, 10.0)
+        f = c * k / (2 * np.pi)
+        band = (f > 4) & (f < 16)
+        dt_travel = (s2 - s1) * 1e-3
+        slope = np.polyfit(f[band], np.log(ratio[band]), 1)[0]
+        res[wave] = -np.pi * dt_travel / slope
+        assert abs(res[wave] / (0.9 * qv) - 1) < 0.15, (wave, res[wave])
+    assert res["P"] > 4 * res["S"]
