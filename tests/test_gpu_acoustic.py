@@ -72,13 +72,30 @@ def test_reference_program_fast(ctx):
     assert e32 <= TOL and e64 <= TOL and s32 <= TOL and s64 <= TOL
 
 
+def floor_bound(o32, o64):
+    """The parity bound of one FP32 result: 1e-5 relative L2 against FP64, or,
+    where the FP32 arithmetic itself cannot reach 1e-5 -- the FP32 oracle
+    (the reference's operation order) is more than 1e-5 / 3 from FP64 --
+    three times that distance.  Such fields are late-time residues: the fast
+    arithmetic folds rho vp^2 dt, dt / rho and C_m / dx into single floats,
+    systematic roundings of <= 2^-24 each that the reference commits in other
+    places (its float dt / dx and DEN Vp^2 dt); a coefficient bias delta
+    shifts the phase of a wave by omega t delta, up to ~4e-5 after 2 s at
+    25 Hz, so two FP32 evaluations differ from FP64 by amounts of one order
+    (measured: 1.2-2.2x the oracle's at step 1999).  Against the FP32 oracle
+    the triangle inequality gives four times the distance."""
+    d = rel_l2(o32, o64)
+    return max(TOL, 3.0 * d), max(TOL, 4.0 * d)
+
+
 @pytest.mark.parametrize("radius", [4, 8])
 def test_layered_301_fast(ctx, radius):
-    """301x301 interior, 40-cell sponge, nt=2000.  Gathers and the snapshots
-    taken while the wavefield is inside the model (steps 500, 999) are within
-    1e-5 of both the FP32 and the FP64 oracle.  The last snapshot (step 1999)
-    holds only the residue the sponge left (the direct wave has gone); there
-    the FP32 oracle is itself ~1e-5 from FP64, so it is held to 1e-4."""
+    """301x301 interior, 40-cell sponge, nt=2000.  Every gather
+    and snapshot is within floor_bound of both oracles: 1e-5 wherever the
+    FP32 oracle is itself within 1e-5/3 of FP64 (gathers, the snapshots
+    taken while the wavefield is inside the model), else 3x the FP32
+    oracle's distance -- the step-1999 snapshot, which holds only the residue
+    the sponge left (FP32 oracle 0.9-3.6e-5 from FP64 there)."""
     steps = (500, 999, 1999)
     shot = W.layered_301(radius=radius, snap_steps=steps)
     print(f"case: layered_301 R={radius} fast", flush=True)
@@ -93,12 +110,14 @@ def test_layered_301_fast(ctx, radius):
             out[f"snap{steps[s]}f{f}_64"] = rel_l2(snaps[s, f], o64["snaps"][s, f])
             out[f"oracle_snap{steps[s]}f{f}_32v64"] = rel_l2(o32["snaps"][s, f], o64["snaps"][s, f])
     print(f"layered_301 R={radius}: " + " ".join(f"{k}={v:.2e}" for k, v in out.items()))
-    assert out["rec32"] <= TOL and out["rec64"] <= TOL
-    for k, v in out.items():
-        if k.startswith("snap1999"):
-            assert v <= 10 * TOL, (k, v)
-        elif k.startswith("snap"):
-            assert v <= TOL, (k, v)
+    assert out["rec32"] <= TOL and out["rec64"] <= TOL  # the gathers meet 1e-5 outright
+    for s in range(len(steps)):
+        for f in range(3):
+            b64, b32 = floor_bound(o32["snaps"][s, f], o64["snaps"][s, f])
+            assert out[f"snap{steps[s]}f{f}_64"] <= b64, (steps[s], f, b64)
+            assert out[f"snap{steps[s]}f{f}_32"] <= b32, (steps[s], f, b32)
+            if steps[s] < 1999:  # inside the model: 1e-5 outright
+                assert max(out[f"snap{steps[s]}f{f}_64"], out[f"snap{steps[s]}f{f}_32"]) <= TOL
 
 
 def test_layered_301_reference_arith_bitwise(ctx):
@@ -157,8 +176,53 @@ def test_cpml_against_oracle(ctx):
     o64 = run_oracle(shot, "f64", snap_steps=[899])
     e32, e64 = rel_l2(rec, o32["record"]), rel_l2(rec, o64["record"])
     sp = [rel_l2(snaps[0, f], o64["snaps"][0, f]) for f in range(3)]
-    print(f"cpml: record vs f32 {e32:.2e} vs f64 {e64:.2e}; snap vs f64 {sp}")
-    assert e32 <= TOL and e64 <= TOL and max(sp) <= 10 * TOL
+    so = [rel_l2(o32["snaps"][0, f], o64["snaps"][0, f]) for f in range(3)]
+    print(f"cpml: record vs f32 {e32:.2e} vs f64 {e64:.2e}; snap vs f64 {sp} (oracle FP32 vs f64 {so})")
+    assert e32 <= TOL and e64 <= TOL
+    for f in range(3):
+        assert sp[f] <= floor_bound(o32["snaps"][0, f], o64["snaps"][0, f])[0], f
+
+
+def test_c1_full_grid_prefix(ctx):
+    """configs[1] at full size (2101 x 4101 stored, 50-cell C-PML, R = 8) over
+    the first 700 of its 6000 steps: the 4001-receiver gather and the P
+    snapshot at step 699 within 1e-5 of the FP32 and FP64 oracles."""
+    shot = W.smooth_random_2001x4001(nt=700, snap_steps=(699,), snap_fields=1)
+    print(f"case: c1 full grid {shot.vp.shape}, 700 steps", flush=True)
+    pl = sfd.AcousticPlan(shot, ctx)
+    pl.autotune(20)
+    pl.run()
+    rec, snaps = pl.fetch()
+    pl.close()
+    o32 = run_oracle(shot, snap_steps=[699])
+    o64 = run_oracle(shot, "f64", snap_steps=[699])
+    e = {"rec32": rel_l2(rec, o32["record"]), "rec64": rel_l2(rec, o64["record"]),
+         "snap32": rel_l2(snaps[0, 0], o32["snaps"][0, 0]), "snap64": rel_l2(snaps[0, 0], o64["snaps"][0, 0]),
+         "oracle_rec32v64": rel_l2(o32["record"], o64["record"])}
+    print("c1 full grid: " + " ".join(f"{k}={v:.2e}" for k, v in e.items()))
+    assert max(e["rec32"], e["rec64"], e["snap32"], e["snap64"]) <= TOL
+
+
+def test_c3_widest_window_full_size(ctx):
+    """configs[3]'s widest aperture window at full size (1101 rows, the
+    window plus its 50-cell C-PML columns) over 600 steps: alone and inside a
+    batch with two other windows (bitwise equal), within 1e-5 of the FP32
+    and FP64 oracles."""
+    _, shots = W.windowed_shots(nshot=64, nt=600)
+    order = np.argsort([s.vp.shape[0] for s in shots])
+    wide = shots[order[-1]]
+    print(f"case: c3 widest window {wide.vp.shape}, 600 steps", flush=True)
+    rec, _ = sfd.acoustic_forward(wide, ctx)
+    b = sfd.AcousticBatch([shots[order[0]], wide, shots[order[32]]], ctx)
+    b.run(sfd.BATCH_GRAPH)
+    np.testing.assert_array_equal(b.fetch(1), rec)
+    b.close()
+    o32 = run_oracle(wide)
+    o64 = run_oracle(wide, "f64")
+    e32, e64 = rel_l2(rec, o32["record"]), rel_l2(rec, o64["record"])
+    print(f"c3 widest window: record vs f32 {e32:.2e} vs f64 {e64:.2e} "
+          f"(oracle f32 vs f64 {rel_l2(o32['record'], o64['record']):.2e})")
+    assert e32 <= TOL and e64 <= TOL
 
 
 @pytest.mark.parametrize("radius", [4, 5, 8])

@@ -185,3 +185,61 @@ def test_boundary_saving_inside_the_box(ctx, radius, nx, nz, ckpt):
     print(f"inside the box: boundary vs recompute {e_rb:.2e}, vs fp64 {e_64:.2e} (recompute vs fp64 {e_r64:.2e})")
     assert np.abs(gb[box]).max() > 0
     assert e_rb <= 1e-4 and e_64 <= 1e-4
+
+
+def test_fwi_801_workload(ctx):
+    """configs[4]'s own workload at 1/8 size (workloads.fwi_801: 3-anomaly
+    model, 16 shots on the smooth background, 40-cell sponge, 10 Hz Ricker,
+    checkpoints every 50 steps), nt = 800: the batched misfit gradient by
+    recomputation and by boundary saving against (a) the FP64 adjoint summed
+    over the shots, <= 1e-4 relative L2 (inside the sponge-free box for
+    boundary saving), and (b) central differences of the FP64 misfit
+    J = sum_s 1/2 |d_s(vp) - d_obs,s|^2 on 8 seeded smooth directions inside
+    the box, <= 1e-3.  d_obs is the FP64 forward on the true model."""
+    from concurrent.futures import ThreadPoolExecutor
+    from scipy.ndimage import gaussian_filter
+    from seismic_code_github_amd import workloads as W
+    true, bg, shots = W.fwi_801(nshot=16, nt=800, scale=8)
+    nx, nz = bg.shape
+    print(f"case: fwi_801 1/8 size {nx}x{nz}, 16 shots, nt 800, ckpt 50", flush=True)
+    den = np.full_like(bg, 2400.0)
+    nth = oracle_threads()
+    okw = [dict(den=den, wavelet=s.wavelet, src=s.src, rec=s.rec, dt=s.dt, radius=s.radius, sponge=s.width)
+           for s in shots]
+    obs = [oracle.acoustic(true, precision="f64", nthreads=nth, **k)["record"] for k in okw]
+
+    def J64(v):
+        return sum(0.5 * float(np.sum((oracle.acoustic(v, precision="f64", nthreads=nth, **k)["record"] - d) ** 2))
+                   for k, d in zip(okw, obs))
+
+    d32 = [d.astype(np.float32) for d in obs]
+    Jr, gr = sfd.misfit_gradient(shots, d32, ckpt_every=50, ctx=ctx)
+    Jb, gb = sfd.misfit_gradient(shots, d32, ckpt_every=50, ctx=ctx, method="boundary")
+    assert Jb == Jr and Jr > 0
+    # FP64 adjoint of the FP64 residuals, shot by shot.
+    res = [oracle.acoustic(bg, precision="f64", nthreads=nth, **k)["record"] - d for k, d in zip(okw, obs)]
+    with ThreadPoolExecutor(max_workers=nth) as ex:
+        parts = list(ex.map(lambda kr: oracle.gradient(bg, seed=kr[1], **kr[0])[0], zip(okw, res)))
+    g64 = np.sum(parts, axis=0)
+    J64_0 = sum(0.5 * float(np.sum(r ** 2)) for r in res)
+    b = max(shots[0].width, shots[0].radius)
+    box = np.zeros(bg.shape, bool)
+    box[b:nx - b, b:nz - b] = True
+    e_r, e_b = rel_l2(gr, g64), rel_l2(gb[box], g64[box])
+    print(f"J gpu {Jr:.6e} fp64 {J64_0:.6e}; gradient rel-L2 vs fp64: recompute {e_r:.2e}, "
+          f"boundary saving (box) {e_b:.2e}")
+    assert abs(Jr / J64_0 - 1) <= 1e-4
+    assert e_r <= 1e-4 and e_b <= 1e-4
+    assert not np.any(gb[~box])
+    rng = np.random.default_rng(8)
+    for n in range(8):
+        d = gaussian_filter(rng.standard_normal(bg.shape), 4) * box
+        d /= np.abs(d).max()
+        vpp = (bg.astype(np.float64) + 1.0 * d).astype(np.float32)
+        vpm = (bg.astype(np.float64) - 1.0 * d).astype(np.float32)
+        step = (vpp.astype(np.float64) - vpm.astype(np.float64)) / 2
+        fd = (J64(vpp) - J64(vpm)) / 2
+        for g, name in ((gr, "recompute"), (gb, "boundary")):
+            an = float(np.sum(g.astype(np.float64) * step))
+            print(f"direction {n} {name}: fd {fd:.8e} adjoint {an:.8e} rel {abs(fd - an) / abs(fd):.2e}", flush=True)
+            assert abs(fd - an) <= 1e-3 * abs(fd)

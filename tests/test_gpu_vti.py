@@ -89,10 +89,10 @@ def test_against_oracle(case):
     r64, p64, q64 = run_oracle(vz, ep, de, precision="f64", **kw)
     e64, e32 = rel(rec[0], r64), rel(rec[0], r32)
     print(f"rel-L2 vs FP64 {e64:.2e}, vs FP32 {e32:.2e}, oracle FP32 vs FP64 {rel(r32, r64):.2e}", flush=True)
-    # the GPU's distance from FP64 within twice the FP32 oracle's own (floor 5e-5)
+    # the GPU's distance from FP64 within twice the FP32 oracle's own (floor 1e-5)
     for got, o32, o64 in ((rec[0], r32, r64), (st[0, 0], p32, p64), (st[0, 1], q32, q64)):
-        assert rel(got, o64) <= max(5e-5, 2 * rel(o32, o64))
-    assert e32 <= 5e-5
+        assert rel(got, o64) <= max(1e-5, 2 * rel(o32, o64))
+    assert e32 <= max(1e-5, 3 * rel(r32, r64))
     assert np.abs(q64).max() > 0
 
 
@@ -148,8 +148,12 @@ def test_edges(edge):
     with sfd.Context(0) as ctx:
         rec, st = run_gpu(vz, ep, de, ctx, **kw)
     r64, p64, q64 = run_oracle(vz, ep, de, precision="f64", **kw)
-    assert rel(rec[0], r64) <= 5e-5
-    assert rel(st[0, 0], p64) <= 1e-4
+    r32, p32, _ = run_oracle(vz, ep, de, **kw)
+    e_r, e_p = rel(rec[0], r64), rel(st[0, 0], p64)
+    print(f"rel-L2 vs FP64: record {e_r:.2e} (oracle FP32 {rel(r32, r64):.2e}), "
+          f"final p {e_p:.2e} (oracle FP32 {rel(p32, p64):.2e})")
+    assert e_r <= max(1e-5, 2 * rel(r32, r64))
+    assert e_p <= max(1e-5, 2 * rel(p32, p64))
 
 
 def test_card_geometry_short():
@@ -160,7 +164,9 @@ def test_card_geometry_short():
         rec, _ = run_gpu(V, E, D, ctx, **kw)
     for s in range(2):
         r64 = run_oracle(V[s], E[s], D[s], precision="f64", **kw)[0]
-        assert rel(rec[s], r64) <= 5e-5
+        r32 = run_oracle(V[s], E[s], D[s], **kw)[0]
+        print(f"shot {s}: rel-L2 vs FP64 {rel(rec[s], r64):.2e} (oracle FP32 {rel(r32, r64):.2e})")
+        assert rel(rec[s], r64) <= max(1e-5, 2 * rel(r32, r64))
 
 
 def test_refuses_bad_input():
