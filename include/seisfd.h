@@ -17,6 +17,9 @@
  *     Every call is synchronous with respect to the host buffers it takes.
  *   - Every entry point returns an int status (SFD_OK = 0, negative on
  *     failure) and never exits or aborts; sfd_last_error() gives the text.
+ *   - Every run entry point checks its outputs (gathers, and the final
+ *     pressure field of acoustic shots) for Inf / NaN and returns SFD_ENAN
+ *     when it finds any (the outputs stay fetchable).
  *   - One context per host thread; contexts are independent.
  */
 #ifndef SEISFD_H
@@ -29,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SEISFD_ABI_VERSION 1
+#define SEISFD_ABI_VERSION 2
 
 enum sfd_status {
     SFD_OK = 0,
@@ -452,6 +455,9 @@ typedef struct sfd_vti_params {
     int nrec;
     const int *rx, *rz;    /* receiver cells, the same in every shot (reference: row
                               nrz0 + nz_bound_u, or row 6 when that is above it, of every window column) */
+    int allow_eps_lt_delta;/* 0: cells with eps < delta are rejected (SFD_EINVAL): the
+                              pseudo-acoustic system grows without bound there; nonzero: run them
+                              anyway, as the reference does (a divergence then returns SFD_ENAN) */
 } sfd_vti_params;
 
 /* vz, eps, delta: [nshot][nx*nz].  wavelet [delay + nt] or NULL = the

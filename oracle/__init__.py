@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SEISFD_ORACLE_LIB selects another build of the same sources (the tests'
+# AddressSanitizer build); the default is the in-tree liboracle.so.
+LIB_PATH = os.environ.get("SEISFD_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 # Reference coefficients C1..C8 as written in Acoustic_1order_FD.cpp:38-45 (float).

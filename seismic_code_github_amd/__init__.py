@@ -107,6 +107,7 @@ class VtiParamsC(ctypes.Structure):
         ("pml_r", ctypes.c_float), ("c2", _FP), ("c1", _FP),
         ("sx", ctypes.c_int), ("sz", ctypes.c_int),
         ("nrec", ctypes.c_int), ("rx", _IP), ("rz", _IP),
+        ("allow_eps_lt_delta", ctypes.c_int),
     ]
 
 
@@ -1021,7 +1022,8 @@ class VtiPlan:
     pml = (left, right, top, bottom) cells; rec = (rx, rz) cell lists."""
 
     def __init__(self, vz, eps, delta, *, dx, dz, dt, nt, src, rec, pml=(0, 0, 0, 0), f0=30.0, delay=None,
-                 pml_r=0.0, wavelet=None, c2=None, c1=None, ctx: Optional[Context] = None):
+                 pml_r=0.0, wavelet=None, c2=None, c1=None, allow_eps_lt_delta=False,
+                 ctx: Optional[Context] = None):
         vz, eps, delta = (_f32(a) for a in (vz, eps, delta))
         if vz.ndim == 2:
             vz, eps, delta = vz[None], eps[None], delta[None]
@@ -1045,7 +1047,8 @@ class VtiPlan:
         self._arr = (vz, eps, delta)
         self.p = VtiParamsC(self.nx, self.nz, int(nt), -1 if delay is None else int(delay), dx, dz, dt, f0,
                             *(int(v) for v in pml), pml_r, _ptr(self._c2), _ptr(self._c1), int(src[0]), int(src[1]),
-                            self._rx.size, _ptr(self._rx, ctypes.c_int), _ptr(self._rz, ctypes.c_int))
+                            self._rx.size, _ptr(self._rx, ctypes.c_int), _ptr(self._rz, ctypes.c_int),
+                            int(bool(allow_eps_lt_delta)))
         self.ctx = ctx or default_context()
         self.ctx.adopt(self)
         h = ctypes.c_void_p()

@@ -336,6 +336,13 @@ void auto_tiling(const sfd_acoustic_plan* pl, int* nw, int* chunk) {
     *chunk = round_chunk(ch, pl->radius);
 }
 
+// Watchdog after a run: the gather and the final pressure field.
+int check_plan_finite(sfd_acoustic_plan* pl) {
+    const int rc = check_finite(pl->ctx, pl->rec_buf.as<float>(), pl->rec_buf.bytes / 4, "the gather");
+    if (rc) return rc;
+    return check_finite(pl->ctx, pl->grid(pl->p[pl->cur]), (size_t)pl->nx * pl->nz, "the pressure field");
+}
+
 int zero_state(sfd_acoustic_plan* pl) {
     hipStream_t s = pl->ctx->stream;
     for (int b = 0; b < 2; ++b) {
@@ -835,7 +842,7 @@ int sfd_acoustic_plan_run(sfd_acoustic_plan* pl, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    return SFD_OK;
+    return check_plan_finite(pl);
 }
 
 int sfd_acoustic_plan_step(sfd_acoustic_plan* pl, int k0, int nsteps) {
@@ -1251,6 +1258,10 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
             if (b->tiled || b->fused) pl->v_raw = b->nt > 0 && b->mode == MODE_FAST_SPONGE;
             if (!b->tiled) pl->cur = b->nt & 1;
         }
+    }
+    for (auto* pl : b->plans) {
+        const int rc = check_plan_finite(pl);
+        if (rc) return rc;
     }
     return SFD_OK;
 }

@@ -116,6 +116,8 @@ int launch_tiled(const sfd_acoustic_plan* pl, StepKernel v, StepKernel p, const 
 // Tiled step shape used unless one is asked for.
 constexpr int kTiledTX = 16, kTiledNW = 4;
 int zero_state(sfd_acoustic_plan* pl);
+// SFD_ENAN when the plan's gather or current pressure field is not finite.
+int check_plan_finite(sfd_acoustic_plan* pl);
 // Sponge factor of cell (i, j) as the step kernel applies it.
 float sponge_at(const sfd_acoustic_plan* pl, int i, int j);
 

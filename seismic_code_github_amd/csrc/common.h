@@ -25,6 +25,7 @@ struct sfd_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int* d_flag = nullptr;  // device word of the non-finite watchdog
     std::string err;
     // Autotuner cache: grid class -> fastest measured shape on this device
     // (saved to and loaded from JSON by sfd_context_save/load_tuning).
@@ -126,6 +127,11 @@ int time_steps(sfd_context* ctx, int n, Run run, float* us) {
     *us = best;
     return SFD_OK;
 }
+
+// Non-finite watchdog: SFD_ENAN (message naming `what`) when any of the n
+// device floats at `a` is Inf or NaN, SFD_OK otherwise.  Synchronises the
+// context stream; allocates nothing.
+int check_finite(sfd_context* ctx, const float* a, size_t n, const char* what);
 
 // Looks up the device address of the allocation-free entry point's scratch
 // array on ctx's device (once per device; the device must be current).

@@ -25,6 +25,29 @@ int hip_check(sfd_context* ctx, hipError_t e, const char* what) {
     return fail(ctx, code, "%s: %s", what, hipGetErrorString(e));
 }
 
+__global__ void nonfinite_kernel(const float* __restrict__ a, size_t n, int* __restrict__ flag) {
+    int bad = 0;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x)
+        bad |= !isfinite(a[c]);
+    if (bad) atomicOr(flag, 1);
+}
+
+__global__ void clear_flag_kernel(int* flag) { *flag = 0; }
+
+int check_finite(sfd_context* ctx, const float* a, size_t n, const char* what) {
+    if (!a || n == 0) return SFD_OK;
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, s, ctx->d_flag);
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(nonfinite_kernel, dim3(nb), dim3(256), 0, s, a, n, ctx->d_flag);
+    SFD_HIP(ctx, hipGetLastError());
+    int h = 0;
+    SFD_HIP(ctx, hipMemcpyAsync(&h, ctx->d_flag, sizeof h, hipMemcpyDeviceToHost, s));
+    SFD_HIP(ctx, hipStreamSynchronize(s));
+    if (h) return fail(ctx, SFD_ENAN, "%s holds non-finite values: the run diverged or an input is not finite", what);
+    return SFD_OK;
+}
+
 int log2_round(double n) { return (int)std::lround(std::log2(std::max(n, 1.0))); }
 
 bool default_coefs(int radius, float* c) {
@@ -116,6 +139,7 @@ int sfd_context_create(int device, sfd_context** out) {
     hipDeviceProp_t prop;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipMalloc(&ctx->d_flag, 256) != hipSuccess ||
         hipGetDeviceProperties(&prop, device) != hipSuccess) {
         sfd_context_destroy(ctx);
         return SFD_EDEVICE;
@@ -135,6 +159,7 @@ int sfd_context_destroy(sfd_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->d_flag) (void)hipFree(ctx->d_flag);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SFD_OK;

@@ -618,7 +618,9 @@ int sfd_elastic_plan_run(sfd_elastic_plan* pl, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    return SFD_OK;
+    rc = check_finite(ctx, pl->rec_vx.as<float>(), pl->rec_vx.bytes / 4, "the vx gather");
+    if (rc) return rc;
+    return check_finite(ctx, pl->rec_vz.as<float>(), pl->rec_vz.bytes / 4, "the vz gather");
 }
 
 int sfd_elastic_plan_fetch(sfd_elastic_plan* pl, float* record_vx, float* record_vz, float* snaps) {

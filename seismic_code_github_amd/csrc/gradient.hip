@@ -693,6 +693,14 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     return SFD_OK;
 }
 
+// Watchdog of a host result: SFD_ENAN when any value is Inf or NaN.
+int host_finite(sfd_context* ctx, const float* a, size_t n, const char* what) {
+    for (size_t c = 0; c < n; ++c)
+        if (!std::isfinite(a[c]))
+            return fail(ctx, SFD_ENAN, "%s holds non-finite values: the run diverged or an input is not finite", what);
+    return SFD_OK;
+}
+
 }  // namespace
 }  // namespace sfd
 
@@ -702,7 +710,9 @@ int sfd_acoustic_gradient(sfd_context* ctx, const sfd_acoustic_params* p, const 
                           const float* wavelet, const float* seed, int ckpt_every, float* grad_vp, float* record) {
     if (!ctx || !p || !vp || !wavelet || !seed || !grad_vp) return SFD_EINVAL;
     std::fill(grad_vp, grad_vp + (size_t)p->nx * p->nz, 0.f);
-    return grad_one(ctx, p, vp, den, wavelet, seed, nullptr, ckpt_every, grad_vp, record, nullptr);
+    const int rc = grad_one(ctx, p, vp, den, wavelet, seed, nullptr, ckpt_every, grad_vp, record, nullptr);
+    if (rc) return rc;
+    return host_finite(ctx, grad_vp, (size_t)p->nx * p->nz, "the gradient");
 }
 
 int sfd_acoustic_misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp,
@@ -741,7 +751,8 @@ int sfd_acoustic_misfit_gradient_ex(sfd_context* ctx, int nshot, const sfd_acous
                                  misfit);
         if (rc) return rc;
     }
-    return SFD_OK;
+    if (misfit && !std::isfinite(*misfit)) return fail(ctx, SFD_ENAN, "the misfit is not finite: the run diverged");
+    return host_finite(ctx, grad_vp, (size_t)params[0].nx * params[0].nz, "the gradient");
 }
 
 }  // extern "C"

@@ -327,7 +327,9 @@ int sfd_acoustic_forward_noalloc(sfd_context* ctx, const sfd_acoustic_params* p,
     }
     SFD_HIP(ctx, hipGetLastError());
     SFD_HIP(ctx, hipStreamSynchronize(s));
-    return SFD_OK;
+    rc = check_finite(ctx, d_record, p->nrec > 0 ? (size_t)p->nrec * nt : 0, "the gather");
+    if (rc) return rc;
+    return check_finite(ctx, a.p_out, (size_t)nx * nz, "the pressure field");
 }
 
 }  // extern "C"

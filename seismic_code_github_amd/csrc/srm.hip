@@ -402,7 +402,7 @@ int sfd_srm_plan_run(sfd_srm_plan* pl, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    return SFD_OK;
+    return check_finite(ctx, pl->out.as<float>(), pl->out.bytes / 4, "the predicted multiples");
 }
 
 int sfd_srm_plan_fetch(sfd_srm_plan* pl, float* out) {

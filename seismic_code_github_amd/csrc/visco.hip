@@ -869,7 +869,8 @@ int sfd_visco_plan_run(sfd_visco_plan* pl, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    return SFD_OK;
+    const sfd::DeviceBuffer& rec = pl->ve_on ? pl->rec_zz : pl->rec_buf;
+    return check_finite(ctx, rec.as<float>(), rec.bytes / 4, "the gather");
 }
 
 int sfd_visco_plan_fetch(sfd_visco_plan* pl, float* record, float* snaps) {

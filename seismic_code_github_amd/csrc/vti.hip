@@ -479,12 +479,21 @@ int sfd_vti_plan_create(sfd_context* ctx, const sfd_vti_params* p, int nshot, co
         K = std::max(K, std::fabs(s));
     }
     double worst = 0;
+    size_t nunstable = 0, first_unstable = 0;
     for (size_t c = 0; c < ntot; ++c) {
+        if (eps[c] < delta[c] && nunstable++ == 0) first_unstable = c;
         if (!(vz[c] > 0.f) || !std::isfinite(vz[c]) || !std::isfinite(eps[c]) || !std::isfinite(delta[c]) ||
             eps[c] <= -0.5f || delta[c] <= -0.5f)
             return fail(ctx, SFD_EINVAL, "vz must be finite and > 0, eps and delta finite and > -1/2 (cell %zu)", c);
         const double an = 1 + 2 * std::max({(double)eps[c], (double)delta[c], 0.0});
         worst = std::max(worst, (double)vz[c] * vz[c] * (an * K / ((double)p->dx * p->dx) + K / ((double)p->dz * p->dz)));
+    }
+    if (nunstable && !p->allow_eps_lt_delta) {
+        const size_t sh = first_unstable / n, c = first_unstable % n;
+        return fail(ctx, SFD_EINVAL,
+                    "eps < delta in %zu cells (first: shot %zu, x %zu, z %zu): the pseudo-acoustic VTI system "
+                    "is unstable there; set allow_eps_lt_delta to run it anyway",
+                    nunstable, sh, c / p->nz, c % p->nz);
     }
     if ((double)p->dt * p->dt * worst > 4.0)
         return fail(ctx, SFD_ECFL, "unstable: dt^2 max(vz^2 ((1+2 max(eps,delta)) K/dx^2 + K/dz^2)) = %.4f exceeds 4",
@@ -668,7 +677,12 @@ int sfd_vti_plan_run(sfd_vti_plan* pl, float* elapsed_ms) {
     SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
-    return SFD_OK;
+    rc = check_finite(ctx, pl->rec.as<float>(), pl->rec.bytes / 4, "the gathers");
+    if (rc) return rc;
+    // the level written last (see sfd_vti_plan_fetch), every shot's p
+    const int cur = (pl->nsteps % 2 == 0) ? 1 : 0;
+    const size_t off = (size_t)kVPad * pl->pitch + kVPad;  // cell (0, 0) within a shot's field
+    return check_finite(ctx, pl->u[cur] - off, pl->fs * pl->nshot, "the p field");
 }
 
 int sfd_vti_plan_fetch(sfd_vti_plan* pl, float* record, float* state) {

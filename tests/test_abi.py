@@ -37,7 +37,7 @@ def test_python_mirror_covers_the_header():
 def test_host_only_entry_points():
     import seismic_code_github_amd as sfd
     L = sfd.lib()
-    assert L.sfd_abi_version() == 1
+    assert L.sfd_abi_version() == 2
     # 1 / sum |C_m| for the reference's C1..C8 (Acoustic_1order_FD.cpp:38-45).
     c8 = np.array([1.2340911, -1.0664985e-1, 2.3036367e-2, -5.3423856e-3,
                    1.0772712e-3, -1.6641888e-4, 1.7021711e-5, -8.5234642e-7], dtype=np.float32)

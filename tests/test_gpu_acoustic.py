@@ -414,3 +414,31 @@ def test_errors_reported_not_raised_in_native_code(ctx):
     # the context stays usable after errors
     rec, _ = sfd.acoustic_forward(shot, ctx)
     assert np.isfinite(rec).all()
+
+
+def test_watchdog_reports_non_finite(ctx):
+    """A NaN in the wavelet spreads through the wavefield: plan runs, batch
+    runs and the gradient return SFD_ENAN instead of success; a clean shot in
+    the same context runs on."""
+    shot = W.layered_301(radius=4, nt=120)
+    bad = W.layered_301(radius=4, nt=120)
+    bad.wavelet = bad.wavelet.copy()
+    bad.wavelet[7] = np.nan
+    print("case: watchdog", flush=True)
+    pl = sfd.AcousticPlan(bad, ctx)
+    with pytest.raises(sfd.SeisfdError) as e:
+        pl.run()
+    assert e.value.code == sfd.SFD_ENAN
+    rec, _ = pl.fetch()
+    assert np.isnan(rec).any()
+    pl.close()
+    b = sfd.AcousticBatch([shot, bad], ctx)
+    with pytest.raises(sfd.SeisfdError) as e:
+        b.run(sfd.BATCH_GRAPH)
+    assert e.value.code == sfd.SFD_ENAN
+    b.close()
+    with pytest.raises(sfd.SeisfdError) as e:
+        sfd.acoustic_gradient(bad, np.ones((301, 120), np.float32), ckpt_every=40, ctx=ctx)
+    assert e.value.code == sfd.SFD_ENAN
+    rec, _ = sfd.acoustic_forward(shot, ctx)
+    assert np.all(np.isfinite(rec)) and np.abs(rec).max() > 0

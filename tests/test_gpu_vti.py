@@ -180,3 +180,33 @@ def test_refuses_bad_input():
             sfd.VtiPlan(vz, ep, de, dt=5e-4, pml=(40, 40, 0, 0), ctx=ctx, **kw)
         with pytest.raises(sfd.SeisfdError):
             sfd.VtiPlan(vz, ep, de, dt=5e-4, ctx=ctx, **dict(kw, src=(64, 0)))
+
+
+def test_eps_below_delta_rejected_then_watchdog():
+    """eps < delta makes the pseudo-acoustic pair grow without bound (the
+    FP32 oracle reaches NaN by step 400 on this model): the plan refuses such
+    cells unless the caller opts in, and a run that opts in and diverges
+    returns SFD_ENAN (the gathers stay fetchable)."""
+    nx, nz = 96, 80
+    vz = W.smooth_random_model(nx, nz, 6, 1800, 3200, 60)
+    ep = np.zeros((nx, nz), np.float32)
+    de = np.zeros((nx, nz), np.float32)
+    de[30:66, 30:60] = 0.3
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=600, src=(48, 30), rec=(np.array([48]), np.array([40])))
+    print("case: vti eps < delta", flush=True)
+    with sfd.Context(0) as ctx:
+        with pytest.raises(sfd.SeisfdError, match="eps < delta in 1080 cells") as e:
+            sfd.VtiPlan(vz, ep, de, ctx=ctx, **kw)
+        assert e.value.code == sfd.SFD_EINVAL
+        pl = sfd.VtiPlan(vz, ep, de, ctx=ctx, allow_eps_lt_delta=True, **kw)
+        with pytest.raises(sfd.SeisfdError) as e:
+            pl.run()
+        assert e.value.code == sfd.SFD_ENAN
+        rec = pl.fetch()
+        assert not np.all(np.isfinite(rec))
+        # the same model with delta <= eps runs clean
+        ok = sfd.VtiPlan(vz, de, de, ctx=ctx, **kw)
+        ok.run()
+        assert np.all(np.isfinite(ok.fetch()))
+        pl.close()
+        ok.close()
