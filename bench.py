@@ -24,6 +24,12 @@ measured with HIP events on the library's stream over the timed shots;
 traffic = both launches' FETCH_SIZE + WRITE_SIZE per step, scaled by a
 256 MiB copy (profiles/acoustic_step_cpml_2101x4101.md).
 
+FP64 errors: every leg states the relative L2 distance of its result from
+the FP64 oracle on a stated sample (the headline: its gather on the full grid
+over the first 800 steps), next to the FP32 oracle's own distance.  The
+streaming ceiling (hbm_ceiling) is measured by hand-written read / write /
+copy kernels in the same run; roofline.ceiling_frac = achieved / copy rate.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -80,42 +86,144 @@ def cpu_baseline(shot, steps):
                       f"(reference arithmetic), OpenMP over x rows, {dt:.1f} s"}
 
 
+def c1_prefix_error(shot, steps):
+    """FP64 error of the headline shot on its full grid over the first
+    `steps` steps: the gather of a plan run that far against the FP64 and
+    FP32 oracles."""
+    import oracle
+    import seismic_code_github_amd as sfd
+    cores = min(16, len(os.sched_getaffinity(0)))
+    short = sfd.AcousticShot(**{**shot.__dict__, "wavelet": shot.wavelet[:steps], "nt": None, "_keep": []})
+    rec, _ = sfd.acoustic_forward(short)
+    den = np.full_like(shot.vp, 2400.0)
+    kw = dict(dx=shot.dx, dz=shot.dz, dt=shot.dt, radius=shot.radius, sponge=shot.width, nt=steps, nthreads=cores,
+              boundary="cpml", pml_r=shot.pml_r, pml_f0=shot.pml_f0, pml_vmax=float(shot.vp.max()))
+    o64 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, precision="f64", **kw)["record"]
+    o32 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, **kw)["record"]
+    nrm = np.linalg.norm(o64)
+    return {"fp64_rel_l2": float(np.linalg.norm(rec - o64) / nrm),
+            "fp32_oracle_vs_fp64_rel_l2": float(np.linalg.norm(o32 - o64) / nrm),
+            "error_sample": f"gather of the same shot on its full {shot.vp.shape[0]}x{shot.vp.shape[1]} grid over "
+                            f"the first {steps} of {len(shot.wavelet)} steps"}
+
+
 def shape_dict(kind, a, b, us):
-    d = {"kernel": "tiled two-launch step", "tile_rows": a, "waves": b} if kind == 0 else \
-        {"kernel": "fused march kernel", "waves": a, "chunk_rows": b}
+    if kind == 0:
+        d = {"kernel": "tiled two-launch step", "tile_rows": a, "waves": b}
+    elif kind == 2:
+        d = {"kernel": "one-launch tiled step", "tile_rows": a, "waves": b}
+    else:
+        d = {"kernel": "fused march kernel", "waves": a, "chunk_rows": b}
     d["autotuned_us_per_step"] = None if us is None else round(us, 2)
     return d
 
 
-def batched_leg(nt, no_tune=False):
+def batched_leg(nt, no_tune=False, no_cpu=False):
     """64 shots on uneven aperture windows of a 1001 x 6001 model (C-PML 50,
-    nt=3000 per shot): one tiled step (two launches) per time step for all
-    shots, replayed from a hipGraph, against the plain path (shot after shot,
-    one tiled step per shot and time step).  Launch shape from the autotuner.
-    Timed on the first `nt` steps; shots/s is scaled to the full 3000 steps
-    (every step costs the same)."""
+    nt=3000 per shot): one tiled step per time step for all shots, replayed
+    from a hipGraph, against the plain path (shot after shot, one step per
+    shot and time step).  Launch shape from the autotuner.  Timed over `nt`
+    steps (default all 3000).  FP64 error: the gather of the median-width
+    window against the FP64 oracle over the same steps."""
     import seismic_code_github_amd as sfd
     from seismic_code_github_amd import workloads as W
     _, shots = W.windowed_shots(nshot=64, nt=nt)
     cells = sum(s.vp.size for s in shots)
+    bytes_step = sum(algorithmic_bytes_per_step(s) for s in shots)
     b = sfd.AcousticBatch(shots)
     tuned_us = None if no_tune else b.autotune(20)
     kind, ta, tb = b.shape()
     b.run(sfd.BATCH_GRAPH)
     g_ms = min(b.run(sfd.BATCH_GRAPH) for _ in range(2))
+    order = np.argsort([s.vp.shape[0] for s in shots])
+    mid = int(order[len(shots) // 2])
+    rec_mid = b.fetch(mid)
     p_ms = b.run(sfd.BATCH_PER_SHOT)
     b.close()
     scale = 3000.0 / nt
-    return {"workload": "64 shots, windows of 1000-2500 columns of a 1001x6001 model, C-PML 50, R=8, nt=3000",
-            "cells": int(cells), "timed_steps": nt,
-            "graph_Mcell_per_s": round(cells * nt / g_ms / 1e3, 1),
-            "plain_Mcell_per_s": round(cells * nt / p_ms / 1e3, 1),
-            "graph_shots_per_s": round(64 / (g_ms * scale / 1e3), 3),
-            "plain_shots_per_s": round(64 / (p_ms * scale / 1e3), 3),
-            "speedup": round(p_ms / g_ms, 2),
-            # uniform density: 28 algorithmic bytes per cell-update
-            "graph_frac_of_8TBs": round(cells * nt * 28.0 / (g_ms / 1e3) / 8e12, 4),
-            "launch_shape": shape_dict(kind, ta, tb, tuned_us)}
+    out = {"workload": "64 shots, windows of 1000-2500 columns of a 1001x6001 model, C-PML 50, R=8, nt=3000",
+           "cells": int(cells), "timed_steps": nt,
+           "graph_Mcell_per_s": round(cells * nt / g_ms / 1e3, 1),
+           "plain_Mcell_per_s": round(cells * nt / p_ms / 1e3, 1),
+           "graph_shots_per_s": round(64 / (g_ms * scale / 1e3), 3),
+           "plain_shots_per_s": round(64 / (p_ms * scale / 1e3), 3),
+           "speedup": round(p_ms / g_ms, 2),
+           "bytes_per_step": bytes_step,
+           "avg_step_us": round(g_ms * 1e3 / nt, 2),
+           # uniform density: 28 algorithmic bytes per cell-update + the C-PML strips
+           "graph_frac_of_8TBs": round(bytes_step * nt / (g_ms / 1e3) / 8e12, 4),
+           "launch_shape": shape_dict(kind, ta, tb, tuned_us)}
+    tpath = os.path.join(ROOT, "profiles", "acoustic_batch_c3_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            out["traffic"] = json.load(f).get("hbm_bytes_per_step")
+    if not no_cpu:
+        import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        sh = shots[mid]
+        kw = dict(dx=sh.dx, dz=sh.dz, dt=sh.dt, radius=sh.radius, sponge=sh.width, nt=nt, nthreads=cores,
+                  boundary="cpml", pml_r=sh.pml_r, pml_f0=sh.pml_f0, pml_vmax=float(sh.vp.max()))
+        den = np.full_like(sh.vp, 2400.0)
+        o64 = oracle.acoustic(sh.vp, den, sh.wavelet, sh.src, sh.rec, precision="f64", **kw)["record"]
+        o32 = oracle.acoustic(sh.vp, den, sh.wavelet, sh.src, sh.rec, **kw)["record"]
+        nrm = np.linalg.norm(o64)
+        out["fp64_rel_l2"] = float(np.linalg.norm(rec_mid - o64) / nrm)
+        out["fp32_oracle_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
+        out["error_sample"] = f"gather of window {mid} ({sh.vp.shape[0]}x{sh.vp.shape[1]} cells), all {nt} steps"
+    return out
+
+
+def c0_leg(no_cpu, no_tune=False):
+    """301 x 301 interior + 40-cell sponge (381 x 381), 3-layer
+    velocity, R = 4 (8th order), dt = 1 ms, nt = 2000, 25 Hz Ricker, 301
+    receivers at z = 2: GPU Mcell-updates/s in fast and reference arithmetic
+    beside the reference CPU path (the FP32 oracle in the reference's
+    operation order) on 1 core and on all host cores, with the FP64 error of
+    the gathers.  Cache-resident and launch-bound, so no roofline fraction."""
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    shot = W.layered_301(radius=4)
+    nx, nz = shot.vp.shape
+    nt = len(shot.wavelet)
+    out = {"workload": "acoustic 301x301 interior + 40-cell sponge (381x381), 3 layers, R=4 (8th order), "
+                       "dt=1 ms, nt=2000, 25 Hz Ricker, 301 receivers", "cells": nx * nz, "nt": nt}
+    recs = {}
+    for arith in ("fast", "reference"):
+        sh = W.layered_301(radius=4, arith=arith)
+        with sfd.AcousticPlan(sh) as pl:
+            us = None if no_tune else pl.autotune(100)
+            shape = pl.shape()
+            pl.run()
+            ms = min(pl.run() for _ in range(3))
+            recs[arith] = pl.fetch()[0]
+        out[arith] = {"ms_per_shot": round(ms, 3), "Mcell_per_s": round(nx * nz * nt / ms / 1e3, 1),
+                      "us_per_step": round(ms * 1e3 / nt, 2), "launch_shape": shape_dict(*shape, us)}
+    if not no_cpu:
+        import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        den = np.full_like(shot.vp, 2400.0)
+        kw = dict(dx=shot.dx, dz=shot.dz, dt=shot.dt, radius=4, sponge=shot.width)
+        o64 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, precision="f64", nthreads=cores,
+                              **kw)["record"]
+        base = {}
+        for th in (1, cores):
+            t0 = time.perf_counter()
+            o32 = oracle.acoustic(shot.vp, den, shot.wavelet, shot.src, shot.rec, nthreads=th, **kw)["record"]
+            base[th] = time.perf_counter() - t0
+        nrm = np.linalg.norm(o64)
+        out["fast"]["fp64_rel_l2"] = float(np.linalg.norm(recs["fast"] - o64) / nrm)
+        out["reference"]["fp64_rel_l2"] = float(np.linalg.norm(recs["reference"] - o64) / nrm)
+        out["reference"]["bitwise_equal_to_cpu_reference_path"] = bool(np.array_equal(recs["reference"], o32))
+        out["fp32_cpu_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
+        out["cpu_baseline"] = {"value": round(nx * nz * nt / base[1] / 1e6, 2), "unit": "Mcell-updates/s",
+                               "cores": 1, "kind": "port",
+                               "value_all_cores": round(nx * nz * nt / base[cores] / 1e6, 2),
+                               "all_cores": cores,
+                               "sample": f"the whole shot ({nt} steps), FP32 oracle in the reference's operation "
+                                         f"order (bitwise the reference program's arithmetic), {base[1]:.2f} s "
+                                         f"on 1 core, {base[cores]:.2f} s on {cores}"}
+        out["gpu_vs_cpu_1core"] = round(out["fast"]["Mcell_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
 
 
 def visco_leg(nt_err, nt, no_cpu):
@@ -332,6 +440,9 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
                     dt=kw1["dt"], pml=kw1["pml"], wavelet=w, nthreads=cores)
         o64 = oracle.vti(V1[0], E1[0], D1[0], precision="f64", **args)
         o32 = oracle.vti(V1[0], E1[0], D1[0], precision="f32", **args)
+        if kw1.get("zero_first_sample"):  # the driver's recording rule, applied by VtiPlan.fetch
+            o64[:, 0] = 0.0
+            o32[:, 0] = 0.0
         nrm = np.linalg.norm(o64)
         out["fp64_rel_l2"] = float(np.linalg.norm(got - o64) / nrm)
         out["fp32_oracle_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
@@ -387,7 +498,7 @@ def srm_leg(nshot, ntrace, lt, no_cpu):
     return res
 
 
-def gradient_leg(nt):
+def gradient_leg(nt, no_cpu=False):
     """Adjoint gradient: 16 shots on 801 x 801 + 40-cell sponge, 10 Hz,
     checkpoints every 50 steps: forward + checkpointed recomputation + adjoint
     per shot (sfd_acoustic_misfit_gradient, set-up and transfers included)
@@ -419,6 +530,9 @@ def gradient_leg(nt):
     box = (slice(b, g.shape[0] - b), slice(b, g.shape[1] - b))
     gi, gbi = g[box].astype(np.float64), gb[box].astype(np.float64)
     bs_err = float(np.linalg.norm(gbi - gi) / max(np.linalg.norm(gi), 1e-300))
+    fp64 = {}
+    if not no_cpu:
+        fp64 = gradient_fp64_checks(shots, d_obs, bg)
     scale = 2500.0 / nt
     n = len(shots)
     # Algorithmic bytes per shot-cell and step of the fused step with uniform
@@ -437,7 +551,53 @@ def gradient_leg(nt):
             "timing": "whole sfd_acoustic_misfit_gradient call: set-up, residuals and transfers included",
             "boundary_saving": {"fwd_adj_shots_per_s": round(n / (bs_s * scale), 3),
                                 "box_rel_l2_vs_recompute": bs_err,
-                                "box": f"[{b}, {g.shape[0] - b}) x [{b}, {g.shape[1] - b})"}}
+                                "box": f"[{b}, {g.shape[0] - b}) x [{b}, {g.shape[1] - b})"},
+            **fp64}
+
+
+def gradient_fp64_checks(shots, d_obs, bg):
+    """FP64 errors of the gradient path: (a) the misfit of one shot of the
+    full-size workload (all its steps) against the FP64 forward with the
+    same observed data; (b) at 1/8 size (nt = 800, 4 of the 16 shots), the
+    summed gradient against the FP64 adjoint."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import oracle
+    import seismic_code_github_amd as sfd
+    from seismic_code_github_amd import workloads as W
+    cores = min(16, len(os.sched_getaffinity(0)))
+    s = len(shots) // 2
+    sh = shots[s]
+    J1, _ = sfd.misfit_gradient([sh], [d_obs[s]], vp=bg, ckpt_every=50)
+    rec, _ = sfd.acoustic_forward(sh)
+    den = np.full_like(bg, 2400.0)
+    kw = dict(dx=sh.dx, dz=sh.dz, dt=sh.dt, radius=sh.radius, sponge=sh.width)
+    r64 = oracle.acoustic(bg, den, sh.wavelet, sh.src, sh.rec, precision="f64", nthreads=cores, **kw)["record"]
+    J64 = 0.5 * float(np.sum((r64 - d_obs[s].astype(np.float64)) ** 2))
+    rnorm = float(np.linalg.norm(r64 - d_obs[s].astype(np.float64)) / np.linalg.norm(r64))
+    true8, bg8, shots8 = W.fwi_801(nshot=16, nt=800, scale=8)
+    sel = shots8[::4]
+    d8 = np.full_like(bg8, 2400.0)
+    obs8 = [oracle.acoustic(true8, d8, t.wavelet, t.src, t.rec, precision="f64", nthreads=cores, dx=t.dx, dz=t.dz,
+                            dt=t.dt, radius=t.radius, sponge=t.width)["record"] for t in sel]
+    _, g8 = sfd.misfit_gradient(sel, [o.astype(np.float32) for o in obs8], vp=bg8, ckpt_every=50)
+
+    def adj(i):
+        t = sel[i]
+        kw8 = dict(dx=t.dx, dz=t.dz, dt=t.dt, radius=t.radius, sponge=t.width)
+        r = oracle.acoustic(bg8, d8, t.wavelet, t.src, t.rec, precision="f64", nthreads=1, **kw8)["record"]
+        return oracle.gradient(bg8, d8, t.wavelet, t.src, t.rec, r - obs8[i], **kw8)[0]
+
+    with ThreadPoolExecutor(max_workers=min(cores, len(sel))) as ex:
+        g64 = np.sum(list(ex.map(adj, range(len(sel)))), axis=0)
+    return {"fp64_rel_l2": float(np.linalg.norm(rec - r64) / np.linalg.norm(r64)),
+            "misfit_fp64_rel_err": abs(J1 - J64) / J64,
+            "residual_over_record": rnorm,
+            "gradient_fp64_rel_l2": float(np.linalg.norm(g8 - g64) / np.linalg.norm(g64)),
+            "error_sample": f"gather and misfit of shot {s} (all steps) against the FP64 forward (the misfit's "
+                            f"relative error is the gather's divided by residual_over_record); gradient of 4 shots "
+                            f"of the workload at 1/8 size ({bg8.shape[0]}x{bg8.shape[1]}, 800 steps) against the "
+                            f"FP64 adjoint"}
 
 
 def main():
@@ -450,7 +610,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch-steps", type=int, default=600)
+    ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-visco", action="store_true")
     ap.add_argument("--visco-err-nt", type=int, default=40)
@@ -465,8 +625,13 @@ def main():
     ap.add_argument("--vti-tiling", default=None, help="tile columns,waves (skips the autotuner)")
     ap.add_argument("--no-srm", action="store_true")
     ap.add_argument("--srm-size", default="401,401,4000", help="nshot,ntrace,lt")
-    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic", "vti", "srm"], default=None,
+    ap.add_argument("--only", choices=["visco", "grad", "batch", "elastic", "vti", "srm", "c0"], default=None,
                     help="run one secondary leg alone (profiling) and print its JSON")
+    ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--tuning-file", default=None,
+                    help="load the autotuner cache from this JSON file when it exists (no tuning runs: "
+                         "a profile then shows the timed shots only), else tune and save it there")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-bandwidth probe")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (the library's default launch shapes)")
@@ -493,29 +658,45 @@ def main():
         log(f"copied {args.copy_mib} MiB device to device")
         del src, dst
 
+    # The tuning cache lives in the default context, which every leg uses.
+    tuned_cache = False
+    if args.tuning_file and os.path.exists(args.tuning_file):
+        sfd.default_context().load_tuning(args.tuning_file)
+        tuned_cache = sfd.default_context().tuning_size() > 0
+        log(f"tuning cache: {sfd.default_context().tuning_size()} classes from {args.tuning_file}")
     vti_tiling = tuple(int(v) for v in args.vti_tiling.split(",")) if args.vti_tiling else None
     srm_size = tuple(int(v) for v in args.srm_size.split(","))
     if args.only:
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
-               "grad": lambda: gradient_leg(args.grad_steps),
-               "batch": lambda: batched_leg(args.batch_steps, args.no_tune),
+               "grad": lambda: gradient_leg(args.grad_steps, args.no_cpu),
+               "batch": lambda: batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu),
+               "c0": lambda: c0_leg(args.no_cpu, args.no_tune),
                "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu),
                "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,
                                       args.no_tune, vti_tiling),
                "srm": lambda: srm_leg(*srm_size, args.no_cpu)}[args.only]
-        print(json.dumps(leg()), flush=True)
+        res = leg()
+        if args.tuning_file and not tuned_cache:
+            sfd.default_context().save_tuning(args.tuning_file)
+        print(json.dumps(res), flush=True)
         return
 
     shot = W.smooth_random_2001x4001(nt=args.nt)
     nx, nz = shot.vp.shape
     nt = len(shot.wavelet)
     log(f"workload: {nx}x{nz} cells, nt={nt}, R={shot.radius}, {shot.boundary}")
-    ctx = sfd.Context(0)
+    ceiling = None
+    if not args.no_ceiling:
+        ceiling = {k: round(sfd.bandwidth_probe(2 << 30, kind), 1)
+                   for k, kind in (("copy_GBs", sfd.BW_COPY), ("read_GBs", sfd.BW_READ), ("write_GBs", sfd.BW_WRITE))}
+        ceiling["probe"] = "hand-written streaming kernels over 2 GiB (copy counts both buffers), best of 5"
+        log(f"streaming ceiling: {ceiling}")
+    ctx = sfd.default_context()
     plan = sfd.AcousticPlan(shot, ctx)
     tuned_us = None
     if args.waves or args.chunk:
         plan.set_tiling(args.waves, args.chunk)
-    elif not args.no_tune:
+    elif not args.no_tune and not tuned_cache:
         tuned_us = plan.autotune(60)
     kind, ta, tb = plan.shape()
     log(f"launch shape: {shape_dict(kind, ta, tb, tuned_us)}")
@@ -564,14 +745,23 @@ def main():
         "shots_per_s": round(args.steps / wall, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "unit_of_measure": ("one time step = velocity launch + pressure launch" if kind == 0 else
-                                         "one time step = one launch of the fused march kernel"),
+                     "ceiling_frac": None if ceiling is None else round(achieved / ceiling["copy_GBs"], 4),
+                     "unit_of_measure": {0: "one time step = velocity launch + pressure launch",
+                                         1: "one time step = one launch of the fused march kernel",
+                                         2: "one time step = one launch of the one-launch tiled step"}[kind],
                      "bytes_per_step": bytes_step, "avg_step_us": round(avg_launch_s * 1e6, 2)},
+        "hbm_ceiling": ceiling,
         "launch_shape": shape_dict(kind, ta, tb, tuned_us),
     }
+    if not args.no_cpu:
+        log("fp64 error of a full-grid prefix ...")
+        out.update(c1_prefix_error(shot, args.cpu_steps))
+    if not args.no_c0:
+        log("c0 leg ...")
+        out["c0"] = c0_leg(args.no_cpu, args.no_tune)
     if not args.no_batch:
         log("batched leg ...")
-        out["batched"] = batched_leg(args.batch_steps, args.no_tune)
+        out["batched"] = batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu)
     if not args.no_visco:
         log("visco leg ...")
         out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)
@@ -586,10 +776,12 @@ def main():
         out["srm"] = srm_leg(*srm_size, args.no_cpu)
     if not args.no_grad:
         log("gradient leg ...")
-        out["gradient"] = gradient_leg(args.grad_steps)
+        out["gradient"] = gradient_leg(args.grad_steps, args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(shot, args.cpu_steps)
+    if args.tuning_file and not tuned_cache:
+        ctx.save_tuning(args.tuning_file)
     print(json.dumps(out), flush=True)
 
 

@@ -1023,7 +1023,11 @@ class VtiPlan:
 
     def __init__(self, vz, eps, delta, *, dx, dz, dt, nt, src, rec, pml=(0, 0, 0, 0), f0=30.0, delay=None,
                  pml_r=0.0, wavelet=None, c2=None, c1=None, allow_eps_lt_delta=False,
-                 ctx: Optional[Context] = None):
+                 zero_first_sample=False, ctx: Optional[Context] = None):
+        # zero_first_sample: the driver's rule when its receiver row was moved
+        # down to row 6 (VTI main:469-476): sample 1 of every trace is never
+        # written.  fetch applies it.
+        self.zero_first_sample = bool(zero_first_sample)
         vz, eps, delta = (_f32(a) for a in (vz, eps, delta))
         if vz.ndim == 2:
             vz, eps, delta = vz[None], eps[None], delta[None]
@@ -1083,6 +1087,8 @@ class VtiPlan:
         st = np.zeros((self.nshot, 2, self.nx, self.nz), dtype=np.float32) if state else None
         self.ctx.check(lib().sfd_vti_plan_fetch(self.handle, _ptr(record) if nrec and nt else None,
                                                 _ptr(st) if state else None), "sfd_vti_plan_fetch")
+        if self.zero_first_sample and nt:
+            record[:, :, 0] = 0.0
         return (record, st) if state else record
 
     def close(self):

@@ -267,10 +267,21 @@ def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
     return ElasticShot(**kw)
 
 
+def vti_receiver_row(nrz0: int, nu: int):
+    """The VTI driver's recording rule (VTI main:469-476): the receiver row
+    nrz0 + nu (1-based), or row 6 when that lies above it -- and in that case
+    the first recorded sample (iit = 1) is never written, so it stays 0.
+    Returns (0-based row, first sample zero)."""
+    if nrz0 + nu >= 6:
+        return nrz0 + nu - 1, False
+    return 6 - 1, True
+
+
 def vti_card_survey(nshot: int = 300, nt: int = 10000, first: int = 1, model_n: int = 300):
     """Shots first .. first + nshot - 1 of the VTI card (1-based, as the
     reference numbers them).  Returns (vz, eps, delta) [nshot, 698, 348] and
-    the keyword arguments of VtiPlan."""
+    the keyword arguments of VtiPlan (the card's receiver row lies above row
+    6, so its records come from row 6 with the first sample zero)."""
     nxv = nzv = model_n
     dx = 5.0
     vz = smooth_random_model(nxv, nzv, 15, 1800.0, 3500.0, 12)
@@ -289,8 +300,8 @@ def vti_card_survey(nshot: int = 300, nt: int = 10000, first: int = 1, model_n: 
     V, E, D = (np.stack([s[k] for s in shots]) for k in range(3))
     sx = (nsx - xmin + nl) - 1               # currshot_nsx, the same for every shot
     sz = (nsz0 + nu) - 1
-    row = (nrz0 + nu if nrz0 + nu >= 6 else 6) - 1
+    row, zero_first = vti_receiver_row(nrz0, nu)
     rx = np.arange(nl, nl + rng, dtype=np.int32)
     kw = dict(dx=dx, dz=dx, dt=5e-4, nt=nt, f0=30.0, src=(sx, sz), rec=(rx, np.full(rng, row, np.int32)),
-              pml=(nl, nr, nu, nd))
+              pml=(nl, nr, nu, nd), zero_first_sample=zero_first)
     return V, E, D, kw

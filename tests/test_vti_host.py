@@ -48,3 +48,29 @@ def test_python_mirrors_refuse_short_inputs():
         sfd.VtiPlan(vz, z, z, c2=np.ones(4), **kw)
     with pytest.raises(ValueError, match="crg"):
         sfd.srm_predict(np.zeros((2, 2, 8), np.float32), np.zeros((6, 7), np.float32))
+
+
+def test_receiver_row_rule_matches_the_driver_loop():
+    """workloads.vti_receiver_row against a step-by-step run of the driver's
+    recording branch (VTI main:469-476) with a field whose row k holds 100 k
+    + it: the same row is recorded, and sample 1 stays 0 exactly when the
+    receiver row lies above row 6 (the card's case)."""
+    from seismic_code_github_amd import workloads as W
+    d, nsteps = 3, 12
+    for nrz0 in range(1, 5):
+        for nu in range(0, 6):
+            ref = np.zeros(nsteps - d)
+            for it in range(1, nsteps + 1):
+                u = {k: 100.0 * k + it for k in range(1, 20)}  # row k (1-based) at step it
+                iit = it - d
+                if iit >= 1 and nrz0 + nu >= 6:
+                    ref[iit - 1] = u[nrz0 + nu]
+                elif iit > 1 and nrz0 + nu < 6:
+                    ref[iit - 1] = u[6]
+            row, zero_first = W.vti_receiver_row(nrz0, nu)
+            ours = np.array([100.0 * (row + 1) + (iit + d) for iit in range(1, nsteps - d + 1)])
+            if zero_first:
+                ours[0] = 0.0
+            np.testing.assert_array_equal(ours, ref, err_msg=f"nrz0={nrz0} nu={nu}")
+    *_, kw = W.vti_card_survey(nshot=1, nt=10, model_n=40)
+    assert kw["zero_first_sample"] and kw["rec"][1][0] == 5
