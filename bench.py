@@ -118,7 +118,7 @@ def shape_dict(kind, a, b, us):
     return d
 
 
-def batched_leg(nt, no_tune=False, no_cpu=False):
+def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
     """64 shots on uneven aperture windows of a 1001 x 6001 model (C-PML 50,
     nt=3000 per shot): one tiled step per time step for all shots, replayed
     from a hipGraph, against the plain path (shot after shot, one step per
@@ -138,7 +138,7 @@ def batched_leg(nt, no_tune=False, no_cpu=False):
     order = np.argsort([s.vp.shape[0] for s in shots])
     mid = int(order[len(shots) // 2])
     rec_mid = b.fetch(mid)
-    p_ms = b.run(sfd.BATCH_PER_SHOT)
+    p_ms = b.run(sfd.BATCH_PER_SHOT) if plain else float("nan")
     b.close()
     scale = 3000.0 / nt
     out = {"workload": "64 shots, windows of 1000-2500 columns of a 1001x6001 model, C-PML 50, R=8, nt=3000",
@@ -612,6 +612,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched leg: skip the shot-after-shot comparison (counter runs)")
     ap.add_argument("--no-visco", action="store_true")
     ap.add_argument("--visco-err-nt", type=int, default=40)
     ap.add_argument("--visco-nt", type=int, default=4000)
@@ -669,7 +671,8 @@ def main():
     if args.only:
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
                "grad": lambda: gradient_leg(args.grad_steps, args.no_cpu),
-               "batch": lambda: batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu),
+               "batch": lambda: batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu,
+                                            not args.no_plain),
                "c0": lambda: c0_leg(args.no_cpu, args.no_tune),
                "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu),
                "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,
