@@ -143,13 +143,17 @@ int sfd_acoustic_plan_set_tiling(sfd_acoustic_plan *plan, int waves, int chunk_r
  * An explicit sfd_acoustic_plan_set_tiling also selects the march kernel.
  * Results are bitwise equal either way; the kernel may change between steps. */
 int sfd_acoustic_plan_set_tiled(sfd_acoustic_plan *plan, int tile_rows, int waves);
-/* Fast arithmetic as ONE launch per step (no counterpart in the reference):
- * each workgroup of `waves` (4 or 8) updates tile_rows (16 or 32) x 64
- * cells, recomputing the new velocities its pressure stencils need around
- * them, on ping-pong buffers, so every field crosses memory once per step
- * (28 B per cell with uniform density, against the two-launch step's 40 B);
- * built shapes 16x4, 16x8, 32x4, 32x8; 0 = 32 x 8.  Results are
- * bitwise equal to the other kernels. */
+/* Fast arithmetic as ONE launch per step (no counterpart in the reference),
+ * on ping-pong buffers, so every field crosses memory about once per step
+ * (28 B per cell with uniform density, against the two-launch step's 40 B).
+ * Each workgroup of `waves` updates tile_rows x 64 cells:
+ *   - tile_rows 16 or 32 (waves 4 or 8): the tiled step, which recomputes
+ *     the new velocities its pressure stencils need around its tile;
+ *   - tile_rows >= 64 (waves 4 or 8): the streaming step, which marches down
+ *     its chunk of rows in blocks of 4 x waves rows, keeping the P and new Vx
+ *     rows its stencils share in on-chip rings (4R - 2 extra P rows and 2R - 1
+ *     extra new Vx rows per chunk).
+ * 0 = 32 x 8.  Results are bitwise equal to the other kernels. */
 int sfd_acoustic_plan_set_fused(sfd_acoustic_plan *plan, int tile_rows, int waves);
 
 /* ---------------------------------------------------------------- batches

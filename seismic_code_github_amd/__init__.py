@@ -427,7 +427,8 @@ class AcousticPlan:
         self.ctx.check(lib().sfd_acoustic_plan_set_tiled(self.handle, tile_rows, waves), "set_tiled")
 
     def set_fused(self, rows: int = 0, waves: int = 0):
-        """One launch per step: tiles of `rows` x 64 cells per workgroup of `waves` (0 = 32 x 8)."""
+        """One launch per step: `rows` x 64 cells per workgroup of `waves` (0 = 32 x 8); rows 16 or
+        32 select the tiled step, rows >= 64 the streaming step (a chunk marched in row blocks)."""
         self.ctx.check(lib().sfd_acoustic_plan_set_fused(self.handle, rows, waves), "set_fused")
 
     def autotune(self, trial_steps: int = 20) -> float:

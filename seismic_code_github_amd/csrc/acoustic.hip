@@ -299,15 +299,15 @@ bool pick_tiled_batch(int radius, int tx, int nw, int mode, BatchKernel* v, Batc
     }
 }
 
-bool pick_fused_r4(int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
-bool pick_fused_r5(int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
-bool pick_fused_r8(int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
+bool pick_fused_r4(int rows, int nw, int mode, bool ubf, StepKernel* k, BatchKernel* b);
+bool pick_fused_r5(int rows, int nw, int mode, bool ubf, StepKernel* k, BatchKernel* b);
+bool pick_fused_r8(int rows, int nw, int mode, bool ubf, StepKernel* k, BatchKernel* b);
 
-bool pick_fused(int radius, int rows, int nw, int mode, StepKernel* k, BatchKernel* b) {
+bool pick_fused(int radius, int rows, int nw, int mode, bool ubf, StepKernel* k, BatchKernel* b) {
     switch (radius) {
-    case 4: return pick_fused_r4(rows, nw, mode, k, b);
-    case 5: return pick_fused_r5(rows, nw, mode, k, b);
-    case 8: return pick_fused_r8(rows, nw, mode, k, b);
+    case 4: return pick_fused_r4(rows, nw, mode, ubf, k, b);
+    case 5: return pick_fused_r5(rows, nw, mode, ubf, k, b);
+    case 8: return pick_fused_r8(rows, nw, mode, ubf, k, b);
     default: return false;
     }
 }
@@ -506,6 +506,7 @@ void tiled_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw) {
 }
 
 void fused_args(const sfd_acoustic_plan* pl, AcousticStepArgs& a, bool v_raw) {
+    a.chunk = pl->ftx;
     a.ntz = (pl->nz - 2 * pl->radius + kTiledTZ - 1) / kTiledTZ;
     a.nchunk = (pl->nx - 2 * pl->radius + pl->ftx - 1) / pl->ftx;
     a.v_raw = v_raw ? 1 : 0;
@@ -821,7 +822,7 @@ int sfd_acoustic_plan_set_fused(sfd_acoustic_plan* pl, int rows, int waves) {
         return fail(pl->ctx, SFD_ENOTSUP, "the one-launch step exists in fast arithmetic only");
     const int tr = rows ? rows : kFusedShapes[0][0], nw = waves ? waves : kFusedShapes[0][1];
     StepKernel k;
-    if (!pick_fused(pl->radius, tr, nw, pl->mode, &k, nullptr))
+    if (!pick_fused(pl->radius, tr, nw, pl->mode, pl->bf_const != 0.f, &k, nullptr))
         return fail(pl->ctx, SFD_EINVAL, "no one-launch step for %d rows x %d waves", tr, nw);
     pl->ftx = tr;
     pl->fnw = nw;
@@ -1169,7 +1170,9 @@ int sfd_acoustic_batch_set_fused(sfd_acoustic_batch* b, int rows, int waves) {
         return fail(b->ctx, SFD_ENOTSUP, "the one-launch step exists in fast arithmetic only");
     const int tr = rows ? rows : kFusedShapes[0][0], nw = waves ? waves : kFusedShapes[0][1];
     BatchKernel k;
-    if (!pick_fused(b->radius, tr, nw, b->mode, nullptr, &k))
+    bool ubf = true;  // every shot has uniform density
+    for (auto* pl : b->plans) ubf = ubf && pl->bf_const != 0.f;
+    if (!pick_fused(b->radius, tr, nw, b->mode, ubf, nullptr, &k))
         return fail(b->ctx, SFD_EINVAL, "no batched one-launch step for %d rows x %d waves", tr, nw);
     for (auto* pl : b->plans) {
         const int rc = sfd_acoustic_plan_set_fused(pl, tr, nw);
@@ -1286,9 +1289,9 @@ int sfd_acoustic_batch_destroy(sfd_acoustic_batch* b) {
 
 namespace {
 
-// Candidates: every built tiled shape and one-launch tiled shape (fast
-// arithmetic) and the march kernel at 1, 2 and 4 waves with its automatic
-// chunk.
+// Candidates: every built tiled shape, the one-launch tiled shapes and
+// streaming chunks (fast arithmetic) and the march kernel at 1, 2 and 4 waves
+// with its automatic chunk.
 std::vector<TunedShape> shape_candidates(int mode, bool batch) {
     std::vector<TunedShape> c;
     if (tiled_mode_ok(mode)) {
@@ -1299,6 +1302,7 @@ std::vector<TunedShape> shape_candidates(int mode, bool batch) {
         else
             for (auto& t : single) c.push_back({0, t[0], t[1], 0.f});
         for (auto& t : kFusedShapes) c.push_back({2, t[0], t[1], 0.f});
+        for (auto& t : kStreamShapes) c.push_back({2, t[0], t[1], 0.f});
     }
     for (int w : {1, 2, 4}) c.push_back({1, w, 0, 0.f});
     return c;

@@ -2,7 +2,7 @@
 #include "acoustic_fused_inst.h"
 
 namespace sfd {
-bool pick_fused_r4(int rows, int nw, int mode, StepKernel* k, BatchKernel* b) {
-    return fused_pick_r<4>(rows, nw, mode, k, b);
+bool pick_fused_r4(int rows, int nw, int mode, bool ubf, StepKernel* k, BatchKernel* b) {
+    return fused_pick_r<4>(rows, nw, mode, ubf, k, b);
 }
 }  // namespace sfd

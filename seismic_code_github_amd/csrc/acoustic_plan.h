@@ -18,14 +18,19 @@ BatchKernel pick_batch_kernel(int radius, int nw, int mode);
 // batches the default shape only.
 bool pick_tiled(int radius, int tx, int nw, int mode, StepKernel* v, StepKernel* p);
 bool pick_tiled_batch(int radius, int tx, int nw, int mode, BatchKernel* v, BatchKernel* p);
-// One-launch tiled step (acoustic_fused.h) of `mode` (C-PML or sponge) at
-// `rows` rows x 64 columns per workgroup of nw waves; k and b (either may be
-// null) receive the single-plan and the batched kernel.  False where no such
-// instance exists.
-bool pick_fused(int radius, int rows, int nw, int mode, StepKernel* k, BatchKernel* b);
-// Launch shapes (tile rows, waves) the one-launch step is built for; the
-// first is the default.
+// One-launch step of `mode` (C-PML or sponge) at `rows` rows x 64 columns
+// per workgroup of nw waves: the tiled step (acoustic_fused.h) for 16 or 32
+// rows, the streaming step (acoustic_stream.h) for chunks of kStreamMinRows
+// rows or more at 4 or 8 waves; k and b (either may be null) receive the
+// single-plan and the batched kernel.  ubf: every updated cell's bf is the
+// argument bf_const (uniform density; the streaming step then reads no bf
+// field).  False where no such instance exists.
+bool pick_fused(int radius, int rows, int nw, int mode, bool ubf, StepKernel* k, BatchKernel* b);
+// Launch shapes (rows, waves) the one-launch step is tuned over; the first
+// is the default.
 constexpr int kFusedShapes[][2] = {{32, 8}, {16, 4}, {16, 8}, {32, 4}};
+constexpr int kStreamMinRows = 64;
+constexpr int kStreamShapes[][2] = {{128, 4}, {192, 4}, {64, 4}, {128, 8}};
 int pad_rows(int radius);
 int round_chunk(int ch, int radius);
 // Host-side argument checks of an acoustic shot (host receiver arrays).

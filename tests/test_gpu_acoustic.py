@@ -26,8 +26,10 @@ from conftest import oracle_threads
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 TOL = 1e-5  # relative L2 bound of FP32 results, gathers and snapshots
-# one-launch tiled step shapes (tile rows, waves per workgroup)
-FUSED_SHAPES = ((32, 8), (16, 4), (16, 8), (32, 4))
+# one-launch step shapes (rows, waves per workgroup): the tiled step at 16 and
+# 32 rows, the streaming step at chunks of 64 rows or more (chunks that are
+# not a multiple of its 4 * waves row blocks included)
+FUSED_SHAPES = ((32, 8), (16, 4), (16, 8), (32, 4), (128, 4), (256, 8), (100, 4), (72, 8), (64, 4))
 
 
 def rel_l2(a, b):
@@ -379,6 +381,41 @@ def test_edge_geometry(ctx):
         np.testing.assert_array_equal(snaps[:, 1], o["snaps"][:, 2])
         assert np.all(rec[1] == 0) and np.all(rec[7] == 0) and np.all(rec[8] == 0)
         np.testing.assert_array_equal(rec[0], rec[2])
+
+
+@pytest.mark.parametrize("boundary", ["cpml", "sponge"])
+def test_one_launch_steps_heterogeneous_density(ctx, boundary):
+    """Per-cell density (the one-launch steps then read bf per cell): every
+    one-launch shape equals the two-launch tiled step and the march kernel bit
+    for bit, with the source and receivers next to chunk and strip edges, a
+    grid that is a multiple of no tile, and a receiver row split over two
+    spans."""
+    rng = np.random.default_rng(11)
+    nx, nz, R = 333, 203, 8
+    vp = (1800 + 900 * rng.random((nx, nz))).astype(np.float32)
+    den = (1900 + 600 * rng.random((nx, nz))).astype(np.float32)
+    w = W.ricker(220, 0.001, 25.0)
+    rx = np.concatenate([np.full(40, 71), np.full(30, 71), np.arange(20, 300, 7)]).astype(np.int32)
+    rz = np.concatenate([np.arange(10, 50), np.arange(120, 150), np.full(40, 9)]).astype(np.int32)
+    shot = sfd.AcousticShot(vp=vp, den=den, wavelet=w, src=(72, 9), rec=(rx, rz), dt=0.0008, radius=R,
+                            width=24, boundary=boundary, snap_steps=(219,), snap_fields=7)
+    print(f"case: one-launch steps, per-cell density, {boundary}", flush=True)
+    ref = sfd.AcousticPlan(shot, ctx)
+    ref.set_tiled(-1)
+    ref.run()
+    r0, s0 = ref.fetch()
+    ref.close()
+    assert np.abs(r0).max() > 0
+    shapes = [("tiled", 0, 0)] + [("fused", r, nw) for r, nw in FUSED_SHAPES]
+    for kind, a, b in shapes:
+        pl = sfd.AcousticPlan(shot, ctx)
+        pl.set_tiled(a, b) if kind == "tiled" else pl.set_fused(a, b)
+        pl.run()
+        r1, s1 = pl.fetch()
+        msg = f"{kind} {a}x{b}"
+        np.testing.assert_array_equal(r1, r0, err_msg=msg)
+        np.testing.assert_array_equal(s1, s0, err_msg=msg)
+        pl.close()
 
 
 def test_minimal_grid_and_no_receivers(ctx):

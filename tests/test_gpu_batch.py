@@ -27,12 +27,14 @@ def small_batch(boundary="cpml", nshot=5, nt=300, arith="fast"):
 
 
 @pytest.mark.parametrize("boundary,tiling", [("cpml", "march"), ("cpml", "default"), ("sponge", "march"),
-                                             ("sponge", "default"), ("cpml", "fused"), ("sponge", "fused")])
+                                             ("sponge", "default"), ("cpml", "fused"), ("sponge", "fused"),
+                                             ("cpml", "stream"), ("sponge", "stream")])
 def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
     """tiling "default": batches run the two-launch tiled step (in place;
     sponge velocities kept unabsorbed between steps), "fused" the one-launch
-    tiled step (ping-pong), the single plans the fused march kernel; still
-    bit for bit."""
+    tiled step (ping-pong), "stream" the one-launch streaming step (chunks of
+    72 rows, not a multiple of its 16-row blocks), the single plans the fused
+    march kernel; still bit for bit."""
     shots = small_batch(boundary)
     print(f"case: batch modes {boundary} {tiling}, shots {[s.vp.shape for s in shots]}", flush=True)
     singles = []
@@ -47,6 +49,8 @@ def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
         b.set_tiling(2, 48)
     elif tiling == "fused":
         b.set_fused(16, 4)
+    elif tiling == "stream":
+        b.set_fused(72, 4)
     for how in (sfd.BATCH_GRAPH, sfd.BATCH_LAUNCHES, sfd.BATCH_PER_SHOT):
         b.run(how)
         for s in range(len(shots)):

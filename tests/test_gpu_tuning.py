@@ -22,7 +22,8 @@ from seismic_code_github_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 TILED = {(8, 4), (8, 8), (16, 4), (16, 8), (16, 16), (32, 8), (32, 16)}
-FUSED = {(32, 8), (16, 4), (16, 8), (32, 4)}  # one-launch tiled step (tile rows, waves)
+# one-launch steps (rows, waves): tiled (16, 32 rows) and streaming (chunks)
+FUSED = {(32, 8), (16, 4), (16, 8), (32, 4), (128, 4), (192, 4), (64, 4), (128, 8)}
 
 
 def small(boundary="cpml", **kw):
