@@ -309,7 +309,7 @@ def visco_leg(nt_err, nt, no_cpu):
     return out
 
 
-def elastic_leg(nt_err, nt, no_cpu):
+def elastic_leg(nt_err, nt, no_cpu, nbatch=32):
     """Elastic free surface on the reference program's own shot grid
     (iso_2d_modelling_sin_test.par: 903 x 603 padded, nt = 3001 recorded + 48
     delay steps): Mcell-updates/s over all stored cells, the algorithmic bytes
@@ -350,6 +350,35 @@ def elastic_leg(nt_err, nt, no_cpu):
     if os.path.exists(tpath):
         with open(tpath) as f:
             out["traffic"] = json.load(f).get("hbm_bytes_per_step")
+    # The card's survey batched: shots 1..nbatch (the window follows the
+    # source), one launch per half-step over every shot's tiles.
+    brec = None
+    if nbatch > 1:
+        shots = W.elastic_par_survey(nshot=nbatch, nt=nt)
+        with sfd.ElasticBatch(shots) as b:
+            btuned = b.autotune(20)
+            bshape = b.shape()
+            b.run()
+            bms = min(b.run() for _ in range(2))
+            brec = b.fetch(0)[:2]
+        assert np.isfinite(brec[0]).all() and np.isfinite(brec[1]).all()
+        bbytes = bytes_step * nbatch
+        out["batch"] = {"workload": f"{nbatch} shots of the card's survey (shots 1..{nbatch}, dsx = 100 m), "
+                                    f"each on the {nx}x{nz} shot grid",
+                        "shots": nbatch, "ms_per_batch": round(bms, 2),
+                        "shots_per_s": round(nbatch / (bms / 1e3), 2),
+                        "Mcell_per_s": round(nbatch * nx * nz * nsteps / bms / 1e3, 1),
+                        "us_per_step": round(bms * 1e3 / nsteps, 2),
+                        "bytes_per_step": bbytes,
+                        "achieved_GBs": round(bbytes * nsteps / (bms / 1e3) / 1e9, 1),
+                        "frac_of_8TBs": round(bbytes * nsteps / (bms / 1e3) / 8e12, 4),
+                        "speedup_vs_single_shot": round(nbatch * ms / bms, 2),
+                        "launch_shape": {"kernel": "batched two-launch step", "tile_cols": bshape[0],
+                                         "waves": bshape[1], "autotuned_us_per_step": round(btuned, 2)}}
+        bpath = os.path.join(ROOT, "profiles", f"elastic_batch_{nbatch}x903x603_traffic.json")
+        if os.path.exists(bpath):
+            with open(bpath) as f:
+                out["batch"]["traffic"] = json.load(f).get("hbm_bytes_per_step")
     if not no_cpu:
         import oracle
         cores = min(16, len(os.sched_getaffinity(0)))
@@ -362,11 +391,17 @@ def elastic_leg(nt_err, nt, no_cpu):
         t0 = time.perf_counter()
         o32 = oracle.elastic(short.vp, short.vs, short.rho, short.itopo, precision="f32", **args)
         dt32 = time.perf_counter() - t0
-        for k, got in (("vx", svx), ("vz", svz)):
+        for q, k in enumerate(("vx", "vz")):
+            got = (svx, svz)[q]
             ref = o64["rec_" + k]
             nrm = np.linalg.norm(ref)
             out[f"fp64_rel_l2_{k}"] = float(np.linalg.norm(got - ref) / nrm)
             out[f"fp32_oracle_vs_fp64_rel_l2_{k}"] = float(np.linalg.norm(o32["rec_" + k] - ref) / nrm)
+            if brec is not None:
+                # the batch's shot 1 is this shot: its first nt_err samples
+                out["batch"][f"fp64_rel_l2_{k}"] = float(np.linalg.norm(brec[q][:, :nt_err] - ref) / nrm)
+        if brec is not None:
+            out["batch"]["error_sample"] = f"shot 1 of the batch, first {nt_err} recorded samples"
         out["error_sample"] = f"first {nt_err} recorded samples ({args['nsteps']} steps) of the same shot"
         out["cpu_baseline"] = {"value": round(nx * nz * args["nsteps"] / dt32 / 1e6, 2), "unit": "Mcell-updates/s",
                                "cores": cores, "kind": "port",
@@ -620,6 +655,7 @@ def main():
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--elastic-err-nt", type=int, default=100)
     ap.add_argument("--elastic-nt", type=int, default=3001)
+    ap.add_argument("--elastic-batch", type=int, default=32, help="shots of the batched elastic survey (0: none)")
     ap.add_argument("--no-vti", action="store_true")
     ap.add_argument("--vti-shots", type=int, default=300)
     ap.add_argument("--vti-nt", type=int, default=10000)
@@ -674,7 +710,7 @@ def main():
                "batch": lambda: batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu,
                                             not args.no_plain),
                "c0": lambda: c0_leg(args.no_cpu, args.no_tune),
-               "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu),
+               "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu, args.elastic_batch),
                "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,
                                       args.no_tune, vti_tiling),
                "srm": lambda: srm_leg(*srm_size, args.no_cpu)}[args.only]
@@ -770,7 +806,7 @@ def main():
         out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)
     if not args.no_elastic:
         log("elastic leg ...")
-        out["elastic"] = elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu)
+        out["elastic"] = elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu, args.elastic_batch)
     if not args.no_vti:
         log("vti leg ...")
         out["vti"] = vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu, args.no_tune, vti_tiling)

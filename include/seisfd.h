@@ -423,12 +423,37 @@ int sfd_elastic_plan_fetch(sfd_elastic_plan *plan, float *record_vx, float *reco
  * not every pair is built); 0 = the default. */
 int sfd_elastic_plan_set_tiling(sfd_elastic_plan *plan, int tile_cols, int waves);
 /* Autotuner (no counterpart in the reference), as sfd_acoustic_plan_autotune:
- * times the seven built tilings over min(trial_steps, delay + nt) steps,
+ * times the eight built tilings over min(trial_steps, delay + nt) steps,
  * keeps the fastest and caches it for the grid class (mode, nx and nz to the
  * nearest power of 2); resets the state. */
 int sfd_elastic_plan_autotune(sfd_elastic_plan *plan, int trial_steps, float *best_us);
 int sfd_elastic_plan_get_shape(const sfd_elastic_plan *plan, int *tile_cols, int *waves);
 int sfd_elastic_plan_destroy(sfd_elastic_plan *plan);
+
+/* Several elastic shots stepped together (no counterpart in the reference:
+ * its driver runs the shots one after another, distributing them over MPI
+ * ranks, Iso_multishot_fd_main_fsbc_zy_m2.f90:71-81).  Shot i is what
+ * sfd_elastic_plan_create(ctx, &params[i], vp[i], vs[i], rho[i], wavelet ? wavelet[i] : NULL)
+ * makes; one launch per half-step covers the tiles of every shot, so a
+ * survey of small shot grids fills the GPU.  Shots may differ in grid and
+ * geometry but share the mode, nt and the delay.  Each shot's gathers and
+ * snapshots are bitwise those of its own plan. */
+typedef struct sfd_elastic_batch sfd_elastic_batch;
+int sfd_elastic_batch_create(sfd_context *ctx, int nshot, const sfd_elastic_params *params,
+                             const float *const *vp, const float *const *vs, const float *const *rho,
+                             const float *const *wavelet, sfd_elastic_batch **batch);
+/* Same tiling for every shot (as sfd_elastic_plan_set_tiling); 0 = the default. */
+int sfd_elastic_batch_set_tiling(sfd_elastic_batch *batch, int tile_cols, int waves);
+/* Autotuner, as sfd_elastic_plan_autotune, cached for the batch's class (mode,
+ * shot count and cells per step to the nearest power of 2); resets the state. */
+int sfd_elastic_batch_autotune(sfd_elastic_batch *batch, int trial_steps, float *best_us);
+int sfd_elastic_batch_get_shape(const sfd_elastic_batch *batch, int *tile_cols, int *waves);
+/* Zeroes every shot's fields and runs all delay + nt steps; elapsed_ms as
+ * sfd_elastic_plan_run.  SFD_ENAN when a gather is not finite. */
+int sfd_elastic_batch_run(sfd_elastic_batch *batch, float *elapsed_ms);
+/* Shot `shot`'s outputs, as sfd_elastic_plan_fetch. */
+int sfd_elastic_batch_fetch(sfd_elastic_batch *batch, int shot, float *record_vx, float *record_vz, float *snaps);
+int sfd_elastic_batch_destroy(sfd_elastic_batch *batch);
 
 /* ---------------------------------------------------------------- VTI
  * VTI qP-acoustic propagator (p and q fields, 2nd order in time, 10th-order

@@ -246,6 +246,16 @@ _SIGNATURES = {
     "sfd_elastic_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP, _FP]),
     "sfd_elastic_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_elastic_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_elastic_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ElasticParamsC),
+                                                ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
+                                                ctypes.POINTER(_FP), ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_elastic_batch_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_elastic_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP]),
+    "sfd_elastic_batch_get_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                   ctypes.POINTER(ctypes.c_int)]),
+    "sfd_elastic_batch_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_elastic_batch_fetch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP, _FP, _FP]),
+    "sfd_elastic_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_window_model": (ctypes.c_int, [_FP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _FP]),
 }
@@ -947,6 +957,81 @@ class ElasticPlan:
     def close(self):
         if getattr(self, "handle", None):
             lib().sfd_elastic_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ElasticBatch:
+    """``sfd_elastic_batch``: several elastic shots advanced by one launch per
+    half-step (the shots share the mode, nt and the delay)."""
+
+    def __init__(self, shots, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
+        self.shots = list(shots)
+        n = len(self.shots)
+        params = (ElasticParamsC * n)()
+        self._keep = []
+        vps, vss, rhos, ws = (_FP * n)(), (_FP * n)(), (_FP * n)(), (_FP * n)()
+        for i, sh in enumerate(self.shots):
+            params[i] = sh.params(self._keep)
+            vp, vs, rho, w = sh._arrays()
+            self._keep += [vp, vs, rho, w, sh]
+            vps[i], vss[i], rhos[i], ws[i] = _ptr(vp), _ptr(vs), _ptr(rho), _ptr(w)
+        self.params = params
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_elastic_batch_create(self.ctx.handle, n, params, vps, vss, rhos, ws,
+                                                      ctypes.byref(h)), "sfd_elastic_batch_create")
+        self.handle = h
+
+    def set_tiling(self, tile_cols: int = 0, waves: int = 0):
+        self.ctx.check(lib().sfd_elastic_batch_set_tiling(self.handle, tile_cols, waves), "elastic batch set_tiling")
+
+    def autotune(self, trial_steps: int = 20) -> float:
+        us = ctypes.c_float()
+        self.ctx.check(lib().sfd_elastic_batch_autotune(self.handle, trial_steps, ctypes.byref(us)),
+                       "elastic batch autotune")
+        return us.value
+
+    def shape(self):
+        """(tile columns, waves)."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(lib().sfd_elastic_batch_get_shape(self.handle, ctypes.byref(a), ctypes.byref(b)),
+                       "elastic batch get_shape")
+        return a.value, b.value
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_elastic_batch_run(self.handle, ctypes.byref(ms)), "sfd_elastic_batch_run")
+        return ms.value
+
+    def fetch(self, shot: int):
+        """Shot `shot`'s (record_vx [nrec, nt], record_vz [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
+        p, sh = self.params[shot], self.shots[shot]
+        nx, nz = np.asarray(sh.vp).shape
+        rvx = np.zeros((p.nrec, p.nt), dtype=np.float32)
+        rvz = np.zeros((p.nrec, p.nt), dtype=np.float32)
+        snaps = np.zeros((p.nsnap, sh.nfields(), nx, nz), dtype=np.float32)
+        has = p.nrec and p.nt
+        self.ctx.check(lib().sfd_elastic_batch_fetch(self.handle, shot, _ptr(rvx) if has else None,
+                                                     _ptr(rvz) if has else None, _ptr(snaps) if p.nsnap else None),
+                       "sfd_elastic_batch_fetch")
+        return rvx, rvz, snaps
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_elastic_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

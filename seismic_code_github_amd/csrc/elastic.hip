@@ -36,6 +36,22 @@ struct sfd_elastic_plan {
     float* cell0(const sfd::Field& f) const { return f.as<float>() + (size_t)sfd::kElPad * pitch + sfd::kElPad; }
 };
 
+// Several shots stepped together: each shot keeps its own plan (fields,
+// model, gathers); one launch per half-step covers the tiles of every shot.
+struct sfd_elastic_batch {
+    sfd_context* ctx = nullptr;
+    std::vector<sfd_elastic_plan*> plans;
+    bool abc = false;
+    int nsteps = 0, delay = 0;
+    int tx = 0, nw = 0, ntile = 0;
+    void (*vel)(sfd::ElasticBatchArgs) = nullptr;
+    void (*str)(sfd::ElasticBatchArgs) = nullptr;
+    sfd::DeviceBuffer shot_args, tile_map;
+    ~sfd_elastic_batch() {
+        for (auto* pl : plans) sfd_elastic_plan_destroy(pl);
+    }
+};
+
 namespace sfd {
 namespace {
 
@@ -196,11 +212,37 @@ bool el_pick(int tx, int nw, bool abc, ElasticKernel* v, ElasticKernel* s) {
     case 1604: el_pair<16, 4>(abc, v, s); return true;
     case 1608: el_pair<16, 8>(abc, v, s); return true;
     case 1616: el_pair<16, 16>(abc, v, s); return true;
+    case 3204: el_pair<32, 4>(abc, v, s); return true;
     case 3208: el_pair<32, 8>(abc, v, s); return true;
     case 3216: el_pair<32, 16>(abc, v, s); return true;
     default: return false;
     }
 }
+
+using ElasticBatchKernel = void (*)(ElasticBatchArgs);
+
+template <int TX, int NW>
+void el_batch_pair(bool abc, ElasticBatchKernel* v, ElasticBatchKernel* s) {
+    *v = abc ? elastic_velocity_batch<TX, NW, true> : elastic_velocity_batch<TX, NW, false>;
+    *s = abc ? elastic_stress_batch<TX, NW, true> : elastic_stress_batch<TX, NW, false>;
+}
+
+bool el_batch_pick(int tx, int nw, bool abc, ElasticBatchKernel* v, ElasticBatchKernel* s) {
+    switch (tx * 100 + nw) {
+    case 804: el_batch_pair<8, 4>(abc, v, s); return true;
+    case 808: el_batch_pair<8, 8>(abc, v, s); return true;
+    case 1604: el_batch_pair<16, 4>(abc, v, s); return true;
+    case 1608: el_batch_pair<16, 8>(abc, v, s); return true;
+    case 1616: el_batch_pair<16, 16>(abc, v, s); return true;
+    case 3204: el_batch_pair<32, 4>(abc, v, s); return true;
+    case 3208: el_batch_pair<32, 8>(abc, v, s); return true;
+    case 3216: el_batch_pair<32, 16>(abc, v, s); return true;
+    default: return false;
+    }
+}
+
+// Tilings the autotuners try (tile columns, waves).
+constexpr int kElShapes[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 4}, {32, 8}, {32, 16}};
 
 int el_set_tiling(sfd_elastic_plan* pl, int tx, int nw) {
     ElasticKernel v, s;
@@ -231,13 +273,31 @@ int elastic_zero(sfd_elastic_plan* pl) {
     return SFD_OK;
 }
 
+// Snapshots the plan takes after step s (the next one is snap_steps[*next]).
+int elastic_snaps(sfd_elastic_plan* pl, int s, size_t* next) {
+    sfd_context* ctx = pl->ctx;
+    const size_t cell_bytes = (size_t)pl->nz * sizeof(float);
+    const ElasticStepArgs& a = pl->args;
+    while (*next < pl->snap_steps.size() && pl->snap_steps[*next] == s) {
+        float* dst = pl->snaps.as<float>() + *next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
+        const float* src[5] = {a.vx, a.vz, a.sxx, a.szz, a.sxz};
+        for (int f = 0; f < 5; ++f) {
+            if (!(pl->snap_fields & (2 << f))) continue;
+            SFD_HIP(ctx, hipMemcpy2DAsync(dst, cell_bytes, src[f], (size_t)pl->pitch * sizeof(float), cell_bytes,
+                                          pl->nx, hipMemcpyDeviceToDevice, ctx->stream));
+            dst += (size_t)pl->nx * pl->nz;
+        }
+        ++*next;
+    }
+    return SFD_OK;
+}
+
 // Enqueues the first nsteps steps (all of them by default).
 int elastic_enqueue(sfd_elastic_plan* pl, int nsteps = -1) {
     sfd_context* ctx = pl->ctx;
     const dim3 grid(pl->args.ntiles), block(64 * pl->nw);
     ElasticStepArgs a = pl->args;
     size_t snap_next = 0;
-    const size_t cell_bytes = (size_t)pl->nz * sizeof(float);
     const int ns = nsteps < 0 ? pl->nsteps : std::min(nsteps, pl->nsteps);
     for (int s = 0; s < ns; ++s) {
         a.step = s;
@@ -246,19 +306,81 @@ int elastic_enqueue(sfd_elastic_plan* pl, int nsteps = -1) {
         SFD_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(pl->str, grid, block, 0, ctx->stream, a);
         SFD_HIP(ctx, hipGetLastError());
-        while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == s) {
-            float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
-            const float* src[5] = {a.vx, a.vz, a.sxx, a.szz, a.sxz};
-            for (int f = 0; f < 5; ++f) {
-                if (!(pl->snap_fields & (2 << f))) continue;
-                SFD_HIP(ctx, hipMemcpy2DAsync(dst, cell_bytes, src[f], (size_t)pl->pitch * sizeof(float), cell_bytes,
-                                              pl->nx, hipMemcpyDeviceToDevice, ctx->stream));
-                dst += (size_t)pl->nx * pl->nz;
-            }
-            ++snap_next;
+        const int rc = elastic_snaps(pl, s, &snap_next);
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
+// Batch tables: every shot's argument block (at tile_cols x waves) and the
+// (shot, tile) of every global tile.
+int el_batch_set_tiling(sfd_elastic_batch* b, int tx, int nw) {
+    ElasticBatchKernel v, s;
+    if (!el_batch_pick(tx, nw, b->abc, &v, &s))
+        return fail(b->ctx, SFD_EINVAL, "no elastic kernel for %d tile columns x %d waves", tx, nw);
+    std::vector<ElasticStepArgs> args;
+    std::vector<int2> map;
+    for (size_t i = 0; i < b->plans.size(); ++i) {
+        const int rc = el_set_tiling(b->plans[i], tx, nw);
+        if (rc) return rc;
+        args.push_back(b->plans[i]->args);
+        for (int t = 0; t < b->plans[i]->args.ntiles; ++t) map.push_back(make_int2((int)i, t));
+    }
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
+    if (e == hipSuccess) e = upload(b->tile_map, map.data(), map.size(), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_check(b->ctx, e, "elastic batch tables");
+    b->tx = tx;
+    b->nw = nw;
+    b->vel = v;
+    b->str = s;
+    b->ntile = (int)map.size();
+    return SFD_OK;
+}
+
+int el_batch_zero(sfd_elastic_batch* b) {
+    for (auto* pl : b->plans) {
+        const int rc = elastic_zero(pl);
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
+// Enqueues the first nsteps steps of every shot (all of them by default).
+int el_batch_enqueue(sfd_elastic_batch* b, int nsteps = -1) {
+    sfd_context* ctx = b->ctx;
+    const dim3 grid(b->ntile), block(64 * b->nw);
+    ElasticBatchArgs a{};
+    a.shots = b->shot_args.as<ElasticStepArgs>();
+    a.tile_map = b->tile_map.as<int2>();
+    a.ntile = b->ntile;
+    std::vector<size_t> snap_next(b->plans.size(), 0);
+    const int ns = nsteps < 0 ? b->nsteps : std::min(nsteps, b->nsteps);
+    for (int s = 0; s < ns; ++s) {
+        a.step = s;
+        a.rec_col = s >= b->delay ? s - b->delay : -1;
+        hipLaunchKernelGGL(b->vel, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(b->str, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        for (size_t i = 0; i < b->plans.size(); ++i) {
+            const int rc = elastic_snaps(b->plans[i], s, &snap_next[i]);
+            if (rc) return rc;
         }
     }
     return SFD_OK;
+}
+
+// Autotuner grid class of a batch: mode, shot count and cells per step to
+// the nearest power of 2.
+std::string elastic_batch_class(const sfd_elastic_batch* b) {
+    size_t cells = 0;
+    for (auto* pl : b->plans) cells += (size_t)pl->nx * pl->nz;
+    char buf[96];
+    snprintf(buf, sizeof buf, "elastic-batch:%s:n%d:c%d", b->abc ? "abc" : "fsbc", log2_round((double)b->plans.size()),
+             log2_round((double)cells));
+    return buf;
 }
 
 }  // namespace
@@ -571,11 +693,12 @@ int sfd_elastic_plan_autotune(sfd_elastic_plan* pl, int trial_steps, float* best
     float warm = 0.f;
     int rc = time_steps(ctx, std::min(pl->nsteps, 4 * n), run, &warm);
     if (rc) return rc;
-    static const int shapes[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 8}, {32, 16}};
-    float us[7];
+    const auto& shapes = kElShapes;
+    constexpr int nshape = (int)(sizeof(kElShapes) / sizeof(kElShapes[0]));
+    float us[nshape];
     for (float& u : us) u = 3.4e38f;
     for (int pass = 0; pass < 2; ++pass)
-        for (int q = 0; q < 7; ++q) {
+        for (int q = 0; q < nshape; ++q) {
             rc = el_set_tiling(pl, shapes[q][0], shapes[q][1]);
             float t = 0.f;
             if (!rc) rc = time_steps(ctx, n, run, &t);
@@ -583,7 +706,7 @@ int sfd_elastic_plan_autotune(sfd_elastic_plan* pl, int trial_steps, float* best
             us[q] = std::min(us[q], t);
         }
     int best = 0;
-    for (int q = 1; q < 7; ++q)
+    for (int q = 1; q < nshape; ++q)
         if (us[q] < us[best]) best = q;
     rc = el_set_tiling(pl, shapes[best][0], shapes[best][1]);
     if (!rc) rc = elastic_zero(pl);
@@ -643,6 +766,135 @@ int sfd_elastic_plan_destroy(sfd_elastic_plan* pl) {
     (void)hipSetDevice(pl->ctx->device);
     (void)hipStreamSynchronize(pl->ctx->stream);
     delete pl;
+    return SFD_OK;
+}
+
+int sfd_elastic_batch_create(sfd_context* ctx, int nshot, const sfd_elastic_params* params, const float* const* vp,
+                             const float* const* vs, const float* const* rho, const float* const* wavelet,
+                             sfd_elastic_batch** out) {
+    if (!ctx || !out) return SFD_EINVAL;
+    *out = nullptr;
+    if (nshot < 1 || !params || !vp || !vs || !rho) return fail(ctx, SFD_EINVAL, "a batch needs nshot >= 1 shots");
+    sfd_elastic_batch* b = new (std::nothrow) sfd_elastic_batch;
+    if (!b) return fail(ctx, SFD_ENOMEM, "host allocation failed");
+    b->ctx = ctx;
+    for (int i = 0; i < nshot; ++i) {
+        sfd_elastic_plan* pl = nullptr;
+        const int rc = sfd_elastic_plan_create(ctx, &params[i], vp[i], vs[i], rho[i], wavelet ? wavelet[i] : nullptr,
+                                               &pl);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        b->plans.push_back(pl);
+        if (i == 0) {
+            b->abc = pl->abc;
+            b->nsteps = pl->nsteps;
+            b->delay = pl->delay;
+        } else if (pl->abc != b->abc || pl->nsteps != b->nsteps || pl->delay != b->delay) {
+            delete b;
+            return fail(ctx, SFD_EINVAL, "shot %d: the shots of a batch share the mode, nt and the delay", i);
+        }
+    }
+    int tx = kElDefaultTX, nw = kElDefaultNW;
+    const auto tuned = ctx->tuning.find(elastic_batch_class(b));
+    if (tuned != ctx->tuning.end()) {
+        tx = tuned->second.a;
+        nw = tuned->second.b;
+    }
+    const int rc = el_batch_set_tiling(b, tx, nw);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SFD_OK;
+}
+
+int sfd_elastic_batch_set_tiling(sfd_elastic_batch* b, int tile_cols, int waves) {
+    if (!b) return SFD_EINVAL;
+    SFD_HIP(b->ctx, hipSetDevice(b->ctx->device));
+    return el_batch_set_tiling(b, tile_cols ? tile_cols : kElDefaultTX, waves ? waves : kElDefaultNW);
+}
+
+int sfd_elastic_batch_autotune(sfd_elastic_batch* b, int trial_steps, float* best_us) {
+    if (!b) return SFD_EINVAL;
+    sfd_context* ctx = b->ctx;
+    if (trial_steps < 1) return fail(ctx, SFD_EINVAL, "trial_steps must be >= 1");
+    const int n = std::min(trial_steps, b->nsteps);
+    if (n < 1) return fail(ctx, SFD_EINVAL, "the batch has no time steps to time");
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    auto run = [&](int m) { return m < 0 ? el_batch_zero(b) : el_batch_enqueue(b, m); };
+    float warm = 0.f;
+    int rc = time_steps(ctx, std::min(b->nsteps, 2 * n), run, &warm);
+    if (rc) return rc;
+    constexpr int ns = (int)(sizeof(kElShapes) / sizeof(kElShapes[0]));
+    float us[ns];
+    for (float& u : us) u = 3.4e38f;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int q = 0; q < ns; ++q) {
+            rc = el_batch_set_tiling(b, kElShapes[q][0], kElShapes[q][1]);
+            float t = 0.f;
+            if (!rc) rc = time_steps(ctx, n, run, &t);
+            if (rc) return rc;
+            us[q] = std::min(us[q], t);
+        }
+    int best = 0;
+    for (int q = 1; q < ns; ++q)
+        if (us[q] < us[best]) best = q;
+    rc = el_batch_set_tiling(b, kElShapes[best][0], kElShapes[best][1]);
+    if (!rc) rc = el_batch_zero(b);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    TunedShape t;
+    t.kind = 0;
+    t.a = kElShapes[best][0];
+    t.b = kElShapes[best][1];
+    t.us = us[best];
+    ctx->tuning[elastic_batch_class(b)] = t;
+    if (best_us) *best_us = us[best];
+    return SFD_OK;
+}
+
+int sfd_elastic_batch_get_shape(const sfd_elastic_batch* b, int* tile_cols, int* waves) {
+    if (!b || !tile_cols || !waves) return SFD_EINVAL;
+    *tile_cols = b->tx;
+    *waves = b->nw;
+    return SFD_OK;
+}
+
+int sfd_elastic_batch_run(sfd_elastic_batch* b, float* elapsed_ms) {
+    if (!b) return SFD_EINVAL;
+    sfd_context* ctx = b->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = el_batch_zero(b);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    rc = el_batch_enqueue(b);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
+    for (auto* pl : b->plans) {
+        rc = check_finite(ctx, pl->rec_vx.as<float>(), pl->rec_vx.bytes / 4, "a vx gather of the batch");
+        if (!rc) rc = check_finite(ctx, pl->rec_vz.as<float>(), pl->rec_vz.bytes / 4, "a vz gather of the batch");
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
+int sfd_elastic_batch_fetch(sfd_elastic_batch* b, int shot, float* record_vx, float* record_vz, float* snaps) {
+    if (!b) return SFD_EINVAL;
+    if (shot < 0 || shot >= (int)b->plans.size())
+        return fail(b->ctx, SFD_EINVAL, "shot %d outside the batch (%zu shots)", shot, b->plans.size());
+    return sfd_elastic_plan_fetch(b->plans[shot], record_vx, record_vz, snaps);
+}
+
+int sfd_elastic_batch_destroy(sfd_elastic_batch* b) {
+    if (!b) return SFD_EINVAL;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
     return SFD_OK;
 }
 

@@ -74,8 +74,8 @@ struct ElasticStepArgs {
 // Strip cell index, or -1: left / right strip columns over all rows, then the
 // bottom-strip rows of the columns between them, then ('abc') the npzn + 1
 // rows t - npzn .. t of those columns.
-template <bool ABC>
-__device__ __forceinline__ int el_strip(const ElasticStepArgs& a, int col, int k, int t) {
+template <bool ABC, class A>
+__device__ __forceinline__ int el_strip(const A& a, int col, int k, int t) {
     if (col < a.nl) return col * a.nz + k;
     if (col >= a.nx - a.nr) return (a.nl + col - (a.nx - a.nr)) * a.nz + k;
     if (k >= a.kb0) return (a.nl + a.nr) * a.nz + (col - a.nl) * a.nb + (k - a.kb0);
@@ -92,8 +92,8 @@ __device__ __forceinline__ int el_order(int d, int zflag) {
 
 // z stencil sum on an LDS column: p points at the output row's own sample;
 // zflag 0 pairs p[n] - p[-n-1], zflag 1 pairs p[n+1] - p[-n].
-template <int ZF>
-__device__ __forceinline__ float el_zsum(const ElasticStepArgs& a, const float* p, int d) {
+template <int ZF, class A>
+__device__ __forceinline__ float el_zsum(const A& a, const float* p, int d) {
     const int lim = ZF ? 4 : 5;
     if (d >= lim) {
         float s = a.a10[0] * (p[ZF] - p[ZF - 1]);
@@ -102,17 +102,17 @@ __device__ __forceinline__ float el_zsum(const ElasticStepArgs& a, const float* 
         return s;
     }
     const int M = el_order(d, ZF);
-    const float* c = M == 4 ? a.a8 : M == 3 ? a.a6 : a.a4;
+    auto c = [&](int n) { return M == 4 ? a.a8[n] : M == 3 ? a.a6[n] : a.a4[n]; };
     if (M == 1) return p[ZF] - p[ZF - 1];
-    float s = c[0] * (p[ZF] - p[ZF - 1]);
-    for (int n = 1; n < M; ++n) s += c[n] * (p[n + ZF] - p[ZF - n - 1]);
+    float s = c(0) * (p[ZF] - p[ZF - 1]);
+    for (int n = 1; n < M; ++n) s += c(n) * (p[n + ZF] - p[ZF - n - 1]);
     return s;
 }
 
 // x stencil sum over LDS columns `stride` floats apart: xflag 0 pairs
 // q[n] - q[-n-1], xflag 1 pairs q[n+1] - q[-n].
-template <int XF>
-__device__ __forceinline__ float el_xsum(const ElasticStepArgs& a, const float* q, int stride) {
+template <int XF, class A>
+__device__ __forceinline__ float el_xsum(const A& a, const float* q, int stride) {
     float s = a.a10[0] * (q[XF * stride] - q[(XF - 1) * stride]);
 #pragma unroll
     for (int n = 1; n < 5; ++n) s += a.a10[n] * (q[(n + XF) * stride] - q[(XF - n - 1) * stride]);
@@ -120,57 +120,131 @@ __device__ __forceinline__ float el_xsum(const ElasticStepArgs& a, const float* 
 }
 
 // Stages columns [x0 - HX, x0 + TX + HX) x rows [z0 - HZ, z0 + kElTZ + HZ)
-// of a padded field into lds[c][r] (a fixed, unrolled count per thread, so
-// all loads are in flight before the first LDS store waits).
+// of a padded field into lds[c][r]: load() issues a fixed, unrolled count of
+// loads per thread into registers and store() writes them to LDS, so the
+// loads of every staged field are in flight before the first LDS store waits.
 template <int TX, int NW, int HX, int HZ>
-__device__ __forceinline__ void el_stage(float* lds, const float* f, int pitch, int x0, int z0) {
-    constexpr int W = TX + 2 * HX, Hh = kElTZ + 2 * HZ, N = W * Hh, T = 64 * NW;
-    constexpr int ITER = (N + T - 1) / T;
+struct ElStage {
+    static constexpr int W = TX + 2 * HX, Hh = kElTZ + 2 * HZ, N = W * Hh, T = 64 * NW;
+    static constexpr int ITER = (N + T - 1) / T;
     float v[ITER];
+    __device__ __forceinline__ void load(const float* f, int pitch, int x0, int z0) {
 #pragma unroll
-    for (int q = 0; q < ITER; ++q) {
-        const int e = threadIdx.x + q * T;
-        const int c = e / Hh, r = e - c * Hh;
-        v[q] = (e < N) ? f[(ptrdiff_t)(x0 - HX + c) * pitch + (z0 - HZ + r)] : 0.f;
+        for (int q = 0; q < ITER; ++q) {
+            const int e = threadIdx.x + q * T;
+            const int c = e / Hh, r = e - c * Hh;
+            v[q] = (e < N) ? f[(ptrdiff_t)(x0 - HX + c) * pitch + (z0 - HZ + r)] : 0.f;
+        }
     }
+    __device__ __forceinline__ void store(float* lds) const {
 #pragma unroll
-    for (int q = 0; q < ITER; ++q) {
-        const int e = threadIdx.x + q * T;
-        if (e < N) lds[e] = v[q];
+        for (int q = 0; q < ITER; ++q) {
+            const int e = threadIdx.x + q * T;
+            if (e < N) lds[e] = v[q];
+        }
     }
-}
+};
 
-// Tile of this workgroup: XCD-aware order over (x band, z tile), z fastest.
-__device__ __forceinline__ void el_tile(const ElasticStepArgs& a, int tx, int* z0, int* x0) {
-    const int t = xcd_tile(blockIdx.x, a.ntiles);
+// Many shots per launch: shots[s] is shot s's argument block (read through
+// the constant address space), tile_map[g] = (shot, the shot's tile) of
+// global tile g; step and rec_col are shared by the shots (one time axis).
+struct ElasticBatchArgs {
+    const ElasticStepArgs* shots;
+    const int2* tile_map;
+    int ntile;
+    int step, rec_col;
+};
+
+using ElasticShotArgs = const __attribute__((address_space(4))) ElasticStepArgs;
+
+// Tile t of a shot: x band t / ntz, z tile t % ntz (tiles are dealt to the
+// XCDs in order, z fastest, so an XCD holds whole x bands).
+template <class A>
+__device__ __forceinline__ void el_tile(const A& a, int tx, int t, int* z0, int* x0) {
     *z0 = (t % a.ntz) * kElTZ;
     *x0 = (t / a.ntz) * tx;
 }
 
 // Velocity half-step (pri:119-151; 'abc': pri:300-318): vx from tau_xx,
 // tau_xz; vz from tau_xz, tau_zz; in 'fsbc' mode the pending vz(t - 1).
-template <int TX, int NW, bool ABC>
-__global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepArgs a) {
+template <int TX>
+struct ElVelLds {
+    static constexpr int WX = TX + 2 * kElH, HZZ = kElTZ + 2 * kElH;
+    float xx[WX * kElTZ], xz[WX * HZZ], zz[TX * HZZ];
+};
+
+template <int TX, int NW, bool ABC, class A>
+__device__ __forceinline__ void el_velocity(const A& a, int tile, ElVelLds<TX>& lds) {
     constexpr int HX = kElH, HZ = kElH;
-    constexpr int WX = TX + 2 * HX, HZZ = kElTZ + 2 * HZ;
-    __shared__ float s_xx[WX * kElTZ];
-    __shared__ float s_xz[WX * HZZ];
-    __shared__ float s_zz[TX * HZZ];
+    constexpr int HZZ = kElTZ + 2 * HZ;
+    constexpr int NC = TX / NW;  // columns per wave
+    float* s_xx = lds.xx;
+    float* s_xz = lds.xz;
+    float* s_zz = lds.zz;
     int z0, x0;
-    el_tile(a, TX, &z0, &x0);
-    el_stage<TX, NW, HX, 0>(s_xx, a.sxx, a.pitch, x0, z0);
-    el_stage<TX, NW, HX, HZ>(s_xz, a.sxz, a.pitch, x0, z0);
-    el_stage<TX, NW, 0, HZ>(s_zz, a.szz, a.pitch, x0, z0);
-    __syncthreads();
-    const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    el_tile(a, TX, tile, &z0, &x0);
+    const int tz = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int k = z0 + tz;
-    if (k >= a.nz) return;
-    for (int j = 0; j < TX / NW; ++j) {
+    const bool kin = k < a.nz;
+    float* P = a.parts;
+    const size_t n = (size_t)a.nstrip;
+    // The bottom sponge's factors of this thread's row.
+    const bool kbot = kin && k >= a.kb0;
+    const float fzk = kbot ? a.fzn[k] : 1.f, fhk = kbot ? a.fzh[k] : 1.f;
+    // Every column's surface row (scalar loads) and every updated cell's
+    // inputs -- busg2e, busg1e and vx, vz, or the four split parts in a
+    // sponge strip -- are loaded before the staging, so their latencies
+    // overlap instead of adding up column after column.
+    int tc[NC], sc[NC];
+    bool uc[NC];
+    float bxc[NC], bzc[NC], q0[NC], q1[NC], q2[NC], q3[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const int col = x0 + wv + j * NW;
+        const bool cin = col < a.nx;  // wave-uniform
+        const int t = cin ? ro(a.itopo, col) : 0;
+        const int k0 = ABC ? (cin ? ro(a.kst, col) : 0) : t;
+        tc[j] = t;
+        uc[j] = cin && kin && k >= k0;
+        sc[j] = uc[j] ? el_strip<ABC>(a, col, k, t) : -1;
+        q0[j] = q1[j] = q2[j] = q3[j] = bxc[j] = bzc[j] = 0.f;
+        if (uc[j]) {
+            const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
+            const int sj = sc[j];
+            bxc[j] = a.b2e[o];
+            bzc[j] = a.b1e[o];
+            if (sj < 0) {
+                q0[j] = a.vx[o];
+                q1[j] = a.vz[o];
+            } else {
+                q0[j] = P[sj];
+                q1[j] = P[n + sj];
+                q2[j] = P[2 * n + sj];
+                q3[j] = P[3 * n + sj];
+            }
+        }
+    }
+    {
+        ElStage<TX, NW, HX, 0> gxx;
+        ElStage<TX, NW, HX, HZ> gxz;
+        ElStage<TX, NW, 0, HZ> gzz;
+        gxx.load(a.sxx, a.pitch, x0, z0);
+        gxz.load(a.sxz, a.pitch, x0, z0);
+        gzz.load(a.szz, a.pitch, x0, z0);
+        gxx.store(s_xx);
+        gxz.store(s_xz);
+        gzz.store(s_zz);
+    }
+    __syncthreads();
+    if (!kin) return;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
         const int cl = wv + j * NW, col = x0 + cl;
         if (col >= a.nx) break;
-        const int t = a.itopo[col];
+        const int t = tc[j];
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
-        if (k < (ABC ? a.kst[col] : t)) {
+        if (!uc[j]) {
             if (!ABC && k == t - 1) a.vz[o] = a.ghost[col];
             continue;
         }
@@ -178,40 +252,38 @@ __global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepAr
         const float* pxx = s_xx + (cl + HX) * kElTZ + tz;
         const float* pxz = s_xz + (cl + HX) * HZZ + tz + HZ;
         const float* pzz = s_zz + cl * HZZ + tz + HZ;
-        const float bx = a.b2e[o], bz = a.b1e[o];
+        const float bx = bxc[j], bz = bzc[j];
         const float ux = a.cx * bx * el_xsum<0>(a, pxx, kElTZ);   // calux xflag 0 on tau_xx
         const float uz = a.cz * bx * el_zsum<0>(a, pxz, d);       // caluz zflag 0 on tau_xz
         const float wx = a.cx * bz * el_xsum<1>(a, pxz, HZZ);     // calux xflag 1 on tau_xz
         const float wz = a.cz * bz * el_zsum<1>(a, pzz, d);       // caluz zflag 1 on tau_zz
-        const int s = el_strip<ABC>(a, col, k, t);
+        const int s = sc[j];
         if (s < 0) {
-            a.vx[o] = a.vx[o] + ux + uz;
-            a.vz[o] = a.vz[o] + wx + wz;
+            a.vx[o] = q0[j] + ux + uz;
+            a.vz[o] = q1[j] + wx + wz;
             continue;
         }
-        float* P = a.parts;
-        const size_t n = (size_t)a.nstrip;
-        const bool dx = k >= a.srow[col];
+        const bool dx = k >= ro(a.srow, col);
         float fzv = 1.f, fzw = 1.f;  // z factors of vx (node table) and vz (half table)
         bool dz = false;
-        if (k >= a.kb0) {
+        if (kbot) {
             dz = true;
-            fzv = a.fzn[k];
-            fzw = a.fzh[k];
+            fzv = fzk;
+            fzw = fhk;
         } else if (ABC && k <= t && k >= t - a.npzn) {
             dz = true;
             fzv = a.tzn[k - (t - a.npzn)];
             fzw = a.tzh[k - (t - a.npzn)];
         }
-        float X = P[s] + ux, Z = P[n + s] + uz;
-        if (dx) X *= a.fxn[col];
+        float X = q0[j] + ux, Z = q1[j] + uz;
+        if (dx) X *= ro(a.fxn, col);
         if (dz) Z *= fzv;
         P[s] = X;
         P[n + s] = Z;
         a.vx[o] = X + Z;
-        X = P[2 * n + s] + wx;
-        Z = P[3 * n + s] + wz;
-        if (dx) X *= a.fxh[col];
+        X = q2[j] + wx;
+        Z = q3[j] + wz;
+        if (dx) X *= ro(a.fxh, col);
         if (dz) Z *= fzw;
         P[2 * n + s] = X;
         P[3 * n + s] = Z;
@@ -221,33 +293,86 @@ __global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepAr
 
 // Stress half-step (pri:153-196; 'abc': pri:320-352) and the step's receiver
 // samples.
-template <int TX, int NW, bool ABC>
-__global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs a) {
+template <int TX>
+struct ElStrLds {
+    static constexpr int WX = TX + 2 * kElH, HZZ = kElTZ + 2 * kElH;
+    float vx[WX * HZZ], vz[WX * HZZ];
+};
+
+template <int TX, int NW, bool ABC, class A>
+__device__ __forceinline__ void el_stress(const A& a, int tile, int step, int rec_col, ElStrLds<TX>& lds) {
     constexpr int HX = kElH, HZ = kElH;
-    constexpr int WX = TX + 2 * HX, HZZ = kElTZ + 2 * HZ;
-    __shared__ float s_vx[WX * HZZ];
-    __shared__ float s_vz[WX * HZZ];
+    constexpr int HZZ = kElTZ + 2 * HZ;
+    float* s_vx = lds.vx;
+    float* s_vz = lds.vz;
     int z0, x0;
-    el_tile(a, TX, &z0, &x0);
-    if (blockIdx.x == 0 && a.rec_col >= 0) {
+    el_tile(a, TX, tile, &z0, &x0);
+    // The shot's first tile samples its receivers (the velocities are final).
+    if (tile == 0 && rec_col >= 0) {
         for (int r = threadIdx.x; r < a.nrec; r += 64 * NW) {
             const ptrdiff_t o = (ptrdiff_t)a.rx[r] * a.pitch + a.rz[r];
-            a.rec_vx[(size_t)r * a.nrt + a.rec_col] = a.vx[o];
-            a.rec_vz[(size_t)r * a.nrt + a.rec_col] = a.vz[o];
+            a.rec_vx[(size_t)r * a.nrt + rec_col] = a.vx[o];
+            a.rec_vz[(size_t)r * a.nrt + rec_col] = a.vz[o];
         }
     }
-    el_stage<TX, NW, HX, HZ>(s_vx, a.vx, a.pitch, x0, z0);
-    el_stage<TX, NW, HX, HZ>(s_vz, a.vz, a.pitch, x0, z0);
-    __syncthreads();
-    const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    constexpr int NC = TX / NW;  // columns per wave
+    const int tz = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int k = z0 + tz;
-    if (k >= a.nz) return;
+    const bool kin = k < a.nz;
+    const bool kbot = kin && k >= a.kb0;
+    const float fzk = kbot ? a.fzn[k] : 1.f, fhk = kbot ? a.fzh[k] : 1.f;
+    // As in the velocity half-step: the column's surface row (scalar loads)
+    // and every updated cell's lambmusg, lambsg, musge and tau_xx, tau_zz,
+    // tau_xz, or the six split stress parts in a sponge strip, are loaded
+    // before the staging.
+    int tc[NC];
+    bool uc[NC];
+    float lmc[NC], lsc[NC], muc[NC], q[6][NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const int col = x0 + wv + j * NW;
+        const bool cin = col < a.nx;  // wave-uniform
+        const int t = cin ? ro(a.itopo, col) : 0;
+        const int k0 = ABC ? (cin ? ro(a.kst, col) : 0) : t;
+        tc[j] = t;
+        uc[j] = cin && kin && k >= k0;
+        lmc[j] = lsc[j] = muc[j] = 0.f;
+        for (int f = 0; f < 6; ++f) q[f][j] = 0.f;
+        if (uc[j]) {
+            const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
+            const int sj = el_strip<ABC>(a, col, k, t);
+            lmc[j] = a.lm[o];
+            lsc[j] = a.ls[o];
+            muc[j] = a.mu[o];
+            if (sj < 0) {
+                q[0][j] = a.sxx[o];
+                q[1][j] = a.szz[o];
+                q[2][j] = a.sxz[o];
+            } else {
+                const float* P = a.parts;
+                const size_t n = (size_t)a.nstrip;
+#pragma unroll
+                for (int f = 0; f < 6; ++f) q[f][j] = P[(4 + f) * n + sj];
+            }
+        }
+    }
+    {
+        ElStage<TX, NW, HX, HZ> gvx, gvz;
+        gvx.load(a.vx, a.pitch, x0, z0);
+        gvz.load(a.vz, a.pitch, x0, z0);
+        gvx.store(s_vx);
+        gvz.store(s_vz);
+    }
+    __syncthreads();
+    if (!kin) return;
     const bool in_box_z = k >= a.sz0 && k < a.sz1;
-    const float w = a.amp[a.step];
-    for (int j = 0; j < TX / NW; ++j) {
+    const float w = a.amp[step];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
         const int cl = wv + j * NW, col = x0 + cl;
         if (col >= a.nx) break;
-        const int t = a.itopo[col];
+        const int t = tc[j];
         const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
         // addsrc (pub:1429-1466): w g dt / dz^2, into tau_zz always and into
         // tau_xx in 'abc' mode or when the source lies below the surface.
@@ -258,20 +383,20 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
         const int s = el_strip<ABC>(a, col, k, t);
         float* P = a.parts;
         const size_t n = (size_t)a.nstrip;
-        if (k < (ABC ? a.kst[col] : t)) {
+        if (!uc[j]) {
             // Above the updated rows only the source acts: the x parts take it
             // (damped where a strip's rows reach up here); 'fsbc' resets tau_zz
             // on and above the surface to it, 'abc' lets it accumulate.
             if (s >= 0) {
-                const float fx = k >= a.srow[col] ? a.fxh[col] : 1.f;
+                const float fx = k >= ro(a.srow, col) ? ro(a.fxh, col) : 1.f;
                 float X = P[4 * n + s];
-                if (k >= a.srow[col]) X *= fx;
+                if (k >= ro(a.srow, col)) X *= fx;
                 X += srx;
                 P[4 * n + s] = X;
                 a.sxx[o] = X + P[5 * n + s];
                 if (ABC) {
                     X = P[6 * n + s];
-                    if (k >= a.srow[col]) X *= fx;
+                    if (k >= ro(a.srow, col)) X *= fx;
                     X += src;
                     P[6 * n + s] = X;
                     a.szz[o] = X + P[7 * n + s];
@@ -293,39 +418,39 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
         const float dvz_z = el_zsum<0>(a, pvz, d);    // caluz zflag 0 on vz
         const float dvz_x = el_xsum<0>(a, pvz, HZZ);  // calux xflag 0 on vz
         const float dvx_z = el_zsum<1>(a, pvx, d);    // caluz zflag 1 on vx
-        const float lm = a.lm[o], ls = a.ls[o], mu = a.mu[o];
+        const float lm = lmc[j], ls = lsc[j], mu = muc[j];
         const float ax = a.cx * lm * dvx_x, az = a.cz * ls * dvz_z;
         const float bx = a.cx * ls * dvx_x, bz = a.cz * lm * dvz_z;
         const float ex = a.cx * mu * dvz_x, ez = a.cz * mu * dvx_z;
         const bool top = !ABC && k == t;  // 'fsbc': tau_zz reset on the surface row
         float xz;
         if (s < 0) {
-            a.sxx[o] = a.sxx[o] + ax + az + srx;
-            a.szz[o] = top ? src : a.szz[o] + bx + bz + src;
-            xz = a.sxz[o] + ex + ez;
+            a.sxx[o] = q[0][j] + ax + az + srx;
+            a.szz[o] = top ? src : q[1][j] + bx + bz + src;
+            xz = q[2][j] + ex + ez;
         } else {
-            const bool dx = k >= a.srow[col];
+            const bool dx = k >= ro(a.srow, col);
             float fzs = 1.f, fzx = 1.f;  // z factors of the normal stresses (node) and tau_xz (half)
             bool dz = false;
-            if (k >= a.kb0) {
+            if (kbot) {
                 dz = true;
-                fzs = a.fzn[k];
-                fzx = a.fzh[k];
+                fzs = fzk;
+                fzx = fhk;
             } else if (ABC && k <= t && k >= t - a.npzn) {
                 dz = true;
                 fzs = a.tzn[k - (t - a.npzn)];
                 fzx = a.tzh[k - (t - a.npzn)];
             }
-            const float fh = a.fxh[col];
-            float X = P[4 * n + s] + ax, Z = P[5 * n + s] + az;
+            const float fh = ro(a.fxh, col);
+            float X = q[0][j] + ax, Z = q[1][j] + az;
             if (dx) X *= fh;
             if (dz) Z *= fzs;
             X += srx;
             P[4 * n + s] = X;
             P[5 * n + s] = Z;
             a.sxx[o] = X + Z;
-            X = P[6 * n + s] + bx;
-            Z = P[7 * n + s] + bz;
+            X = q[2][j] + bx;
+            Z = q[3][j] + bz;
             if (dx) X *= fh;
             if (dz) Z *= fzs;
             if (top) X = Z = 0.f;
@@ -333,9 +458,9 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
             P[6 * n + s] = X;
             P[7 * n + s] = Z;
             a.szz[o] = X + Z;
-            X = P[8 * n + s] + ex;
-            Z = P[9 * n + s] + ez;
-            if (dx) X *= a.fxn[col];
+            X = q[4][j] + ex;
+            Z = q[5][j] + ez;
+            if (dx) X *= ro(a.fxn, col);
             if (dz) Z *= fzx;
             P[8 * n + s] = X;
             P[9 * n + s] = Z;
@@ -350,18 +475,46 @@ __global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs
         // sum, so the condition is applied to the parts there.
         float gh = 0.f;
         if (t >= 1 && col < a.nx - 1) {
-            const float r = a.ratio[col];
-            if (col >= a.nx - a.nr && a.srow[col] <= t - 1) {
+            const float r = ro(a.ratio, col);
+            if (col >= a.nx - a.nr && ro(a.srow, col) <= t - 1) {
                 const int s1 = el_strip<ABC>(a, col + 1, t, a.itopo[col + 1]);
                 const float gx = P[2 * n + s] + r * (P[s1] - P[s]);
                 const float gz = P[3 * n + s] + r * (P[n + s1] - P[n + s]);
-                gh = gx * a.fxh[col] + gz;
+                gh = gx * ro(a.fxh, col) + gz;
             } else {
                 gh = pvz[0] + r * (pvx[HZZ] - pvx[0]);
             }
         }
         a.ghost[col] = gh;
     }
+}
+
+template <int TX, int NW, bool ABC>
+__global__ __launch_bounds__(64 * NW) void elastic_velocity_kernel(ElasticStepArgs a) {
+    __shared__ ElVelLds<TX> lds;
+    el_velocity<TX, NW, ABC>(a, xcd_tile(blockIdx.x, a.ntiles), lds);
+}
+
+template <int TX, int NW, bool ABC>
+__global__ __launch_bounds__(64 * NW) void elastic_stress_kernel(ElasticStepArgs a) {
+    __shared__ ElStrLds<TX> lds;
+    el_stress<TX, NW, ABC>(a, xcd_tile(blockIdx.x, a.ntiles), a.step, a.rec_col, lds);
+}
+
+template <int TX, int NW, bool ABC>
+__global__ __launch_bounds__(64 * NW) void elastic_velocity_batch(ElasticBatchArgs b) {
+    __shared__ ElVelLds<TX> lds;
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile(blockIdx.x, b.ntile);
+    ElasticShotArgs& a = *(ElasticShotArgs*)(b.shots + ro(m, 0));
+    el_velocity<TX, NW, ABC>(a, ro(m, 1), lds);
+}
+
+template <int TX, int NW, bool ABC>
+__global__ __launch_bounds__(64 * NW) void elastic_stress_batch(ElasticBatchArgs b) {
+    __shared__ ElStrLds<TX> lds;
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile(blockIdx.x, b.ntile);
+    ElasticShotArgs& a = *(ElasticShotArgs*)(b.shots + ro(m, 0));
+    el_stress<TX, NW, ABC>(a, ro(m, 1), b.step, b.rec_col, lds);
 }
 
 }  // namespace sfd

@@ -225,9 +225,12 @@ def fwi_801(nshot: int = 16, nt: int = 2500, scale: int = 1, seed: int = 6):
     return true, bg, shots
 
 
-def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
+def elastic_par_903x603(nt: int = 3001, topo=None, shot: int = 1, **overrides) -> ElasticShot:
     """``topo`` (optional, [501] metres, >= 0) replaces the flat topography:
-    itopoe = nint(topo / dz) + npzn (get_currshot_parameters, pub:269-272)."""
+    itopoe = nint(topo / dz) + npzn (get_currshot_parameters, pub:269-272).
+    ``shot`` is the card's 1-based shot number csno: the source moves by dsx =
+    100 m per shot, isx = isx0 + nint((csno - 1) dsx / dx), and the window
+    follows it, ixmine = isx + nint(ofsmin / dx) (spcalpa, pri:36-45)."""
     nxv, nzv, dx = 501, 401, 5.0
     npxn = npxp = npzp = 201
     npzn = 1
@@ -239,8 +242,10 @@ def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
     model = {"vs": vsm, "vp": 2.0 * vsm, "rho": np.full((nxv, nzv), 2250.0, np.float32)}
     tp = np.zeros(nxv, np.float32) if topo is None else np.asarray(topo, np.float32)
     # get_currshot_parameters: window columns i = 1..nxe take model column
-    # i + ixmine (clipped), ixmine = 1 for this shot; rows pad by repetition.
-    ixmine = 1
+    # i + ixmine (clipped); rows pad by repetition.  sx0 = 1250 m, ofsmin =
+    # -1250 m, dsx = 100 m.
+    isx = 251 + int(np.floor((shot - 1) * 100.0 / dx + 0.5))
+    ixmine = isx - 250
     cols = np.clip(np.arange(1, nxe + 1) + ixmine, 1, nxv) - 1
     grids = {}
     for k, m in model.items():
@@ -256,7 +261,7 @@ def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
     topoe[:npxn] = topoe[npxn]
     topoe[npxn + nxe:] = topoe[npxn + nxe - 1]
     itopo = (np.floor(topoe / dx + 0.5).astype(np.int32) + npzn) - 1  # 0-based
-    isxe = 251 - ixmine + npxn + 1                  # spcalpa + get_currshot_parameters (1-based)
+    isxe = isx - ixmine + npxn + 1                  # spcalpa + get_currshot_parameters (1-based)
     sx = isxe - 1
     sz = 1 + (itopo[sx] + 1) - 1                   # isze = isz + itopoe(isxe), isz = 1
     rx = np.arange(npxn, npxn + nxe, dtype=np.int32)
@@ -265,6 +270,12 @@ def elastic_par_903x603(nt: int = 3001, topo=None, **overrides) -> ElasticShot:
               dx=dx, dz=dx, dt=1e-3, f0=25.0, sponge=(npxn, npxp, npzp), src_radius=50, delay=48)
     kw.update(overrides)
     return ElasticShot(**kw)
+
+
+def elastic_par_survey(nshot: int = 16, nt: int = 3001, first: int = 1, **overrides):
+    """Shots first .. first + nshot - 1 of the elastic card's survey on the
+    903 x 603 shot grid (``elastic_par_903x603``), for a batch."""
+    return [elastic_par_903x603(nt=nt, shot=first + i, **overrides) for i in range(nshot)]
 
 
 def vti_receiver_row(nrz0: int, nu: int):

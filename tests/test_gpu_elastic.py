@@ -15,6 +15,10 @@ a surface on the top row, a fluid layer (vs = 0), receivers on and above the
 surface, grids that are no multiple of the tiles, the reference program's
 own 903 x 603 shot grid, and the 'abc' mode (absorbing top following the
 topography, smoothed model above it) on hills and on that grid.
+Batches (sfd_elastic_batch_*): shots of different grids, topographies,
+sources and receivers stepped by one launch per half-step give each shot's
+gathers and snapshots bitwise equal to its own plan's, in both modes and at
+every tiling.
 """
 import numpy as np
 import pytest
@@ -201,3 +205,65 @@ def test_invalid_arguments(ctx):
         sfd.elastic_forward(nz0, ctx)
     assert e.value.code == -1
     del shot
+
+
+def batch_shots(abc):
+    """Four shots of one mode and time axis that differ in grid, topography,
+    fluid rows, source and receivers."""
+    specs = [dict(nx=133, nz=97), dict(nx=77, nz=70, sponge=(10, 12, 9), topo="flat0"),
+             dict(nx=150, nz=88, topo=6, fluid_rows=4), dict(nx=121, nz=110, sponge=(16, 19, 14))]
+    shots = []
+    for i, sp in enumerate(specs):
+        sh = small_shot(nt=260, delay=40, snap_steps=(100, 299), snap_fields=ALL, **sp)
+        nx, nz = sh.vp.shape
+        t = sh.itopo.astype(np.int32)
+        if abc:
+            t = (t + 12).astype(np.int32)
+            sh.itopo = t
+            sh.mode = sfd.ELASTIC_ABSORBING_TOP
+            sh.sponge_zlo = 9
+            sh.rec = (sh.rec[0], np.minimum(t[sh.rec[0]] + 1, nz - sh.sponge[2] - 2).astype(np.int32))
+        sx = nx // 2 - 5 * i
+        sh.src = (sx, int(t[sx]) + (0 if i == 2 and not abc else 3))  # shot 2: source on the surface
+        shots.append(sh)
+    return shots
+
+
+@pytest.mark.parametrize("mode", ["fsbc", "abc"])
+def test_batch_equals_single_plans(ctx, mode):
+    shots = batch_shots(mode == "abc")
+    print(f"case: elastic batch {mode}, grids {[s.vp.shape for s in shots]}", flush=True)
+    singles = []
+    for sh in shots:
+        with sfd.ElasticPlan(sh, ctx) as pl:
+            pl.run()
+            singles.append(pl.fetch())
+    assert all(np.abs(r[0]).max() > 0 for r in singles)
+    with sfd.ElasticBatch(shots, ctx) as b:
+        for tiling in ((0, 0), (16, 8), (32, 16), (8, 4)):
+            b.set_tiling(*tiling)
+            b.run()
+            for i in range(len(shots)):
+                got = b.fetch(i)
+                for g, w, what in zip(got, singles[i], ("vx", "vz", "snaps")):
+                    np.testing.assert_array_equal(g, w, err_msg=f"{mode} tiling {tiling} shot {i} {what}")
+        us = b.autotune(20)
+        assert us > 0 and b.shape() in {(8, 4), (8, 8), (16, 4), (16, 8), (16, 16), (32, 4), (32, 8), (32, 16)}
+        b.run()
+        for i in range(len(shots)):
+            np.testing.assert_array_equal(b.fetch(i)[0], singles[i][0])
+
+
+def test_batch_rejects_mixed_shots(ctx):
+    fs = batch_shots(False)
+    ab = batch_shots(True)
+    with pytest.raises(sfd.SeisfdError, match="share the mode") as e:
+        sfd.ElasticBatch([fs[0], ab[1]], ctx)
+    assert e.value.code == sfd.SFD_EINVAL
+    other = small_shot(nt=200, delay=40)
+    with pytest.raises(sfd.SeisfdError, match="share the mode"):
+        sfd.ElasticBatch([fs[0], other], ctx)
+    with sfd.ElasticBatch(fs[:2], ctx) as b:
+        with pytest.raises(sfd.SeisfdError):
+            lib_rc = sfd.lib().sfd_elastic_batch_fetch(b.handle, 2, None, None, None)
+            b.ctx.check(lib_rc, "fetch")

@@ -154,7 +154,7 @@ def test_elastic_autotune_keeps_results():
         us = pl.autotune(10)
         tx, nw = pl.shape()
         print(f"tuned: {tx} x {nw} {us:.2f} us/step", flush=True)
-        assert (tx, nw) in TILED | {(8, 4), (8, 8)} and ctx.tuning_size() == 1
+        assert (tx, nw) in TILED | {(8, 4), (8, 8), (32, 4)} and ctx.tuning_size() == 1
         pl.run()
         r1 = pl.fetch()
         for a, b in zip(r0, r1):
