@@ -107,15 +107,44 @@ def c1_prefix_error(shot, steps):
                             f"the first {steps} of {len(shot.wavelet)} steps"}
 
 
+STREAM_MIN_ROWS = 64  # one-launch shapes of this many rows or more are the streaming step
+
+
 def shape_dict(kind, a, b, us):
     if kind == 0:
         d = {"kernel": "tiled two-launch step", "tile_rows": a, "waves": b}
+    elif kind == 2 and a >= STREAM_MIN_ROWS:
+        d = {"kernel": "streaming one-launch step", "chunk_rows": a, "waves": b}
     elif kind == 2:
         d = {"kernel": "one-launch tiled step", "tile_rows": a, "waves": b}
     else:
         d = {"kernel": "fused march kernel", "waves": a, "chunk_rows": b}
     d["autotuned_us_per_step"] = None if us is None else round(us, 2)
     return d
+
+
+def kernel_stem(kind, a, batch):
+    """Name stem of the step kernel(s) a launch shape runs."""
+    if kind == 0:
+        stem = "acoustic_tiled_velocity"
+    elif kind == 2:
+        stem = "acoustic_stream_step" if a >= STREAM_MIN_ROWS else "acoustic_fused_step"
+    else:
+        return "acoustic_batch_step_kernel" if batch else "acoustic_step_kernel"
+    return stem + ("_batch" if batch else "")
+
+
+def load_traffic(name, key, stem):
+    """A counter profile's HBM bytes, only when the profiled kernels are the
+    ones the chosen launch shape runs (else None: the profile is stale)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if stem and not any(stem + "<" in k for k in t.get("kernels", [])):
+        return None
+    return t.get(key)
 
 
 def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
@@ -133,6 +162,16 @@ def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
     b = sfd.AcousticBatch(shots)
     tuned_us = None if no_tune else b.autotune(20)
     kind, ta, tb = b.shape()
+    if plain and not no_tune:
+        # The plain path runs every shot at its own grid class's shape: tune
+        # each class once on one of its shots.
+        seen = set()
+        for s in shots:
+            cls = tuple(int(round(np.log2(n))) for n in s.vp.shape)
+            if cls not in seen:
+                seen.add(cls)
+                with sfd.AcousticPlan(s) as pl:
+                    pl.autotune(20)
     b.run(sfd.BATCH_GRAPH)
     g_ms = min(b.run(sfd.BATCH_GRAPH) for _ in range(2))
     order = np.argsort([s.vp.shape[0] for s in shots])
@@ -153,10 +192,7 @@ def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
            # uniform density: 28 algorithmic bytes per cell-update + the C-PML strips
            "graph_frac_of_8TBs": round(bytes_step * nt / (g_ms / 1e3) / 8e12, 4),
            "launch_shape": shape_dict(kind, ta, tb, tuned_us)}
-    tpath = os.path.join(ROOT, "profiles", "acoustic_batch_c3_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            out["traffic"] = json.load(f).get("hbm_bytes_per_step")
+    out["traffic"] = load_traffic("acoustic_batch_c3_traffic.json", "hbm_bytes_per_step", kernel_stem(kind, ta, True))
     if not no_cpu:
         import oracle
         cores = min(16, len(os.sched_getaffinity(0)))
@@ -757,11 +793,8 @@ def main():
     avg_launch_s = (sum(dev_ms) / len(dev_ms)) / 1e3 / nt
     bytes_step = algorithmic_bytes_per_step(shot)
     achieved = bytes_step / avg_launch_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "acoustic_step_cpml_2101x4101_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = load_traffic("acoustic_step_cpml_2101x4101_traffic.json", "hbm_bytes_per_launch",
+                           kernel_stem(kind, ta, False))
 
     out = {
         "metric": METRIC,
@@ -787,7 +820,7 @@ def main():
                      "ceiling_frac": None if ceiling is None else round(achieved / ceiling["copy_GBs"], 4),
                      "unit_of_measure": {0: "one time step = velocity launch + pressure launch",
                                          1: "one time step = one launch of the fused march kernel",
-                                         2: "one time step = one launch of the one-launch tiled step"}[kind],
+                                         2: "one time step = one launch of the one-launch step"}[kind],
                      "bytes_per_step": bytes_step, "avg_step_us": round(avg_launch_s * 1e6, 2)},
         "hbm_ceiling": ceiling,
         "launch_shape": shape_dict(kind, ta, tb, tuned_us),

@@ -171,7 +171,10 @@ enum sfd_batch_run {
                                 captured once in a hipGraph and replayed */
     SFD_BATCH_LAUNCHES = 1,  /* one launch per step for all shots, no graph */
     SFD_BATCH_PER_SHOT = 2   /* the plain path: shot after shot, one launch per
-                                shot and step, no graph */
+                                shot and step, no graph, each shot at the
+                                shape a plan of its own starts with (the
+                                context's cached shape for its grid class, else
+                                the library's default), not the batch's */
 };
 
 /* params[s], vp[s], den[s] (den may be NULL, or den[s] NULL), wavelet[s] as
@@ -229,11 +232,14 @@ int sfd_context_load_tuning(sfd_context *ctx, const char *path);
 int sfd_context_tuning_size(const sfd_context *ctx);
 
 /* ---------------------------------------------------------------- gradient
- * Velocity gradient by the discrete adjoint of the fast-arithmetic sponge
- * step (no reference counterpart: the reference models data only).  The
- * wavefield is checkpointed every ckpt_every steps and recomputed one
- * segment at a time during the reverse sweep, so device memory grows with
- * nt / ckpt_every rather than nt.
+ * Velocity gradient by the discrete adjoint of the fast-arithmetic sponge or
+ * C-PML step (no reference counterpart: the reference models data only).
+ * The wavefield (with C-PML, its memory fields too) is checkpointed every
+ * ckpt_every steps and recomputed one segment at a time during the reverse
+ * sweep, so device memory grows with nt / ckpt_every rather than nt.  The
+ * C-PML profiles depend on max(vp); the gradient holds them fixed (the
+ * derivative through the profiles' reference velocity is not included).
+ * Every shot of one call needs the same boundary kind.
  *
  * sfd_acoustic_gradient: J = sum_{r,k} seed[r][k] record[r][k] (seed is
  *   [nrec][nt], the vector of a vector-Jacobian product); grad_vp [nx*nz]
@@ -257,11 +263,12 @@ enum sfd_gradient_method {
        divergence fields (what sfd_acoustic_misfit_gradient does) */
     SFD_GRADIENT_RECOMPUTE = 0,
     /* boundary saving: the forward keeps the R-deep band just outside the box
-       [max(w, R), n - max(w, R)) of both axes (w = sponge width) after every
-       step; the reverse sweep runs the step backwards inside the box from the
-       band and the checkpoints (reset at every segment end), with one
-       divergence field per shot.  The sponge cannot be run backwards, so
-       grad_vp is zero outside the box (the source cell's term aside). */
+       [max(w, R), n - max(w, R)) of both axes (w = sponge width; the C-PML
+       width + 1 with C-PML) after every step; the reverse sweep runs the step
+       backwards inside the box from the band and the checkpoints (reset at
+       every segment end), with one divergence field per shot.  The absorbing
+       layers cannot be run backwards, so grad_vp is zero outside the box
+       (the source cell's term aside). */
     SFD_GRADIENT_BOUNDARY_SAVING = 1,
 };
 int sfd_acoustic_misfit_gradient_ex(sfd_context *ctx, int nshot, const sfd_acoustic_params *params,

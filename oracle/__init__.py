@@ -175,9 +175,11 @@ def acoustic(vp, den, wavelet, src, rec, *, dx=10.0, dz=10.0, dt=5e-4, radius=8,
 
 
 def gradient(vp, den, wavelet, src, rec, seed, *, dx=10.0, dz=10.0, dt=5e-4, radius=8, sponge=20,
-             coef=None):
+             coef=None, boundary="sponge", pml_r=1e-3, pml_vmax=None, pml_f0=None):
     """FP64 discrete adjoint (oracle/adjoint.c): dJ/dvp for
-    J = sum(seed * record); returns (grad [nx, nz], record [nrec, nt])."""
+    J = sum(seed * record); returns (grad [nx, nz], record [nrec, nt]).
+    boundary "cpml": the C-PML step of width `sponge`, its profiles fixed by
+    pml_vmax (default: the model's maximum) and held constant."""
     vp = np.ascontiguousarray(vp, dtype=np.float32)
     den = np.ascontiguousarray(den, dtype=np.float32)
     nx, nz = vp.shape
@@ -188,9 +190,12 @@ def gradient(vp, den, wavelet, src, rec, seed, *, dx=10.0, dz=10.0, dt=5e-4, rad
     rz = np.ascontiguousarray(rec[1], dtype=np.int32)
     seed = np.ascontiguousarray(seed, dtype=np.float64)
     assert seed.shape == (rx.size, nt)
+    bflag = {"sponge": 0, "cpml": 1}[boundary]
     args = _Args(nx, nz, nt, 0, dx, dz, dt, radius, _ptr(coef), sponge, _ptr(None), _ptr(vp), _ptr(den),
                  _ptr(wavelet), int(src[0]), int(src[1]), rx.size, _ptr(rx, ctypes.c_int), _ptr(rz, ctypes.c_int),
-                 0, _ptr(None, ctypes.c_int), 1, 0, 0.0, 0.0, 0.0)
+                 0, _ptr(None, ctypes.c_int), 1, bflag, pml_r if bflag else 0.0,
+                 float(pml_vmax if pml_vmax is not None else vp.max()) if bflag else 0.0,
+                 float(pml_f0 or 0.0) if bflag else 0.0)
     grad = np.zeros((nx, nz), dtype=np.float64)
     record = np.zeros((rx.size, nt), dtype=np.float64)
     rc = lib().oracle_acoustic_gradient_f64(ctypes.byref(args), _ptr(seed, ctypes.c_double),

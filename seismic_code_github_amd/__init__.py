@@ -521,8 +521,9 @@ def acoustic_forward(shot: AcousticShot, ctx: Optional[Context] = None):
 
 def acoustic_gradient(shot: AcousticShot, seed, ckpt_every: int = 50, ctx: Optional[Context] = None):
     """dJ/dvp for J = sum(seed * record) through ``sfd_acoustic_gradient``
-    (discrete adjoint of the fast sponge step, checkpoints every
-    ``ckpt_every`` steps).  Returns (grad [nx, nz], record [nrec, nt])."""
+    (discrete adjoint of the fast sponge or C-PML step, checkpoints every
+    ``ckpt_every`` steps; C-PML profiles held fixed).  Returns (grad [nx, nz],
+    record [nrec, nt])."""
     ctx = ctx or default_context()
     pkeep = []
     p = shot.params(pkeep)
@@ -549,7 +550,8 @@ def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: 
     its dJ/dvp, through ``sfd_acoustic_misfit_gradient_ex``.  The model is
     ``vp``/``den`` (default: the first shot's).  method "recompute"
     (checkpoints + segment recomputation) or "boundary" (checkpoints +
-    boundary saving; the gradient inside the sponge-free box only).
+    boundary saving; the gradient inside the box the sponge or C-PML leaves
+    alone only).  Every shot needs the same boundary kind.
     Returns (J, grad [nx, nz])."""
     ctx = ctx or default_context()
     shots = list(shots)

@@ -7,6 +7,7 @@
 #include <new>
 #include <tuple>
 
+#include "acoustic_adjoint_cpml.h"
 #include "acoustic_plan.h"
 #include "acoustic_tiled.h"
 
@@ -213,13 +214,22 @@ bool tiled_mode(int mode, StepKernel* v, StepKernel* p) {
 // at the default shape.
 template <int R>
 bool pick_tiled_r(int tx, int nw, int mode, StepKernel* v, StepKernel* p) {
-    if (tx == kTiledTX && nw == kTiledNW && (mode == MODE_FWD_DIV || mode == MODE_ADJ)) {
+    if (tx == kTiledTX && nw == kTiledNW &&
+        (mode == MODE_FWD_DIV || mode == MODE_ADJ || mode == MODE_FWD_DIV_CPML || mode == MODE_ADJ_CPML)) {
         if (mode == MODE_FWD_DIV) {
             *v = acoustic_tiled_velocity<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
             *p = acoustic_tiled_pressure<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
-        } else {
+        } else if (mode == MODE_ADJ) {
             *v = acoustic_tiled_velocity<R, kTiledTX, kTiledNW, MODE_ADJ>;
             *p = acoustic_tiled_pressure<R, kTiledTX, kTiledNW, MODE_ADJ>;
+        } else if (mode == MODE_FWD_DIV_CPML) {
+            *v = acoustic_tiled_velocity<R, kTiledTX, kTiledNW, MODE_FWD_DIV_CPML>;
+            *p = acoustic_tiled_pressure<R, kTiledTX, kTiledNW, MODE_FWD_DIV_CPML>;
+        } else {
+            // The C-PML adjoint's two launches (adjoint of the pressure half,
+            // then of the velocity half) take the velocity / pressure slots.
+            *v = acoustic_adjc_a1<R, kTiledTX, kTiledNW>;
+            *p = acoustic_adjc_a2<R, kTiledTX, kTiledNW>;
         }
         return true;
     }
@@ -268,10 +278,18 @@ bool tiled_batch_mode(int mode, BatchKernel* v, BatchKernel* p) {
 // at the default shape.
 template <int R>
 bool pick_tiled_batch_r(int tx, int nw, int mode, BatchKernel* v, BatchKernel* p) {
-    if (tx == kTiledTX && nw == kTiledNW && (mode == MODE_FWD_DIV || mode == MODE_ADJ || mode == MODE_REV)) {
+    if (tx == kTiledTX && nw == kTiledNW &&
+        (mode == MODE_FWD_DIV || mode == MODE_ADJ || mode == MODE_REV || mode == MODE_FWD_DIV_CPML ||
+         mode == MODE_ADJ_CPML)) {
         if (mode == MODE_FWD_DIV) {
             *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
             *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV>;
+        } else if (mode == MODE_FWD_DIV_CPML) {
+            *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV_CPML>;
+            *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_FWD_DIV_CPML>;
+        } else if (mode == MODE_ADJ_CPML) {
+            *v = acoustic_adjc_a1_batch<R, kTiledTX, kTiledNW>;
+            *p = acoustic_adjc_a2_batch<R, kTiledTX, kTiledNW>;
         } else if (mode == MODE_ADJ) {
             *v = acoustic_tiled_velocity_batch<R, kTiledTX, kTiledNW, MODE_ADJ>;
             *p = acoustic_tiled_pressure_batch<R, kTiledTX, kTiledNW, MODE_ADJ>;
@@ -416,14 +434,17 @@ AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* P, const Fi
     a.div_period = pl->div_fields > 0 ? pl->div_fields : 1;
     a.grad = pl->grad_period > 0 ? pl->grid(pl->grad) : nullptr;
     a.src_adj = pl->src_adj.as<float>();
-    // Box of the reverse reconstruction: the cells the sponge leaves alone
-    // and the updated region contains.
-    const int bx = std::max(pl->width, pl->radius);
+    // Box of the reverse reconstruction.
+    const int bx = rev_margin(pl);
     a.rev_x0 = bx;
     a.rev_x1 = pl->nx - bx;
     a.rev_z0 = bx;
     a.rev_z1 = pl->nz - bx;
     return a;
+}
+
+int rev_margin(const sfd_acoustic_plan* pl) {
+    return std::max(pl->mode == MODE_FAST_CPML ? pl->width + 1 : pl->width, pl->radius);
 }
 
 AcousticStepArgs make_args(const sfd_acoustic_plan* pl, int cur, int k) {
@@ -454,6 +475,39 @@ std::string plan_class(const sfd_acoustic_plan* pl) {
 int apply_shape(sfd_acoustic_plan* pl, const TunedShape& t) {
     if (t.kind == 2) return sfd_acoustic_plan_set_fused(pl, t.a, t.b);
     return t.kind == 0 ? sfd_acoustic_plan_set_tiled(pl, t.a, t.b) : sfd_acoustic_plan_set_tiling(pl, t.a, t.b);
+}
+
+// A plan's launch-shape fields, saved and put back around a run at another
+// shape.
+struct PlanShape {
+    int nw, chunk, ttx, tnw, ftx, fnw;
+    StepKernel kernel, tvel, tpres, fstep;
+};
+
+PlanShape save_shape(const sfd_acoustic_plan* pl) {
+    return {pl->nw, pl->chunk, pl->ttx, pl->tnw, pl->ftx, pl->fnw, pl->kernel, pl->tvel, pl->tpres, pl->fstep};
+}
+
+void restore_shape(sfd_acoustic_plan* pl, const PlanShape& s) {
+    pl->nw = s.nw;
+    pl->chunk = s.chunk;
+    pl->ttx = s.ttx;
+    pl->tnw = s.tnw;
+    pl->ftx = s.ftx;
+    pl->fnw = s.fnw;
+    pl->kernel = s.kernel;
+    pl->tvel = s.tvel;
+    pl->tpres = s.tpres;
+    pl->fstep = s.fstep;
+}
+
+// The shape a plan created on its own starts with: the context's cached shape
+// for its grid class, else the tiled step's default (fast arithmetic) or the
+// march kernel's automatic tiling.
+int apply_plain_shape(sfd_acoustic_plan* pl) {
+    const auto tuned = pl->ctx->tuning.find(plan_class(pl));
+    if (tuned != pl->ctx->tuning.end() && apply_shape(pl, tuned->second) == SFD_OK) return SFD_OK;
+    return sfd_acoustic_plan_set_tiling(pl, 0, 0);
 }
 
 float sponge_at(const sfd_acoustic_plan* pl, int i, int j) {
@@ -1248,8 +1302,15 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch* b, int how, float* elapsed_ms) {
         const int rc = enqueue_batch_steps(b, b->nt);
         if (rc) return rc;
     } else {
+        // The plain path: every shot at the shape a plan of its own starts
+        // with (the context's cached shape for its grid class, else the
+        // tiled default), not the batch's shape; the batch's shape is put
+        // back afterwards (host fields only, so nothing is timed for it).
         for (auto* pl : b->plans) {
-            const int rc = enqueue_steps(pl, 0, pl->nt);
+            const PlanShape keep = save_shape(pl);
+            int rc = apply_plain_shape(pl);
+            if (!rc) rc = enqueue_steps(pl, 0, pl->nt);
+            restore_shape(pl, keep);
             if (rc) return rc;
         }
     }

@@ -57,6 +57,11 @@ enum KernelMode : int {
     // then the velocity launch (V_k = V_k+1 + bf D(P_k - source)).  Cells
     // outside the box are neither read as outputs nor written.
     MODE_REV = 6,
+    // Tiled step only.  The C-PML step that also stores div = t_x + Wx' + t_z
+    // + Wz' (the divergences with their memory terms) of every cell.
+    MODE_FWD_DIV_CPML = 7,
+    // Tiled step only.  Adjoint of the C-PML step (acoustic_adjoint_cpml.h).
+    MODE_ADJ_CPML = 8,
 };
 
 struct AcousticStepArgs {

@@ -121,6 +121,11 @@ int launch_tiled(const sfd_acoustic_plan* pl, StepKernel v, StepKernel p, const 
 // Tiled step shape used unless one is asked for.
 constexpr int kTiledTX = 16, kTiledNW = 4;
 int zero_state(sfd_acoustic_plan* pl);
+// Margin of the box the gradient's reverse reconstruction covers, [m, n - m)
+// on both axes: the cells no absorbing term touches (the sponge width; one
+// more for C-PML, whose half-node profiles reach a row further), at least
+// the stencil radius.
+int rev_margin(const sfd_acoustic_plan* pl);
 // SFD_ENAN when the plan's gather or current pressure field is not finite.
 int check_plan_finite(sfd_acoustic_plan* pl);
 // Sponge factor of cell (i, j) as the step kernel applies it.

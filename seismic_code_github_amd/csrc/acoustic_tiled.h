@@ -5,7 +5,8 @@
 //   acoustic_tiled_velocity  Vx, Vz from P (source injected)      (:247-286)
 //   acoustic_tiled_pressure  P from the new Vx, Vz; receivers     (:289-314, :376-377)
 // MODE is one of MODE_FAST_CPML, MODE_FAST_SPONGE (sponge of :329-340),
-// MODE_FWD_DIV and MODE_ADJ (the gradient's recomputation and adjoint steps).
+// MODE_FWD_DIV, MODE_FWD_DIV_CPML and MODE_ADJ (the gradient's recomputation
+// and adjoint steps; the C-PML adjoint is acoustic_adjoint_cpml.h).
 // A workgroup of NW waves owns a tile of TX rows x 64 columns of the updated
 // region [R, nx - R) x [R, nz - R), one column per lane, TX / NW rows per
 // wave.  The tile and the R-cell halo its stencils read are staged in LDS
@@ -81,7 +82,8 @@ __device__ __forceinline__ float tiled_sponge(int i, int nx, int sw, float ax_i,
 // k the step.
 template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, float* s_p) {
-    constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ, REV = MODE == MODE_REV;
+    constexpr bool CPML = MODE == MODE_FAST_CPML || MODE == MODE_FWD_DIV_CPML, ADJ = MODE == MODE_ADJ;
+    constexpr bool REV = MODE == MODE_REV;
     constexpr int W = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
@@ -181,7 +183,8 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
 
 template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, float* s_vx, float* s_vz) {
-    constexpr bool CPML = MODE == MODE_FAST_CPML, ADJ = MODE == MODE_ADJ, DIVOUT = MODE == MODE_FWD_DIV;
+    constexpr bool CPML = MODE == MODE_FAST_CPML || MODE == MODE_FWD_DIV_CPML, ADJ = MODE == MODE_ADJ;
+    constexpr bool DIVOUT = MODE == MODE_FWD_DIV || MODE == MODE_FWD_DIV_CPML;
     constexpr bool REV = MODE == MODE_REV;
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;

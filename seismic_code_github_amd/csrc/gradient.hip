@@ -56,13 +56,80 @@ struct GradShot {
 };
 
 int check_gradient_params(sfd_context* ctx, const sfd_acoustic_params* p) {
-    if (p->boundary != SFD_BOUNDARY_SPONGE || p->arith != SFD_ARITH_FAST)
-        return fail(ctx, SFD_ENOTSUP, "the gradient exists for the sponge boundary in fast arithmetic");
+    if ((p->boundary != SFD_BOUNDARY_SPONGE && p->boundary != SFD_BOUNDARY_CPML) || p->arith != SFD_ARITH_FAST)
+        return fail(ctx, SFD_ENOTSUP, "the gradient exists for the sponge and C-PML boundaries in fast arithmetic");
     if (p->nt < 1) return fail(ctx, SFD_EINVAL, "nt < 1");
     const int R = p->radius;
     if (p->sx < R || p->sx >= p->nx - R || p->sz < R || p->sz >= p->nz - R)
         return fail(ctx, SFD_EINVAL, "the gradient needs the source inside the updated cells");
     return SFD_OK;
+}
+
+bool is_cpml(const sfd_acoustic_plan* pl) { return pl->mode == MODE_FAST_CPML; }
+
+// The C-PML memory fields of a plan's state (buffer 0): x strips, z strips.
+int psi_fields(sfd_acoustic_plan* pl, Field* f[4], int buf = 0) {
+    if (!is_cpml(pl)) return 0;
+    f[0] = &pl->psi_px[buf];
+    f[1] = &pl->psi_vx[buf];
+    f[2] = &pl->psi_pz[buf];
+    f[3] = &pl->psi_vz[buf];
+    return 4;
+}
+
+// Floats of one checkpoint: P, Vx, Vz and, with C-PML, the memory fields.
+size_t ckpt_floats(sfd_acoustic_plan* pl) {
+    size_t n = 3 * (size_t)pl->nx * pl->nz;
+    Field* f[4];
+    for (int q = 0, m = psi_fields(pl, f); q < m; ++q) n += f[q]->bytes / 4;
+    return n;
+}
+
+// Copies the state (buffer 0) to or from checkpoint `seg` (1-based).
+int ckpt_copy(sfd_acoustic_plan* pl, int seg, bool save) {
+    sfd_context* ctx = pl->ctx;
+    const size_t n = (size_t)pl->nx * pl->nz;
+    float* ck = pl->ckpt.as<float>() + (size_t)(seg - 1) * ckpt_floats(pl);
+    Field* f[3] = {&pl->p[0], &pl->vx[0], &pl->vz[0]};
+    for (int q = 0; q < 3; ++q) {
+        float* dst = save ? ck + q * n : pl->grid(*f[q]);
+        const float* src = save ? pl->grid(*f[q]) : ck + q * n;
+        SFD_HIP(ctx, hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ck += 3 * n;
+    Field* m[4];
+    for (int q = 0, c = psi_fields(pl, m); q < c; ++q) {
+        if (save)
+            SFD_HIP(ctx, hipMemcpyAsync(ck, m[q]->ptr, m[q]->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        else
+            SFD_HIP(ctx, hipMemcpyAsync(m[q]->ptr, ck, m[q]->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        ck += m[q]->bytes / 4;
+    }
+    return SFD_OK;
+}
+
+// Zeroes the adjoint state: P, Vx, Vz and (C-PML) the adjoint strip fields,
+// which live in the second copies of the plan's strip fields.
+int zero_adjoint(sfd_acoustic_plan* pl) {
+    sfd_context* ctx = pl->ctx;
+    hipStream_t s = ctx->stream;
+    for (Field* f : {&pl->ap[0], &pl->avx[0], &pl->avz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+    Field* m[4];
+    for (int q = 0, c = psi_fields(pl, m, 1); q < c; ++q) SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
+    return SFD_OK;
+}
+
+// Arguments of an adjoint step (in place on the adjoint fields).
+AcousticStepArgs adjoint_args(const sfd_acoustic_plan* pl, int k) {
+    AcousticStepArgs a = make_args(pl, pl->ap, pl->avx, pl->avz, 0, k);
+    tiled_args(pl, a, true);
+    if (pl->mode == MODE_FAST_CPML) {
+        a.psi_px_in = a.psi_px_out = pl->psi_px[1].as<float>();
+        a.psi_vx_in = a.psi_vx_out = pl->psi_vx[1].as<float>();
+        a.psi_pz_in = a.psi_pz_out = pl->psi_pz[1].as<float>();
+        a.psi_vz_in = a.psi_vz_out = pl->psi_vz[1].as<float>();
+    }
+    return a;
 }
 
 // Creates the plan with its gradient workspace and the receiver-cell tables.
@@ -100,21 +167,23 @@ int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, 
             const bool upd = i >= p->radius && i < pl->nx - p->radius && j >= p->radius && j < pl->nz - p->radius;
             const float d = den ? den[c] : 2400.0f;
             cell[slot] = (int)((long)(i + pl->pad) * pl->nz + j);
-            w[slot] = upd ? sponge_at(pl, i, j) * (d * vp[c] * vp[c] * p->dt) : 0.f;
+            // The sponge adjoint runs on K-scaled fields (K = rho vp^2 dt,
+            // times the cell's sponge factor); the C-PML adjoint unscaled.
+            if (!upd) w[slot] = 0.f;
+            else w[slot] = is_cpml(pl) ? 1.f : sponge_at(pl, i, j) * (d * vp[c] * vp[c] * p->dt);
         }
     }
     g->slot_cell = slot_of;
     hipStream_t s = ctx->stream;
-    const size_t n = (size_t)pl->nx * pl->nz;
     hipError_t e = upload(pl->inj_cell, cell.data(), cell.size(), s);
     if (e == hipSuccess) e = upload(pl->inj_w, w.data(), w.size(), s);
     if (e == hipSuccess) e = pl->res.alloc((size_t)std::max(pl->nslot, 1) * pl->nt * 4);
     if (e == hipSuccess) e = pl->src_adj.alloc((size_t)pl->nt * 4);
-    if (e == hipSuccess) e = pl->ckpt.alloc((size_t)std::max(g->nseg - 1, 1) * 3 * n * 4);
+    if (e == hipSuccess) e = pl->ckpt.alloc((size_t)std::max(g->nseg - 1, 1) * ckpt_floats(pl) * 4);
     if (e == hipSuccess && bsave) {
         // Band cells: within R of the box along x (same columns) or along z
         // (same rows), the reach of the stencils of the box's cells.
-        const int b = std::max(pl->width, pl->radius), R = pl->radius;
+        const int b = rev_margin(pl), R = pl->radius;
         const int x0 = b, x1 = pl->nx - b, z0 = b, z1 = pl->nz - b;
         if (x0 >= x1 || z0 >= z1) return fail(ctx, SFD_EINVAL, "grid too small for boundary saving");
         std::vector<int> band;
@@ -134,18 +203,14 @@ int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, 
 // Forward over all steps, saving the state before every segment but the first.
 int grad_forward(GradShot* g) {
     sfd_acoustic_plan* pl = g->pl;
-    sfd_context* ctx = pl->ctx;
     int rc = zero_state(pl);
     if (rc) return rc;
     const int T = pl->grad_period;
-    const size_t n = (size_t)pl->nx * pl->nz;
     for (int seg = 0; seg < g->nseg; ++seg) {
         const int k0 = seg * T;
         if (seg > 0) {
-            float* dst = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
-            const Field* f[3] = {&pl->p[pl->cur], &pl->vx[pl->cur], &pl->vz[pl->cur]};
-            for (int q = 0; q < 3; ++q)
-                SFD_HIP(ctx, hipMemcpyAsync(dst + q * n, pl->grid(*f[q]), n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+            rc = ckpt_copy(pl, seg, true);  // the tiled step runs in place: the state is in buffer 0
+            if (rc) return rc;
         }
         rc = enqueue_steps(pl, k0, std::min(T, pl->nt - k0));
         if (rc) return rc;
@@ -181,16 +246,17 @@ int grad_backward(GradShot* g) {
     sfd_context* ctx = pl->ctx;
     hipStream_t s = ctx->stream;
     const int T = pl->grad_period, nt = pl->nt;
-    const size_t n = (size_t)pl->nx * pl->nz;
-    for (Field* f : {&pl->ap[0], &pl->avx[0], &pl->avz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+    int rc = zero_adjoint(pl);
+    if (rc) return rc;
     SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
     SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
     StepKernel dv, dp, av, ap;
-    if (!pick_tiled(pl->radius, pl->ttx, pl->tnw, MODE_FWD_DIV, &dv, &dp) ||
-        !pick_tiled(pl->radius, pl->ttx, pl->tnw, MODE_ADJ, &av, &ap))
+    const bool cpml = is_cpml(pl);
+    if (!pick_tiled(pl->radius, pl->ttx, pl->tnw, cpml ? MODE_FWD_DIV_CPML : MODE_FWD_DIV, &dv, &dp) ||
+        !pick_tiled(pl->radius, pl->ttx, pl->tnw, cpml ? MODE_ADJ_CPML : MODE_ADJ, &av, &ap))
         return fail(ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
     // Every step runs in place on buffer 0 of the state and adjoint fields.
-    int rc = launch_inject(pl, pl->grid(pl->ap[0]), nt - 1);
+    rc = launch_inject(pl, pl->grid(pl->ap[0]), nt - 1);
     if (rc) return rc;
     for (int seg = g->nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * T, k1 = std::min(k0 + T, nt);
@@ -198,11 +264,12 @@ int grad_backward(GradShot* g) {
         pl->cur = 0;
         if (seg == 0) {
             for (Field* f : {&pl->p[0], &pl->vx[0], &pl->vz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+            Field* m[4];
+            for (int q = 0, c = psi_fields(pl, m); q < c; ++q)
+                SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
         } else {
-            const float* src = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
-            const Field* f[3] = {&pl->p[0], &pl->vx[0], &pl->vz[0]};
-            for (int q = 0; q < 3; ++q)
-                SFD_HIP(ctx, hipMemcpyAsync(pl->grid(*f[q]), src + q * n, n * 4, hipMemcpyDeviceToDevice, s));
+            rc = ckpt_copy(pl, seg, false);
+            if (rc) return rc;
         }
         for (int k = k0; k < k1; ++k) {
             AcousticStepArgs a = make_args(pl, 0, k);
@@ -212,8 +279,7 @@ int grad_backward(GradShot* g) {
         }
         // Adjoint steps of the segment, in reverse.
         for (int k = k1 - 1; k >= k0; --k) {
-            AcousticStepArgs a = make_args(pl, pl->ap, pl->avx, pl->avz, 0, k);
-            tiled_args(pl, a, true);
+            const AcousticStepArgs a = adjoint_args(pl, k);
             rc = launch_tiled(pl, av, ap, a);
             if (rc) return rc;
             if (k >= 1) {
@@ -222,7 +288,7 @@ int grad_backward(GradShot* g) {
             }
         }
     }
-    pl->v_raw = true;
+    pl->v_raw = !cpml;
     return SFD_OK;
 }
 
@@ -420,9 +486,8 @@ int group_tables(GradGroup* G) {
     for (int s = 0; s < n; ++s) {
         sfd_acoustic_plan* pl = G->shots[s].pl;
         fa[s] = make_args(pl, 0, 0);
-        aa[s] = make_args(pl, pl->ap, pl->avx, pl->avz, 0, 0);
         tiled_args(pl, fa[s], true);
-        tiled_args(pl, aa[s], true);
+        aa[s] = adjoint_args(pl, 0);
         if (share && s > 0) {
             fa[s].kf = aa[s].kf = fa[0].kf;
             fa[s].bf = aa[s].bf = fa[0].bf;
@@ -516,17 +581,9 @@ int group_launch(GradGroup* G, const BatchKernel kern[2], const DeviceBuffer& bl
 
 // Copies the current state of every shot to (or from) its checkpoint `seg`.
 int group_checkpoint(GradGroup* G, int seg, bool save) {
-    sfd_context* ctx = G->ctx;
     for (auto& g : G->shots) {
-        sfd_acoustic_plan* pl = g.pl;
-        const size_t n = (size_t)pl->nx * pl->nz;
-        float* ck = pl->ckpt.as<float>() + (size_t)(seg - 1) * 3 * n;
-        Field* f[3] = {&pl->p[0], &pl->vx[0], &pl->vz[0]};
-        for (int q = 0; q < 3; ++q) {
-            float* dst = save ? ck + q * n : pl->grid(*f[q]);
-            const float* src = save ? pl->grid(*f[q]) : ck + q * n;
-            SFD_HIP(ctx, hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
-        }
+        const int rc = ckpt_copy(g.pl, seg, save);
+        if (rc) return rc;
     }
     return SFD_OK;
 }
@@ -551,9 +608,13 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (rc) return rc;
     }
     const int R = params[0].radius;
-    if (!pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_FAST_SPONGE, &G.kfwd[0], &G.kfwd[1]) ||
-        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_FWD_DIV, &G.kdiv[0], &G.kdiv[1]) ||
-        !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_ADJ, &G.kadj[0], &G.kadj[1]) ||
+    for (int s = 1; s < nshot; ++s)
+        if (params[s].boundary != params[0].boundary)
+            return fail(ctx, SFD_EINVAL, "shot %d: every shot of a gradient needs the same boundary kind", s);
+    const bool cpml = is_cpml(G.shots[0].pl);
+    if (!pick_tiled_batch(R, kTiledTX, kTiledNW, cpml ? MODE_FAST_CPML : MODE_FAST_SPONGE, &G.kfwd[0], &G.kfwd[1]) ||
+        !pick_tiled_batch(R, kTiledTX, kTiledNW, cpml ? MODE_FWD_DIV_CPML : MODE_FWD_DIV, &G.kdiv[0], &G.kdiv[1]) ||
+        !pick_tiled_batch(R, kTiledTX, kTiledNW, cpml ? MODE_ADJ_CPML : MODE_ADJ, &G.kadj[0], &G.kadj[1]) ||
         !pick_tiled_batch(R, kTiledTX, kTiledNW, MODE_REV, &G.krev[1], &G.krev[0]))
         return fail(ctx, SFD_ENOTSUP, "no gradient kernel for radius %d", R);
     G.nseg = (G.nt + G.period - 1) / G.period;
@@ -575,7 +636,7 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (!rc && G.bsave) rc = group_band(&G, true, k);
         if (rc) return rc;
     }
-    for (auto& g : G.shots) g.pl->v_raw = true;
+    for (auto& g : G.shots) g.pl->v_raw = !cpml;
     // 2. residuals, misfit, seeds, on the device from the gather buffers.
     {
         DeviceBuffer obs, starts, recvs, partial;
@@ -617,7 +678,8 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     // 3. backward: per segment, restore, recompute with divergences, adjoint.
     for (auto& g : G.shots) {
         sfd_acoustic_plan* pl = g.pl;
-        for (Field* f : {&pl->ap[0], &pl->avx[0], &pl->avz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+        rc = zero_adjoint(pl);
+        if (rc) return rc;
         SFD_HIP(ctx, hipMemsetAsync(pl->grad.ptr, 0, pl->grad.bytes, s));
         SFD_HIP(ctx, hipMemsetAsync(pl->src_adj.ptr, 0, pl->src_adj.bytes, s));
         if (G.bsave) SFD_HIP(ctx, hipMemsetAsync(pl->div.ptr, 0, pl->div.bytes, s));
@@ -643,9 +705,13 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     for (int seg = G.bsave ? -1 : G.nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * G.period, k1 = std::min(k0 + G.period, G.nt);
         if (seg == 0) {
-            for (auto& g : G.shots)
+            for (auto& g : G.shots) {
                 for (Field* f : {&g.pl->p[0], &g.pl->vx[0], &g.pl->vz[0]})
                     SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+                Field* m[4];
+                for (int q = 0, c = psi_fields(g.pl, m); q < c; ++q)
+                    SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
+            }
         } else {
             rc = group_checkpoint(&G, seg, false);
             if (rc) return rc;

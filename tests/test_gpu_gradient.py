@@ -15,6 +15,11 @@ sfd_acoustic_misfit_gradient) on MI355X.
   the gradient matches the recomputation's and the FP64 adjoint's to 1e-4
   (the reverse steps differ from the forward ones by rounding only), with
   and without checkpoint resets; outside the box it is exactly zero.
+* C-PML (the adjoint of acoustic_adjoint_cpml.h): the same checks against
+  the FP64 C-PML adjoint and finite differences of the FP64 C-PML forward
+  (profiles fixed by the model's maximum velocity on both sides), including
+  a direction confined to a PML strip; checkpoints carry the memory fields
+  (every interval bitwise the same); shot groups and boundary saving.
 """
 import numpy as np
 import pytest
@@ -131,8 +136,8 @@ def test_gradient_errors(ctx):
     with pytest.raises(sfd.SeisfdError, match="EINVAL"):
         sfd.acoustic_gradient(bad, sd, ctx=ctx)
     pml = sfd.AcousticShot(**{**shot.__dict__, "boundary": "cpml", "pml_f0": 30.0, "_keep": []})
-    with pytest.raises(sfd.SeisfdError, match="ENOTSUP"):
-        sfd.acoustic_gradient(pml, sd, ctx=ctx)
+    with pytest.raises(sfd.SeisfdError, match="EINVAL"):  # one boundary kind per call
+        sfd.misfit_gradient([shot, pml], [np.zeros_like(sd)] * 2, ctx=ctx)
     ref = sfd.AcousticShot(**{**shot.__dict__, "arith": "reference", "_keep": []})
     with pytest.raises(sfd.SeisfdError, match="ENOTSUP"):
         sfd.acoustic_gradient(ref, sd, ctx=ctx)
@@ -243,3 +248,101 @@ def test_fwi_801_workload(ctx):
             an = float(np.sum(g.astype(np.float64) * step))
             print(f"direction {n} {name}: fd {fd:.8e} adjoint {an:.8e} rel {abs(fd - an) / abs(fd):.2e}", flush=True)
             assert abs(fd - an) <= 1e-3 * abs(fd)
+
+
+def cpml_case(nx=72, nz=64, nt=220, radius=4, width=12, seed=1, src=None):
+    """case() with a C-PML of `width` cells (R = 1e-3, f0 = 15 Hz) instead of
+    the sponge; receivers two cells inside the strips' inner edge."""
+    shot, okw, sd = case(nx=nx, nz=nz, nt=nt, radius=radius, sponge=width, seed=seed, src=src)
+    shot = sfd.AcousticShot(**{**shot.__dict__, "boundary": "cpml", "pml_r": 1e-3, "pml_f0": 15.0, "_keep": []})
+    okw = {**okw, "boundary": "cpml", "pml_r": 1e-3, "pml_f0": 15.0, "pml_vmax": float(okw["vp"].max())}
+    return shot, okw, sd
+
+
+@pytest.mark.parametrize("radius,nx,nz,ckpt", [(4, 72, 64, 37), (8, 96, 200, 50)])
+def test_cpml_gradient_matches_fp64_adjoint(ctx, radius, nx, nz, ckpt):
+    shot, okw, sd = cpml_case(nx=nx, nz=nz, radius=radius)
+    print(f"case: C-PML gradient vs fp64 adjoint R={radius} {nx}x{nz} ckpt {ckpt}", flush=True)
+    g, rec = sfd.acoustic_gradient(shot, sd, ckpt_every=ckpt, ctx=ctx)
+    g64, rec64 = oracle.gradient(**okw, seed=sd.astype(np.float64))
+    e_rec, e_g = rel_l2(rec, rec64), rel_l2(g, g64)
+    strip = rel_l2(g[:okw["sponge"]], g64[:okw["sponge"]])
+    print(f"record rel-L2 vs fp64 {e_rec:.2e}, gradient {e_g:.2e}, top strip {strip:.2e}")
+    assert np.abs(g64[:okw["sponge"]]).max() > 1e-3 * np.abs(g64).max()
+    assert e_rec <= 1e-5
+    assert e_g <= 1e-4 and strip <= 1e-4
+
+
+def test_cpml_gradient_matches_finite_differences(ctx):
+    from scipy.ndimage import gaussian_filter
+    shot, okw, sd = cpml_case()
+    print("case: C-PML gradient vs finite differences", flush=True)
+    g, _ = sfd.acoustic_gradient(shot, sd, ckpt_every=50, ctx=ctx)
+    vp = okw["vp"]
+
+    def J(v):
+        o = oracle.acoustic(**{**okw, "vp": v}, precision="f64", nthreads=oracle_threads())
+        return float(np.sum(sd.astype(np.float64) * o["record"]))
+
+    rng = np.random.default_rng(5)
+    dirs = [gaussian_filter(rng.standard_normal(vp.shape), 3) for _ in range(2)]
+    strip = np.zeros(vp.shape)
+    strip[:14] = gaussian_filter(rng.standard_normal((14, vp.shape[1])), 2)
+    dirs.append(strip)
+    e = np.zeros(vp.shape)
+    e[shot.src] = 1.0
+    dirs.append(e)
+    for n, d in enumerate(dirs):
+        vpp = (vp.astype(np.float64) + 1e-2 * d).astype(np.float32)
+        vpm = (vp.astype(np.float64) - 1e-2 * d).astype(np.float32)
+        step = (vpp.astype(np.float64) - vpm.astype(np.float64)) / 2
+        fd = (J(vpp) - J(vpm)) / 2
+        an = float(np.sum(g.astype(np.float64) * step))
+        print(f"direction {n}: fd {fd:.8e} adjoint {an:.8e} rel {abs(fd - an) / abs(fd):.2e}")
+        assert abs(fd - an) <= 1e-3 * abs(fd)
+
+
+def test_cpml_checkpoints_carry_memory_fields(ctx):
+    """Every checkpoint interval gives the C-PML gradient bit for bit: the
+    recomputation restarts from checkpoints that hold the memory fields."""
+    shot, _, sd = cpml_case(nt=150)
+    print("case: C-PML checkpoint intervals", flush=True)
+    g0, r0 = sfd.acoustic_gradient(shot, sd, ckpt_every=150, ctx=ctx)
+    for c in (1, 7, 50, 149):
+        g, r = sfd.acoustic_gradient(shot, sd, ckpt_every=c, ctx=ctx)
+        np.testing.assert_array_equal(r, r0, err_msg=f"record, interval {c}")
+        np.testing.assert_array_equal(g, g0, err_msg=f"gradient, interval {c}")
+
+
+def test_cpml_shot_group_and_boundary_saving(ctx):
+    """Three C-PML shots as one group: the misfit gradient equals the sum of
+    single-shot gradients of the residuals; boundary saving matches it and
+    the FP64 adjoint inside the box [w + 1, n - w - 1) and is zero outside."""
+    print("case: C-PML shot group + boundary saving", flush=True)
+    shots, obs, okws = [], [], []
+    nx, nz, w = 80, 70, 12
+    for f in (0.3, 0.5, 0.7):
+        shot, okw, _ = cpml_case(nx=nx, nz=nz, width=w, src=(int(f * nx), 16))
+        shots.append(shot)
+        okws.append(okw)
+        true = sfd.AcousticShot(**{**shot.__dict__, "vp": shot.vp * np.float32(1.015), "_keep": []})
+        obs.append(sfd.acoustic_forward(true, ctx)[0])
+    Jr, gr = sfd.misfit_gradient(shots, obs, ckpt_every=45, ctx=ctx)
+    Jb, gb = sfd.misfit_gradient(shots, obs, ckpt_every=45, ctx=ctx, method="boundary")
+    assert Jb == Jr
+    gref = np.zeros(gr.shape)
+    g64 = np.zeros(gr.shape)
+    for shot, okw, d in zip(shots, okws, obs):
+        r = sfd.acoustic_forward(shot, ctx)[0] - d
+        gref += sfd.acoustic_gradient(shot, r, ckpt_every=45, ctx=ctx)[0]
+        g64 += oracle.gradient(**okw, seed=r.astype(np.float64))[0]
+    assert rel_l2(gr, gref) <= 1e-6
+    assert rel_l2(gr, g64) <= 1e-4
+    b = w + 1
+    box = np.zeros(gr.shape, bool)
+    box[b:nx - b, b:nz - b] = True
+    assert not np.any(gb[~box])
+    e_rb, e_64 = rel_l2(gb[box], gr[box].astype(np.float64)), rel_l2(gb[box], g64[box])
+    print(f"inside the box: boundary vs recompute {e_rb:.2e}, vs fp64 {e_64:.2e}")
+    assert np.abs(gb[box]).max() > 0
+    assert e_rb <= 1e-4 and e_64 <= 1e-4
