@@ -262,7 +262,7 @@ def c0_leg(no_cpu, no_tune=False):
     return out
 
 
-def visco_leg(nt_err, nt, no_cpu):
+def visco_leg(nt_err, nt, no_cpu, nbatch=256):
     """Visco-acoustic 1001 x 2001 + 40-cell sponge, L = 3 (one shot, nt = 4000
     recorded + the reference's 60-step delay): Mcell-updates/s over all stored
     cells, the algorithmic bytes of the step kernel, the error against the
@@ -318,6 +318,53 @@ def visco_leg(nt_err, nt, no_cpu):
                             "bytes_per_cell_update": 80 + 24 * L,
                             "achieved_GBs": round(ebytes * nsteps / (ems / 1e3) / 1e9, 1),
                             "frac_of_8TBs": round(ebytes * nsteps / (ems / 1e3) / 8e12, 4)}
+    # The Q_forward card's survey batched: nbatch shots (visco-elastic, L = 1,
+    # six window columns + 21-cell sponges each), one launch per half-step for
+    # all of them, against the same shots one plan at a time (a sample of 8).
+    if nbatch > 0:
+        shots = W.visco_card_survey(nshot=nbatch)
+        s0 = shots[0]
+        bnx, bnz = s0.vp.shape
+        bsteps = s0.ref_delay() + s0.nt
+        with sfd.ViscoBatch(shots) as b:
+            b.run()
+            bms = min(b.run() for _ in range(2))
+            brec = b.fetch(0)[0]
+        assert np.isfinite(brec).all()
+        one = []
+        for sh in shots[:8]:
+            with sfd.ViscoPlan(sh) as pl:
+                pl.run()
+                one.append(pl.run())
+        sms = float(np.mean(one)) * nbatch
+        sp = s0.sponge
+        xc = min(sp[0] + sp[1] + 2, bnx) * bnz
+        zc = min(sp[2] + sp[3] + 2, bnz) * bnx
+        Lb = s0.nmech
+        bb = ((80.0 + 24.0 * Lb) * bnx * bnz + 8.0 * (5 + 3 * Lb) * (xc + zc)) * nbatch
+        out["batch"] = {"workload": f"{nbatch} shots of the Q_forward card's survey (isoq_2d_modelling_simple.par: "
+                                    f"401x401 two-layer synthetic model, offsets 0-50 m, 200 m sponges), each "
+                                    f"{bnx}x{bnz}, visco-elastic, L={Lb}, nt={s0.nt} (+{s0.ref_delay()} delay steps), "
+                                    f"10 Hz",
+                        "shots": nbatch, "ms_per_batch": round(bms, 2), "shots_per_s": round(nbatch / (bms / 1e3), 1),
+                        "plain_shots_per_s": round(nbatch / (sms / 1e3), 1), "speedup": round(sms / bms, 2),
+                        "Mcell_per_s": round(nbatch * bnx * bnz * bsteps / bms / 1e3, 1),
+                        "us_per_step": round(bms * 1e3 / bsteps, 2),
+                        "bytes_per_step": bb, "achieved_GBs": round(bb * bsteps / (bms / 1e3) / 1e9, 1),
+                        "frac_of_8TBs": round(bb * bsteps / (bms / 1e3) / 8e12, 4),
+                        "plain_sample": "8 shots one plan at a time (tiled step, two launches per step), scaled"}
+        if not no_cpu:
+            import oracle
+            cores = min(16, len(os.sched_getaffinity(0)))
+            kw = dict(src=s0.src, rec=s0.rec, nsteps=bsteps, dx=s0.dx, dz=s0.dz, dt=s0.dt, f0=s0.f0, vs=s0.vs,
+                      qs=s0.qs, radius=s0.radius, nmech=Lb, sponge=s0.sponge, rec_delay=s0.ref_delay(),
+                      nthreads=cores)
+            o64 = oracle.visco(s0.vp, s0.rho, s0.qp, precision="f64", **kw)["rec_zz"]
+            o32 = oracle.visco(s0.vp, s0.rho, s0.qp, precision="f32", **kw)["rec_zz"]
+            nrm = np.linalg.norm(o64)
+            out["batch"]["fp64_rel_l2"] = float(np.linalg.norm(brec - o64) / nrm)
+            out["batch"]["fp32_oracle_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
+            out["batch"]["error_sample"] = f"strzz gather of shot 1, all {s0.nt} samples"
     tpath = os.path.join(ROOT, "profiles", "visco_step_L3_2081x1081_traffic.json")
     if os.path.exists(tpath) and nt == 4000:
         with open(tpath) as f:
@@ -688,6 +735,7 @@ def main():
     ap.add_argument("--no-visco", action="store_true")
     ap.add_argument("--visco-err-nt", type=int, default=40)
     ap.add_argument("--visco-nt", type=int, default=4000)
+    ap.add_argument("--visco-batch", type=int, default=256, help="shots of the batched visco card survey (0: none)")
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--elastic-err-nt", type=int, default=100)
     ap.add_argument("--elastic-nt", type=int, default=3001)
@@ -741,7 +789,7 @@ def main():
     vti_tiling = tuple(int(v) for v in args.vti_tiling.split(",")) if args.vti_tiling else None
     srm_size = tuple(int(v) for v in args.srm_size.split(","))
     if args.only:
-        leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu),
+        leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu, args.visco_batch),
                "grad": lambda: gradient_leg(args.grad_steps, args.no_cpu),
                "batch": lambda: batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu,
                                             not args.no_plain),
@@ -836,7 +884,7 @@ def main():
         out["batched"] = batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu)
     if not args.no_visco:
         log("visco leg ...")
-        out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu)
+        out["visco"] = visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu, args.visco_batch)
     if not args.no_elastic:
         log("elastic leg ...")
         out["elastic"] = elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu, args.elastic_batch)

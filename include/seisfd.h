@@ -361,6 +361,25 @@ int sfd_visco_plan_autotune(sfd_visco_plan *plan, int trial_steps, float *best_u
 int sfd_visco_plan_get_shape(const sfd_visco_plan *plan, int *kind, int *a, int *b);
 int sfd_visco_plan_destroy(sfd_visco_plan *plan);
 
+/* Visco batches: nshot shots advanced together, one launch of each half-step
+ * of the tiled step over the tiles of all of them per time step (the
+ * reference runs its shots one after another, one per MPI rank, Q main:124).
+ * params[s], vp[s], vs[s], rho[s], qp[s], wavelet[s] as in
+ * sfd_visco_plan_create (vs, rho, wavelet may be NULL, or their entries);
+ * each shot keeps its own grid (windows of any width).  Every shot needs the
+ * same nt, delay, radius, L and the same kind (vs = 0 everywhere, or not).
+ * Results are bitwise those of the shots' own plans. */
+typedef struct sfd_visco_batch sfd_visco_batch;
+int sfd_visco_batch_create(sfd_context *ctx, int nshot, const sfd_visco_params *params,
+                           const float *const *vp, const float *const *vs, const float *const *rho,
+                           const float *const *qp, const float *const *wavelet, sfd_visco_batch **batch);
+/* Zeroes every wavefield and propagates all shots; elapsed_ms: device time. */
+int sfd_visco_batch_run(sfd_visco_batch *batch, float *elapsed_ms);
+/* Shot `shot`'s strzz gather [nrec][nt], its strxx gather (may be NULL) and
+ * snapshots (may be NULL), as sfd_visco_plan_fetch / _fetch_xx. */
+int sfd_visco_batch_fetch(sfd_visco_batch *batch, int shot, float *record, float *record_xx, float *snaps);
+int sfd_visco_batch_destroy(sfd_visco_batch *batch);
+
 /* ---------------------------------------------------------------- elastic
  * Isotropic elastic standard-staggered-grid propagator with a free surface
  * over topography: the drop-in for the time loop of

@@ -238,6 +238,13 @@ _SIGNATURES = {
     "sfd_visco_plan_fetch_xx": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_visco_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_visco_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ViscoParamsC),
+                                              ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
+                                              ctypes.POINTER(_FP), ctypes.POINTER(_FP),
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_visco_batch_run": (ctypes.c_int, [ctypes.c_void_p, _FP]),
+    "sfd_visco_batch_fetch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _FP, _FP, _FP]),
+    "sfd_visco_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_elastic_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ElasticParamsC), _FP, _FP, _FP, _FP,
                                            _FP, _FP, _FP]),
     "sfd_elastic_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ElasticParamsC), _FP, _FP, _FP, _FP,
@@ -807,6 +814,67 @@ class ViscoPlan:
     def close(self):
         if getattr(self, "handle", None):
             lib().sfd_visco_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ViscoBatch:
+    """``sfd_visco_batch``: visco shots (windows of any width) advanced by one
+    launch per half-step for all of them; the shots share nt, the delay, the
+    radius, L and whether vs > 0."""
+
+    def __init__(self, shots, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.ctx.adopt(self)
+        self.shots = list(shots)
+        n = len(self.shots)
+        params = (ViscoParamsC * n)()
+        self._keep = []
+        arrs = [(_FP * n)() for _ in range(5)]  # vp, vs, rho, qp, wavelet
+        for i, sh in enumerate(self.shots):
+            params[i] = sh.params(self._keep)
+            vp, vs, rho, qp, w = sh._arrays()
+            self._keep += [vp, vs, rho, qp, w, sh]
+            for a, v in zip(arrs, (vp, vs, rho, qp, w)):
+                a[i] = _ptr(v)
+        self.params = params
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_visco_batch_create(self.ctx.handle, n, params, arrs[0], arrs[1], arrs[2], arrs[3],
+                                                    arrs[4], ctypes.byref(h)), "sfd_visco_batch_create")
+        self.handle = h
+
+    def run(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().sfd_visco_batch_run(self.handle, ctypes.byref(ms)), "sfd_visco_batch_run")
+        return ms.value
+
+    def fetch(self, shot: int):
+        """Shot `shot`'s (record_zz [nrec, nt], record_xx [nrec, nt], snaps [nsnap, nfields, nx, nz])."""
+        p, sh = self.params[shot], self.shots[shot]
+        nx, nz = np.asarray(sh.vp).shape
+        rzz = np.zeros((p.nrec, p.nt), dtype=np.float32)
+        rxx = np.zeros((p.nrec, p.nt), dtype=np.float32)
+        snaps = np.zeros((p.nsnap, sh.nfields(), nx, nz), dtype=np.float32)
+        has = p.nrec and p.nt
+        self.ctx.check(lib().sfd_visco_batch_fetch(self.handle, shot, _ptr(rzz) if has else None,
+                                                   _ptr(rxx) if has else None, _ptr(snaps) if p.nsnap else None),
+                       "sfd_visco_batch_fetch")
+        return rzz, rxx, snaps
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sfd_visco_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

@@ -47,6 +47,21 @@ struct sfd_visco_plan {
     size_t field_floats() const { return (size_t)(nx + 2 * pad) * nz; }
 };
 
+// Several shots of the same length stepped together (the reference's shot
+// loop, Q main:124, on one GPU): each shot keeps its own plan (fields, model,
+// gathers); one launch per half-step covers the tiles of every shot.
+struct sfd_visco_batch {
+    sfd_context* ctx = nullptr;
+    std::vector<sfd_visco_plan*> plans;
+    int nsteps = 0, delay = 0, nt = 0, ntile = 0;
+    void (*vel)(sfd::ViscoBatchArgs) = nullptr;
+    void (*str)(sfd::ViscoBatchArgs) = nullptr;
+    sfd::DeviceBuffer shot_args, tile_map;
+    ~sfd_visco_batch() {
+        for (auto* pl : plans) sfd_visco_plan_destroy(pl);
+    }
+};
+
 namespace sfd {
 
 __global__ void visco_gather_kernel(const float* __restrict__ buf, const int* __restrict__ slot, float* record,
@@ -113,6 +128,31 @@ bool va_pick_r(int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
 bool ve_pick(int R, int L, bool elastic, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
     if (R == 4) return elastic ? ve_pick_r<4>(L, v, st) : va_pick_r<4>(L, v, st);
     if (R == 5) return elastic ? ve_pick_r<5>(L, v, st) : va_pick_r<5>(L, v, st);
+    return false;
+}
+
+using ViscoBatchKernel = void (*)(ViscoBatchArgs);
+
+template <int R, int L>
+void ve_batch_pair(bool elastic, ViscoBatchKernel* v, ViscoBatchKernel* st) {
+    *v = elastic ? visco_el_velocity_batch<R, kVeTX, kVeNW> : visco_ac_velocity_batch<R, kVeTX, kVeNW>;
+    *st = elastic ? visco_el_stress_batch<R, L, kVeTX, kVeNW> : visco_ac_stress_batch<R, L, kVeTX, kVeNW>;
+}
+
+template <int R>
+bool ve_batch_pick_r(int L, bool elastic, ViscoBatchKernel* v, ViscoBatchKernel* st) {
+    switch (L) {
+    case 1: ve_batch_pair<R, 1>(elastic, v, st); return true;
+    case 2: ve_batch_pair<R, 2>(elastic, v, st); return true;
+    case 3: ve_batch_pair<R, 3>(elastic, v, st); return true;
+    default: return false;
+    }
+}
+
+// The batched forms of ve_pick's launches.
+bool ve_batch_pick(int R, int L, bool elastic, ViscoBatchKernel* v, ViscoBatchKernel* st) {
+    if (R == 4) return ve_batch_pick_r<4>(L, elastic, v, st);
+    if (R == 5) return ve_batch_pick_r<5>(L, elastic, v, st);
     return false;
 }
 
@@ -392,12 +432,41 @@ int ve_zero(sfd_visco_plan* pl) {
     return SFD_OK;
 }
 
+// Snapshots of the tiled step's fields after step k.
+int ve_snaps(sfd_visco_plan* pl, int k, size_t* snap_next) {
+    sfd_context* ctx = pl->ctx;
+    const ViscoElArgs& a = pl->ve;
+    const size_t nb = (size_t)pl->nz * sizeof(float), pb = (size_t)pl->ve_pitch * sizeof(float);
+    while (*snap_next < pl->snap_steps.size() && pl->snap_steps[*snap_next] == k) {
+        float* dst = pl->snaps.as<float>() + *snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
+        const float* src[6] = {a.szz, a.vx, a.vz, a.sxx, a.szz, a.sxz};  // sxz only with vs > 0
+        for (int f = 0; f < 6; ++f) {
+            if (!(pl->snap_fields & (1 << f))) continue;
+            SFD_HIP(ctx, hipMemcpy2DAsync(dst, nb, src[f], pb, nb, pl->nx, hipMemcpyDeviceToDevice, ctx->stream));
+            dst += (size_t)pl->nx * pl->nz;
+        }
+        ++*snap_next;
+    }
+    return SFD_OK;
+}
+
+// The gather of the last of ns steps (the velocity launch of the next step
+// gathers every other).
+int ve_last_gather(sfd_visco_plan* pl, int ns) {
+    if (pl->nrec > 0 && pl->nt > 0 && ns - 1 >= pl->delay) {
+        ViscoElArgs a = pl->ve;
+        a.rec_col = ns - 1 - pl->delay;
+        hipLaunchKernelGGL(visco_el_gather_kernel, dim3((pl->nrec + 255) / 256), dim3(256), 0, pl->ctx->stream, a);
+        SFD_HIP(pl->ctx, hipGetLastError());
+    }
+    return SFD_OK;
+}
+
 // Enqueues the first nsteps steps (all of them by default).
 int ve_enqueue(sfd_visco_plan* pl, int nsteps = -1) {
     sfd_context* ctx = pl->ctx;
     ViscoElArgs a = pl->ve;
     const dim3 grid(a.ntiles), block(64 * kVeNW);
-    const size_t nb = (size_t)pl->nz * sizeof(float), pb = (size_t)pl->ve_pitch * sizeof(float);
     const bool rec = pl->nrec > 0 && pl->nt > 0;
     size_t snap_next = 0;
     const int ns = nsteps < 0 ? pl->nsteps : std::min(nsteps, pl->nsteps);
@@ -409,21 +478,37 @@ int ve_enqueue(sfd_visco_plan* pl, int nsteps = -1) {
         SFD_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(pl->ve_str, grid, block, 0, ctx->stream, a);
         SFD_HIP(ctx, hipGetLastError());
-        while (snap_next < pl->snap_steps.size() && pl->snap_steps[snap_next] == k) {
-            float* dst = pl->snaps.as<float>() + snap_next * pl->nsnap_fields * (size_t)pl->nx * pl->nz;
-            const float* src[6] = {a.szz, a.vx, a.vz, a.sxx, a.szz, a.sxz};  // sxz only with vs > 0
-            for (int f = 0; f < 6; ++f) {
-                if (!(pl->snap_fields & (1 << f))) continue;
-                SFD_HIP(ctx, hipMemcpy2DAsync(dst, nb, src[f], pb, nb, pl->nx, hipMemcpyDeviceToDevice, ctx->stream));
-                dst += (size_t)pl->nx * pl->nz;
-            }
-            ++snap_next;
+        const int rc = ve_snaps(pl, k, &snap_next);
+        if (rc) return rc;
+    }
+    return ve_last_gather(pl, ns);
+}
+
+// A batch's steps: one launch of each half-step over every shot's tiles.
+int ve_batch_enqueue(sfd_visco_batch* b, int nsteps = -1) {
+    sfd_context* ctx = b->ctx;
+    const dim3 grid(b->ntile), block(64 * kVeNW);
+    ViscoBatchArgs a{};
+    a.shots = b->shot_args.as<ViscoElArgs>();
+    a.tile_map = b->tile_map.as<int2>();
+    a.ntile = b->ntile;
+    std::vector<size_t> snap_next(b->plans.size(), 0);
+    const int ns = nsteps < 0 ? b->nsteps : std::min(nsteps, b->nsteps);
+    for (int k = 0; k < ns; ++k) {
+        a.step = k;
+        a.rec_col = (b->nt > 0 && k - 1 >= b->delay) ? k - 1 - b->delay : -1;
+        hipLaunchKernelGGL(b->vel, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(b->str, grid, block, 0, ctx->stream, a);
+        SFD_HIP(ctx, hipGetLastError());
+        for (size_t i = 0; i < b->plans.size(); ++i) {
+            const int rc = ve_snaps(b->plans[i], k, &snap_next[i]);
+            if (rc) return rc;
         }
     }
-    if (rec && ns - 1 >= pl->delay) {
-        a.rec_col = ns - 1 - pl->delay;
-        hipLaunchKernelGGL(visco_el_gather_kernel, dim3((pl->nrec + 255) / 256), dim3(256), 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
+    for (auto* pl : b->plans) {
+        const int rc = ve_last_gather(pl, ns);
+        if (rc) return rc;
     }
     return SFD_OK;
 }
@@ -917,6 +1002,104 @@ int sfd_visco_plan_destroy(sfd_visco_plan* pl) {
     (void)hipSetDevice(pl->ctx->device);
     (void)hipStreamSynchronize(pl->ctx->stream);
     delete pl;
+    return SFD_OK;
+}
+
+int sfd_visco_batch_create(sfd_context* ctx, int nshot, const sfd_visco_params* params, const float* const* vp,
+                           const float* const* vs, const float* const* rho, const float* const* qp,
+                           const float* const* wavelet, sfd_visco_batch** out) {
+    if (!ctx || !out) return SFD_EINVAL;
+    *out = nullptr;
+    if (nshot < 1 || !params || !vp || !qp) return fail(ctx, SFD_EINVAL, "a batch needs nshot >= 1 shots");
+    sfd_visco_batch* b = new (std::nothrow) sfd_visco_batch;
+    if (!b) return fail(ctx, SFD_ENOMEM, "host allocation failed");
+    b->ctx = ctx;
+    for (int i = 0; i < nshot; ++i) {
+        sfd_visco_plan* pl = nullptr;
+        int rc = sfd_visco_plan_create(ctx, &params[i], vp[i], vs ? vs[i] : nullptr, rho ? rho[i] : nullptr, qp[i],
+                                       wavelet ? wavelet[i] : nullptr, &pl);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+        b->plans.push_back(pl);
+        const sfd_visco_plan* p0 = b->plans[0];
+        if (pl->nsteps != p0->nsteps || pl->delay != p0->delay || pl->nt != p0->nt || pl->radius != p0->radius ||
+            pl->nmech != p0->nmech || pl->elastic != p0->elastic) {
+            delete b;
+            return fail(ctx, SFD_EINVAL,
+                        "shot %d: the shots of a batch share nt, the delay, the radius, L and whether vs > 0", i);
+        }
+        // The batch runs the tiled step (a cached march shape does not apply).
+        rc = sfd_visco_plan_set_tiling(pl, 0, 0);
+        if (rc) {
+            delete b;
+            return rc;
+        }
+    }
+    const sfd_visco_plan* p0 = b->plans[0];
+    b->nsteps = p0->nsteps;
+    b->delay = p0->delay;
+    b->nt = p0->nt;
+    if (!ve_batch_pick(p0->radius, p0->nmech, p0->elastic, &b->vel, &b->str)) {
+        delete b;
+        return fail(ctx, SFD_ENOTSUP, "no batched visco kernel for R = %d, L = %d", p0->radius, p0->nmech);
+    }
+    std::vector<ViscoElArgs> args;
+    std::vector<int2> map;
+    for (size_t i = 0; i < b->plans.size(); ++i) {
+        args.push_back(b->plans[i]->ve);
+        for (int t = 0; t < b->plans[i]->ve.ntiles; ++t) map.push_back(make_int2((int)i, t));
+    }
+    hipStream_t st = ctx->stream;
+    hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
+    if (e == hipSuccess) e = upload(b->tile_map, map.data(), map.size(), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        const int rc = hip_check(ctx, e, "visco batch tables");
+        delete b;
+        return rc;
+    }
+    b->ntile = (int)map.size();
+    *out = b;
+    return SFD_OK;
+}
+
+int sfd_visco_batch_run(sfd_visco_batch* b, float* elapsed_ms) {
+    if (!b) return SFD_EINVAL;
+    sfd_context* ctx = b->ctx;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    for (auto* pl : b->plans) {
+        const int rc = ve_zero(pl);
+        if (rc) return rc;
+    }
+    SFD_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    int rc = ve_batch_enqueue(b);
+    if (rc) return rc;
+    SFD_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    SFD_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (elapsed_ms) SFD_HIP(ctx, hipEventElapsedTime(elapsed_ms, ctx->ev0, ctx->ev1));
+    for (auto* pl : b->plans) {
+        rc = check_finite(ctx, pl->rec_zz.as<float>(), pl->rec_zz.bytes / 4, "a gather of the batch");
+        if (rc) return rc;
+    }
+    return SFD_OK;
+}
+
+int sfd_visco_batch_fetch(sfd_visco_batch* b, int shot, float* record, float* record_xx, float* snaps) {
+    if (!b) return SFD_EINVAL;
+    if (shot < 0 || shot >= (int)b->plans.size())
+        return fail(b->ctx, SFD_EINVAL, "shot %d outside the batch (%zu shots)", shot, b->plans.size());
+    int rc = sfd_visco_plan_fetch(b->plans[shot], record, snaps);
+    if (!rc && record_xx) rc = sfd_visco_plan_fetch_xx(b->plans[shot], record_xx);
+    return rc;
+}
+
+int sfd_visco_batch_destroy(sfd_visco_batch* b) {
+    if (!b) return SFD_EINVAL;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
     return SFD_OK;
 }
 

@@ -67,8 +67,22 @@ struct ViscoElArgs {
     float *rec_zz, *rec_xx;
 };
 
-__device__ __forceinline__ void ve_tile(const ViscoElArgs& a, int tx, int* i0, int* k0) {
-    const int t = xcd_tile(blockIdx.x, a.ntiles);
+// Several shots per launch: shots[s] is shot s's argument block (read
+// through the constant address space), tile_map[g] = (shot, the shot's tile)
+// of global tile g; step and rec_col are the shots' common step index and
+// gather column.
+struct ViscoBatchArgs {
+    const ViscoElArgs* shots;
+    const int2* tile_map;
+    int ntile;
+    int step, rec_col;
+};
+
+using ViscoShotArgs = const __attribute__((address_space(4))) ViscoElArgs;
+
+// Tile t of a shot: row band t / ntz, z tile t % ntz.
+template <class A>
+__device__ __forceinline__ void ve_tile(const A& a, int tx, int t, int* i0, int* k0) {
     *k0 = (t % a.ntz) * kVeTZ;
     *i0 = (t / a.ntz) * tx;
 }
@@ -111,17 +125,20 @@ __device__ __forceinline__ void ve_diag(const float* p, const float* co, float& 
     }
 }
 
-template <int R, int TX, int NW>
-__global__ __launch_bounds__(64 * NW) void visco_el_velocity_kernel(ViscoElArgs a) {
-    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
-    __shared__ float s_xx[HH * W], s_zz[HH * W], s_xz[HH * W];
+// Bodies of the four launches for tile t of a shot (A = ViscoElArgs in
+// kernel arguments, or a batch's block in the constant address space); the
+// workgroup of tile 0 gathers the receivers (column rec_col, < 0: none).
+template <int R, int TX, int NW, class A>
+__device__ __forceinline__ void ve_el_velocity(const A& a, int t, int rec_col, float* s_xx, float* s_zz,
+                                               float* s_xz) {
+    constexpr int W = kVeTZ + 2 * R;
     int i0, k0;
-    ve_tile(a, TX, &i0, &k0);
-    if (blockIdx.x == 0 && a.rec_col >= 0) {
+    ve_tile(a, TX, t, &i0, &k0);
+    if (t == 0 && rec_col >= 0) {
         for (int r = threadIdx.x; r < a.nrec; r += 64 * NW) {
             const ptrdiff_t o = (ptrdiff_t)a.rx[r] * a.pitch + a.rz[r];
-            a.rec_zz[(size_t)r * a.nrt + a.rec_col] = a.szz[o];
-            a.rec_xx[(size_t)r * a.nrt + a.rec_col] = a.sxx[o];
+            a.rec_zz[(size_t)r * a.nrt + rec_col] = a.szz[o];
+            a.rec_xx[(size_t)r * a.nrt + rec_col] = a.sxx[o];
         }
     }
     ve_stage<R, TX, NW>(s_xx, a.sxx, a.pitch, i0, k0);
@@ -177,12 +194,11 @@ __global__ __launch_bounds__(64 * NW) void visco_el_velocity_kernel(ViscoElArgs 
     }
 }
 
-template <int R, int L, int TX, int NW>
-__global__ __launch_bounds__(64 * NW) void visco_el_stress_kernel(ViscoElArgs a) {
-    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
-    __shared__ float s_vx[HH * W], s_vz[HH * W];
+template <int R, int L, int TX, int NW, class A>
+__device__ __forceinline__ void ve_el_stress(const A& a, int t, int step, float* s_vx, float* s_vz) {
+    constexpr int W = kVeTZ + 2 * R;
     int i0, k0;
-    ve_tile(a, TX, &i0, &k0);
+    ve_tile(a, TX, t, &i0, &k0);
     ve_stage<R, TX, NW>(s_vx, a.vx, a.pitch, i0, k0);
     ve_stage<R, TX, NW>(s_vz, a.vz, a.pitch, i0, k0);
     __syncthreads();
@@ -196,7 +212,7 @@ __global__ __launch_bounds__(64 * NW) void visco_el_stress_kernel(ViscoElArgs a)
     int zc = -1;
     if (k < a.zlo) zc = k;
     else if (k >= a.nz - a.zhi) zc = k - (a.nz - a.zhi) + a.zlo;
-    const float amp = a.amp[a.step];
+    const float amp = a.amp[step];
     const bool src_col = k >= a.sz0 && k < a.sz1;
     for (int j = 0; j < TX / NW; ++j) {
         const int il = wv + j * NW, i = i0 + il;
@@ -284,15 +300,14 @@ __global__ __launch_bounds__(64 * NW) void visco_el_stress_kernel(ViscoElArgs a)
 // krp) minus the terms that vanish: sigma_xz and its memory variables stay
 // zero, so vx has an x part only and vz a z part only.  Strips: xs [2 + L]
 // [xlo + xhi][nz] = vx, sigma, r_l; zs [2 + L][nx][zlo + zhi] = vz, sigma, r_l.
-template <int R, int TX, int NW>
-__global__ __launch_bounds__(64 * NW) void visco_ac_velocity_kernel(ViscoElArgs a) {
-    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
-    __shared__ float s_s[HH * W];
+template <int R, int TX, int NW, class A>
+__device__ __forceinline__ void ve_ac_velocity(const A& a, int t, int rec_col, float* s_s) {
+    constexpr int W = kVeTZ + 2 * R;
     int i0, k0;
-    ve_tile(a, TX, &i0, &k0);
-    if (blockIdx.x == 0 && a.rec_col >= 0) {
+    ve_tile(a, TX, t, &i0, &k0);
+    if (t == 0 && rec_col >= 0) {
         for (int r = threadIdx.x; r < a.nrec; r += 64 * NW)
-            a.rec_zz[(size_t)r * a.nrt + a.rec_col] = a.szz[(ptrdiff_t)a.rx[r] * a.pitch + a.rz[r]];
+            a.rec_zz[(size_t)r * a.nrt + rec_col] = a.szz[(ptrdiff_t)a.rx[r] * a.pitch + a.rz[r]];
     }
     ve_stage<R, TX, NW>(s_s, a.szz, a.pitch, i0, k0);
     __syncthreads();
@@ -337,12 +352,11 @@ __global__ __launch_bounds__(64 * NW) void visco_ac_velocity_kernel(ViscoElArgs 
     }
 }
 
-template <int R, int L, int TX, int NW>
-__global__ __launch_bounds__(64 * NW) void visco_ac_stress_kernel(ViscoElArgs a) {
-    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
-    __shared__ float s_vx[HH * W], s_vz[HH * W];
+template <int R, int L, int TX, int NW, class A>
+__device__ __forceinline__ void ve_ac_stress(const A& a, int t, int step, float* s_vx, float* s_vz) {
+    constexpr int W = kVeTZ + 2 * R;
     int i0, k0;
-    ve_tile(a, TX, &i0, &k0);
+    ve_tile(a, TX, t, &i0, &k0);
     ve_stage<R, TX, NW>(s_vx, a.vx, a.pitch, i0, k0);
     ve_stage<R, TX, NW>(s_vz, a.vz, a.pitch, i0, k0);
     __syncthreads();
@@ -356,7 +370,7 @@ __global__ __launch_bounds__(64 * NW) void visco_ac_stress_kernel(ViscoElArgs a)
     int zc = -1;
     if (k < a.zlo) zc = k;
     else if (k >= a.nz - a.zhi) zc = k - (a.nz - a.zhi) + a.zlo;
-    const float amp = a.amp[a.step];
+    const float amp = a.amp[step];
     const bool src_col = k >= a.sz0 && k < a.sz1;
     for (int j = 0; j < TX / NW; ++j) {
         const int il = wv + j * NW, i = i0 + il;
@@ -416,6 +430,71 @@ __global__ __launch_bounds__(64 * NW) void visco_ac_stress_kernel(ViscoElArgs a)
 #pragma unroll
         for (int l = 0; l < L; ++l) a.r[(size_t)l * a.rstride + o] = tot[1 + l];
     }
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_el_velocity_kernel(ViscoElArgs a) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_xx[HH * W], s_zz[HH * W], s_xz[HH * W];
+    ve_el_velocity<R, TX, NW>(a, xcd_tile(blockIdx.x, a.ntiles), a.rec_col, s_xx, s_zz, s_xz);
+}
+
+template <int R, int L, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_el_stress_kernel(ViscoElArgs a) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_vx[HH * W], s_vz[HH * W];
+    ve_el_stress<R, L, TX, NW>(a, xcd_tile(blockIdx.x, a.ntiles), a.step, s_vx, s_vz);
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_ac_velocity_kernel(ViscoElArgs a) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_s[HH * W];
+    ve_ac_velocity<R, TX, NW>(a, xcd_tile(blockIdx.x, a.ntiles), a.rec_col, s_s);
+}
+
+template <int R, int L, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_ac_stress_kernel(ViscoElArgs a) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_vx[HH * W], s_vz[HH * W];
+    ve_ac_stress<R, L, TX, NW>(a, xcd_tile(blockIdx.x, a.ntiles), a.step, s_vx, s_vz);
+}
+
+// The same launches over the tiles of several shots.
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_el_velocity_batch(ViscoBatchArgs b) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_xx[HH * W], s_zz[HH * W], s_xz[HH * W];
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile(blockIdx.x, b.ntile);
+    ViscoShotArgs& a = *(ViscoShotArgs*)(b.shots + ro(m, 0));
+    ve_el_velocity<R, TX, NW>(a, ro(m, 1), b.rec_col, s_xx, s_zz, s_xz);
+}
+
+template <int R, int L, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_el_stress_batch(ViscoBatchArgs b) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_vx[HH * W], s_vz[HH * W];
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile(blockIdx.x, b.ntile);
+    ViscoShotArgs& a = *(ViscoShotArgs*)(b.shots + ro(m, 0));
+    ve_el_stress<R, L, TX, NW>(a, ro(m, 1), b.step, s_vx, s_vz);
+}
+
+template <int R, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_ac_velocity_batch(ViscoBatchArgs b) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_s[HH * W];
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile(blockIdx.x, b.ntile);
+    ViscoShotArgs& a = *(ViscoShotArgs*)(b.shots + ro(m, 0));
+    ve_ac_velocity<R, TX, NW>(a, ro(m, 1), b.rec_col, s_s);
+}
+
+template <int R, int L, int TX, int NW>
+__global__ __launch_bounds__(64 * NW) void visco_ac_stress_batch(ViscoBatchArgs b) {
+    constexpr int W = kVeTZ + 2 * R, HH = TX + 2 * R;
+    __shared__ float s_vx[HH * W], s_vz[HH * W];
+    const int* m = reinterpret_cast<const int*>(b.tile_map) + 2 * xcd_tile(blockIdx.x, b.ntile);
+    ViscoShotArgs& a = *(ViscoShotArgs*)(b.shots + ro(m, 0));
+    ve_ac_stress<R, L, TX, NW>(a, ro(m, 1), b.step, s_vx, s_vz);
 }
 
 // Receivers of the last step (the velocity launch of the next step gathers

@@ -192,6 +192,45 @@ def visco_1001x2001(nt: int = 4000, nmech: int = 3, scale: int = 1, vp_seed: int
     return ViscoShot(**kw)
 
 
+def visco_card_survey(nshot: int = 256, nt: int = 2000, first: int = 1, elastic: bool = True,
+                      offset_max: float = 50.0, nmech: int = 1):
+    """Shots first .. first + nshot - 1 of the Q_forward card
+    (forward/Q_forward/isoq_2d_modelling_simple.par): a 401 x 401 model,
+    dx = dz = 10 m, offsets 0 .. offset_max (the card: 50 m, six window
+    columns), 200 m sponges on every side (npxn = nint(200/10) + 1 = 21 cells,
+    Q main:105-108), sx0 = sz0 = rz0 = 0, dsx = 10 m, nt = 2000, dt = 1 ms,
+    f0 = 10 Hz, L = 1 (the reference's).  Per shot (Q main:124-129, Q
+    sub:380-382, main:456-457): isx = 1 + (shot - 1), window model columns
+    ixmine + w (get_currshot_parameters' i + ixmine) edge-padded by 21 cells,
+    source at (npxn, npzn + 1), receivers on row npzn + 1 of every window
+    column (0-based).  The card's model files are absent: a synthetic two-layer
+    model (interface at row 200) with vp 2000 / 3000 m/s, vs = vp / sqrt(3)
+    (0 with elastic=False), rho 2000 / 2300, qp 60 / 120, qs 40 / 80 (finite
+    Q: the reference's override of Q to 1e5, main:418-419, is not applied).
+    Returns the list of ViscoShot."""
+    n = 401
+    top = np.zeros((n, n), dtype=bool)
+    top[:, :200] = True
+    vp = np.where(top, 2000.0, 3000.0).astype(np.float32)
+    vs = (vp / np.float32(np.sqrt(3.0))).astype(np.float32) if elastic else np.zeros_like(vp)
+    rho = np.where(top, 2000.0, 2300.0).astype(np.float32)
+    qp = np.where(top, 60.0, 120.0).astype(np.float32)
+    qs = np.where(top, 40.0, 80.0).astype(np.float32)
+    npx = npz = 21
+    nxe = int(round(offset_max / 10.0)) + 1
+    shots = []
+    for i in range(first, first + nshot):
+        ixmine = i  # isx = isx0 + (i - 1), isx0 = nint(sx0/dx) + 1 = 1; offset_min = 0
+        win = [window_model(f, ixmine, nxe, (npx, npx), (npz, npz)) for f in (vp, vs, rho, qp, qs)]
+        rx = np.arange(npx, npx + nxe, dtype=np.int32)
+        rz = np.full_like(rx, npz + 1)
+        shots.append(ViscoShot(vp=win[0], vs=win[1] if elastic else None, rho=win[2], qp=win[3],
+                               qs=win[4] if elastic else None, src=(npx, npz + 1), rec=(rx, rz), nt=nt,
+                               dx=10.0, dz=10.0, dt=1e-3, f0=10.0, radius=5, nmech=nmech,
+                               sponge=(npx, npx, npz, npz)))
+    return shots
+
+
 def fwi_801(nshot: int = 16, nt: int = 2500, scale: int = 1, seed: int = 6):
     """Adjoint-gradient workload: 801 x 801 model + 40-cell sponge (881 x 881
     stored), dx = dz = 10 m, dt = 1 ms, nt = 2500, 10 Hz Ricker.  Background:

@@ -217,3 +217,60 @@ def test_visco_elastic_fluid_cells_and_default_qs(ctx):
     rzz, _ = sfd.visco_forward(shot, ctx)
     o32 = run_oracle(shot)
     assert rel(rzz, o32["rec_zz"]) <= TOL_F32
+
+
+def test_visco_batch_equals_plans_card_survey(ctx):
+    """Three shots of the Q_forward card's survey (visco-elastic, L = 1) in one
+    batch: both gathers bitwise equal to each shot's own plan; the first
+    against the FP32 and FP64 oracles."""
+    shots = W.visco_card_survey(nshot=3, nt=300, first=100)
+    print("case: visco batch, card survey, 3 shots", flush=True)
+    with sfd.ViscoBatch(shots, ctx) as b:
+        b.run()
+        got = [b.fetch(i) for i in range(3)]
+    for i, sh in enumerate(shots):
+        with sfd.ViscoPlan(sh, ctx) as pl:
+            pl.run()
+            rzz, _ = pl.fetch()
+            rxx = pl.fetch_xx()
+        np.testing.assert_array_equal(got[i][0], rzz, err_msg=f"shot {i} rec_zz")
+        np.testing.assert_array_equal(got[i][1], rxx, err_msg=f"shot {i} rec_xx")
+    o32 = run_oracle(shots[0])
+    assert np.abs(o32["rec_zz"]).max() > 0
+    assert rel(got[0][0], o32["rec_zz"]) <= TOL_F32
+    assert rel(got[0][1], o32["rec_xx"]) <= TOL_F32
+
+
+@pytest.mark.parametrize("nmech", [1, 3])
+def test_visco_batch_uneven_windows_with_snapshots(ctx, nmech):
+    """Visco-acoustic shots on windows of three widths (ragged against the
+    8 x 64 tiles), with snapshots: every gather and snapshot bitwise equal to
+    the shots' own plans."""
+    print(f"case: visco batch, uneven windows, L={nmech}", flush=True)
+    shots = []
+    for k, width in enumerate((77, 130, 201)):
+        base = W.visco_1001x2001(nt=240, nmech=nmech, scale=8, nb=20, snap_steps=(100, 250), vp_seed=2 + k)
+        shots.append(sfd.ViscoShot(**{**base.__dict__, "vp": base.vp[:width], "qp": base.qp[:width],
+                                      "src": (width // 2, 22),
+                                      "rec": (np.arange(20, width - 20, dtype=np.int32),
+                                              np.full(width - 40, 22, dtype=np.int32)),
+                                      "_keep": []}))
+    with sfd.ViscoBatch(shots, ctx) as b:
+        b.run()
+        got = [b.fetch(i) for i in range(len(shots))]
+    for i, sh in enumerate(shots):
+        with sfd.ViscoPlan(sh, ctx) as pl:
+            pl.run()
+            rec, snaps = pl.fetch()
+        assert np.abs(rec).max() > 0
+        np.testing.assert_array_equal(got[i][0], rec, err_msg=f"shot {i} record")
+        np.testing.assert_array_equal(got[i][2], snaps, err_msg=f"shot {i} snapshots")
+
+
+def test_visco_batch_rejects_mixed_shots(ctx):
+    a = W.visco_card_survey(nshot=1, nt=50)[0]
+    b_ac = W.visco_card_survey(nshot=1, nt=50, elastic=False)[0]
+    c_nt = W.visco_card_survey(nshot=1, nt=60)[0]
+    for other in (b_ac, c_nt):
+        with pytest.raises(sfd.SeisfdError, match="EINVAL"):
+            sfd.ViscoBatch([a, other], ctx)
