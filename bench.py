@@ -365,6 +365,8 @@ def visco_leg(nt_err, nt, no_cpu, nbatch=256):
             out["batch"]["fp64_rel_l2"] = float(np.linalg.norm(brec - o64) / nrm)
             out["batch"]["fp32_oracle_vs_fp64_rel_l2"] = float(np.linalg.norm(o32 - o64) / nrm)
             out["batch"]["error_sample"] = f"strzz gather of shot 1, all {s0.nt} samples"
+        out["batch"]["traffic"] = load_traffic(f"visco_batch_{nbatch}x{bnx}x{bnz}_traffic.json", "hbm_bytes_per_step",
+                                               "visco_el_velocity_batch" if s0.elastic() else "visco_ac_velocity_batch")
     tpath = os.path.join(ROOT, "profiles", "visco_step_L3_2081x1081_traffic.json")
     if os.path.exists(tpath) and nt == 4000:
         with open(tpath) as f:
