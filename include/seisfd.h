@@ -200,6 +200,16 @@ int sfd_acoustic_batch_run(sfd_acoustic_batch *batch, int how, float *elapsed_ms
 /* Gather of one shot, [nrec][nt] trace-major. */
 int sfd_acoustic_batch_fetch(sfd_acoustic_batch *batch, int shot, float *record);
 int sfd_acoustic_batch_destroy(sfd_acoustic_batch *batch);
+/* Shot groups (no counterpart in the reference): a batch runs its shots in
+ * groups of consecutive shots, each group for all nt steps before the next,
+ * so that a group's fields stay in the GPU's 256 MB last-level cache (MALL)
+ * across its steps.  bytes = the budget of one group's resident fields
+ * (P, Vx, Vz, kf, a per-cell bf, C-PML memory fields); 0 = the library's
+ * default (192 MiB), SIZE_MAX = every shot in one group.  Every group holds
+ * at least one shot.  Results are bitwise the same for any grouping.
+ * Invalidates a captured graph.  get_groups: the current number of groups. */
+int sfd_acoustic_batch_set_group_bytes(sfd_acoustic_batch *batch, size_t bytes);
+int sfd_acoustic_batch_get_groups(const sfd_acoustic_batch *batch, int *ngroups);
 
 /* ---------------------------------------------------------------- autotuner
  * (no counterpart in the reference).  After a warm-up of up to 4 trials'

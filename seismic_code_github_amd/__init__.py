@@ -238,6 +238,8 @@ _SIGNATURES = {
     "sfd_visco_plan_fetch_xx": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_visco_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sfd_acoustic_batch_set_group_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
+    "sfd_acoustic_batch_get_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "sfd_visco_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ViscoParamsC),
                                               ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
                                               ctypes.POINTER(_FP), ctypes.POINTER(_FP),
@@ -633,6 +635,15 @@ class AcousticBatch:
 
     def set_tiled(self, tile_rows: int = 0, waves: int = 0):
         self.ctx.check(lib().sfd_acoustic_batch_set_tiled(self.handle, tile_rows, waves), "batch_set_tiled")
+
+    def set_group_bytes(self, nbytes: int = 0):
+        """Budget of one shot group's resident fields (0 = the default,
+        -1 = every shot in one group); returns the number of groups."""
+        v = (1 << 64) - 1 if nbytes < 0 else int(nbytes)
+        self.ctx.check(lib().sfd_acoustic_batch_set_group_bytes(self.handle, v), "batch set_group_bytes")
+        n = ctypes.c_int()
+        self.ctx.check(lib().sfd_acoustic_batch_get_groups(self.handle, ctypes.byref(n)), "batch get_groups")
+        return n.value
 
     def set_fused(self, rows: int = 0, waves: int = 0):
         """One launch per step for every shot (0 = 32-row tiles, 8 waves)."""

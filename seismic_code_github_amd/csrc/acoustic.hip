@@ -1065,6 +1065,16 @@ struct sfd_acoustic_batch {
     BatchKernel fstep = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // Groups of consecutive shots run one after another, each for all its
+    // steps, so that a group's fields stay in the 256 MB MALL across steps:
+    // shots [s0, s1), their argument blocks from args_off (parity-major) and
+    // tiles from tile_off in the tables.  group_bytes: the budget of one
+    // group's resident fields (0 = the default, SIZE_MAX = one group).
+    struct Group {
+        int s0, s1, args_off, tile_off, ntile;
+    };
+    std::vector<Group> groups;
+    size_t group_bytes = 0;
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1097,27 +1107,59 @@ std::string batch_class(const sfd_acoustic_batch* b) {
     return buf;
 }
 
+// Bytes a shot's step keeps resident when it runs in place: P, Vx, Vz, kf
+// (and bf unless the density is uniform) and the C-PML memory fields.
+size_t resident_bytes(const sfd_acoustic_plan* pl) {
+    size_t n = (size_t)pl->nx * pl->nz * (pl->bf_const != 0.f ? 16 : 20);
+    for (const Field* f : {&pl->psi_px[0], &pl->psi_vx[0], &pl->psi_pz[0], &pl->psi_vz[0]}) n += f->bytes;
+    return n;
+}
+
+// Default budget of one group's resident fields: measured on MI355X (256 MB
+// MALL) with the 64 windowed shots of 1101 x 1100-2600 cells.
+constexpr size_t kBatchGroupBytes = (size_t)192 << 20;
+
 // Tile tables and per-shot argument blocks (both ping-pong orders; the
-// two-launch tiled step runs in place and needs one).
+// two-launch tiled step runs in place and needs one), group by group.
 int build_batch_tables(sfd_acoustic_batch* b) {
     const int n = (int)b->plans.size();
+    const size_t budget = b->group_bytes ? b->group_bytes : kBatchGroupBytes;
+    b->groups.clear();
+    for (int s = 0; s < n;) {
+        size_t used = 0;
+        int e = s;
+        do used += resident_bytes(b->plans[e++]);
+        while (e < n && used + resident_bytes(b->plans[e]) <= budget);
+        b->groups.push_back({s, e, 0, 0, 0});
+        s = e;
+    }
     std::vector<AcousticStepArgs> args(2 * (size_t)n);
     std::vector<int> base(n), tshot;
     std::vector<int2> tmap;
     int t = 0;
-    for (int s = 0; s < n; ++s) {
-        sfd_acoustic_plan* pl = b->plans[s];
-        for (int par = 0; par < 2; ++par) args[(size_t)par * n + s] = make_args(pl, par, 0);
-        // Batches start from zero fields, so the tiled sponge steps may read
-        // their velocities as unabsorbed from the first step on.
-        if (b->tiled) tiled_args(pl, args[s], b->mode == MODE_FAST_SPONGE);
-        if (b->fused)
-            for (int par = 0; par < 2; ++par) fused_args(pl, args[(size_t)par * n + s], b->mode == MODE_FAST_SPONGE);
-        const int nt = args[s].ntz * args[s].nchunk;
-        base[s] = t;
-        tshot.insert(tshot.end(), nt, s);
-        for (int q = 0; q < nt; ++q) tmap.push_back(make_int2(s, q));
-        t += nt;
+    for (auto& g : b->groups) {
+        const int ng = g.s1 - g.s0;
+        g.args_off = 2 * g.s0;
+        g.tile_off = t;
+        int tl = 0;
+        for (int s = g.s0; s < g.s1; ++s) {
+            sfd_acoustic_plan* pl = b->plans[s];
+            const int ls = s - g.s0;
+            AcousticStepArgs* ga = args.data() + g.args_off;
+            for (int par = 0; par < 2; ++par) ga[(size_t)par * ng + ls] = make_args(pl, par, 0);
+            // Batches start from zero fields, so the tiled sponge steps may
+            // read their velocities as unabsorbed from the first step on.
+            if (b->tiled) tiled_args(pl, ga[ls], b->mode == MODE_FAST_SPONGE);
+            if (b->fused)
+                for (int par = 0; par < 2; ++par) fused_args(pl, ga[(size_t)par * ng + ls], b->mode == MODE_FAST_SPONGE);
+            const int nt = ga[ls].ntz * ga[ls].nchunk;
+            base[s] = tl;  // tile_base and tile_shot hold group-local values
+            tshot.insert(tshot.end(), nt, ls);
+            for (int q = 0; q < nt; ++q) tmap.push_back(make_int2(ls, q));
+            tl += nt;
+        }
+        g.ntile = tl;
+        t += tl;
     }
     b->ntile = t;
     hipStream_t st = b->ctx->stream;
@@ -1130,28 +1172,31 @@ int build_batch_tables(sfd_acoustic_batch* b) {
     return SFD_OK;
 }
 
+// The first nsteps steps of every shot, group after group.
 int enqueue_batch_steps(sfd_acoustic_batch* b, int nsteps) {
-    BatchStepArgs a{};
-    a.shots = b->shot_args.as<AcousticStepArgs>();
-    a.tile_shot = b->tile_shot.as<int>();
-    a.tile_base = b->tile_base.as<int>();
-    a.tile_map = b->tile_map.as<int2>();
-    a.nshot = (int)b->plans.size();
-    a.ntile = b->ntile;
-    for (int k = 0; k < nsteps; ++k) {
-        a.k = k;
-        a.parity = k & 1;
-        if (b->tiled) {
-            const dim3 block(64 * b->tnw);
-            hipLaunchKernelGGL(b->tvel, dim3(b->ntile), block, 0, b->ctx->stream, a);
+    for (const auto& g : b->groups) {
+        BatchStepArgs a{};
+        a.shots = b->shot_args.as<AcousticStepArgs>() + g.args_off;
+        a.tile_shot = b->tile_shot.as<int>() + g.tile_off;
+        a.tile_base = b->tile_base.as<int>() + g.s0;
+        a.tile_map = b->tile_map.as<int2>() + g.tile_off;
+        a.nshot = g.s1 - g.s0;
+        a.ntile = g.ntile;
+        for (int k = 0; k < nsteps; ++k) {
+            a.k = k;
+            a.parity = k & 1;
+            if (b->tiled) {
+                const dim3 block(64 * b->tnw);
+                hipLaunchKernelGGL(b->tvel, dim3(g.ntile), block, 0, b->ctx->stream, a);
+                SFD_HIP(b->ctx, hipGetLastError());
+                hipLaunchKernelGGL(b->tpres, dim3(g.ntile), block, 0, b->ctx->stream, a);
+            } else if (b->fused) {
+                hipLaunchKernelGGL(b->fstep, dim3(g.ntile), dim3(64 * b->fnw), 0, b->ctx->stream, a);
+            } else {
+                hipLaunchKernelGGL(b->kernel, dim3(g.ntile), dim3(64 * b->nw), 0, b->ctx->stream, a);
+            }
             SFD_HIP(b->ctx, hipGetLastError());
-            hipLaunchKernelGGL(b->tpres, dim3(b->ntile), block, 0, b->ctx->stream, a);
-        } else if (b->fused) {
-            hipLaunchKernelGGL(b->fstep, dim3(b->ntile), dim3(64 * b->fnw), 0, b->ctx->stream, a);
-        } else {
-            hipLaunchKernelGGL(b->kernel, dim3(b->ntile), dim3(64 * b->nw), 0, b->ctx->stream, a);
         }
-        SFD_HIP(b->ctx, hipGetLastError());
     }
     return SFD_OK;
 }
@@ -1238,6 +1283,18 @@ int sfd_acoustic_batch_set_fused(sfd_acoustic_batch* b, int rows, int waves) {
     b->fnw = nw;
     b->fstep = k;
     return build_batch_tables(b);
+}
+
+int sfd_acoustic_batch_set_group_bytes(sfd_acoustic_batch* b, size_t bytes) {
+    if (!b) return SFD_EINVAL;
+    b->group_bytes = bytes;
+    return build_batch_tables(b);
+}
+
+int sfd_acoustic_batch_get_groups(const sfd_acoustic_batch* b, int* ngroups) {
+    if (!b || !ngroups) return SFD_EINVAL;
+    *ngroups = (int)b->groups.size();
+    return SFD_OK;
 }
 
 int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch* b, int waves, int chunk_rows) {

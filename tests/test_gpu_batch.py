@@ -112,3 +112,25 @@ def test_batch_rejects_mixed_shots(ctx):
     with pytest.raises(sfd.SeisfdError) as e:
         sfd.AcousticBatch(shots, ctx)
     assert e.value.code == sfd.SFD_EINVAL
+
+
+def test_shot_groups_bitwise(ctx):
+    """Shot groups (sfd_acoustic_batch_set_group_bytes): a budget small
+    enough for one shot per group, a middle one and one group give the same
+    gathers bit for bit, for the two-launch and the one-launch steps."""
+    _, shots = W.windowed_shots(nshot=5, nt=160, scale=8, width=12)
+    print("case: shot groups", flush=True)
+    with sfd.AcousticBatch(shots, ctx) as b:
+        for kind, a, c in (("tiled", 16, 4), ("fused", 128, 4)):
+            (b.set_tiled if kind == "tiled" else b.set_fused)(a, c)
+            recs = {}
+            for budget in (-1, 1, 3 * (1 << 20)):
+                ng = b.set_group_bytes(budget)
+                b.run(sfd.BATCH_GRAPH)
+                recs[ng] = [b.fetch(i) for i in range(len(shots))]
+            assert 1 in recs and len(shots) in recs, list(recs)
+            ref = recs[1]
+            assert np.abs(ref[0]).max() > 0
+            for ng, rr in recs.items():
+                for i in range(len(shots)):
+                    np.testing.assert_array_equal(rr[i], ref[i], err_msg=f"{kind}: {ng} groups, shot {i}")
