@@ -147,21 +147,27 @@ def load_traffic(name, key, stem):
     return t.get(key)
 
 
-def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
+def batched_leg(nt, no_tune=False, no_cpu=False, plain=True, launches=False):
     """64 shots on uneven aperture windows of a 1001 x 6001 model (C-PML 50,
-    nt=3000 per shot): one tiled step per time step for all shots, replayed
+    nt=3000 per shot): one step per time step for all shots of a group (the
+    library's default groups, sized to the MALL, one after another), replayed
     from a hipGraph, against the plain path (shot after shot, one step per
-    shot and time step).  Launch shape from the autotuner.  Timed over `nt`
-    steps (default all 3000).  FP64 error: the gather of the median-width
-    window against the FP64 oracle over the same steps."""
+    shot and time step, each at its own grid class's tuned shape).  Launch
+    shape from the autotuner.  Timed over `nt` steps (default all 3000).
+    FP64 error: the gather of the median-width window against the FP64
+    oracle over the same steps."""
     import seismic_code_github_amd as sfd
     from seismic_code_github_amd import workloads as W
     _, shots = W.windowed_shots(nshot=64, nt=nt)
     cells = sum(s.vp.size for s in shots)
     bytes_step = sum(algorithmic_bytes_per_step(s) for s in shots)
     b = sfd.AcousticBatch(shots)
-    tuned_us = None if no_tune else b.autotune(20)
+    # 60 trial steps: every shot group (the batch runs its shots in groups
+    # sized to the MALL) starts its trial from cold caches.
+    tuned_us = None if no_tune else b.autotune(60)
     kind, ta, tb = b.shape()
+    ngroups = b.groups()
+    log(f"batched leg: shape {shape_dict(kind, ta, tb, tuned_us)}, {ngroups} shot groups")
     if plain and not no_tune:
         # The plain path runs every shot at its own grid class's shape: tune
         # each class once on one of its shots.
@@ -172,8 +178,10 @@ def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
                 seen.add(cls)
                 with sfd.AcousticPlan(s) as pl:
                     pl.autotune(20)
-    b.run(sfd.BATCH_GRAPH)
-    g_ms = min(b.run(sfd.BATCH_GRAPH) for _ in range(2))
+    how = sfd.BATCH_LAUNCHES if launches else sfd.BATCH_GRAPH
+    b.run(how)
+    g_ms = min(b.run(how) for _ in range(2))
+    log(f"batched leg: {'graph' if how == sfd.BATCH_GRAPH else 'launches'} {g_ms:.1f} ms for {nt} steps")
     order = np.argsort([s.vp.shape[0] for s in shots])
     mid = int(order[len(shots) // 2])
     rec_mid = b.fetch(mid)
@@ -181,7 +189,7 @@ def batched_leg(nt, no_tune=False, no_cpu=False, plain=True):
     b.close()
     scale = 3000.0 / nt
     out = {"workload": "64 shots, windows of 1000-2500 columns of a 1001x6001 model, C-PML 50, R=8, nt=3000",
-           "cells": int(cells), "timed_steps": nt,
+           "cells": int(cells), "timed_steps": nt, "shot_groups": ngroups,
            "graph_Mcell_per_s": round(cells * nt / g_ms / 1e3, 1),
            "plain_Mcell_per_s": round(cells * nt / p_ms / 1e3, 1),
            "graph_shots_per_s": round(64 / (g_ms * scale / 1e3), 3),
@@ -732,6 +740,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-launches", action="store_true",
+                    help="batched leg: plain launches instead of the graph replay (kernel-trace runs)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched leg: skip the shot-after-shot comparison (counter runs)")
     ap.add_argument("--no-visco", action="store_true")
@@ -794,7 +804,7 @@ def main():
         leg = {"visco": lambda: visco_leg(args.visco_err_nt, args.visco_nt, args.no_cpu, args.visco_batch),
                "grad": lambda: gradient_leg(args.grad_steps, args.no_cpu),
                "batch": lambda: batched_leg(args.batch_steps, args.no_tune or tuned_cache, args.no_cpu,
-                                            not args.no_plain),
+                                            not args.no_plain, args.batch_launches),
                "c0": lambda: c0_leg(args.no_cpu, args.no_tune),
                "elastic": lambda: elastic_leg(args.elastic_err_nt, args.elastic_nt, args.no_cpu, args.elastic_batch),
                "vti": lambda: vti_leg(args.vti_shots, args.vti_nt, args.vti_err_nt, args.no_cpu,

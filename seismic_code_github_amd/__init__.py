@@ -641,6 +641,10 @@ class AcousticBatch:
         -1 = every shot in one group); returns the number of groups."""
         v = (1 << 64) - 1 if nbytes < 0 else int(nbytes)
         self.ctx.check(lib().sfd_acoustic_batch_set_group_bytes(self.handle, v), "batch set_group_bytes")
+        return self.groups()
+
+    def groups(self) -> int:
+        """Number of shot groups the batch runs one after another."""
         n = ctypes.c_int()
         self.ctx.check(lib().sfd_acoustic_batch_get_groups(self.handle, ctypes.byref(n)), "batch get_groups")
         return n.value
