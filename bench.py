@@ -335,6 +335,7 @@ def visco_leg(nt_err, nt, no_cpu, nbatch=256):
         bnx, bnz = s0.vp.shape
         bsteps = s0.ref_delay() + s0.nt
         with sfd.ViscoBatch(shots) as b:
+            vgroups = b.groups()
             b.run()
             bms = min(b.run() for _ in range(2))
             brec = b.fetch(0)[0]
@@ -354,7 +355,8 @@ def visco_leg(nt_err, nt, no_cpu, nbatch=256):
                                     f"401x401 two-layer synthetic model, offsets 0-50 m, 200 m sponges), each "
                                     f"{bnx}x{bnz}, visco-elastic, L={Lb}, nt={s0.nt} (+{s0.ref_delay()} delay steps), "
                                     f"10 Hz",
-                        "shots": nbatch, "ms_per_batch": round(bms, 2), "shots_per_s": round(nbatch / (bms / 1e3), 1),
+                        "shots": nbatch, "shot_groups": vgroups, "ms_per_batch": round(bms, 2),
+                        "shots_per_s": round(nbatch / (bms / 1e3), 1),
                         "plain_shots_per_s": round(nbatch / (sms / 1e3), 1), "speedup": round(sms / bms, 2),
                         "Mcell_per_s": round(nbatch * bnx * bnz * bsteps / bms / 1e3, 1),
                         "us_per_step": round(bms * 1e3 / bsteps, 2),
@@ -451,6 +453,7 @@ def elastic_leg(nt_err, nt, no_cpu, nbatch=32):
         with sfd.ElasticBatch(shots) as b:
             btuned = b.autotune(20)
             bshape = b.shape()
+            bgroups = b.groups()
             b.run()
             bms = min(b.run() for _ in range(2))
             brec = b.fetch(0)[:2]
@@ -458,7 +461,7 @@ def elastic_leg(nt_err, nt, no_cpu, nbatch=32):
         bbytes = bytes_step * nbatch
         out["batch"] = {"workload": f"{nbatch} shots of the card's survey (shots 1..{nbatch}, dsx = 100 m), "
                                     f"each on the {nx}x{nz} shot grid",
-                        "shots": nbatch, "ms_per_batch": round(bms, 2),
+                        "shots": nbatch, "shot_groups": bgroups, "ms_per_batch": round(bms, 2),
                         "shots_per_s": round(nbatch / (bms / 1e3), 2),
                         "Mcell_per_s": round(nbatch * nx * nz * nsteps / bms / 1e3, 1),
                         "us_per_step": round(bms * 1e3 / nsteps, 2),
@@ -533,6 +536,7 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
         elif not no_tune:
             tuned_us = pl.autotune(20)
         shape = pl.shape()
+        vgroups = pl.groups()
         pl.run()
         ms = pl.run()
         rec = pl.fetch()
@@ -543,7 +547,8 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
     out = {"workload": f"VTI qP-acoustic (p, q; 2nd order in time, 10th order in space, split PML), "
                        f"vti_qa_2d_modelling_new.par survey: {nshot} shots of {nx}x{nz} (600 window columns "
                        f"+ 50-cell PML left, right, below), dt=0.5 ms, nt={nt} (+140 delay steps), 30 Hz",
-           "shots": nshot, "cells_per_shot": nx * nz, "steps": nsteps, "ms_per_survey": round(ms, 1),
+           "shots": nshot, "shot_groups": vgroups, "cells_per_shot": nx * nz, "steps": nsteps,
+           "ms_per_survey": round(ms, 1),
            "shots_per_s": round(nshot / s, 2),
            "Mcell_per_s": round(nx * nz * nshot * nsteps / s / 1e6, 1),
            "us_per_step": round(ms * 1e3 / nsteps, 2),

@@ -389,6 +389,11 @@ int sfd_visco_batch_run(sfd_visco_batch *batch, float *elapsed_ms);
  * snapshots (may be NULL), as sfd_visco_plan_fetch / _fetch_xx. */
 int sfd_visco_batch_fetch(sfd_visco_batch *batch, int shot, float *record, float *record_xx, float *snaps);
 int sfd_visco_batch_destroy(sfd_visco_batch *batch);
+/* Shot groups, as for acoustic batches (sfd_acoustic_batch_set_group_bytes):
+ * the budget of one group's fields (state, model, strip parts); 0 = the
+ * default (192 MiB), SIZE_MAX = one group; bitwise the same results. */
+int sfd_visco_batch_set_group_bytes(sfd_visco_batch *batch, size_t bytes);
+int sfd_visco_batch_get_groups(const sfd_visco_batch *batch, int *ngroups);
 
 /* ---------------------------------------------------------------- elastic
  * Isotropic elastic standard-staggered-grid propagator with a free surface
@@ -484,6 +489,11 @@ int sfd_elastic_batch_set_tiling(sfd_elastic_batch *batch, int tile_cols, int wa
  * shot count and cells per step to the nearest power of 2); resets the state. */
 int sfd_elastic_batch_autotune(sfd_elastic_batch *batch, int trial_steps, float *best_us);
 int sfd_elastic_batch_get_shape(const sfd_elastic_batch *batch, int *tile_cols, int *waves);
+/* Shot groups, as for acoustic batches: the budget of one group's fields
+ * (state, staggered model, strip parts); 0 = the default (256 MiB),
+ * SIZE_MAX = one group; bitwise the same results. */
+int sfd_elastic_batch_set_group_bytes(sfd_elastic_batch *batch, size_t bytes);
+int sfd_elastic_batch_get_groups(const sfd_elastic_batch *batch, int *ngroups);
 /* Zeroes every shot's fields and runs all delay + nt steps; elapsed_ms as
  * sfd_elastic_plan_run.  SFD_ENAN when a gather is not finite. */
 int sfd_elastic_batch_run(sfd_elastic_batch *batch, float *elapsed_ms);
@@ -547,6 +557,14 @@ int sfd_vti_plan_set_tiling(sfd_vti_plan *plan, int tile_cols, int waves);
  * resets the state. */
 int sfd_vti_plan_autotune(sfd_vti_plan *plan, int trial_steps, float *best_us);
 int sfd_vti_plan_get_shape(const sfd_vti_plan *plan, int *tile_cols, int *waves);
+/* Shot groups (no counterpart in the reference), as for acoustic batches:
+ * the plan's shots run in groups of consecutive shots, each group for all
+ * its steps before the next, so that a group's fields (p, q at two levels,
+ * vz, eps, delta, PML parts) stay in the 256 MB MALL.  bytes = the budget of
+ * one group (0 = the default, 192 MiB; SIZE_MAX = every shot in one group);
+ * results are bitwise the same for any grouping. */
+int sfd_vti_plan_set_group_bytes(sfd_vti_plan *plan, size_t bytes);
+int sfd_vti_plan_get_groups(const sfd_vti_plan *plan, int *ngroups);
 int sfd_vti_plan_destroy(sfd_vti_plan *plan);
 /* Host helpers (no device or context needed):
  * coefficient_2nd (kind 2) / coefficient_1st (kind 1) with k <= 8 weights, solved in

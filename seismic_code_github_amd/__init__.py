@@ -240,6 +240,12 @@ _SIGNATURES = {
     "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_acoustic_batch_set_group_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
     "sfd_acoustic_batch_get_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "sfd_vti_plan_set_group_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
+    "sfd_vti_plan_get_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "sfd_visco_batch_set_group_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
+    "sfd_visco_batch_get_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "sfd_elastic_batch_set_group_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
+    "sfd_elastic_batch_get_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "sfd_visco_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ViscoParamsC),
                                               ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.POINTER(_FP),
                                               ctypes.POINTER(_FP), ctypes.POINTER(_FP),
@@ -869,6 +875,18 @@ class ViscoBatch:
                                                     arrs[4], ctypes.byref(h)), "sfd_visco_batch_create")
         self.handle = h
 
+    def set_group_bytes(self, nbytes: int = 0) -> int:
+        """Budget of one shot group's fields (0 = the default, -1 = every shot
+        in one group); returns the number of groups."""
+        v = (1 << 64) - 1 if nbytes < 0 else int(nbytes)
+        self.ctx.check(lib().sfd_visco_batch_set_group_bytes(self.handle, v), "visco_batch set_group_bytes")
+        return self.groups()
+
+    def groups(self) -> int:
+        n = ctypes.c_int()
+        self.ctx.check(lib().sfd_visco_batch_get_groups(self.handle, ctypes.byref(n)), "visco_batch get_groups")
+        return n.value
+
     def run(self) -> float:
         ms = ctypes.c_float()
         self.ctx.check(lib().sfd_visco_batch_run(self.handle, ctypes.byref(ms)), "sfd_visco_batch_run")
@@ -1096,6 +1114,18 @@ class ElasticBatch:
                        "elastic batch get_shape")
         return a.value, b.value
 
+    def set_group_bytes(self, nbytes: int = 0) -> int:
+        """Budget of one shot group's fields (0 = the default, -1 = every shot
+        in one group); returns the number of groups."""
+        v = (1 << 64) - 1 if nbytes < 0 else int(nbytes)
+        self.ctx.check(lib().sfd_elastic_batch_set_group_bytes(self.handle, v), "elastic_batch set_group_bytes")
+        return self.groups()
+
+    def groups(self) -> int:
+        n = ctypes.c_int()
+        self.ctx.check(lib().sfd_elastic_batch_get_groups(self.handle, ctypes.byref(n)), "elastic_batch get_groups")
+        return n.value
+
     def run(self) -> float:
         ms = ctypes.c_float()
         self.ctx.check(lib().sfd_elastic_batch_run(self.handle, ctypes.byref(ms)), "sfd_elastic_batch_run")
@@ -1245,6 +1275,18 @@ class VtiPlan:
         a, b = ctypes.c_int(), ctypes.c_int()
         self.ctx.check(lib().sfd_vti_plan_get_shape(self.handle, ctypes.byref(a), ctypes.byref(b)), "vti get_shape")
         return a.value, b.value
+
+    def set_group_bytes(self, nbytes: int = 0) -> int:
+        """Budget of one shot group's fields (0 = the default, -1 = every shot
+        in one group); returns the number of groups."""
+        v = (1 << 64) - 1 if nbytes < 0 else int(nbytes)
+        self.ctx.check(lib().sfd_vti_plan_set_group_bytes(self.handle, v), "vti set_group_bytes")
+        return self.groups()
+
+    def groups(self) -> int:
+        n = ctypes.c_int()
+        self.ctx.check(lib().sfd_vti_plan_get_groups(self.handle, ctypes.byref(n)), "vti get_groups")
+        return n.value
 
     def run(self) -> float:
         ms = ctypes.c_float()

@@ -47,6 +47,14 @@ struct sfd_elastic_batch {
     void (*vel)(sfd::ElasticBatchArgs) = nullptr;
     void (*str)(sfd::ElasticBatchArgs) = nullptr;
     sfd::DeviceBuffer shot_args, tile_map;
+    // Groups of consecutive shots run one after another, each for all its
+    // steps, so that a group's fields stay in the 256 MB MALL: shots
+    // [s0, s1), tiles from tile_off (shot indices group-local).
+    struct Group {
+        int s0, s1, tile_off, ntile;
+    };
+    std::vector<Group> groups;
+    size_t group_bytes = 0;  // budget of one group's fields; 0 = the default
     ~sfd_elastic_batch() {
         for (auto* pl : plans) sfd_elastic_plan_destroy(pl);
     }
@@ -312,19 +320,40 @@ int elastic_enqueue(sfd_elastic_plan* pl, int nsteps = -1) {
     return SFD_OK;
 }
 
+// Default budget of one shot group's fields (state, staggered model, strip
+// parts, the pending vz row): measured on MI355X with the parameter file's
+// 32-shot survey (per step: one group 516, 160 MiB 428, 192 MiB 422, 256 MiB
+// 408 us).
+constexpr size_t kElGroupBytes = (size_t)256 << 20;
+
 // Batch tables: every shot's argument block (at tile_cols x waves) and the
-// (shot, tile) of every global tile.
+// (group-local shot, tile) of every tile, group after group.
 int el_batch_set_tiling(sfd_elastic_batch* b, int tx, int nw) {
     ElasticBatchKernel v, s;
     if (!el_batch_pick(tx, nw, b->abc, &v, &s))
         return fail(b->ctx, SFD_EINVAL, "no elastic kernel for %d tile columns x %d waves", tx, nw);
+    const int n = (int)b->plans.size();
+    const size_t budget = b->group_bytes ? b->group_bytes : kElGroupBytes;
+    b->groups.clear();
+    for (int s0 = 0; s0 < n;) {
+        size_t used = 0;
+        int e = s0;
+        do used += b->plans[e++]->arena.bytes;
+        while (e < n && used + b->plans[e]->arena.bytes <= budget);
+        b->groups.push_back({s0, e, 0, 0});
+        s0 = e;
+    }
     std::vector<ElasticStepArgs> args;
     std::vector<int2> map;
-    for (size_t i = 0; i < b->plans.size(); ++i) {
-        const int rc = el_set_tiling(b->plans[i], tx, nw);
-        if (rc) return rc;
-        args.push_back(b->plans[i]->args);
-        for (int t = 0; t < b->plans[i]->args.ntiles; ++t) map.push_back(make_int2((int)i, t));
+    for (auto& g : b->groups) {
+        g.tile_off = (int)map.size();
+        for (int i = g.s0; i < g.s1; ++i) {
+            const int rc = el_set_tiling(b->plans[i], tx, nw);
+            if (rc) return rc;
+            args.push_back(b->plans[i]->args);
+            for (int t = 0; t < b->plans[i]->args.ntiles; ++t) map.push_back(make_int2(i - g.s0, t));
+        }
+        g.ntile = (int)map.size() - g.tile_off;
     }
     hipStream_t st = b->ctx->stream;
     hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
@@ -347,26 +376,30 @@ int el_batch_zero(sfd_elastic_batch* b) {
     return SFD_OK;
 }
 
-// Enqueues the first nsteps steps of every shot (all of them by default).
+// Enqueues the first nsteps steps of every shot (all of them by default),
+// shot group after shot group.
 int el_batch_enqueue(sfd_elastic_batch* b, int nsteps = -1) {
     sfd_context* ctx = b->ctx;
-    const dim3 grid(b->ntile), block(64 * b->nw);
-    ElasticBatchArgs a{};
-    a.shots = b->shot_args.as<ElasticStepArgs>();
-    a.tile_map = b->tile_map.as<int2>();
-    a.ntile = b->ntile;
-    std::vector<size_t> snap_next(b->plans.size(), 0);
+    const dim3 block(64 * b->nw);
     const int ns = nsteps < 0 ? b->nsteps : std::min(nsteps, b->nsteps);
-    for (int s = 0; s < ns; ++s) {
-        a.step = s;
-        a.rec_col = s >= b->delay ? s - b->delay : -1;
-        hipLaunchKernelGGL(b->vel, grid, block, 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(b->str, grid, block, 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
-        for (size_t i = 0; i < b->plans.size(); ++i) {
-            const int rc = elastic_snaps(b->plans[i], s, &snap_next[i]);
-            if (rc) return rc;
+    for (const auto& g : b->groups) {
+        const dim3 grid(g.ntile);
+        ElasticBatchArgs a{};
+        a.shots = b->shot_args.as<ElasticStepArgs>() + g.s0;
+        a.tile_map = b->tile_map.as<int2>() + g.tile_off;
+        a.ntile = g.ntile;
+        std::vector<size_t> snap_next(g.s1 - g.s0, 0);
+        for (int s = 0; s < ns; ++s) {
+            a.step = s;
+            a.rec_col = s >= b->delay ? s - b->delay : -1;
+            hipLaunchKernelGGL(b->vel, grid, block, 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(b->str, grid, block, 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            for (int i = g.s0; i < g.s1; ++i) {
+                const int rc = elastic_snaps(b->plans[i], s, &snap_next[i - g.s0]);
+                if (rc) return rc;
+            }
         }
     }
     return SFD_OK;
@@ -853,6 +886,18 @@ int sfd_elastic_batch_autotune(sfd_elastic_batch* b, int trial_steps, float* bes
     t.us = us[best];
     ctx->tuning[elastic_batch_class(b)] = t;
     if (best_us) *best_us = us[best];
+    return SFD_OK;
+}
+
+int sfd_elastic_batch_set_group_bytes(sfd_elastic_batch* b, size_t bytes) {
+    if (!b) return SFD_EINVAL;
+    b->group_bytes = bytes;
+    return el_batch_set_tiling(b, b->tx, b->nw);
+}
+
+int sfd_elastic_batch_get_groups(const sfd_elastic_batch* b, int* ngroups) {
+    if (!b || !ngroups) return SFD_EINVAL;
+    *ngroups = (int)b->groups.size();
     return SFD_OK;
 }
 

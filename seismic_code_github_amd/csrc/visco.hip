@@ -57,6 +57,14 @@ struct sfd_visco_batch {
     void (*vel)(sfd::ViscoBatchArgs) = nullptr;
     void (*str)(sfd::ViscoBatchArgs) = nullptr;
     sfd::DeviceBuffer shot_args, tile_map;
+    // Groups of consecutive shots run one after another, each for all its
+    // steps, so that a group's fields stay in the 256 MB MALL: shots
+    // [s0, s1), tiles from tile_off (shot indices group-local).
+    struct Group {
+        int s0, s1, tile_off, ntile;
+    };
+    std::vector<Group> groups;
+    size_t group_bytes = 0;  // budget of one group's fields; 0 = the default
     ~sfd_visco_batch() {
         for (auto* pl : plans) sfd_visco_plan_destroy(pl);
     }
@@ -484,32 +492,76 @@ int ve_enqueue(sfd_visco_plan* pl, int nsteps = -1) {
     return ve_last_gather(pl, ns);
 }
 
-// A batch's steps: one launch of each half-step over every shot's tiles.
+// A batch's steps, shot group after shot group: one launch of each
+// half-step over the group's tiles.
 int ve_batch_enqueue(sfd_visco_batch* b, int nsteps = -1) {
     sfd_context* ctx = b->ctx;
-    const dim3 grid(b->ntile), block(64 * kVeNW);
-    ViscoBatchArgs a{};
-    a.shots = b->shot_args.as<ViscoElArgs>();
-    a.tile_map = b->tile_map.as<int2>();
-    a.ntile = b->ntile;
-    std::vector<size_t> snap_next(b->plans.size(), 0);
+    const dim3 block(64 * kVeNW);
     const int ns = nsteps < 0 ? b->nsteps : std::min(nsteps, b->nsteps);
-    for (int k = 0; k < ns; ++k) {
-        a.step = k;
-        a.rec_col = (b->nt > 0 && k - 1 >= b->delay) ? k - 1 - b->delay : -1;
-        hipLaunchKernelGGL(b->vel, grid, block, 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(b->str, grid, block, 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
-        for (size_t i = 0; i < b->plans.size(); ++i) {
-            const int rc = ve_snaps(b->plans[i], k, &snap_next[i]);
+    for (const auto& g : b->groups) {
+        const dim3 grid(g.ntile);
+        ViscoBatchArgs a{};
+        a.shots = b->shot_args.as<ViscoElArgs>() + g.s0;
+        a.tile_map = b->tile_map.as<int2>() + g.tile_off;
+        a.ntile = g.ntile;
+        std::vector<size_t> snap_next(g.s1 - g.s0, 0);
+        for (int k = 0; k < ns; ++k) {
+            a.step = k;
+            a.rec_col = (b->nt > 0 && k - 1 >= b->delay) ? k - 1 - b->delay : -1;
+            hipLaunchKernelGGL(b->vel, grid, block, 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(b->str, grid, block, 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            for (int i = g.s0; i < g.s1; ++i) {
+                const int rc = ve_snaps(b->plans[i], k, &snap_next[i - g.s0]);
+                if (rc) return rc;
+            }
+        }
+        for (int i = g.s0; i < g.s1; ++i) {
+            const int rc = ve_last_gather(b->plans[i], ns);
             if (rc) return rc;
         }
     }
-    for (auto* pl : b->plans) {
-        const int rc = ve_last_gather(pl, ns);
-        if (rc) return rc;
+    return SFD_OK;
+}
+
+// Default budget of one shot group's fields (state, model, strip parts):
+// measured on MI355X with the Q_forward card's survey.
+constexpr size_t kVeGroupBytes = (size_t)192 << 20;
+
+// Shot groups and the tile tables (group-local shot indices).
+int ve_batch_tables(sfd_visco_batch* b) {
+    const int n = (int)b->plans.size();
+    const size_t budget = b->group_bytes ? b->group_bytes : kVeGroupBytes;
+    auto bytes = [&](int i) {
+        const sfd_visco_plan* pl = b->plans[i];
+        return pl->ve_arena.bytes + pl->ve_xs.bytes + pl->ve_zs.bytes;
+    };
+    b->groups.clear();
+    for (int s0 = 0; s0 < n;) {
+        size_t used = 0;
+        int e = s0;
+        do used += bytes(e++);
+        while (e < n && used + bytes(e) <= budget);
+        b->groups.push_back({s0, e, 0, 0});
+        s0 = e;
     }
+    std::vector<ViscoElArgs> args;
+    std::vector<int2> map;
+    for (auto& g : b->groups) {
+        g.tile_off = (int)map.size();
+        for (int i = g.s0; i < g.s1; ++i) {
+            args.push_back(b->plans[i]->ve);
+            for (int t = 0; t < b->plans[i]->ve.ntiles; ++t) map.push_back(make_int2(i - g.s0, t));
+        }
+        g.ntile = (int)map.size() - g.tile_off;
+    }
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
+    if (e == hipSuccess) e = upload(b->tile_map, map.data(), map.size(), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_check(b->ctx, e, "visco batch tables");
+    b->ntile = (int)map.size();
     return SFD_OK;
 }
 
@@ -1045,23 +1097,24 @@ int sfd_visco_batch_create(sfd_context* ctx, int nshot, const sfd_visco_params* 
         delete b;
         return fail(ctx, SFD_ENOTSUP, "no batched visco kernel for R = %d, L = %d", p0->radius, p0->nmech);
     }
-    std::vector<ViscoElArgs> args;
-    std::vector<int2> map;
-    for (size_t i = 0; i < b->plans.size(); ++i) {
-        args.push_back(b->plans[i]->ve);
-        for (int t = 0; t < b->plans[i]->ve.ntiles; ++t) map.push_back(make_int2((int)i, t));
-    }
-    hipStream_t st = ctx->stream;
-    hipError_t e = upload(b->shot_args, args.data(), args.size(), st);
-    if (e == hipSuccess) e = upload(b->tile_map, map.data(), map.size(), st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-        const int rc = hip_check(ctx, e, "visco batch tables");
+    const int rc = ve_batch_tables(b);
+    if (rc) {
         delete b;
         return rc;
     }
-    b->ntile = (int)map.size();
     *out = b;
+    return SFD_OK;
+}
+
+int sfd_visco_batch_set_group_bytes(sfd_visco_batch* b, size_t bytes) {
+    if (!b) return SFD_EINVAL;
+    b->group_bytes = bytes;
+    return ve_batch_tables(b);
+}
+
+int sfd_visco_batch_get_groups(const sfd_visco_batch* b, int* ngroups) {
+    if (!b || !ngroups) return SFD_EINVAL;
+    *ngroups = (int)b->groups.size();
     return SFD_OK;
 }
 

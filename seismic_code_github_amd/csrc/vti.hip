@@ -63,6 +63,7 @@ struct VtiArgs {
     int nrec, nt, sample;         // sample < 0: nothing to record
     const int *rx, *rz;
     float* rec;                   // [shot][nrec][nt]
+    int shot0;                    // shot of blockIdx.y = 0 (shot groups)
 };
 
 // p of the previous step's output at the receivers.
@@ -73,7 +74,7 @@ __device__ __forceinline__ void vti_record(const VtiArgs& a, int shot, int tid, 
         a.rec[((size_t)shot * a.nrec + r) * a.nt + a.sample] = u[(ptrdiff_t)a.rx[r] * a.pitch + a.rz[r]];
 }
 
-__global__ void vti_gather_kernel(VtiArgs a) { vti_record(a, blockIdx.y, threadIdx.x, blockDim.x); }
+__global__ void vti_gather_kernel(VtiArgs a) { vti_record(a, a.shot0 + blockIdx.y, threadIdx.x, blockDim.x); }
 
 // The split-PML recursions (sub:651-663 and 1141-1153), slots [old, cur] of
 // P1 (slot base 0) and Pt (slot base 2), P2 single-level (slot 4); the old
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
     __shared__ float s_s[WX * kVTZ];  // p + q: tile columns and the x halo
     __shared__ float s_u[TX * HZ];    // p: tile columns, rows with the z halo
     __shared__ float s_q[TX * kVTZ];  // q: tile
-    const int shot = blockIdx.y;
+    const int shot = a.shot0 + blockIdx.y;
     if (blockIdx.x == 0) vti_record(a, shot, threadIdx.x, T);
     const int t = xcd_tile(blockIdx.x, a.ntiles);
     const int z0 = (t % a.ntz) * kVTZ, x0 = (t / a.ntz) * TX;
@@ -339,6 +340,10 @@ struct sfd_vti_plan {
     float* q[2] = {nullptr, nullptr};
     size_t state_bytes = 0;
     std::vector<float> wav;  // [nsteps + 1], zero past the wavelet
+    // Shots run in groups of gshot consecutive shots, each group for all its
+    // steps before the next, so a group's fields stay in the 256 MB MALL.
+    int gshot = 0;
+    size_t group_bytes = 0;  // budget of one group's fields; 0 = the default
     VtiArgs args{};
     VtiKernel kern = nullptr;
 };
@@ -368,32 +373,49 @@ int vti_zero(sfd_vti_plan* pl) {
     return SFD_OK;
 }
 
-// Enqueues the first nsteps steps (all of them by default) from a zero state.
+// Default budget of one shot group's fields (p, q at two levels, vz, eps,
+// delta and the PML parts): measured on MI355X with the VTI card's survey.
+constexpr size_t kVtiGroupBytes = (size_t)192 << 20;
+
+// Shots per group for the plan's budget (at least one).
+void vti_set_groups(sfd_vti_plan* pl) {
+    const size_t per = pl->fs * 7 * sizeof(float) + (12 * (size_t)pl->args.nxs + 7 * (size_t)pl->args.nzs) * sizeof(float);
+    const size_t budget = pl->group_bytes ? pl->group_bytes : kVtiGroupBytes;
+    pl->gshot = (int)std::max<size_t>(1, std::min<size_t>(pl->nshot, budget / std::max<size_t>(per, 1)));
+}
+
+// Enqueues the first nsteps steps (all of them by default) from a zero
+// state, shot group after shot group.
 int vti_enqueue(sfd_vti_plan* pl, int nsteps = -1) {
     sfd_context* ctx = pl->ctx;
     const int ns = nsteps < 0 ? pl->nsteps : std::min(nsteps, pl->nsteps);
-    const dim3 grid(pl->args.ntiles, pl->nshot), block(64 * pl->nw);
-    VtiArgs a = pl->args;
-    int cur = 1;
-    for (int s = 0; s < ns; ++s) {
-        const int old = 1 - cur;
-        a.uc = pl->u[cur];
-        a.qc = pl->q[cur];
-        a.un = pl->u[old];
-        a.qn = pl->q[old];
-        a.par = s & 1;
-        a.w_cur = pl->wav[s];
-        a.w_prev = s > 0 ? pl->wav[s - 1] : 0.f;
-        a.sample = (s - 1 >= pl->delay && s - 1 - pl->delay < pl->nt) ? s - 1 - pl->delay : -1;
-        hipLaunchKernelGGL(pl->kern, grid, block, 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
-        cur = old;
-    }
-    if (ns > 0 && ns - 1 >= pl->delay && pl->nrec > 0) {
-        a.uc = pl->u[cur];
-        a.sample = ns - 1 - pl->delay < pl->nt ? ns - 1 - pl->delay : -1;
-        hipLaunchKernelGGL(vti_gather_kernel, dim3(1, pl->nshot), dim3(256), 0, ctx->stream, a);
-        SFD_HIP(ctx, hipGetLastError());
+    const dim3 block(64 * pl->nw);
+    for (int g0 = 0; g0 < pl->nshot; g0 += pl->gshot) {
+        const int gn = std::min(pl->gshot, pl->nshot - g0);
+        const dim3 grid(pl->args.ntiles, gn);
+        VtiArgs a = pl->args;
+        a.shot0 = g0;
+        int cur = 1;
+        for (int s = 0; s < ns; ++s) {
+            const int old = 1 - cur;
+            a.uc = pl->u[cur];
+            a.qc = pl->q[cur];
+            a.un = pl->u[old];
+            a.qn = pl->q[old];
+            a.par = s & 1;
+            a.w_cur = pl->wav[s];
+            a.w_prev = s > 0 ? pl->wav[s - 1] : 0.f;
+            a.sample = (s - 1 >= pl->delay && s - 1 - pl->delay < pl->nt) ? s - 1 - pl->delay : -1;
+            hipLaunchKernelGGL(pl->kern, grid, block, 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+            cur = old;
+        }
+        if (ns > 0 && ns - 1 >= pl->delay && pl->nrec > 0) {
+            a.uc = pl->u[cur];
+            a.sample = ns - 1 - pl->delay < pl->nt ? ns - 1 - pl->delay : -1;
+            hipLaunchKernelGGL(vti_gather_kernel, dim3(1, gn), dim3(256), 0, ctx->stream, a);
+            SFD_HIP(ctx, hipGetLastError());
+        }
     }
     return SFD_OK;
 }
@@ -602,6 +624,7 @@ int sfd_vti_plan_create(sfd_context* ctx, const sfd_vti_params* p, int nshot, co
     a.rx = pl->rx.as<int>();
     a.rz = pl->rz.as<int>();
     a.rec = pl->rec.as<float>();
+    vti_set_groups(pl);
     int rc = vti_set_tiling(pl, kVtiDefaultTX, kVtiDefaultNW);
     if (rc) return bail(rc);
     const auto tuned = ctx->tuning.find(vti_class(pl));
@@ -617,6 +640,19 @@ int sfd_vti_plan_create(sfd_context* ctx, const sfd_vti_params* p, int nshot, co
 int sfd_vti_plan_set_tiling(sfd_vti_plan* pl, int tile_cols, int waves) {
     if (!pl) return SFD_EINVAL;
     return vti_set_tiling(pl, tile_cols ? tile_cols : kVtiDefaultTX, waves ? waves : kVtiDefaultNW);
+}
+
+int sfd_vti_plan_set_group_bytes(sfd_vti_plan* pl, size_t bytes) {
+    if (!pl) return SFD_EINVAL;
+    pl->group_bytes = bytes;
+    vti_set_groups(pl);
+    return SFD_OK;
+}
+
+int sfd_vti_plan_get_groups(const sfd_vti_plan* pl, int* ngroups) {
+    if (!pl || !ngroups) return SFD_EINVAL;
+    *ngroups = (pl->nshot + pl->gshot - 1) / pl->gshot;
+    return SFD_OK;
 }
 
 int sfd_vti_plan_get_shape(const sfd_vti_plan* pl, int* tile_cols, int* waves) {

@@ -252,6 +252,14 @@ def test_batch_equals_single_plans(ctx, mode):
         b.run()
         for i in range(len(shots)):
             np.testing.assert_array_equal(b.fetch(i)[0], singles[i][0])
+        # shot groups: one shot per group and one group (bitwise the same)
+        for budget, want in ((1, len(shots)), (-1, 1)):
+            assert b.set_group_bytes(budget) == want
+            b.run()
+            for i in range(len(shots)):
+                got = b.fetch(i)
+                for g, w, what in zip(got, singles[i], ("vx", "vz", "snaps")):
+                    np.testing.assert_array_equal(g, w, err_msg=f"{mode} {want} groups shot {i} {what}")
 
 
 def test_batch_rejects_mixed_shots(ctx):

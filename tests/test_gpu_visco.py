@@ -274,3 +274,21 @@ def test_visco_batch_rejects_mixed_shots(ctx):
     for other in (b_ac, c_nt):
         with pytest.raises(sfd.SeisfdError, match="EINVAL"):
             sfd.ViscoBatch([a, other], ctx)
+
+
+def test_visco_batch_shot_groups_bitwise(ctx):
+    """Shot groups (sfd_visco_batch_set_group_bytes): one shot per group, a
+    middle budget and one group give the same gathers bit for bit."""
+    shots = W.visco_card_survey(nshot=5, nt=200, first=40)
+    print("case: visco batch shot groups", flush=True)
+    out = {}
+    with sfd.ViscoBatch(shots, ctx) as b:
+        for budget in (-1, 1, 3 << 20, 6 << 20):
+            ng = b.set_group_bytes(budget)
+            b.run()
+            out[ng] = [b.fetch(i)[:2] for i in range(len(shots))]
+    assert 1 in out and 5 in out and len(out) >= 3, sorted(out)
+    for ng, recs in out.items():
+        for i in range(len(shots)):
+            np.testing.assert_array_equal(recs[i][0], out[1][i][0], err_msg=f"{ng} groups, shot {i} zz")
+            np.testing.assert_array_equal(recs[i][1], out[1][i][1], err_msg=f"{ng} groups, shot {i} xx")

@@ -215,3 +215,27 @@ def test_eps_below_delta_rejected_then_watchdog():
         assert np.all(np.isfinite(ok.fetch()))
         pl.close()
         ok.close()
+
+
+def test_shot_groups_bitwise():
+    """Shot groups (sfd_vti_plan_set_group_bytes): one shot per group, two
+    groups and one group give the same records and final p, q bit for bit."""
+    nx, nz = 130, 100
+    models = [model(nx, nz, 40 + 3 * s, 7, (20, 20, 0, 20)) for s in range(5)]
+    V, E, D = (np.stack([m[k] for m in models]) for k in range(3))
+    kw = dict(dx=5.0, dz=5.0, dt=5e-4, nt=260, src=(65, 0), rec=(np.arange(nx), np.full(nx, 5)),
+              pml=(20, 20, 0, 20))
+    print("case: vti shot groups", flush=True)
+    with sfd.Context(0) as ctx:
+        out = {}
+        with sfd.VtiPlan(V, E, D, ctx=ctx, **kw) as pl:
+            for budget in (-1, 1, 1_500_000, 2_000_000, 3_000_000):
+                ng = pl.set_group_bytes(budget)
+                pl.run()
+                out[ng] = pl.fetch(state=True)
+        assert 1 in out and 5 in out and len(out) >= 3, sorted(out)
+        ref = out[1]
+        assert np.abs(ref[0]).max() > 0
+        for ng, (rec, st) in out.items():
+            np.testing.assert_array_equal(rec, ref[0], err_msg=f"{ng} groups: records")
+            np.testing.assert_array_equal(st, ref[1], err_msg=f"{ng} groups: state")
