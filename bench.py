@@ -142,7 +142,8 @@ def load_traffic(name, key, stem):
         return None
     with open(path) as f:
         t = json.load(f)
-    if stem and not any(stem + "<" in k for k in t.get("kernels", [])):
+    if stem and not any((stem if stem.endswith(">") or "<" in stem else stem + "<") in k
+                        for k in t.get("kernels", [])):
         return None
     return t.get(key)
 
@@ -555,12 +556,10 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
            "bytes_per_shot_step": bstep,
            "achieved_GBs": round(bstep * nshot * nsteps / s / 1e9, 1),
            "frac_of_8TBs": round(bstep * nshot * nsteps / s / 8e12, 4),
-           "launch_shape": {"kernel": "one fused launch per step, all shots", "tile_cols": shape[0],
+           "launch_shape": {"kernel": "one fused launch per step for the shots of a group", "tile_cols": shape[0],
                             "waves": shape[1], "autotuned_us_per_step": None if tuned_us is None else round(tuned_us, 2)}}
-    tpath = os.path.join(ROOT, "profiles", "vti_step_card_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            out["traffic"] = json.load(f).get("hbm_bytes_per_shot_step")
+    out["traffic"] = load_traffic("vti_step_card_traffic.json", "hbm_bytes_per_shot_step",
+                                  f"vti_step_kernel<{shape[0]}, {shape[1]}>")
     if not no_cpu:
         import oracle
         cores = min(16, len(os.sched_getaffinity(0)))
