@@ -535,7 +535,10 @@ def vti_leg(nshot, nt, nt_err, no_cpu, no_tune=False, tiling=None):
         if tiling:
             pl.set_tiling(*tiling)
         elif not no_tune:
-            tuned_us = pl.autotune(20)
+            # 200 trial steps: shorter trials of the shot groups rank the
+            # tilings by their cold-cache starts (20 steps picked 8 x 8 at
+            # 788 us per step in full runs; 200 pick 32 x 8 at 767).
+            tuned_us = pl.autotune(200)
         shape = pl.shape()
         vgroups = pl.groups()
         pl.run()
