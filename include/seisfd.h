@@ -188,7 +188,7 @@ int sfd_acoustic_batch_create(sfd_context *ctx, int nshot, const sfd_acoustic_pa
  * (fast arithmetic) or the march kernel's. */
 int sfd_acoustic_batch_set_tiling(sfd_acoustic_batch *batch, int waves, int chunk_rows);
 /* Tiled step for every shot at tile_rows x 64 columns per workgroup of
- * `waves` (8x4, 16x4, 16x8, 32x8 built); 0 = the default shape, tile_rows < 0
+ * `waves` (8x4, 16x4, 16x8, 32x4, 32x8 built); 0 = the default shape, tile_rows < 0
  * = the march kernel.  Invalidates a captured graph. */
 int sfd_acoustic_batch_set_tiled(sfd_acoustic_batch *batch, int tile_rows, int waves);
 /* One launch per step for every shot (as sfd_acoustic_plan_set_fused, the

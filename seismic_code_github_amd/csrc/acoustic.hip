@@ -241,6 +241,7 @@ bool pick_tiled_r(int tx, int nw, int mode, StepKernel* v, StepKernel* p) {
         SFD_TILED(16, 4)
         SFD_TILED(16, 8)
         SFD_TILED(16, 16)
+        SFD_TILED(32, 4)
         SFD_TILED(32, 8)
         SFD_TILED(32, 16)
 #undef SFD_TILED
@@ -303,6 +304,7 @@ bool pick_tiled_batch_r(int tx, int nw, int mode, BatchKernel* v, BatchKernel* p
     case 804: return tiled_batch_mode<R, 8, 4>(mode, v, p);
     case 1604: return tiled_batch_mode<R, 16, 4>(mode, v, p);
     case 1608: return tiled_batch_mode<R, 16, 8>(mode, v, p);
+    case 3204: return tiled_batch_mode<R, 32, 4>(mode, v, p);
     case 3208: return tiled_batch_mode<R, 32, 8>(mode, v, p);
     default: return false;
     }
@@ -1413,8 +1415,8 @@ namespace {
 std::vector<TunedShape> shape_candidates(int mode, bool batch) {
     std::vector<TunedShape> c;
     if (tiled_mode_ok(mode)) {
-        static const int single[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 8}, {32, 16}};
-        static const int batched[][2] = {{8, 4}, {16, 4}, {16, 8}, {32, 8}};
+        static const int single[][2] = {{8, 4}, {8, 8}, {16, 4}, {16, 8}, {16, 16}, {32, 4}, {32, 8}, {32, 16}};
+        static const int batched[][2] = {{8, 4}, {16, 4}, {16, 8}, {32, 4}, {32, 8}};
         if (batch)
             for (auto& t : batched) c.push_back({0, t[0], t[1], 0.f});
         else

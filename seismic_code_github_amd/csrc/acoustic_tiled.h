@@ -47,29 +47,67 @@ __device__ __forceinline__ void tiled_origin(int t, int ntz, int tx, int R, int*
 }
 
 // Rows [i0 - HR, i0 + TX + HR) x columns [j0 - HC, j0 + 64 + HC) of a field
-// (row 0 at f, rows nz floats apart) into lds[row][col].  With src, the
-// source amplitude is taken off cell (sx, sz) on the way.
+// (f addresses row 0, rows rb bytes apart) into lds[row][col], W = 64 + 2 HC
+// floats a row.  Each wave copies whole rows of the 64 tile columns (lane =
+// column, the row's byte offset a scalar, so no per-lane address arithmetic),
+// then the workgroup copies the 2 HC edge columns of every row.  All loads
+// are issued before the first LDS store.  With SRC, the source amplitude is
+// taken off cell (sx, sz) on the way (same float subtraction as the fused
+// kernel's).  wv is the wave's index, wave-uniform.
 template <int TX, int NW, int HR, int HC, bool SRC>
-__device__ __forceinline__ void tiled_stage(float* lds, const float* f, int nz, int i0, int j0, int sx, int sz,
+__device__ __forceinline__ void tiled_stage(float* lds, const Buf& f, int rb, int i0, int j0, int wv, int sx, int sz,
                                             float amp) {
-    constexpr int W = kTiledTZ + 2 * HC, Hh = TX + 2 * HR, N = W * Hh, T = 64 * NW;
-    constexpr int ITER = (N + T - 1) / T;
-    float v[ITER];
+    constexpr int W = kTiledTZ + 2 * HC, Hh = TX + 2 * HR, T = 64 * NW;
+    constexpr int RI = (Hh + NW - 1) / NW;           // rows per wave
+    constexpr int EW = 2 * HC, E = EW * Hh;          // edge cells
+    constexpr int EWd = EW > 0 ? EW : 1;
+    constexpr int EI = E > 0 ? (E + T - 1) / T : 0;  // edge cells per thread
+    const int lane = threadIdx.x & 63;
+    const uint32_t srow = (uint32_t)(i0 - HR) * (uint32_t)rb;  // byte offset of the first staged row
+    float v[RI];
 #pragma unroll
-    for (int q = 0; q < ITER; ++q) {
+    for (int q = 0; q < RI; ++q) {
+        const int r = wv + q * NW;
+        v[q] = (RI * NW == Hh || r < Hh) ? f.ld(lane * 4, srow + (uint32_t)r * rb + (uint32_t)j0 * 4u) : 0.f;
+    }
+    float ve[EI > 0 ? EI : 1];
+    int le[EI > 0 ? EI : 1];
+#pragma unroll
+    for (int q = 0; q < EI; ++q) {
         const int e = threadIdx.x + q * T;
-        const int r = e / W, c = e - r * W;
-        const int gi = i0 - HR + r, gj = j0 - HC + c;
-        float x = (e < N) ? f[(ptrdiff_t)gi * nz + gj] : 0.f;
-        if (SRC && gi == sx && gj == sz) x -= amp;
-        v[q] = x;
+        const int r = e / EWd, c = e - r * EWd;
+        const int col = c < HC ? c : c + kTiledTZ;  // column in the staged row
+        le[q] = (e < E) ? r * W + col : -1;
+        ve[q] = (e < E) ? f.ld(r * rb + col * 4, srow + (uint32_t)(j0 - HC) * 4u) : 0.f;
+    }
+    if constexpr (SRC) {
+        // Wave-uniform tests first: only the wave (row) or edge holding the source does vector work.
+        const int rs = sx - (i0 - HR), cs = sz - (j0 - HC);
+        if (rs >= 0 && rs < Hh && cs >= 0 && cs < W) {
+            if (cs >= HC && cs < HC + kTiledTZ) {
+#pragma unroll
+                for (int q = 0; q < RI; ++q)
+                    if (wv + q * NW == rs && lane == cs - HC) v[q] -= amp;
+            } else {
+#pragma unroll
+                for (int q = 0; q < EI; ++q)
+                    if (le[q] == rs * W + cs) ve[q] -= amp;
+            }
+        }
     }
 #pragma unroll
-    for (int q = 0; q < ITER; ++q) {
-        const int e = threadIdx.x + q * T;
-        if (e < N) lds[e] = v[q];
+    for (int q = 0; q < RI; ++q) {
+        const int r = wv + q * NW;
+        if (RI * NW == Hh || r < Hh) lds[r * W + HC + lane] = v[q];
     }
+#pragma unroll
+    for (int q = 0; q < EI; ++q)
+        if (le[q] >= 0) lds[le[q]] = ve[q];
 }
+
+// Raw buffer view of a field from its row-0 pointer (every offset the tiled
+// kernels form from it is non-negative and below 2 GiB).
+__device__ __forceinline__ Buf field_buf(const float* p) { return Buf(p, 0x7fffffffu); }
 
 // Sponge factor of cell (i, j): the x-edge rows' factor ax_i overrides the
 // z-edge columns' az_j in the corners (the fused kernel's absorb_row).
@@ -79,7 +117,11 @@ __device__ __forceinline__ float tiled_sponge(int i, int nx, int sw, float ax_i,
 
 // Args is AcousticStepArgs, in kernel arguments or (batched shots) in device
 // memory read through the constant address space; t is the shot's tile and
-// k the step.
+// k the step.  Fields are read and written through raw buffer views: a lane's
+// offset is its column (lane * 4 bytes), a row's offset a wave-uniform scalar
+// (each wave's rows are wave-uniform), so a cell costs no 64-bit address
+// arithmetic.  Own-cell loads of rows past nx - R (the last row band) read
+// the fields' padding rows and are not used.
 template <int R, int TX, int NW, int MODE, class Args>
 __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, float* s_p) {
     constexpr bool CPML = MODE == MODE_FAST_CPML || MODE == MODE_FWD_DIV_CPML, ADJ = MODE == MODE_ADJ;
@@ -87,9 +129,12 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
     constexpr int W = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
-    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = j0 + lane;
+    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = j0 + lane, rb = nz * 4, vl = lane * 4;
     const bool jok = j < nz - R;
+    auto row = [&](int i) { return (uint32_t)i * (uint32_t)rb + (uint32_t)j0 * 4u; };  // byte offset of (i, j0)
+    const Buf VXI = field_buf(a.vx_in), VZI = field_buf(a.vz_in);
     // This lane's own-cell inputs, loaded before the tile so that their
     // latency overlaps the staging.
     float vx0[RPW], vz0[RPW], bf[RPW], axr[RPW];
@@ -97,27 +142,23 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int i = i0 + wv + q * NW;
-        const bool ok = jok && i < nx - R;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
-        vx0[q] = ok ? a.vx_in[o] : 0.f;
-        vz0[q] = ok ? a.vz_in[o] : 0.f;
-        if constexpr (!CPML && !REV) axr[q] = ok ? a.ax[i] : 1.f;
+        vx0[q] = VXI.ld(vl, row(i));
+        vz0[q] = VZI.ld(vl, row(i));
+        if constexpr (!CPML && !REV) axr[q] = ro(a.ax, i);
     }
     // Uniform density: one bf for every cell, 4 B per cell less to read.
     if (bfc != 0.f) {
 #pragma unroll
         for (int q = 0; q < RPW; ++q) bf[q] = bfc;
     } else {
+        const Buf BF = field_buf(a.bf);
 #pragma unroll
-        for (int q = 0; q < RPW; ++q) {
-            const int i = i0 + wv + q * NW;
-            bf[q] = (jok && i < nx - R) ? a.bf[(ptrdiff_t)i * nz + j] : 0.f;
-        }
+        for (int q = 0; q < RPW; ++q) bf[q] = BF.ld(vl, row(i0 + wv + q * NW));
     }
     float az_j = 1.f;
     if constexpr (!CPML && !REV) az_j = jok ? a.az[j] : 1.f;
     const float amp = ADJ ? 0.f : a.amp[k];
-    tiled_stage<TX, NW, R, R, !ADJ>(s_p, a.p_in, nz, i0, j0, a.sx, a.sz, amp);
+    tiled_stage<TX, NW, R, R, !ADJ>(s_p, field_buf(a.p_in), rb, i0, j0, wv, a.sx, a.sz, amp);
     __syncthreads();
     if (!jok) return;
     float cx[R];
@@ -126,10 +167,16 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
     const float zx = a.zx;
     const int xw = a.pml_xw, zw = a.pml_zw, sw = a.sponge_w;
     const bool raw = a.v_raw != 0;
+    const Buf VXO = field_buf(a.vx_out), VZO = field_buf(a.vz_out);
     int zc = -1;
+    float zb2 = 0.f, za2 = 0.f;
     if (CPML) {
         if (j < zw) zc = j;
         else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+        if (zc >= 0) {
+            zb2 = a.pml_z[2 * nz + j];
+            za2 = a.pml_z[3 * nz + j];
+        }
     }
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
@@ -144,7 +191,6 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
 #pragma unroll
         for (int m = 2; m <= R; ++m) sz = __builtin_fmaf(cx[m - 1], p[m] - p[1 - m], sz);
         sz = zx * sz;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         float dx = sx, dz = sz;
         float v0x = vx0[q], v0z = vz0[q];
         if constexpr (CPML) {
@@ -152,31 +198,31 @@ __device__ __forceinline__ void tiled_velocity(const Args& a, int t, int k, floa
             if (i < xw) xr = i;
             else if (i >= nx - xw) xr = i - (nx - xw) + xw;
             if (xr >= 0) {
-                const size_t e = (size_t)xr * nz + j;
-                const float psi = __builtin_fmaf(a.pml_x[3 * nx + i], sx, a.pml_x[2 * nx + i] * a.psi_px_in[e]);
-                a.psi_px_out[e] = psi;
+                const Buf PI = field_buf(a.psi_px_in);
+                const float psi = __builtin_fmaf(ro(a.pml_x, 3 * nx + i), sx, ro(a.pml_x, 2 * nx + i) * PI.ld(vl, row(xr)));
+                field_buf(a.psi_px_out).st(psi, vl, row(xr));
                 dx = sx + psi;
             }
             if (zc >= 0) {
-                const size_t e = (size_t)i * (2 * zw) + zc;
-                const float psi = __builtin_fmaf(a.pml_z[3 * nz + j], sz, a.pml_z[2 * nz + j] * a.psi_pz_in[e]);
-                a.psi_pz_out[e] = psi;
+                const uint32_t zrow = (uint32_t)i * (uint32_t)(8 * zw);
+                const float psi = __builtin_fmaf(za2, sz, zb2 * field_buf(a.psi_pz_in).ld(zc * 4, zrow));
+                field_buf(a.psi_pz_out).st(psi, zc * 4, zrow);
                 dz = sz + psi;
             }
         }
         if constexpr (REV) {
             // Inverse of the forward update (sponge factor 1 in the box).
             if (i < a.rev_x0 || i >= a.rev_x1 || j < a.rev_z0 || j >= a.rev_z1) continue;
-            a.vx_out[o] = __builtin_fmaf(bf[q], dx, v0x);
-            a.vz_out[o] = __builtin_fmaf(bf[q], dz, v0z);
+            VXO.st(__builtin_fmaf(bf[q], dx, v0x), vl, row(i));
+            VZO.st(__builtin_fmaf(bf[q], dz, v0z), vl, row(i));
         } else {
             if (!CPML && raw) {
                 const float f = tiled_sponge(i, nx, sw, axr[q], az_j);
                 v0x *= f;
                 v0z *= f;
             }
-            a.vx_out[o] = __builtin_fmaf(-bf[q], dx, v0x);
-            a.vz_out[o] = __builtin_fmaf(-bf[q], dz, v0z);
+            VXO.st(__builtin_fmaf(-bf[q], dx, v0x), vl, row(i));
+            VZO.st(__builtin_fmaf(-bf[q], dz, v0z), vl, row(i));
         }
     }
 }
@@ -189,35 +235,37 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
-    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = j0 + lane;
+    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = j0 + lane, rb = nz * 4, vl = lane * 4;
     const bool jok = j < nz - R;
+    auto row = [&](int i) { return (uint32_t)i * (uint32_t)rb + (uint32_t)j0 * 4u; };  // byte offset of (i, j0)
     const float amp = ADJ ? 0.f : a.amp[k];
     // Divergences of step k (gradient modes).
     float* dv = nullptr;
     if constexpr (ADJ || DIVOUT || REV) dv = a.div + (size_t)(k % a.div_period) * a.div_stride;
+    const Buf PI = field_buf(a.p_in), KF = field_buf(a.kf);
     // Own-cell inputs and the rows' receiver spans, before the tile.
     float p0[RPW], kf[RPW], axr[RPW], dk[RPW], gr[RPW];
     int4 span[RPW];
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int i = i0 + wv + q * NW;
-        const bool ok = jok && i < nx - R;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
-        p0[q] = ok ? a.p_in[o] : 0.f;
-        kf[q] = ok ? a.kf[o] : 0.f;
-        if constexpr (!CPML && !REV) axr[q] = ok ? a.ax[i] : 1.f;
+        p0[q] = PI.ld(vl, row(i));
+        kf[q] = KF.ld(vl, row(i));
+        if constexpr (!CPML && !REV) axr[q] = ro(a.ax, i);
         if constexpr (ADJ) {
-            dk[q] = ok ? dv[o] : 0.f;
-            gr[q] = ok ? a.grad[o] : 0.f;
+            dk[q] = field_buf(dv).ld(vl, row(i));
+            gr[q] = field_buf(a.grad).ld(vl, row(i));
         } else if constexpr (!REV) {
-            span[q] = a.rec_row[min(i, nx - 1)];
+            const int* rr = reinterpret_cast<const int*>(a.rec_row) + 4 * min(i, nx - 1);
+            span[q] = make_int4(ro(rr, 0), ro(rr, 1), ro(rr, 2), ro(rr, 3));
         }
     }
     float az_j = 1.f;
     if constexpr (!CPML && !REV) az_j = jok ? a.az[j] : 1.f;
-    tiled_stage<TX, NW, R, 0, false>(s_vx, a.vx_out, nz, i0, j0, 0, 0, 0.f);
-    tiled_stage<TX, NW, 0, R, false>(s_vz, a.vz_out, nz, i0, j0, 0, 0, 0.f);
+    tiled_stage<TX, NW, R, 0, false>(s_vx, field_buf(a.vx_out), rb, i0, j0, wv, 0, 0, 0.f);
+    tiled_stage<TX, NW, 0, R, false>(s_vz, field_buf(a.vz_out), rb, i0, j0, wv, 0, 0, 0.f);
     __syncthreads();
     if (!jok) return;
     float cx[R];
@@ -225,10 +273,16 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
     for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
     const float zx = a.zx;
     const int xw = a.pml_xw, zw = a.pml_zw, sw = a.sponge_w;
+    const Buf PO = field_buf(a.p_out);
     int zc = -1;
+    float zb0 = 0.f, za0 = 0.f;
     if (CPML) {
         if (j < zw) zc = j;
         else if (j >= nz - zw) zc = j - (nz - zw) + zw;
+        if (zc >= 0) {
+            zb0 = a.pml_z[j];
+            za0 = a.pml_z[nz + j];
+        }
     }
     float* gk = a.rec_buf + (size_t)k * a.nslot;
 #pragma unroll
@@ -249,38 +303,36 @@ __device__ __forceinline__ void tiled_pressure(const Args& a, int t, int k, floa
             if (i < xw) xr = i;
             else if (i >= nx - xw) xr = i - (nx - xw) + xw;
             if (xr >= 0) {
-                const size_t e = (size_t)xr * nz + j;
-                const float psi = __builtin_fmaf(a.pml_x[nx + i], sx, a.pml_x[i] * a.psi_vx_in[e]);
-                a.psi_vx_out[e] = psi;
+                const float psi = __builtin_fmaf(ro(a.pml_x, nx + i), sx, ro(a.pml_x, i) * field_buf(a.psi_vx_in).ld(vl, row(xr)));
+                field_buf(a.psi_vx_out).st(psi, vl, row(xr));
                 sx += psi;
             }
             if (zc >= 0) {
-                const size_t e = (size_t)i * (2 * zw) + zc;
-                const float psi = __builtin_fmaf(a.pml_z[nz + j], sz, a.pml_z[j] * a.psi_vz_in[e]);
-                a.psi_vz_out[e] = psi;
+                const uint32_t zrow = (uint32_t)i * (uint32_t)(8 * zw);
+                const float psi = __builtin_fmaf(za0, sz, zb0 * field_buf(a.psi_vz_in).ld(zc * 4, zrow));
+                field_buf(a.psi_vz_out).st(psi, zc * 4, zrow);
                 sz += psi;
             }
         }
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         if constexpr (REV) {
             // Inverse of the forward update (sponge factor 1 in the box).
             if (i < a.rev_x0 || i >= a.rev_x1 || j < a.rev_z0 || j >= a.rev_z1) continue;
             const float s = sx + sz;
             float p = __builtin_fmaf(kf[q], s, p0[q]);
             if (i == a.sx && j == a.sz) p += amp;
-            dv[o] = s;
-            a.p_out[o] = p;
+            field_buf(dv).st(s, vl, row(i));
+            PO.st(p, vl, row(i));
         } else {
             float p = p0[q];
             if (!ADJ && i == a.sx && j == a.sz) p -= amp;
             const float pc = __builtin_fmaf(-kf[q], sx + sz, p);
-            if constexpr (DIVOUT) dv[o] = sx + sz;
+            if constexpr (DIVOUT) field_buf(dv).st(sx + sz, vl, row(i));
             if constexpr (ADJ) {
-                a.grad[o] = __builtin_fmaf(p0[q], dk[q], gr[q]);
+                field_buf(a.grad).st(__builtin_fmaf(p0[q], dk[q], gr[q]), vl, row(i));
                 if (i == a.sx && j == a.sz) a.src_adj[k] = pc;
             }
             const float pn = CPML ? pc : pc * tiled_sponge(i, nx, sw, axr[q], az_j);
-            a.p_out[o] = pn;
+            PO.st(pn, vl, row(i));
             if constexpr (!ADJ) {
                 // Receivers of row i: the first span, then further spans (rare).
                 const int4 sp = span[q];

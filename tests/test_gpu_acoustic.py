@@ -240,7 +240,7 @@ def test_cpml_tiled_step_bitwise_equals_march(ctx, radius):
     r0, s0 = ref.fetch()
     st0, pm0 = ref.get_state()
     assert np.abs(r0).max() > 0
-    shapes = [("tiled", tx, nw) for tx, nw in ((16, 8), (8, 4), (8, 8), (16, 4), (16, 16), (32, 8), (32, 16))]
+    shapes = [("tiled", tx, nw) for tx, nw in ((16, 8), (8, 4), (8, 8), (16, 4), (16, 16), (32, 4), (32, 8), (32, 16))]
     shapes += [("fused", r, nw) for r, nw in FUSED_SHAPES]
     for kind, a, b in shapes:
         pl = sfd.AcousticPlan(shot, ctx)
@@ -271,7 +271,7 @@ def test_sponge_tiled_step_bitwise_equals_march(ctx, radius):
     r0, s0 = ref.fetch()
     st0, _ = ref.get_state()
     assert np.abs(r0).max() > 0 and np.abs(s0[:, 1:]).max() > 0
-    shapes = [("tiled", tx, nw) for tx, nw in ((16, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 8), (32, 16))]
+    shapes = [("tiled", tx, nw) for tx, nw in ((16, 4), (8, 4), (8, 8), (16, 8), (16, 16), (32, 4), (32, 8), (32, 16))]
     shapes += [("fused", r, nw) for r, nw in FUSED_SHAPES]
     for kind, a, b in shapes:
         pl = sfd.AcousticPlan(shot, ctx)
@@ -284,6 +284,44 @@ def test_sponge_tiled_step_bitwise_equals_march(ctx, radius):
         np.testing.assert_array_equal(s1, s0, err_msg=msg)
         np.testing.assert_array_equal(st1, st0, err_msg=msg)
         pl.close()
+
+
+@pytest.mark.parametrize("radius", [4, 5, 8])
+@pytest.mark.parametrize("boundary", ["cpml", "sponge"])
+def test_tiled_source_on_tile_edges(ctx, radius, boundary):
+    """The tiled velocity launch takes the source amplitude off P while it
+    stages the tile: a source in a tile's own columns, in the halo columns of
+    its z neighbours (tiles are 64 columns from column R) and in the halo
+    rows of its x neighbours is seen by every stencil that reads it, so
+    records and final state equal the march kernel's bitwise at every shape."""
+    base = W.smooth_random_2001x4001(nt=160, scale=10, radius=radius, boundary=boundary, snap_steps=(159,),
+                                     snap_fields=7)
+    R = radius
+    cols = (R + 63, R + 64, R + 64 - R, R + 63 + R, R + 128 - 1)
+    rows = (R + 32 - 1, R + 32, R + 16 + R - 1, R + 64 - R)
+    for sx, sz in zip(rows + rows[:1], cols):
+        shot = W.smooth_random_2001x4001(nt=160, scale=10, radius=radius, boundary=boundary, snap_steps=(159,),
+                                         snap_fields=7, src=(sx, sz))
+        assert shot.vp.shape == base.vp.shape
+        print(f"case: tiled source at ({sx}, {sz}) R={radius} {boundary}", flush=True)
+        ref = sfd.AcousticPlan(shot, ctx)
+        ref.set_tiled(-1)
+        ref.run()
+        r0, s0 = ref.fetch()
+        st0, _ = ref.get_state()
+        ref.close()
+        assert np.abs(r0).max() > 0
+        for tx, nw in ((16, 4), (8, 4), (32, 4), (32, 8)):
+            pl = sfd.AcousticPlan(shot, ctx)
+            pl.set_tiled(tx, nw)
+            pl.run()
+            r1, s1 = pl.fetch()
+            st1, _ = pl.get_state()
+            pl.close()
+            msg = f"source ({sx}, {sz}) tiled {tx}x{nw}"
+            np.testing.assert_array_equal(r1, r0, err_msg=msg)
+            np.testing.assert_array_equal(s1, s0, err_msg=msg)
+            np.testing.assert_array_equal(st1, st0, err_msg=msg)
 
 
 def test_sponge_tiled_switches_and_resume(ctx):

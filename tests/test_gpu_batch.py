@@ -28,10 +28,13 @@ def small_batch(boundary="cpml", nshot=5, nt=300, arith="fast"):
 
 @pytest.mark.parametrize("boundary,tiling", [("cpml", "march"), ("cpml", "default"), ("sponge", "march"),
                                              ("sponge", "default"), ("cpml", "fused"), ("sponge", "fused"),
-                                             ("cpml", "stream"), ("sponge", "stream")])
+                                             ("cpml", "stream"), ("sponge", "stream"), ("cpml", "8x4"),
+                                             ("sponge", "16x8"), ("cpml", "32x4"), ("sponge", "32x4"),
+                                             ("cpml", "32x8")])
 def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
     """tiling "default": batches run the two-launch tiled step (in place;
-    sponge velocities kept unabsorbed between steps), "fused" the one-launch
+    sponge velocities kept unabsorbed between steps; "TXxNW" names another
+    built tile shape of it), "fused" the one-launch
     tiled step (ping-pong), "stream" the one-launch streaming step (chunks of
     72 rows, not a multiple of its 16-row blocks), the single plans the fused
     march kernel; still bit for bit."""
@@ -51,6 +54,8 @@ def test_batch_modes_equal_single_plans(ctx, boundary, tiling):
         b.set_fused(16, 4)
     elif tiling == "stream":
         b.set_fused(72, 4)
+    elif "x" in tiling:
+        b.set_tiled(*map(int, tiling.split("x")))
     for how in (sfd.BATCH_GRAPH, sfd.BATCH_LAUNCHES, sfd.BATCH_PER_SHOT):
         b.run(how)
         for s in range(len(shots)):
