@@ -63,6 +63,73 @@ struct Buf {
     }
 };
 
+// Tiles of the two-launch steps are 64 columns wide, one per lane.
+constexpr int kTileCols = 64;
+
+// Rows [i0 - HR, i0 + TX + HR) x columns [j0 - HC, j0 + 64 + HC) of a field
+// (rows and columns counted from f's origin, rows rb bytes apart; the first
+// staged row and column are >= 0) into lds[row][col], W = 64 + 2 HC
+// floats a row.  Each wave copies whole rows of the 64 tile columns (lane =
+// column, the row's byte offset a scalar, so no per-lane address arithmetic),
+// then the workgroup copies the 2 HC edge columns of every row.  All loads
+// are issued before the first LDS store.  With SRC, the source amplitude is
+// taken off cell (sx, sz) on the way (same float subtraction as the fused
+// kernel's).  wv is the wave's index, wave-uniform.
+template <int TX, int NW, int HR, int HC, bool SRC>
+__device__ __forceinline__ void tiled_stage(float* lds, const Buf& f, int rb, int i0, int j0, int wv, int sx, int sz,
+                                            float amp) {
+    constexpr int W = kTileCols + 2 * HC, Hh = TX + 2 * HR, T = 64 * NW;
+    constexpr int RI = (Hh + NW - 1) / NW;           // rows per wave
+    constexpr int EW = 2 * HC, E = EW * Hh;          // edge cells
+    constexpr int EWd = EW > 0 ? EW : 1;
+    constexpr int EI = E > 0 ? (E + T - 1) / T : 0;  // edge cells per thread
+    const int lane = threadIdx.x & 63;
+    const uint32_t srow = (uint32_t)(i0 - HR) * (uint32_t)rb;  // byte offset of the first staged row
+    float v[RI];
+#pragma unroll
+    for (int q = 0; q < RI; ++q) {
+        const int r = wv + q * NW;
+        v[q] = (RI * NW == Hh || r < Hh) ? f.ld(lane * 4, srow + (uint32_t)r * rb + (uint32_t)j0 * 4u) : 0.f;
+    }
+    float ve[EI > 0 ? EI : 1];
+    int le[EI > 0 ? EI : 1];
+#pragma unroll
+    for (int q = 0; q < EI; ++q) {
+        const int e = threadIdx.x + q * T;
+        const int r = e / EWd, c = e - r * EWd;
+        const int col = c < HC ? c : c + kTileCols;  // column in the staged row
+        le[q] = (e < E) ? r * W + col : -1;
+        ve[q] = (e < E) ? f.ld(r * rb + col * 4, srow + (uint32_t)(j0 - HC) * 4u) : 0.f;
+    }
+    if constexpr (SRC) {
+        // Wave-uniform tests first: only the wave (row) or edge holding the source does vector work.
+        const int rs = sx - (i0 - HR), cs = sz - (j0 - HC);
+        if (rs >= 0 && rs < Hh && cs >= 0 && cs < W) {
+            if (cs >= HC && cs < HC + kTileCols) {
+#pragma unroll
+                for (int q = 0; q < RI; ++q)
+                    if (wv + q * NW == rs && lane == cs - HC) v[q] -= amp;
+            } else {
+#pragma unroll
+                for (int q = 0; q < EI; ++q)
+                    if (le[q] == rs * W + cs) ve[q] -= amp;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RI; ++q) {
+        const int r = wv + q * NW;
+        if (RI * NW == Hh || r < Hh) lds[r * W + HC + lane] = v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < EI; ++q)
+        if (le[q] >= 0) lds[le[q]] = ve[q];
+}
+
+// Raw buffer view of a field from its row-0 pointer (every offset the tiled
+// kernels form from it is non-negative and below 2 GiB).
+__device__ __forceinline__ Buf field_buf(const float* p) { return Buf(p, 0x7fffffffu); }
+
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
 // Remap so each XCD gets a contiguous run of tiles in (x chunk, z tile)
 // order, z fastest: an XCD holds a band of x chunks across all z tiles, so
