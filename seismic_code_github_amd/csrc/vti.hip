@@ -79,20 +79,21 @@ __global__ void vti_gather_kernel(VtiArgs a) { vti_record(a, a.shot0 + blockIdx.
 // The split-PML recursions (sub:651-663 and 1141-1153), slots [old, cur] of
 // P1 (slot base 0) and Pt (slot base 2), P2 single-level (slot 4); the old
 // slot takes the new value.  S2 = the second x (z) derivative, S1 the first;
-// returns P1 + P2.  Operand order as the reference writes it.
-__device__ __forceinline__ float vti_pml(float* P, size_t n, size_t s, int o, int c, float k, float S2, float S1,
+// returns P1 + P2.  Operand order as the reference writes it.  P views one
+// shot's parts, slot q at byte offset q * nb, this lane's cell at byte vs.
+__device__ __forceinline__ float vti_pml(const Buf& P, uint32_t nb, int vs, int o, int c, float k, float S2, float S1,
                                          float d, float dd, float h, float dt, float dt2) {
     const float h2 = h * h;
-    const float p1 = (P[c * n + s] * (2.f * h2 + 2.f * d * dt * h2 - d * d * h2 * dt2) - h2 * P[o * n + s] +
-                      k * dt2 * (S2 * h2)) /
+    const float p1 = (P.ld(vs, (uint32_t)c * nb) * (2.f * h2 + 2.f * d * dt * h2 - d * d * h2 * dt2) -
+                      h2 * P.ld(vs, (uint32_t)o * nb) + k * dt2 * (S2 * h2)) /
                      (h2 + 2.f * d * dt * h2);
-    const float pt = (P[(2 + c) * n + s] * (2.f * h + 2.f * d * dt * h - dt2 * h * (d * d)) - h * P[(2 + o) * n + s] -
-                      k * dt2 * dd * (S1 * h)) /
+    const float pt = (P.ld(vs, (uint32_t)(2 + c) * nb) * (2.f * h + 2.f * d * dt * h - dt2 * h * (d * d)) -
+                      h * P.ld(vs, (uint32_t)(2 + o) * nb) - k * dt2 * dd * (S1 * h)) /
                      (h + 2.f * d * dt * h);
-    const float p2 = P[4 * n + s] * (1.f - d * dt) + pt * dt;
-    P[o * n + s] = p1;
-    P[(2 + o) * n + s] = pt;
-    P[4 * n + s] = p2;
+    const float p2 = P.ld(vs, 4 * nb) * (1.f - d * dt) + pt * dt;
+    P.st(p1, vs, (uint32_t)o * nb);
+    P.st(pt, vs, (uint32_t)(2 + o) * nb);
+    P.st(p2, vs, 4 * nb);
     return p1 + p2;
 }
 
@@ -103,6 +104,11 @@ __device__ __forceinline__ void vti_damp(float v, float a, float L, float lnr, f
     *dd = g * (2.f * a / (L * L)) * lnr;
 }
 
+// Fields are read and written through raw buffer views of the shot's copy
+// from its padded origin (cell (-kVPad, -kVPad)): a lane's offset is its row
+// (z), a column's offset a wave-uniform scalar, so a cell costs no 64-bit
+// address arithmetic.  Own-cell loads of columns past nx and rows past nz
+// read the padding and are not used.
 template <int TX, int NW>
 __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
     constexpr int WX = TX + 2 * kVH, HZ = kVTZ + 2 * kVH, T = 64 * NW;
@@ -113,72 +119,81 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
     if (blockIdx.x == 0) vti_record(a, shot, threadIdx.x, T);
     const int t = xcd_tile(blockIdx.x, a.ntiles);
     const int z0 = (t % a.ntz) * kVTZ, x0 = (t / a.ntz) * TX;
-    const long long so = shot * a.fs;
-    const float* uc = a.uc + so;
-    const float* qc = a.qc + so;
-    float* un = a.un + so;
-    float* qn = a.qn + so;
-    const int tz = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tz = threadIdx.x & 63, vl = tz * 4;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int k = z0 + tz;
     const int n_x = a.nx;
     const bool inz = k >= a.nu && k < a.nz - a.nd;
+    const long long pad0 = shot * a.fs - ((long long)kVPad * a.pitch + kVPad);
+    const Buf UC = field_buf(a.uc + pad0), QC = field_buf(a.qc + pad0);
+    const Buf UN = field_buf(a.un + pad0), QN = field_buf(a.qn + pad0);
+    const uint32_t cb = (uint32_t)a.pitch * 4u;  // bytes per column
+    auto colo = [&](int col) { return (uint32_t)(col + kVPad) * cb + (uint32_t)(z0 + kVPad) * 4u; };  // (col, z0)
     // The cells' own old level and model are loaded first, so they are in
     // flight together with the staging loads (the old p only where the
     // interior formula reads it, the old q only outside the x strips).
     constexpr int J = TX / NW;
     float r_uo[J], r_qo[J], r_vz[J], r_ep[J], r_de[J];
+    {
+        const Buf VZ = field_buf(a.vz + pad0), EP = field_buf(a.ep + pad0), DE = field_buf(a.de + pad0);
+        const int vu = inz ? vl : kOOB;
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const int col = x0 + wv + j * NW;
-        const bool ok = k < a.nz && col < n_x;
-        const bool inx = col >= a.nl && col < n_x - a.nr;
-        const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
-        r_vz[j] = ok ? a.vz[so + o] : 0.f;
-        r_ep[j] = ok ? a.ep[so + o] : 0.f;
-        r_de[j] = ok ? a.de[so + o] : 0.f;
-        r_qo[j] = ok && inx ? qn[o] : 0.f;
-        r_uo[j] = ok && inx && inz ? un[o] : 0.f;
+        for (int j = 0; j < J; ++j) {
+            const int col = x0 + wv + j * NW;
+            const uint32_t o = colo(col);
+            r_vz[j] = VZ.ld(vl, o);
+            r_ep[j] = EP.ld(vl, o);
+            r_de[j] = DE.ld(vl, o);
+            const bool inx = col >= a.nl && col < n_x - a.nr;
+            r_qo[j] = inx ? QN.ld(vl, o) : 0.f;
+            r_uo[j] = inx ? UN.ld(vu, o) : 0.f;
+        }
     }
     {
-        // s region: WX columns x 64 rows; p region: TX columns x HZ rows.
-        constexpr int NS = WX * kVTZ, NU = TX * HZ;
-        constexpr int IS = (NS + T - 1) / T, IU = (NU + T - 1) / T;
-        float vu[IS], vq[IS], vp[IU];
+        // s = p + q over WX columns x 64 rows, whole columns per wave; q and
+        // p of the tile's own columns from the same loads; p's 5-row z halo
+        // of the own columns (10 cells a column) spread over all threads.
+        constexpr int QS = (WX + NW - 1) / NW;
+        constexpr int NH = TX * 2 * kVH, QH = (NH + T - 1) / T;
+        float vu[QS], vq[QS], vh[QH];
+        int hl[QH];
+        const uint32_t cs0 = colo(x0 - kVH);
 #pragma unroll
-        for (int i = 0; i < IS; ++i) {
-            const int e = threadIdx.x + i * T;
-            const int c = e / kVTZ, r = e - c * kVTZ;
-            const ptrdiff_t o = (ptrdiff_t)(x0 - kVH + c) * a.pitch + (z0 + r);
-            vu[i] = e < NS ? uc[o] : 0.f;
-            vq[i] = e < NS ? qc[o] : 0.f;
+        for (int q = 0; q < QS; ++q) {
+            const int c = wv + q * NW;
+            vu[q] = vq[q] = 0.f;
+            if (QS * NW == WX || c < WX) {
+                vu[q] = UC.ld(vl, cs0 + (uint32_t)c * cb);
+                vq[q] = QC.ld(vl, cs0 + (uint32_t)c * cb);
+            }
+        }
+        const uint32_t h0 = (uint32_t)(x0 + kVPad) * cb + (uint32_t)(z0 + kVPad - kVH) * 4u;  // (x0, z0 - 5)
+#pragma unroll
+        for (int q = 0; q < QH; ++q) {
+            const int e = threadIdx.x + q * T;
+            const int c = e / (2 * kVH), r = e - c * (2 * kVH);
+            const int rr = r < kVH ? r : r + kVTZ;  // row - (z0 - 5)
+            hl[q] = e < NH ? c * HZ + rr : -1;
+            float u = e < NH ? UC.ld((int)((uint32_t)c * cb) + rr * 4, h0) : 0.f;
+            if (e < NH && x0 + c == a.sx && z0 - kVH + rr == a.sz) u += a.w_cur;
+            vh[q] = u;
         }
 #pragma unroll
-        for (int i = 0; i < IU; ++i) {
-            const int e = threadIdx.x + i * T;
-            const int c = e / HZ, r = e - c * HZ;
-            vp[i] = e < NU ? uc[(ptrdiff_t)(x0 + c) * a.pitch + (z0 - kVH + r)] : 0.f;
-        }
-#pragma unroll
-        for (int i = 0; i < IS; ++i) {
-            const int e = threadIdx.x + i * T;
-            if (e < NS) {
-                const int c = e / kVTZ, r = e - c * kVTZ;
-                float u = vu[i];
-                if (x0 - kVH + c == a.sx && z0 + r == a.sz) u += a.w_cur;
-                s_s[e] = u + vq[i];
-                if (c >= kVH && c < kVH + TX) s_q[(c - kVH) * kVTZ + r] = vq[i];
+        for (int q = 0; q < QS; ++q) {
+            const int c = wv + q * NW;
+            if (QS * NW == WX || c < WX) {
+                float u = vu[q];
+                if (x0 - kVH + c == a.sx && k == a.sz) u += a.w_cur;
+                s_s[c * kVTZ + tz] = u + vq[q];
+                if (c >= kVH && c < kVH + TX) {
+                    s_q[(c - kVH) * kVTZ + tz] = vq[q];
+                    s_u[(c - kVH) * HZ + kVH + tz] = u;
+                }
             }
         }
 #pragma unroll
-        for (int i = 0; i < IU; ++i) {
-            const int e = threadIdx.x + i * T;
-            if (e < NU) {
-                const int c = e / HZ, r = e - c * HZ;
-                float u = vp[i];
-                if (x0 + c == a.sx && z0 - kVH + r == a.sz) u += a.w_cur;
-                s_u[e] = u;
-            }
-        }
+        for (int q = 0; q < QH; ++q)
+            if (hl[q] >= 0) s_u[hl[q]] = vh[q];
     }
     __syncthreads();
     if (k >= a.nz) return;
@@ -186,7 +201,7 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
     for (int j = 0; j < J; ++j) {
         const int cl = wv + j * NW, col = x0 + cl;
         if (col >= n_x) break;
-        const ptrdiff_t o = (ptrdiff_t)col * a.pitch + k;
+        const uint32_t o = colo(col);
         const float* ps = s_s + (cl + kVH) * kVTZ + tz;
         const float* pu = s_u + cl * HZ + tz + kVH;
         const float s0 = ps[0], u0 = pu[0], q0 = s_q[cl * kVTZ + tz];
@@ -203,11 +218,11 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
         const bool inx = col >= a.nl && col < n_x - a.nr;
         const int po = a.par, pc = 1 - a.par;
         if (inx) {
-            qn[o] = 2.f * q0 - r_qo[j] + t * (2.f * (ep - de) * dxx);
+            QN.st(2.f * q0 - r_qo[j] + t * (2.f * (ep - de) * dxx), vl, o);
             if (inz) {
                 float uo = r_uo[j];
                 if (col == a.sx && k == a.sz) uo += a.w_prev;
-                un[o] = 2.f * u0 - uo + t * ((1.f + 2.f * de) * dxx + dzz);
+                UN.st(2.f * u0 - uo + t * ((1.f + 2.f * de) * dxx + dzz), vl, o);
             } else {
                 // top / bottom strip (sub:785-853, 869-947): z-split p
                 float s1 = 0.f;
@@ -217,16 +232,17 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
                 const bool top = k < a.nu;
                 const float az = top ? (float)(k - a.nu) * a.dz : (float)(k + 1 - (a.nz - a.nd)) * a.dz;
                 const float L = (float)(top ? a.nu : a.nd) * a.dz;
-                const size_t n = (size_t)a.nzs;
-                float* P = a.zs + (size_t)shot * 7 * n;
-                const size_t s = (size_t)(col - a.nl) * (a.nu + a.nd) + (top ? k : a.nu + k - (a.nz - a.nd));
+                const uint32_t nb = (uint32_t)a.nzs * 4u;
+                const Buf P = field_buf(a.zs + (size_t)shot * 7 * a.nzs);
+                const int vs = ((col - a.nl) * (a.nu + a.nd) + (top ? k : a.nu + k - (a.nz - a.nd))) * 4;
                 float d, dd;
                 vti_damp(vz, az, L, a.lnr, &d, &dd);
-                const float uz = vti_pml(P, n, s, po, pc, vz * vz, dzz, s1, d, dd, a.dz, a.dt, a.dt2);
+                const float uz = vti_pml(P, nb, vs, po, pc, vz * vz, dzz, s1, d, dd, a.dz, a.dt, a.dt2);
                 const float vn2 = (vz * sqrtf(1.f + 2.f * de)) * (vz * sqrtf(1.f + 2.f * de));
-                const float u3 = 2.f * P[(5 + pc) * n + s] - P[(5 + po) * n + s] + vn2 * a.dt2 * dxx;
-                P[(5 + po) * n + s] = u3;
-                un[o] = uz + u3;
+                const float u3 =
+                    2.f * P.ld(vs, (uint32_t)(5 + pc) * nb) - P.ld(vs, (uint32_t)(5 + po) * nb) + vn2 * a.dt2 * dxx;
+                P.st(u3, vs, (uint32_t)(5 + po) * nb);
+                UN.st(uz + u3, vl, o);
             }
         } else {
             // left / right strip (sub:561-667, 669-781; q: 1072-1256)
@@ -237,18 +253,21 @@ __global__ __launch_bounds__(64 * NW) void vti_step_kernel(VtiArgs a) {
             const bool left = col < a.nl;
             const float ax = left ? (float)(col - a.nl) * a.dx : (float)(col + 1 - (n_x - a.nr)) * a.dx;
             const float L = (float)(left ? a.nl : a.nr) * a.dx;
-            const size_t n = (size_t)a.nxs;
-            float* P = a.xs + (size_t)shot * 12 * n;
-            const size_t s = (size_t)(left ? col : a.nl + col - (n_x - a.nr)) * a.nz + k;
+            const uint32_t nb = (uint32_t)a.nxs * 4u;
+            const float* xp = a.xs + (size_t)shot * 12 * a.nxs;
+            const Buf P = field_buf(xp);
+            const int vs = ((left ? col : a.nl + col - (n_x - a.nr)) * a.nz + k) * 4;
             const float vn = vz * sqrtf(1.f + 2.f * de), vx = vz * sqrtf(1.f + 2.f * ep);
             float d, dd;
             vti_damp(vx, ax, L, a.lnr, &d, &dd);
-            const float ux = vti_pml(P, n, s, po, pc, vn * vn, dxx, s1, d, dd, a.dx, a.dt, a.dt2);
-            const float u3 =
-                2.f * P[(5 + pc) * n + s] - P[(5 + po) * n + s] + vz * vz * a.dt2 * (dzz * (a.dz * a.dz)) / (a.dz * a.dz);
-            P[(5 + po) * n + s] = u3;
-            un[o] = ux + u3;
-            qn[o] = vti_pml(P + 7 * n, n, s, po, pc, vx * vx - vn * vn, dxx, s1, d, dd, a.dx, a.dt, a.dt2);
+            const float ux = vti_pml(P, nb, vs, po, pc, vn * vn, dxx, s1, d, dd, a.dx, a.dt, a.dt2);
+            const float u3 = 2.f * P.ld(vs, (uint32_t)(5 + pc) * nb) - P.ld(vs, (uint32_t)(5 + po) * nb) +
+                             vz * vz * a.dt2 * (dzz * (a.dz * a.dz)) / (a.dz * a.dz);
+            P.st(u3, vs, (uint32_t)(5 + po) * nb);
+            UN.st(ux + u3, vl, o);
+            QN.st(vti_pml(field_buf(xp + 7 * (size_t)a.nxs), nb, vs, po, pc, vx * vx - vn * vn, dxx, s1, d, dd, a.dx,
+                          a.dt, a.dt2),
+                  vl, o);
         }
     }
 }
