@@ -98,6 +98,13 @@ int sfd_context_create(int device, sfd_context **ctx);
 int sfd_context_destroy(sfd_context *ctx);
 const char *sfd_last_error(const sfd_context *ctx);
 int sfd_abi_version(void);
+/* Device memory of the context's GPU for inputs kept resident across calls,
+ * such as the observed gathers of FWI iterations (no counterpart in the
+ * reference): allocate, copy (kind 0 = host to device, 1 = device to host;
+ * complete on return) and free. */
+int sfd_device_alloc(sfd_context *ctx, size_t bytes, void **ptr);
+int sfd_device_copy(sfd_context *ctx, void *dst, const void *src, size_t bytes, int kind);
+int sfd_device_free(sfd_context *ctx, void *ptr);
 
 /* One shot, end to end: the drop-in for Acoustic_1order_FD.cpp:237-393.
  *   vp, den  [nx*nz]   model (den NULL = 2400 kg/m^3 everywhere, :141)
@@ -285,6 +292,17 @@ int sfd_acoustic_misfit_gradient_ex(sfd_context *ctx, int nshot, const sfd_acous
                                     const float *vp, const float *den, const float *const *wavelet,
                                     const float *const *d_obs, int ckpt_every, int method,
                                     float *grad_vp, double *misfit);
+/* As sfd_acoustic_misfit_gradient_ex (no counterpart in the reference), with
+ * the observed gathers already in device memory: d_obs_dev[s] is a device
+ * pointer to shot s's [nrec][nt] float gather, written before the call on
+ * any stream and left unchanged during it.  FWI iterations keep their data
+ * resident, so no host transfer of it enters the call.  (With host data,
+ * sfd_acoustic_misfit_gradient_ex copies it on a second stream while the
+ * forward runs.) */
+int sfd_acoustic_misfit_gradient_dev(sfd_context *ctx, int nshot, const sfd_acoustic_params *params,
+                                     const float *vp, const float *den, const float *const *wavelet,
+                                     const float *const *d_obs_dev, int ckpt_every, int method,
+                                     float *grad_vp, double *misfit);
 
 /* Aperture window of one shot, the drop-in for get_currshot_parameters
  * (forward/Q_forward/Q_multishot_fd_subroutine_fsbc_zy_rotated.f90:315-373;

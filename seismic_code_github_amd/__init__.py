@@ -147,6 +147,10 @@ _SIGNATURES = {
     "sfd_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "sfd_context_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "sfd_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "sfd_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_int]),
+    "sfd_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "sfd_acoustic_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AcousticParamsC),
                                             _FP, _FP, _FP, _FP, _FP]),
     "sfd_acoustic_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(AcousticParamsC),
@@ -229,6 +233,10 @@ _SIGNATURES = {
                                                        ctypes.POINTER(AcousticParamsC), _FP, _FP,
                                                        ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.c_int,
                                                        ctypes.c_int, _FP, ctypes.POINTER(ctypes.c_double)]),
+    "sfd_acoustic_misfit_gradient_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                                        ctypes.POINTER(AcousticParamsC), _FP, _FP,
+                                                        ctypes.POINTER(_FP), ctypes.POINTER(_FP), ctypes.c_int,
+                                                        ctypes.c_int, _FP, ctypes.POINTER(ctypes.c_double)]),
     "sfd_visco_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViscoParamsC), _FP, _FP, _FP, _FP, _FP,
                                          _FP, _FP]),
     "sfd_visco_plan_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViscoParamsC), _FP, _FP, _FP, _FP, _FP,
@@ -367,6 +375,46 @@ def default_context() -> Context:
     if _default_ctx is None:
         _default_ctx = Context(0)
     return _default_ctx
+
+
+class DeviceArray:
+    """A float32 array in the context's device memory (``sfd_device_alloc``),
+    for inputs kept resident across calls, such as observed gathers over FWI
+    iterations.  Exposes ``__cuda_array_interface__`` (version 3)."""
+
+    def __init__(self, a, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        a = _f32(a)
+        self.shape = tuple(a.shape)
+        self.nbytes = a.nbytes
+        p = ctypes.c_void_p()
+        self.ctx.check(lib().sfd_device_alloc(self.ctx.handle, self.nbytes, ctypes.byref(p)), "sfd_device_alloc")
+        self.ptr = p.value or 0
+        if self.nbytes:
+            self.ctx.check(lib().sfd_device_copy(self.ctx.handle, self.ptr, a.ctypes.data, self.nbytes, 0),
+                           "sfd_device_copy")
+
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": self.shape, "typestr": "<f4", "data": (self.ptr, False), "version": 3, "strides": None}
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float32)
+        if self.nbytes:
+            self.ctx.check(lib().sfd_device_copy(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes, 1),
+                           "sfd_device_copy")
+        return out
+
+    def close(self):
+        if self.ptr and self.ctx.handle:
+            lib().sfd_device_free(self.ctx.handle, self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -562,8 +610,11 @@ GRADIENT_METHODS = {"recompute": 0, "boundary": 1}
 def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: Optional[Context] = None,
                     method: str = "recompute"):
     """J = sum_s 1/2 |record_s - d_obs[s]|^2 over shots sharing one model and
-    its dJ/dvp, through ``sfd_acoustic_misfit_gradient_ex``.  The model is
-    ``vp``/``den`` (default: the first shot's).  method "recompute"
+    its dJ/dvp, through ``sfd_acoustic_misfit_gradient_ex`` (d_obs as host
+    arrays) or ``sfd_acoustic_misfit_gradient_dev`` (d_obs as contiguous
+    float32 device arrays on the context's GPU -- ``DeviceArray`` or any
+    object with ``__cuda_array_interface__`` -- read in place).  The
+    model is ``vp``/``den`` (default: the first shot's).  method "recompute"
     (checkpoints + segment recomputation) or "boundary" (checkpoints +
     boundary saving; the gradient inside the box the sponge or C-PML leaves
     alone only).  Every shot needs the same boundary kind.
@@ -577,19 +628,33 @@ def misfit_gradient(shots, d_obs, vp=None, den=None, ckpt_every: int = 50, ctx: 
     params = (AcousticParamsC * n)()
     ws, ds = (_FP * n)(), (_FP * n)()
     keep = []
+    on_device = n > 0 and all(hasattr(d, "__cuda_array_interface__") for d in d_obs)
     for i, sh in enumerate(shots):
         params[i] = sh.params(keep)
         w = _f32(sh.wavelet)
-        d = _f32(d_obs[i])
-        if d.shape != (params[i].nrec, params[i].nt):
+        d = d_obs[i]
+        if on_device:
+            iface = d.__cuda_array_interface__
+            shape = tuple(iface["shape"])
+            if iface["typestr"] != "<f4" or iface.get("strides") not in (None, (4 * shape[-1], 4) if shape else None):
+                raise ValueError(f"d_obs[{i}] must be a contiguous float32 device array")
+            ds[i] = ctypes.cast(iface["data"][0], _FP)
+        else:
+            d = _f32(d)
+            shape = d.shape
+            ds[i] = _ptr(d)
+        if shape != (params[i].nrec, params[i].nt):
             raise ValueError(f"d_obs[{i}] must be [nrec, nt]")
         keep += [w, d, sh]
-        ws[i], ds[i] = _ptr(w), _ptr(d)
+        ws[i] = _ptr(w)
     grad = np.zeros(vp.shape, dtype=np.float32)
     J = ctypes.c_double()
-    ctx.check(lib().sfd_acoustic_misfit_gradient_ex(ctx.handle, n, params, _ptr(vp), _ptr(den), ws, ds, ckpt_every,
-                                                    GRADIENT_METHODS[method], _ptr(grad), ctypes.byref(J)),
-              "sfd_acoustic_misfit_gradient_ex")
+    if on_device:
+        fn, name = lib().sfd_acoustic_misfit_gradient_dev, "sfd_acoustic_misfit_gradient_dev"
+    else:
+        fn, name = lib().sfd_acoustic_misfit_gradient_ex, "sfd_acoustic_misfit_gradient_ex"
+    ctx.check(fn(ctx.handle, n, params, _ptr(vp), _ptr(den), ws, ds, ckpt_every, GRADIENT_METHODS[method],
+                 _ptr(grad), ctypes.byref(J)), name)
     return J.value, grad
 
 

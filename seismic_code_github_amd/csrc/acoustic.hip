@@ -642,7 +642,8 @@ int sfd_acoustic_plan_create(sfd_context* ctx, const sfd_acoustic_params* p, con
 }  // extern "C"
 
 int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
-                     const float* wavelet, int grad_period, sfd_acoustic_plan** out, int div_fields) {
+                     const float* wavelet, int grad_period, sfd_acoustic_plan** out, int div_fields,
+                     PlanModelCache* mc) {
     if (!ctx || !out) return SFD_EINVAL;
     *out = nullptr;
     int rc = validate(ctx, p, vp, wavelet);
@@ -678,10 +679,13 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
 
     const int nx = p->nx, nz = p->nz;
     const size_t n = (size_t)nx * nz;
+    // The model's fields from an earlier plan of the same call, if it had them.
+    const bool shared = mc && mc->valid && mc->vp == vp && mc->den == den && mc->nx == nx && mc->nz == nz &&
+                        mc->radius == p->radius && mc->mode == pl->mode && mc->dt == p->dt;
 
     // Stability: vmax dt sqrt(1/dx^2 + 1/dz^2) sum|C| <= 1.
-    float vmax = 0.f;
-    for (size_t c = 0; c < n; ++c) {
+    float vmax = shared ? mc->vmax : 0.f;
+    for (size_t c = 0; !shared && c < n; ++c) {
         if (!(vp[c] > 0.f) || !std::isfinite(vp[c]))
             return bail(fail(ctx, SFD_EINVAL, "vp must be finite and > 0 (cell %zu)", c));
         vmax = std::max(vmax, vp[c]);
@@ -695,12 +699,12 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
 
     // Per-cell coefficients.  FAST: kf = rho vp^2 dt, bf = dt/rho.  REF: kf is
     // the reference's float product DEN*Vp*Vp*dt (:293) and bf holds DEN.
-    std::vector<float> hk(n), hb(n);
+    std::vector<float> hk(shared ? 0 : n), hb(shared ? 0 : n);
     const bool ref = pl->mode == MODE_REF_SPONGE;
     // In fast arithmetic both are zero outside the updated cells [R, N-R)^2,
     // so the kernel's reads there yield zero velocities without masking.
     const int R = p->radius;
-    for (int i = 0; i < nx; ++i) {
+    for (int i = 0; !shared && i < nx; ++i) {
         for (int jj = 0; jj < nz; ++jj) {
             const size_t c = (size_t)i * nz + jj;
             const float d = den ? den[c] : 2400.0f;
@@ -710,7 +714,9 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
         }
     }
     // Uniform density: every updated cell's bf is the same number.
-    if (!ref) {
+    if (shared) {
+        pl->bf_const = mc->bf_const;
+    } else if (!ref) {
         bool uniform = true;
         const float d0 = den ? den[0] : 2400.0f;
         for (size_t c = 1; den && uniform && c < n; ++c) uniform = den[c] == d0;
@@ -782,13 +788,20 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
     }
     // Per-row arrays are padded with kPadRows rows on both sides (zeros; the
     // sponge row table with 2.0, "not an x-edge row").
-    std::vector<float> hk_pad((size_t)(nx + 2 * pad) * nz, 0.f), hb_pad(hk_pad.size(), 0.f);
-    std::copy(hk.begin(), hk.end(), hk_pad.begin() + (size_t)pad * nz);
-    std::copy(hb.begin(), hb.end(), hb_pad.begin() + (size_t)pad * nz);
+    std::vector<float> hk_pad, hb_pad;
+    if (shared) {
+        if (e == hipSuccess) e = hipMemcpyAsync(pl->kf.ptr, mc->kf, pl->kf.bytes, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(pl->bf.ptr, mc->bf, pl->bf.bytes, hipMemcpyDeviceToDevice, s);
+    } else {
+        hk_pad.assign((size_t)(nx + 2 * pad) * nz, 0.f);
+        hb_pad.assign(hk_pad.size(), 0.f);
+        std::copy(hk.begin(), hk.end(), hk_pad.begin() + (size_t)pad * nz);
+        std::copy(hb.begin(), hb.end(), hb_pad.begin() + (size_t)pad * nz);
+        if (e == hipSuccess) e = hipMemcpyAsync(pl->kf.ptr, hk_pad.data(), pl->kf.bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(pl->bf.ptr, hb_pad.data(), pl->bf.bytes, hipMemcpyHostToDevice, s);
+    }
     std::vector<float> hax_pad(nx + 2 * pad, 2.0f);
     std::copy(hax.begin(), hax.end(), hax_pad.begin() + pad);
-    if (e == hipSuccess) e = hipMemcpyAsync(pl->kf.ptr, hk_pad.data(), pl->kf.bytes, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(pl->bf.ptr, hb_pad.data(), pl->bf.bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = upload(pl->ax, hax_pad.data(), hax_pad.size(), s);
     if (e == hipSuccess) e = upload(pl->az, haz.data(), haz.size(), s);
     if (e == hipSuccess && !hpx.empty()) e = upload(pl->pml_x, hpx.data(), hpx.size(), s);
@@ -820,6 +833,20 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
     if (rc) return bail(rc);
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return bail(hip_check(ctx, e, "plan setup"));
+    if (mc && !shared) {
+        mc->valid = true;
+        mc->vp = vp;
+        mc->den = den;
+        mc->nx = nx;
+        mc->nz = nz;
+        mc->radius = p->radius;
+        mc->mode = pl->mode;
+        mc->dt = p->dt;
+        mc->vmax = vmax;
+        mc->bf_const = pl->bf_const;
+        mc->kf = pl->kf.ptr;
+        mc->bf = pl->bf.ptr;
+    }
     *out = pl;
     return SFD_OK;
 }

@@ -101,10 +101,23 @@ struct sfd_acoustic_plan {
 
 namespace sfd {
 
+// Model-derived fields shared by the plans of one call on one model (the
+// same vp and den arrays, grid, dt, radius and arithmetic): the first plan
+// fills it; later plans take its checked maximum velocity and copy its kf and
+// bf on the device instead of recomputing and uploading them.
+struct PlanModelCache {
+    bool valid = false;
+    const float *vp = nullptr, *den = nullptr;
+    int nx = 0, nz = 0, radius = 0, mode = -1;
+    float dt = 0.f, vmax = 0.f, bf_const = 0.f;
+    const void *kf = nullptr, *bf = nullptr;  // the first plan's device fields
+};
+
 // Creates a plan; grad_period > 0 also reserves the gradient workspace with
 // div_fields divergence fields (0 = grad_period of them).
 int plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
-                const float* wavelet, int grad_period, sfd_acoustic_plan** out, int div_fields = 0);
+                const float* wavelet, int grad_period, sfd_acoustic_plan** out, int div_fields = 0,
+                PlanModelCache* mc = nullptr);
 // Step arguments over the state fields (p, vx, vz)[cur] -> [cur ^ 1].
 AcousticStepArgs make_args(const sfd_acoustic_plan* pl, const Field* p, const Field* vx, const Field* vz, int cur,
                            int k);

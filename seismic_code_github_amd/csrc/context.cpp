@@ -165,6 +165,40 @@ int sfd_context_destroy(sfd_context* ctx) {
     return SFD_OK;
 }
 
+int sfd_device_alloc(sfd_context* ctx, size_t bytes, void** ptr) {
+    if (!ctx || !ptr) return SFD_EINVAL;
+    *ptr = nullptr;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    if (bytes == 0) return SFD_OK;
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return e == hipErrorOutOfMemory ? sfd::fail(ctx, SFD_ENOMEM, "device allocation of %zu bytes failed", bytes)
+                                        : sfd::hip_check(ctx, e, "hipMalloc");
+    }
+    return SFD_OK;
+}
+
+int sfd_device_copy(sfd_context* ctx, void* dst, const void* src, size_t bytes, int kind) {
+    if (!ctx || (bytes && (!dst || !src))) return SFD_EINVAL;
+    if (kind != 0 && kind != 1) return sfd::fail(ctx, SFD_EINVAL, "copy kind must be 0 (to device) or 1 (to host)");
+    if (bytes == 0) return SFD_OK;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    SFD_HIP(ctx, hipMemcpyAsync(dst, src, bytes, kind == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                                ctx->stream));
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SFD_OK;
+}
+
+int sfd_device_free(sfd_context* ctx, void* ptr) {
+    if (!ctx) return SFD_EINVAL;
+    if (!ptr) return SFD_OK;
+    SFD_HIP(ctx, hipSetDevice(ctx->device));
+    SFD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SFD_HIP(ctx, hipFree(ptr));
+    return SFD_OK;
+}
+
 const char* sfd_last_error(const sfd_context* ctx) {
     return ctx ? ctx->err.c_str() : "null context";
 }

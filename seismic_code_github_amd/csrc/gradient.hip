@@ -134,12 +134,13 @@ AcousticStepArgs adjoint_args(const sfd_acoustic_plan* pl, int k) {
 
 // Creates the plan with its gradient workspace and the receiver-cell tables.
 int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, const float* den,
-               const float* wavelet, int period, GradShot* g, int method = SFD_GRADIENT_RECOMPUTE) {
+               const float* wavelet, int period, GradShot* g, int method = SFD_GRADIENT_RECOMPUTE,
+               PlanModelCache* mc = nullptr) {
     sfd_acoustic_params q = *p;
     q.nsnap = 0;
     q.snap_steps = nullptr;
     const bool bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
-    int rc = plan_create(ctx, &q, vp, den, wavelet, period, &g->pl, bsave ? 1 : 0);
+    int rc = plan_create(ctx, &q, vp, den, wavelet, period, &g->pl, bsave ? 1 : 0, mc);
     if (rc) return rc;
     sfd_acoustic_plan* pl = g->pl;
     // Forward, recomputation and adjoint all run the tiled step at the
@@ -591,20 +592,21 @@ int group_checkpoint(GradGroup* G, int seg, bool save) {
 // Misfit gradient of up to G->shots.size() shots on one model; adds into
 // grad_vp and *misfit.
 int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp, const float* den,
-              const float* const* wavelet, const float* const* d_obs, int period, int method, float* grad_vp,
-              double* misfit) {
+              const float* const* wavelet, const float* const* d_obs, bool obs_dev, int period, int method,
+              float* grad_vp, double* misfit) {
     GradGroup G;
     G.ctx = ctx;
     G.nt = params[0].nt;
     G.period = std::min(period, G.nt);
     G.bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
     G.shots.resize(nshot);
+    PlanModelCache mc;  // the shots share vp and den: one computation of kf and bf
     for (int s = 0; s < nshot; ++s) {
         int rc = check_gradient_params(ctx, &params[s]);
         if (!rc && params[s].nt != G.nt) rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same nt", s);
         if (!rc && params[s].radius != params[0].radius)
             rc = fail(ctx, SFD_EINVAL, "shot %d: every shot needs the same radius", s);
-        if (!rc) rc = grad_setup(ctx, &params[s], vp, den, wavelet[s], G.period, &G.shots[s], method);
+        if (!rc) rc = grad_setup(ctx, &params[s], vp, den, wavelet[s], G.period, &G.shots[s], method, &mc);
         if (rc) return rc;
     }
     const int R = params[0].radius;
@@ -622,6 +624,36 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     if (rc) return rc;
     hipStream_t s = ctx->stream;
 
+    // Residual tables of every shot (receivers of each slot, in receiver
+    // order), uploaded before the forward is queued; one block of misfit
+    // partials per 256 residual cells.
+    DeviceBuffer tables, partial, obs_all;
+    std::vector<size_t> t_st(nshot), t_rv(nshot), poff(nshot + 1, 0), ooff(nshot + 1, 0);
+    {
+        std::vector<int> tab;
+        for (int q = 0; q < nshot; ++q) {
+            GradShot& g = G.shots[q];
+            const int nrec = params[q].nrec, nslot = g.pl->nslot;
+            const bool any = nrec > 0 && nslot > 0;
+            ooff[q + 1] = ooff[q] + (size_t)nrec * G.nt;
+            poff[q + 1] = poff[q] + (any ? ((size_t)nslot * G.nt + 255) / 256 : 0);
+            t_st[q] = tab.size();
+            if (!any) continue;
+            std::vector<int> st(nslot + 1, 0), rv(nrec);
+            for (int r = 0; r < nrec; ++r) ++st[g.slot_cell[r] + 1];
+            for (int k = 0; k < nslot; ++k) st[k + 1] += st[k];
+            std::vector<int> fill(st.begin(), st.end() - 1);
+            for (int r = 0; r < nrec; ++r) rv[fill[g.slot_cell[r]]++] = r;
+            tab.insert(tab.end(), st.begin(), st.end());
+            t_rv[q] = tab.size();
+            tab.insert(tab.end(), rv.begin(), rv.end());
+        }
+        hipError_t e = upload(tables, tab.data(), tab.size(), s);
+        if (e == hipSuccess) e = partial.alloc(std::max<size_t>(poff[nshot], 1) * 8);
+        if (e == hipSuccess && !obs_dev) e = obs_all.alloc(std::max<size_t>(ooff[nshot], 1) * 4);
+        if (e != hipSuccess) return hip_check(ctx, e, "residual buffers");
+    }
+
     // 1. forward, checkpointing before every segment but the first.
     for (auto& g : G.shots) {
         rc = zero_state(g.pl);
@@ -637,43 +669,33 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (rc) return rc;
     }
     for (auto& g : G.shots) g.pl->v_raw = !cpml;
-    // 2. residuals, misfit, seeds, on the device from the gather buffers.
-    {
-        DeviceBuffer obs, starts, recvs, partial;
-        size_t obs_max = 1;
-        for (int q = 0; q < nshot; ++q) obs_max = std::max(obs_max, (size_t)params[q].nrec * G.nt);
-        const hipError_t e0 = obs.alloc(obs_max * 4);
-        if (e0 != hipSuccess) return hip_check(ctx, e0, "residual buffers");
-        std::vector<double> parts;
-        for (int q = 0; q < nshot; ++q) {
-            GradShot& g = G.shots[q];
-            sfd_acoustic_plan* pl = g.pl;
-            const int nrec = params[q].nrec, nslot = pl->nslot;
-            if (nrec == 0 || nslot == 0) continue;
-            // Receivers of each slot, in receiver order.
-            std::vector<int> st(nslot + 1, 0), rv(nrec);
-            for (int r = 0; r < nrec; ++r) ++st[g.slot_cell[r] + 1];
-            for (int k = 0; k < nslot; ++k) st[k + 1] += st[k];
-            std::vector<int> fill(st.begin(), st.end() - 1);
-            for (int r = 0; r < nrec; ++r) rv[fill[g.slot_cell[r]]++] = r;
-            const size_t cells = (size_t)nslot * G.nt;
-            const int nb = (int)((cells + 255) / 256);
-            hipError_t e = upload(starts, st.data(), st.size(), s);
-            if (e == hipSuccess) e = upload(recvs, rv.data(), rv.size(), s);
-            if (e == hipSuccess) e = hipMemcpyAsync(obs.ptr, d_obs[q], (size_t)nrec * G.nt * 4, hipMemcpyHostToDevice, s);
-            if (e == hipSuccess && partial.bytes < (size_t)nb * 8) e = partial.alloc((size_t)nb * 8);
-            if (e != hipSuccess) return hip_check(ctx, e, "residual tables");
-            hipLaunchKernelGGL(residual_kernel, dim3(nb), dim3(256), 0, s, pl->rec_buf.as<float>(), nslot, G.nt,
-                               obs.as<float>(), starts.as<int>(), recvs.as<int>(), pl->res.as<float>(),
-                               partial.as<double>());
-            SFD_HIP(ctx, hipGetLastError());
-            parts.resize(nb);
-            SFD_HIP(ctx, hipMemcpyAsync(parts.data(), partial.ptr, (size_t)nb * 8, hipMemcpyDeviceToHost, s));
-            SFD_HIP(ctx, hipStreamSynchronize(s));
-            double m = 0.0;
-            for (double v : parts) m += v;
-            if (misfit) *misfit += m;
-        }
+    // 2. residuals and seeds on the device from the gather buffers.  Host
+    // data is copied on a second stream while the forward runs; device data
+    // is read in place.  The misfit partials come back with the gradient.
+    if (!obs_dev && ooff[nshot] > 0) {
+        hipStream_t side = nullptr;
+        hipEvent_t done = nullptr;
+        hipError_t e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+        for (int q = 0; q < nshot && e == hipSuccess; ++q)
+            if (ooff[q + 1] > ooff[q])
+                e = hipMemcpyAsync(obs_all.as<float>() + ooff[q], d_obs[q], (ooff[q + 1] - ooff[q]) * 4,
+                                   hipMemcpyHostToDevice, side);
+        if (e == hipSuccess) e = hipEventRecord(done, side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, done, 0);
+        if (done) (void)hipEventDestroy(done);
+        if (side) (void)hipStreamDestroy(side);
+        if (e != hipSuccess) return hip_check(ctx, e, "observed data transfer");
+    }
+    for (int q = 0; q < nshot; ++q) {
+        sfd_acoustic_plan* pl = G.shots[q].pl;
+        const int nslot = pl->nslot;
+        if (poff[q + 1] == poff[q]) continue;
+        const float* o = obs_dev ? d_obs[q] : obs_all.as<float>() + ooff[q];
+        hipLaunchKernelGGL(residual_kernel, dim3((unsigned)(poff[q + 1] - poff[q])), dim3(256), 0, s,
+                           pl->rec_buf.as<float>(), nslot, G.nt, o, tables.as<int>() + t_st[q],
+                           tables.as<int>() + t_rv[q], pl->res.as<float>(), partial.as<double>() + poff[q]);
+        SFD_HIP(ctx, hipGetLastError());
     }
     // 3. backward: per segment, restore, recompute with divergences, adjoint.
     for (auto& g : G.shots) {
@@ -730,7 +752,18 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         }
     }
     // 4. dJ/dvp summed over the group's shots on the device, then the source
-    // terms of each shot.
+    // terms of each shot; the misfit from the partials (per shot in block
+    // order, then shot after shot).
+    if (misfit && poff[nshot] > 0) {
+        std::vector<double> parts(poff[nshot]);
+        SFD_HIP(ctx, hipMemcpyAsync(parts.data(), partial.ptr, poff[nshot] * 8, hipMemcpyDeviceToHost, s));
+        SFD_HIP(ctx, hipStreamSynchronize(s));
+        for (int q = 0; q < nshot; ++q) {
+            double m = 0.0;
+            for (size_t b = poff[q]; b < poff[q + 1]; ++b) m += parts[b];
+            *misfit += m;
+        }
+    }
     {
         sfd_acoustic_plan* p0 = G.shots[0].pl;
         const size_t n = (size_t)p0->nx * p0->nz;
@@ -767,6 +800,41 @@ int host_finite(sfd_context* ctx, const float* a, size_t n, const char* what) {
     return SFD_OK;
 }
 
+// The misfit gradient behind both entry points (observed data on the host
+// or, obs_dev, in device memory).
+int misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp, const float* den,
+                    const float* const* wavelet, const float* const* d_obs, bool obs_dev, int ckpt_every, int method,
+                    float* grad_vp, double* misfit) {
+    if (!ctx || nshot < 1 || !params || !vp || !wavelet || !d_obs || !grad_vp) return SFD_EINVAL;
+    if (method != SFD_GRADIENT_RECOMPUTE && method != SFD_GRADIENT_BOUNDARY_SAVING)
+        return fail(ctx, SFD_EINVAL, "unknown gradient method %d", method);
+    for (int s = 1; s < nshot; ++s)
+        if (params[s].nx != params[0].nx || params[s].nz != params[0].nz)
+            return fail(ctx, SFD_EINVAL, "shot %d: every shot of a gradient shares the model grid", s);
+    if (ckpt_every < 1) return fail(ctx, SFD_EINVAL, "checkpoint interval must be >= 1");
+    std::fill(grad_vp, grad_vp + (size_t)params[0].nx * params[0].nz, 0.f);
+    if (misfit) *misfit = 0.0;
+    // Groups sized so that their workspaces (state, adjoint state, the
+    // divergences of one segment or, with boundary saving, one divergence
+    // field and the band of every step, and the checkpoints of every shot)
+    // stay below 64 GiB of device memory.
+    const double field = (double)params[0].nx * params[0].nz * 4.0;
+    const int period = std::min(ckpt_every, std::max(params[0].nt, 1));
+    const bool bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
+    const double band = 4.0 * params[0].radius * (params[0].nx + params[0].nz) * 4.0 * 3.0 * params[0].nt;
+    const double per_shot = field * (16.0 + (bsave ? 1 : period)) * 1.2 +
+                            field * 3.0 * ((params[0].nt + period - 1) / period) + (bsave ? band : 0.0);
+    const int gmax = (int)std::max(1.0, std::min((double)nshot, 64.0 * (1ull << 30) / per_shot));
+    for (int s0 = 0; s0 < nshot; s0 += gmax) {
+        const int n = std::min(gmax, nshot - s0);
+        const int rc = group_run(ctx, n, params + s0, vp, den, wavelet + s0, d_obs + s0, obs_dev, period, method,
+                                 grad_vp, misfit);
+        if (rc) return rc;
+    }
+    if (misfit && !std::isfinite(*misfit)) return fail(ctx, SFD_ENAN, "the misfit is not finite: the run diverged");
+    return host_finite(ctx, grad_vp, (size_t)params[0].nx * params[0].nz, "the gradient");
+}
+
 }  // namespace
 }  // namespace sfd
 
@@ -791,34 +859,15 @@ int sfd_acoustic_misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic
 int sfd_acoustic_misfit_gradient_ex(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp,
                                     const float* den, const float* const* wavelet, const float* const* d_obs,
                                     int ckpt_every, int method, float* grad_vp, double* misfit) {
-    if (!ctx || nshot < 1 || !params || !vp || !wavelet || !d_obs || !grad_vp) return SFD_EINVAL;
-    if (method != SFD_GRADIENT_RECOMPUTE && method != SFD_GRADIENT_BOUNDARY_SAVING)
-        return fail(ctx, SFD_EINVAL, "unknown gradient method %d", method);
-    for (int s = 1; s < nshot; ++s)
-        if (params[s].nx != params[0].nx || params[s].nz != params[0].nz)
-            return fail(ctx, SFD_EINVAL, "shot %d: every shot of a gradient shares the model grid", s);
-    if (ckpt_every < 1) return fail(ctx, SFD_EINVAL, "checkpoint interval must be >= 1");
-    std::fill(grad_vp, grad_vp + (size_t)params[0].nx * params[0].nz, 0.f);
-    if (misfit) *misfit = 0.0;
-    // Groups sized so that their workspaces (state, adjoint state, the
-    // divergences of one segment or, with boundary saving, one divergence
-    // field and the band of every step, and the checkpoints of every shot)
-    // stay below 64 GiB of device memory.
-    const double field = (double)params[0].nx * params[0].nz * 4.0;
-    const int period = std::min(ckpt_every, std::max(params[0].nt, 1));
-    const bool bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
-    const double band = 4.0 * params[0].radius * (params[0].nx + params[0].nz) * 4.0 * 3.0 * params[0].nt;
-    const double per_shot = field * (16.0 + (bsave ? 1 : period)) * 1.2 +
-                            field * 3.0 * ((params[0].nt + period - 1) / period) + (bsave ? band : 0.0);
-    const int gmax = (int)std::max(1.0, std::min((double)nshot, 64.0 * (1ull << 30) / per_shot));
-    for (int s0 = 0; s0 < nshot; s0 += gmax) {
-        const int n = std::min(gmax, nshot - s0);
-        const int rc = group_run(ctx, n, params + s0, vp, den, wavelet + s0, d_obs + s0, period, method, grad_vp,
-                                 misfit);
-        if (rc) return rc;
-    }
-    if (misfit && !std::isfinite(*misfit)) return fail(ctx, SFD_ENAN, "the misfit is not finite: the run diverged");
-    return host_finite(ctx, grad_vp, (size_t)params[0].nx * params[0].nz, "the gradient");
+    return sfd::misfit_gradient(ctx, nshot, params, vp, den, wavelet, d_obs, false, ckpt_every, method, grad_vp,
+                                misfit);
+}
+
+int sfd_acoustic_misfit_gradient_dev(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, const float* vp,
+                                     const float* den, const float* const* wavelet, const float* const* d_obs_dev,
+                                     int ckpt_every, int method, float* grad_vp, double* misfit) {
+    return sfd::misfit_gradient(ctx, nshot, params, vp, den, wavelet, d_obs_dev, true, ckpt_every, method, grad_vp,
+                                misfit);
 }
 
 }  // extern "C"

@@ -346,3 +346,29 @@ def test_cpml_shot_group_and_boundary_saving(ctx):
     print(f"inside the box: boundary vs recompute {e_rb:.2e}, vs fp64 {e_64:.2e}")
     assert np.abs(gb[box]).max() > 0
     assert e_rb <= 1e-4 and e_64 <= 1e-4
+
+
+def test_misfit_gradient_device_data_bitwise(ctx):
+    """Observed gathers resident on the GPU (sfd_acoustic_misfit_gradient_dev,
+    through sfd_device_alloc / sfd_device_copy) give the host path's misfit
+    and gradient bit for bit, with both methods; a wrongly shaped device
+    gather is rejected before the call; the device copy round-trips."""
+    print("case: device-resident observed data", flush=True)
+    shots, obs = [], []
+    for sx in (20, 34, 48):
+        shot, _, _ = case(nt=150, src=(sx, 15))
+        shots.append(shot)
+        true = sfd.AcousticShot(**{**shot.__dict__, "vp": shot.vp * np.float32(1.02), "_keep": []})
+        obs.append(sfd.acoustic_forward(true, ctx)[0])
+    dev = [sfd.DeviceArray(o, ctx) for o in obs]
+    np.testing.assert_array_equal(dev[1].numpy(), obs[1])
+    for method in ("recompute", "boundary"):
+        J0, g0 = sfd.misfit_gradient(shots, obs, ckpt_every=40, ctx=ctx, method=method)
+        J1, g1 = sfd.misfit_gradient(shots, dev, ckpt_every=40, ctx=ctx, method=method)
+        assert J0 > 0 and J1 == J0, method
+        np.testing.assert_array_equal(g1, g0, err_msg=method)
+    short = [sfd.DeviceArray(o[:, :-1], ctx) for o in obs]
+    with pytest.raises(ValueError):
+        sfd.misfit_gradient(shots, short, ckpt_every=40, ctx=ctx)
+    for d in dev + short:
+        d.close()
