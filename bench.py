@@ -378,10 +378,9 @@ def visco_leg(nt_err, nt, no_cpu, nbatch=256):
             out["batch"]["error_sample"] = f"strzz gather of shot 1, all {s0.nt} samples"
         out["batch"]["traffic"] = load_traffic(f"visco_batch_{nbatch}x{bnx}x{bnz}_traffic.json", "hbm_bytes_per_step",
                                                "visco_el_velocity_batch" if s0.elastic() else "visco_ac_velocity_batch")
-    tpath = os.path.join(ROOT, "profiles", "visco_step_L3_2081x1081_traffic.json")
-    if os.path.exists(tpath) and nt == 4000:
-        with open(tpath) as f:
-            out["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    if nt == 4000 and vshape[0] == 0:
+        out["traffic"] = load_traffic("visco_step_L3_2081x1081_traffic.json", "hbm_bytes_per_launch",
+                                      f"visco_ac_velocity_kernel<{shot.radius}, {vshape[1]}, {vshape[2]}>")
     if not no_cpu:
         import oracle
         cores = min(16, len(os.sched_getaffinity(0)))

@@ -378,13 +378,17 @@ int sfd_visco_plan_fetch(sfd_visco_plan *plan, float *record, float *snaps);
 int sfd_visco_plan_fetch_xx(sfd_visco_plan *plan, float *record_xx);
 /* vs = 0: explicit waves per workgroup (2 or 4) and rows marched per
  * workgroup select the fused march kernel; (0, 0) = the tiled step.  The
- * visco-elastic kernels have one tiling (no effect). */
+ * visco-elastic step has the tiled launches only (no effect). */
 int sfd_visco_plan_set_tiling(sfd_visco_plan *plan, int waves, int chunk_rows);
+/* The tiled step (two launches per step) at tile_rows x 64 columns per
+ * workgroup of `waves` waves: 8 x 8 (the default; (0, 0)) or 16 x 8.
+ * Results are bitwise the same at either shape. */
+int sfd_visco_plan_set_tiled(sfd_visco_plan *plan, int tile_rows, int waves);
 /* Autotuner (no counterpart in the reference), as sfd_acoustic_plan_autotune:
- * the tiled step against (vs = 0) the fused march kernel at 2 and 4 waves,
- * cached for the grid class (radius, L, vs = 0 or not, nx and nz to the
- * nearest power of 2); resets the state.  get_shape: kind 0 = tiled (a = 8
- * rows, b = 8 waves), 1 = march (a = waves, b = rows per chunk). */
+ * the tiled step at 8 x 8 and 16 x 8 against (vs = 0) the fused march kernel
+ * at 2 and 4 waves, cached for the grid class (radius, L, vs = 0 or not, nx
+ * and nz to the nearest power of 2); resets the state.  get_shape: kind 0 =
+ * tiled (a = rows, b = waves), 1 = march (a = waves, b = rows per chunk). */
 int sfd_visco_plan_autotune(sfd_visco_plan *plan, int trial_steps, float *best_us);
 int sfd_visco_plan_get_shape(const sfd_visco_plan *plan, int *kind, int *a, int *b);
 int sfd_visco_plan_destroy(sfd_visco_plan *plan);

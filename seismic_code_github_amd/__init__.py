@@ -245,6 +245,7 @@ _SIGNATURES = {
     "sfd_visco_plan_fetch": (ctypes.c_int, [ctypes.c_void_p, _FP, _FP]),
     "sfd_visco_plan_fetch_xx": (ctypes.c_int, [ctypes.c_void_p, _FP]),
     "sfd_visco_plan_set_tiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "sfd_visco_plan_set_tiled": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "sfd_visco_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sfd_acoustic_batch_set_group_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t]),
     "sfd_acoustic_batch_get_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
@@ -864,6 +865,11 @@ class ViscoPlan:
 
     def set_tiling(self, waves: int = 0, chunk_rows: int = 0):
         self.ctx.check(lib().sfd_visco_plan_set_tiling(self.handle, waves, chunk_rows), "visco set_tiling")
+
+    def set_tiled(self, tile_rows: int = 0, waves: int = 0):
+        """The tiled two-launch step at tile_rows x 64 columns, `waves` waves
+        (8 x 8, the default, or 16 x 8)."""
+        self.ctx.check(lib().sfd_visco_plan_set_tiled(self.handle, tile_rows, waves), "visco set_tiled")
 
     def autotune(self, trial_steps: int = 20) -> float:
         us = ctypes.c_float()

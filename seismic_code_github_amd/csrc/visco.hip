@@ -40,6 +40,7 @@ struct sfd_visco_plan {
     sfd::ViscoElArgs ve{};
     sfd::DeviceBuffer ve_arena, ve_xs, ve_zs, ve_rx, ve_rz, rec_zz, rec_xx;
     int ve_pitch = 0, ve_rows = 0;
+    int ve_tx = 8, ve_nw = 8;  // tile rows and waves of the two launches
     void (*ve_vel)(sfd::ViscoElArgs) = nullptr;
     void (*ve_str)(sfd::ViscoElArgs) = nullptr;
 
@@ -108,34 +109,35 @@ ViscoKernel visco_pick(int R, int L, int nw) {
     }
 }
 
-constexpr int kVeTX = 8, kVeNW = 8;  // 8 rows x 64 columns, one row per wave
+// Tile shapes of the two launches (rows x 64 columns, waves): 8 x 8 (one
+// row per wave, the default and the batches' shape) and 16 x 8.
+constexpr int kVeTX = 8, kVeNW = 8;
+constexpr int kVeTXMax = 16;  // the padded fields' extra rows cover the tallest tile
+constexpr int kVeShapes[][2] = {{8, 8}, {16, 8}};
 
-template <int R>
-bool ve_pick_r(int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
-    *v = visco_el_velocity_kernel<R, kVeTX, kVeNW>;
+template <int R, int TX, int NW>
+bool ve_pick_shape(int L, bool elastic, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    *v = elastic ? visco_el_velocity_kernel<R, TX, NW> : visco_ac_velocity_kernel<R, TX, NW>;
     switch (L) {
-    case 1: *st = visco_el_stress_kernel<R, 1, kVeTX, kVeNW>; return true;
-    case 2: *st = visco_el_stress_kernel<R, 2, kVeTX, kVeNW>; return true;
-    case 3: *st = visco_el_stress_kernel<R, 3, kVeTX, kVeNW>; return true;
+    case 1: *st = elastic ? visco_el_stress_kernel<R, 1, TX, NW> : visco_ac_stress_kernel<R, 1, TX, NW>; return true;
+    case 2: *st = elastic ? visco_el_stress_kernel<R, 2, TX, NW> : visco_ac_stress_kernel<R, 2, TX, NW>; return true;
+    case 3: *st = elastic ? visco_el_stress_kernel<R, 3, TX, NW> : visco_ac_stress_kernel<R, 3, TX, NW>; return true;
     default: return false;
     }
 }
 
 template <int R>
-bool va_pick_r(int L, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
-    *v = visco_ac_velocity_kernel<R, kVeTX, kVeNW>;
-    switch (L) {
-    case 1: *st = visco_ac_stress_kernel<R, 1, kVeTX, kVeNW>; return true;
-    case 2: *st = visco_ac_stress_kernel<R, 2, kVeTX, kVeNW>; return true;
-    case 3: *st = visco_ac_stress_kernel<R, 3, kVeTX, kVeNW>; return true;
-    default: return false;
-    }
+bool ve_pick_r(int L, bool elastic, int tx, int nw, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    if (tx == 8 && nw == 8) return ve_pick_shape<R, 8, 8>(L, elastic, v, st);
+    if (tx == 16 && nw == 8) return ve_pick_shape<R, 16, 8>(L, elastic, v, st);
+    return false;
 }
 
-// The two tiled launches of the visco-elastic (elastic) or visco-acoustic step.
-bool ve_pick(int R, int L, bool elastic, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
-    if (R == 4) return elastic ? ve_pick_r<4>(L, v, st) : va_pick_r<4>(L, v, st);
-    if (R == 5) return elastic ? ve_pick_r<5>(L, v, st) : va_pick_r<5>(L, v, st);
+// The two tiled launches of the visco-elastic (elastic) or visco-acoustic
+// step at tile shape tx x nw.
+bool ve_pick(int R, int L, bool elastic, int tx, int nw, void (**v)(ViscoElArgs), void (**st)(ViscoElArgs)) {
+    if (R == 4) return ve_pick_r<4>(L, elastic, tx, nw, v, st);
+    if (R == 5) return ve_pick_r<5>(L, elastic, tx, nw, v, st);
     return false;
 }
 
@@ -331,9 +333,12 @@ int ve_setup(sfd_visco_plan* pl, const sfd_visco_params* p, bool elastic, const 
              const std::vector<float>* hkrs) {
     sfd_context* ctx = pl->ctx;
     const int nx = pl->nx, nz = pl->nz, L = pl->nmech, R = pl->radius;
-    if (!ve_pick(R, L, elastic, &pl->ve_vel, &pl->ve_str)) return fail(ctx, SFD_ENOTSUP, "no tiled visco kernel");
+    pl->ve_tx = kVeTX;
+    pl->ve_nw = kVeNW;
+    if (!ve_pick(R, L, elastic, pl->ve_tx, pl->ve_nw, &pl->ve_vel, &pl->ve_str))
+        return fail(ctx, SFD_ENOTSUP, "no tiled visco kernel");
     pl->ve_pitch = (nz + kVeTZ - 1) / kVeTZ * kVeTZ + 2 * kVePad;
-    pl->ve_rows = nx + kVeTX + 2 * kVePad;
+    pl->ve_rows = nx + kVeTXMax + 2 * kVePad;
     const size_t fe = (size_t)pl->ve_pitch * pl->ve_rows;  // floats per padded field
     const int nstate = elastic ? 5 + 3 * L : 3 + L, nmodel = elastic ? 5 : 3, nparts = elastic ? 5 + 3 * L : 2 + L;
     const int xw = pl->xlo + pl->xhi, zw = pl->zlo + pl->zhi;
@@ -349,7 +354,7 @@ int ve_setup(sfd_visco_plan* pl, const sfd_visco_params* p, bool elastic, const 
     a.nz = nz;
     a.pitch = pl->ve_pitch;
     a.ntz = (nz + kVeTZ - 1) / kVeTZ;
-    a.ntiles = a.ntz * ((nx + kVeTX - 1) / kVeTX);
+    a.ntiles = a.ntz * ((nx + pl->ve_tx - 1) / pl->ve_tx);
     a.vx = cell0(0);
     a.vz = cell0(1);
     std::vector<const std::vector<float>*> model;
@@ -428,6 +433,22 @@ std::string visco_class(const sfd_visco_plan* pl) {
     return buf;
 }
 
+// Switches the plan to the two tiled launches at tile_rows x waves.
+int ve_set_shape(sfd_visco_plan* pl, int tile_rows, int waves) {
+    void (*v)(ViscoElArgs) = nullptr;
+    void (*st)(ViscoElArgs) = nullptr;
+    if (!ve_pick(pl->radius, pl->nmech, pl->elastic, tile_rows, waves, &v, &st))
+        return fail(pl->ctx, SFD_EINVAL, "no tiled visco step of %d rows x %d waves (8 x 8 and 16 x 8 are built)",
+                    tile_rows, waves);
+    pl->ve_tx = tile_rows;
+    pl->ve_nw = waves;
+    pl->ve_vel = v;
+    pl->ve_str = st;
+    pl->ve.ntiles = pl->ve.ntz * ((pl->nx + tile_rows - 1) / tile_rows);
+    pl->ve_on = true;
+    return SFD_OK;
+}
+
 int ve_zero(sfd_visco_plan* pl) {
     sfd_context* ctx = pl->ctx;
     // State (vx, vz, the stresses, the memory fields) lies at the front of
@@ -474,7 +495,7 @@ int ve_last_gather(sfd_visco_plan* pl, int ns) {
 int ve_enqueue(sfd_visco_plan* pl, int nsteps = -1) {
     sfd_context* ctx = pl->ctx;
     ViscoElArgs a = pl->ve;
-    const dim3 grid(a.ntiles), block(64 * kVeNW);
+    const dim3 grid(a.ntiles), block(64 * pl->ve_nw);
     const bool rec = pl->nrec > 0 && pl->nt > 0;
     size_t snap_next = 0;
     const int ns = nsteps < 0 ? pl->nsteps : std::min(nsteps, pl->nsteps);
@@ -820,6 +841,10 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
         if (e2 != hipSuccess) return bail(hip_check(ctx, e2, "visco-elastic set-up"));
         rc = ve_setup(pl, p, true, hb, hku, hkr, &hkus, &hkrs);
         if (rc) return bail(rc);
+        // A tile shape the context's autotuner measured for this grid class.
+        const auto tuned = ctx->tuning.find(visco_class(pl));
+        if (tuned != ctx->tuning.end() && tuned->second.kind == 0)
+            (void)ve_set_shape(pl, tuned->second.a, tuned->second.b);
         rc = ve_zero(pl);
         if (rc) return bail(rc);
         e2 = hipStreamSynchronize(sm);
@@ -895,9 +920,11 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     // The tiled two-launch step runs by default.
     rc = ve_setup(pl, p, false, hb, hku, hkr, nullptr, nullptr);
     if (rc) return bail(rc);
-    // A march tiling the context's autotuner measured faster for this grid
-    // class wins over the tiled step.
+    // A shape the context's autotuner measured for this grid class: a tile
+    // shape of the tiled step, or a march tiling.
     const auto tuned = ctx->tuning.find(visco_class(pl));
+    if (tuned != ctx->tuning.end() && tuned->second.kind == 0)
+        (void)ve_set_shape(pl, tuned->second.a, tuned->second.b);
     if (tuned != ctx->tuning.end() && tuned->second.kind == 1)
         (void)sfd_visco_plan_set_tiling(pl, tuned->second.a, tuned->second.b);
     rc = ve_zero(pl);
@@ -909,9 +936,18 @@ int sfd_visco_plan_create(sfd_context* ctx, const sfd_visco_params* p, const flo
     return SFD_OK;
 }
 
+int sfd_visco_plan_set_tiled(sfd_visco_plan* pl, int tile_rows, int waves) {
+    if (!pl) return SFD_EINVAL;
+    if (tile_rows == 0 && waves == 0) {
+        tile_rows = kVeTX;
+        waves = kVeNW;
+    }
+    return ve_set_shape(pl, tile_rows, waves);
+}
+
 int sfd_visco_plan_set_tiling(sfd_visco_plan* pl, int waves, int chunk_rows) {
     if (!pl) return SFD_EINVAL;
-    if (pl->elastic) return SFD_OK;  // the visco-elastic kernels have one tiling
+    if (pl->elastic) return SFD_OK;  // the visco-elastic step has the tiled launches only
     // vs = 0: automatic = the tiled step; an explicit tiling selects the
     // fused march kernel.
     pl->ve_on = waves == 0 && chunk_rows == 0;
@@ -944,12 +980,16 @@ int sfd_visco_plan_autotune(sfd_visco_plan* pl, int trial_steps, float* best_us)
         if (m < 0) return pl->ve_on ? ve_zero(pl) : visco_zero(pl);
         return pl->ve_on ? ve_enqueue(pl, m) : visco_enqueue(pl, 0, m);
     };
-    // Candidates: the tiled step, and with vs = 0 the march kernel at 2 and
-    // 4 waves with its automatic chunk.
-    std::vector<TunedShape> cand(1);
-    cand[0].kind = 0;
-    cand[0].a = kVeTX;
-    cand[0].b = kVeNW;
+    // Candidates: the tiled step at each built tile shape, and with vs = 0
+    // the march kernel at 2 and 4 waves with its automatic chunk.
+    std::vector<TunedShape> cand;
+    for (auto& sh : kVeShapes) {
+        TunedShape t;
+        t.kind = 0;
+        t.a = sh[0];
+        t.b = sh[1];
+        cand.push_back(t);
+    }
     if (!pl->elastic)
         for (int w : {2, 4}) {
             TunedShape t;
@@ -958,7 +998,7 @@ int sfd_visco_plan_autotune(sfd_visco_plan* pl, int trial_steps, float* best_us)
             cand.push_back(t);
         }
     auto apply = [&](const TunedShape& t) {
-        return t.kind == 0 ? sfd_visco_plan_set_tiling(pl, 0, 0) : sfd_visco_plan_set_tiling(pl, t.a, t.b);
+        return t.kind == 0 ? ve_set_shape(pl, t.a, t.b) : sfd_visco_plan_set_tiling(pl, t.a, t.b);
     };
     float warm = 0.f;
     int rc = time_steps(ctx, std::min(pl->nsteps, 4 * n), run, &warm);
@@ -989,8 +1029,8 @@ int sfd_visco_plan_autotune(sfd_visco_plan* pl, int trial_steps, float* best_us)
 int sfd_visco_plan_get_shape(const sfd_visco_plan* pl, int* kind, int* a, int* b) {
     if (!pl || !kind || !a || !b) return SFD_EINVAL;
     *kind = pl->ve_on ? 0 : 1;
-    *a = pl->ve_on ? kVeTX : pl->nw;
-    *b = pl->ve_on ? kVeNW : pl->chunk;
+    *a = pl->ve_on ? pl->ve_tx : pl->nw;
+    *b = pl->ve_on ? pl->ve_nw : pl->chunk;
     return SFD_OK;
 }
 
@@ -1082,8 +1122,9 @@ int sfd_visco_batch_create(sfd_context* ctx, int nshot, const sfd_visco_params* 
             return fail(ctx, SFD_EINVAL,
                         "shot %d: the shots of a batch share nt, the delay, the radius, L and whether vs > 0", i);
         }
-        // The batch runs the tiled step (a cached march shape does not apply).
-        rc = sfd_visco_plan_set_tiling(pl, 0, 0);
+        // The batch runs the tiled step at its one tile shape (a cached march
+        // tiling or tile shape does not apply).
+        rc = ve_set_shape(pl, kVeTX, kVeNW);
         if (rc) {
             delete b;
             return rc;

@@ -174,7 +174,7 @@ def test_visco_autotune():
         us = pl.autotune(10)
         kind, a, b = pl.shape()
         print(f"tuned: kind {kind} ({a}, {b}) {us:.2f} us/step", flush=True)
-        assert (kind, a, b) == (0, 8, 8) or (kind == 1 and a in (2, 4) and b > 0)
+        assert (kind == 0 and (a, b) in {(8, 8), (16, 8)}) or (kind == 1 and a in (2, 4) and b > 0)
         pl.run()
         r1 = pl.fetch()[0]
         e = float(np.linalg.norm(r1.astype(np.float64) - r0) / np.linalg.norm(r0))

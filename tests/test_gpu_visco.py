@@ -99,6 +99,33 @@ def test_visco_tilings_bitwise_identical(ctx):
         assert np.array_equal(r, recs[0])
 
 
+@pytest.mark.parametrize("elastic", [False, True])
+def test_visco_tile_shapes_bitwise(ctx, elastic):
+    """The tiled step at 16 x 8 tiles (sfd_visco_plan_set_tiled) gives the 8 x 8
+    default's gathers (strzz and strxx) and snapshots bit for bit, vs = 0 and
+    vs > 0, on a grid whose rows are not a multiple of 16."""
+    kw = dict(snap_steps=(200, 399), snap_fields=sfd.FIELD_P | sfd.FIELD_VX | sfd.FIELD_VZ)
+    shot = small_shot(nmech=3 if not elastic else 2, nx=171, nz=150, nt=400, **kw)
+    if elastic:
+        shot = sfd.ViscoShot(**{**shot.__dict__, "vs": (shot.vp / np.float32(1.9)).astype(np.float32),
+                                "qs": (0.8 * shot.qp).astype(np.float32), "_keep": []})
+    print(f"case: visco tile shapes elastic={elastic}", flush=True)
+    out = []
+    for shape in ((8, 8), (16, 8), (0, 0)):
+        with sfd.ViscoPlan(shot, ctx) as pl:
+            pl.set_tiled(*shape)
+            assert pl.shape() == (0, shape[0] or 8, shape[1] or 8)
+            pl.run()
+            out.append((*pl.fetch(), pl.fetch_xx()))
+    assert np.abs(out[0][0]).max() > 0
+    for o in out[1:]:
+        for a, b in zip(o, out[0]):
+            np.testing.assert_array_equal(a, b)
+    with sfd.ViscoPlan(shot, ctx) as pl:
+        with pytest.raises(sfd.SeisfdError):
+            pl.set_tiled(32, 8)
+
+
 @pytest.mark.parametrize("radius,nmech", [(5, 3), (4, 1)])
 def test_visco_tiled_step_and_march_kernel(ctx, radius, nmech):
     """vs = 0 runs two tiled launches per step by default; an explicit tiling
