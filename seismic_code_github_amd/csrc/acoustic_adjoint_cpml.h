@@ -40,6 +40,8 @@
 #include "acoustic_tiled.h"
 #include "device_util.h"
 
+#include <type_traits>
+
 namespace sfd {
 
 // Strip row of row i (x strips: the first xw rows, then the last xw), -1
@@ -50,66 +52,110 @@ __device__ __forceinline__ int adj_xstrip(int i, int nx, int xw) {
 
 // Rows [i0 - HR, i0 + TX + HR) x columns [j0 - HC, j0 + 64 + HC) of the value
 // f(i, j) (0 off the updated cells [R, nx - R) x [R, nz - R)) into
-// lds[row][col]; all loads issued before the first store.
+// lds[row][col]; all loads issued before the first store.  Each wave takes
+// whole rows of the 64 tile columns (f called with a wave-uniform row and
+// std::true_type), then the workgroup the 2 HC edge columns of every row (f
+// called with std::false_type: row and column per lane).  wv: the wave's
+// index, wave-uniform.
 template <int TX, int NW, int HR, int HC, class F>
-__device__ __forceinline__ void adj_stage(float* lds, int i0, int j0, int nx, int nz, int R, F f) {
-    constexpr int W = kTiledTZ + 2 * HC, Hh = TX + 2 * HR, N = W * Hh, T = 64 * NW;
-    constexpr int ITER = (N + T - 1) / T;
-    float v[ITER];
+__device__ __forceinline__ void adj_stage(float* lds, int i0, int j0, int nx, int nz, int R, int wv, F f) {
+    constexpr int W = kTiledTZ + 2 * HC, Hh = TX + 2 * HR, T = 64 * NW;
+    constexpr int RI = (Hh + NW - 1) / NW;
+    constexpr int EW = 2 * HC, E = EW * Hh, EWd = EW > 0 ? EW : 1;
+    constexpr int EI = E > 0 ? (E + T - 1) / T : 0;
+    const int lane = threadIdx.x & 63;
+    const int gj = j0 + lane;
+    const bool jin = gj >= R && gj < nz - R;
+    float v[RI];
 #pragma unroll
-    for (int q = 0; q < ITER; ++q) {
+    for (int q = 0; q < RI; ++q) {
+        const int r = wv + q * NW, gi = i0 - HR + r;
+        v[q] = 0.f;
+        if ((RI * NW == Hh || r < Hh) && gi >= R && gi < nx - R) {
+            const float x = f(gi, jin ? gj : R, std::true_type{});  // every lane reads an updated cell
+            v[q] = jin ? x : 0.f;
+        }
+    }
+    float ve[EI > 0 ? EI : 1];
+    int le[EI > 0 ? EI : 1];
+#pragma unroll
+    for (int q = 0; q < EI; ++q) {
         const int e = threadIdx.x + q * T;
-        const int r = e / W, c = e - r * W;
-        const int gi = i0 - HR + r, gj = j0 - HC + c;
-        const bool upd = e < N && gi >= R && gi < nx - R && gj >= R && gj < nz - R;
-        v[q] = upd ? f(gi, gj) : 0.f;
+        const int r = e / EWd, c = e - r * EWd;
+        const int col = c < HC ? c : c + kTiledTZ;
+        const int gi = i0 - HR + r, gjj = j0 - HC + col;
+        le[q] = e < E ? r * W + col : -1;
+        const bool upd = e < E && gi >= R && gi < nx - R && gjj >= R && gjj < nz - R;
+        ve[q] = upd ? f(gi, gjj, std::false_type{}) : 0.f;
     }
 #pragma unroll
-    for (int q = 0; q < ITER; ++q) {
-        const int e = threadIdx.x + q * T;
-        if (e < N) lds[e] = v[q];
+    for (int q = 0; q < RI; ++q) {
+        const int r = wv + q * NW;
+        if (RI * NW == Hh || r < Hh) lds[r * W + HC + lane] = v[q];
     }
+#pragma unroll
+    for (int q = 0; q < EI; ++q)
+        if (le[q] >= 0) lds[le[q]] = ve[q];
+}
+
+// A load of a field (row stride rs bytes) at (row, col) through a buffer
+// view: the row's offset as a scalar where the row is wave-uniform (U), else
+// both in the lane's offset.
+template <class U>
+__device__ __forceinline__ float adj_ld(const Buf& B, int row, int col, uint32_t rs, U) {
+    if constexpr (U::value) return B.ld(col * 4, (uint32_t)row * rs);
+    else return B.ld((int)((uint32_t)row * rs) + col * 4, 0u);
+}
+// A per-row or per-column profile value: through the constant address space
+// where the index is wave-uniform.
+template <class U>
+__device__ __forceinline__ float adj_prof(const float* p, int idx, U) {
+    if constexpr (U::value) return ro(p, idx);
+    else return p[idx];
 }
 
 // A1.  Args: p_in = lP, vx_in/vx_out = lVx, vz = lVz, psi_vx_in = lWx,
 // psi_vz_in = lWz, psi_px_in/out = qx, psi_pz_in/out = qz (in place).
+// Fields through raw buffer views (a lane's offset its column, a row's a
+// wave-uniform scalar) as in acoustic_tiled.h.
 template <int R, int TX, int NW, class Args>
 __device__ __forceinline__ void adjc_a1(const Args& a, int t, float* s_x, float* s_z) {
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
-    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = j0 + lane;
+    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = j0 + lane, vl = j * 4;
     const bool jok = j < nz - R;
     const int xw = a.pml_xw, zw = a.pml_zw;
+    const uint32_t rb = (uint32_t)nz * 4u, zb = (uint32_t)zw * 8u;  // row bytes of the fields, of the z strips
     const int zc = j < zw ? j : j >= nz - zw ? j - (nz - zw) + zw : -1;
+    const Buf KF = field_buf(a.kf), LP = field_buf(a.p_in), LWX = field_buf(a.psi_vx_in), LWZ = field_buf(a.psi_vz_in);
+    const Buf VXI = field_buf(a.vx_in), VZI = field_buf(a.vz_in), QX = field_buf(a.psi_px_in), QZ = field_buf(a.psi_pz_in);
     float vx0[RPW], vz0[RPW], bf[RPW], qx0[RPW], qz0[RPW];
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int i = i0 + wv + q * NW;
-        const bool ok = jok && i < nx - R;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
-        vx0[q] = ok ? a.vx_in[o] : 0.f;
-        vz0[q] = ok ? a.vz_in[o] : 0.f;
-        bf[q] = a.bf_const != 0.f ? a.bf_const : ok ? a.bf[o] : 0.f;
-        const int xr = ok ? adj_xstrip(i, nx, xw) : -1;
-        qx0[q] = xr >= 0 ? a.psi_px_in[(size_t)xr * nz + j] : 0.f;
-        qz0[q] = ok && zc >= 0 ? a.psi_pz_in[(size_t)i * (2 * zw) + zc] : 0.f;
+        const bool iok = i < nx - R;
+        vx0[q] = VXI.ld(vl, (uint32_t)i * rb);
+        vz0[q] = VZI.ld(vl, (uint32_t)i * rb);
+        bf[q] = a.bf_const != 0.f ? a.bf_const : field_buf(a.bf).ld(vl, (uint32_t)i * rb);
+        const int xr = iok ? adj_xstrip(i, nx, xw) : -1;
+        qx0[q] = xr >= 0 ? QX.ld(jok ? vl : kOOB, (uint32_t)xr * rb) : 0.f;
+        qz0[q] = iok && jok && zc >= 0 ? QZ.ld(zc * 4, (uint32_t)i * zb) : 0.f;
     }
     // lt_x with an x halo, lt_z with a z halo.
-    adj_stage<TX, NW, R, 0>(s_x, i0, j0, nx, nz, R, [&](int gi, int gj) {
-        const ptrdiff_t o = (ptrdiff_t)gi * nz + gj;
-        const float mu = a.kf[o] * a.p_in[o];
+    adj_stage<TX, NW, R, 0>(s_x, i0, j0, nx, nz, R, wv, [&](int gi, int gj, auto u) {
+        const float mu = adj_ld(KF, gi, gj, rb, u) * adj_ld(LP, gi, gj, rb, u);
         const int xr = adj_xstrip(gi, nx, xw);
         if (xr < 0) return -mu;
-        return __builtin_fmaf(a.pml_x[nx + gi], a.psi_vx_in[(size_t)xr * nz + gj] - mu, -mu);
+        return __builtin_fmaf(adj_prof(a.pml_x, nx + gi, u), adj_ld(LWX, xr, gj, rb, u) - mu, -mu);
     });
-    adj_stage<TX, NW, 0, R>(s_z, i0, j0, nx, nz, R, [&](int gi, int gj) {
-        const ptrdiff_t o = (ptrdiff_t)gi * nz + gj;
-        const float mu = a.kf[o] * a.p_in[o];
+    adj_stage<TX, NW, 0, R>(s_z, i0, j0, nx, nz, R, wv, [&](int gi, int gj, auto u) {
+        const float mu = adj_ld(KF, gi, gj, rb, u) * adj_ld(LP, gi, gj, rb, u);
         const int zz = gj < zw ? gj : gj >= nz - zw ? gj - (nz - zw) + zw : -1;
         if (zz < 0) return -mu;
-        return __builtin_fmaf(a.pml_z[nz + gj], a.psi_vz_in[(size_t)gi * (2 * zw) + zz] - mu, -mu);
+        return __builtin_fmaf(a.pml_z[nz + gj], adj_ld(LWZ, gi, zz, zb, std::false_type{}) - mu, -mu);
     });
     __syncthreads();
     if (!jok) return;
@@ -117,6 +163,7 @@ __device__ __forceinline__ void adjc_a1(const Args& a, int t, float* s_x, float*
 #pragma unroll
     for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
     const float zx = a.zx;
+    const Buf VXO = field_buf(a.vx_out), VZO = field_buf(a.vz_out);
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int il = wv + q * NW, i = i0 + il;
@@ -131,15 +178,16 @@ __device__ __forceinline__ void adjc_a1(const Args& a, int t, float* s_x, float*
 #pragma unroll
         for (int m = 2; m <= R; ++m) sz = __builtin_fmaf(cx[m - 1], pz[m] - pz[1 - m], sz);
         sz = zx * sz;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
         const float lvx = vx0[q] - sx, lvz = vz0[q] - sz;
-        a.vx_out[o] = lvx;
-        a.vz_out[o] = lvz;
+        VXO.st(lvx, vl, (uint32_t)i * rb);
+        VZO.st(lvz, vl, (uint32_t)i * rb);
         const int xr = adj_xstrip(i, nx, xw);
         if (xr >= 0)
-            a.psi_px_out[(size_t)xr * nz + j] = __builtin_fmaf(a.pml_x[2 * nx + i], qx0[q], -bf[q] * lvx);
+            field_buf(a.psi_px_out).st(__builtin_fmaf(ro(a.pml_x, 2 * nx + i), qx0[q], -bf[q] * lvx), vl,
+                                       (uint32_t)xr * rb);
         if (zc >= 0)
-            a.psi_pz_out[(size_t)i * (2 * zw) + zc] = __builtin_fmaf(a.pml_z[2 * nz + j], qz0[q], -bf[q] * lvz);
+            field_buf(a.psi_pz_out).st(__builtin_fmaf(a.pml_z[2 * nz + j], qz0[q], -bf[q] * lvz), zc * 4,
+                                       (uint32_t)i * zb);
     }
 }
 
@@ -151,43 +199,47 @@ __device__ __forceinline__ void adjc_a2(const Args& a, int t, int k, float* s_x,
     constexpr int WX = kTiledTZ, WZ = kTiledTZ + 2 * R, RPW = TX / NW;
     int i0, j0;
     tiled_origin(t, a.ntz, TX, R, &i0, &j0);
-    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = j0 + lane;
+    const int nx = a.nx, nz = a.nz, lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = j0 + lane, vl = j * 4;
     const bool jok = j < nz - R;
     const int xw = a.pml_xw, zw = a.pml_zw;
+    const uint32_t rb = (uint32_t)nz * 4u, zb = (uint32_t)zw * 8u;
     const int zc = j < zw ? j : j >= nz - zw ? j - (nz - zw) + zw : -1;
     const float* dv = a.div + (size_t)(k % a.div_period) * a.div_stride;
     const float bfc = a.bf_const;
+    const Buf LP = field_buf(a.p_in), KF = field_buf(a.kf), DV = field_buf(dv), GR = field_buf(a.grad);
+    const Buf LWX = field_buf(a.psi_vx_in), LWZ = field_buf(a.psi_vz_in);
     float p0[RPW], kf[RPW], dk[RPW], gr[RPW], wx0[RPW], wz0[RPW];
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int i = i0 + wv + q * NW;
-        const bool ok = jok && i < nx - R;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
-        p0[q] = ok ? a.p_in[o] : 0.f;
-        kf[q] = ok ? a.kf[o] : 0.f;
-        dk[q] = ok ? dv[o] : 0.f;
-        gr[q] = ok ? a.grad[o] : 0.f;
-        const int xr = ok ? adj_xstrip(i, nx, xw) : -1;
-        wx0[q] = xr >= 0 ? a.psi_vx_in[(size_t)xr * nz + j] : 0.f;
-        wz0[q] = ok && zc >= 0 ? a.psi_vz_in[(size_t)i * (2 * zw) + zc] : 0.f;
+        const bool iok = i < nx - R;
+        const uint32_t o = (uint32_t)i * rb;
+        p0[q] = LP.ld(vl, o);
+        kf[q] = KF.ld(vl, o);
+        dk[q] = DV.ld(vl, o);
+        gr[q] = GR.ld(vl, o);
+        const int xr = iok ? adj_xstrip(i, nx, xw) : -1;
+        wx0[q] = xr >= 0 ? LWX.ld(jok ? vl : kOOB, (uint32_t)xr * rb) : 0.f;
+        wz0[q] = iok && jok && zc >= 0 ? LWZ.ld(zc * 4, (uint32_t)i * zb) : 0.f;
     }
     // ls_x with an x halo, ls_z with a z halo.
-    adj_stage<TX, NW, R, 0>(s_x, i0, j0, nx, nz, R, [&](int gi, int gj) {
-        const ptrdiff_t o = (ptrdiff_t)gi * nz + gj;
-        const float b = bfc != 0.f ? bfc : a.bf[o];
-        const float nu = b * a.vx_in[o];
+    const Buf BF = field_buf(a.bf), LVX = field_buf(a.vx_in), LVZ = field_buf(a.vz_in);
+    const Buf QX = field_buf(a.psi_px_in), QZ = field_buf(a.psi_pz_in);
+    adj_stage<TX, NW, R, 0>(s_x, i0, j0, nx, nz, R, wv, [&](int gi, int gj, auto u) {
+        const float b = bfc != 0.f ? bfc : adj_ld(BF, gi, gj, rb, u);
+        const float nu = b * adj_ld(LVX, gi, gj, rb, u);
         const int xr = adj_xstrip(gi, nx, xw);
         if (xr < 0) return -nu;
-        return __builtin_fmaf(a.pml_x[3 * nx + gi], a.psi_px_in[(size_t)xr * nz + gj], -nu);
+        return __builtin_fmaf(adj_prof(a.pml_x, 3 * nx + gi, u), adj_ld(QX, xr, gj, rb, u), -nu);
     });
-    adj_stage<TX, NW, 0, R>(s_z, i0, j0, nx, nz, R, [&](int gi, int gj) {
-        const ptrdiff_t o = (ptrdiff_t)gi * nz + gj;
-        const float b = bfc != 0.f ? bfc : a.bf[o];
-        const float nu = b * a.vz_in[o];
+    adj_stage<TX, NW, 0, R>(s_z, i0, j0, nx, nz, R, wv, [&](int gi, int gj, auto u) {
+        const float b = bfc != 0.f ? bfc : adj_ld(BF, gi, gj, rb, u);
+        const float nu = b * adj_ld(LVZ, gi, gj, rb, u);
         const int zz = gj < zw ? gj : gj >= nz - zw ? gj - (nz - zw) + zw : -1;
         if (zz < 0) return -nu;
-        return __builtin_fmaf(a.pml_z[3 * nz + gj], a.psi_pz_in[(size_t)gi * (2 * zw) + zz], -nu);
+        return __builtin_fmaf(a.pml_z[3 * nz + gj], adj_ld(QZ, gi, zz, zb, std::false_type{}), -nu);
     });
     __syncthreads();
     if (!jok) return;
@@ -195,6 +247,7 @@ __device__ __forceinline__ void adjc_a2(const Args& a, int t, int k, float* s_x,
 #pragma unroll
     for (int m = 0; m < R; ++m) cx[m] = a.cx[m];
     const float zx = a.zx;
+    const Buf LPO = field_buf(a.p_out);
 #pragma unroll
     for (int q = 0; q < RPW; ++q) {
         const int il = wv + q * NW, i = i0 + il;
@@ -209,15 +262,15 @@ __device__ __forceinline__ void adjc_a2(const Args& a, int t, int k, float* s_x,
 #pragma unroll
         for (int m = 2; m <= R; ++m) sz = __builtin_fmaf(cx[m - 1], pz[m - 1] - pz[-m], sz);
         sz = zx * sz;
-        const ptrdiff_t o = (ptrdiff_t)i * nz + j;
+        const uint32_t o = (uint32_t)i * rb;
         const float mu = kf[q] * p0[q];
         const float pn = p0[q] - (sx + sz);
-        a.p_out[o] = pn;
-        a.grad[o] = __builtin_fmaf(mu, dk[q], gr[q]);
+        LPO.st(pn, vl, o);
+        GR.st(__builtin_fmaf(mu, dk[q], gr[q]), vl, o);
         if (i == a.sx && j == a.sz) a.src_adj[k] = kf[q] * pn;
         const int xr = adj_xstrip(i, nx, xw);
-        if (xr >= 0) a.psi_vx_out[(size_t)xr * nz + j] = a.pml_x[i] * (wx0[q] - mu);
-        if (zc >= 0) a.psi_vz_out[(size_t)i * (2 * zw) + zc] = a.pml_z[j] * (wz0[q] - mu);
+        if (xr >= 0) field_buf(a.psi_vx_out).st(ro(a.pml_x, i) * (wx0[q] - mu), vl, (uint32_t)xr * rb);
+        if (zc >= 0) field_buf(a.psi_vz_out).st(a.pml_z[j] * (wz0[q] - mu), zc * 4, (uint32_t)i * zb);
     }
 }
 
