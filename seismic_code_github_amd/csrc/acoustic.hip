@@ -776,7 +776,6 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
                 pl->avz[b] = cv.take(npad * 4);
             }
             pl->grad = cv.take(npad * 4);
-            pl->div = cv.take(npad * 4 * (size_t)pl->div_fields);
         }
         if (pass == 0) {
             if (cv.used >= (size_t)1 << 31)
@@ -785,6 +784,15 @@ int sfd::plan_create(sfd_context* ctx, const sfd_acoustic_params* p, const float
             e = pl->arena.alloc(cv.used);
             if (e != hipSuccess) return bail(hip_check(ctx, e, "arena allocation"));
         }
+    }
+    if (grad_period > 0) {
+        // The divergence ring is an allocation of its own: the kernels form
+        // each step's field base in 64-bit arithmetic, so the ring may pass
+        // 2 GiB (nt fields when the gradient keeps every step's divergence).
+        const size_t db = npad * 4 * (size_t)pl->div_fields;
+        e = pl->div_mem.alloc(db);
+        if (e != hipSuccess) return bail(hip_check(ctx, e, "divergence allocation"));
+        pl->div = Field{pl->div_mem.ptr, db};
     }
     // Per-row arrays are padded with kPadRows rows on both sides (zeros; the
     // sponge row table with 2.0, "not an x-edge row").

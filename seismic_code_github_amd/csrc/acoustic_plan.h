@@ -86,6 +86,7 @@ struct sfd_acoustic_plan {
     int div_fields = 0;   // divergence fields kept (grad_period, or 1 with boundary saving)
     Field ap[2], avx[2], avz[2], grad, div;
     DeviceBuffer ckpt, res, inj_w, inj_cell, src_adj;
+    DeviceBuffer div_mem;  // the divergence ring (pl->div)
     // Boundary saving: offsets (from row 0) of the band cells just outside
     // the reconstructed box and their P, Vx, Vz after every forward step,
     // [nt][3][nband].

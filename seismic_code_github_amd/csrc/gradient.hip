@@ -14,6 +14,9 @@
 //   4. dJ/dvp = -2 g / vp, plus the source term -sum_k adj_src_k w_k / vp_s^2
 //      (the injected amplitude dt rho_s vp_s w_k depends on vp at the source).
 // Memory: one segment of divergences plus nt/period checkpoints per shot.
+// With period >= nt the forward itself stores all nt divergences (the
+// recomputation's kernels) and step 3 recomputes nothing: one forward pass
+// fewer for nt fields of device memory per shot.
 // Every step is the two-launch tiled step (acoustic_tiled.h) in place on the
 // first buffer of each field; the velocities, checkpoints included, keep the
 // tiled sponge step's unabsorbed values.
@@ -202,11 +205,30 @@ int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, 
 }
 
 // Forward over all steps, saving the state before every segment but the first.
+// With one segment (checkpoint interval >= nt) the forward runs the
+// recomputation's kernels, which also store every step's divergence, so the
+// backward sweep recomputes nothing (the same state, records and divergences
+// bit for bit).
 int grad_forward(GradShot* g) {
     sfd_acoustic_plan* pl = g->pl;
     int rc = zero_state(pl);
     if (rc) return rc;
     const int T = pl->grad_period;
+    if (g->nseg == 1) {
+        const bool cpml = is_cpml(pl);
+        StepKernel dv, dp;
+        if (!pick_tiled(pl->radius, pl->ttx, pl->tnw, cpml ? MODE_FWD_DIV_CPML : MODE_FWD_DIV, &dv, &dp))
+            return fail(pl->ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
+        pl->cur = 0;
+        for (int k = 0; k < pl->nt; ++k) {
+            AcousticStepArgs a = make_args(pl, 0, k);
+            tiled_args(pl, a, true);
+            rc = launch_tiled(pl, dv, dp, a);
+            if (rc) return rc;
+        }
+        pl->v_raw = !cpml;
+        return SFD_OK;
+    }
     for (int seg = 0; seg < g->nseg; ++seg) {
         const int k0 = seg * T;
         if (seg > 0) {
@@ -261,22 +283,26 @@ int grad_backward(GradShot* g) {
     if (rc) return rc;
     for (int seg = g->nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * T, k1 = std::min(k0 + T, nt);
-        // Restore the forward state of step k0 and recompute the divergences.
+        // Restore the forward state of step k0 and recompute the divergences
+        // (one segment: the forward stored them).
         pl->cur = 0;
-        if (seg == 0) {
-            for (Field* f : {&pl->p[0], &pl->vx[0], &pl->vz[0]}) SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
-            Field* m[4];
-            for (int q = 0, c = psi_fields(pl, m); q < c; ++q)
-                SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
-        } else {
-            rc = ckpt_copy(pl, seg, false);
-            if (rc) return rc;
-        }
-        for (int k = k0; k < k1; ++k) {
-            AcousticStepArgs a = make_args(pl, 0, k);
-            tiled_args(pl, a, true);
-            rc = launch_tiled(pl, dv, dp, a);
-            if (rc) return rc;
+        if (g->nseg > 1) {
+            if (seg == 0) {
+                for (Field* f : {&pl->p[0], &pl->vx[0], &pl->vz[0]})
+                    SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+                Field* m[4];
+                for (int q = 0, c = psi_fields(pl, m); q < c; ++q)
+                    SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
+            } else {
+                rc = ckpt_copy(pl, seg, false);
+                if (rc) return rc;
+            }
+            for (int k = k0; k < k1; ++k) {
+                AcousticStepArgs a = make_args(pl, 0, k);
+                tiled_args(pl, a, true);
+                rc = launch_tiled(pl, dv, dp, a);
+                if (rc) return rc;
+            }
         }
         // Adjoint steps of the segment, in reverse.
         for (int k = k1 - 1; k >= k0; --k) {
@@ -654,7 +680,10 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (e != hipSuccess) return hip_check(ctx, e, "residual buffers");
     }
 
-    // 1. forward, checkpointing before every segment but the first.
+    // 1. forward, checkpointing before every segment but the first.  With
+    // one segment (and no boundary saving) the forward runs the
+    // recomputation's kernels, which store every step's divergence as well.
+    const bool keep_div = G.nseg == 1 && !G.bsave;
     for (auto& g : G.shots) {
         rc = zero_state(g.pl);
         if (rc) return rc;
@@ -664,7 +693,7 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
             rc = group_checkpoint(&G, k / G.period, true);
             if (rc) return rc;
         }
-        rc = group_launch(&G, G.kfwd, G.fwd_args, k);
+        rc = group_launch(&G, keep_div ? G.kdiv : G.kfwd, G.fwd_args, k);
         if (!rc && G.bsave) rc = group_band(&G, true, k);
         if (rc) return rc;
     }
@@ -726,21 +755,23 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     }
     for (int seg = G.bsave ? -1 : G.nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * G.period, k1 = std::min(k0 + G.period, G.nt);
-        if (seg == 0) {
-            for (auto& g : G.shots) {
-                for (Field* f : {&g.pl->p[0], &g.pl->vx[0], &g.pl->vz[0]})
-                    SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
-                Field* m[4];
-                for (int q = 0, c = psi_fields(g.pl, m); q < c; ++q)
-                    SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
+        if (!keep_div) {
+            if (seg == 0) {
+                for (auto& g : G.shots) {
+                    for (Field* f : {&g.pl->p[0], &g.pl->vx[0], &g.pl->vz[0]})
+                        SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
+                    Field* m[4];
+                    for (int q = 0, c = psi_fields(g.pl, m); q < c; ++q)
+                        SFD_HIP(ctx, hipMemsetAsync(m[q]->ptr, 0, m[q]->bytes, s));
+                }
+            } else {
+                rc = group_checkpoint(&G, seg, false);
+                if (rc) return rc;
             }
-        } else {
-            rc = group_checkpoint(&G, seg, false);
-            if (rc) return rc;
-        }
-        for (int k = k0; k < k1; ++k) {
-            rc = group_launch(&G, G.kdiv, G.fwd_args, k);
-            if (rc) return rc;
+            for (int k = k0; k < k1; ++k) {
+                rc = group_launch(&G, G.kdiv, G.fwd_args, k);
+                if (rc) return rc;
+            }
         }
         for (int k = k1 - 1; k >= k0; --k) {
             rc = group_launch(&G, G.kadj, G.adj_args, k);
@@ -817,14 +848,19 @@ int misfit_gradient(sfd_context* ctx, int nshot, const sfd_acoustic_params* para
     // Groups sized so that their workspaces (state, adjoint state, the
     // divergences of one segment or, with boundary saving, one divergence
     // field and the band of every step, and the checkpoints of every shot)
-    // stay below 64 GiB of device memory.
+    // stay below 80 % of the device memory free at the call (64 GiB where the
+    // query fails).  A checkpoint interval >= nt keeps all nt divergences of
+    // every shot: 16 shots of 881 x 881 over 2500 steps take 124 GB.
+    size_t free_b = 0, total_b = 0;
+    const double budget =
+        hipMemGetInfo(&free_b, &total_b) == hipSuccess ? 0.8 * (double)free_b : 64.0 * (1ull << 30);
     const double field = (double)params[0].nx * params[0].nz * 4.0;
     const int period = std::min(ckpt_every, std::max(params[0].nt, 1));
     const bool bsave = method == SFD_GRADIENT_BOUNDARY_SAVING;
     const double band = 4.0 * params[0].radius * (params[0].nx + params[0].nz) * 4.0 * 3.0 * params[0].nt;
     const double per_shot = field * (16.0 + (bsave ? 1 : period)) * 1.2 +
                             field * 3.0 * ((params[0].nt + period - 1) / period) + (bsave ? band : 0.0);
-    const int gmax = (int)std::max(1.0, std::min((double)nshot, 64.0 * (1ull << 30) / per_shot));
+    const int gmax = (int)std::max(1.0, std::min((double)nshot, budget / per_shot));
     for (int s0 = 0; s0 < nshot; s0 += gmax) {
         const int n = std::min(gmax, nshot - s0);
         const int rc = group_run(ctx, n, params + s0, vp, den, wavelet + s0, d_obs + s0, obs_dev, period, method,

@@ -8,7 +8,8 @@ sfd_acoustic_misfit_gradient) on MI355X.
   the source cell: relative error <= 1e-3.
 * The checkpoint interval changes memory, never the result: every interval
   gives the same gradient bit for bit (the recompute repeats the forward's
-  arithmetic exactly).
+  arithmetic exactly), an interval >= nt included (the forward stores every
+  divergence and nothing is recomputed), for single shots and shot groups.
 * The misfit entry point equals the sum of per-shot gradients seeded with
   the residual, and returns J = 1/2 |d - d_obs|^2.
 * Boundary saving (SFD_GRADIENT_BOUNDARY_SAVING): inside the sponge-free box
@@ -160,6 +161,29 @@ def test_shot_group_matches_single_shots_uneven_segments(ctx):
         r = sfd.acoustic_forward(shot, ctx)[0] - d
         gref += sfd.acoustic_gradient(shot, r, ckpt_every=40, ctx=ctx)[0]
     assert rel_l2(g, gref) <= 1e-6
+
+
+@pytest.mark.parametrize("boundary", ["sponge", "cpml"])
+def test_shot_group_divergences_kept_bitwise(ctx, boundary):
+    """A checkpoint interval >= nt: the group's forward stores all nt
+    divergences and the reverse sweep recomputes nothing; J and the gradient
+    equal the checkpointed run's bit for bit."""
+    print(f"case: shot group, divergences kept, {boundary}", flush=True)
+    shots, obs = [], []
+    for sx in (20, 35, 50):
+        if boundary == "sponge":
+            shot, _, _ = case(nt=150, src=(sx, 15))
+        else:
+            shot, _, _ = cpml_case(nt=150, nx=80, nz=70, width=12, src=(sx, 16))
+        shots.append(shot)
+        true = sfd.AcousticShot(**{**shot.__dict__, "vp": shot.vp * np.float32(0.985), "_keep": []})
+        obs.append(sfd.acoustic_forward(true, ctx)[0])
+    J0, g0 = sfd.misfit_gradient(shots, obs, ckpt_every=40, ctx=ctx)
+    for c in (150, 1000):
+        J, g = sfd.misfit_gradient(shots, obs, ckpt_every=c, ctx=ctx)
+        assert J == J0, c
+        np.testing.assert_array_equal(g, g0, err_msg=f"interval {c}")
+    assert np.abs(g0).max() > 0
 
 
 @pytest.mark.parametrize("radius,nx,nz,ckpt", [(4, 70, 60, 40), (8, 90, 300, 50), (5, 260, 150, 37),
