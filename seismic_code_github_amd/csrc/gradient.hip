@@ -14,9 +14,9 @@
 //   4. dJ/dvp = -2 g / vp, plus the source term -sum_k adj_src_k w_k / vp_s^2
 //      (the injected amplitude dt rho_s vp_s w_k depends on vp at the source).
 // Memory: one segment of divergences plus nt/period checkpoints per shot.
-// With period >= nt the forward itself stores all nt divergences (the
-// recomputation's kernels) and step 3 recomputes nothing: one forward pass
-// fewer for nt fields of device memory per shot.
+// The forward's last segment stores its own divergences, so step 3
+// recomputes every segment but the last; with period >= nt (nt divergence
+// fields per shot) it recomputes nothing.
 // Every step is the two-launch tiled step (acoustic_tiled.h) in place on the
 // first buffer of each field; the velocities, checkpoints included, keep the
 // tiled sponge step's unabsorbed values.
@@ -205,38 +205,37 @@ int grad_setup(sfd_context* ctx, const sfd_acoustic_params* p, const float* vp, 
 }
 
 // Forward over all steps, saving the state before every segment but the first.
-// With one segment (checkpoint interval >= nt) the forward runs the
-// recomputation's kernels, which also store every step's divergence, so the
-// backward sweep recomputes nothing (the same state, records and divergences
-// bit for bit).
+// The last segment runs the recomputation's kernels, which also store each of
+// its steps' divergences (the same state and records bit for bit), so the
+// backward sweep starts with that segment's adjoint and recomputes only the
+// others; with one segment (checkpoint interval >= nt) it recomputes nothing.
 int grad_forward(GradShot* g) {
     sfd_acoustic_plan* pl = g->pl;
     int rc = zero_state(pl);
     if (rc) return rc;
     const int T = pl->grad_period;
-    if (g->nseg == 1) {
-        const bool cpml = is_cpml(pl);
-        StepKernel dv, dp;
-        if (!pick_tiled(pl->radius, pl->ttx, pl->tnw, cpml ? MODE_FWD_DIV_CPML : MODE_FWD_DIV, &dv, &dp))
-            return fail(pl->ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
-        pl->cur = 0;
-        for (int k = 0; k < pl->nt; ++k) {
+    const bool cpml = is_cpml(pl);
+    StepKernel dv, dp;
+    if (!pick_tiled(pl->radius, pl->ttx, pl->tnw, cpml ? MODE_FWD_DIV_CPML : MODE_FWD_DIV, &dv, &dp))
+        return fail(pl->ctx, SFD_ENOTSUP, "no gradient kernel for this radius / tiling");
+    for (int seg = 0; seg < g->nseg; ++seg) {
+        const int k0 = seg * T, n = std::min(T, pl->nt - k0);
+        if (seg > 0) {
+            rc = ckpt_copy(pl, seg, true);  // the tiled step runs in place: the state is in buffer 0
+            if (rc) return rc;
+        }
+        if (seg < g->nseg - 1) {
+            rc = enqueue_steps(pl, k0, n);
+            if (rc) return rc;
+            continue;
+        }
+        for (int k = k0; k < k0 + n; ++k) {
             AcousticStepArgs a = make_args(pl, 0, k);
             tiled_args(pl, a, true);
             rc = launch_tiled(pl, dv, dp, a);
             if (rc) return rc;
         }
         pl->v_raw = !cpml;
-        return SFD_OK;
-    }
-    for (int seg = 0; seg < g->nseg; ++seg) {
-        const int k0 = seg * T;
-        if (seg > 0) {
-            rc = ckpt_copy(pl, seg, true);  // the tiled step runs in place: the state is in buffer 0
-            if (rc) return rc;
-        }
-        rc = enqueue_steps(pl, k0, std::min(T, pl->nt - k0));
-        if (rc) return rc;
     }
     return SFD_OK;
 }
@@ -284,9 +283,9 @@ int grad_backward(GradShot* g) {
     for (int seg = g->nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * T, k1 = std::min(k0 + T, nt);
         // Restore the forward state of step k0 and recompute the divergences
-        // (one segment: the forward stored them).
+        // (the last segment's: the forward stored them).
         pl->cur = 0;
-        if (g->nseg > 1) {
+        if (seg < g->nseg - 1) {
             if (seg == 0) {
                 for (Field* f : {&pl->p[0], &pl->vx[0], &pl->vz[0]})
                     SFD_HIP(ctx, hipMemsetAsync(f->ptr, 0, f->bytes, s));
@@ -680,10 +679,11 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
         if (e != hipSuccess) return hip_check(ctx, e, "residual buffers");
     }
 
-    // 1. forward, checkpointing before every segment but the first.  With
-    // one segment (and no boundary saving) the forward runs the
-    // recomputation's kernels, which store every step's divergence as well.
-    const bool keep_div = G.nseg == 1 && !G.bsave;
+    // 1. forward, checkpointing before every segment but the first.  Without
+    // boundary saving the last segment runs the recomputation's kernels,
+    // which store its steps' divergences as well (with one segment: every
+    // step's), so step 3 recomputes only the other segments.
+    const int kdiv0 = G.bsave ? G.nt : (G.nseg - 1) * G.period;
     for (auto& g : G.shots) {
         rc = zero_state(g.pl);
         if (rc) return rc;
@@ -693,7 +693,7 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
             rc = group_checkpoint(&G, k / G.period, true);
             if (rc) return rc;
         }
-        rc = group_launch(&G, keep_div ? G.kdiv : G.kfwd, G.fwd_args, k);
+        rc = group_launch(&G, k >= kdiv0 ? G.kdiv : G.kfwd, G.fwd_args, k);
         if (!rc && G.bsave) rc = group_band(&G, true, k);
         if (rc) return rc;
     }
@@ -755,7 +755,7 @@ int group_run(sfd_context* ctx, int nshot, const sfd_acoustic_params* params, co
     }
     for (int seg = G.bsave ? -1 : G.nseg - 1; seg >= 0; --seg) {
         const int k0 = seg * G.period, k1 = std::min(k0 + G.period, G.nt);
-        if (!keep_div) {
+        if (k0 < kdiv0) {
             if (seg == 0) {
                 for (auto& g : G.shots) {
                     for (Field* f : {&g.pl->p[0], &g.pl->vx[0], &g.pl->vz[0]})
