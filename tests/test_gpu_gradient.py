@@ -186,6 +186,25 @@ def test_shot_group_divergences_kept_bitwise(ctx, boundary):
     assert np.abs(g0).max() > 0
 
 
+def test_divergences_kept_past_2gib(ctx):
+    """One 881 x 881 shot of c4's geometry over 700 steps with every
+    divergence kept: a 2.2 GB divergence ring (an allocation of its own,
+    addressed from a 64-bit base per step), bitwise the gradient with
+    checkpoints every 50 steps."""
+    from seismic_code_github_amd import workloads as W
+    _, bg, shots = W.fwi_801(nshot=1, nt=700)
+    print("case: 881x881, nt=700, divergences kept (2.2 GB ring)", flush=True)
+    # Observed data from a 1 % faster model (c4's anomalies lie deeper than
+    # 700 steps reach and back).
+    true = sfd.AcousticShot(**{**shots[0].__dict__, "vp": bg * np.float32(1.01), "_keep": []})
+    obs = [sfd.acoustic_forward(true, ctx)[0]]
+    J0, g0 = sfd.misfit_gradient(shots, obs, vp=bg, ckpt_every=50, ctx=ctx)
+    J1, g1 = sfd.misfit_gradient(shots, obs, vp=bg, ckpt_every=700, ctx=ctx)
+    assert J0 > 0 and J1 == J0
+    np.testing.assert_array_equal(g1, g0)
+    assert np.abs(g0).max() > 0
+
+
 @pytest.mark.parametrize("radius,nx,nz,ckpt", [(4, 70, 60, 40), (8, 90, 300, 50), (5, 260, 150, 37),
                                                (4, 70, 60, 220)])
 def test_boundary_saving_inside_the_box(ctx, radius, nx, nz, ckpt):
